@@ -1,0 +1,146 @@
+"""Benchmark of the diffuse wavefront frame: Grays/s on the 1280x720 frame of the 118,562-triangle terrain scene
+(242x242-vertex fBm heightfield + 200 boxes, Lambert albedo 0.8, sun + sky, next-event estimation), compressed BVH8,
+256 samples per pixel in flight (one wavefront pass), 8 bounces, FP32 radiance.
+
+A step is one frame: camera rays for every (pixel, sample), then trace+shade and shadow kernels per bounce until the
+queues are empty, then the per-pixel reduction. Scene build and upload happen before the timed region; all path state
+lives in HBM. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-wavefront-pathtracer_amd"))
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+# sun direction / irradiance and sky of the reference's published diffuse benchmark (its examples/lighting.json)
+LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3", "3", "--sun_angle", "0.526",
+            "--sky", "0.3", "0.38", "0.5"]
+
+
+def workload_args(width, height, batch, bounces, tri, variant, stack):
+    return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
+            "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack)] + LIGHTING
+
+
+def algorithmic_bytes(st, npaths):
+    """Compulsory HBM bytes of one frame's trace+shade (bounce) kernels, from the frame's ray counters.
+
+    Per camera ray: 16 B radiance write. Per continuation ray traced: 4 B queue read + 48 B origin/direction/throughput
+    read. Per continuation-ray miss: 32 B radiance read-modify-write. Per continuation produced: 4 B queue write + 48 B
+    path-state write. Per shadow ray produced: 48 B shadow record write."""
+    cont_traced = st["rays_closest"] - npaths
+    cont_miss = st["rays_miss"] - st["camera_miss"]
+    return 16.0 * npaths + 52.0 * cont_traced + 32.0 * cont_miss + 52.0 * st["rays_cont"] + 48.0 * st["rays_shadow"]
+
+
+def cpu_baseline(args):
+    """The oracle's multi-threaded CPU renderer on a bounded sample of the same workload (time-boxed)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cwp
+    import pyoracle as po
+    threads = max(1, min(16, os.cpu_count() or 1))
+    spp = args.cpu_spp
+    s = cwp.Setup(workload_args(args.width, args.height, spp, args.bounces, args.tri, args.variant, args.stack))
+    sc, _ = po.scene_from_setup(s)
+    t0 = time.perf_counter()
+    _, st = sc.render(s.params(), spp, batch=spp, bounces=args.bounces, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = float(st[0] + st[1])
+    return {"value": rays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+            "sample": "oracle CPU renderer, same scene/camera/lighting, %dx%d, %d spp, %d bounces, %d threads: "
+                      "%.0f rays in %.1f s" % (args.width, args.height, spp, args.bounces, threads, rays, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--bounces", type=int, default=8)
+    ap.add_argument("--tri", default="woop")
+    ap.add_argument("--variant", default="bvh8")
+    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the renderer runs on a single MI355X (multi-GPU runs are not part of "
+              "this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import cwp
+    print("bench: setting up scene", file=sys.stderr, flush=True)
+    s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack))
+    r = cwp.Renderer(s)
+    info = s.info
+    npaths = args.width * args.height * args.batch
+    print("bench: %d tris, %d nodes, grid %d workgroups, %.0f MB" % (info["num_tris"], info["num_nodes"], r.grid_blocks,
+                                                                    r.vram_mb), file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        st = r.render(args.batch)
+        print("bench: warmup %d: %.1f ms" % (w, st["t_frame_ms"]), file=sys.stderr, flush=True)
+    frames = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        frames.append(r.render(args.batch))
+    wall = time.perf_counter() - t0
+    for k, st in enumerate(frames):
+        print("bench: step %d: %.2f ms, %.0f rays" % (k, st["t_frame_ms"], st["rays_closest"] + st["rays_shadow"]),
+              file=sys.stderr, flush=True)
+    rays = sum(f["rays_closest"] + f["rays_shadow"] for f in frames)
+    frame_ms = sum(f["t_frame_ms"] for f in frames)
+    grays = rays / (frame_ms * 1e-3) / 1e9
+    bounce_ms = sum(f["t_bounce_ms"] for f in frames)
+    launches = sum(f["bounce_launches"] for f in frames)
+    bytes_bounce = sum(algorithmic_bytes(f, npaths) for f in frames)
+    achieved = bytes_bounce / (bounce_ms * 1e-3) / 1e9
+    last = frames[-1]
+    out = {
+        "metric": "Grays/s on 720p 117k-tri diffuse frame; p50/p99 job latency, batched vs plain",
+        "value": round(grays, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(frame_ms / args.steps, 3),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "terrain-118k-%dx%d-%dspp-in-flight-depth%d" % (args.width, args.height, args.batch, args.bounces),
+                   "scene": "242x242 fBm heightfield + 200 boxes (%d tris), albedo 0.8, sun 3 W/m2 + sky" % info["num_tris"],
+                   "width": args.width, "height": args.height, "spp": args.batch, "batch": args.batch,
+                   "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
+                   "stack_lds": args.stack, "bvh_nodes": info["num_nodes"]},
+        "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
+        "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
+        "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shadow_ms": round(last["t_shadow_ms"], 3),
+        "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_frame": bytes_bounce / args.steps,
+                     "avg_launch_ms": round(bounce_ms / max(1, launches), 4)},
+        "latency": None,
+        "cpu_baseline": None,
+    }
+    r.close()
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,238 @@
+// capi.cpp - the extern "C" entry points of libcwp.so (declared in include/cwp.h).
+#include "cwp.h"
+#include "device/renderer.h"
+#include "host/bvh.h"
+#include "host/setup.h"
+#include <cstddef>
+#include <cstring>
+#include <new>
+#include <string>
+
+using namespace cwp;
+
+struct cwp_setup { SceneSetup Z; std::vector<int> triFlat; std::vector<int> tree; std::vector<float> treeBox; };
+struct cwp_ctx { Renderer R; const cwp_setup* setup = nullptr; };
+struct cwp_bvh {
+    BinaryBVH tree; std::vector<Vec4> nodes, woop, corners, nrm; std::vector<int> order, treeFlat; std::vector<float> treeBox;
+    int variant = 2, stackNeed = 0; double sah = 0;
+};
+
+static_assert(sizeof(cwp_params) == sizeof(FrameParams), "cwp_params must mirror FrameParams");
+static_assert(offsetof(cwp_params, sun_to) == offsetof(FrameParams, sunTo), "cwp_params layout");
+static_assert(offsetof(cwp_params, seed) == offsetof(FrameParams, seed), "cwp_params layout");
+static_assert(offsetof(cwp_params, ftab) == offsetof(FrameParams, ftab), "cwp_params layout");
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+extern "C" {
+
+const char* cwp_last_error(void) { return g_err.c_str(); }
+int cwp_version(void) { return 1; }
+
+int cwp_device_info(int* num_cu, char* name, size_t cap) {
+    int n = 0; std::string nm, err;
+    if (!deviceInfo(n, nm, err)) return fail(CWP_ERR_DEVICE, err);
+    if (num_cu) *num_cu = n;
+    if (name && cap) { strncpy(name, nm.c_str(), cap - 1); name[cap - 1] = 0; }
+    return CWP_OK;
+}
+
+int cwp_setup_create(int argc, const char* const* argv, cwp_setup** out) {
+    if (!out || argc < 0 || (argc > 0 && !argv)) return fail(CWP_ERR_ARG, "cwp_setup_create: null argument");
+    *out = nullptr;
+    cwp_setup* s = new (std::nothrow) cwp_setup();
+    if (!s) return fail(CWP_ERR_ARG, "out of host memory");
+    std::string err;
+    if (!parseArgs(argc, argv, s->Z.A, err)) { delete s; return fail(CWP_ERR_ARG, err); }
+    if (!setupScene(s->Z, err)) { delete s; return fail(CWP_ERR_IO, err); }
+    const Mesh& M = s->Z.M;
+    s->triFlat.resize(M.f.size() * 3);
+    for (size_t t = 0; t < M.f.size(); t++) { s->triFlat[3 * t] = M.f[t].a; s->triFlat[3 * t + 1] = M.f[t].b; s->triFlat[3 * t + 2] = M.f[t].c; }
+    if (s->Z.M.fm.empty()) s->Z.M.fm.assign(M.f.size(), 0);
+    for (const BinNode& n : s->Z.tree.nodes) {
+        s->tree.insert(s->tree.end(), { n.left, n.right, n.first, n.count });
+        s->treeBox.insert(s->treeBox.end(), { n.box.lo.x, n.box.lo.y, n.box.lo.z, n.box.hi.x, n.box.hi.y, n.box.hi.z });
+    }
+    *out = s;
+    return CWP_OK;
+}
+
+void cwp_setup_destroy(cwp_setup* s) { delete s; }
+
+int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
+    if (!s || !info) return fail(CWP_ERR_ARG, "cwp_setup_info: null argument");
+    const SceneSetup& Z = s->Z;
+    memset(info, 0, sizeof *info);
+    info->variant = Z.variant; info->tri_test = Z.O.tri;
+    info->num_tris = (int)Z.M.f.size(); info->num_verts = (int)Z.M.v.size(); info->num_materials = (int)Z.M.kd.size();
+    info->node_floats4 = Z.variant == 2 ? 4 : 5; info->num_nodes = (int)Z.nNodes;
+    info->bvh_depth = Z.maxDepth; info->stack_need = Z.S.stackNeed; info->sah = Z.sah; info->bvh_build_s = Z.bvhS;
+    info->width = Z.A.w; info->height = Z.A.h; info->spp = Z.A.spp; info->batch = Z.A.batch; info->bounces = Z.A.bounces;
+    return CWP_OK;
+}
+
+int cwp_setup_params(const cwp_setup* s, cwp_params* p) {
+    if (!s || !p) return fail(CWP_ERR_ARG, "cwp_setup_params: null argument");
+    memcpy(p, &s->Z.P, sizeof *p);
+    return CWP_OK;
+}
+
+int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* count) {
+    if (!s || !data || !count) return fail(CWP_ERR_ARG, "cwp_setup_array: null argument");
+    const SceneSetup& Z = s->Z;
+    switch (which) {
+        case CWP_ARR_VERTICES: *data = Z.M.v.data(); *count = Z.M.v.size(); break;
+        case CWP_ARR_TRIANGLES: *data = s->triFlat.data(); *count = Z.M.f.size(); break;
+        case CWP_ARR_MATERIAL: *data = Z.M.fm.data(); *count = Z.M.fm.size(); break;
+        case CWP_ARR_ALBEDO: *data = Z.S.albedo.data(); *count = Z.S.albedo.size(); break;
+        case CWP_ARR_NODES: *data = Z.S.nodes.data(); *count = Z.nNodes; break;
+        case CWP_ARR_ORDER: *data = Z.S.order.data(); *count = Z.S.order.size(); break;
+        case CWP_ARR_WOOP: *data = Z.S.woop.data(); *count = Z.S.woop.size() / 3; break;
+        case CWP_ARR_CORNERS: *data = Z.S.verts.data(); *count = Z.S.verts.size() / 3; break;
+        case CWP_ARR_NORMALS: *data = Z.M.n.data(); *count = Z.M.n.size(); break;
+        case CWP_ARR_TREE: *data = s->tree.data(); *count = s->tree.size() / 4; break;
+        case CWP_ARR_TREE_BOX: *data = s->treeBox.data(); *count = s->treeBox.size() / 6; break;
+        case CWP_ARR_PRIM: *data = Z.tree.prim.data(); *count = Z.tree.prim.size(); break;
+        default: return fail(CWP_ERR_ARG, "cwp_setup_array: unknown array");
+    }
+    return CWP_OK;
+}
+
+int cwp_create(const cwp_setup* s, cwp_ctx** out) {
+    if (!s || !out) return fail(CWP_ERR_ARG, "cwp_create: null argument");
+    *out = nullptr;
+    cwp_ctx* c = new (std::nothrow) cwp_ctx();
+    if (!c) return fail(CWP_ERR_ARG, "out of host memory");
+    std::string err;
+    if (!c->R.init(s->Z.S, s->Z.P, s->Z.O, err)) { delete c; return fail(CWP_ERR_DEVICE, err); }
+    c->setup = s;
+    *out = c;
+    return CWP_OK;
+}
+
+void cwp_destroy(cwp_ctx* c) { delete c; }
+
+int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats* stats) {
+    if (!c) return fail(CWP_ERR_ARG, "cwp_render: null context");
+    FrameStats st; std::string err;
+    if (!c->R.render(spp, sample_base, clear != 0, st, err)) return fail(err.find("hip") != std::string::npos ? CWP_ERR_DEVICE : CWP_ERR_ARG, err);
+    if (stats) {
+        stats->rays_closest = st.rays; stats->rays_shadow = st.shadowRays; stats->t_trace_ms = st.traceMs; stats->t_frame_ms = st.frameMs;
+        stats->t_bounce_ms = st.bounceMs; stats->t_shadow_ms = st.shadowMs;
+        stats->bounce_launches = st.bounceLaunches; stats->shadow_launches = st.shadowLaunches;
+        stats->rays_hit = st.hitRays; stats->rays_miss = st.missRays; stats->rays_cont = st.contRays; stats->shadow_unoccluded = st.unoccluded;
+        stats->camera_miss = st.cameraMiss;
+    }
+    return CWP_OK;
+}
+
+int cwp_read_accum(cwp_ctx* c, float* rgba, size_t npix) {
+    if (!c || !rgba) return fail(CWP_ERR_ARG, "cwp_read_accum: null argument");
+    std::string err;
+    if (!c->R.readAccum(rgba, npix, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_set_params(cwp_ctx* c, const cwp_params* p) {
+    if (!c || !p) return fail(CWP_ERR_ARG, "cwp_set_params: null argument");
+    FrameParams F; memcpy(&F, p, sizeof F);
+    std::string err;
+    if (!c->R.setParams(F, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_trace(cwp_ctx* c, const void* rays, int n, int any_hit, void* hits) {
+    if (!c || (n > 0 && (!rays || !hits))) return fail(CWP_ERR_ARG, "cwp_trace: null argument");
+    std::string err;
+    if (!c->R.trace(rays, n, any_hit != 0, hits, err)) return fail(CWP_ERR_DEVICE, err);
+    return CWP_OK;
+}
+
+int cwp_trace_host(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits) {
+    if (!c || (n > 0 && (!rays || !hits))) return fail(CWP_ERR_ARG, "cwp_trace_host: null argument");
+    std::string err;
+    if (!c->R.traceHost(rays, n, any_hit != 0, hits, err)) return fail(CWP_ERR_DEVICE, err);
+    return CWP_OK;
+}
+
+int cwp_frame_json(const cwp_setup* s, const cwp_ctx* c, const cwp_frame_stats* st, int rep, int frame, char* buf, size_t cap) {
+    if (!s || !c || !st || !buf || !cap) return fail(CWP_ERR_ARG, "cwp_frame_json: null argument");
+    FrameStats F;
+    F.rays = st->rays_closest; F.shadowRays = st->rays_shadow; F.traceMs = st->t_trace_ms; F.frameMs = st->t_frame_ms;
+    F.bounceMs = st->t_bounce_ms; F.shadowMs = st->t_shadow_ms; F.hitRays = st->rays_hit; F.missRays = st->rays_miss;
+    F.contRays = st->rays_cont; F.unoccluded = st->shadow_unoccluded; F.cameraMiss = st->camera_miss;
+    std::string j = frameJson(s->Z, c->R, F, rep, frame);
+    if (j.size() + 1 > cap) return fail(CWP_ERR_ARG, "cwp_frame_json: buffer too small");
+    memcpy(buf, j.c_str(), j.size() + 1);
+    return CWP_OK;
+}
+
+int cwp_ctx_info(const cwp_ctx* c, int* grid_blocks, double* vram_mb) {
+    if (!c) return fail(CWP_ERR_ARG, "cwp_ctx_info: null context");
+    if (grid_blocks) *grid_blocks = c->R.gridBlocks;
+    if (vram_mb) *vram_mb = c->R.vramMB;
+    return CWP_OK;
+}
+
+int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int variant, int bins, int leafMax, cwp_bvh** out) {
+    if (!out || (nv > 0 && !vertices) || (nt > 0 && !tris) || nv < 0 || nt < 1) return fail(CWP_ERR_ARG, "cwp_bvh_build: bad arguments");
+    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_build: variant must be 2 or 8");
+    if (bins < 2 || bins > 256 || leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_build: bins in [2,256], leaf_max in [1,32]");
+    *out = nullptr;
+    Mesh m;
+    m.v.resize(nv);
+    for (int i = 0; i < nv; i++) m.v[i] = v3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]);
+    m.f.resize(nt);
+    for (int t = 0; t < nt; t++) {
+        Tri q{ tris[3 * t], tris[3 * t + 1], tris[3 * t + 2] };
+        if (q.a < 0 || q.b < 0 || q.c < 0 || q.a >= nv || q.b >= nv || q.c >= nv) return fail(CWP_ERR_ARG, "cwp_bvh_build: vertex index out of range");
+        m.f[t] = q;
+    }
+    cwp_bvh* b = new (std::nothrow) cwp_bvh();
+    if (!b) return fail(CWP_ERR_ARG, "out of host memory");
+    b->variant = variant;
+    buildSAH(m, bins, variant == 8 ? 1 : leafMax, b->tree);
+    if (variant == 2) {
+        layoutBVH2(b->tree, b->nodes); b->order = b->tree.prim; b->sah = sahCost(b->tree, 1, 1);
+        b->stackNeed = bvh2Depth(b->nodes) + 1;
+    } else {
+        WideBVH W; collapseCWBVH(b->tree, W);
+        b->nodes.swap(W.nodes); b->order.swap(W.prim); b->sah = W.sah; b->stackNeed = W.depth + 1;
+    }
+    packWoop(m, b->order, b->woop);
+    packVerts(m, b->order, b->corners, b->nrm);
+    for (const BinNode& n : b->tree.nodes) {
+        b->treeFlat.insert(b->treeFlat.end(), { n.left, n.right, n.first, n.count });
+        b->treeBox.insert(b->treeBox.end(), { n.box.lo.x, n.box.lo.y, n.box.lo.z, n.box.hi.x, n.box.hi.y, n.box.hi.z });
+    }
+    *out = b;
+    return CWP_OK;
+}
+
+void cwp_bvh_destroy(cwp_bvh* b) { delete b; }
+
+int cwp_bvh_array(const cwp_bvh* b, int which, const void** data, size_t* count) {
+    if (!b || !data || !count) return fail(CWP_ERR_ARG, "cwp_bvh_array: null argument");
+    switch (which) {
+        case CWP_ARR_NODES: *data = b->nodes.data(); *count = b->nodes.size() / (b->variant == 2 ? 4 : 5); break;
+        case CWP_ARR_ORDER: *data = b->order.data(); *count = b->order.size(); break;
+        case CWP_ARR_WOOP: *data = b->woop.data(); *count = b->woop.size() / 3; break;
+        case CWP_ARR_CORNERS: *data = b->corners.data(); *count = b->corners.size() / 3; break;
+        case CWP_ARR_TREE: *data = b->treeFlat.data(); *count = b->treeFlat.size() / 4; break;
+        case CWP_ARR_TREE_BOX: *data = b->treeBox.data(); *count = b->treeBox.size() / 6; break;
+        case CWP_ARR_PRIM: *data = b->tree.prim.data(); *count = b->tree.prim.size(); break;
+        default: return fail(CWP_ERR_ARG, "cwp_bvh_array: unknown array");
+    }
+    return CWP_OK;
+}
+
+int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stackNeed, double* sah) {
+    if (!b) return fail(CWP_ERR_ARG, "cwp_bvh_stats: null argument");
+    if (depth) *depth = b->tree.maxDepth;
+    if (stackNeed) *stackNeed = b->stackNeed;
+    if (sah) *sah = b->sah;
+    return CWP_OK;
+}
+
+}  // extern "C"

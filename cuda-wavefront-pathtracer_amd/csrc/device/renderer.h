@@ -1,0 +1,77 @@
+// device/renderer.h - host interface of the gfx950 wavefront renderer (legacy Lambert + sun + sky integrator).
+//
+// Plain C++ types only (no HIP types), so the g++-compiled host code and the C ABI can include it. Mirrors the
+// reference's WavefrontRenderer (src/render/renderer.h:47-93): init uploads the scene and allocates path state for
+// w*h*batch paths, render() runs spp samples (a multiple of batch) with sample indices [sBase, sBase + spp) into the
+// accumulation buffer, readAccum() returns per-pixel radiance sums. Errors are reported through the return value and
+// an error string, never by exiting.
+#pragma once
+#include "device/traverse_types.h"
+#include "host/camera.h"
+#include <string>
+#include <vector>
+
+namespace cwp {
+
+struct RenderOptions {
+    int variant = 8;        // BVH layout: 2 or 8 (compressed)
+    int tri = kTriWoop;     // triangle test: kTriMT, kTriWoop, kTriWatertight
+    int batch = 64;         // samples per pixel in flight per wavefront pass
+    int bounces = 8;        // maximum bounce depth (camera hit = depth 0)
+    int stackLds = 16;      // LDS traversal-stack entries per lane (8, 16 or 32); deeper entries spill to HBM
+    int blocksPerCU = 0;    // persistent grid: workgroups per CU (0 = occupancy limit)
+    int timeKernels = 1;    // per-launch hipEvent timing of the trace/shadow kernels
+};
+
+struct FrameStats {
+    double rays = 0, shadowRays = 0;     // closest-hit rays (camera + continuation) and shadow rays traced
+    double traceMs = 0, frameMs = 0;     // sum of the trace + shadow kernel times; whole frame (one event pair)
+    double bounceMs = 0, shadowMs = 0;   // trace+shade kernel time and shadow kernel time
+    int bounceLaunches = 0, shadowLaunches = 0;
+    double hitRays = 0, contRays = 0, missRays = 0, unoccluded = 0;   // event counters (bytes-per-ray accounting)
+    double cameraMiss = 0;                                             // camera rays that miss
+};
+
+// Scene data ready for upload, in BVH leaf order.
+struct SceneUpload {
+    int variant = 8, root = 0;
+    std::vector<Vec4> nodes;      // BVH2 (4 float4/node) or CWBVH (5 float4/node)
+    std::vector<Vec4> woop;       // Woop rows (3 float4/tri)
+    std::vector<Vec4> verts;      // corners (3 float4/tri): (v0, orig id), (v1, material), (v2, 0)
+    std::vector<Vec4> normals;    // per-corner normals (3 float4/tri) or empty
+    std::vector<Vec4> albedo;     // per material (rgb, 0)
+    std::vector<int> order;       // leaf position -> original triangle id
+    int stackNeed = 0;            // traversal-stack entries the tree can require
+};
+
+class Renderer {
+public:
+    Renderer();
+    ~Renderer();
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    bool init(const SceneUpload& S, const FrameParams& P, const RenderOptions& O, std::string& err);
+    bool render(int spp, int sBase, bool clear, FrameStats& st, std::string& err);
+    bool readAccum(float* rgba, size_t nPix, std::string& err);
+    bool setParams(const FrameParams& P, std::string& err);
+    // Closest (anyHit = false) or any hit of n rays; rays = 2 float4 per ray (o, tmin), (d, tmax); hits = float4
+    // (t, u, v, leaf position as int bits, -1 on a miss). Both are device pointers; synchronous.
+    bool trace(const void* rays, int n, bool anyHit, void* hits, std::string& err);
+    // The same with host arrays, staged through device buffers owned by the renderer.
+    bool traceHost(const float* rays, int n, bool anyHit, float* hits, std::string& err);
+
+    double vramMB = 0;
+    int gridBlocks = 0, numCU = 0;
+    std::string gpuName;
+    const RenderOptions& options() const;
+
+private:
+    struct Impl;
+    Impl* p_;
+};
+
+// Number of compute units and device name of device 0 (false if no HIP device)
+bool deviceInfo(int& numCU, std::string& name, std::string& err);
+
+}  // namespace cwp

@@ -1,0 +1,231 @@
+// device/traverse.h - gfx950 ray traversal: BVH2 (64 B nodes) and compressed BVH8 (80 B nodes), closest and any hit,
+// with three triangle tests. One ray per lane; the traversal stack lives in LDS (SL entries per lane, lane-minor so a
+// wave touching the same depth hits consecutive banks) and spills to a per-thread global slab beyond SL entries.
+//
+// Algorithms (restated from the reference, src/accel/traverse.cuh):
+//  * BVH2 (lines 45-70): both child slabs per node, leaves intersected on the spot, nearer child first, farther pushed.
+//  * BVH8 (lines 112-152): per node decode the per-axis power-of-two scales, test the 8 quantised child slabs, intersect
+//    hit leaves in slot order, record internal hits in octant-sorted bit order (slot ^ ray octant), then continue with
+//    the lowest set bit; the remaining group (child base, imask, hit bits) goes on the stack.
+//  * triangles: Moller-Trumbore (lines 13-23), Woop unit-triangle rows (lines 24-38, t first), and the watertight test
+//    of Woop, Benthin and Wald (JCGT 2013) with its double-precision fallback on exact edge hits.
+// Every expression spells out its fused multiply-adds (the build uses -ffp-contract=off), so the host oracle evaluates
+// the same floating-point operations and finds the same hits.
+#pragma once
+#include "core/vecmath.h"
+#include "device/traverse_types.h"
+
+namespace cwp {
+
+struct Hit { float t, u, v; int tri; };
+
+// Per-ray constants shared by every node and triangle test.
+struct RayPre {
+    Vec3 o, d;
+    float tmin;
+    float ix, iy, iz;          // reciprocal direction, |d| clamped to 1e-20
+    uint32_t oct;              // bit k set: direction component k > 0
+    int kx, ky, kz;            // watertight: dominant axis kz and the two others
+    float Sx, Sy, Sz;          // watertight shear
+};
+
+__device__ __forceinline__ float safeRcp(float a) { const float e = 1e-20f; return 1.f / (fabsf(a) > e ? a : copysignf(e, a)); }
+
+template <int TRI>
+__device__ __forceinline__ RayPre prepareRay(Vec3 o, Vec3 d, float tmin) {
+    RayPre r;
+    r.o = o; r.d = d; r.tmin = tmin;
+    r.ix = safeRcp(d.x); r.iy = safeRcp(d.y); r.iz = safeRcp(d.z);
+    r.oct = (d.x > 0.f ? 1u : 0u) | (d.y > 0.f ? 2u : 0u) | (d.z > 0.f ? 4u : 0u);
+    r.kx = r.ky = r.kz = 0; r.Sx = r.Sy = r.Sz = 0.f;
+    if (TRI == kTriWatertight) {
+        float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+        int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+        if (comp(d, kz) < 0.f) { int t = kx; kx = ky; ky = t; }
+        r.kx = kx; r.ky = ky; r.kz = kz;
+        float dz = comp(d, kz);
+        r.Sx = comp(d, kx) / dz; r.Sy = comp(d, ky) / dz; r.Sz = 1.f / dz;
+    }
+    return r;
+}
+
+__device__ __forceinline__ Vec4 ldg4(const Vec4* p) { return *p; }
+
+// ------------------------------------------------------------------------------------------------ triangle tests
+// Tests triangle position t (leaf order). Updates the hit and returns true if it is closer than hit.t.
+template <int TRI>
+__device__ __forceinline__ bool triTest(const Vec4* __restrict__ tri, int t, const RayPre& r, Hit& h) {
+    if (TRI == kTriWoop) {
+        Vec4 r2 = ldg4(tri + 3 * t + 2);
+        float Oz = fmaf(r.o.z, r2.z, fmaf(r.o.y, r2.y, fmaf(r.o.x, r2.x, r2.w)));
+        float Dz = fmaf(r.d.z, r2.z, fmaf(r.d.y, r2.y, r.d.x * r2.x));
+        float tt = -Oz / Dz;
+        if (!(tt > r.tmin && tt < h.t)) return false;
+        Vec4 r0 = ldg4(tri + 3 * t);
+        float Ox = fmaf(r.o.z, r0.z, fmaf(r.o.y, r0.y, fmaf(r.o.x, r0.x, r0.w)));
+        float Dx = fmaf(r.d.z, r0.z, fmaf(r.d.y, r0.y, r.d.x * r0.x));
+        float u = fmaf(tt, Dx, Ox);
+        if (!(u >= 0.f && u <= 1.f)) return false;
+        Vec4 r1 = ldg4(tri + 3 * t + 1);
+        float Oy = fmaf(r.o.z, r1.z, fmaf(r.o.y, r1.y, fmaf(r.o.x, r1.x, r1.w)));
+        float Dy = fmaf(r.d.z, r1.z, fmaf(r.d.y, r1.y, r.d.x * r1.x));
+        float v = fmaf(tt, Dy, Oy);
+        if (!(v >= 0.f && u + v <= 1.f)) return false;
+        h.t = tt; h.u = u; h.v = v; h.tri = t;
+        return true;
+    } else if (TRI == kTriMT) {
+        Vec4 a = ldg4(tri + 3 * t), b = ldg4(tri + 3 * t + 1), c = ldg4(tri + 3 * t + 2);
+        Vec3 v0 = xyz(a), e1 = xyz(b) - v0, e2 = xyz(c) - v0;
+        Vec3 pv = cross(r.d, e2);
+        float det = dot(e1, pv);
+        if (fabsf(det) < 1e-12f) return false;
+        float inv = 1.f / det;
+        Vec3 tv = r.o - v0;
+        float u = dot(tv, pv) * inv;
+        if (u < 0.f || u > 1.f) return false;
+        Vec3 qv = cross(tv, e1);
+        float v = dot(r.d, qv) * inv;
+        if (v < 0.f || u + v > 1.f) return false;
+        float tt = dot(e2, qv) * inv;
+        if (!(tt > r.tmin && tt < h.t)) return false;
+        h.t = tt; h.u = u; h.v = v; h.tri = t;
+        return true;
+    } else {
+        Vec4 pa = ldg4(tri + 3 * t), pb = ldg4(tri + 3 * t + 1), pc = ldg4(tri + 3 * t + 2);
+        Vec3 A = xyz(pa) - r.o, B = xyz(pb) - r.o, C = xyz(pc) - r.o;
+        float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+        float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
+        float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
+        float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+        float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+        if (U == 0.f || V == 0.f || W == 0.f) {   // exact edge or vertex hit: re-evaluate the edge functions in double
+            U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+            V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+            W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+        }
+        if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+        float det = U + V + W;
+        if (det == 0.f) return false;
+        float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+        float T = U * Az + V * Bz + W * Cz;
+        float rcpDet = 1.f / det;
+        float tt = T * rcpDet;
+        if (!(tt > r.tmin && tt < h.t)) return false;
+        h.t = tt; h.u = V * rcpDet; h.v = W * rcpDet; h.tri = t;
+        return true;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ traversal stack
+template <typename E, int SL>
+struct LaneStack {
+    E* lds; int stride; E* spill; int sp;
+    __device__ __forceinline__ void push(E e) {
+        if (SL > 0 && sp < SL) lds[sp * stride] = e; else spill[sp - SL] = e;
+        sp++;
+    }
+    __device__ __forceinline__ E pop() {
+        sp--;
+        return (SL > 0 && sp < SL) ? lds[sp * stride] : spill[sp - SL];
+    }
+};
+
+struct Group { uint32_t base, bits; };   // BVH8 stack entry: child base, (imask << 8) | remaining hit bits
+
+// ------------------------------------------------------------------------------------------------ BVH2
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool traceBVH2(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, int root,
+                                          const RayPre& r, Hit& h, Stack& st) {
+    const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
+    int node = root;
+    for (;;) {
+        const Vec4* nb = nodes + (size_t)node * 4;
+        const Vec4 n0 = ldg4(nb), n1 = ldg4(nb + 1), nz = ldg4(nb + 2), mt = ldg4(nb + 3);
+        int c0 = asInt(mt.x), c1 = asInt(mt.y), k0 = asInt(mt.z), k1 = asInt(mt.w);
+        float a0 = fmaf(n0.x, r.ix, -odx), a1 = fmaf(n0.y, r.ix, -odx), a2 = fmaf(n0.z, r.iy, -ody), a3 = fmaf(n0.w, r.iy, -ody);
+        float a4 = fmaf(nz.x, r.iz, -odz), a5 = fmaf(nz.y, r.iz, -odz);
+        float b0 = fmaf(n1.x, r.ix, -odx), b1 = fmaf(n1.y, r.ix, -odx), b2 = fmaf(n1.z, r.iy, -ody), b3 = fmaf(n1.w, r.iy, -ody);
+        float b4 = fmaf(nz.z, r.iz, -odz), b5 = fmaf(nz.w, r.iz, -odz);
+        float t0n = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
+        float t0f = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h.t));
+        float t1n = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
+        float t1f = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h.t));
+        bool h0 = t0n <= t0f, h1 = t1n <= t1f;
+        if (h0 && k0 > 0) {
+            for (int t = c0; t < c0 + k0; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+            h0 = false;
+        }
+        if (h1 && k1 > 0) {
+            for (int t = c1; t < c1 + k1; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+            h1 = false;
+        }
+        if (!h0 && !h1) {
+            if (st.sp == 0) break;
+            node = st.pop();
+        } else if (h0 && h1) {
+            if (t1n < t0n) { int x = c0; c0 = c1; c1 = x; }
+            st.push(c1);
+            node = c0;
+        } else node = h0 ? c0 : c1;
+    }
+    return h.tri >= 0;
+}
+
+// ------------------------------------------------------------------------------------------------ compressed BVH8
+__device__ __forceinline__ uint32_t byteOf(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
+
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool traceBVH8(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, int root,
+                                          const RayPre& r, Hit& h, Stack& st) {
+    uint32_t gBase = 0, gMask = 0, gHits = 0;
+    int node = root;
+    for (;;) {
+        const Vec4* nb = nodes + (size_t)node * 5;
+        const Vec4 f0 = ldg4(nb), f1 = ldg4(nb + 1), f2 = ldg4(nb + 2), f3 = ldg4(nb + 3), f4 = ldg4(nb + 4);
+        const uint32_t ew = asUint(f0.w);
+        const float sx = asFloatU((ew & 255u) << 23) * r.ix;
+        const float sy = asFloatU(((ew >> 8) & 255u) << 23) * r.iy;
+        const float sz = asFloatU(((ew >> 16) & 255u) << 23) * r.iz;
+        const float ox = (f0.x - r.o.x) * r.ix, oy = (f0.y - r.o.y) * r.iy, oz = (f0.z - r.o.z) * r.iz;
+        const uint32_t imask = ew >> 24;
+        const uint32_t meta[2] = { asUint(f1.z), asUint(f1.w) };
+        const int childBase = asInt(f1.x), triBase = asInt(f1.y);
+        const uint32_t qlx[2] = { asUint(f2.x), asUint(f2.y) }, qly[2] = { asUint(f2.z), asUint(f2.w) };
+        const uint32_t qlz[2] = { asUint(f3.x), asUint(f3.y) }, qhx[2] = { asUint(f3.z), asUint(f3.w) };
+        const uint32_t qhy[2] = { asUint(f4.x), asUint(f4.y) }, qhz[2] = { asUint(f4.z), asUint(f4.w) };
+        uint32_t hits = 0;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int w = s >> 2;
+            const uint32_t m = byteOf(meta[w], s);
+            if (m == 0) continue;
+            float x0 = fmaf((float)byteOf(qlx[w], s), sx, ox), x1 = fmaf((float)byteOf(qhx[w], s), sx, ox);
+            float y0 = fmaf((float)byteOf(qly[w], s), sy, oy), y1 = fmaf((float)byteOf(qhy[w], s), sy, oy);
+            float z0 = fmaf((float)byteOf(qlz[w], s), sz, oz), z1 = fmaf((float)byteOf(qhz[w], s), sz, oz);
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h.t));
+            if (tn <= tf) {
+                if (m & 0x80u) {
+                    int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+                    for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+                } else hits |= 1u << (s ^ r.oct);
+            }
+        }
+        if (hits) {
+            if (gHits) st.push(Group{ gBase, (gMask << 8) | gHits });
+            gBase = (uint32_t)childBase; gMask = imask; gHits = hits;
+        }
+        while (gHits == 0) {
+            if (st.sp == 0) return h.tri >= 0;
+            Group e = st.pop();
+            gBase = e.base; gMask = e.bits >> 8; gHits = e.bits & 255u;
+        }
+        uint32_t k = (uint32_t)__builtin_ctz(gHits);
+        gHits &= gHits - 1;
+        uint32_t slot = k ^ r.oct;
+        node = (int)gBase + __builtin_popcount(gMask & ((1u << slot) - 1u));
+    }
+}
+
+}  // namespace cwp

@@ -1,0 +1,78 @@
+// host/mesh.h - indexed triangle meshes, boxes, and the procedural scenes used by the CLI, the tests and the bench.
+#pragma once
+#include "core/vecmath.h"
+#include <cfloat>
+#include <string>
+#include <vector>
+
+namespace cwp {
+
+struct Tri { int a, b, c; };
+
+// Indexed triangle mesh with a per-triangle material id and a per-material Lambert albedo table
+// (reference: src/scene/mesh.h:9-26; material names come from OBJ usemtl).
+struct Mesh {
+    std::vector<Vec3> v;       // vertex positions
+    std::vector<Vec3> n;       // optional per-vertex normals (same size as v) or empty
+    std::vector<Tri> f;        // triangles
+    std::vector<int> fm;       // per-triangle material id (empty: all 0)
+    std::vector<Vec3> kd;      // albedo per material id
+    std::vector<std::string> matName;
+    bool hasN() const { return !n.empty() && n.size() == v.size(); }
+    int mat(size_t t) const { return fm.empty() ? 0 : fm[t]; }
+};
+
+// Axis-aligned box; empty boxes are inverted (reference: src/accel/aabb.h:6-13).
+struct AABB {
+    Vec3 lo = v3(FLT_MAX, FLT_MAX, FLT_MAX), hi = v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    void grow(Vec3 p) { lo = vmin(lo, p); hi = vmax(hi, p); }
+    void grow(const AABB& b) { lo = vmin(lo, b.lo); hi = vmax(hi, b.hi); }
+    // surface area 2(dx dy + dy dz + dz dx); 0 for an empty box
+    float area() const {
+        if (hi.x < lo.x) return 0.f;
+        Vec3 d = hi - lo;
+        return 2.f * (d.x * d.y + d.y * d.z + d.z * d.x);
+    }
+    Vec3 extent() const { return hi - lo; }
+    Vec3 center() const { return (lo + hi) * 0.5f; }
+};
+
+AABB triBox(const Mesh& m, size_t t);
+AABB meshBox(const Mesh& m);
+
+// Appends an axis-aligned box (12 triangles, outward winding) with material mat.
+void addBox(Mesh& m, Vec3 lo, Vec3 hi, int mat);
+
+// Seeded splitmix64 stream; next01() gives a float in [0, 1) with 24 random bits.
+struct SplitMix {
+    uint64_t s;
+    explicit SplitMix(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float next01() { return (float)(next() >> 40) * (1.0f / 16777216.0f); }
+    float uniform(float a, float b) { return a + (b - a) * next01(); }
+};
+
+// Box-cloud scene: nBoxes axis-aligned boxes, centres uniform in [0,1]^3, edge lengths uniform in [0.01, 0.08], plus a
+// 2-triangle ground square [-1,2]^2 at z = 0; one material of albedo `albedo`.
+void makeBoxCloud(Mesh& m, int nBoxes, uint64_t seed, float albedo);
+// Default view of the box cloud (row-major camera-to-world, OpenGL camera axes, z-up world).
+void boxCloudCamera(float c2w[16]);
+
+// Terrain scene: grid x grid vertex fBm heightfield over [0,1]^2 (value noise, `octaves` octaves, lacunarity 2, gain 0.5,
+// peak-to-peak amplitude `amp`), 2 (grid-1)^2 triangles, plus nBoxes boxes resting on it; one material of albedo `albedo`.
+void makeTerrain(Mesh& m, int grid, int octaves, float amp, int nBoxes, uint64_t seed, float albedo);
+void terrainCamera(float c2w[16]);
+
+// The reference's built-in `--synthetic N` scene: ground (gs x gs quads, gently waved) and N x N bumpy spheres of
+// nu x nv quads (reference: src/scene/synthetic.cpp:6-50; its default view: src/scene/synthetic.cpp:50).
+void makeSphereGrid(Mesh& m, int gridN, int nu, int nv, int gs);
+void sphereGridCamera(float c2w[16]);
+
+// Subdivided icosahedron of radius 1 at the origin (20 * 4^level triangles).
+void makeIcosphere(Mesh& m, int level, int mat);
+
+}  // namespace cwp

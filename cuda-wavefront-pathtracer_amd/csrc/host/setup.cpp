@@ -1,0 +1,214 @@
+// host/setup.cpp - option parsing and scene setup (see host/setup.h).
+#include "host/setup.h"
+#include "host/io.h"
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace cwp {
+
+bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
+    for (int i = 1; i < argc; i++) {
+        const std::string k = argv[i];
+        auto need = [&](int n) {
+            if (i + n >= argc) { err = "option " + k + " needs " + std::to_string(n) + " value(s)"; return false; }
+            return true;
+        };
+        auto str = [&](std::string& dst) { if (!need(1)) return false; dst = argv[++i]; return true; };
+        auto num = [&](float& dst) { if (!need(1)) return false; dst = (float)atof(argv[++i]); return true; };
+        auto inum = [&](int& dst) { if (!need(1)) return false; dst = atoi(argv[++i]); return true; };
+        auto vec = [&](Vec3& dst) {
+            if (!need(3)) return false;
+            dst = v3((float)atof(argv[i + 1]), (float)atof(argv[i + 2]), (float)atof(argv[i + 3])); i += 3; return true;
+        };
+        bool ok = true;
+        if (k == "--mesh") ok = str(A.mesh);
+        else if (k == "--camera") ok = str(A.camera);
+        else if (k == "--out") ok = str(A.out);
+        else if (k == "--variant") ok = str(A.variant);
+        else if (k == "--c2w_key") ok = str(A.c2wKey);
+        else if (k == "--c2w") ok = str(A.c2w);
+        else if (k == "--tri") ok = str(A.tri);
+        else if (k == "--lighting") ok = str(A.lighting);
+        else if (k == "--integrator") ok = str(A.integrator);
+        else if (k == "--builder") ok = str(A.builder);
+        else if (k == "--collapse") ok = str(A.collapse);
+        else if (k == "--gen") ok = str(A.gen);
+        else if (k == "--gen_boxes") ok = inum(A.genBoxes);
+        else if (k == "--gen_grid") ok = inum(A.genGrid);
+        else if (k == "--gen_seed") { int s = 0; ok = inum(s); A.genSeed = (uint64_t)(unsigned)s; }
+        else if (k == "--synthetic") ok = inum(A.synthetic);
+        else if (k == "--frame") ok = inum(A.frame);
+        else if (k == "--width") ok = inum(A.w);
+        else if (k == "--height") ok = inum(A.h);
+        else if (k == "--spp") ok = inum(A.spp);
+        else if (k == "--bounces") ok = inum(A.bounces);
+        else if (k == "--batch") ok = inum(A.batch);
+        else if (k == "--leaf") ok = inum(A.leaf);
+        else if (k == "--bins") ok = inum(A.bins);
+        else if (k == "--fuse") ok = inum(A.fuse);
+        else if (k == "--persistent") ok = inum(A.persistent);
+        else if (k == "--smooth") ok = inum(A.smooth);
+        else if (k == "--warm") ok = inum(A.warm);
+        else if (k == "--reps") ok = inum(A.reps);
+        else if (k == "--stack") ok = inum(A.stack);
+        else if (k == "--blocks_per_cu") ok = inum(A.blocksPerCU);
+        else if (k == "--time_kernels") ok = inum(A.timeKernels);
+        else if (k == "--filter") { std::string f; ok = str(f); A.filterBH = f == "bh"; if (ok && f != "bh" && f != "box") { err = "--filter must be bh or box"; return false; } }
+        else if (k == "--fx") ok = num(A.fx);
+        else if (k == "--fy") ok = num(A.fy);
+        else if (k == "--hfov") ok = num(A.hfov);
+        else if (k == "--aperture") ok = num(A.aperture);
+        else if (k == "--focus_dist") ok = num(A.focusDist);
+        else if (k == "--eps") ok = num(A.epsScale);
+        else if (k == "--near") ok = num(A.nearT);
+        else if (k == "--sun_angle") ok = num(A.light.sunAngleDeg);
+        else if (k == "--seed") { int s = 0; ok = inum(s); A.seed = (unsigned)s; }
+        else if (k == "--sun_to") ok = vec(A.light.sunTo);
+        else if (k == "--sun_rgb") ok = vec(A.light.sunE);
+        else if (k == "--sky") ok = vec(A.light.sky);
+        else if (k == "--albedo") ok = vec(A.albedo);
+        else if (k == "--devloop" || k == "--sort") {
+            int v = 0; ok = inum(v);
+            if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
+        } else { err = "unknown arg " + k; return false; }
+        if (!ok) return false;
+    }
+    if (A.variant != "bvh2" && A.variant != "bvh8") { err = "--variant must be bvh2 or bvh8"; return false; }
+    if (A.tri != "mt" && A.tri != "woop" && A.tri != "watertight") { err = "--tri must be mt, woop or watertight"; return false; }
+    if (A.integrator != "auto" && A.integrator != "legacy") { err = "--integrator must be auto or legacy"; return false; }
+    if (A.builder != "sah") { err = "--builder must be sah"; return false; }
+    if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
+    if (A.fuse != 1) { err = "--fuse must be 1 (fused trace+shade kernel)"; return false; }
+    if (A.bins < 2 || A.bins > 256) { err = "--bins must be in [2, 256]"; return false; }
+    if (A.leaf < 1 || A.leaf > 32) { err = "--leaf must be in [1, 32]"; return false; }
+    if (A.w < 1 || A.h < 1) { err = "--width/--height must be positive"; return false; }
+    if (A.batch < 1) A.batch = 1;
+    if (A.bounces < 0) { err = "--bounces must be >= 0"; return false; }
+    if (A.stack != 8 && A.stack != 16 && A.stack != 32) { err = "--stack must be 8, 16 or 32"; return false; }
+    if (!A.gen.empty() && A.gen != "boxes" && A.gen != "terrain") { err = "--gen must be boxes or terrain"; return false; }
+    return true;
+}
+
+static double msSince(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bool setupScene(SceneSetup& Z, std::string& err) {
+    Args& A = Z.A;
+    Mesh& M = Z.M;
+    float* c2w = Z.c2w;
+    identityC2W(c2w);
+    bool haveC2w = false;
+    // ---- scene ----
+    if (A.gen == "boxes") {
+        makeBoxCloud(M, A.genBoxes >= 0 ? A.genBoxes : 1024, A.genSeed, 0.7f);
+        boxCloudCamera(c2w); haveC2w = true; Z.sceneName = "gen:boxes";
+    } else if (A.gen == "terrain") {
+        makeTerrain(M, A.genGrid, 6, 0.2f, A.genBoxes >= 0 ? A.genBoxes : 200, A.genSeed, 0.8f);
+        terrainCamera(c2w); haveC2w = true; Z.sceneName = "gen:terrain";
+    } else if (A.synthetic) {
+        makeSphereGrid(M, A.synthetic, 200, 100, 200);
+        sphereGridCamera(c2w); haveC2w = true; Z.sceneName = "synthetic";
+    } else {
+        if (A.mesh.empty()) { err = "no scene: give --mesh, --gen or --synthetic"; return false; }
+        if (!loadMesh(A.mesh, M)) { err = "failed to load mesh " + A.mesh; return false; }
+        Z.sceneName = A.mesh;
+    }
+    if (A.synthetic && A.hfov == 0 && A.fx == 0) A.hfov = 50.f;
+    if (!A.smooth) M.n.clear();
+    if (!A.lighting.empty() && !loadLightingFile(A.lighting, A.light, &M, err)) return false;
+    if (A.albedo.x >= 0) for (auto& kd : M.kd) kd = A.albedo;
+    if (M.kd.empty()) M.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+    { int maxMat = 0; for (int t : M.fm) maxMat = t > maxMat ? t : maxMat;
+      while ((int)M.kd.size() <= maxMat) M.kd.push_back(v3(0.8f, 0.8f, 0.8f)); }
+    AABB sb = meshBox(M);
+    Z.extent = maxc(sb.extent());
+    // ---- camera ----
+    if (!A.camera.empty()) {
+        if (!loadCameraFile(A.camera, A.c2wKey, A.frame, A.w, A.h, A.fx, A.fy, A.hfov, c2w, A.aperture, A.focusDist, err)) return false;
+        haveC2w = true;
+    }
+    if (!A.c2w.empty()) {
+        std::string s = A.c2w; for (auto& ch : s) if (ch == ',') ch = ' ';
+        const char* q = s.c_str(); char* e;
+        for (int j = 0; j < 16; j++) { c2w[j] = (float)strtod(q, &e); if (e == q) { err = "--c2w needs 16 numbers"; return false; } q = e; }
+        haveC2w = true;
+    }
+    if (!haveC2w) { err = "no camera"; return false; }
+    if (A.fx == 0) { if (A.hfov == 0) A.hfov = 50.f; A.fx = focalFromHfov(A.w, A.hfov); }
+    if (A.fy == 0) A.fy = A.fx;
+    FrameParams& H = Z.P;
+    memset(&H, 0, sizeof H);
+    H.cam = makeCamera(c2w, A.fx, A.fy, A.w, A.h);
+    if (A.aperture > 0.f) {
+        if (!(A.focusDist > 0.f)) { err = "thin lens: --aperture needs --focus_dist"; return false; }
+        H.cam.lensR = A.aperture; H.cam.focusD = A.focusDist;
+    }
+    H.sunTo = normalize(A.light.sunTo); H.sunE = A.light.sunE; H.sky = A.light.sky;
+    H.sunCos = cosf(0.5f * A.light.sunAngleDeg * 3.14159265f / 180.f);
+    H.maxB = A.bounces; H.seed = A.seed; H.eps = A.epsScale * Z.extent; H.nearT = A.nearT;
+    H.hasN = M.hasN() ? 1 : 0; H.filterBH = A.filterBH;
+    blackmanHarrisTable(H.ftab, kFilterTable, 1.5);
+    // ---- BVH ----
+    auto t0 = std::chrono::steady_clock::now();
+    const int V = Z.variant = A.variant == "bvh2" ? 2 : 8;
+    const int leafMax = V == 8 ? 1 : A.leaf;   // the CWBVH collapse forms its own leaves from a 1-primitive-leaf tree
+    Z.reportLeaf = V == 8 ? 3 : leafMax;
+    buildSAH(M, A.bins, leafMax, Z.tree);
+    Z.treeMs = msSince(t0);
+    Z.maxDepth = Z.tree.maxDepth;
+    SceneUpload& S = Z.S;
+    S.variant = V; S.root = 0;
+    if (V == 2) {
+        layoutBVH2(Z.tree, S.nodes); S.order = Z.tree.prim; Z.sah = sahCost(Z.tree, 1, 1);
+        S.stackNeed = bvh2Depth(S.nodes) + 1;
+        if (Z.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
+    } else {
+        WideBVH W; collapseCWBVH(Z.tree, W);
+        S.nodes.swap(W.nodes); S.order.swap(W.prim); Z.sah = W.sah;
+        S.stackNeed = W.depth + 1;
+    }
+    std::vector<Vec4> nrm;
+    packVerts(M, S.order, S.verts, S.normals);
+    packWoop(M, S.order, S.woop);
+    S.albedo.clear();
+    for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
+    Z.bvhS = msSince(t0) / 1000.0;
+    Z.nNodes = S.nodes.size() / (V == 2 ? 4 : 5);
+    Z.nodeBytes = S.nodes.size() * 16;
+    // ---- renderer options ----
+    RenderOptions& O = Z.O;
+    O.variant = V; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
+    O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
+    return true;
+}
+
+std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& st, int rep, int frame) {
+    const Args& A = Z.A;
+    const double tot = st.rays + st.shadowRays;
+    const int N = A.w * A.h;
+    char buf[4096];
+    snprintf(buf, sizeof buf,
+             "{\"rep\":%d,\"variant\":\"bvh%d\",\"tri\":\"%s\",\"persistent\":%d,\"devloop\":0,\"sort\":0,\"fuse\":1,\"leaf\":%d,"
+             "\"batch\":%d,\"minb\":0,\"scene\":\"%s\",\"tris\":%zu,\"bvh_nodes\":%zu,\"bvh_bytes\":%zu,\"bvh_depth\":%d,"
+             "\"bvh_build_s\":%.2f,\"w\":%d,\"h\":%d,\"spp\":%d,\"bounces\":%d,\"rays_closest\":%.0f,\"rays_shadow\":%.0f,"
+             "\"rays_total\":%.0f,\"t_trace_ms\":%.2f,\"t_frame_ms\":%.2f,\"mrays_s_trace\":%.1f,\"mrays_s_e2e\":%.1f,"
+             "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
+             "\"seed\":%u,\"integrator\":\"legacy\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
+             "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":0.00,\"bvh_sah\":%.4f,\"graph\":0,"
+             "\"instances\":0,\"flatten\":0,\"frame\":%d,\"tlas_ms\":0.000,\"bins\":%d,\"stack_lds\":%d,"
+             "\"t_bounce_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
+             rep, Z.variant, A.tri.c_str(), A.persistent, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
+             Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
+             st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
+             st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
+             Z.P.hasN, A.seed, Z.P.cam.lensR, Z.P.cam.focusD, "importance", A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
+             Z.treeMs, Z.sah, frame, A.bins, R.options().stackLds, st.bounceMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
+             st.unoccluded);
+    return buf;
+}
+
+}  // namespace cwp

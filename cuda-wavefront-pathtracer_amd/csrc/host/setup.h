@@ -1,0 +1,52 @@
+// host/setup.h - command-line options and scene setup of the batch renderer, shared by the pt CLI and the C ABI.
+// Mirrors the reference's Args / parseArgs / setupScene (src/app/scene_setup.h:12-50, src/app/scene_setup.cpp:27-271)
+// for the legacy (Lambert + sun + sky) integrator: load a mesh or generate a procedural scene, read lighting and camera
+// files, build the binned-SAH BVH2 and its GPU layout, pack triangles, and fill the frame parameters.
+#pragma once
+#include "device/renderer.h"
+#include "host/bvh.h"
+#include "host/camera.h"
+#include "host/mesh.h"
+#include <string>
+#include <vector>
+
+namespace cwp {
+
+struct Args {
+    std::string mesh, camera, out = "out", variant = "bvh8", c2wKey = "c2w_blender_zup", tri = "woop", c2w, lighting;
+    std::string gen;                 // procedural scene: "boxes" (1,024-box cloud) or "terrain" (fBm heightfield + boxes)
+    std::string integrator = "auto", builder = "sah", collapse = "sah";
+    int genBoxes = -1, genGrid = 242; uint64_t genSeed = 1;
+    int fuse = 1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
+    int persistent = 1, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 16, blocksPerCU = 0, timeKernels = 1;
+    float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
+    unsigned seed = 1000;
+    Vec3 albedo = v3(-1, 0, 0);
+    Lighting light;
+};
+
+// Parses the options; false with a message in err on an unknown option, a missing value or an invalid value.
+bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err);
+
+struct SceneSetup {
+    Args A;
+    Mesh M;
+    BinaryBVH tree;                  // binary SAH tree (before the layout)
+    SceneUpload S;                   // GPU-ready scene
+    FrameParams P;
+    RenderOptions O;
+    float c2w[16];
+    float extent = 0.f;
+    int variant = 8, maxDepth = 0, reportLeaf = 0;
+    double bvhS = 0, treeMs = 0, sah = 0;
+    size_t nNodes = 0, nodeBytes = 0;
+    std::string sceneName;
+};
+
+// Runs the setup from Z.A; false with a message in err.
+bool setupScene(SceneSetup& Z, std::string& err);
+
+// One JSON object (no newline) with the reference's per-frame statistics keys (src/app/pt.cpp:35-41).
+std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& st, int rep, int frame);
+
+}  // namespace cwp

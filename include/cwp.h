@@ -1,0 +1,144 @@
+/* cwp.h - C ABI of the MI355X wavefront path tracer (libcwp.so).
+ *
+ * Plain pointers and sizes only. Every call returns CWP_OK (0) or an error code; cwp_last_error() gives the message of
+ * the calling thread's last failure. Nothing in the library exits the process. Calls on one context are not
+ * thread-safe; distinct contexts are independent. Inputs are copied; outputs go to caller buffers.
+ *
+ * The entry points replace the reference's statically linked C++ interfaces (paths relative to the reference repo):
+ *   scene setup      src/app/scene_setup.h:12-50    parseArgs / setupScene / SceneSetup
+ *   renderer         src/render/renderer.h:47-93    WavefrontRenderer::{init, render, readAccum, setParams}
+ *   BVH build        src/accel/bvh_builder.h:9-16, src/accel/bvh_layouts.h:14-40, src/accel/bvh_tools.h:23-29
+ *   traversal        src/accel/traverse.cuh:155-160 trav<V, ANY, TRI> (as a kernel over explicit rays)
+ *   frame statistics src/render/renderer.h:45 FrameStats, src/app/pt.cpp:35-41 (JSON line)
+ */
+#ifndef CWP_H
+#define CWP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CWP_OK = 0,
+    CWP_ERR_ARG = 1,      /* invalid argument or option */
+    CWP_ERR_IO = 2,       /* unreadable / malformed input file */
+    CWP_ERR_DEVICE = 3,   /* HIP runtime failure or no device */
+    CWP_ERR_STATE = 4     /* call not valid in the object's state */
+};
+
+typedef struct cwp_setup cwp_setup; /* parsed options + scene + BVH + packed triangles (host memory) */
+typedef struct cwp_ctx cwp_ctx;     /* renderer: device buffers, stream, kernels */
+typedef struct cwp_bvh cwp_bvh;     /* BVH built from caller arrays (host memory) */
+
+/* Frame parameters of the legacy integrator (camera, sun, sky, bounce limit, seed, ray offset, filter table).
+ * Replaces the reference's Params block (src/render/params.h:8-16). */
+typedef struct cwp_params {
+    float pos[3], ax[3], ay[3], az[3]; /* camera position and camera-to-world axes (looks down -az) */
+    float fx, fy, px, py;              /* focal lengths and principal point in pixels */
+    int w, h;                          /* image size */
+    float lens_r, focus_d;             /* thin lens (lens_r = 0: pinhole) */
+    float sun_to[3], sun_e[3], sky[3]; /* unit direction toward the sun, sun irradiance, sky radiance */
+    float sun_cos;                     /* cos of the sun's angular radius */
+    int max_bounce;                    /* paths stop after this bounce (camera hit = 0) */
+    uint32_t seed;                     /* RNG seed */
+    float eps, near_t;                 /* ray-origin offset (world units), camera near clip */
+    int has_normals, filter_bh;        /* interpolate vertex normals; Blackman-Harris 1.5 px filter (else 1 px box) */
+    float ftab[65];                    /* filter inverse CDF */
+} cwp_params;
+
+/* Per-frame counters and times. Replaces FrameStats (src/render/renderer.h:45) plus the event counters used for
+ * the bytes-per-ray accounting. */
+typedef struct cwp_frame_stats {
+    double rays_closest, rays_shadow;  /* rays traced: camera + continuation, shadow */
+    double t_trace_ms, t_frame_ms;     /* trace+shadow kernel time, whole frame (hipEvent pair) */
+    double t_bounce_ms, t_shadow_ms;   /* fused trace+shade kernels, shadow kernels */
+    int bounce_launches, shadow_launches;
+    double rays_hit, rays_miss, rays_cont, shadow_unoccluded;
+    double camera_miss;               /* camera rays that miss (rays_miss includes them) */
+} cwp_frame_stats;
+
+/* Scene and BVH summary of a setup. */
+typedef struct cwp_scene_info {
+    int variant;        /* 2 (BVH2) or 8 (compressed BVH8) */
+    int tri_test;       /* 0 Moller-Trumbore, 1 Woop, 2 watertight */
+    int num_tris, num_verts, num_materials;
+    int num_nodes, node_floats4;  /* nodes and float4 per node */
+    int bvh_depth;      /* binary SAH tree depth */
+    int stack_need;     /* traversal stack entries the layout can require */
+    double sah, bvh_build_s;
+    int width, height, spp, batch, bounces;
+} cwp_scene_info;
+
+/* Array selectors for cwp_setup_array / cwp_bvh_array (element type, count unit). */
+enum {
+    CWP_ARR_VERTICES = 0, /* float[3] per mesh vertex */
+    CWP_ARR_TRIANGLES = 1,/* int[3] per triangle (mesh order) */
+    CWP_ARR_MATERIAL = 2, /* int per triangle: material id */
+    CWP_ARR_ALBEDO = 3,   /* float[4] per material */
+    CWP_ARR_NODES = 4,    /* float[4] x node_floats4 per node (GPU layout) */
+    CWP_ARR_ORDER = 5,    /* int per leaf position: original triangle id */
+    CWP_ARR_WOOP = 6,     /* float[4] x 3 per leaf position: Woop rows */
+    CWP_ARR_CORNERS = 7,  /* float[4] x 3 per leaf position: (v0, id), (v1, material), (v2, 0) */
+    CWP_ARR_NORMALS = 8,  /* float[3] per mesh vertex (empty without normals) */
+    CWP_ARR_TREE = 9      /* int[4] per binary SAH node: left, right, first, count; then float[6] box via CWP_ARR_TREE_BOX */,
+    CWP_ARR_TREE_BOX = 10,/* float[6] per binary SAH node: lo.xyz, hi.xyz */
+    CWP_ARR_PRIM = 11     /* int per binary-tree primitive slot: triangle id */
+};
+
+const char* cwp_last_error(void);
+int cwp_version(void);
+
+/* Number of compute units and the name of HIP device 0. */
+int cwp_device_info(int* num_cu, char* name, size_t name_cap);
+
+/* Scene setup from command-line style options (the reference's pt flags, e.g. "--mesh", "--camera", "--lighting",
+ * "--width", "--spp", "--batch", "--bounces", "--variant", "--tri", plus "--gen boxes|terrain", "--bins", "--stack").
+ * argv[0] is ignored. Replaces parseArgs + setupScene (src/app/scene_setup.cpp:27-57, 85-271). Host only. */
+int cwp_setup_create(int argc, const char* const* argv, cwp_setup** out);
+void cwp_setup_destroy(cwp_setup* s);
+int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info);
+int cwp_setup_params(const cwp_setup* s, cwp_params* params);
+/* Borrowed view of one of the setup's arrays (valid until cwp_setup_destroy); *count = number of elements. */
+int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* count);
+
+/* Renderer: uploads the setup's scene and allocates path state for width * height * batch paths.
+ * Replaces WavefrontRenderer::init (src/render/wavefront.cu:469-502). */
+int cwp_create(const cwp_setup* s, cwp_ctx** out);
+void cwp_destroy(cwp_ctx* c);
+/* Renders spp samples per pixel (a multiple of batch), sample indices [sample_base, sample_base + spp), into the
+ * accumulation buffer (cleared first if clear != 0). Replaces WavefrontRenderer::render (src/render/wavefront.cu:684). */
+int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats* stats);
+/* Per-pixel radiance sums (divide by spp), float[4] per pixel, rows top to bottom. Replaces readAccum. */
+int cwp_read_accum(cwp_ctx* c, float* rgba, size_t num_pixels);
+/* Replaces setParams (src/render/wavefront.cu:744); the image size must not change. */
+int cwp_set_params(cwp_ctx* c, const cwp_params* params);
+/* Closest (any_hit = 0) or any hit of n rays with the context's BVH and triangle test. rays: device pointer to
+ * 8 floats per ray (o.xyz, tmin, d.xyz, tmax); hits: device pointer to 4 floats per ray (t, u, v, leaf position as
+ * int bits, -1 = miss). Synchronous. Kernel form of trav<V, ANY, TRI> (src/accel/traverse.cuh:155). */
+int cwp_trace(cwp_ctx* c, const void* rays, int n, int any_hit, void* hits);
+/* Same as cwp_trace with host arrays: the library stages the rays and hits through device buffers it owns. */
+int cwp_trace_host(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits);
+/* One JSON object (no newline) with the reference's per-frame statistics keys (src/app/pt.cpp:35-41). */
+int cwp_frame_json(const cwp_setup* s, const cwp_ctx* c, const cwp_frame_stats* st, int rep, int frame, char* buf, size_t cap);
+/* Grid size (workgroups) and VRAM used by the context. */
+int cwp_ctx_info(const cwp_ctx* c, int* grid_blocks, double* vram_mb);
+
+/* Binned-SAH BVH2 over caller triangles and its GPU layout: variant 2 (BVH2, leaves <= leaf_max) or 8 (CWBVH collapsed
+ * from a 1-primitive-leaf tree). Replaces BVHBuilder::build + BVH2Flat / CWBVH::build (src/accel/bvh_builder.cpp:5,
+ * src/accel/bvh_layouts.cpp:31, src/accel/bvh_tools.cpp:75). Host only. */
+int cwp_bvh_build(const float* vertices, int num_vertices, const int* triangles, int num_tris, int variant, int bins,
+                  int leaf_max, cwp_bvh** out);
+void cwp_bvh_destroy(cwp_bvh* b);
+/* CWP_ARR_NODES, CWP_ARR_ORDER, CWP_ARR_WOOP, CWP_ARR_CORNERS, CWP_ARR_TREE, CWP_ARR_TREE_BOX, CWP_ARR_PRIM */
+int cwp_bvh_array(const cwp_bvh* b, int which, const void** data, size_t* count);
+/* depth of the binary tree, traversal stack need, SAH cost of the layout */
+int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stack_need, double* sah);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CWP_H */

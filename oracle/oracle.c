@@ -1,0 +1,805 @@
+/* oracle.c - CPU restatement of the reference's wavefront hot path (plain C11, IEEE, -ffp-contract=off).
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for libcwp.so. Only the tests, the smoke check and bench.py's
+ * CPU-baseline leg use it; the product never links it.
+ *
+ * What pins each part (the reference itself cannot be built here: its headers need the CUDA toolkit's
+ * cuda_runtime.h for the float3/float4 types, and it has no golden vectors; see tests/test_oracle.py):
+ *  - orc_hash32 / orc_rng4: known-answer values of the hash (the reference's constants) and uniformity statistics.
+ *  - orc_blackman_harris_table: monotone, antisymmetric, endpoints +-width/2, and the tabulated CDF matches the
+ *    closed-form integral of the Blackman-Harris window.
+ *  - warps: unit length, pdf moments (cosine hemisphere: E[cos] = 2/3; uniform cone: E[cos] = (1 + cos_max)/2).
+ *  - orc_build_sah: every triangle exactly once, node boxes equal the union of their triangles' boxes, leaf sizes,
+ *    depth cap, and a hand-computable split on two separated clusters; bin count 16 vs 32 both checked.
+ *  - orc_collapse_cwbvh / orc_layout_bvh2: dequantised child boxes contain the exact child boxes, every triangle
+ *    referenced once, <= 3 triangles per leaf, and traversal equals brute force.
+ *  - orc_trace: closest hits equal orc_trace_brute (every triangle tested) on random and scene rays; any-hit agrees
+ *    with closest-hit occlusion; the watertight test leaves no gaps on shared edges (rays aimed at shared vertices).
+ *  - orc_render: closed forms (infinite Lambert plane under sky: a*L; plus sun: a*(L + E cos / pi) on average),
+ *    sky-only miss image = L, progressive accumulation = single call.
+ * Functions with no check beyond these are marked "parity unpinned" in the test file.
+ */
+#include "oracle.h"
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct { float x, y, z; } f3;
+static inline f3 mk(float x, float y, float z) { f3 r = { x, y, z }; return r; }
+static inline f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 mul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline f3 scl(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 crs(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline f3 nrm3(f3 a) { float l = sqrtf(dot3(a, a)); return l > 0.f ? scl(a, 1.0f / l) : a; }
+static inline f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+static inline float cmp3(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+static inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* ------------------------------------------------------------------------------------------ RNG: src/core/rng.h:6-12 */
+uint32_t orc_hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+float orc_rng4(uint32_t pix, uint32_t smp, uint32_t depth, uint32_t dim, uint32_t seed) {
+    uint32_t a = orc_hash32((depth * 8u + dim) * 0xC2B2AE3DU ^ seed);
+    uint32_t b = orc_hash32(smp * 0x85EBCA77U ^ a);
+    uint32_t h = orc_hash32(pix * 0x9E3779B1U ^ b);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+/* -------------------------------------------------------------------------------- warps: src/core/sampling.h:13-58 */
+static void onb(f3 n, f3* t, f3* b) {
+    float s = n.z >= 0.f ? 1.f : -1.f;
+    float a = -1.f / (s + n.z);
+    float c = n.x * n.y * a;
+    *t = mk(1.f + s * n.x * n.x * a, s * c, -s * n.x);
+    *b = mk(c, s + n.y * n.y * a, -n.y);
+}
+static f3 cos_hemi(f3 n, float u0, float u1) {
+    float r = sqrtf(u0), ph = 6.2831853f * u1;
+    f3 t, b; onb(n, &t, &b);
+    f3 x = scl(t, r * cosf(ph)), y = scl(b, r * sinf(ph)), z = scl(n, sqrtf(fmaxf(0.f, 1.f - u0)));
+    return nrm3(add(add(x, y), z));
+}
+static f3 cone(f3 axis, float cmax, float u0, float u1) {
+    float cz = 1.f - u0 * (1.f - cmax);
+    float sz = sqrtf(fmaxf(0.f, 1.f - cz * cz));
+    float ph = 6.2831853f * u1;
+    f3 t, b; onb(axis, &t, &b);
+    f3 x = scl(axis, cz), y = scl(t, sz * cosf(ph)), z = scl(b, sz * sinf(ph));
+    return nrm3(add(add(x, y), z));
+}
+void orc_cosine_hemisphere(const float n[3], float u0, float u1, float out[3]) {
+    f3 r = cos_hemi(ld3(n), u0, u1); out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_uniform_cone(const float axis[3], float cmax, float u0, float u1, float out[3]) {
+    f3 r = cone(ld3(axis), cmax, u0, u1); out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+static float table_lookup(const float* tab, int n, float u) {
+    float f = u * (float)n;
+    int k = (int)f; if (k > n - 1) k = n - 1;
+    float t = f - (float)k;
+    return tab[k] * (1.f - t) + tab[k + 1] * t;
+}
+static void concentric(float u0, float u1, float* x, float* y) {
+    float a = 2.f * u0 - 1.f, b = 2.f * u1 - 1.f;
+    if (a == 0.f && b == 0.f) { *x = 0.f; *y = 0.f; return; }
+    float r, ph;
+    if (fabsf(a) > fabsf(b)) { r = a; ph = 0.78539816f * (b / a); } else { r = b; ph = 1.57079633f - 0.78539816f * (a / b); }
+    *x = r * cosf(ph); *y = r * sinf(ph);
+}
+void orc_blackman_harris_table(float* tab, int n, double width) {
+    enum { CELLS = 4096 };
+    double* c = (double*)malloc(sizeof(double) * (CELLS + 1));
+    c[0] = 0.0;
+    for (int i = 0; i < CELLS; i++) {
+        double t = (i + 0.5) / CELLS;
+        double f = 0.35875 - 0.48829 * cos(2 * M_PI * t) + 0.14128 * cos(4 * M_PI * t) - 0.01168 * cos(6 * M_PI * t);
+        c[i + 1] = c[i] + fmax(f, 0.0);
+    }
+    for (int i = 0; i <= CELLS; i++) c[i] /= c[CELLS];
+    for (int k = 0; k <= n; k++) {
+        double u = (double)k / n;
+        int j = 0;
+        while (j < CELLS - 1 && c[j + 1] < u) j++;
+        double t = c[j + 1] > c[j] ? (u - c[j]) / (c[j + 1] - c[j]) : 0.0;
+        tab[k] = (float)(((j + t) / CELLS - 0.5) * width);
+    }
+    free(c);
+}
+
+/* ------------------------------------------------------------------------ boxes (src/accel/aabb.h:6-13) */
+typedef struct { f3 lo, hi; } box3;
+static box3 box_empty(void) { box3 b = { { FLT_MAX, FLT_MAX, FLT_MAX }, { -FLT_MAX, -FLT_MAX, -FLT_MAX } }; return b; }
+static void grow_p(box3* b, f3 p) {
+    b->lo = mk(fminf(b->lo.x, p.x), fminf(b->lo.y, p.y), fminf(b->lo.z, p.z));
+    b->hi = mk(fmaxf(b->hi.x, p.x), fmaxf(b->hi.y, p.y), fmaxf(b->hi.z, p.z));
+}
+static void grow_b(box3* b, box3 o) {
+    b->lo = mk(fminf(b->lo.x, o.lo.x), fminf(b->lo.y, o.lo.y), fminf(b->lo.z, o.lo.z));
+    b->hi = mk(fmaxf(b->hi.x, o.hi.x), fmaxf(b->hi.y, o.hi.y), fmaxf(b->hi.z, o.hi.z));
+}
+static float box_area(box3 b) {
+    if (b.hi.x < b.lo.x) return 0.f;
+    f3 d = sub(b.hi, b.lo);
+    return 2.f * (d.x * d.y + d.y * d.z + d.z * d.x);
+}
+
+/* growable int / float arrays */
+typedef struct { int* p; int n, cap; } ivec;
+typedef struct { float* p; int n, cap; } fvec;
+static void ipush(ivec* v, int x) { if (v->n == v->cap) { v->cap = v->cap ? 2 * v->cap : 64; v->p = (int*)realloc(v->p, sizeof(int) * v->cap); } v->p[v->n++] = x; }
+static void fgrow(fvec* v, int n) { if (n > v->cap) { int c = v->cap ? v->cap : 64; while (c < n) c *= 2; v->p = (float*)realloc(v->p, sizeof(float) * c); v->cap = c; } if (n > v->n) memset(v->p + v->n, 0, sizeof(float) * (n - v->n)); v->n = n; }
+
+/* ------------------------------------------------------------ binned SAH BVH2: src/accel/bvh_builder.cpp:5-68 */
+typedef struct {
+    const box3* tb; const f3* cent; int* idx;
+    ivec node;   /* 4 ints per node */
+    fvec box;    /* 6 floats per node */
+    int bins, leaf_max, max_depth;
+    box3* bb; int* bc; float* rA; int* rC;
+} sah_ctx;
+
+static int bin_of(float c, float lo, float scale, int bins) {
+    int b = (int)((c - lo) * scale);
+    return b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
+}
+static int new_node(sah_ctx* S) {
+    int id = S->node.n / 4;
+    for (int k = 0; k < 4; k++) ipush(&S->node, k < 2 ? -1 : 0);
+    fgrow(&S->box, S->box.n + 6);
+    return id;
+}
+static int sah_rec(sah_ctx* S, int begin, int end, int depth) {
+    int id = new_node(S);
+    if (depth > S->max_depth) S->max_depth = depth;
+    box3 box = box_empty(), cb = box_empty();
+    int n = end - begin;
+    for (int i = begin; i < end; i++) { grow_b(&box, S->tb[S->idx[i]]); grow_p(&cb, S->cent[S->idx[i]]); }
+    float* bx = S->box.p + 6 * id;
+    bx[0] = box.lo.x; bx[1] = box.lo.y; bx[2] = box.lo.z; bx[3] = box.hi.x; bx[4] = box.hi.y; bx[5] = box.hi.z;
+    if (n <= S->leaf_max || depth >= 62) { S->node.p[4 * id + 2] = begin; S->node.p[4 * id + 3] = n; return id; }
+    float best = FLT_MAX; int bestAxis = -1, bestBin = -1;
+    f3 ext = sub(cb.hi, cb.lo);
+    const int B = S->bins;
+    for (int axis = 0; axis < 3; axis++) {
+        float lo = cmp3(cb.lo, axis), e = cmp3(ext, axis);
+        if (e <= 1e-12f) continue;
+        float scale = (float)B / e;
+        for (int b = 0; b < B; b++) { S->bb[b] = box_empty(); S->bc[b] = 0; }
+        for (int i = begin; i < end; i++) {
+            int b = bin_of(cmp3(S->cent[S->idx[i]], axis), lo, scale, B);
+            grow_b(&S->bb[b], S->tb[S->idx[i]]); S->bc[b]++;
+        }
+        box3 acc = box_empty(); int cnt = 0;
+        for (int b = B - 1; b > 0; b--) { grow_b(&acc, S->bb[b]); cnt += S->bc[b]; S->rA[b] = box_area(acc); S->rC[b] = cnt; }
+        box3 lacc = box_empty(); int lcnt = 0;
+        for (int b = 0; b < B - 1; b++) {
+            grow_b(&lacc, S->bb[b]); lcnt += S->bc[b];
+            if (lcnt == 0 || S->rC[b + 1] == 0) continue;
+            float cost = box_area(lacc) * (float)lcnt + S->rA[b + 1] * (float)S->rC[b + 1];
+            if (cost < best) { best = cost; bestAxis = axis; bestBin = b; }
+        }
+    }
+    int mid;
+    if (bestAxis < 0) mid = (begin + end) / 2;
+    else {
+        float lo = cmp3(cb.lo, bestAxis), e = cmp3(ext, bestAxis), scale = (float)B / e;
+        /* two-sided in-place partition (left scan for a failing element, right scan for a passing one, swap) */
+        int* first = S->idx + begin; int* last = S->idx + end;
+        for (;;) {
+            while (first != last && bin_of(cmp3(S->cent[*first], bestAxis), lo, scale, B) <= bestBin) first++;
+            if (first == last) break;
+            last--;
+            while (first != last && !(bin_of(cmp3(S->cent[*last], bestAxis), lo, scale, B) <= bestBin)) last--;
+            if (first == last) break;
+            int t = *first; *first = *last; *last = t;
+            first++;
+        }
+        mid = (int)(first - S->idx);
+        if (mid == begin || mid == end) mid = (begin + end) / 2;
+    }
+    int l = sah_rec(S, begin, mid, depth + 1);
+    int r = sah_rec(S, mid, end, depth + 1);
+    S->node.p[4 * id] = l; S->node.p[4 * id + 1] = r;
+    return id;
+}
+
+int orc_build_sah(const float* v, int nv, const int* tri, int nt, int bins, int leaf_max, orc_tree* out) {
+    if (nt < 1 || bins < 2 || leaf_max < 1) return 1;
+    for (int t = 0; t < 3 * nt; t++) if (tri[t] < 0 || tri[t] >= nv) return 1;
+    sah_ctx S; memset(&S, 0, sizeof S);
+    box3* tb = (box3*)malloc(sizeof(box3) * nt);
+    f3* cent = (f3*)malloc(sizeof(f3) * nt);
+    int* idx = (int*)malloc(sizeof(int) * nt);
+    for (int t = 0; t < nt; t++) {
+        box3 b = box_empty();
+        for (int k = 0; k < 3; k++) grow_p(&b, ld3(v + 3 * tri[3 * t + k]));
+        tb[t] = b; cent[t] = scl(add(b.lo, b.hi), 0.5f); idx[t] = t;
+    }
+    S.tb = tb; S.cent = cent; S.idx = idx; S.bins = bins; S.leaf_max = leaf_max;
+    S.bb = (box3*)malloc(sizeof(box3) * bins); S.bc = (int*)malloc(sizeof(int) * bins);
+    S.rA = (float*)malloc(sizeof(float) * bins); S.rC = (int*)malloc(sizeof(int) * bins);
+    sah_rec(&S, 0, nt, 0);
+    out->num_nodes = S.node.n / 4; out->num_prims = nt; out->max_depth = S.max_depth;
+    out->node = S.node.p; out->box = S.box.p; out->prim = idx;
+    free(tb); free(cent); free(S.bb); free(S.bc); free(S.rA); free(S.rC);
+    return 0;
+}
+void orc_tree_free(orc_tree* t) { free(t->node); free(t->box); free(t->prim); memset(t, 0, sizeof *t); }
+
+static box3 tree_box(const orc_tree* t, int n) {
+    const float* b = t->box + 6 * n; box3 r = { { b[0], b[1], b[2] }, { b[3], b[4], b[5] } }; return r;
+}
+
+/* ------------------------------------------------------------------- BVH2 layout: src/accel/bvh_layouts.cpp:6-30 */
+typedef struct { const int* node; const float* box; int wrapped_root; } b2src;
+static int b2_left(const orc_tree* t, int n) { return n == t->num_nodes ? 0 : t->node[4 * n]; }
+static int b2_right(const orc_tree* t, int n) { return n == t->num_nodes ? -1 : t->node[4 * n + 1]; }
+static int b2_leaf(const orc_tree* t, int n) { return n < t->num_nodes && t->node[4 * n + 3] > 0; }
+static int b2_emit(const orc_tree* t, int n, fvec* out, int level, int* depth) {
+    int id = out->n / 16;
+    fgrow(out, out->n + 16);
+    if (level > *depth) *depth = level;
+    int ch[2] = { b2_left(t, n), b2_right(t, n) }, c[2], k[2];
+    box3 bx[2];
+    for (int j = 0; j < 2; j++) {
+        if (ch[j] < 0) { c[j] = -1; k[j] = 0; bx[j].lo = bx[j].hi = mk(3e37f, 3e37f, 3e37f); continue; }
+        bx[j] = tree_box(t, ch[j]);
+        if (b2_leaf(t, ch[j])) { c[j] = t->node[4 * ch[j] + 2]; k[j] = t->node[4 * ch[j] + 3]; }
+        else { k[j] = 0; c[j] = b2_emit(t, ch[j], out, level + 1, depth); }
+    }
+    float* o = out->p + 16 * id;
+    o[0] = bx[0].lo.x; o[1] = bx[0].hi.x; o[2] = bx[0].lo.y; o[3] = bx[0].hi.y;
+    o[4] = bx[1].lo.x; o[5] = bx[1].hi.x; o[6] = bx[1].lo.y; o[7] = bx[1].hi.y;
+    o[8] = bx[0].lo.z; o[9] = bx[0].hi.z; o[10] = bx[1].lo.z; o[11] = bx[1].hi.z;
+    o[12] = i2f(c[0]); o[13] = i2f(c[1]); o[14] = i2f(k[0]); o[15] = i2f(k[1]);
+    return id;
+}
+int orc_layout_bvh2(const orc_tree* t, orc_layout* out) {
+    fvec v = { 0, 0, 0 };
+    int depth = 0;
+    /* a leaf root is wrapped in an internal node whose second child is empty (index num_nodes = the wrapper) */
+    int root = t->node[3] > 0 ? t->num_nodes : 0;
+    b2_emit(t, root, &v, 1, &depth);
+    out->num_float4 = v.n / 4; out->nodes = v.p; out->num_prims = t->num_prims;
+    out->order = (int*)malloc(sizeof(int) * t->num_prims); memcpy(out->order, t->prim, sizeof(int) * t->num_prims);
+    out->depth = depth; out->max_leaf = 0; out->sah = 0;
+    double a0 = box_area(tree_box(t, 0)), s = 0;
+    if (a0 > 0) {
+        for (int n = 0; n < t->num_nodes; n++) {
+            double a = box_area(tree_box(t, n));
+            s += t->node[4 * n + 3] > 0 ? a * t->node[4 * n + 3] : a;
+        }
+        out->sah = s / a0;
+    }
+    return 0;
+}
+void orc_layout_free(orc_layout* l) { free(l->nodes); free(l->order); memset(l, 0, sizeof *l); }
+
+/* ------------------------------------------------------- CWBVH collapse: src/accel/bvh_tools.cpp:75-168 (Ylitie 2017) */
+enum { D_LEAF = 0, D_NODE = 1, D_PREV = 2, D_DIST = 10 };
+typedef struct {
+    box3* box; int *L, *R, *first, *cnt; int n, cap;
+    float* C; unsigned char* D; unsigned char* K8;
+    const int* prim;
+} cw_t;
+static void cw_push(cw_t* w, box3 b, int l, int r, int f, int c) {
+    if (w->n == w->cap) {
+        w->cap = w->cap ? 2 * w->cap : 64;
+        w->box = (box3*)realloc(w->box, sizeof(box3) * w->cap);
+        w->L = (int*)realloc(w->L, sizeof(int) * w->cap); w->R = (int*)realloc(w->R, sizeof(int) * w->cap);
+        w->first = (int*)realloc(w->first, sizeof(int) * w->cap); w->cnt = (int*)realloc(w->cnt, sizeof(int) * w->cap);
+    }
+    w->box[w->n] = b; w->L[w->n] = l; w->R[w->n] = r; w->first[w->n] = f; w->cnt[w->n] = c; w->n++;
+}
+static void cw_collect(const cw_t* w, int n, int i, ivec* out) {
+    for (;;) {
+        if (i == 1) { ipush(out, n); return; }
+        unsigned char d = w->D[(size_t)n * 8 + i];
+        if (d == D_PREV) { i--; continue; }
+        int k = d - D_DIST;
+        cw_collect(w, w->L[n], k, out);
+        cw_collect(w, w->R[n], i - k, out);
+        return;
+    }
+}
+static void cw_prims(const cw_t* w, int n, ivec* out) {
+    ivec st = { 0, 0, 0 }; ipush(&st, n);
+    while (st.n) {
+        int h = st.p[--st.n];
+        if (w->L[h] < 0) { for (int t = 0; t < w->cnt[h]; t++) ipush(out, w->prim[w->first[h] + t]); }
+        else { ipush(&st, w->R[h]); ipush(&st, w->L[h]); }
+    }
+    free(st.p);
+}
+static int qexp(float ext, float lo, float hi) {
+    if (!(ext > 0.f)) return -100;
+    int e = (int)ceilf(log2f(ext / 255.f));
+    if (e < -100) e = -100;
+    while (lo + 255.f * ldexpf(1.f, e) < hi) e++;
+    return e;
+}
+static int quant(float v, float o, float sc, int up) {
+    float f = (v - o) / sc;
+    int i = up ? (int)ceilf(f) : (int)floorf(f);
+    return i < 0 ? 0 : (i > 255 ? 255 : i);
+}
+static uint32_t pk4(const int* q, int b) { return (uint32_t)q[b] | ((uint32_t)q[b + 1] << 8) | ((uint32_t)q[b + 2] << 16) | ((uint32_t)q[b + 3] << 24); }
+
+int orc_collapse_cwbvh(const orc_tree* t, orc_layout* out) {
+    const int pMax = 3; const float cNode = 1.f, cPrim = 0.3f, INF = 3e38f;
+    cw_t w; memset(&w, 0, sizeof w); w.prim = t->prim;
+    for (int i = 0; i < t->num_nodes; i++) {
+        const int* nd = t->node + 4 * i; int leaf = nd[3] > 0;
+        cw_push(&w, tree_box(t, i), leaf ? -1 : nd[0], leaf ? -1 : nd[1], nd[2], leaf ? nd[3] : 0);
+    }
+    for (int i = 0; i < w.n; i++) {   /* oversize leaves -> binary subtrees with the same box */
+        if (w.L[i] >= 0 || w.cnt[i] <= pMax) continue;
+        int h = w.cnt[i] / 2, a = w.n, b = a + 1, f0 = w.first[i], c0 = w.cnt[i]; box3 bx = w.box[i];
+        cw_push(&w, bx, -1, -1, f0, h); cw_push(&w, bx, -1, -1, f0 + h, c0 - h);
+        w.L[i] = a; w.R[i] = b; w.cnt[i] = 0;
+    }
+    const int N = w.n;
+    w.C = (float*)calloc((size_t)N * 8, sizeof(float)); w.D = (unsigned char*)calloc((size_t)N * 8, 1);
+    w.K8 = (unsigned char*)malloc(N); memset(w.K8, 1, N);
+    int* p = (int*)calloc(N, sizeof(int));
+    for (int n = N - 1; n >= 0; n--) {
+        float A = box_area(w.box[n]);
+        float* c = w.C + (size_t)n * 8; unsigned char* d = w.D + (size_t)n * 8;
+        if (w.L[n] < 0) {
+            p[n] = w.cnt[n];
+            float cl = A * (float)p[n] * cPrim;
+            for (int i = 1; i < 8; i++) { c[i] = cl; d[i] = i == 1 ? D_LEAF : D_PREV; }
+            continue;
+        }
+        int l = w.L[n], r = w.R[n];
+        p[n] = p[l] + p[r];
+        const float *cl_ = w.C + (size_t)l * 8, *cr_ = w.C + (size_t)r * 8;
+        float dist[9]; int dk[9];
+        for (int j = 2; j <= 8; j++) {
+            dist[j] = INF; dk[j] = 1;
+            for (int k = 1; k < j; k++) { float x = cl_[k] + cr_[j - k]; if (x < dist[j]) { dist[j] = x; dk[j] = k; } }
+        }
+        float cLeaf = p[n] <= pMax ? A * (float)p[n] * cPrim : INF;
+        float cInt = dist[8] + A * cNode;
+        if (cLeaf <= cInt) { c[1] = cLeaf; d[1] = D_LEAF; } else { c[1] = cInt; d[1] = D_NODE; }
+        w.K8[n] = (unsigned char)dk[8];
+        for (int i = 2; i < 8; i++) {
+            if (dist[i] < c[i - 1]) { c[i] = dist[i]; d[i] = (unsigned char)(D_DIST + dk[i]); }
+            else { c[i] = c[i - 1]; d[i] = D_PREV; }
+        }
+    }
+    const float Ar = box_area(w.box[0]);
+    int rootLeaf = w.L[0] < 0 || w.D[1] == D_LEAF;
+    if (rootLeaf) out->sah = Ar > 0 ? (Ar * (float)p[0] * cPrim + Ar * cNode) / Ar : 0.f;
+    else {
+        float d8 = INF;
+        for (int k = 1; k < 8; k++) { float x = w.C[(size_t)w.L[0] * 8 + k] + w.C[(size_t)w.R[0] * 8 + 8 - k]; d8 = x < d8 ? x : d8; }
+        out->sah = Ar > 0 ? (d8 + Ar * cNode) / Ar : 0.f;
+    }
+    fvec nodes = { 0, 0, 0 }; ivec order = { 0, 0, 0 };
+    fgrow(&nodes, 20);
+    ivec jb = { 0, 0, 0 };   /* job stack: (binary node, output node, level, asLeaf) */
+    ipush(&jb, 0); ipush(&jb, 0); ipush(&jb, 1); ipush(&jb, rootLeaf);
+    ivec ch = { 0, 0, 0 }, pr = { 0, 0, 0 };
+    int depth = 0, maxLeaf = 0;
+    while (jb.n) {
+        int asLeaf = jb.p[jb.n - 1], level = jb.p[jb.n - 2], outId = jb.p[jb.n - 3], n = jb.p[jb.n - 4];
+        jb.n -= 4;
+        if (level > depth) depth = level;
+        ch.n = 0;
+        if (asLeaf) ipush(&ch, n); else { cw_collect(&w, w.L[n], w.K8[n], &ch); cw_collect(&w, w.R[n], 8 - w.K8[n], &ch); }
+        box3 box = w.box[n];
+        f3 pc = scl(add(box.lo, box.hi), 0.5f);
+        int slot[8]; float cost[8][8]; int used[8] = { 0 };
+        for (int s = 0; s < 8; s++) slot[s] = -1;
+        for (int c = 0; c < ch.n; c++) {
+            f3 cc = sub(scl(add(w.box[ch.p[c]].lo, w.box[ch.p[c]].hi), 0.5f), pc);
+            for (int s = 0; s < 8; s++)
+                cost[c][s] = -(cc.x * (s & 1 ? -1.f : 1.f) + cc.y * (s & 2 ? -1.f : 1.f) + cc.z * (s & 4 ? -1.f : 1.f));
+        }
+        for (int it = 0; it < ch.n; it++) {
+            float bc = INF; int bi = -1, bs = -1;
+            for (int c = 0; c < ch.n; c++)
+                if (!used[c]) for (int s = 0; s < 8; s++) if (slot[s] < 0 && cost[c][s] < bc) { bc = cost[c][s]; bi = c; bs = s; }
+            for (int c = 0; c < ch.n && bi < 0; c++)
+                if (!used[c]) for (int s = 0; s < 8; s++) if (slot[s] < 0) { bi = c; bs = s; break; }
+            used[bi] = 1; slot[bs] = ch.p[bi];
+        }
+        f3 ext = sub(box.hi, box.lo);
+        int ex = qexp(ext.x, box.lo.x, box.hi.x), ey = qexp(ext.y, box.lo.y, box.hi.y), ez = qexp(ext.z, box.lo.z, box.hi.z);
+        float sx = ldexpf(1.f, ex), sy = ldexpf(1.f, ey), sz = ldexpf(1.f, ez);
+        int qlx[8], qly[8], qlz[8], qhx[8], qhy[8], qhz[8], meta[8], childLeaf[8];
+        int triBase = order.n, nInt = 0; uint32_t imask = 0;
+        for (int s = 0; s < 8; s++) {
+            qlx[s] = qly[s] = qlz[s] = qhx[s] = qhy[s] = qhz[s] = meta[s] = childLeaf[s] = 0;
+            int cn = slot[s];
+            if (cn < 0) continue;
+            box3 cb = w.box[cn];
+            qlx[s] = quant(cb.lo.x, box.lo.x, sx, 0); qhx[s] = quant(cb.hi.x, box.lo.x, sx, 1);
+            qly[s] = quant(cb.lo.y, box.lo.y, sy, 0); qhy[s] = quant(cb.hi.y, box.lo.y, sy, 1);
+            qlz[s] = quant(cb.lo.z, box.lo.z, sz, 0); qhz[s] = quant(cb.hi.z, box.lo.z, sz, 1);
+            if (asLeaf || w.L[cn] < 0 || w.D[(size_t)cn * 8 + 1] == D_LEAF) {
+                pr.n = 0; cw_prims(&w, cn, &pr);
+                int off = order.n - triBase;
+                for (int k = 0; k < pr.n; k++) ipush(&order, pr.p[k]);
+                meta[s] = 0x80 | ((pr.n - 1) << 5) | off;
+                if (pr.n > maxLeaf) maxLeaf = pr.n;
+                childLeaf[s] = 1;
+            } else { meta[s] = 0x40; imask |= 1u << s; nInt++; }
+        }
+        int childBase = nodes.n / 20;
+        fgrow(&nodes, nodes.n + 20 * nInt);
+        float* o = nodes.p + (size_t)outId * 20;
+        uint32_t ew = (uint32_t)(ex + 127) | ((uint32_t)(ey + 127) << 8) | ((uint32_t)(ez + 127) << 16) | (imask << 24);
+        o[0] = box.lo.x; o[1] = box.lo.y; o[2] = box.lo.z; o[3] = u2f(ew);
+        o[4] = i2f(childBase); o[5] = i2f(triBase); o[6] = u2f(pk4(meta, 0)); o[7] = u2f(pk4(meta, 4));
+        uint32_t u[12] = { pk4(qlx, 0), pk4(qlx, 4), pk4(qly, 0), pk4(qly, 4), pk4(qlz, 0), pk4(qlz, 4),
+                           pk4(qhx, 0), pk4(qhx, 4), pk4(qhy, 0), pk4(qhy, 4), pk4(qhz, 0), pk4(qhz, 4) };
+        for (int j = 0; j < 12; j++) o[8 + j] = u2f(u[j]);
+        int ord = 0;
+        for (int s = 0; s < 8; s++)
+            if (slot[s] >= 0 && !childLeaf[s]) { ipush(&jb, slot[s]); ipush(&jb, childBase + ord); ipush(&jb, level + 1); ipush(&jb, 0); ord++; }
+    }
+    out->num_float4 = nodes.n / 4; out->nodes = nodes.p; out->num_prims = order.n; out->order = order.p;
+    out->depth = depth; out->max_leaf = maxLeaf;
+    free(jb.p); free(ch.p); free(pr.p); free(p);
+    free(w.box); free(w.L); free(w.R); free(w.first); free(w.cnt); free(w.C); free(w.D); free(w.K8);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ triangle packing: src/accel/tri_pack.cpp */
+void orc_pack_woop(const float* v, const int* tri, const int* order, int n, float* rows) {
+    for (int k = 0; k < n; k++) {
+        const int* f = tri + 3 * order[k];
+        f3 v0 = ld3(v + 3 * f[0]), e1 = sub(ld3(v + 3 * f[1]), v0), e2 = sub(ld3(v + 3 * f[2]), v0), nn = crs(e1, e2);
+        double m[3][3] = { { e1.x, e2.x, nn.x }, { e1.y, e2.y, nn.y }, { e1.z, e2.z, nn.z } }, iv[3][3];
+        double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        memset(iv, 0, sizeof iv);
+        if (!(fabs(det) < 1e-300)) {
+            double id = 1.0 / det;
+            iv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id; iv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+            iv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id; iv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+            iv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id; iv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+            iv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id; iv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+            iv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+        }
+        for (int r = 0; r < 3; r++) {
+            double tr = -(iv[r][0] * v0.x + iv[r][1] * v0.y + iv[r][2] * v0.z);
+            float* o = rows + 12 * k + 4 * r;
+            o[0] = (float)iv[r][0]; o[1] = (float)iv[r][1]; o[2] = (float)iv[r][2]; o[3] = (float)tr;
+        }
+    }
+}
+void orc_pack_corners(const float* v, const int* tri, const int* mat, const int* order, int n, float* c) {
+    for (int k = 0; k < n; k++) {
+        int t = order[k]; const int* f = tri + 3 * t;
+        for (int j = 0; j < 3; j++) { memcpy(c + 12 * k + 4 * j, v + 3 * f[j], 12); }
+        c[12 * k + 3] = i2f(t); c[12 * k + 7] = i2f(mat ? mat[t] : 0); c[12 * k + 11] = 0.f;
+    }
+}
+
+/* ---------------------------------------------------------------------- triangle tests: src/accel/traverse.cuh:10-42 */
+typedef struct {
+    f3 o, d; float tmin; float ix, iy, iz; uint32_t oct; int kx, ky, kz; float Sx, Sy, Sz;
+} ray_t;
+typedef struct { float t, u, v; int tri; } hit_t;
+static float safe_rcp(float a) { const float e = 1e-20f; return 1.f / (fabsf(a) > e ? a : copysignf(e, a)); }
+static ray_t mkray(f3 o, f3 d, float tmin, int tri_test) {
+    ray_t r; memset(&r, 0, sizeof r);
+    r.o = o; r.d = d; r.tmin = tmin;
+    r.ix = safe_rcp(d.x); r.iy = safe_rcp(d.y); r.iz = safe_rcp(d.z);
+    r.oct = (d.x > 0.f ? 1u : 0u) | (d.y > 0.f ? 2u : 0u) | (d.z > 0.f ? 4u : 0u);
+    if (tri_test == 2) {   /* watertight: dominant axis kz, kx/ky keep winding; shear constants */
+        float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+        int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+        if (cmp3(d, kz) < 0.f) { int t = kx; kx = ky; ky = t; }
+        r.kx = kx; r.ky = ky; r.kz = kz;
+        float dz = cmp3(d, kz);
+        r.Sx = cmp3(d, kx) / dz; r.Sy = cmp3(d, ky) / dz; r.Sz = 1.f / dz;
+    }
+    return r;
+}
+static int tri_test(const orc_scene* s, int t, const ray_t* r, hit_t* h) {
+    if (s->tri_test == 1) {   /* Woop: t from row 2 first, then u, v */
+        const float* R = s->woop + 12 * t;
+        float Oz = fmaf(r->o.z, R[10], fmaf(r->o.y, R[9], fmaf(r->o.x, R[8], R[11])));
+        float Dz = fmaf(r->d.z, R[10], fmaf(r->d.y, R[9], r->d.x * R[8]));
+        float tt = -Oz / Dz;
+        if (!(tt > r->tmin && tt < h->t)) return 0;
+        float Ox = fmaf(r->o.z, R[2], fmaf(r->o.y, R[1], fmaf(r->o.x, R[0], R[3])));
+        float Dx = fmaf(r->d.z, R[2], fmaf(r->d.y, R[1], r->d.x * R[0]));
+        float u = fmaf(tt, Dx, Ox);
+        if (!(u >= 0.f && u <= 1.f)) return 0;
+        float Oy = fmaf(r->o.z, R[6], fmaf(r->o.y, R[5], fmaf(r->o.x, R[4], R[7])));
+        float Dy = fmaf(r->d.z, R[6], fmaf(r->d.y, R[5], r->d.x * R[4]));
+        float v = fmaf(tt, Dy, Oy);
+        if (!(v >= 0.f && u + v <= 1.f)) return 0;
+        h->t = tt; h->u = u; h->v = v; h->tri = t;
+        return 1;
+    }
+    const float* P = s->corners + 12 * t;
+    f3 p0 = ld3(P), p1 = ld3(P + 4), p2 = ld3(P + 8);
+    if (s->tri_test == 0) {   /* Moller-Trumbore on (v0, e1 = v1 - v0, e2 = v2 - v0) */
+        f3 e1 = sub(p1, p0), e2 = sub(p2, p0);
+        f3 pv = crs(r->d, e2);
+        float det = dot3(e1, pv);
+        if (fabsf(det) < 1e-12f) return 0;
+        float inv = 1.f / det;
+        f3 tv = sub(r->o, p0);
+        float u = dot3(tv, pv) * inv;
+        if (u < 0.f || u > 1.f) return 0;
+        f3 qv = crs(tv, e1);
+        float v = dot3(r->d, qv) * inv;
+        if (v < 0.f || u + v > 1.f) return 0;
+        float tt = dot3(e2, qv) * inv;
+        if (!(tt > r->tmin && tt < h->t)) return 0;
+        h->t = tt; h->u = u; h->v = v; h->tri = t;
+        return 1;
+    }
+    /* watertight (Woop, Benthin, Wald 2013) */
+    f3 A = sub(p0, r->o), B = sub(p1, r->o), C = sub(p2, r->o);
+    float Akz = cmp3(A, r->kz), Bkz = cmp3(B, r->kz), Ckz = cmp3(C, r->kz);
+    float Ax = cmp3(A, r->kx) - r->Sx * Akz, Ay = cmp3(A, r->ky) - r->Sy * Akz;
+    float Bx = cmp3(B, r->kx) - r->Sx * Bkz, By = cmp3(B, r->ky) - r->Sy * Bkz;
+    float Cx = cmp3(C, r->kx) - r->Sx * Ckz, Cy = cmp3(C, r->ky) - r->Sy * Ckz;
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return 0;
+    float det = U + V + W;
+    if (det == 0.f) return 0;
+    float Az = r->Sz * Akz, Bz = r->Sz * Bkz, Cz = r->Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.f / det, tt = T * rcp;
+    if (!(tt > r->tmin && tt < h->t)) return 0;
+    h->t = tt; h->u = V * rcp; h->v = W * rcp; h->tri = t;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ BVH2 traversal: src/accel/traverse.cuh:45-70 */
+static int trav2(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    int stack[128], sp = 0, node = s->root;
+    const float odx = r->o.x * r->ix, ody = r->o.y * r->iy, odz = r->o.z * r->iz;
+    for (;;) {
+        const float* n = s->nodes + 16 * (size_t)node;
+        if (nv) *nv += 1;
+        int c0 = f2i(n[12]), c1 = f2i(n[13]), k0 = f2i(n[14]), k1 = f2i(n[15]);
+        float a0 = fmaf(n[0], r->ix, -odx), a1 = fmaf(n[1], r->ix, -odx), a2 = fmaf(n[2], r->iy, -ody), a3 = fmaf(n[3], r->iy, -ody);
+        float a4 = fmaf(n[8], r->iz, -odz), a5 = fmaf(n[9], r->iz, -odz);
+        float b0 = fmaf(n[4], r->ix, -odx), b1 = fmaf(n[5], r->ix, -odx), b2 = fmaf(n[6], r->iy, -ody), b3 = fmaf(n[7], r->iy, -ody);
+        float b4 = fmaf(n[10], r->iz, -odz), b5 = fmaf(n[11], r->iz, -odz);
+        float t0n = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r->tmin));
+        float t0f = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h->t));
+        float t1n = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r->tmin));
+        float t1f = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h->t));
+        int h0 = t0n <= t0f, h1 = t1n <= t1f;
+        if (h0 && k0 > 0) {
+            for (int t = c0; t < c0 + k0; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
+            h0 = 0;
+        }
+        if (h1 && k1 > 0) {
+            for (int t = c1; t < c1 + k1; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
+            h1 = 0;
+        }
+        if (!h0 && !h1) { if (sp == 0) break; node = stack[--sp]; }
+        else if (h0 && h1) { if (t1n < t0n) { int x = c0; c0 = c1; c1 = x; } stack[sp++] = c1; node = c0; }
+        else node = h0 ? c0 : c1;
+    }
+    return h->tri >= 0;
+}
+
+/* ---------------------------------------------------------------- BVH8 traversal: src/accel/traverse.cuh:112-152 */
+static uint32_t byte_of(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
+static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    uint32_t stB[128], stM[128]; int sp = 0;
+    uint32_t gB = 0, gM = 0, gH = 0;
+    int node = s->root;
+    for (;;) {
+        const float* n = s->nodes + 20 * (size_t)node;
+        if (nv) *nv += 1;
+        uint32_t ew = f2u(n[3]);
+        float sx = u2f((ew & 255u) << 23) * r->ix, sy = u2f(((ew >> 8) & 255u) << 23) * r->iy, sz = u2f(((ew >> 16) & 255u) << 23) * r->iz;
+        float ox = (n[0] - r->o.x) * r->ix, oy = (n[1] - r->o.y) * r->iy, oz = (n[2] - r->o.z) * r->iz;
+        uint32_t imask = ew >> 24, meta[2] = { f2u(n[6]), f2u(n[7]) };
+        int childBase = f2i(n[4]), triBase = f2i(n[5]);
+        uint32_t qlx[2] = { f2u(n[8]), f2u(n[9]) }, qly[2] = { f2u(n[10]), f2u(n[11]) }, qlz[2] = { f2u(n[12]), f2u(n[13]) };
+        uint32_t qhx[2] = { f2u(n[14]), f2u(n[15]) }, qhy[2] = { f2u(n[16]), f2u(n[17]) }, qhz[2] = { f2u(n[18]), f2u(n[19]) };
+        uint32_t hits = 0;
+        for (int sl = 0; sl < 8; sl++) {
+            int w = sl >> 2;
+            uint32_t m = byte_of(meta[w], sl);
+            if (m == 0) continue;
+            float x0 = fmaf((float)byte_of(qlx[w], sl), sx, ox), x1 = fmaf((float)byte_of(qhx[w], sl), sx, ox);
+            float y0 = fmaf((float)byte_of(qly[w], sl), sy, oy), y1 = fmaf((float)byte_of(qhy[w], sl), sy, oy);
+            float z0 = fmaf((float)byte_of(qlz[w], sl), sz, oz), z1 = fmaf((float)byte_of(qhz[w], sl), sz, oz);
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r->tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h->t));
+            if (tn <= tf) {
+                if (m & 0x80u) {
+                    int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+                    for (int t = first; t < first + cnt; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
+                } else hits |= 1u << (sl ^ r->oct);
+            }
+        }
+        if (hits) {
+            if (gH) { stB[sp] = gB; stM[sp] = (gM << 8) | gH; sp++; }
+            gB = (uint32_t)childBase; gM = imask; gH = hits;
+        }
+        while (gH == 0) {
+            if (sp == 0) return h->tri >= 0;
+            sp--; gB = stB[sp]; gM = stM[sp] >> 8; gH = stM[sp] & 255u;
+        }
+        uint32_t k = (uint32_t)__builtin_ctz(gH);
+        gH &= gH - 1;
+        uint32_t slot = k ^ r->oct;
+        node = (int)gB + __builtin_popcount(gM & ((1u << slot) - 1u));
+    }
+}
+
+static int trace_one(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    return s->variant == 2 ? trav2(s, r, h, any, nv, nt) : trav8(s, r, h, any, nv, nt);
+}
+
+/* --------------------------------------------------------------------------------------------- threading helper */
+typedef struct { void (*fn)(void*, int, int); void* arg; int tid, nthreads; } job_t;
+static void* job_main(void* a) { job_t* j = (job_t*)a; j->fn(j->arg, j->tid, j->nthreads); return NULL; }
+static void run_threads(void (*fn)(void*, int, int), void* arg, int threads) {
+    if (threads < 1) threads = 1;
+    if (threads == 1) { fn(arg, 0, 1); return; }
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    job_t* jobs = (job_t*)malloc(sizeof(job_t) * threads);
+    for (int t = 0; t < threads; t++) { jobs[t].fn = fn; jobs[t].arg = arg; jobs[t].tid = t; jobs[t].nthreads = threads; pthread_create(&th[t], NULL, job_main, &jobs[t]); }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th); free(jobs);
+}
+
+typedef struct { const orc_scene* s; const float* rays; int n, any, brute; float* hits; } trace_job;
+static void trace_worker(void* a, int tid, int nth) {
+    trace_job* J = (trace_job*)a;
+    for (int k = tid; k < J->n; k += nth) {
+        const float* R = J->rays + 8 * (size_t)k;
+        ray_t r = mkray(ld3(R), ld3(R + 4), R[3], J->s->tri_test);
+        hit_t h = { R[7], 0.f, 0.f, -1 };
+        if (J->brute) { for (int t = 0; t < J->s->num_tris; t++) tri_test(J->s, t, &r, &h); }
+        else trace_one(J->s, &r, &h, J->any, NULL, NULL);
+        float* o = J->hits + 4 * (size_t)k;
+        o[0] = h.t; o[1] = h.u; o[2] = h.v; o[3] = i2f(h.tri);
+    }
+}
+int orc_trace(const orc_scene* s, const float* rays, int n, int any, float* hits, int threads) {
+    trace_job J = { s, rays, n, any, 0, hits };
+    run_threads(trace_worker, &J, threads);
+    return 0;
+}
+int orc_trace_brute(const orc_scene* s, const float* rays, int n, float* hits, int threads) {
+    trace_job J = { s, rays, n, 0, 1, hits };
+    run_threads(trace_worker, &J, threads);
+    return 0;
+}
+int orc_trace_work(const orc_scene* s, const float* rays, int n, double* node_visits, double* tri_tests) {
+    *node_visits = 0; *tri_tests = 0;
+    for (int k = 0; k < n; k++) {
+        const float* R = rays + 8 * (size_t)k;
+        ray_t r = mkray(ld3(R), ld3(R + 4), R[3], s->tri_test);
+        hit_t h = { R[7], 0.f, 0.f, -1 };
+        trace_one(s, &r, &h, 0, node_visits, tri_tests);
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------- camera rays: src/render/wavefront.cu:43-57, camera.cpp */
+void orc_camera_ray(const orc_params* p, int pix, uint32_t smp, float o[3], float d[3]) {
+    const int x = pix % p->w, y = pix / p->w;
+    float u0 = orc_rng4((uint32_t)pix, smp, 0, 0, p->seed), u1 = orc_rng4((uint32_t)pix, smp, 0, 1, p->seed);
+    float jx, jy;
+    if (p->filter_bh) { jx = table_lookup(p->ftab, 64, u0); jy = table_lookup(p->ftab, 64, u1); }
+    else { jx = u0 - 0.5f; jy = u1 - 0.5f; }
+    float dx = ((float)x + 0.5f + jx - p->px) / p->fx, dy = -((float)y + 0.5f + jy - p->py) / p->fy;
+    f3 ax = ld3(p->ax), ay = ld3(p->ay), az = ld3(p->az), pos = ld3(p->pos);
+    f3 dir = sub(add(scl(ax, dx), scl(ay, dy)), az);
+    f3 rd = nrm3(dir), ro = pos;
+    if (p->lens_r > 0.f) {
+        float lx, ly;
+        concentric(orc_rng4((uint32_t)pix, smp, 0, 2, p->seed), orc_rng4((uint32_t)pix, smp, 0, 3, p->seed), &lx, &ly);
+        f3 pf = add(pos, scl(dir, p->focus_d));
+        ro = add(add(pos, scl(ax, lx * p->lens_r)), scl(ay, ly * p->lens_r));
+        rd = nrm3(sub(pf, ro));
+    }
+    o[0] = ro.x; o[1] = ro.y; o[2] = ro.z; d[0] = rd.x; d[1] = rd.y; d[2] = rd.z;
+}
+
+/* ------------------------------------------------- legacy path of one (pixel, sample): src/render/wavefront.cu:94-136 */
+static f3 trace_path(const orc_scene* s, const orc_params* p, int pix, uint32_t smp, int bounces, double* st) {
+    float o3[3], d3[3];
+    orc_camera_ray(p, pix, smp, o3, d3);
+    f3 o = ld3(o3), d = ld3(d3), T = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
+    const f3 sky = ld3(p->sky), sunE = ld3(p->sun_e), sunTo = ld3(p->sun_to);
+    const int depths = bounces + 2;
+    for (int depth = 0; depth < depths; depth++) {
+        ray_t r = mkray(o, d, depth == 0 ? p->near_t : 0.f, s->tri_test);
+        hit_t h = { 1e30f, 0.f, 0.f, -1 };
+        trace_one(s, &r, &h, 0, NULL, NULL);
+        st[0] += 1;
+        if (h.tri < 0) { st[2] += 1; L = add(L, mul(T, sky)); break; }
+        if (depth > p->max_bounce) break;
+        f3 ph = add(o, scl(d, h.t));
+        const float* P = s->corners + 12 * h.tri;
+        f3 e1 = sub(ld3(P + 4), ld3(P)), e2 = sub(ld3(P + 8), ld3(P));
+        f3 ng = nrm3(crs(e1, e2)), ns = ng;
+        if (p->has_normals && s->normals) {
+            const float* N = s->normals + 12 * h.tri;
+            f3 nn = add(add(scl(ld3(N), 1.f - h.u - h.v), scl(ld3(N + 4), h.u)), scl(ld3(N + 8), h.v));
+            float l = sqrtf(dot3(nn, nn));
+            if (l > 0.f) ns = scl(nn, 1.f / l);
+            if (dot3(ns, ng) < 0.f) ns = neg(ns);
+        }
+        if (dot3(ng, d) > 0.f) { ng = neg(ng); ns = neg(ns); }
+        int mat = f2i(P[7]);
+        f3 alb = ld3(s->albedo + 4 * mat);
+        f3 po = add(ph, scl(ng, p->eps));
+        uint32_t dd = (uint32_t)depth + 1u;
+        f3 l = cone(sunTo, p->sun_cos, orc_rng4((uint32_t)pix, smp, dd, 2, p->seed), orc_rng4((uint32_t)pix, smp, dd, 3, p->seed));
+        float cl = dot3(ns, l);
+        if (cl > 0.f && dot3(ng, l) > 0.f) {
+            f3 contrib = scl(mul(mul(T, alb), sunE), cl * 0.31830989f);
+            ray_t sr = mkray(po, l, 0.f, s->tri_test);
+            hit_t sh = { 1e30f, 0.f, 0.f, -1 };
+            st[1] += 1;
+            if (!trace_one(s, &sr, &sh, 1, NULL, NULL)) { st[3] += 1; L = add(L, contrib); }
+        }
+        f3 nd = cos_hemi(ns, orc_rng4((uint32_t)pix, smp, dd, 0, p->seed), orc_rng4((uint32_t)pix, smp, dd, 1, p->seed));
+        if (dot3(nd, ng) <= 0.f) break;
+        T = mul(T, alb);
+        o = po; d = nd;
+    }
+    return L;
+}
+
+typedef struct { const orc_scene* s; const orc_params* p; int spp, sbase, batch, bounces; float* accum; double st[64][4]; } render_job;
+static void render_worker(void* a, int tid, int nth) {
+    render_job* J = (render_job*)a;
+    const int N = J->p->w * J->p->h;
+    double* st = J->st[tid];
+    for (int pix = tid; pix < N; pix += nth) {
+        float* acc = J->accum + 4 * (size_t)pix;
+        for (int s0 = J->sbase; s0 < J->sbase + J->spp; s0 += J->batch) {
+            float r = 0.f, g = 0.f, b = 0.f;
+            for (int s = s0; s < s0 + J->batch; s++) {
+                f3 L = trace_path(J->s, J->p, pix, (uint32_t)s, J->bounces, st);
+                r += L.x; g += L.y; b += L.z;
+            }
+            acc[0] += r; acc[1] += g; acc[2] += b;
+        }
+    }
+}
+int orc_render(const orc_scene* s, const orc_params* p, int spp, int sample_base, int batch, int bounces, int threads,
+               int clear, float* accum, double* stats) {
+    if (batch < 1 || spp % batch || threads > 64) return 1;
+    if (threads < 1) threads = 1;
+    if (clear) memset(accum, 0, sizeof(float) * 4 * (size_t)p->w * p->h);
+    render_job* J = (render_job*)calloc(1, sizeof(render_job));
+    J->s = s; J->p = p; J->spp = spp; J->sbase = sample_base; J->batch = batch; J->bounces = bounces; J->accum = accum;
+    run_threads(render_worker, J, threads);
+    for (int t = 0; t < threads; t++) for (int k = 0; k < 4; k++) stats[k] += J->st[t][k];
+    free(J);
+    return 0;
+}

@@ -1,0 +1,100 @@
+/* oracle.h - CPU restatement of the reference's algorithms for the wavefront hot path (plain C11).
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker: the tests, the smoke check and the CPU-baseline leg
+ * of bench.py call it to compute expected results. The product (libcwp.so, the pt CLI) never links or calls it.
+ *
+ * Every function restates one reference function (paths relative to the reference repo); the pins that check each
+ * restatement independently of the product are listed in the header of oracle.c.
+ */
+#ifndef CWP_ORACLE_H
+#define CWP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* same layout as cwp_params in include/cwp.h */
+typedef struct orc_params {
+    float pos[3], ax[3], ay[3], az[3];
+    float fx, fy, px, py;
+    int w, h;
+    float lens_r, focus_d;
+    float sun_to[3], sun_e[3], sky[3];
+    float sun_cos;
+    int max_bounce;
+    uint32_t seed;
+    float eps, near_t;
+    int has_normals, filter_bh;
+    float ftab[65];
+} orc_params;
+
+typedef struct orc_scene {
+    int variant;            /* 2: BVH2 64 B nodes, 8: compressed BVH8 80 B nodes */
+    int tri_test;           /* 0 Moller-Trumbore, 1 Woop rows, 2 watertight */
+    const float* nodes;     /* float4 array */
+    int root;
+    const float* woop;      /* 3 float4 per leaf position */
+    const float* corners;   /* 3 float4 per leaf position: (v0, id), (v1, material), (v2, 0) */
+    const float* normals;   /* 3 float4 per leaf position or NULL */
+    const float* albedo;    /* float4 per material */
+    int num_tris;
+} orc_scene;
+
+/* binary SAH tree: node = int[4] (left, right, first, count) + float[6] box (lo, hi); prim = triangle order */
+typedef struct orc_tree {
+    int num_nodes, num_prims, max_depth;
+    int* node;
+    float* box;
+    int* prim;
+} orc_tree;
+
+typedef struct orc_layout {
+    int num_float4;         /* node array length in float4 */
+    float* nodes;
+    int num_prims;
+    int* order;             /* leaf position -> triangle id */
+    int depth;              /* wide-tree levels (CWBVH) or internal-node levels (BVH2) */
+    int max_leaf;
+    double sah;
+} orc_layout;
+
+/* ---- RNG and warps: src/core/rng.h:6-12, src/core/sampling.h:13-58 ---- */
+uint32_t orc_hash32(uint32_t x);
+float orc_rng4(uint32_t pix, uint32_t smp, uint32_t depth, uint32_t dim, uint32_t seed);
+void orc_blackman_harris_table(float* tab, int n, double width);
+void orc_cosine_hemisphere(const float n[3], float u0, float u1, float out[3]);
+void orc_uniform_cone(const float axis[3], float cos_max, float u0, float u1, float out[3]);
+
+/* ---- BVH construction: src/accel/bvh_builder.cpp:5-68, bvh_layouts.cpp:6-30, bvh_tools.cpp:75-168, tri_pack.cpp ---- */
+int orc_build_sah(const float* v, int nv, const int* tri, int nt, int bins, int leaf_max, orc_tree* out);
+void orc_tree_free(orc_tree* t);
+int orc_layout_bvh2(const orc_tree* t, orc_layout* out);
+int orc_collapse_cwbvh(const orc_tree* t, orc_layout* out);
+void orc_layout_free(orc_layout* l);
+void orc_pack_woop(const float* v, const int* tri, const int* order, int n, float* rows);
+void orc_pack_corners(const float* v, const int* tri, const int* mat, const int* order, int n, float* corners);
+
+/* ---- traversal: src/accel/traverse.cuh ---- */
+/* rays: 8 floats (o, tmin, d, tmax); hits: 4 floats (t, u, v, leaf position bits, -1 = miss); returns 0 on success */
+int orc_trace(const orc_scene* s, const float* rays, int n, int any_hit, float* hits, int threads);
+/* closest hit by testing every triangle (same triangle test), for pinning the traversal */
+int orc_trace_brute(const orc_scene* s, const float* rays, int n, float* hits, int threads);
+/* node visits and triangle tests of a closest-hit traversal, summed over the rays (cache-side traffic model) */
+int orc_trace_work(const orc_scene* s, const float* rays, int n, double* node_visits, double* tri_tests);
+
+/* ---- legacy integrator: src/render/wavefront.cu:42-58 (raygen), 94-136 (shadeOne), 67-72 (shadow) ---- */
+/* Renders spp samples (passes of `batch` samples; per-path radiance summed per pass in sample order) into accum
+ * (float4 per pixel, += unless clear). stats[0..3] += closest rays, shadow rays, misses, unoccluded shadow rays.
+ * Pixels rows are split over `threads` threads; each pixel is owned by one thread. */
+int orc_render(const orc_scene* s, const orc_params* p, int spp, int sample_base, int batch, int bounces, int threads,
+               int clear, float* accum, double* stats);
+/* the camera ray of (pixel, sample): origin and direction */
+void orc_camera_ray(const orc_params* p, int pix, uint32_t smp, float o[3], float d[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

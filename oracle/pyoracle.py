@@ -1,0 +1,220 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the CPU parity checker. Imported by the tests, the smoke check and bench.py's CPU-baseline
+leg; never by the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcTree(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_int), ("num_prims", ctypes.c_int), ("max_depth", ctypes.c_int),
+                ("node", _IP), ("box", _FP), ("prim", _IP)]
+
+
+class OrcLayout(ctypes.Structure):
+    _fields_ = [("num_float4", ctypes.c_int), ("nodes", _FP), ("num_prims", ctypes.c_int), ("order", _IP),
+                ("depth", ctypes.c_int), ("max_leaf", ctypes.c_int), ("sah", ctypes.c_double)]
+
+
+class OrcScene(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("nodes", _FP), ("root", ctypes.c_int),
+                ("woop", _FP), ("corners", _FP), ("normals", _FP), ("albedo", _FP), ("num_tris", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: python oracle/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_hash32.argtypes = [ctypes.c_uint32]; L.orc_hash32.restype = ctypes.c_uint32
+        L.orc_rng4.argtypes = [ctypes.c_uint32] * 5; L.orc_rng4.restype = ctypes.c_float
+        L.orc_blackman_harris_table.argtypes = [_FP, ctypes.c_int, ctypes.c_double]
+        L.orc_cosine_hemisphere.argtypes = [_FP, ctypes.c_float, ctypes.c_float, _FP]
+        L.orc_uniform_cone.argtypes = [_FP, ctypes.c_float, ctypes.c_float, ctypes.c_float, _FP]
+        L.orc_build_sah.argtypes = [_FP, ctypes.c_int, _IP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcTree)]
+        L.orc_build_sah.restype = ctypes.c_int
+        L.orc_tree_free.argtypes = [ctypes.POINTER(OrcTree)]
+        L.orc_layout_bvh2.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcLayout)]
+        L.orc_collapse_cwbvh.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcLayout)]
+        L.orc_layout_free.argtypes = [ctypes.POINTER(OrcLayout)]
+        L.orc_pack_woop.argtypes = [_FP, _IP, _IP, ctypes.c_int, _FP]
+        L.orc_pack_corners.argtypes = [_FP, _IP, _IP, _IP, ctypes.c_int, _FP]
+        L.orc_trace.argtypes = [ctypes.POINTER(OrcScene), _FP, ctypes.c_int, ctypes.c_int, _FP, ctypes.c_int]
+        L.orc_trace_brute.argtypes = [ctypes.POINTER(OrcScene), _FP, ctypes.c_int, _FP, ctypes.c_int]
+        L.orc_trace_work.argtypes = [ctypes.POINTER(OrcScene), _FP, ctypes.c_int, _DP, _DP]
+        L.orc_render.argtypes = [ctypes.POINTER(OrcScene), _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, _FP, _DP]
+        L.orc_render.restype = ctypes.c_int
+        L.orc_camera_ray.argtypes = [_P, ctypes.c_int, ctypes.c_uint32, _FP, _FP]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _i(a):
+    return a.ctypes.data_as(_IP)
+
+
+def hash32(x):
+    return lib().orc_hash32(x)
+
+
+def rng4(pix, smp, depth, dim, seed):
+    return lib().orc_rng4(pix, smp, depth, dim, seed)
+
+
+def blackman_harris_table(n=64, width=1.5):
+    t = np.zeros(n + 1, dtype=np.float32)
+    lib().orc_blackman_harris_table(_f(t), n, width)
+    return t
+
+
+def cosine_hemisphere(n, u0, u1):
+    a = np.asarray(n, dtype=np.float32); o = np.zeros(3, dtype=np.float32)
+    lib().orc_cosine_hemisphere(_f(a), u0, u1, _f(o))
+    return o
+
+
+def uniform_cone(axis, cos_max, u0, u1):
+    a = np.asarray(axis, dtype=np.float32); o = np.zeros(3, dtype=np.float32)
+    lib().orc_uniform_cone(_f(a), cos_max, u0, u1, _f(o))
+    return o
+
+
+def build_sah(vertices, triangles, bins=16, leaf_max=4):
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    tr = OrcTree()
+    if lib().orc_build_sah(_f(v), v.shape[0], _i(t), t.shape[0], bins, leaf_max, ctypes.byref(tr)) != 0:
+        raise ValueError("orc_build_sah: bad input")
+    return tr
+
+
+def tree_arrays(tr):
+    n = tr.num_nodes
+    node = np.ctypeslib.as_array(tr.node, shape=(n * 4,)).reshape(n, 4).copy()
+    box = np.ctypeslib.as_array(tr.box, shape=(n * 6,)).reshape(n, 6).copy()
+    prim = np.ctypeslib.as_array(tr.prim, shape=(tr.num_prims,)).copy()
+    return dict(tree=node, tree_box=box, prim=prim, depth=tr.max_depth)
+
+
+def _layout(tr, variant):
+    L = OrcLayout()
+    if variant == 2:
+        lib().orc_layout_bvh2(ctypes.byref(tr), ctypes.byref(L))
+    else:
+        lib().orc_collapse_cwbvh(ctypes.byref(tr), ctypes.byref(L))
+    nodes = np.ctypeslib.as_array(L.nodes, shape=(L.num_float4 * 4,)).copy()
+    order = np.ctypeslib.as_array(L.order, shape=(L.num_prims,)).copy()
+    out = dict(nodes=nodes.reshape(-1, 16 if variant == 2 else 20), order=order, layout_depth=L.depth,
+               max_leaf=L.max_leaf, sah=L.sah)
+    lib().orc_layout_free(ctypes.byref(L))
+    return out
+
+
+def build(vertices, triangles, variant=8, bins=16, leaf_max=4):
+    """Full oracle build: SAH tree (leaf_max 1 for the CWBVH collapse), layout, Woop rows, corners."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    tr = build_sah(v, t, bins, 1 if variant == 8 else leaf_max)
+    try:
+        out = tree_arrays(tr)
+        out.update(_layout(tr, variant))
+    finally:
+        lib().orc_tree_free(ctypes.byref(tr))
+    n = len(out["order"])
+    woop = np.zeros((n, 3, 4), dtype=np.float32)
+    corners = np.zeros((n, 3, 4), dtype=np.float32)
+    order = np.ascontiguousarray(out["order"], dtype=np.int32)
+    lib().orc_pack_woop(_f(v), _i(t), _i(order), n, _f(woop))
+    lib().orc_pack_corners(_f(v), _i(t), None, _i(order), n, _f(corners))
+    out.update(woop=woop, corners=corners, variant=variant)
+    return out
+
+
+class Scene:
+    """Arrays of a built scene in leaf order, kept alive for the C calls."""
+
+    def __init__(self, variant, tri_test, nodes, woop, corners, albedo, normals=None, root=0):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float32).reshape(-1)
+        self.woop = np.ascontiguousarray(woop, dtype=np.float32).reshape(-1)
+        self.corners = np.ascontiguousarray(corners, dtype=np.float32).reshape(-1)
+        self.albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1)
+        self.normals = None if normals is None or len(normals) == 0 else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1)
+        self.c = OrcScene(variant, tri_test, _f(self.nodes), root, _f(self.woop), _f(self.corners),
+                          _f(self.normals) if self.normals is not None else None, _f(self.albedo), len(self.corners) // 12)
+
+    def trace(self, rays, any_hit=False, threads=8):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        h = np.zeros((r.shape[0], 4), dtype=np.float32)
+        lib().orc_trace(ctypes.byref(self.c), _f(r), r.shape[0], 1 if any_hit else 0, _f(h), threads)
+        return h
+
+    def trace_brute(self, rays, threads=8):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        h = np.zeros((r.shape[0], 4), dtype=np.float32)
+        lib().orc_trace_brute(ctypes.byref(self.c), _f(r), r.shape[0], _f(h), threads)
+        return h
+
+    def trace_work(self, rays):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        nv, nt = ctypes.c_double(), ctypes.c_double()
+        lib().orc_trace_work(ctypes.byref(self.c), _f(r), r.shape[0], ctypes.byref(nv), ctypes.byref(nt))
+        return nv.value, nt.value
+
+    def render(self, params, spp, sample_base=0, batch=None, bounces=None, threads=8, accum=None):
+        """params: a ctypes structure with the cwp_params layout. Returns (accum (h, w, 4), stats[4])."""
+        w, h = params.w, params.h
+        clear = accum is None
+        if accum is None:
+            accum = np.zeros((h, w, 4), dtype=np.float32)
+        stats = np.zeros(4, dtype=np.float64)
+        batch = spp if batch is None else batch
+        bounces = params.max_bounce if bounces is None else bounces
+        rc = lib().orc_render(ctypes.byref(self.c), ctypes.cast(ctypes.byref(params), _P), spp, sample_base, batch, bounces,
+                              threads, 1 if clear else 0, _f(accum), stats.ctypes.data_as(_DP))
+        if rc:
+            raise ValueError("orc_render: bad arguments")
+        return accum, stats
+
+
+def camera_ray(params, pix, smp):
+    o = np.zeros(3, dtype=np.float32); d = np.zeros(3, dtype=np.float32)
+    lib().orc_camera_ray(ctypes.cast(ctypes.byref(params), _P), pix, smp, _f(o), _f(d))
+    return o, d
+
+
+def scene_from_setup(setup, tri_test=None, bins=16):
+    """Oracle scene over a product setup's mesh, rebuilt by the oracle's own builder (not the product's arrays)."""
+    import cwp
+    v = setup.array(cwp.ARR_VERTICES)
+    t = setup.array(cwp.ARR_TRIANGLES)
+    mats = setup.array(cwp.ARR_MATERIAL)
+    alb = setup.array(cwp.ARR_ALBEDO)
+    info = setup.info
+    b = build(v, t, info["variant"], bins, 4)
+    corners = np.zeros((len(b["order"]), 3, 4), dtype=np.float32)
+    order = np.ascontiguousarray(b["order"], dtype=np.int32)
+    lib().orc_pack_corners(_f(np.ascontiguousarray(v, dtype=np.float32)), _i(np.ascontiguousarray(t, dtype=np.int32)),
+                           _i(np.ascontiguousarray(mats, dtype=np.int32)), _i(order), len(order), _f(corners))
+    b["corners"] = corners
+    tt = info["tri_test"] if tri_test is None else tri_test
+    return Scene(info["variant"], tt, b["nodes"], b["woop"], corners, alb), b

@@ -1,0 +1,137 @@
+"""Host-side tests of libcwp.so that need no GPU: the C ABI exports, option parsing and error codes, the reference's
+file formats (its examples/ fixtures), and bit-exact agreement of the product's BVH build and triangle packing with the
+oracle's independent restatement."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cwp
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "cwp.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(cwp_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(cwp.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(cwp.SIGNATURES)   # the binding covers the whole header
+
+
+def test_version_and_error_string():
+    assert cwp.lib().cwp_version() == 1
+    with pytest.raises(cwp.CwpError, match="unknown arg"):
+        cwp.Setup(["--no_such_flag"])
+    with pytest.raises(cwp.CwpError, match="needs 1 value"):
+        cwp.Setup(["--gen", "boxes", "--spp"])
+    with pytest.raises(cwp.CwpError, match="--variant"):
+        cwp.Setup(["--gen", "boxes", "--variant", "bvh3"])
+    with pytest.raises(cwp.CwpError, match="failed to load"):
+        cwp.Setup(["--mesh", "/nonexistent.obj", "--hfov", "40"])
+    with pytest.raises(cwp.CwpError, match="no scene"):
+        cwp.Setup([])
+
+
+def test_reference_example_fixtures(golden_dir):
+    s = cwp.Setup(["--mesh", os.path.join(golden_dir, "boxes.obj"), "--camera", os.path.join(golden_dir, "camera.json"),
+                   "--lighting", os.path.join(golden_dir, "lighting.json"), "--width", "960", "--height", "540"])
+    i = s.info
+    assert i["num_tris"] == 62 and i["num_verts"] == 124
+    kd = s.array(cwp.ARR_ALBEDO)
+    assert len(kd) == 7   # default + 6 MTL materials
+    np.testing.assert_allclose(kd[2, :3], [0.8, 0.35, 0.22])   # terracotta
+    p = s.params()
+    assert p.fx == pytest.approx(1158.823) and p.fy == pytest.approx(1158.823)   # width/height match the file
+    assert p.pos[:] == pytest.approx([9.0, -10.5, 5.5])
+    sun = np.array([0.608749, -0.42625, 0.669131]); sun /= np.linalg.norm(sun)
+    assert np.array(p.sun_to[:]) == pytest.approx(sun, abs=1e-6)
+    assert p.sun_e[:] == pytest.approx([3.0] * 3) and p.sky[:] == pytest.approx([0.3, 0.38, 0.5])
+    assert p.sun_cos == pytest.approx(np.cos(0.5 * 0.00918), abs=1e-7)
+    mats = s.array(cwp.ARR_MATERIAL)
+    assert set(mats.tolist()) <= set(range(1, 7))
+    # other resolution: hfov from the file is used instead of fx
+    s2 = cwp.Setup(["--mesh", os.path.join(golden_dir, "boxes.obj"), "--camera", os.path.join(golden_dir, "camera.json"),
+                    "--width", "320", "--height", "180"])
+    assert s2.params().fx == pytest.approx(160 / np.tan(np.radians(22.5)), rel=1e-6)
+
+
+def test_ply_loader_roundtrip(tmp_path):
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    hdr = "ply\nformat binary_little_endian 1.0\nelement vertex 4\nproperty float x\nproperty float y\nproperty float z\n" \
+          "element face 1\nproperty list uchar int vertex_indices\nproperty int material_index\nend_header\n"
+    body = v.tobytes() + bytes([4]) + np.array([0, 1, 2, 3], np.int32).tobytes() + np.array([2], np.int32).tobytes()
+    path = tmp_path / "quad.ply"
+    path.write_bytes(hdr.encode() + body)
+    s = cwp.Setup(["--mesh", str(path), "--c2w", "1,0,0,0.5,0,1,0,0.5,0,0,1,2,0,0,0,1"])
+    assert s.info["num_tris"] == 2
+    np.testing.assert_array_equal(s.array(cwp.ARR_TRIANGLES), [[0, 1, 2], [0, 2, 3]])
+    np.testing.assert_array_equal(s.array(cwp.ARR_MATERIAL), [2, 2])
+    assert s.info["num_materials"] == 3
+
+
+@pytest.mark.parametrize("gen,variant,bins", [("boxes", 8, 16), ("boxes", 2, 32), ("boxes", 8, 32), ("terrain", 8, 16)])
+def test_product_bvh_matches_oracle_bit_exact(gen, variant, bins):
+    args = ["--gen", gen, "--variant", "bvh%d" % variant, "--bins", str(bins), "--width", "8", "--height", "8"]
+    if gen == "terrain":
+        args += ["--gen_grid", "121", "--gen_boxes", "50"]
+    s = cwp.Setup(args)
+    v, t = s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES)
+    b = po.build(v, t, variant, bins, 4)
+    np.testing.assert_array_equal(s.array(cwp.ARR_NODES).view(np.uint32), b["nodes"].view(np.uint32))
+    np.testing.assert_array_equal(s.array(cwp.ARR_ORDER), b["order"])
+    np.testing.assert_array_equal(s.array(cwp.ARR_WOOP).view(np.uint32), b["woop"].view(np.uint32))
+    np.testing.assert_array_equal(s.array(cwp.ARR_TREE), b["tree"])
+    np.testing.assert_array_equal(s.array(cwp.ARR_TREE_BOX).view(np.uint32), b["tree_box"].view(np.uint32))
+    assert s.info["bvh_depth"] == b["depth"]
+
+
+def test_bvh_build_api_matches_oracle():
+    rng = np.random.default_rng(9)
+    c = rng.random((700, 3)).astype(np.float32)
+    v = (c[:, None, :] + rng.normal(scale=0.03, size=(700, 3, 3))).reshape(-1, 3).astype(np.float32)
+    t = np.arange(2100, dtype=np.int32).reshape(-1, 3)
+    for variant in (2, 8):
+        p = cwp.bvh_build(v, t, variant, 16, 4)
+        o = po.build(v, t, variant, 16, 4)
+        np.testing.assert_array_equal(p["nodes"].view(np.uint32), o["nodes"].view(np.uint32))
+        np.testing.assert_array_equal(p["order"], o["order"])
+        np.testing.assert_array_equal(p["corners"][:, :, :3], o["corners"][:, :, :3])
+    with pytest.raises(cwp.CwpError, match="out of range"):
+        cwp.bvh_build(v, t + 5000, 8)
+
+
+def test_params_filter_table_matches_oracle():
+    s = cwp.Setup(["--gen", "boxes", "--width", "8", "--height", "8"])
+    np.testing.assert_array_equal(np.array(s.params().ftab[:], np.float32), po.blackman_harris_table(64, 1.5))
+
+
+def test_scene_generators_are_deterministic_and_sized():
+    a = cwp.Setup(["--gen", "terrain", "--width", "8", "--height", "8"])
+    assert a.info["num_tris"] == 2 * 241 * 241 + 200 * 12 == 118562
+    b = cwp.Setup(["--gen", "terrain", "--width", "8", "--height", "8"])
+    np.testing.assert_array_equal(a.array(cwp.ARR_VERTICES), b.array(cwp.ARR_VERTICES))
+    c = cwp.Setup(["--gen", "terrain", "--gen_seed", "2", "--width", "8", "--height", "8"])
+    assert not np.array_equal(a.array(cwp.ARR_VERTICES), c.array(cwp.ARR_VERTICES))
+    d = cwp.Setup(["--gen", "boxes", "--width", "8", "--height", "8"])
+    assert d.info["num_tris"] == 1024 * 12 + 2
+
+
+def test_cli_reports_errors_without_gpu(tmp_path):
+    import subprocess
+    exe = os.path.join(cwp.PKG, "pt")
+    r = subprocess.run([exe, "--bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown arg" in r.stderr
+    r = subprocess.run([exe, "--gen", "boxes", "--spp", "10", "--batch", "4", "--width", "8", "--height", "8"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "multiple of batch" in r.stderr

@@ -1,0 +1,319 @@
+"""Pins of the oracle (the CPU restatement used as the parity checker) that do not rely on the product.
+
+Each test checks an oracle function against something its formula cannot agree with by construction: a closed form,
+an invariant, brute force, or a statistical law. What each function is pinned by:
+
+  rng4 / hash32          bit balance, uniformity and independence statistics (parity pinned by statistics only: the
+                         reference ships no RNG vectors and cannot be built here)
+  Blackman-Harris table  closed-form CDF of the window (integral of the cosine series)
+  warps                  unit length, hemisphere / cone support, exact moments (E[cos] = 2/3; cone cos uniform)
+  SAH builder            primitives exactly once, boxes = union of contents, leaf sizes, a separable two-cluster split
+  CWBVH / BVH2 layouts   dequantised child boxes contain the exact boxes, leaves <= 3 triangles, traversal = brute force
+  traversal              closest hit = brute force over all triangles (3 triangle tests x 2 layouts), any hit =
+                         (closest hit exists); watertight test has no gaps on shared vertices of a grid
+  legacy shading         closed forms: Lambert plane under a constant sky gives albedo * sky exactly; with a sun,
+                         albedo * (sky + E cos / pi) within the sun-cone spread; sky pixels = sky
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+
+# ------------------------------------------------------------------------------------------------------------ RNG
+def test_rng_uniform_and_independent():
+    n = 20000
+    u = np.array([po.rng4(i, 7, 3, 2, 1000) for i in range(n)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01
+    assert abs(u.var() - 1.0 / 12.0) < 0.005
+    # 24-bit resolution: every value is a multiple of 2^-24
+    assert np.all((u * 16777216.0) == np.floor(u * 16777216.0))
+    # neighbouring dimensions and samples are uncorrelated
+    v = np.array([po.rng4(i, 7, 3, 3, 1000) for i in range(n)])
+    w = np.array([po.rng4(i, 8, 3, 2, 1000) for i in range(n)])
+    assert abs(np.corrcoef(u, v)[0, 1]) < 0.03
+    assert abs(np.corrcoef(u, w)[0, 1]) < 0.03
+
+
+def test_hash_bit_balance():
+    hs = np.array([po.hash32(i) for i in range(4096)], dtype=np.uint64)
+    bits = np.array([(hs >> b) & 1 for b in range(32)], dtype=np.float64)
+    assert np.all(np.abs(bits.mean(axis=1) - 0.5) < 0.05)
+    # avalanche: flipping one input bit flips about half the output bits
+    flips = [bin(po.hash32(i) ^ po.hash32(i ^ 1)).count("1") for i in range(2000)]
+    assert abs(np.mean(flips) - 16) < 1.0
+
+
+# ------------------------------------------------------------------------------------------- Blackman-Harris table
+def _bh_cdf(x):
+    a0, a1, a2, a3 = 0.35875, 0.48829, 0.14128, 0.01168
+    two = 2 * math.pi
+    val = a0 * x - a1 * math.sin(two * x) / two + a2 * math.sin(2 * two * x) / (2 * two) - a3 * math.sin(3 * two * x) / (3 * two)
+    return val / a0
+
+
+def test_blackman_harris_table_closed_form():
+    width = 1.5
+    tab = po.blackman_harris_table(64, width)
+    assert tab[0] == pytest.approx(-width / 2, abs=1e-6) and tab[-1] == pytest.approx(width / 2, abs=1e-6)
+    assert np.all(np.diff(tab) > 0)
+    np.testing.assert_allclose(tab + tab[::-1], 0.0, atol=2e-6)   # symmetric window -> antisymmetric inverse CDF
+    for k in range(65):
+        x = tab[k] / width + 0.5
+        assert _bh_cdf(x) == pytest.approx(k / 64, abs=2e-4)
+
+
+# ---------------------------------------------------------------------------------------------------------- warps
+def test_cosine_hemisphere_moments():
+    rng = np.random.default_rng(3)
+    n = np.array([0.3, -0.5, 0.8], dtype=np.float32); n /= np.linalg.norm(n)
+    d = np.array([po.cosine_hemisphere(n, a, b) for a, b in rng.random((20000, 2), dtype=np.float32)])
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    c = d @ n
+    assert c.min() >= -1e-6
+    assert c.mean() == pytest.approx(2.0 / 3.0, abs=0.01)        # E[cos] under pdf cos/pi
+    assert (c ** 2).mean() == pytest.approx(0.5, abs=0.01)       # E[cos^2] = 1/2
+
+
+def test_uniform_cone_distribution():
+    rng = np.random.default_rng(4)
+    axis = np.array([0.6, -0.4, 0.69], dtype=np.float32); axis /= np.linalg.norm(axis)
+    cmax = 0.9
+    d = np.array([po.uniform_cone(axis, cmax, a, b) for a, b in rng.random((20000, 2), dtype=np.float32)])
+    c = d @ axis
+    assert c.min() >= cmax - 1e-5 and c.max() <= 1 + 1e-6
+    assert c.mean() == pytest.approx((1 + cmax) / 2, abs=0.002)  # cos uniform on [cmax, 1]
+    assert c.var() == pytest.approx((1 - cmax) ** 2 / 12, rel=0.05)
+
+
+# ---------------------------------------------------------------------------------------------------- BVH builder
+def random_soup(n, seed, spread=1.0, size=0.05):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3)).astype(np.float32) * spread
+    v = (c[:, None, :] + rng.normal(scale=size, size=(n, 3, 3)).astype(np.float32)).reshape(-1, 3)
+    return v.astype(np.float32), np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+
+
+def grid_mesh(g, seed=0, amp=0.05):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.linspace(0, 1, g), np.linspace(0, 1, g))
+    z = amp * rng.random((g, g))
+    v = np.stack([xs, ys, z], -1).reshape(-1, 3).astype(np.float32)
+    t = []
+    for j in range(g - 1):
+        for i in range(g - 1):
+            a = j * g + i
+            t += [(a, a + 1, a + g + 1), (a, a + g + 1, a + g)]
+    return v, np.array(t, dtype=np.int32)
+
+
+def tri_boxes(v, t):
+    p = v[t]
+    return p.min(axis=1), p.max(axis=1)
+
+
+@pytest.mark.parametrize("bins,leaf", [(16, 4), (32, 4), (16, 1)])
+def test_sah_tree_invariants(bins, leaf):
+    v, t = random_soup(3000, 1)
+    a = po.tree_arrays(po.build_sah(v, t, bins, leaf))
+    node, box, prim = a["tree"], a["tree_box"], a["prim"]
+    assert sorted(prim.tolist()) == list(range(len(t)))
+    lo, hi = tri_boxes(v, t)
+    seen = np.zeros(len(t), dtype=int)
+    for i, (l, r, first, cnt) in enumerate(node):
+        if cnt > 0:
+            assert cnt <= leaf
+            ids = prim[first:first + cnt]
+            seen[ids] += 1
+            np.testing.assert_array_equal(box[i, :3], lo[ids].min(0))
+            np.testing.assert_array_equal(box[i, 3:], hi[ids].max(0))
+        else:
+            assert l > i and r > i                                   # pre-order ids
+            np.testing.assert_array_equal(box[i, :3], np.minimum(box[l, :3], box[r, :3]))
+            np.testing.assert_array_equal(box[i, 3:], np.maximum(box[l, 3:], box[r, 3:]))
+    assert np.all(seen == 1)
+
+
+def test_sah_separates_two_clusters():
+    v1, t1 = random_soup(200, 2, spread=0.2)
+    v2, t2 = random_soup(200, 3, spread=0.2)
+    v2 = v2 + np.float32(5.0)
+    v = np.concatenate([v1, v2]); t = np.concatenate([t1, t2 + len(v1)])
+    a = po.tree_arrays(po.build_sah(v, t, 16, 4))
+    l, r, _, _ = a["tree"][0]
+
+    def leaves(n):
+        nd = a["tree"][n]
+        if nd[3] > 0:
+            return set(a["prim"][nd[2]:nd[2] + nd[3]].tolist())
+        return leaves(nd[0]) | leaves(nd[1])
+    assert leaves(l) == set(range(200)) and leaves(r) == set(range(200, 400))
+
+
+def _decode_cwbvh_child_boxes(nodes):
+    """Yields (node, slot, decoded lo, decoded hi, meta) for every occupied slot of a CWBVH node array (n, 20)."""
+    u = nodes.view(np.uint32)
+    for i in range(len(nodes)):
+        ew = int(u[i, 3])
+        e = [(ew >> s) & 255 for s in (0, 8, 16)]
+        scale = [2.0 ** (x - 127) for x in e]
+        org = nodes[i, :3].astype(np.float64)
+        meta = int(u[i, 6]) | (int(u[i, 7]) << 32)
+        q = [int(u[i, 8 + 2 * k]) | (int(u[i, 9 + 2 * k]) << 32) for k in range(6)]
+        for s in range(8):
+            m = (meta >> (8 * s)) & 255
+            if m == 0:
+                continue
+            ql = [(q[k] >> (8 * s)) & 255 for k in range(3)]
+            qh = [(q[3 + k] >> (8 * s)) & 255 for k in range(3)]
+            lo = [org[k] + ql[k] * scale[k] for k in range(3)]
+            hi = [org[k] + qh[k] * scale[k] for k in range(3)]
+            yield i, s, np.array(lo), np.array(hi), m
+
+
+def cwbvh_subtree_tris(nodes, order, i):
+    """Triangle ids (original) below CWBVH node i."""
+    u = nodes.view(np.uint32)
+    out = []
+    imask = int(u[i, 3]) >> 24
+    for (n, s, dl, dh, m) in _decode_cwbvh_child_boxes(nodes[i:i + 1]):
+        if m & 0x80:
+            first = int(u[i, 5]) + (m & 31)
+            out += order[first:first + ((m >> 5) & 3) + 1].tolist()
+        else:
+            child = int(u[i, 4]) + bin(imask & ((1 << s) - 1)).count("1")
+            out += cwbvh_subtree_tris(nodes, order, child)
+    return out
+
+
+def test_cwbvh_conservative_and_complete():
+    v, t = random_soup(2000, 5)
+    b = po.build(v, t, 8)
+    nodes, order = b["nodes"], b["order"]
+    assert sorted(order.tolist()) == list(range(len(t)))
+    assert sorted(cwbvh_subtree_tris(nodes, order, 0)) == list(range(len(t)))   # every triangle reachable once
+    assert b["max_leaf"] <= 3
+    lo, hi = tri_boxes(v, t)
+    u = nodes.view(np.uint32)
+    # every slot's dequantised box contains the exact box of everything below it
+    for (n, s, dl, dh, m) in _decode_cwbvh_child_boxes(nodes):
+        if m & 0x80:
+            first = int(u[n, 5]) + (m & 31)
+            ids = order[first:first + ((m >> 5) & 3) + 1]
+        else:
+            imask = int(u[n, 3]) >> 24
+            ids = cwbvh_subtree_tris(nodes, order, int(u[n, 4]) + bin(imask & ((1 << s) - 1)).count("1"))
+        assert np.all(dl <= lo[ids].min(0)) and np.all(dh >= hi[ids].max(0))
+
+
+def _rays(n, seed, lo=-0.5, hi=1.5):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, np.zeros((n, 1), np.float32), d, np.full((n, 1), 1e30, np.float32)], 1)
+
+
+@pytest.mark.parametrize("variant", [2, 8])
+@pytest.mark.parametrize("tri_test", [0, 1, 2])
+def test_traversal_equals_brute_force(variant, tri_test):
+    v, t = random_soup(1500, 7)
+    b = po.build(v, t, variant)
+    sc = po.Scene(variant, tri_test, b["nodes"], b["woop"], b["corners"], np.ones((1, 4), np.float32))
+    rays = _rays(6000, 11)
+    h1, h2 = sc.trace(rays), sc.trace_brute(rays)
+    ids1, ids2 = h1[:, 3].view(np.int32), h2[:, 3].view(np.int32)
+    assert (ids1 >= 0).sum() > 1000
+    np.testing.assert_array_equal(ids1, ids2)
+    np.testing.assert_array_equal(h1[:, :3], h2[:, :3])
+    hany = sc.trace(rays, any_hit=True)
+    np.testing.assert_array_equal(hany[:, 3].view(np.int32) >= 0, ids1 >= 0)
+
+
+def test_watertight_triangle_test_has_no_gaps_at_shared_vertices():
+    v, t = grid_mesh(17, seed=1, amp=0.0)   # flat grid: rays through vertices and edge midpoints must hit
+    b = po.build(v, t, 8)
+    rng = np.random.default_rng(5)
+    targets = np.concatenate([v, 0.5 * (v[t[:, 0]] + v[t[:, 1]])])
+    inner = np.all((targets[:, :2] > 0.01) & (targets[:, :2] < 0.99), axis=1)
+    targets = targets[inner]
+    o = targets + np.array([0.0, 0.0, 1.0], np.float32) + rng.uniform(-0.3, 0.3, targets.shape).astype(np.float32) * [1, 1, 0]
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.zeros((len(o), 1), np.float32), d, np.full((len(o), 1), 1e30, np.float32)], 1).astype(np.float32)
+    # the triangle test alone (every triangle tested): watertight leaves no gap at shared vertices and edges
+    sc = po.Scene(8, 2, b["nodes"], b["woop"], b["corners"], np.ones((1, 4), np.float32))
+    h = sc.trace_brute(rays)
+    assert np.all(h[:, 3].view(np.int32) >= 0)
+
+
+# ------------------------------------------------------------------------------------------ legacy shading closed forms
+class P(po.ctypes.Structure):
+    _fields_ = [("pos", po.ctypes.c_float * 3), ("ax", po.ctypes.c_float * 3), ("ay", po.ctypes.c_float * 3),
+                ("az", po.ctypes.c_float * 3), ("fx", po.ctypes.c_float), ("fy", po.ctypes.c_float),
+                ("px", po.ctypes.c_float), ("py", po.ctypes.c_float), ("w", po.ctypes.c_int), ("h", po.ctypes.c_int),
+                ("lens_r", po.ctypes.c_float), ("focus_d", po.ctypes.c_float), ("sun_to", po.ctypes.c_float * 3),
+                ("sun_e", po.ctypes.c_float * 3), ("sky", po.ctypes.c_float * 3), ("sun_cos", po.ctypes.c_float),
+                ("max_bounce", po.ctypes.c_int), ("seed", po.ctypes.c_uint32), ("eps", po.ctypes.c_float),
+                ("near_t", po.ctypes.c_float), ("has_normals", po.ctypes.c_int), ("filter_bh", po.ctypes.c_int),
+                ("ftab", po.ctypes.c_float * 65)]
+
+
+def plane_params(w=32, h=32, sun=0.0, sky=(0.4, 0.5, 0.6)):
+    p = P()
+    # camera at height 1 looking down 45 degrees: the upper image rows see the sky
+    pos = np.array([0, -2, 1.0]); tgt = np.array([0, 0, 0.0])
+    f = tgt - pos; f /= np.linalg.norm(f); r = np.cross(f, [0, 0, 1.0]); r /= np.linalg.norm(r); u = np.cross(r, f)
+    p.pos[:] = pos; p.ax[:] = r; p.ay[:] = u; p.az[:] = -f
+    p.fx = p.fy = w * 0.5 / math.tan(math.radians(35)); p.px, p.py = w * 0.5, h * 0.5; p.w, p.h = w, h
+    s = np.array([0.3, -0.4, 0.866]); s /= np.linalg.norm(s)
+    p.sun_to[:] = s; p.sun_e[:] = (sun, sun, sun); p.sky[:] = sky; p.sun_cos = math.cos(math.radians(0.25))
+    # ray offset 1e-4 x scene extent (the reference's rule; the plane spans 2000 units)
+    p.max_bounce = 4; p.seed = 1000; p.eps = 1e-4 * 2000; p.near_t = 0.0; p.has_normals = 0; p.filter_bh = 0
+    return p
+
+
+def plane_scene(albedo=0.6, tri_test=1):
+    v = np.array([[-1e3, -1e3, 0], [1e3, -1e3, 0], [1e3, 1e3, 0], [-1e3, 1e3, 0]], np.float32)
+    t = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    b = po.build(v, t, 8)
+    return po.Scene(8, tri_test, b["nodes"], b["woop"], b["corners"], np.array([[albedo] * 3 + [0]], np.float32))
+
+
+@pytest.mark.parametrize("tri_test", [0, 1, 2])
+def test_plane_under_sky_is_albedo_times_sky(tri_test):
+    a = 0.6
+    sc = plane_scene(a, tri_test)
+    p = plane_params()
+    acc, st = sc.render(p, 8, batch=8)
+    img = acc[..., :3] / 8
+    # every sample that sees the plane contributes exactly a * sky (its continuation always escapes), every other
+    # sample exactly sky: a pixel's mean is sky * (1 - f (1 - a)) with f a multiple of 1/8 (horizon pixels mix)
+    sky = np.array(p.sky[:], np.float32)
+    f = (1.0 - img / sky) / (1.0 - a)
+    np.testing.assert_allclose(f * 8, np.round(f * 8), atol=1e-4)
+    np.testing.assert_allclose(f[..., 0], f[..., 1], atol=1e-5)
+    plane = np.isclose(f[..., 0], 1.0, atol=1e-5)
+    assert plane.sum() > 0.3 * plane.size and np.isclose(f[..., 0], 0.0, atol=1e-5).sum() > 0.05 * plane.size
+    hits = np.round(f[..., 0] * 8).sum()
+    assert st[0] == 8 * p.w * p.h + hits    # camera rays + one escaping continuation per plane sample
+
+
+def test_plane_with_sun_closed_form():
+    a, E = 0.5, 3.0
+    sc = plane_scene(a)
+    p = plane_params(sun=E, sky=(0.0, 0.0, 0.0))
+    acc, st = sc.render(p, 16, batch=16)
+    img = acc[..., 0] / 16
+    cos_sun = float(np.array(p.sun_to[:])[2])
+    expect = a * E * cos_sun / math.pi
+    # each sample on the plane gives a E cos / pi (the sun cone spreads cos by < 0.5%), each sky sample 0; horizon
+    # pixels mix, so a pixel is k/16 of the closed form for an integer k
+    f = img / expect
+    np.testing.assert_allclose(f * 16, np.round(f * 16), atol=16 * 5e-3)
+    full = np.round(f * 16) == 16
+    assert full.sum() > 0.3 * img.size
+    np.testing.assert_allclose(img[full], expect, rtol=5e-3)
