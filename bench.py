@@ -24,9 +24,10 @@ LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3
             "--sky", "0.3", "0.38", "0.5"]
 
 
-def workload_args(width, height, batch, bounces, tri, variant, stack):
+def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=6):
     return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
-            "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack)] + LIGHTING
+            "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack),
+            "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves)] + LIGHTING
 
 
 def algorithmic_bytes(st, npaths):
@@ -58,8 +59,24 @@ def cpu_baseline(args):
                       "%.0f rays in %.1f s" % (args.width, args.height, spp, args.bounces, threads, rays, dt)}
 
 
+def calibrate_copy(mib):
+    """A plain device copy of known size, to scale rocprof's FETCH_SIZE / WRITE_SIZE (run in its own process: it uses
+    torch, whose bundled HIP runtime must not share a process with libcwp.so)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
+                      "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (rocprof traffic calibration) and exit")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -69,10 +86,15 @@ def main():
     ap.add_argument("--bounces", type=int, default=8)
     ap.add_argument("--tri", default="woop")
     ap.add_argument("--variant", default="bvh8")
-    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--stack", type=int, default=8)
+    ap.add_argument("--bounce-waves", type=int, default=6)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy_mib > 0:
+        calibrate_copy(args.calibrate_copy_mib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the renderer runs on a single MI355X (multi-GPU runs are not part of "
               "this build)", file=sys.stderr)
@@ -80,7 +102,8 @@ def main():
 
     import cwp
     print("bench: setting up scene", file=sys.stderr, flush=True)
-    s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack))
+    s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
+                                args.blocks_per_cu, args.bounce_waves))
     r = cwp.Renderer(s)
     info = s.info
     npaths = args.width * args.height * args.batch
@@ -123,7 +146,7 @@ def main():
                    "scene": "242x242 fBm heightfield + 200 boxes (%d tris), albedo 0.8, sun 3 W/m2 + sky" % info["num_tris"],
                    "width": args.width, "height": args.height, "spp": args.batch, "batch": args.batch,
                    "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
-                   "stack_lds": args.stack, "bvh_nodes": info["num_nodes"]},
+                   "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"]},
         "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
         "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
         "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shadow_ms": round(last["t_shadow_ms"], 3),

@@ -20,6 +20,7 @@ struct RenderOptions {
     int bounces = 8;        // maximum bounce depth (camera hit = depth 0)
     int stackLds = 16;      // LDS traversal-stack entries per lane (8, 16 or 32); deeper entries spill to HBM
     int blocksPerCU = 0;    // persistent grid: workgroups per CU (0 = occupancy limit)
+    int bounceWaves = 6;    // trace+shade kernel: register budget for at least this many waves per SIMD (1, 6 or 8)
     int timeKernels = 1;    // per-launch hipEvent timing of the trace/shadow kernels
 };
 

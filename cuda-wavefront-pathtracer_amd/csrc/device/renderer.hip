@@ -32,10 +32,14 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 64;       // one wave per workgroup: LDS stacks are sized per wave
-constexpr int kCntStride = 8;    // per-depth counters: [0] queue, [1] shadow rays, [2] misses, [3] unoccluded shadow rays
+constexpr int kParts = 8;        // queue partitions, one per XCD (workgroup w runs on XCD w % 8)
+// Per-depth counters (kCntStride ints): [x] path-queue length of partition x, [kParts + x] shadow rays produced by
+// partition x, [2 kParts] misses, [2 kParts + 1] unoccluded shadow rays.
+constexpr int kCntStride = 32;
+constexpr int kCntMiss = 2 * kParts, kCntUnocc = 2 * kParts + 1;
 
 struct DevScene {
-    const Vec4* nodes;    // BVH layout
+    const Vec4* nodes;    // BVH layout (BVH8: 128-byte node stride)
     const Vec4* tri;      // traversal triangle data (Woop rows or corners)
     const Vec4* verts;    // corners for shading
     const Vec4* nrm;      // per-corner normals or null
@@ -43,6 +47,8 @@ struct DevScene {
     int root;
 };
 
+// Path state. Path id i = pixel * K + sample: the 64 lanes of a wave start as 64 samples of one pixel, and each
+// partition's slice of the id space is a band of the image.
 struct DevPaths {
     Vec4 *O, *D, *T, *L;  // origin, direction, throughput, radiance per path id
     int* q[2];            // ping-pong queues of path ids: depth d reads q[d & 1]
@@ -79,6 +85,13 @@ __device__ __forceinline__ void waveCount(uint64_t mask, int* counter) {
     if (mask && laneId() == __builtin_ctzll(mask)) atomicAdd(counter, (int)__builtin_popcountll(mask));
 }
 
+// start of partition `part` in a region laid out as partitions of the given lengths
+__device__ __forceinline__ int partStart(const int* len, int part) {
+    int s = 0;
+    for (int y = 0; y < part; y++) s += len[y];
+    return s;
+}
+
 // camera ray of path (pix, smp): filter importance sampling (Blackman-Harris table or 1 px box), pinhole or thin lens
 // (reference: src/render/wavefront.cu:43-57)
 __device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_t smp, Vec3& o, Vec3& d) {
@@ -102,37 +115,48 @@ __device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_
 }
 
 // ------------------------------------------------------------------------------------------------------- kernels
-// One bounce of the legacy integrator over the queue of depth `depth` (depth 0: all NK paths, camera rays made here).
-template <int V, int TRI, int SL>
-__global__ void __launch_bounds__(kBlock) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int N, int NK) {
+// One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here).
+// The queue is split into kParts partitions, one per XCD; the waves of XCD x walk partition x, and append their
+// continuations to partition x of the next queue and their shadow rays to partition x of the shadow queue, so a
+// path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue starts where the
+// depth-d partitions before x end, which bounds it by the rays partition x traced.
+// WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
+template <int V, int TRI, int SL, int WAVES>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int lane = laneId();
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ lds + threadIdx.x, kBlock, (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int n = depth == 0 ? NK : cnt[0];
+    const int n = cnt[part];
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
     const int* qIn = P.q[depth & 1];
     int* qOut = P.q[(depth + 1) & 1];
-    const int stride = gridDim.x * kWave;
-    for (int base = blockIdx.x * kWave; base < n; base += stride) {
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + lane;
         const bool act = k < n;
         bool cont = false, shadow = false, miss = false;
         int i = 0;
         Vec3 po = v3(0.f, 0.f, 0.f), nd = po, Tn = po, sdir = po, scon = po;
         if (act) {
-            i = depth == 0 ? k : qIn[k];
-            const int pix = i % N;
-            const uint32_t smp = (uint32_t)(smpBase + i / N);
-            Vec3 o, d, T;
+            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            Vec3 o, d;
             float tmin;
-            if (depth == 0) { cameraRay(F, pix, smp, o, d); T = v3(1.f, 1.f, 1.f); tmin = F.nearT; }
-            else { o = xyz(P.O[i]); d = xyz(P.D[i]); T = xyz(P.T[i]); tmin = 0.f; }
+            if (depth == 0) { cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
+            else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
             const RayPre r = prepareRay<TRI>(o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
             st.sp = 0;
             traceScene<V, false, TRI>(S, r, h, st);
+            // throughput, pixel and sample are fetched / recomputed after the traversal so that no registers hold
+            // them across it
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            const int pix = i / K;
+            const uint32_t smp = (uint32_t)(smpBase + i % K);
+            o = r.o; d = r.d;
             if (h.tri < 0) {
                 miss = true;
                 Vec3 c = T * F.sky;
@@ -166,49 +190,51 @@ __global__ void __launch_bounds__(kBlock) k_bounce(DevScene S, DevPaths P, Frame
         }
         const uint64_t mc = __ballot(cont), ms = __ballot(shadow), mm = __ballot(miss);
         if (mc) {
-            const int pos = waveAppend(mc, cnt + kCntStride);
+            const int pos = outStart + waveAppend(mc, cnt + kCntStride + part);
             if (cont) { qOut[pos] = i; P.O[i] = v4(po, 0.f); P.D[i] = v4(nd, 0.f); P.T[i] = v4(Tn, 0.f); }
         }
         if (ms) {
-            const int pos = waveAppend(ms, cnt + 1);
+            const int pos = outStart + waveAppend(ms, cnt + kParts + part);
             if (shadow) { P.sO[pos] = v4(po, asFloat(i)); P.sD[pos] = v4(sdir, 1e30f); P.sC[pos] = v4(scon, 0.f); }
         }
-        waveCount(mm, cnt + 2);
+        waveCount(mm, cnt + kCntMiss);
     }
 }
 
-// Shadow rays of bounce `depth`: any hit; unoccluded contributions are added to the path's radiance.
+// Shadow rays of bounce `depth` (partition per XCD as in k_bounce): any hit; unoccluded contributions are added to
+// the path's radiance. The any-hit traversal fits 64 registers without spilling: 8 waves per SIMD.
 template <int V, int TRI, int SL>
-__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, DevPaths P, int depth) {
+__global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, int depth) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ lds + threadIdx.x, kBlock, (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int n = cnt[1];
-    const int stride = gridDim.x * kWave;
-    for (int base = blockIdx.x * kWave; base < n; base += stride) {
+    const int n = cnt[kParts + part];
+    const int start = partStart(cnt, part);
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + laneId();
         bool vis = false;
         if (k < n) {
-            const Vec4 O = P.sO[k], D = P.sD[k];
+            const Vec4 O = P.sO[start + k], D = P.sD[start + k];
             const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
             st.sp = 0;
             if (!traceScene<V, true, TRI>(S, r, h, st)) {
                 vis = true;
                 const int i = asInt(O.w);
-                const Vec4 c = P.sC[k];
+                const Vec4 c = P.sC[start + k];
                 Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
             }
         }
-        waveCount(__ballot(vis), cnt + 3);
+        waveCount(__ballot(vis), cnt + kCntUnocc);
     }
 }
 
 // Closest or any hit of explicit rays (parity and kernel-level entry point).
 template <int V, int TRI, bool ANY, int SL>
-__global__ void __launch_bounds__(kBlock) k_trace(DevScene S, const Vec4* __restrict__ rays, int n, Vec4* __restrict__ hits, void* spill, int spillCap) {
+__global__ void __launch_bounds__(kBlock, 8) k_trace(DevScene S, const Vec4* __restrict__ rays, int n, Vec4* __restrict__ hits, void* spill, int spillCap) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -224,28 +250,37 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, const Vec4* __rest
     }
 }
 
-__global__ void k_pass_begin(int* cnt, int words, int NK) {
-    for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = j == 0 ? NK : 0;
+// zero the pass counters; the depth-0 queue of partition x is the x-th of kParts equal slices of the path ids
+__global__ void k_pass_begin(int* cnt, int words, long long NK) {
+    for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = 0;
+    __syncthreads();
+    if (threadIdx.x < kParts) {
+        const long long x = threadIdx.x;
+        cnt[x] = (int)((x + 1) * NK / kParts - x * NK / kParts);
+    }
 }
 
 // per-pass totals: [0] closest rays, [1] shadow rays, [2] misses, [3] unoccluded, [4] continuations,
 // [5] camera-ray misses
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot) {
     if (threadIdx.x != 0) return;
-    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = (unsigned)cnt[2];
+    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = (unsigned)cnt[kCntMiss];
     for (int d = 0; d < depths; d++) {
-        a += (unsigned)cnt[d * kCntStride]; b += (unsigned)cnt[d * kCntStride + 1];
-        m += (unsigned)cnt[d * kCntStride + 2]; u += (unsigned)cnt[d * kCntStride + 3];
-        if (d > 0) c += (unsigned)cnt[d * kCntStride];
+        const int* cd = cnt + d * kCntStride;
+        unsigned long long q = 0;
+        for (int x = 0; x < kParts; x++) { q += (unsigned)cd[x]; b += (unsigned)cd[kParts + x]; }
+        a += q; if (d > 0) c += q;
+        m += (unsigned)cd[kCntMiss]; u += (unsigned)cd[kCntUnocc];
     }
     tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
 }
 
-// accum[pix] += sum over the K samples of the pass, in sample order
+// accum[pix] += sum over the K samples of the pass (path ids pix*K .. pix*K + K-1), in sample order
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K) {
     for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < N; pix += gridDim.x * blockDim.x) {
         float r = 0.f, g = 0.f, b = 0.f;
-        for (int s = 0; s < K; s++) { const Vec4 l = L[(size_t)s * N + pix]; r += l.x; g += l.y; b += l.z; }
+        const Vec4* l = L + (size_t)pix * K;
+        for (int s = 0; s < K; s++) { const Vec4 v = l[s]; r += v.x; g += v.y; b += v.z; }
         Vec4 a = accum[pix]; a.x += r; a.y += g; a.z += b; accum[pix] = a;
     }
 }
@@ -255,23 +290,29 @@ __global__ void k_fill(uint32_t* p, size_t words, uint32_t v) {
 }
 
 // ------------------------------------------------------------------------------------------------------- dispatch
-using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int, int);
+using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int);
 using ShadowFn = void (*)(DevScene, DevPaths, int);
 using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, void*, int);
 
+template <int V, int TRI, int SL>
+BounceFn pickBounce(int waves) {
+    if (waves >= 8) return k_bounce<V, TRI, SL, 8>;
+    if (waves >= 6) return k_bounce<V, TRI, SL, 6>;
+    return k_bounce<V, TRI, SL, 1>;
+}
 template <int V, int TRI>
-void pickBySL(int sl, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
+void pickBySL(int sl, int waves, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
     switch (sl) {
-        case 8: b = k_bounce<V, TRI, 8>; s = k_shadow<V, TRI, 8>; tc = k_trace<V, TRI, false, 8>; ta = k_trace<V, TRI, true, 8>; break;
-        case 32: b = k_bounce<V, TRI, 32>; s = k_shadow<V, TRI, 32>; tc = k_trace<V, TRI, false, 32>; ta = k_trace<V, TRI, true, 32>; break;
-        default: b = k_bounce<V, TRI, 16>; s = k_shadow<V, TRI, 16>; tc = k_trace<V, TRI, false, 16>; ta = k_trace<V, TRI, true, 16>; break;
+        case 8: b = pickBounce<V, TRI, 8>(waves); s = k_shadow<V, TRI, 8>; tc = k_trace<V, TRI, false, 8>; ta = k_trace<V, TRI, true, 8>; break;
+        case 32: b = pickBounce<V, TRI, 32>(waves); s = k_shadow<V, TRI, 32>; tc = k_trace<V, TRI, false, 32>; ta = k_trace<V, TRI, true, 32>; break;
+        default: b = pickBounce<V, TRI, 16>(waves); s = k_shadow<V, TRI, 16>; tc = k_trace<V, TRI, false, 16>; ta = k_trace<V, TRI, true, 16>; break;
     }
 }
 template <int V>
-void pickByTri(int tri, int sl, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
-    if (tri == kTriMT) pickBySL<V, kTriMT>(sl, b, s, tc, ta);
-    else if (tri == kTriWatertight) pickBySL<V, kTriWatertight>(sl, b, s, tc, ta);
-    else pickBySL<V, kTriWoop>(sl, b, s, tc, ta);
+void pickByTri(int tri, int sl, int waves, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
+    if (tri == kTriMT) pickBySL<V, kTriMT>(sl, waves, b, s, tc, ta);
+    else if (tri == kTriWatertight) pickBySL<V, kTriWatertight>(sl, waves, b, s, tc, ta);
+    else pickBySL<V, kTriWoop>(sl, waves, b, s, tc, ta);
 }
 
 #define HIPCHK(x)                                                                                                      \
@@ -369,20 +410,29 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     HIPCHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&I.f0)); HIPCHK(hipEventCreate(&I.f1));
 
+    // BVH8 nodes are re-strided from 80 to 128 bytes: each node then fills exactly one 128-byte cache line instead of
+    // straddling two in 3 cases out of 5
+    std::vector<Vec4> devNodes;
+    if (S.variant == 8) {
+        const size_t nn = S.nodes.size() / 5;
+        devNodes.assign(nn * kBvh8Stride, v4(0.f, 0.f, 0.f, 0.f));
+        for (size_t j = 0; j < nn; j++)
+            for (int q = 0; q < 5; q++) devNodes[j * kBvh8Stride + q] = S.nodes[j * 5 + q];
+    } else devNodes = S.nodes;
     Vec4 *dn, *dw, *dv, *dnr, *da;
-    if (!I.upload(dn, S.nodes, err) || !I.upload(dw, S.woop, err) || !I.upload(dv, S.verts, err) ||
+    if (!I.upload(dn, devNodes, err) || !I.upload(dw, S.woop, err) || !I.upload(dv, S.verts, err) ||
         !I.upload(dnr, S.normals, err) || !I.upload(da, S.albedo, err)) return false;
     I.S.nodes = dn; I.S.verts = dv; I.S.nrm = dnr; I.S.albedo = da; I.S.root = S.root;
     I.S.tri = O.tri == kTriWoop ? dw : dv;
     if (F.hasN && !dnr) { err = "hasN set without normals"; return false; }
 
-    if (S.variant == 2) pickByTri<2>(O.tri, I.sl, I.bounce, I.shadow, I.traceC, I.traceA);
-    else pickByTri<8>(O.tri, I.sl, I.bounce, I.shadow, I.traceC, I.traceA);
+    if (S.variant == 2) pickByTri<2>(O.tri, I.sl, O.bounceWaves, I.bounce, I.shadow, I.traceC, I.traceA);
+    else pickByTri<8>(O.tri, I.sl, O.bounceWaves, I.bounce, I.shadow, I.traceC, I.traceA);
     auto gridOf = [&](const void* fn) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kBlock, 0) != hipSuccess || b < 1) b = 1;
         if (O.blocksPerCU > 0) b = std::min(b, O.blocksPerCU);
-        return b * numCU;
+        return std::max(kParts, (b * numCU) / kParts * kParts);   // a whole number of waves per partition
     };
     I.gridBounce = gridOf((const void*)I.bounce);
     I.gridShadow = gridOf((const void*)I.shadow);
@@ -441,7 +491,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
         for (int depth = 0; depth < I.depths; depth++) {
             hipEvent_t a = nullptr, b = nullptr;
             if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-            I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.N, I.NK);
+            I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.K);
             if (I.O.timeKernels) { HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b }); }
             if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
             I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);

@@ -19,12 +19,16 @@ namespace cwp {
 
 struct Hit { float t, u, v; int tri; };
 
+// Device stride of a compressed BVH8 node in float4: the 80-byte node padded to one 128-byte cache line.
+constexpr int kBvh8Stride = 8;
+
 // Per-ray constants shared by every node and triangle test.
 struct RayPre {
     Vec3 o, d;
     float tmin;
     float ix, iy, iz;          // reciprocal direction, |d| clamped to 1e-20
-    uint32_t oct;              // bit k set: direction component k > 0
+    uint32_t oct;              // bit k set: direction component k > 0 (child visiting order)
+    uint32_t pos;              // bit k set: reciprocal direction component k >= 0 (near slab plane is the low one)
     int kx, ky, kz;            // watertight: dominant axis kz and the two others
     float Sx, Sy, Sz;          // watertight shear
 };
@@ -37,6 +41,7 @@ __device__ __forceinline__ RayPre prepareRay(Vec3 o, Vec3 d, float tmin) {
     r.o = o; r.d = d; r.tmin = tmin;
     r.ix = safeRcp(d.x); r.iy = safeRcp(d.y); r.iz = safeRcp(d.z);
     r.oct = (d.x > 0.f ? 1u : 0u) | (d.y > 0.f ? 2u : 0u) | (d.z > 0.f ? 4u : 0u);
+    r.pos = (r.ix >= 0.f ? 1u : 0u) | (r.iy >= 0.f ? 2u : 0u) | (r.iz >= 0.f ? 4u : 0u);
     r.kx = r.ky = r.kz = 0; r.Sx = r.Sy = r.Sz = 0.f;
     if (TRI == kTriWatertight) {
         float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -181,7 +186,7 @@ __device__ __forceinline__ bool traceBVH8(const Vec4* __restrict__ nodes, const 
     uint32_t gBase = 0, gMask = 0, gHits = 0;
     int node = root;
     for (;;) {
-        const Vec4* nb = nodes + (size_t)node * 5;
+        const Vec4* nb = nodes + (size_t)node * kBvh8Stride;
         const Vec4 f0 = ldg4(nb), f1 = ldg4(nb + 1), f2 = ldg4(nb + 2), f3 = ldg4(nb + 3), f4 = ldg4(nb + 4);
         const uint32_t ew = asUint(f0.w);
         const float sx = asFloatU((ew & 255u) << 23) * r.ix;
@@ -194,24 +199,43 @@ __device__ __forceinline__ bool traceBVH8(const Vec4* __restrict__ nodes, const 
         const uint32_t qlx[2] = { asUint(f2.x), asUint(f2.y) }, qly[2] = { asUint(f2.z), asUint(f2.w) };
         const uint32_t qlz[2] = { asUint(f3.x), asUint(f3.y) }, qhx[2] = { asUint(f3.z), asUint(f3.w) };
         const uint32_t qhy[2] = { asUint(f4.x), asUint(f4.y) }, qhz[2] = { asUint(f4.z), asUint(f4.w) };
-        uint32_t hits = 0;
+        // Phase 1: the 8 child slabs against the hit distance at node entry. With the ray's direction signs the near
+        // plane of each axis is known up front (lo for a positive direction, hi for a negative one), so each slot needs
+        // no per-axis min/max; the result is bit-identical to min/max of the two planes.
+        const uint32_t nx[2] = { r.pos & 1u ? qlx[0] : qhx[0], r.pos & 1u ? qlx[1] : qhx[1] };
+        const uint32_t fx[2] = { r.pos & 1u ? qhx[0] : qlx[0], r.pos & 1u ? qhx[1] : qlx[1] };
+        const uint32_t ny[2] = { r.pos & 2u ? qly[0] : qhy[0], r.pos & 2u ? qly[1] : qhy[1] };
+        const uint32_t fy[2] = { r.pos & 2u ? qhy[0] : qly[0], r.pos & 2u ? qhy[1] : qly[1] };
+        const uint32_t nz[2] = { r.pos & 4u ? qlz[0] : qhz[0], r.pos & 4u ? qlz[1] : qhz[1] };
+        const uint32_t fz[2] = { r.pos & 4u ? qhz[0] : qlz[0], r.pos & 4u ? qhz[1] : qlz[1] };
+        const float tEntry = h.t;
+        uint32_t hitSlots = 0, leafSlots = 0;
 #pragma unroll
         for (int s = 0; s < 8; s++) {
             const int w = s >> 2;
             const uint32_t m = byteOf(meta[w], s);
-            if (m == 0) continue;
-            float x0 = fmaf((float)byteOf(qlx[w], s), sx, ox), x1 = fmaf((float)byteOf(qhx[w], s), sx, ox);
-            float y0 = fmaf((float)byteOf(qly[w], s), sy, oy), y1 = fmaf((float)byteOf(qhy[w], s), sy, oy);
-            float z0 = fmaf((float)byteOf(qlz[w], s), sz, oz), z1 = fmaf((float)byteOf(qhz[w], s), sz, oz);
-            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h.t));
-            if (tn <= tf) {
-                if (m & 0x80u) {
-                    int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
-                    for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
-                } else hits |= 1u << (s ^ r.oct);
-            }
+            const float xn = fmaf((float)byteOf(nx[w], s), sx, ox), xf = fmaf((float)byteOf(fx[w], s), sx, ox);
+            const float yn = fmaf((float)byteOf(ny[w], s), sy, oy), yf = fmaf((float)byteOf(fy[w], s), sy, oy);
+            const float zn = fmaf((float)byteOf(nz[w], s), sz, oz), zf = fmaf((float)byteOf(fz[w], s), sz, oz);
+            const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
+            const float tf = fminf(fminf(xf, yf), fminf(zf, tEntry));
+            hitSlots |= (uint32_t)(m != 0 && tn <= tf) << s;
+            leafSlots |= (m >> 7) << s;
         }
+        // Phase 2: hit leaves in slot order, one shared loop
+        uint32_t lb = hitSlots & leafSlots;
+        while (lb) {
+            const int s = __builtin_ctz(lb);
+            lb &= lb - 1;
+            const uint32_t m = byteOf(s < 4 ? meta[0] : meta[1], s);
+            const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+            for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+        }
+        // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
+        uint32_t hits = hitSlots & ~leafSlots;
+        if (r.oct & 1u) hits = ((hits & 0x55u) << 1) | ((hits >> 1) & 0x55u);
+        if (r.oct & 2u) hits = ((hits & 0x33u) << 2) | ((hits >> 2) & 0x33u);
+        if (r.oct & 4u) hits = ((hits & 0x0Fu) << 4) | ((hits >> 4) & 0x0Fu);
         if (hits) {
             if (gHits) st.push(Group{ gBase, (gMask << 8) | gHits });
             gBase = (uint32_t)childBase; gMask = imask; gHits = hits;

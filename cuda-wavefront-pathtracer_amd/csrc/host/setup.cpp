@@ -54,6 +54,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--warm") ok = inum(A.warm);
         else if (k == "--reps") ok = inum(A.reps);
         else if (k == "--stack") ok = inum(A.stack);
+        else if (k == "--bounce_waves") ok = inum(A.bounceWaves);
         else if (k == "--blocks_per_cu") ok = inum(A.blocksPerCU);
         else if (k == "--time_kernels") ok = inum(A.timeKernels);
         else if (k == "--filter") { std::string f; ok = str(f); A.filterBH = f == "bh"; if (ok && f != "bh" && f != "box") { err = "--filter must be bh or box"; return false; } }
@@ -88,6 +89,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.batch < 1) A.batch = 1;
     if (A.bounces < 0) { err = "--bounces must be >= 0"; return false; }
     if (A.stack != 8 && A.stack != 16 && A.stack != 32) { err = "--stack must be 8, 16 or 32"; return false; }
+    if (A.bounceWaves != 1 && A.bounceWaves != 6 && A.bounceWaves != 8) { err = "--bounce_waves must be 1, 6 or 8"; return false; }
     if (!A.gen.empty() && A.gen != "boxes" && A.gen != "terrain") { err = "--gen must be boxes or terrain"; return false; }
     return true;
 }
@@ -183,6 +185,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     RenderOptions& O = Z.O;
     O.variant = V; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
+    O.bounceWaves = A.bounceWaves;
     return true;
 }
 

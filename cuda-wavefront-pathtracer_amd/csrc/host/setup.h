@@ -18,7 +18,8 @@ struct Args {
     std::string integrator = "auto", builder = "sah", collapse = "sah";
     int genBoxes = -1, genGrid = 242; uint64_t genSeed = 1;
     int fuse = 1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
-    int persistent = 1, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 16, blocksPerCU = 0, timeKernels = 1;
+    int persistent = 1, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
+    int bounceWaves = 6;
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);

@@ -623,7 +623,10 @@ static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
         int childBase = f2i(n[4]), triBase = f2i(n[5]);
         uint32_t qlx[2] = { f2u(n[8]), f2u(n[9]) }, qly[2] = { f2u(n[10]), f2u(n[11]) }, qlz[2] = { f2u(n[12]), f2u(n[13]) };
         uint32_t qhx[2] = { f2u(n[14]), f2u(n[15]) }, qhy[2] = { f2u(n[16]), f2u(n[17]) }, qhz[2] = { f2u(n[18]), f2u(n[19]) };
-        uint32_t hits = 0;
+        /* all 8 child slabs against the hit distance at node entry, then the hit leaves in slot order (the reference
+         * interleaves them; the closest hit is the same, and the order of triangle tests within a node is kept) */
+        const float tEntry = h->t;
+        uint32_t hitSlots = 0, leafSlots = 0;
         for (int sl = 0; sl < 8; sl++) {
             int w = sl >> 2;
             uint32_t m = byte_of(meta[w], sl);
@@ -632,13 +635,18 @@ static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
             float y0 = fmaf((float)byte_of(qly[w], sl), sy, oy), y1 = fmaf((float)byte_of(qhy[w], sl), sy, oy);
             float z0 = fmaf((float)byte_of(qlz[w], sl), sz, oz), z1 = fmaf((float)byte_of(qhz[w], sl), sz, oz);
             float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r->tmin));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), h->t));
-            if (tn <= tf) {
-                if (m & 0x80u) {
-                    int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
-                    for (int t = first; t < first + cnt; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
-                } else hits |= 1u << (sl ^ r->oct);
-            }
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tEntry));
+            if (tn <= tf) hitSlots |= 1u << sl;
+            if (m & 0x80u) leafSlots |= 1u << sl;
+        }
+        uint32_t hits = 0;
+        for (int sl = 0; sl < 8; sl++) {
+            if (!(hitSlots >> sl & 1u)) continue;
+            if (leafSlots >> sl & 1u) {
+                uint32_t m = byte_of(meta[sl >> 2], sl);
+                int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+                for (int t = first; t < first + cnt; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
+            } else hits |= 1u << (sl ^ r->oct);
         }
         if (hits) {
             if (gH) { stB[sp] = gB; stM[sp] = (gM << 8) | gH; sp++; }
