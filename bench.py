@@ -24,10 +24,12 @@ LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3
             "--sky", "0.3", "0.38", "0.5"]
 
 
-def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=6):
+def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=8, fuse=1,
+                  persistent=0, shadow_refill=0):
     return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
             "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack),
-            "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves)] + LIGHTING
+            "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves), "--fuse", str(fuse),
+            "--persistent", str(persistent), "--shadow_refill", str(shadow_refill)] + LIGHTING
 
 
 def algorithmic_bytes(st, npaths):
@@ -87,8 +89,11 @@ def main():
     ap.add_argument("--tri", default="woop")
     ap.add_argument("--variant", default="bvh8")
     ap.add_argument("--stack", type=int, default=8)
-    ap.add_argument("--bounce-waves", type=int, default=6)
+    ap.add_argument("--bounce-waves", type=int, default=8)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--fuse", type=int, default=1, help="1: fused trace+shade kernel; 0: split trace and shade kernels")
+    ap.add_argument("--persistent", type=int, default=0, help="1: split kernels with lanes refilled mid-traversal")
+    ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--cpu-spp", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -103,7 +108,7 @@ def main():
     import cwp
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
-                                args.blocks_per_cu, args.bounce_waves))
+                                args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill))
     r = cwp.Renderer(s)
     info = s.info
     npaths = args.width * args.height * args.batch
@@ -146,11 +151,15 @@ def main():
                    "scene": "242x242 fBm heightfield + 200 boxes (%d tris), albedo 0.8, sun 3 W/m2 + sky" % info["num_tris"],
                    "width": args.width, "height": args.height, "spp": args.batch, "batch": args.batch,
                    "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
-                   "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"]},
+                   "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"],
+                   "kernels": "persistent" if args.persistent else ("fused" if args.fuse else "split"),
+                   "shadow_refill": args.shadow_refill},
         "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
         "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
-        "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shadow_ms": round(last["t_shadow_ms"], 3),
-        "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)",
+        "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shade_ms": round(last["t_shade_ms"], 3),
+        "t_shadow_ms": round(last["t_shadow_ms"], 3),
+        "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
+                     else "k_trace_paths + k_shade (split trace / shade)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "bytes_per_frame": bytes_bounce / args.steps,

@@ -20,8 +20,8 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcwp.so")
 CLI = os.path.join(PKG, "pt")
 
-HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/bvh.cpp", "host/setup.cpp", "capi.cpp"]
-HIP_SOURCES = ["device/renderer.hip"]
+HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/bvh.cpp", "host/scene_check.cpp", "host/setup.cpp", "capi.cpp"]
+HIP_SOURCES = ["device/renderer.hip", "device/hploc.hip"]
 CLI_SOURCES = ["app/pt.cpp"]
 
 INCLUDES = ["-I" + CSRC, "-I" + os.path.join(ROOT, "include")]

@@ -2,6 +2,8 @@
 #include "cwp.h"
 #include "device/renderer.h"
 #include "host/bvh.h"
+#include "host/scene_check.h"
+#include "device/hploc.h"
 #include "host/setup.h"
 #include <cstddef>
 #include <cstring>
@@ -12,6 +14,7 @@ using namespace cwp;
 
 struct cwp_setup { SceneSetup Z; std::vector<int> triFlat; std::vector<int> tree; std::vector<float> treeBox; };
 struct cwp_ctx { Renderer R; const cwp_setup* setup = nullptr; };
+struct cwp_hploc { HplocBuilder B; };
 struct cwp_bvh {
     BinaryBVH tree; std::vector<Vec4> nodes, woop, corners, nrm; std::vector<int> order, treeFlat; std::vector<float> treeBox;
     int variant = 2, stackNeed = 0; double sah = 0;
@@ -122,7 +125,7 @@ int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats*
         stats->t_bounce_ms = st.bounceMs; stats->t_shadow_ms = st.shadowMs;
         stats->bounce_launches = st.bounceLaunches; stats->shadow_launches = st.shadowLaunches;
         stats->rays_hit = st.hitRays; stats->rays_miss = st.missRays; stats->rays_cont = st.contRays; stats->shadow_unoccluded = st.unoccluded;
-        stats->camera_miss = st.cameraMiss;
+        stats->camera_miss = st.cameraMiss; stats->t_shade_ms = st.shadeMs;
     }
     return CWP_OK;
 }
@@ -161,10 +164,72 @@ int cwp_frame_json(const cwp_setup* s, const cwp_ctx* c, const cwp_frame_stats* 
     FrameStats F;
     F.rays = st->rays_closest; F.shadowRays = st->rays_shadow; F.traceMs = st->t_trace_ms; F.frameMs = st->t_frame_ms;
     F.bounceMs = st->t_bounce_ms; F.shadowMs = st->t_shadow_ms; F.hitRays = st->rays_hit; F.missRays = st->rays_miss;
-    F.contRays = st->rays_cont; F.unoccluded = st->shadow_unoccluded; F.cameraMiss = st->camera_miss;
+    F.contRays = st->rays_cont; F.unoccluded = st->shadow_unoccluded; F.cameraMiss = st->camera_miss; F.shadeMs = st->t_shade_ms;
     std::string j = frameJson(s->Z, c->R, F, rep, frame);
     if (j.size() + 1 > cap) return fail(CWP_ERR_ARG, "cwp_frame_json: buffer too small");
     memcpy(buf, j.c_str(), j.size() + 1);
+    return CWP_OK;
+}
+
+void cwp_options_default(cwp_options* o) {
+    if (!o) return;
+    const RenderOptions d;
+    o->tri_test = d.tri; o->batch = d.batch; o->bounces = d.bounces; o->fuse = d.fuse; o->persistent = d.persistent;
+    o->shadow_refill = d.shadowRefill; o->stack_lds = 8; o->blocks_per_cu = d.blocksPerCU; o->bounce_waves = d.bounceWaves;
+    o->time_kernels = d.timeKernels;
+}
+
+static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& err) {
+    if (d->variant != 2 && d->variant != 8) { err = "variant must be 2 or 8"; return false; }
+    if (!d->nodes || !d->num_nodes || !d->corners || !d->num_tris || !d->albedo || !d->num_materials) {
+        err = "nodes, corners and albedo are required"; return false;
+    }
+    if (d->num_nodes > 0x7fffffff || d->num_tris > 0x7fffffff || d->num_materials > 0x7fffffff) { err = "arrays too large"; return false; }
+    const size_t per = d->variant == 2 ? 4 : 5;
+    auto take = [](const float* p, size_t n4, std::vector<Vec4>& out) {
+        out.resize(n4);
+        if (n4) memcpy(out.data(), p, n4 * sizeof(Vec4));
+    };
+    S.variant = d->variant; S.root = d->root;
+    take(d->nodes, d->num_nodes * per, S.nodes);
+    take(d->corners, d->num_tris * 3, S.verts);
+    if (d->tri_form != CWP_TRIS_CORNERS && d->tri_form != CWP_TRIS_EDGES) { err = "tri_form must be corners or edges"; return false; }
+    if (d->tri_form == CWP_TRIS_EDGES)
+        for (size_t t = 0; t < d->num_tris; t++) {
+            const Vec3 v0 = xyz(S.verts[3 * t]);
+            for (int k = 1; k < 3; k++) { Vec4& e = S.verts[3 * t + k]; e = v4(v0 + xyz(e), e.w); }
+        }
+    if (d->woop) take(d->woop, d->num_tris * 3, S.woop); else S.woop.clear();
+    if (d->normals) take(d->normals, d->num_tris * 3, S.normals); else S.normals.clear();
+    take(d->albedo, d->num_materials, S.albedo);
+    return checkScene(S, S.stackNeed, err);
+}
+
+int cwp_scene_check(const cwp_scene_desc* d, int* stack_need) {
+    if (!d) return fail(CWP_ERR_ARG, "cwp_scene_check: null argument");
+    SceneUpload S; std::string err;
+    if (!descToUpload(d, S, err)) return fail(CWP_ERR_ARG, err);
+    if (stack_need) *stack_need = S.stackNeed;
+    return CWP_OK;
+}
+
+int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* p, const cwp_options* o, cwp_ctx** out) {
+    if (!d || !p || !o || !out) return fail(CWP_ERR_ARG, "cwp_create_scene: null argument");
+    *out = nullptr;
+    SceneUpload S; std::string err;
+    if (!descToUpload(d, S, err)) return fail(CWP_ERR_ARG, err);
+    FrameParams F; memcpy(&F, p, sizeof F);
+    RenderOptions O;
+    O.variant = d->variant; O.tri = o->tri_test; O.batch = o->batch; O.bounces = o->bounces; O.fuse = o->fuse;
+    O.persistent = o->persistent; O.shadowRefill = o->shadow_refill; O.stackLds = o->stack_lds;
+    O.blocksPerCU = o->blocks_per_cu; O.bounceWaves = o->bounce_waves; O.timeKernels = o->time_kernels;
+    if (O.batch < 1) return fail(CWP_ERR_ARG, "batch must be >= 1");
+    if (O.fuse != 0 && O.fuse != 1) return fail(CWP_ERR_ARG, "fuse must be 0 or 1");
+    if (O.persistent && O.fuse) return fail(CWP_ERR_ARG, "persistent needs fuse 0");
+    cwp_ctx* c = new (std::nothrow) cwp_ctx();
+    if (!c) return fail(CWP_ERR_ARG, "out of host memory");
+    if (!c->R.init(S, F, O, err)) { delete c; return fail(CWP_ERR_DEVICE, err); }
+    *out = c;
     return CWP_OK;
 }
 
@@ -175,24 +240,22 @@ int cwp_ctx_info(const cwp_ctx* c, int* grid_blocks, double* vram_mb) {
     return CWP_OK;
 }
 
-int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int variant, int bins, int leafMax, cwp_bvh** out) {
-    if (!out || (nv > 0 && !vertices) || (nt > 0 && !tris) || nv < 0 || nt < 1) return fail(CWP_ERR_ARG, "cwp_bvh_build: bad arguments");
-    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_build: variant must be 2 or 8");
-    if (bins < 2 || bins > 256 || leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_build: bins in [2,256], leaf_max in [1,32]");
-    *out = nullptr;
-    Mesh m;
+static bool meshFromArrays(const float* vertices, int nv, const int* tris, int nt, Mesh& m, std::string& err) {
+    if ((nv > 0 && !vertices) || (nt > 0 && !tris) || nv < 0 || nt < 1) { err = "bad mesh arguments"; return false; }
     m.v.resize(nv);
     for (int i = 0; i < nv; i++) m.v[i] = v3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]);
     m.f.resize(nt);
     for (int t = 0; t < nt; t++) {
         Tri q{ tris[3 * t], tris[3 * t + 1], tris[3 * t + 2] };
-        if (q.a < 0 || q.b < 0 || q.c < 0 || q.a >= nv || q.b >= nv || q.c >= nv) return fail(CWP_ERR_ARG, "cwp_bvh_build: vertex index out of range");
+        if (q.a < 0 || q.b < 0 || q.c < 0 || q.a >= nv || q.b >= nv || q.c >= nv) { err = "vertex index out of range"; return false; }
         m.f[t] = q;
     }
-    cwp_bvh* b = new (std::nothrow) cwp_bvh();
-    if (!b) return fail(CWP_ERR_ARG, "out of host memory");
+    return true;
+}
+
+// layout, triangle packs and flat tree arrays of b->tree
+static void finishBVH(const Mesh& m, int variant, cwp_bvh* b) {
     b->variant = variant;
-    buildSAH(m, bins, variant == 8 ? 1 : leafMax, b->tree);
     if (variant == 2) {
         layoutBVH2(b->tree, b->nodes); b->order = b->tree.prim; b->sah = sahCost(b->tree, 1, 1);
         b->stackNeed = bvh2Depth(b->nodes) + 1;
@@ -206,7 +269,109 @@ int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int va
         b->treeFlat.insert(b->treeFlat.end(), { n.left, n.right, n.first, n.count });
         b->treeBox.insert(b->treeBox.end(), { n.box.lo.x, n.box.lo.y, n.box.lo.z, n.box.hi.x, n.box.hi.y, n.box.hi.z });
     }
+}
+
+int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int variant, int bins, int leafMax, cwp_bvh** out) {
+    if (!out) return fail(CWP_ERR_ARG, "cwp_bvh_build: null argument");
+    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_build: variant must be 2 or 8");
+    if (bins < 2 || bins > 256 || leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_build: bins in [2,256], leaf_max in [1,32]");
+    *out = nullptr;
+    Mesh m; std::string err;
+    if (!meshFromArrays(vertices, nv, tris, nt, m, err)) return fail(CWP_ERR_ARG, "cwp_bvh_build: " + err);
+    cwp_bvh* b = new (std::nothrow) cwp_bvh();
+    if (!b) return fail(CWP_ERR_ARG, "out of host memory");
+    buildSAH(m, bins, variant == 8 ? 1 : leafMax, b->tree);
+    finishBVH(m, variant, b);
     *out = b;
+    return CWP_OK;
+}
+
+int cwp_bvh_from_hploc(const float* vertices, int nv, const int* tris, int nt, const float* hnodes, int root, int variant,
+                       int leafMax, cwp_bvh** out) {
+    if (!out || !hnodes) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: null argument");
+    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: variant must be 2 or 8");
+    if (leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: leaf_max in [1,32]");
+    *out = nullptr;
+    Mesh m; std::string err;
+    if (!meshFromArrays(vertices, nv, tris, nt, m, err)) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: " + err);
+    const size_t tot = 2 * (size_t)nt - 1;
+    std::vector<Vec4> hn(2 * tot);
+    memcpy(hn.data(), hnodes, hn.size() * sizeof(Vec4));
+    // the conversion trusts the ids: check them (children before parents, leaves name triangles, root in range)
+    if (root < 0 || (size_t)root >= tot) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: root out of range");
+    for (size_t h = 0; h < tot; h++) {
+        const int l = asInt(hn[2 * h].w), r = asInt(hn[2 * h + 1].w);
+        if (h < (size_t)nt) {
+            if (l < 0 || l >= nt || r != -1) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: bad leaf node");
+        } else if (l < 0 || r < 0 || (size_t)l >= h || (size_t)r >= h) {
+            return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: child ids must be smaller than the parent's");
+        }
+    }
+    cwp_bvh* b = new (std::nothrow) cwp_bvh();
+    if (!b) return fail(CWP_ERR_ARG, "out of host memory");
+    treeFromHploc(hn, nt, root, variant == 8 ? 1 : leafMax, b->tree);
+    if ((int)b->tree.prim.size() != nt) { delete b; return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: the tree does not hold every triangle once"); }
+    finishBVH(m, variant, b);
+    *out = b;
+    return CWP_OK;
+}
+
+int cwp_hploc_build_host(const float* lo, const float* hi, int n, float* nodes, int* root, double* gpu_ms) {
+    if (!lo || !hi || !nodes || n < 1) return fail(CWP_ERR_ARG, "cwp_hploc_build_host: bad arguments");
+    int r = 0; double ms = 0; std::string err;
+    if (!hplocBuildHost((const Vec4*)lo, (const Vec4*)hi, n, (Vec4*)nodes, r, ms, err)) return fail(CWP_ERR_DEVICE, err);
+    if (root) *root = r;
+    if (gpu_ms) *gpu_ms = ms;
+    return CWP_OK;
+}
+
+int cwp_hploc_create(int capacity, cwp_hploc** out) {
+    if (!out || capacity < 1) return fail(CWP_ERR_ARG, "cwp_hploc_create: bad arguments");
+    *out = nullptr;
+    cwp_hploc* b = new (std::nothrow) cwp_hploc();
+    if (!b) return fail(CWP_ERR_ARG, "out of host memory");
+    std::string err;
+    if (!b->B.reserve(capacity, err)) { delete b; return fail(CWP_ERR_DEVICE, err); }
+    *out = b;
+    return CWP_OK;
+}
+
+void cwp_hploc_destroy(cwp_hploc* b) { delete b; }
+
+int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root) {
+    if (!b || !lo || !hi || n < 1) return fail(CWP_ERR_ARG, "cwp_hploc_build: bad arguments");
+    int r = 0; std::string err;
+    if (!b->B.build((const Vec4*)lo, (const Vec4*)hi, n, stream, r, err)) return fail(CWP_ERR_DEVICE, err);
+    if (root) *root = r;
+    return CWP_OK;
+}
+
+int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes) {
+    if (!b || !nodes) return fail(CWP_ERR_ARG, "cwp_hploc_nodes: null argument");
+    *nodes = b->B.nodes();
+    return CWP_OK;
+}
+
+int cwp_hploc_to_bvh2(const void* hnodes, int n, void* out, void* stream) {
+    std::string err;
+    if (!hplocToBVH2((const Vec4*)hnodes, n, (Vec4*)out, stream, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_device_malloc(size_t bytes, void** ptr) {
+    std::string err;
+    if (!ptr) return fail(CWP_ERR_ARG, "cwp_device_malloc: null argument");
+    if (!deviceMalloc(bytes, ptr, err)) return fail(CWP_ERR_DEVICE, err);
+    return CWP_OK;
+}
+
+void cwp_device_free(void* ptr) { deviceFree(ptr); }
+
+int cwp_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+    std::string err;
+    if ((!dst || !src) && bytes) return fail(CWP_ERR_ARG, "cwp_memcpy: null pointer");
+    if (kind < CWP_COPY_H2D || kind > CWP_COPY_D2D) return fail(CWP_ERR_ARG, "cwp_memcpy: unknown kind");
+    if (!deviceCopy(dst, src, bytes, kind, err)) return fail(CWP_ERR_DEVICE, err);
     return CWP_OK;
 }
 

@@ -1,0 +1,59 @@
+// device/hploc.h - GPU BVH2 builder by hierarchical PLOC (H-PLOC, Benthin et al. 2024) for gfx950.
+//
+// Same contract as the reference's HplocBuilder (src/accel/hploc.h:21-40): primitive boxes in, 2n - 1 nodes of two
+// float4 out, (lo.xyz, left id bits), (hi.xyz, right id bits). Nodes [0, n) are the leaves in Morton order
+// (left = primitive id, right = -1); internal nodes [n, 2n - 1) come from an atomic counter, so every child id is
+// smaller than its parent's and the root is node 2n - 2 (node 0 when n == 1).
+//
+// Steps (all stream-ordered on the caller's stream, no host synchronisation):
+//  1. centroid box: wave reductions + one order-preserving integer atomic per wave and axis;
+//  2. 64-bit keys: Morton code (mb bits per axis) << ib | primitive index, ib = ceil(log2 n), mb = min(21, (64-ib)/3);
+//  3. LSD radix sort of the keys, 8 bits per pass over the 3 mb + ib used bits (one workgroup for small n, else
+//     count / scan / stable scatter passes); keys are unique, so the order is the only correct one;
+//  4. leaves, then one kernel of single-wave workgroups: every lane climbs the implicit LBVH from its leaf (the
+//     parent is picked by the key deltas at the two ends of the range; the first lane to reach a node stops, the
+//     second continues with the union), and each LBVH node whose range exceeds 16 clusters (or the root) is reduced
+//     by PLOC iterations (nearest neighbour by merged surface area within 8 positions; mutual pairs merge) to 16
+//     clusters (1 at the root). A wave64 runs two such merges at once, one per half-wave of 32 lanes: the merge
+//     window is 2 x 16 clusters, as in the reference's 32-lane merge, so the trees are the same.
+#pragma once
+#include "core/vecmath.h"
+#include <string>
+
+namespace cwp {
+
+class HplocBuilder {
+public:
+    HplocBuilder() = default;
+    ~HplocBuilder();
+    HplocBuilder(const HplocBuilder&) = delete;
+    HplocBuilder& operator=(const HplocBuilder&) = delete;
+
+    bool reserve(int n, std::string& err);
+    // lo / hi: n float4 per primitive box (device). stream: a hipStream_t (nullptr = default stream). Returns the root
+    // node id through `root`; the nodes are ready when the stream reaches this point.
+    bool build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err);
+    const Vec4* nodes() const { return nodes_; }   // device, 2 float4 per node
+    const uint64_t* sortedKeys() const { return keysA_; }
+    void release();
+    int capacity() const { return cap_; }
+
+private:
+    int cap_ = 0;
+    Vec4* nodes_ = nullptr;
+    uint64_t *keysA_ = nullptr, *keysB_ = nullptr;
+    int *clusters_ = nullptr, *bounds_ = nullptr, *counter_ = nullptr, *counts_ = nullptr;
+    uint32_t* cbox_ = nullptr;
+    int tiles_ = 0;
+};
+
+// BVH2 layout (4 float4 per node, reference layout of src/accel/bvh_layouts.h:12-13) of H-PLOC nodes on the device:
+// internal node j <- H-PLOC node n + j, root n - 2; a leaf child becomes (primitive id, count 1). n == 1: node 0 holds
+// the leaf and an empty slot (root 0). Reference: src/accel/hploc.cu:138-160.
+bool hplocToBVH2(const Vec4* hnodes, int n, Vec4* out, void* stream, std::string& err);
+inline int hplocBVH2Root(int n) { return n > 1 ? n - 2 : 0; }
+
+// Host convenience: uploads the boxes, builds (once to warm up, once timed with events), downloads 2n - 1 nodes.
+bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err);
+
+}  // namespace cwp

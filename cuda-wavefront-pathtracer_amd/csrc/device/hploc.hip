@@ -1,0 +1,503 @@
+// device/hploc.hip - H-PLOC BVH2 builder for gfx950 (see hploc.h). Written from the H-PLOC paper (Benthin et al.
+// 2024) and the reference's description of its variant (src/accel/hploc.h:1-11): Morton-keyed LBVH climbed bottom-up
+// (Apetrei 2014), PLOC merges of 2 x 16 clusters within a radius of 8 positions.
+#include "device/hploc.h"
+#include <hip/hip_runtime.h>
+#include <algorithm>
+
+namespace cwp {
+
+namespace {
+
+constexpr int kT = 16;          // clusters kept per LBVH subtree
+constexpr int kR = 8;           // PLOC nearest-neighbour radius (positions in Morton order)
+constexpr int kSortBlock = 256; // radix sort workgroup: 4 waves, one thread per digit value
+constexpr int kSortItems = 16;  // keys per thread per pass
+constexpr int kTile = kSortBlock * kSortItems;
+constexpr int kSmallSort = 16384;   // up to this many keys are sorted by one workgroup (no inter-workgroup passes)
+
+// order-preserving float <-> uint mapping, for min / max through integer atomics
+__device__ __forceinline__ uint32_t f2o(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+// spreads the low 21 bits of v to every third bit of a 63-bit word (bit b -> bit 3 b)
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x001f00000000ffffull;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ int lanePrefix(uint64_t mask) {   // set bits of mask below this lane
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// min / max as comparisons (not fminf / fmaxf), so signed zeros resolve the same way on every machine
+__device__ __forceinline__ float mn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float mx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ Vec3 vmn(Vec3 a, Vec3 b) { return v3(mn(a.x, b.x), mn(a.y, b.y), mn(a.z, b.z)); }
+__device__ __forceinline__ Vec3 vmx(Vec3 a, Vec3 b) { return v3(mx(a.x, b.x), mx(a.y, b.y), mx(a.z, b.z)); }
+__device__ __forceinline__ float halfArea(Vec3 lo, Vec3 hi) {
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+__device__ __forceinline__ Vec3 shfl3(Vec3 v, int src) { return v3(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)); }
+
+// ------------------------------------------------------------------------------------------------ keys
+__global__ void k_hp_init(uint32_t* cbox, int* counter, int n) {
+    if (threadIdx.x < 3) { cbox[threadIdx.x] = 0xffffffffu; cbox[3 + threadIdx.x] = 0u; }
+    if (threadIdx.x == 0) *counter = n;
+}
+
+__global__ void k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
+    uint32_t a[3] = { 0xffffffffu, 0xffffffffu, 0xffffffffu }, b[3] = { 0u, 0u, 0u };
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Vec4 l = lo[i], h = hi[i];
+        const uint32_t c[3] = { f2o((l.x + h.x) * 0.5f), f2o((l.y + h.y) * 0.5f), f2o((l.z + h.z) * 0.5f) };
+        for (int k = 0; k < 3; k++) { a[k] = min(a[k], c[k]); b[k] = max(b[k], c[k]); }
+    }
+    for (int k = 0; k < 3; k++)
+        for (int off = 32; off > 0; off >>= 1) {
+            a[k] = min(a[k], (uint32_t)__shfl_xor((int)a[k], off));
+            b[k] = max(b[k], (uint32_t)__shfl_xor((int)b[k], off));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; k++) { atomicMin(&cbox[k], a[k]); atomicMax(&cbox[3 + k], b[k]); }
+}
+
+// key = Morton code of the box centroid (mb bits per axis, x << 2 | y << 1 | z) << ib | primitive index
+__global__ void k_hp_keys(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, const uint32_t* cbox,
+                          uint64_t* keys, int mb, int ib) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float bl[3] = { o2f(cbox[0]), o2f(cbox[1]), o2f(cbox[2]) }, bh[3] = { o2f(cbox[3]), o2f(cbox[4]), o2f(cbox[5]) };
+    const Vec4 l = lo[i], h = hi[i];
+    const float c[3] = { (l.x + h.x) * 0.5f, (l.y + h.y) * 0.5f, (l.z + h.z) * 0.5f };
+    const float s = (float)(1u << mb), smax = s - 1.f;
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        const float e = bh[k] - bl[k];
+        const float t = e > 0.f ? (c[k] - bl[k]) / e : 0.f;
+        float v = t * s;
+        v = v > 0.f ? v : 0.f;
+        v = v < smax ? v : smax;
+        q[k] = (uint32_t)v;
+    }
+    const uint64_t m = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    keys[i] = (m << ib) | (uint64_t)i;
+}
+
+// ------------------------------------------------------------------------------------------------ radix sort
+// Stable ranking of one 256-key round inside a 4-wave workgroup: each lane gets the number of keys with the same
+// digit before it in its wave (8 ballots over the digit bits) and the wave's count per digit goes to wh[wave][digit].
+__device__ __forceinline__ int rankInWave(bool valid, uint32_t d, int* whRow) {
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < 8; b++) {
+        const uint64_t bb = __ballot((d >> b) & 1u);
+        m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const int rank = lanePrefix(m);
+    if (valid && rank == 0) whRow[d] = (int)__builtin_popcountll(m);
+    return rank;
+}
+
+// exclusive prefix sum over the 256 threads of a workgroup (tmp: 4 ints of LDS); returns this thread's prefix
+__device__ __forceinline__ int blockScan256(int v, int* tmp, int& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; k++) before += tmp[k];
+    total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();
+    return before + x - v;
+}
+
+// digit histogram of one tile: counts[digit * tiles + tile]
+__global__ void __launch_bounds__(kSortBlock) k_rs_count(const uint64_t* __restrict__ keys, int n, int shift, int* counts, int tiles) {
+    __shared__ int hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+    for (int j = 0; j < kSortItems; j++) {
+        const int idx = base + j * kSortBlock + (int)threadIdx.x;
+        if (idx < n) atomicAdd(&hist[(uint32_t)(keys[idx] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// exclusive scan of counts[0, total) in place (one workgroup of 1024 threads)
+__global__ void __launch_bounds__(1024) k_rs_scan(int* counts, int total) {
+    __shared__ int part[1024];
+    const int per = (total + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(b + per, total);
+    int s = 0;
+    for (int j = b; j < e; j++) s += counts[j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {   // inclusive Hillis-Steele scan of the partial sums
+        const int y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int j = b; j < e; j++) { const int c = counts[j]; counts[j] = run; run += c; }
+}
+
+// stable scatter of one tile by the digit at `shift`, at the offsets scanned from k_rs_count
+__global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int n,
+                                                           int shift, const int* __restrict__ offsets, int tiles) {
+    __shared__ int base[256], running[256], wh[4][256];
+    const int t = threadIdx.x, w = t >> 6;
+    base[t] = offsets[t * tiles + blockIdx.x];
+    running[t] = 0;
+    const int tile = blockIdx.x * kTile;
+    for (int j = 0; j < kSortItems; j++) {
+        const int idx = tile + j * kSortBlock + t;
+        const bool valid = idx < n;
+        const uint64_t key = valid ? in[idx] : 0ull;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        wh[0][t] = 0; wh[1][t] = 0; wh[2][t] = 0; wh[3][t] = 0;
+        __syncthreads();
+        const int rank = rankInWave(valid, d, wh[w]);
+        __syncthreads();
+        if (valid) {
+            int pos = base[d] + running[d] + rank;
+            for (int k = 0; k < w; k++) pos += wh[k][d];
+            out[pos] = key;
+        }
+        __syncthreads();
+        running[t] += wh[0][t] + wh[1][t] + wh[2][t] + wh[3][t];
+        __syncthreads();
+    }
+}
+
+// whole sort in one workgroup (small n): every pass counts, scans and scatters a <-> b; the result ends in a if
+// `passes` is even, else in b
+__global__ void __launch_bounds__(kSortBlock) k_rs_single(uint64_t* a, uint64_t* b, int n, int passes) {
+    __shared__ int hist[256], running[256], wh[4][256], tmp[4];
+    const int t = threadIdx.x, w = t >> 6;
+    uint64_t* src = a;
+    uint64_t* dst = b;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        hist[t] = 0;
+        __syncthreads();
+        for (int idx = t; idx < n; idx += kSortBlock) atomicAdd(&hist[(uint32_t)(src[idx] >> shift) & 255u], 1);
+        __syncthreads();
+        int total;
+        const int start = blockScan256(hist[t], tmp, total);
+        __syncthreads();
+        hist[t] = start;
+        running[t] = 0;
+        __syncthreads();
+        for (int r = 0; r < n; r += kSortBlock) {
+            const int idx = r + t;
+            const bool valid = idx < n;
+            const uint64_t key = valid ? src[idx] : 0ull;
+            const uint32_t d = (uint32_t)(key >> shift) & 255u;
+            wh[0][t] = 0; wh[1][t] = 0; wh[2][t] = 0; wh[3][t] = 0;
+            __syncthreads();
+            const int rank = rankInWave(valid, d, wh[w]);
+            __syncthreads();
+            if (valid) {
+                int pos = hist[d] + running[d] + rank;
+                for (int k = 0; k < w; k++) pos += wh[k][d];
+                dst[pos] = key;
+            }
+            __syncthreads();
+            running[t] += wh[0][t] + wh[1][t] + wh[2][t] + wh[3][t];
+            __syncthreads();
+        }
+        __threadfence_block();
+        uint64_t* x = src; src = dst; dst = x;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ H-PLOC
+__global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, const uint64_t* __restrict__ keys,
+                            int n, int ib, Vec4* nodes, int* C, int* bounds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = (int)(keys[i] & ((1ull << ib) - 1ull));
+    const Vec4 l = lo[p], h = hi[p];
+    nodes[2 * i] = v4(xyz(l), asFloat(p));
+    nodes[2 * i + 1] = v4(xyz(h), asFloat(-1));
+    C[i] = i;
+    bounds[i] = -1;
+}
+
+// PLOC on the clusters of LBVH node [L, R] split at s, run by one half-wave (32 lanes; both halves call together, a
+// half with on = false idles). Its clusters sit at the start of each child's range in C (-1 = empty slot): up to 16
+// from each side, compacted into lanes in order. Each iteration: every lane finds the pair of smallest merged
+// half-area among the lanes within kR of it (pairs ordered by (area bits, lower lane, upper lane), so the global best
+// pair is mutual and every iteration merges); mutual pairs merge into a new node (lower lane = left child, id from the
+// counter in lane order); survivors are compacted in lane order. Stops at kT clusters (1 at the root); the survivors
+// are written back to the start of [L, R].
+__device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, int* C, int* counter, int* shId, float* shB) {
+    const int lane = (int)threadIdx.x & 63, hl = lane & 31, hb = lane & 32;
+    const uint64_t ltMask = (1ull << lane) - 1ull, halfMask = 0xffffffffull << hb;
+    int nL = 0, nR = 0;
+    if (on) { nL = min(s - L + 1, kT); nR = min(R - s, kT); }
+    const int slot = hl < nL ? L + hl : (hl < nL + nR ? s + 1 + (hl - nL) : -1);
+    int id = slot >= 0 ? C[slot] : -1;
+    const uint64_t vm = __ballot(id >= 0) & halfMask;
+    int num = (int)__builtin_popcountll(vm);
+    if (id >= 0) shId[__builtin_popcountll(vm & ltMask)] = id;
+    __syncthreads();
+    id = hl < num ? shId[hl] : -1;
+    __syncthreads();
+    Vec3 lo = v3(0.f, 0.f, 0.f), hi = lo;
+    if (id >= 0) { lo = xyz(nodes[2 * id]); hi = xyz(nodes[2 * id + 1]); }
+    const int target = root ? 1 : kT;
+    while (__ballot(num > target)) {
+        const bool act = num > target;
+        uint64_t best = ~0ull;
+        for (int r = 1; r <= kR; r++) {
+            const Vec3 olo = shfl3(lo, hb + min(hl + r, 31)), ohi = shfl3(hi, hb + min(hl + r, 31));
+            uint64_t key = ~0ull;
+            if (hl + r < num)
+                key = ((uint64_t)__float_as_uint(halfArea(vmn(lo, olo), vmx(hi, ohi))) << 32) | ((uint64_t)hl << 5) | (uint64_t)(hl + r);
+            if (key < best) best = key;
+            const uint64_t k2 = __shfl(key, hb + max(hl - r, 0));
+            if (hl >= r && k2 < best) best = k2;
+        }
+        const int pa = (int)((best >> 5) & 31u), pb = (int)(best & 31u);
+        const int nn = hl < num ? (pa == hl ? pb : pa) : hl;
+        const int nnn = __shfl(nn, hb + nn);
+        const bool mutual = act && hl < num && nn != hl && nnn == hl;
+        const bool lead = mutual && hl < nn;
+        const Vec3 plo = shfl3(lo, hb + nn), phi = shfl3(hi, hb + nn);
+        const int pid = __shfl(id, hb + nn);
+        const uint64_t lm = __ballot(lead) & halfMask;
+        int base = 0;
+        if (hl == 0 && lm) base = atomicAdd(counter, (int)__builtin_popcountll(lm));
+        base = __shfl(base, hb);
+        if (lead) {
+            const int nid = base + (int)__builtin_popcountll(lm & ltMask);
+            lo = vmn(lo, plo); hi = vmx(hi, phi);
+            nodes[2 * nid] = v4(lo, asFloat(id));
+            nodes[2 * nid + 1] = v4(hi, asFloat(pid));
+            id = nid;
+        }
+        const bool keep = act && hl < num && !(mutual && hl > nn);
+        const uint64_t km = __ballot(keep) & halfMask;
+        if (keep) {
+            const int dd = (int)__builtin_popcountll(km & ltMask);
+            shId[dd] = id;
+            float* b = shB + 6 * dd;
+            b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
+        }
+        __syncthreads();
+        if (act) {
+            num = (int)__builtin_popcountll(km);
+            if (hl < num) {
+                id = shId[hl];
+                const float* b = shB + 6 * hl;
+                lo = v3(b[0], b[1], b[2]); hi = v3(b[3], b[4], b[5]);
+            } else id = -1;
+        }
+        __syncthreads();
+    }
+    if (slot >= 0) C[slot] = -1;
+    __syncthreads();
+    if (on && hl < num) C[L + hl] = id;
+}
+
+// One lane per sorted primitive climbs the implicit LBVH: the parent of range [L, R] is [L, R'] (this range is the
+// left child, split at R) when L == 0 or the key delta at R is smaller than at L - 1, else [L', R] (split at L - 1).
+// The lane exchanges its range end into bounds[split]; the first lane to arrive stops, the second takes the union
+// range and, when it holds more than kT clusters or is the root, the wave runs the PLOC merge on it, two ranges at a
+// time. Fences order the cluster and node writes before the hand-over and the reads after it, across XCDs.
+__global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __restrict__ keys, int* C, int* bounds, int* counter, int n) {
+    __shared__ int shId[2][32];
+    __shared__ float shB[2][32 * 6];
+    const int lane = (int)threadIdx.x, half = lane >> 5;
+    const int i = blockIdx.x * 64 + lane;
+    bool active = i < n;
+    int L = i, R = i;
+    while (__ballot(active)) {
+        __threadfence();   // release: this wave's earlier merges before the hand-overs below
+        int s = 0;
+        bool mrg = false;
+        if (active) {
+            int prev;
+            if (L == 0 || (R != n - 1 && (keys[R] ^ keys[R + 1]) < (keys[L - 1] ^ keys[L]))) {
+                s = R; prev = atomicExch(&bounds[R], L);
+                if (prev != -1) R = prev;
+            } else {
+                s = L - 1; prev = atomicExch(&bounds[L - 1], R);
+                if (prev != -1) L = prev;
+            }
+            if (prev == -1) active = false;   // first to arrive: the sibling's lane continues
+            else mrg = (R - L + 1 > kT) || (L == 0 && R == n - 1);
+        }
+        __threadfence();   // acquire: the sibling subtree's clusters and nodes
+        uint64_t mm = __ballot(mrg);
+        while (mm) {
+            const int a = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            const int b = mm ? __builtin_ctzll(mm) : -1;
+            if (mm) mm &= mm - 1;
+            const int src = half == 0 ? a : b;
+            const int sl = src < 0 ? 0 : src;
+            const int l = __shfl(L, sl), r = __shfl(R, sl), sp = __shfl(s, sl);
+            plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, nodes, C, counter, shId[half], shB[half]);
+        }
+        if (active && L == 0 && R == n - 1) active = false;
+    }
+}
+
+__global__ void k_hp_to_bvh2(const Vec4* __restrict__ hn, int n, Vec4* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const float E = 3e37f;
+    if (n == 1) {
+        if (j == 0) {
+            const Vec4 a = hn[0], b = hn[1];
+            out[0] = v4(a.x, b.x, a.y, b.y);
+            out[1] = v4(E, E, E, E);
+            out[2] = v4(a.z, b.z, E, E);
+            out[3] = v4(a.w, asFloat(-1), asFloat(1), asFloat(0));
+        }
+        return;
+    }
+    if (j >= n - 1) return;
+    const int id = n + j;
+    const int ch[2] = { asInt(hn[2 * id].w), asInt(hn[2 * id + 1].w) };
+    Vec4 cl[2], chh[2];
+    int c[2], k[2];
+    for (int q = 0; q < 2; q++) {
+        const int x = ch[q];
+        cl[q] = hn[2 * x]; chh[q] = hn[2 * x + 1];
+        if (x < n) { c[q] = asInt(cl[q].w); k[q] = 1; } else { c[q] = x - n; k[q] = 0; }
+    }
+    out[4 * j] = v4(cl[0].x, chh[0].x, cl[0].y, chh[0].y);
+    out[4 * j + 1] = v4(cl[1].x, chh[1].x, cl[1].y, chh[1].y);
+    out[4 * j + 2] = v4(cl[0].z, chh[0].z, cl[1].z, chh[1].z);
+    out[4 * j + 3] = v4(asFloat(c[0]), asFloat(c[1]), asFloat(k[0]), asFloat(k[1]));
+}
+
+#define HCHK(x)                                                                                                        \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) { err = std::string(#x) + ": " + hipGetErrorString(e_); return false; }                  \
+    } while (0)
+
+}  // namespace
+
+HplocBuilder::~HplocBuilder() { release(); }
+
+void HplocBuilder::release() {
+    if (!cap_) return;
+    hipFree(nodes_); hipFree(keysA_); hipFree(keysB_); hipFree(clusters_); hipFree(bounds_); hipFree(counter_);
+    hipFree(cbox_); hipFree(counts_);
+    nodes_ = nullptr; keysA_ = keysB_ = nullptr; clusters_ = bounds_ = counter_ = counts_ = nullptr; cbox_ = nullptr;
+    cap_ = 0; tiles_ = 0;
+}
+
+bool HplocBuilder::reserve(int n, std::string& err) {
+    if (n <= cap_) return true;
+    if (n < 1 || n > (1 << 30)) { err = "H-PLOC: primitive count out of range"; return false; }
+    release();
+    tiles_ = (n + kTile - 1) / kTile;
+    HCHK(hipMalloc(&nodes_, (size_t)2 * (2 * (size_t)n - 1) * sizeof(Vec4)));
+    HCHK(hipMalloc(&keysA_, (size_t)n * 8));
+    HCHK(hipMalloc(&keysB_, (size_t)n * 8));
+    HCHK(hipMalloc(&clusters_, (size_t)n * 4));
+    HCHK(hipMalloc(&bounds_, (size_t)n * 4));
+    HCHK(hipMalloc(&counter_, 4));
+    HCHK(hipMalloc(&cbox_, 24));
+    HCHK(hipMalloc(&counts_, (size_t)256 * tiles_ * 4));
+    cap_ = n;
+    return true;
+}
+
+bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err) {
+    if (n < 1) { err = "H-PLOC: no primitives"; return false; }
+    if (!lo || !hi) { err = "H-PLOC: null box arrays"; return false; }
+    if (!reserve(n, err)) return false;
+    hipStream_t st = (hipStream_t)stream;
+    const int g = (n + 255) / 256;
+    k_hp_init<<<1, 64, 0, st>>>(cbox_, counter_, n);
+    k_hp_bounds<<<std::min(g, 2048), 256, 0, st>>>(lo, hi, n, cbox_);
+    int ib = 1;
+    while ((1ll << ib) < n) ib++;
+    const int mb = (64 - ib) / 3 < 21 ? (64 - ib) / 3 : 21;
+    k_hp_keys<<<g, 256, 0, st>>>(lo, hi, n, cbox_, keysA_, mb, ib);
+    const int passes = (3 * mb + ib + 7) / 8;
+    const uint64_t* sorted = keysA_;
+    if (n <= kSmallSort) {
+        k_rs_single<<<1, kSortBlock, 0, st>>>(keysA_, keysB_, n, passes);
+        sorted = (passes & 1) ? keysB_ : keysA_;
+    } else {
+        const int tiles = (n + kTile - 1) / kTile;
+        uint64_t *src = keysA_, *dst = keysB_;
+        for (int p = 0; p < passes; p++) {
+            k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, 8 * p, counts_, tiles);
+            k_rs_scan<<<1, 1024, 0, st>>>(counts_, 256 * tiles);
+            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, 8 * p, counts_, tiles);
+            std::swap(src, dst);
+        }
+        sorted = src;
+    }
+    k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, bounds_);
+    if (n > 1) k_hploc<<<(n + 63) / 64, 64, 0, st>>>(nodes_, sorted, clusters_, bounds_, counter_, n);
+    HCHK(hipGetLastError());
+    root = n > 1 ? 2 * n - 2 : 0;
+    return true;
+}
+
+bool hplocToBVH2(const Vec4* hnodes, int n, Vec4* out, void* stream, std::string& err) {
+    if (n < 1 || !hnodes || !out) { err = "hplocToBVH2: bad arguments"; return false; }
+    const int m = n > 1 ? n - 1 : 1;
+    k_hp_to_bvh2<<<(m + 127) / 128, 128, 0, (hipStream_t)stream>>>(hnodes, n, out);
+    HCHK(hipGetLastError());
+    return true;
+}
+
+bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err) {
+    if (n < 1) { err = "H-PLOC: no primitives"; return false; }
+    Vec4 *dLo = nullptr, *dHi = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HplocBuilder B;
+    bool ok = false;
+    do {
+        if (hipMalloc(&dLo, (size_t)n * 16) != hipSuccess || hipMalloc(&dHi, (size_t)n * 16) != hipSuccess) { err = "H-PLOC: out of device memory"; break; }
+        if (hipMemcpy(dLo, lo, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dHi, hi, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess) { err = "H-PLOC: upload failed"; break; }
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { err = "H-PLOC: event creation failed"; break; }
+        if (!B.build(dLo, dHi, n, nullptr, root, err)) break;   // first build: loads the kernels (not timed)
+        hipEventRecord(e0, nullptr);
+        if (!B.build(dLo, dHi, n, nullptr, root, err)) break;
+        hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) { err = "H-PLOC: build failed"; break; }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        gpuMs = ms;
+        if (hipMemcpy(nodesOut, B.nodes(), (size_t)2 * (2 * (size_t)n - 1) * 16, hipMemcpyDeviceToHost) != hipSuccess) {
+            err = "H-PLOC: download failed"; break;
+        }
+        ok = true;
+    } while (false);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (dLo) hipFree(dLo);
+    if (dHi) hipFree(dHi);
+    return ok;
+}
+
+}  // namespace cwp

@@ -20,8 +20,11 @@ struct RenderOptions {
     int bounces = 8;        // maximum bounce depth (camera hit = depth 0)
     int stackLds = 16;      // LDS traversal-stack entries per lane (8, 16 or 32); deeper entries spill to HBM
     int blocksPerCU = 0;    // persistent grid: workgroups per CU (0 = occupancy limit)
-    int bounceWaves = 6;    // trace+shade kernel: register budget for at least this many waves per SIMD (1, 6 or 8)
+    int bounceWaves = 8;    // trace+shade kernel: register budget for at least this many waves per SIMD (1, 6 or 8)
     int timeKernels = 1;    // per-launch hipEvent timing of the trace/shadow kernels
+    int fuse = 1;           // 1: trace and shade in one kernel; 0: a closest-hit trace kernel, then a shade kernel
+    int persistent = 0;     // 1: split kernels whose waves refill idle lanes with new rays mid-traversal (implies fuse 0)
+    int shadowRefill = 0;   // shadow kernel: 1 = refill idle lanes mid-traversal, 0 = one ray per lane per round
 };
 
 struct FrameStats {
@@ -31,6 +34,7 @@ struct FrameStats {
     int bounceLaunches = 0, shadowLaunches = 0;
     double hitRays = 0, contRays = 0, missRays = 0, unoccluded = 0;   // event counters (bytes-per-ray accounting)
     double cameraMiss = 0;                                             // camera rays that miss
+    double shadeMs = 0;                  // split kernels: the shade kernels' part of bounceMs
 };
 
 // Scene data ready for upload, in BVH leaf order.
@@ -74,5 +78,11 @@ private:
 
 // Number of compute units and device name of device 0 (false if no HIP device)
 bool deviceInfo(int& numCU, std::string& name, std::string& err);
+
+// Plain device memory for callers without a GPU array library: allocate, free, copy (kind 1 host->device,
+// 2 device->host, 3 device->device; synchronous).
+bool deviceMalloc(size_t bytes, void** ptr, std::string& err);
+void deviceFree(void* ptr);
+bool deviceCopy(void* dst, const void* src, size_t bytes, int kind, std::string& err);
 
 }  // namespace cwp

@@ -22,6 +22,7 @@
 #include "core/sampling.h"
 #include "device/renderer.h"
 #include "device/traverse.h"
+#include "host/scene_check.h"
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -32,11 +33,14 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 64;       // one wave per workgroup: LDS stacks are sized per wave
+static_assert(kBlock == kStackStride, "the LDS stack is strided by the workgroup size");
 constexpr int kParts = 8;        // queue partitions, one per XCD (workgroup w runs on XCD w % 8)
-// Per-depth counters (kCntStride ints): [x] path-queue length of partition x, [kParts + x] shadow rays produced by
-// partition x, [2 kParts] misses, [2 kParts + 1] unoccluded shadow rays.
-constexpr int kCntStride = 32;
-constexpr int kCntMiss = 2 * kParts, kCntUnocc = 2 * kParts + 1;
+// Per-depth counters: one 128-byte line per partition, so the atomics of different partitions never meet on a line.
+// Line x: [kCQueue] path-queue length of partition x, [kCShadow] shadow rays it produced, [kCMiss] its misses,
+// [kCUnocc] its unoccluded shadow rays. Misses and unoccluded rays are summed per wave and added once at kernel end.
+constexpr int kLine = 32;
+constexpr int kCntStride = kParts * kLine;
+constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
 
 struct DevScene {
     const Vec4* nodes;    // BVH layout (BVH8: 128-byte node stride)
@@ -51,9 +55,10 @@ struct DevScene {
 // partition's slice of the id space is a band of the image.
 struct DevPaths {
     Vec4 *O, *D, *T, *L;  // origin, direction, throughput, radiance per path id
+    Vec4* H;              // closest hit of the current bounce (t, u, v, triangle bits) when tracing and shading are split
     int* q[2];            // ping-pong queues of path ids: depth d reads q[d & 1]
     Vec4 *sO, *sD, *sC;   // shadow rays of the current bounce: (origin, path id bits), (dir, tmax), (contribution)
-    int* cnt;             // kCntStride counters per depth
+    int* cnt;             // kCntStride counters per depth (one line per partition)
     void* spill;          // traversal-stack spill slab, spillCap entries per thread
     int spillCap;
 };
@@ -81,14 +86,19 @@ __device__ __forceinline__ int waveAppend(uint64_t mask, int* counter) {
     return base + rank;
 }
 
-__device__ __forceinline__ void waveCount(uint64_t mask, int* counter) {
-    if (mask && laneId() == __builtin_ctzll(mask)) atomicAdd(counter, (int)__builtin_popcountll(mask));
+// adds a wave-uniform count to *counter once per wave (nothing when it is zero)
+__device__ __forceinline__ void waveAdd(int count, int* counter) {
+    if (count && laneId() == 0) atomicAdd(counter, count);
 }
 
-// start of partition `part` in a region laid out as partitions of the given lengths
-__device__ __forceinline__ int partStart(const int* len, int part) {
+// counter `which` of partition `part` in a depth's counter block
+__device__ __forceinline__ int* ctr(int* cnt, int part, int which) { return cnt + part * kLine + which; }
+__device__ __forceinline__ int ctrv(const int* cnt, int part, int which) { return cnt[part * kLine + which]; }
+
+// start of partition `part` in a region laid out as the partitions' queue lengths of a depth's counter block
+__device__ __forceinline__ int partStart(const int* cnt, int part) {
     int s = 0;
-    for (int y = 0; y < part; y++) s += len[y];
+    for (int y = 0; y < part; y++) s += ctrv(cnt, y, kCQueue);
     return s;
 }
 
@@ -115,11 +125,84 @@ __device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_
 }
 
 // ------------------------------------------------------------------------------------------------------- kernels
-// One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here).
-// The queue is split into kParts partitions, one per XCD; the waves of XCD x walk partition x, and append their
-// continuations to partition x of the next queue and their shadow rays to partition x of the shadow queue, so a
-// path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue starts where the
-// depth-d partitions before x end, which bounds it by the rays partition x traced.
+// Legacy shading of one closest hit of path i (reference: shadeOne, src/render/wavefront.cu:94-136). Writes the
+// path's radiance on a miss (depth 0 also initialises it on a hit) and returns the continuation (cont: po, nd, Tn) and
+// the shadow ray (shadow: po, sdir, scon) for the caller's wave-level appends.
+struct ShadeOut { bool cont, shadow, miss; Vec3 po, nd, Tn, sdir, scon; };
+
+__device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
+                                                int i, int pix, uint32_t smp, Vec3 o, Vec3 d, Vec3 T, const Hit& h) {
+    ShadeOut out;
+    out.cont = false; out.shadow = false; out.miss = false;
+    out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po;
+    if (h.tri < 0) {
+        out.miss = true;
+        Vec3 c = T * F.sky;
+        if (depth == 0) P.L[i] = v4(c, 0.f);
+        else { Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l; }
+        return out;
+    }
+    if (depth == 0) P.L[i] = v4(0.f, 0.f, 0.f, 0.f);
+    if (depth > F.maxB) return out;
+    const Vec3 p = o + d * h.t;
+    const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
+    const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
+    Vec3 ng = normalize(cross(e1, e2)), ns = ng;
+    if (F.hasN) {
+        Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
+        Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
+        float l = length(nn);
+        if (l > 0.f) ns = nn * (1.f / l);
+        if (dot(ns, ng) < 0.f) ns = -ns;
+    }
+    if (dot(ng, d) > 0.f) { ng = -ng; ns = -ns; }
+    const Vec3 alb = xyz(S.albedo[asInt(b.w)]);
+    out.po = p + ng * F.eps;
+    const uint32_t dd = (uint32_t)depth + 1u;
+    const Vec3 l = uniformCone(F.sunTo, F.sunCos, rnd((uint32_t)pix, smp, dd, 2, F.seed), rnd((uint32_t)pix, smp, dd, 3, F.seed));
+    const float cl = dot(ns, l);
+    if (cl > 0.f && dot(ng, l) > 0.f) { out.shadow = true; out.sdir = l; out.scon = T * alb * F.sunE * (cl * kInvPi); }
+    out.nd = cosineHemisphere(ns, rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed));
+    if (dot(out.nd, ng) > 0.f) { out.cont = true; out.Tn = T * alb; }
+    return out;
+}
+
+// Wave-level appends of the shading results: continuations to partition `part` of the next queue, shadow rays to
+// partition `part` of the shadow queue. Returns the wave's number of misses.
+__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int* qOut, int i,
+                                             const ShadeOut& r) {
+    const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
+    if (mc) {
+        const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
+        if (r.cont) { qOut[pos] = i; P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, 0.f); P.T[i] = v4(r.Tn, 0.f); }
+    }
+    if (ms) {
+        const int pos = outStart + waveAppend(ms, ctr(cnt, part, kCShadow));
+        if (r.shadow) { P.sO[pos] = v4(r.po, asFloat(i)); P.sD[pos] = v4(r.sdir, 1e30f); P.sC[pos] = v4(r.scon, 0.f); }
+    }
+    return (int)__builtin_popcountll(mm);
+}
+
+// The ray of path i at this depth: its camera ray at depth 0 (recomputed from the path id), else the stored state.
+__device__ __forceinline__ void pathRay(const DevPaths& P, const FrameParams& F, int depth, int i, int smpBase, int K,
+                                        Vec3& o, Vec3& d, float& tmin) {
+    if (depth == 0) { cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
+    else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+}
+
+// Queue position of the f-th entry a wave hands out: the wave walks the chunks lw, lw + nw, lw + 2 nw, ... of its
+// partition, 64 entries each (increasing in f).
+__device__ __forceinline__ int fetchPos(uint32_t f, int lw, int nw) { return (lw + (int)(f >> 6) * nw) * kWave + (int)(f & 63u); }
+
+// idle lanes of a persistent wave are refilled once at least this many are idle (kWave: only when all are, which is
+// the plain one-ray-per-lane schedule)
+constexpr int kRefill = 16;
+
+// One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here),
+// traversal and shading fused. The queue is split into kParts partitions, one per XCD; the waves of XCD x walk
+// partition x, and append their continuations to partition x of the next queue and their shadow rays to partition x of
+// the shadow queue, so a path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue
+// starts where the depth-d partitions before x end, which bounds it by the rays partition x traced.
 // WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
 template <int V, int TRI, int SL, int WAVES>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
@@ -127,78 +210,113 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int lane = laneId();
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ lds + threadIdx.x, kBlock, (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int n = cnt[part];
+    const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
     const int* qIn = P.q[depth & 1];
     int* qOut = P.q[(depth + 1) & 1];
+    int misses = 0;
     for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + lane;
-        const bool act = k < n;
-        bool cont = false, shadow = false, miss = false;
         int i = 0;
-        Vec3 po = v3(0.f, 0.f, 0.f), nd = po, Tn = po, sdir = po, scon = po;
-        if (act) {
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
             i = depth == 0 ? inStart + k : qIn[inStart + k];
-            Vec3 o, d;
-            float tmin;
-            if (depth == 0) { cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
-            else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+            Vec3 o, d; float tmin;
+            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
             const RayPre r = prepareRay<TRI>(o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
             st.sp = 0;
             traceScene<V, false, TRI>(S, r, h, st);
-            // throughput, pixel and sample are fetched / recomputed after the traversal so that no registers hold
-            // them across it
+            // throughput is fetched after the traversal so that no registers hold it across the traversal loop
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
-            const int pix = i / K;
-            const uint32_t smp = (uint32_t)(smpBase + i % K);
-            o = r.o; d = r.d;
-            if (h.tri < 0) {
-                miss = true;
-                Vec3 c = T * F.sky;
-                if (depth == 0) P.L[i] = v4(c, 0.f);
-                else { Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l; }
-            } else {
-                if (depth == 0) P.L[i] = v4(0.f, 0.f, 0.f, 0.f);
-                if (depth <= F.maxB) {
-                    const Vec3 p = o + d * h.t;
-                    const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
-                    const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
-                    Vec3 ng = normalize(cross(e1, e2)), ns = ng;
-                    if (F.hasN) {
-                        Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
-                        Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
-                        float l = length(nn);
-                        if (l > 0.f) ns = nn * (1.f / l);
-                        if (dot(ns, ng) < 0.f) ns = -ns;
-                    }
-                    if (dot(ng, d) > 0.f) { ng = -ng; ns = -ns; }
-                    const Vec3 alb = xyz(S.albedo[asInt(b.w)]);
-                    po = p + ng * F.eps;
-                    const uint32_t dd = (uint32_t)depth + 1u;
-                    const Vec3 l = uniformCone(F.sunTo, F.sunCos, rnd((uint32_t)pix, smp, dd, 2, F.seed), rnd((uint32_t)pix, smp, dd, 3, F.seed));
-                    const float cl = dot(ns, l);
-                    if (cl > 0.f && dot(ng, l) > 0.f) { shadow = true; sdir = l; scon = T * alb * F.sunE * (cl * kInvPi); }
-                    nd = cosineHemisphere(ns, rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed));
-                    if (dot(nd, ng) > 0.f) { cont = true; Tn = T * alb; }
+            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// Closest hits of the depth-`depth` queue with persistent lanes and dynamic ray fetching (after Aila & Laine 2009;
+// reference: persistentTrace, src/render/wavefront.cu:403-434): a lane whose ray finishes stays idle until at least
+// REFILL lanes of its wave are idle, then they all take the next entries of the wave's share of its partition, so
+// long rays do not leave the rest of the wave empty. Hits go to H[path id] for k_shade.
+template <int V, int TRI, int SL, int REFILL>
+__global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    const int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int* qIn = P.q[depth & 1];
+    int i = -1;                       // path id of this lane's ray, -1 = idle
+    RayPre r; Hit h; TravState ts;
+    uint32_t f = 0;                   // wave-uniform: entries of the wave's share handed out so far
+    bool more = fetchPos(0, lw, nw) < n;
+    for (;;) {
+        const uint64_t idle = __ballot(i < 0);
+        if (more && (int)__builtin_popcountll(idle) >= REFILL) {
+            if (i < 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int k = fetchPos(f + rank, lw, nw);
+                if (k < n) {
+                    i = depth == 0 ? inStart + k : qIn[inStart + k];
+                    Vec3 o, d; float tmin;
+                    pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+                    r = prepareRay<TRI>(o, d, tmin);
+                    h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
+                    travBegin(ts, S.root);
+                    st.sp = 0;
                 }
             }
+            f += (uint32_t)__builtin_popcountll(idle);
+            more = fetchPos(f, lw, nw) < n;
         }
-        const uint64_t mc = __ballot(cont), ms = __ballot(shadow), mm = __ballot(miss);
-        if (mc) {
-            const int pos = outStart + waveAppend(mc, cnt + kCntStride + part);
-            if (cont) { qOut[pos] = i; P.O[i] = v4(po, 0.f); P.D[i] = v4(nd, 0.f); P.T[i] = v4(Tn, 0.f); }
+        if (__ballot(i >= 0) == 0) break;   // no ray in flight: with `more` false the wave's share is done
+        if (i >= 0) {
+            bool done;
+            if constexpr (V == 2) done = stepBVH2<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else done = stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            if (done) { P.H[i] = v4(h.t, h.u, h.v, asFloat(h.tri)); i = -1; }
         }
-        if (ms) {
-            const int pos = outStart + waveAppend(ms, cnt + kParts + part);
-            if (shadow) { P.sO[pos] = v4(po, asFloat(i)); P.sD[pos] = v4(sdir, 1e30f); P.sC[pos] = v4(scon, 0.f); }
-        }
-        waveCount(mm, cnt + kCntMiss);
     }
+}
+
+// Shading of the depth-`depth` queue from the hits of k_trace_paths (no traversal: every lane of a wave shades).
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+    const int lane = laneId();
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const int* qIn = P.q[depth & 1];
+    int* qOut = P.q[(depth + 1) & 1];
+    int misses = 0;
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
+        const int k = base + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            Vec3 o, d; float tmin;
+            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+            const Vec4 hv = P.H[i];
+            Hit h; h.t = hv.x; h.u = hv.y; h.v = hv.z; h.tri = asInt(hv.w);
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
 }
 
 // Shadow rays of bounce `depth` (partition per XCD as in k_bounce): any hit; unoccluded contributions are added to
@@ -208,11 +326,12 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, in
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ lds + threadIdx.x, kBlock, (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int n = cnt[kParts + part];
+    const int n = ctrv(cnt, part, kCShadow);
     const int start = partStart(cnt, part);
+    int unocc = 0;
     for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + laneId();
         bool vis = false;
@@ -228,8 +347,65 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, in
                 Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
             }
         }
-        waveCount(__ballot(vis), cnt + kCntUnocc);
+        unocc += (int)__builtin_popcountll(__ballot(vis));
     }
+    waveAdd(unocc, ctr(cnt, part, kCUnocc));
+}
+
+// Shadow rays with persistent lanes and dynamic ray fetching (as k_trace_paths).
+template <int V, int TRI, int SL, int REFILL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_refill(DevScene S, DevPaths P, int depth) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCShadow);
+    const int start = partStart(cnt, part);
+    int k = -1;                       // shadow-queue entry of this lane's ray, -1 = idle
+    RayPre r; Hit h; TravState ts;
+    uint32_t f = 0;
+    bool more = fetchPos(0, lw, nw) < n;
+    int unocc = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(k < 0);
+        if (more && (int)__builtin_popcountll(idle) >= REFILL) {
+            if (k < 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int kk = fetchPos(f + rank, lw, nw);
+                if (kk < n) {
+                    k = kk;
+                    const Vec4 O = P.sO[start + k], D = P.sD[start + k];
+                    r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
+                    h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
+                    travBegin(ts, S.root);
+                    st.sp = 0;
+                }
+            }
+            f += (uint32_t)__builtin_popcountll(idle);
+            more = fetchPos(f, lw, nw) < n;
+        }
+        if (__ballot(k >= 0) == 0) break;
+        if (k >= 0) {
+            bool done;
+            if constexpr (V == 2) done = stepBVH2<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else done = stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            if (done) {
+                if (h.tri < 0) {
+                    unocc++;
+                    const int i = asInt(P.sO[start + k].w);
+                    const Vec4 c = P.sC[start + k];
+                    Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+                }
+                k = -1;
+            }
+        }
+    }
+    // one atomic per wave for the unoccluded count
+    int sum = unocc;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    waveAdd(sum, ctr(cnt, part, kCUnocc));
 }
 
 // Closest or any hit of explicit rays (parity and kernel-level entry point).
@@ -238,7 +414,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace(DevScene S, const Vec4* __r
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ lds + threadIdx.x, kBlock, (E*)spill + (size_t)gtid * spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)spill + (size_t)gtid * spillCap, 0 };
     for (int k = gtid; k - (int)threadIdx.x < n; k += gridDim.x * kBlock) {
         if (k >= n) continue;
         const Vec4 O = rays[2 * (size_t)k], D = rays[2 * (size_t)k + 1];
@@ -256,7 +432,7 @@ __global__ void k_pass_begin(int* cnt, int words, long long NK) {
     __syncthreads();
     if (threadIdx.x < kParts) {
         const long long x = threadIdx.x;
-        cnt[x] = (int)((x + 1) * NK / kParts - x * NK / kParts);
+        *ctr(cnt, (int)x, kCQueue) = (int)((x + 1) * NK / kParts - x * NK / kParts);
     }
 }
 
@@ -264,13 +440,16 @@ __global__ void k_pass_begin(int* cnt, int words, long long NK) {
 // [5] camera-ray misses
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot) {
     if (threadIdx.x != 0) return;
-    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = (unsigned)cnt[kCntMiss];
+    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = 0;
+    for (int x = 0; x < kParts; x++) m0 += (unsigned)ctrv(cnt, x, kCMiss);
     for (int d = 0; d < depths; d++) {
         const int* cd = cnt + d * kCntStride;
         unsigned long long q = 0;
-        for (int x = 0; x < kParts; x++) { q += (unsigned)cd[x]; b += (unsigned)cd[kParts + x]; }
+        for (int x = 0; x < kParts; x++) {
+            q += (unsigned)ctrv(cd, x, kCQueue); b += (unsigned)ctrv(cd, x, kCShadow);
+            m += (unsigned)ctrv(cd, x, kCMiss); u += (unsigned)ctrv(cd, x, kCUnocc);
+        }
         a += q; if (d > 0) c += q;
-        m += (unsigned)cd[kCntMiss]; u += (unsigned)cd[kCntUnocc];
     }
     tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
 }
@@ -294,25 +473,35 @@ using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int);
 using ShadowFn = void (*)(DevScene, DevPaths, int);
 using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, void*, int);
 
+// the kernels of one (layout, triangle test, LDS stack) configuration
+struct Kernels {
+    BounceFn bounce = nullptr;                          // fused trace + shade
+    BounceFn tracePaths = nullptr, tracePlain = nullptr; // split: closest-hit trace with / without refill (then k_shade)
+    ShadowFn shadow = nullptr, shadowRefill = nullptr;
+    TraceFn traceC = nullptr, traceA = nullptr;
+};
+
 template <int V, int TRI, int SL>
-BounceFn pickBounce(int waves) {
-    if (waves >= 8) return k_bounce<V, TRI, SL, 8>;
-    if (waves >= 6) return k_bounce<V, TRI, SL, 6>;
-    return k_bounce<V, TRI, SL, 1>;
+void pickSL(int waves, Kernels& k) {
+    k.bounce = waves >= 8 ? k_bounce<V, TRI, SL, 8> : (waves >= 6 ? k_bounce<V, TRI, SL, 6> : k_bounce<V, TRI, SL, 1>);
+    k.tracePaths = k_trace_paths<V, TRI, SL, kRefill>;
+    k.tracePlain = k_trace_paths<V, TRI, SL, kWave>;
+    k.shadow = k_shadow<V, TRI, SL>;
+    k.shadowRefill = k_shadow_refill<V, TRI, SL, kRefill>;
+    k.traceC = k_trace<V, TRI, false, SL>;
+    k.traceA = k_trace<V, TRI, true, SL>;
 }
 template <int V, int TRI>
-void pickBySL(int sl, int waves, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
-    switch (sl) {
-        case 8: b = pickBounce<V, TRI, 8>(waves); s = k_shadow<V, TRI, 8>; tc = k_trace<V, TRI, false, 8>; ta = k_trace<V, TRI, true, 8>; break;
-        case 32: b = pickBounce<V, TRI, 32>(waves); s = k_shadow<V, TRI, 32>; tc = k_trace<V, TRI, false, 32>; ta = k_trace<V, TRI, true, 32>; break;
-        default: b = pickBounce<V, TRI, 16>(waves); s = k_shadow<V, TRI, 16>; tc = k_trace<V, TRI, false, 16>; ta = k_trace<V, TRI, true, 16>; break;
-    }
+void pickTri(int sl, int waves, Kernels& k) {
+    if (sl == 8) pickSL<V, TRI, 8>(waves, k);
+    else if (sl == 32) pickSL<V, TRI, 32>(waves, k);
+    else pickSL<V, TRI, 16>(waves, k);
 }
 template <int V>
-void pickByTri(int tri, int sl, int waves, BounceFn& b, ShadowFn& s, TraceFn& tc, TraceFn& ta) {
-    if (tri == kTriMT) pickBySL<V, kTriMT>(sl, waves, b, s, tc, ta);
-    else if (tri == kTriWatertight) pickBySL<V, kTriWatertight>(sl, waves, b, s, tc, ta);
-    else pickBySL<V, kTriWoop>(sl, waves, b, s, tc, ta);
+void pickKernels(int tri, int sl, int waves, Kernels& k) {
+    if (tri == kTriMT) pickTri<V, kTriMT>(sl, waves, k);
+    else if (tri == kTriWatertight) pickTri<V, kTriWatertight>(sl, waves, k);
+    else pickTri<V, kTriWoop>(sl, waves, k);
 }
 
 #define HIPCHK(x)                                                                                                      \
@@ -336,8 +525,10 @@ struct Renderer::Impl {
     unsigned long long* tot = nullptr;
     unsigned long long* hostTot = nullptr;
     std::vector<void*> allocs;
-    BounceFn bounce = nullptr; ShadowFn shadow = nullptr; TraceFn traceC = nullptr, traceA = nullptr;
-    int gridBounce = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0;
+    Kernels kn;
+    BounceFn bounce = nullptr, shade = nullptr;   // fused: bounce; split: bounce = trace kernel, then shade
+    ShadowFn shadow = nullptr;
+    int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0;
     void* stageRays = nullptr; void* stageHits = nullptr; size_t stageCap = 0;   // traceHost staging buffers
 
     template <class T> bool alloc(T*& p, size_t count, std::string& err) {
@@ -393,13 +584,30 @@ bool deviceInfo(int& numCU, std::string& name, std::string& err) {
     return true;
 }
 
+bool deviceMalloc(size_t bytes, void** ptr, std::string& err) {
+    *ptr = nullptr;
+    HIPCHK(hipMalloc(ptr, std::max<size_t>(bytes, 16)));
+    return true;
+}
+void deviceFree(void* ptr) { if (ptr) hipFree(ptr); }
+bool deviceCopy(void* dst, const void* src, size_t bytes, int kind, std::string& err) {
+    if (!bytes) return true;
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIPCHK(hipMemcpy(dst, src, bytes, k));
+    return true;
+}
+
 bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOptions& O, std::string& err) {
     Impl& I = *p_;
-    if (S.variant != 2 && S.variant != 8) { err = "variant must be 2 or 8"; return false; }
+    // the kernels trust every index in the scene: it is checked on the host first
+    int stackNeed = 0;
+    if (!checkScene(S, stackNeed, err)) return false;
     if (O.tri < kTriMT || O.tri > kTriWatertight) { err = "unknown triangle test"; return false; }
     if (O.tri == kTriWoop && S.woop.size() != S.verts.size()) { err = "Woop rows missing"; return false; }
     if (F.cam.w <= 0 || F.cam.h <= 0) { err = "image size must be positive"; return false; }
-    if (S.verts.empty() || S.nodes.empty()) { err = "empty scene"; return false; }
+    if (O.stackLds != 8 && O.stackLds != 16 && O.stackLds != 32) { err = "stack_lds must be 8, 16 or 32"; return false; }
+    if (O.bounceWaves != 1 && O.bounceWaves != 6 && O.bounceWaves != 8) { err = "bounce_waves must be 1, 6 or 8"; return false; }
+    if (O.bounces < 0 || O.bounces > 64) { err = "bounces must be in [0, 64]"; return false; }
     I.O = O; I.F = F;
     I.sl = O.stackLds == 8 ? 8 : (O.stackLds == 32 ? 32 : 16);
     I.O.stackLds = I.sl;
@@ -426,8 +634,13 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.S.tri = O.tri == kTriWoop ? dw : dv;
     if (F.hasN && !dnr) { err = "hasN set without normals"; return false; }
 
-    if (S.variant == 2) pickByTri<2>(O.tri, I.sl, O.bounceWaves, I.bounce, I.shadow, I.traceC, I.traceA);
-    else pickByTri<8>(O.tri, I.sl, O.bounceWaves, I.bounce, I.shadow, I.traceC, I.traceA);
+    if (O.fuse != 0 && O.fuse != 1) { err = "fuse must be 0 or 1"; return false; }
+    if (O.persistent && O.fuse) { err = "persistent traversal needs the split kernels (fuse 0)"; return false; }
+    if (S.variant == 2) pickKernels<2>(O.tri, I.sl, O.bounceWaves, I.kn);
+    else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
+    I.bounce = O.fuse ? I.kn.bounce : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
+    I.shade = O.fuse ? nullptr : k_shade;
+    I.shadow = (O.persistent || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
     auto gridOf = [&](const void* fn) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kBlock, 0) != hipSuccess || b < 1) b = 1;
@@ -435,13 +648,14 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         return std::max(kParts, (b * numCU) / kParts * kParts);   // a whole number of waves per partition
     };
     I.gridBounce = gridOf((const void*)I.bounce);
+    I.gridShade = I.shade ? gridOf((const void*)I.shade) : 0;
     I.gridShadow = gridOf((const void*)I.shadow);
-    I.gridTrace = std::max(gridOf((const void*)I.traceC), gridOf((const void*)I.traceA));
+    I.gridTrace = std::max(gridOf((const void*)I.kn.traceC), gridOf((const void*)I.kn.traceA));
     I.maxGrid = std::max(I.gridBounce, std::max(I.gridShadow, I.gridTrace));
     gridBlocks = I.gridBounce;
 
     // stack spill beyond the LDS part: the tree bounds the entries a traversal can hold
-    I.P.spillCap = S.stackNeed > I.sl ? S.stackNeed - I.sl : 0;
+    I.P.spillCap = stackNeed > I.sl ? stackNeed - I.sl : 0;
     if (I.P.spillCap > 0) {
         uint64_t* sp;
         if (!I.alloc(sp, (size_t)I.maxGrid * kBlock * I.P.spillCap, err)) return false;
@@ -459,6 +673,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         !I.alloc(I.P.sD, NK, err) || !I.alloc(I.P.sC, NK, err) || !I.alloc(I.P.cnt, (size_t)(I.depths + 1) * kCntStride, err) ||
         !I.alloc(I.accum, (size_t)I.N, err) || !I.alloc(I.tot, 8, err))
         return false;
+    if (!O.fuse && !I.alloc(I.P.H, NK, err)) return false;
     HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     if (!I.fill(I.accum, (size_t)I.N * 16, err)) return false;
     HIPCHK(hipStreamSynchronize(I.st));
@@ -481,7 +696,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     if (sBase < 0) { err = "sample base must be >= 0"; return false; }
     S = FrameStats();
     I.evUsed = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadowEv;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv;
     if (clear && !I.fill(I.accum, (size_t)I.N * 16, err)) return false;
     if (!I.fill(I.tot, 8 * sizeof(unsigned long long), err)) return false;
     HIPCHK(hipEventRecord(I.f0, I.st));
@@ -492,7 +707,14 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
             hipEvent_t a = nullptr, b = nullptr;
             if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
             I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.K);
-            if (I.O.timeKernels) { HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b }); }
+            if (I.shade) {
+                if (I.O.timeKernels) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
+                I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.K);
+            }
+            if (I.O.timeKernels) {
+                HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });
+                if (I.shade) shadeEv.back().second = b;
+            }
             if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
             I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
             if (I.O.timeKernels) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
@@ -510,6 +732,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     S.frameMs = ms;
     for (auto& e : bounceEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.bounceMs += ms; S.bounceLaunches++; }
     for (auto& e : shadowEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadowMs += ms; S.shadowLaunches++; }
+    for (auto& e : shadeEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadeMs += ms; }
     S.traceMs = I.O.timeKernels ? S.bounceMs + S.shadowMs : S.frameMs;
     S.rays = (double)I.hostTot[0]; S.shadowRays = (double)I.hostTot[1];
     S.missRays = (double)I.hostTot[2]; S.unoccluded = (double)I.hostTot[3]; S.contRays = (double)I.hostTot[4];
@@ -528,7 +751,7 @@ bool Renderer::trace(const void* rays, int n, bool anyHit, void* hits, std::stri
     Impl& I = *p_;
     if (n < 0) { err = "negative ray count"; return false; }
     if (n == 0) return true;
-    TraceFn fn = anyHit ? I.traceA : I.traceC;
+    TraceFn fn = anyHit ? I.kn.traceA : I.kn.traceC;
     fn<<<I.gridTrace, kBlock, 0, I.st>>>(I.S, (const Vec4*)rays, n, (Vec4*)hits, I.P.spill, I.P.spillCap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(I.st));

@@ -123,133 +123,166 @@ __device__ __forceinline__ bool triTest(const Vec4* __restrict__ tri, int t, con
 }
 
 // ------------------------------------------------------------------------------------------------ traversal stack
+// LDS pointers are typed as address space 3, so every stack access is a ds_read/ds_write (never a flat access that
+// also waits on vector memory), whatever the optimiser does with the loop around it.
+template <class T> using LdsPtr = __attribute__((address_space(3))) T*;
+
+// Per-lane traversal stack: the first SL entries in LDS, lane-minor (entry j of lane l at lds[j * kStackStride + l],
+// conflict-free), deeper entries in the lane's HBM spill area.
+constexpr int kStackStride = 64;   // lanes per workgroup: one wave
 template <typename E, int SL>
 struct LaneStack {
-    E* lds; int stride; E* spill; int sp;
+    LdsPtr<E> lds; E* spill; int sp;
     __device__ __forceinline__ void push(E e) {
-        if (SL > 0 && sp < SL) lds[sp * stride] = e; else spill[sp - SL] = e;
+        if (SL > 0 && sp < SL) lds[sp * kStackStride] = e; else spill[sp - SL] = e;
         sp++;
     }
     __device__ __forceinline__ E pop() {
         sp--;
-        return (SL > 0 && sp < SL) ? lds[sp * stride] : spill[sp - SL];
+        if (SL > 0 && sp < SL) return lds[sp * kStackStride];
+        return spill[sp - SL];
     }
 };
 
-struct Group { uint32_t base, bits; };   // BVH8 stack entry: child base, (imask << 8) | remaining hit bits
+// BVH8 stack entry: child base in the low word, (internal-child mask << 8) | remaining hit bits in the high word
+using Group = uint64_t;
+__device__ __forceinline__ Group makeGroup(uint32_t base, uint32_t bits) { return ((uint64_t)bits << 32) | base; }
+
+// ------------------------------------------------------------------------------------------------ traversal state
+// Resumable state of one ray's traversal: the node to visit next and, for BVH8, the group of hit children still to
+// visit (child base, internal-child mask, octant-ordered hit bits). With the lane's stack this is everything a
+// persistent kernel needs to suspend a ray between node visits.
+struct TravState { int node; uint32_t gBase, gMask, gHits; };
+
+__device__ __forceinline__ void travBegin(TravState& s, int root) { s.node = root; s.gBase = 0; s.gMask = 0; s.gHits = 0; }
 
 // ------------------------------------------------------------------------------------------------ BVH2
+// One node visit. Returns true when the traversal is over (stack exhausted, or for ANY a hit was found).
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool stepBVH2(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, const RayPre& r,
+                                         Hit& h, Stack& st, TravState& ts) {
+    const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
+    const Vec4* nb = nodes + (size_t)ts.node * 4;
+    const Vec4 n0 = ldg4(nb), n1 = ldg4(nb + 1), nz = ldg4(nb + 2), mt = ldg4(nb + 3);
+    int c0 = asInt(mt.x), c1 = asInt(mt.y);
+    const int k0 = asInt(mt.z), k1 = asInt(mt.w);
+    float a0 = fmaf(n0.x, r.ix, -odx), a1 = fmaf(n0.y, r.ix, -odx), a2 = fmaf(n0.z, r.iy, -ody), a3 = fmaf(n0.w, r.iy, -ody);
+    float a4 = fmaf(nz.x, r.iz, -odz), a5 = fmaf(nz.y, r.iz, -odz);
+    float b0 = fmaf(n1.x, r.ix, -odx), b1 = fmaf(n1.y, r.ix, -odx), b2 = fmaf(n1.z, r.iy, -ody), b3 = fmaf(n1.w, r.iy, -ody);
+    float b4 = fmaf(nz.z, r.iz, -odz), b5 = fmaf(nz.w, r.iz, -odz);
+    float t0n = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
+    float t0f = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h.t));
+    float t1n = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
+    float t1f = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h.t));
+    bool h0 = t0n <= t0f, h1 = t1n <= t1f;
+    if (h0 && k0 > 0) {
+        for (int t = c0; t < c0 + k0; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+        h0 = false;
+    }
+    if (h1 && k1 > 0) {
+        for (int t = c1; t < c1 + k1; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+        h1 = false;
+    }
+    if (!h0 && !h1) {
+        if (st.sp == 0) return true;
+        ts.node = st.pop();
+    } else if (h0 && h1) {
+        if (t1n < t0n) { int x = c0; c0 = c1; c1 = x; }
+        st.push(c1);
+        ts.node = c0;
+    } else ts.node = h0 ? c0 : c1;
+    return false;
+}
+
 template <bool ANY, int TRI, class Stack>
 __device__ __forceinline__ bool traceBVH2(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, int root,
                                           const RayPre& r, Hit& h, Stack& st) {
-    const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
-    int node = root;
-    for (;;) {
-        const Vec4* nb = nodes + (size_t)node * 4;
-        const Vec4 n0 = ldg4(nb), n1 = ldg4(nb + 1), nz = ldg4(nb + 2), mt = ldg4(nb + 3);
-        int c0 = asInt(mt.x), c1 = asInt(mt.y), k0 = asInt(mt.z), k1 = asInt(mt.w);
-        float a0 = fmaf(n0.x, r.ix, -odx), a1 = fmaf(n0.y, r.ix, -odx), a2 = fmaf(n0.z, r.iy, -ody), a3 = fmaf(n0.w, r.iy, -ody);
-        float a4 = fmaf(nz.x, r.iz, -odz), a5 = fmaf(nz.y, r.iz, -odz);
-        float b0 = fmaf(n1.x, r.ix, -odx), b1 = fmaf(n1.y, r.ix, -odx), b2 = fmaf(n1.z, r.iy, -ody), b3 = fmaf(n1.w, r.iy, -ody);
-        float b4 = fmaf(nz.z, r.iz, -odz), b5 = fmaf(nz.w, r.iz, -odz);
-        float t0n = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
-        float t0f = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h.t));
-        float t1n = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
-        float t1f = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h.t));
-        bool h0 = t0n <= t0f, h1 = t1n <= t1f;
-        if (h0 && k0 > 0) {
-            for (int t = c0; t < c0 + k0; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
-            h0 = false;
-        }
-        if (h1 && k1 > 0) {
-            for (int t = c1; t < c1 + k1; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
-            h1 = false;
-        }
-        if (!h0 && !h1) {
-            if (st.sp == 0) break;
-            node = st.pop();
-        } else if (h0 && h1) {
-            if (t1n < t0n) { int x = c0; c0 = c1; c1 = x; }
-            st.push(c1);
-            node = c0;
-        } else node = h0 ? c0 : c1;
-    }
+    TravState ts; travBegin(ts, root);
+    while (!stepBVH2<ANY, TRI>(nodes, tri, r, h, st, ts)) {}
     return h.tri >= 0;
 }
 
 // ------------------------------------------------------------------------------------------------ compressed BVH8
 __device__ __forceinline__ uint32_t byteOf(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
 
+// One node visit: the node's 8 child slabs, its hit leaves, then the choice of the next node. Returns true when the
+// traversal is over (nothing left to visit, or for ANY a hit was found).
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, const RayPre& r,
+                                         Hit& h, Stack& st, TravState& ts) {
+    const Vec4* nb = nodes + (size_t)ts.node * kBvh8Stride;
+    const Vec4 f0 = ldg4(nb), f1 = ldg4(nb + 1), f2 = ldg4(nb + 2), f3 = ldg4(nb + 3), f4 = ldg4(nb + 4);
+    const uint32_t ew = asUint(f0.w);
+    const float sx = asFloatU((ew & 255u) << 23) * r.ix;
+    const float sy = asFloatU(((ew >> 8) & 255u) << 23) * r.iy;
+    const float sz = asFloatU(((ew >> 16) & 255u) << 23) * r.iz;
+    const float ox = (f0.x - r.o.x) * r.ix, oy = (f0.y - r.o.y) * r.iy, oz = (f0.z - r.o.z) * r.iz;
+    const uint32_t imask = ew >> 24;
+    const uint32_t meta[2] = { asUint(f1.z), asUint(f1.w) };
+    const int childBase = asInt(f1.x), triBase = asInt(f1.y);
+    const uint32_t qlx[2] = { asUint(f2.x), asUint(f2.y) }, qly[2] = { asUint(f2.z), asUint(f2.w) };
+    const uint32_t qlz[2] = { asUint(f3.x), asUint(f3.y) }, qhx[2] = { asUint(f3.z), asUint(f3.w) };
+    const uint32_t qhy[2] = { asUint(f4.x), asUint(f4.y) }, qhz[2] = { asUint(f4.z), asUint(f4.w) };
+    // Phase 1: the 8 child slabs against the hit distance at node entry. With the ray's direction signs the near
+    // plane of each axis is known up front (lo for a positive direction, hi for a negative one), so each slot needs
+    // no per-axis min/max; the result is bit-identical to min/max of the two planes.
+    const uint32_t nx[2] = { r.pos & 1u ? qlx[0] : qhx[0], r.pos & 1u ? qlx[1] : qhx[1] };
+    const uint32_t fx[2] = { r.pos & 1u ? qhx[0] : qlx[0], r.pos & 1u ? qhx[1] : qlx[1] };
+    const uint32_t ny[2] = { r.pos & 2u ? qly[0] : qhy[0], r.pos & 2u ? qly[1] : qhy[1] };
+    const uint32_t fy[2] = { r.pos & 2u ? qhy[0] : qly[0], r.pos & 2u ? qhy[1] : qly[1] };
+    const uint32_t nz[2] = { r.pos & 4u ? qlz[0] : qhz[0], r.pos & 4u ? qlz[1] : qhz[1] };
+    const uint32_t fz[2] = { r.pos & 4u ? qhz[0] : qlz[0], r.pos & 4u ? qhz[1] : qlz[1] };
+    const float tEntry = h.t;
+    uint32_t hitSlots = 0, leafSlots = 0;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int w = s >> 2;
+        const uint32_t m = byteOf(meta[w], s);
+        const float xn = fmaf((float)byteOf(nx[w], s), sx, ox), xf = fmaf((float)byteOf(fx[w], s), sx, ox);
+        const float yn = fmaf((float)byteOf(ny[w], s), sy, oy), yf = fmaf((float)byteOf(fy[w], s), sy, oy);
+        const float zn = fmaf((float)byteOf(nz[w], s), sz, oz), zf = fmaf((float)byteOf(fz[w], s), sz, oz);
+        const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
+        const float tf = fminf(fminf(xf, yf), fminf(zf, tEntry));
+        hitSlots |= (uint32_t)(m != 0 && tn <= tf) << s;
+        leafSlots |= (m >> 7) << s;
+    }
+    // Phase 2: hit leaves in slot order, one shared loop
+    uint32_t lb = hitSlots & leafSlots;
+    while (lb) {
+        const int s = __builtin_ctz(lb);
+        lb &= lb - 1;
+        const uint32_t m = byteOf(s < 4 ? meta[0] : meta[1], s);
+        const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+        for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+    }
+    // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
+    uint32_t hits = hitSlots & ~leafSlots;
+    if (r.oct & 1u) hits = ((hits & 0x55u) << 1) | ((hits >> 1) & 0x55u);
+    if (r.oct & 2u) hits = ((hits & 0x33u) << 2) | ((hits >> 2) & 0x33u);
+    if (r.oct & 4u) hits = ((hits & 0x0Fu) << 4) | ((hits >> 4) & 0x0Fu);
+    if (hits) {
+        if (ts.gHits) st.push(makeGroup(ts.gBase, (ts.gMask << 8) | ts.gHits));
+        ts.gBase = (uint32_t)childBase; ts.gMask = imask; ts.gHits = hits;
+    }
+    while (ts.gHits == 0) {
+        if (st.sp == 0) return true;
+        const Group e = st.pop();
+        const uint32_t bits = (uint32_t)(e >> 32);
+        ts.gBase = (uint32_t)e; ts.gMask = bits >> 8; ts.gHits = bits & 255u;
+    }
+    const uint32_t k = (uint32_t)__builtin_ctz(ts.gHits);
+    ts.gHits &= ts.gHits - 1;
+    const uint32_t slot = k ^ r.oct;
+    ts.node = (int)ts.gBase + __builtin_popcount(ts.gMask & ((1u << slot) - 1u));
+    return false;
+}
+
 template <bool ANY, int TRI, class Stack>
 __device__ __forceinline__ bool traceBVH8(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, int root,
                                           const RayPre& r, Hit& h, Stack& st) {
-    uint32_t gBase = 0, gMask = 0, gHits = 0;
-    int node = root;
-    for (;;) {
-        const Vec4* nb = nodes + (size_t)node * kBvh8Stride;
-        const Vec4 f0 = ldg4(nb), f1 = ldg4(nb + 1), f2 = ldg4(nb + 2), f3 = ldg4(nb + 3), f4 = ldg4(nb + 4);
-        const uint32_t ew = asUint(f0.w);
-        const float sx = asFloatU((ew & 255u) << 23) * r.ix;
-        const float sy = asFloatU(((ew >> 8) & 255u) << 23) * r.iy;
-        const float sz = asFloatU(((ew >> 16) & 255u) << 23) * r.iz;
-        const float ox = (f0.x - r.o.x) * r.ix, oy = (f0.y - r.o.y) * r.iy, oz = (f0.z - r.o.z) * r.iz;
-        const uint32_t imask = ew >> 24;
-        const uint32_t meta[2] = { asUint(f1.z), asUint(f1.w) };
-        const int childBase = asInt(f1.x), triBase = asInt(f1.y);
-        const uint32_t qlx[2] = { asUint(f2.x), asUint(f2.y) }, qly[2] = { asUint(f2.z), asUint(f2.w) };
-        const uint32_t qlz[2] = { asUint(f3.x), asUint(f3.y) }, qhx[2] = { asUint(f3.z), asUint(f3.w) };
-        const uint32_t qhy[2] = { asUint(f4.x), asUint(f4.y) }, qhz[2] = { asUint(f4.z), asUint(f4.w) };
-        // Phase 1: the 8 child slabs against the hit distance at node entry. With the ray's direction signs the near
-        // plane of each axis is known up front (lo for a positive direction, hi for a negative one), so each slot needs
-        // no per-axis min/max; the result is bit-identical to min/max of the two planes.
-        const uint32_t nx[2] = { r.pos & 1u ? qlx[0] : qhx[0], r.pos & 1u ? qlx[1] : qhx[1] };
-        const uint32_t fx[2] = { r.pos & 1u ? qhx[0] : qlx[0], r.pos & 1u ? qhx[1] : qlx[1] };
-        const uint32_t ny[2] = { r.pos & 2u ? qly[0] : qhy[0], r.pos & 2u ? qly[1] : qhy[1] };
-        const uint32_t fy[2] = { r.pos & 2u ? qhy[0] : qly[0], r.pos & 2u ? qhy[1] : qly[1] };
-        const uint32_t nz[2] = { r.pos & 4u ? qlz[0] : qhz[0], r.pos & 4u ? qlz[1] : qhz[1] };
-        const uint32_t fz[2] = { r.pos & 4u ? qhz[0] : qlz[0], r.pos & 4u ? qhz[1] : qlz[1] };
-        const float tEntry = h.t;
-        uint32_t hitSlots = 0, leafSlots = 0;
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int w = s >> 2;
-            const uint32_t m = byteOf(meta[w], s);
-            const float xn = fmaf((float)byteOf(nx[w], s), sx, ox), xf = fmaf((float)byteOf(fx[w], s), sx, ox);
-            const float yn = fmaf((float)byteOf(ny[w], s), sy, oy), yf = fmaf((float)byteOf(fy[w], s), sy, oy);
-            const float zn = fmaf((float)byteOf(nz[w], s), sz, oz), zf = fmaf((float)byteOf(fz[w], s), sz, oz);
-            const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
-            const float tf = fminf(fminf(xf, yf), fminf(zf, tEntry));
-            hitSlots |= (uint32_t)(m != 0 && tn <= tf) << s;
-            leafSlots |= (m >> 7) << s;
-        }
-        // Phase 2: hit leaves in slot order, one shared loop
-        uint32_t lb = hitSlots & leafSlots;
-        while (lb) {
-            const int s = __builtin_ctz(lb);
-            lb &= lb - 1;
-            const uint32_t m = byteOf(s < 4 ? meta[0] : meta[1], s);
-            const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
-            for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
-        }
-        // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
-        uint32_t hits = hitSlots & ~leafSlots;
-        if (r.oct & 1u) hits = ((hits & 0x55u) << 1) | ((hits >> 1) & 0x55u);
-        if (r.oct & 2u) hits = ((hits & 0x33u) << 2) | ((hits >> 2) & 0x33u);
-        if (r.oct & 4u) hits = ((hits & 0x0Fu) << 4) | ((hits >> 4) & 0x0Fu);
-        if (hits) {
-            if (gHits) st.push(Group{ gBase, (gMask << 8) | gHits });
-            gBase = (uint32_t)childBase; gMask = imask; gHits = hits;
-        }
-        while (gHits == 0) {
-            if (st.sp == 0) return h.tri >= 0;
-            Group e = st.pop();
-            gBase = e.base; gMask = e.bits >> 8; gHits = e.bits & 255u;
-        }
-        uint32_t k = (uint32_t)__builtin_ctz(gHits);
-        gHits &= gHits - 1;
-        uint32_t slot = k ^ r.oct;
-        node = (int)gBase + __builtin_popcount(gMask & ((1u << slot) - 1u));
-    }
+    TravState ts; travBegin(ts, root);
+    while (!stepBVH8<ANY, TRI>(nodes, tri, r, h, st, ts)) {}
+    return h.tri >= 0;
 }
 
 }  // namespace cwp

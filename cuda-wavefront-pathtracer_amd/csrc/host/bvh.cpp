@@ -112,6 +112,50 @@ void buildSAH(const Mesh& m, int bins, int leafMax, BinaryBVH& out) {
     out.maxDepth = S.maxDepth;
 }
 
+void treeFromHploc(const std::vector<Vec4>& hn, int n, int root, int leafMax, BinaryBVH& out) {
+    const int tot = 2 * n - 1;
+    std::vector<int> lc(tot), rc(tot), cnt(tot);
+    std::vector<double> cost(tot);
+    std::vector<char> asLeaf(tot);
+    std::vector<AABB> box(tot);
+    for (int h = 0; h < tot; h++) {   // ids increase from children to parents: one forward sweep is bottom-up
+        box[h].lo = xyz(hn[2 * (size_t)h]); box[h].hi = xyz(hn[2 * (size_t)h + 1]);
+        lc[h] = asInt(hn[2 * (size_t)h].w); rc[h] = asInt(hn[2 * (size_t)h + 1].w);
+        const double A = box[h].area();
+        if (h < n) { cnt[h] = 1; cost[h] = A; asLeaf[h] = 1; continue; }
+        cnt[h] = cnt[lc[h]] + cnt[rc[h]];
+        const double inner = A + cost[lc[h]] + cost[rc[h]], leaf = cnt[h] <= leafMax ? A * cnt[h] : 1e300;
+        asLeaf[h] = leaf <= inner;
+        cost[h] = std::min(leaf, inner);
+    }
+    out.nodes.clear(); out.prim.clear(); out.maxDepth = 0; out.leafMax = leafMax;
+    out.nodes.reserve(2 * (size_t)n); out.prim.reserve(n);
+    struct Item { int h, parent, side, depth; };
+    std::vector<Item> st = { { root, -1, 0, 0 } };
+    std::vector<int> walk;
+    while (!st.empty()) {
+        const Item e = st.back(); st.pop_back();
+        const int id = (int)out.nodes.size();
+        BinNode bn; bn.box = box[e.h];
+        out.nodes.push_back(bn);
+        if (e.parent >= 0) (e.side ? out.nodes[e.parent].right : out.nodes[e.parent].left) = id;
+        out.maxDepth = std::max(out.maxDepth, e.depth);
+        if (asLeaf[e.h]) {   // the subtree's primitives, left to right
+            const int first = (int)out.prim.size();
+            walk.assign(1, e.h);
+            while (!walk.empty()) {
+                const int h = walk.back(); walk.pop_back();
+                if (h < n) out.prim.push_back(lc[h]);
+                else { walk.push_back(rc[h]); walk.push_back(lc[h]); }
+            }
+            out.nodes[id].first = first; out.nodes[id].count = (int)out.prim.size() - first;
+        } else {
+            st.push_back({ rc[e.h], id, 1, e.depth + 1 });
+            st.push_back({ lc[e.h], id, 0, e.depth + 1 });
+        }
+    }
+}
+
 double sahCost(const BinaryBVH& b, double cNode, double cPrim) {
     double a0 = b.nodes[0].box.area();
     if (!(a0 > 0)) return 0;

@@ -30,6 +30,10 @@ struct BinaryBVH {
 };
 
 void buildSAH(const Mesh& m, int bins, int leafMax, BinaryBVH& out);
+// Binary tree from H-PLOC nodes (2 float4 per node, 2n - 1 nodes, children before parents): a bottom-up SAH choice
+// per node between a leaf of all its primitives (cost A * count, only when count <= leafMax) and an internal node
+// (A + cost of the children), then a pre-order tree (reference: treeFromHploc, src/accel/bvh_tools.cpp:12-38).
+void treeFromHploc(const std::vector<Vec4>& hn, int n, int root, int leafMax, BinaryBVH& out);
 // SAH cost normalised by the root area: sum over internal nodes A cNode + leaves A count cPrim
 double sahCost(const BinaryBVH& b, double cNode, double cPrim);
 

@@ -1,0 +1,119 @@
+// host/scene_check.cpp - see scene_check.h.
+#include "host/scene_check.h"
+#include "host/bvh.h"
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace cwp {
+
+namespace {
+
+bool fail(std::string& err, const std::string& m) { err = m; return false; }
+
+// Child lists are checked for range, every node may have at most one parent and the root none, so the nodes
+// reachable from the root form a tree and every traversal ends; depth = levels of internal nodes below the root.
+struct Links {
+    std::vector<int> first, kids;   // CSR child lists of internal children
+};
+
+bool treeDepth(const Links& G, size_t nn, int root, int& depth, std::string& err) {
+    std::vector<unsigned char> parents(nn, 0);
+    for (int c : G.kids) {
+        if (parents[c]) return fail(err, "node " + std::to_string(c) + " has more than one parent");
+        parents[c] = 1;
+    }
+    if (parents[root]) return fail(err, "the root has a parent");
+    depth = 0;
+    std::vector<std::pair<int, int>> st = { { root, 1 } };
+    while (!st.empty()) {
+        const auto [j, d] = st.back(); st.pop_back();
+        depth = std::max(depth, d);
+        for (int k = G.first[j]; k < G.first[j + 1]; k++) st.push_back({ G.kids[k], d + 1 });
+    }
+    return true;
+}
+
+bool checkBVH2(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
+    Links G; G.first.assign(nn + 1, 0);
+    for (size_t j = 0; j < nn; j++) {
+        const Vec4* n = &S.nodes[j * 4];
+        const int c[2] = { asInt(n[3].x), asInt(n[3].y) }, k[2] = { asInt(n[3].z), asInt(n[3].w) };
+        const float box[2][6] = { { n[0].x, n[0].y, n[0].z, n[0].w, n[2].x, n[2].y },
+                                  { n[1].x, n[1].y, n[1].z, n[1].w, n[2].z, n[2].w } };
+        G.first[j] = (int)G.kids.size();
+        for (int s = 0; s < 2; s++) {
+            const std::string at = "BVH2 node " + std::to_string(j) + " child " + std::to_string(s);
+            if (k[s] < 0) return fail(err, at + ": negative triangle count");
+            if (k[s] > 0) {
+                if (c[s] < 0 || (size_t)c[s] + (size_t)k[s] > nt) return fail(err, at + ": triangle range out of bounds");
+                continue;
+            }
+            if (c[s] == -1) {   // empty slot: its box must be the empty box so that no ray enters it
+                for (float v : box[s])
+                    if (!(v >= kEmptyBox)) return fail(err, at + ": empty slot without the empty box");
+                continue;
+            }
+            if (c[s] < 0 || (size_t)c[s] >= nn) return fail(err, at + ": child index out of range");
+            G.kids.push_back(c[s]);
+        }
+    }
+    G.first[nn] = (int)G.kids.size();
+    return treeDepth(G, nn, S.root, depth, err);
+}
+
+bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
+    Links G; G.first.assign(nn + 1, 0);
+    for (size_t j = 0; j < nn; j++) {
+        const Vec4* n = &S.nodes[j * 5];
+        const uint32_t ew = asUint(n[0].w), imask = ew >> 24;
+        const int childBase = asInt(n[1].x), triBase = asInt(n[1].y);
+        const uint32_t meta[2] = { asUint(n[1].z), asUint(n[1].w) };
+        const std::string at = "BVH8 node " + std::to_string(j);
+        uint32_t internal = 0;
+        for (int s = 0; s < 8; s++) {
+            const uint32_t m = (meta[s >> 2] >> (8 * (s & 3))) & 255u;
+            if (m == 0) continue;
+            if (m & 0x80u) {
+                const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
+                if (triBase < 0 || first < 0 || (size_t)first + (size_t)cnt > nt)
+                    return fail(err, at + " slot " + std::to_string(s) + ": triangle range out of bounds");
+            } else internal |= 1u << s;
+        }
+        if (internal != imask) return fail(err, at + ": internal-child mask disagrees with the slot metadata");
+        G.first[j] = (int)G.kids.size();
+        const int ni = __builtin_popcount(internal);
+        if (ni == 0) continue;
+        if (childBase < 0 || (size_t)childBase + (size_t)ni > nn) return fail(err, at + ": children outside the node array");
+        for (int c = childBase; c < childBase + ni; c++) G.kids.push_back(c);
+    }
+    G.first[nn] = (int)G.kids.size();
+    return treeDepth(G, nn, S.root, depth, err);
+}
+
+}  // namespace
+
+bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
+    if (S.variant != 2 && S.variant != 8) return fail(err, "variant must be 2 or 8");
+    const size_t per = S.variant == 2 ? 4 : 5;
+    if (S.nodes.empty() || S.nodes.size() % per) return fail(err, "node array size is not a whole number of nodes");
+    if (S.root < 0 || (size_t)S.root >= S.nodes.size() / per) return fail(err, "root node out of range");
+    if (S.verts.empty() || S.verts.size() % 3) return fail(err, "corner array size is not a whole number of triangles");
+    const size_t nn = S.nodes.size() / per, nt = S.verts.size() / 3;
+    if (nn > 0x7fffffff || nt > 0x7fffffff) return fail(err, "too many nodes or triangles");
+    if (!S.woop.empty() && S.woop.size() != S.verts.size()) return fail(err, "Woop rows must be 3 per triangle");
+    if (!S.normals.empty() && S.normals.size() != S.verts.size()) return fail(err, "normals must be 3 per triangle");
+    if (S.albedo.empty()) return fail(err, "no materials");
+    for (size_t t = 0; t < nt; t++) {
+        const int m = asInt(S.verts[3 * t + 1].w);
+        if (m < 0 || (size_t)m >= S.albedo.size())
+            return fail(err, "triangle " + std::to_string(t) + ": material id out of range");
+    }
+    int depth = 0;
+    if (!(S.variant == 2 ? checkBVH2(S, nn, nt, depth, err) : checkBVH8(S, nn, nt, depth, err))) return false;
+    // a traversal holds at most one deferred entry per level below the root, plus one
+    stackNeed = depth + 1;
+    return true;
+}
+
+}  // namespace cwp

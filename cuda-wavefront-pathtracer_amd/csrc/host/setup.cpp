@@ -1,5 +1,6 @@
 // host/setup.cpp - option parsing and scene setup (see host/setup.h).
 #include "host/setup.h"
+#include "device/hploc.h"
 #include "host/io.h"
 #include <chrono>
 #include <cmath>
@@ -50,6 +51,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--bins") ok = inum(A.bins);
         else if (k == "--fuse") ok = inum(A.fuse);
         else if (k == "--persistent") ok = inum(A.persistent);
+        else if (k == "--shadow_refill") ok = inum(A.shadowRefill);
         else if (k == "--smooth") ok = inum(A.smooth);
         else if (k == "--warm") ok = inum(A.warm);
         else if (k == "--reps") ok = inum(A.reps);
@@ -80,9 +82,13 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.variant != "bvh2" && A.variant != "bvh8") { err = "--variant must be bvh2 or bvh8"; return false; }
     if (A.tri != "mt" && A.tri != "woop" && A.tri != "watertight") { err = "--tri must be mt, woop or watertight"; return false; }
     if (A.integrator != "auto" && A.integrator != "legacy") { err = "--integrator must be auto or legacy"; return false; }
-    if (A.builder != "sah") { err = "--builder must be sah"; return false; }
+    if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
-    if (A.fuse != 1) { err = "--fuse must be 1 (fused trace+shade kernel)"; return false; }
+    if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
+    if (A.persistent != 0 && A.persistent != 1) { err = "--persistent must be 0 or 1"; return false; }
+    if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
+    if (A.fuse < 0) A.fuse = 1;
+    if (A.persistent) A.fuse = 0;   // persistent traversal runs in the split trace kernel
     if (A.bins < 2 || A.bins > 256) { err = "--bins must be in [2, 256]"; return false; }
     if (A.leaf < 1 || A.leaf > 32) { err = "--leaf must be in [1, 32]"; return false; }
     if (A.w < 1 || A.h < 1) { err = "--width/--height must be positive"; return false; }
@@ -159,8 +165,25 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     const int V = Z.variant = A.variant == "bvh2" ? 2 : 8;
     const int leafMax = V == 8 ? 1 : A.leaf;   // the CWBVH collapse forms its own leaves from a 1-primitive-leaf tree
     Z.reportLeaf = V == 8 ? 3 : leafMax;
-    buildSAH(M, A.bins, leafMax, Z.tree);
-    Z.treeMs = msSince(t0);
+    if (A.builder == "hploc") {
+        // GPU build over the triangle boxes, then the host SAH leaf collapse (reference: buildTree,
+        // src/app/scene_setup.cpp:76-83); treeMs is the device time, postMs the host conversion
+        const int n = (int)M.f.size();
+        std::vector<Vec4> lo(n), hi(n), hn((size_t)2 * (2 * (size_t)n - 1));
+        for (int t = 0; t < n; t++) {
+            AABB b; b.grow(M.v[M.f[t].a]); b.grow(M.v[M.f[t].b]); b.grow(M.v[M.f[t].c]);
+            lo[t] = v4(b.lo, 0.f); hi[t] = v4(b.hi, 0.f);
+        }
+        int root = 0; double gpuMs = 0;
+        if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err)) return false;
+        const auto t1 = std::chrono::steady_clock::now();
+        treeFromHploc(hn, n, root, leafMax, Z.tree);
+        Z.postMs = msSince(t1);
+        Z.treeMs = gpuMs;
+    } else {
+        buildSAH(M, A.bins, leafMax, Z.tree);
+        Z.treeMs = msSince(t0);
+    }
     Z.maxDepth = Z.tree.maxDepth;
     SceneUpload& S = Z.S;
     S.variant = V; S.root = 0;
@@ -185,7 +208,8 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     RenderOptions& O = Z.O;
     O.variant = V; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
-    O.bounceWaves = A.bounceWaves;
+    O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
+    O.shadowRefill = A.shadowRefill;
     return true;
 }
 
@@ -195,21 +219,21 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
     const int N = A.w * A.h;
     char buf[4096];
     snprintf(buf, sizeof buf,
-             "{\"rep\":%d,\"variant\":\"bvh%d\",\"tri\":\"%s\",\"persistent\":%d,\"devloop\":0,\"sort\":0,\"fuse\":1,\"leaf\":%d,"
+             "{\"rep\":%d,\"variant\":\"bvh%d\",\"tri\":\"%s\",\"persistent\":%d,\"devloop\":0,\"sort\":0,\"fuse\":%d,\"leaf\":%d,"
              "\"batch\":%d,\"minb\":0,\"scene\":\"%s\",\"tris\":%zu,\"bvh_nodes\":%zu,\"bvh_bytes\":%zu,\"bvh_depth\":%d,"
              "\"bvh_build_s\":%.2f,\"w\":%d,\"h\":%d,\"spp\":%d,\"bounces\":%d,\"rays_closest\":%.0f,\"rays_shadow\":%.0f,"
              "\"rays_total\":%.0f,\"t_trace_ms\":%.2f,\"t_frame_ms\":%.2f,\"mrays_s_trace\":%.1f,\"mrays_s_e2e\":%.1f,"
              "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
              "\"seed\":%u,\"integrator\":\"legacy\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
-             "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":0.00,\"bvh_sah\":%.4f,\"graph\":0,"
+             "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":0,"
              "\"instances\":0,\"flatten\":0,\"frame\":%d,\"tlas_ms\":0.000,\"bins\":%d,\"stack_lds\":%d,"
-             "\"t_bounce_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
-             rep, Z.variant, A.tri.c_str(), A.persistent, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
+             "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
+             rep, Z.variant, A.tri.c_str(), A.persistent, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
              Z.P.hasN, A.seed, Z.P.cam.lensR, Z.P.cam.focusD, "importance", A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
-             Z.treeMs, Z.sah, frame, A.bins, R.options().stackLds, st.bounceMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
+             Z.treeMs, Z.postMs, Z.sah, frame, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
              st.unoccluded);
     return buf;
 }

@@ -17,9 +17,9 @@ struct Args {
     std::string gen;                 // procedural scene: "boxes" (1,024-box cloud) or "terrain" (fBm heightfield + boxes)
     std::string integrator = "auto", builder = "sah", collapse = "sah";
     int genBoxes = -1, genGrid = 242; uint64_t genSeed = 1;
-    int fuse = 1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
-    int persistent = 1, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
-    int bounceWaves = 6;
+    int fuse = -1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
+    int persistent = 0, shadowRefill = 0, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
+    int bounceWaves = 8;
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);
@@ -39,7 +39,7 @@ struct SceneSetup {
     float c2w[16];
     float extent = 0.f;
     int variant = 8, maxDepth = 0, reportLeaf = 0;
-    double bvhS = 0, treeMs = 0, sah = 0;
+    double bvhS = 0, treeMs = 0, postMs = 0, sah = 0;
     size_t nNodes = 0, nodeBytes = 0;
     std::string sceneName;
 };

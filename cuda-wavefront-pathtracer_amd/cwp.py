@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(PKG, "libcwp.so")
 ARR_VERTICES, ARR_TRIANGLES, ARR_MATERIAL, ARR_ALBEDO, ARR_NODES, ARR_ORDER, ARR_WOOP, ARR_CORNERS, ARR_NORMALS, \
     ARR_TREE, ARR_TREE_BOX, ARR_PRIM = range(12)
 TRI_MT, TRI_WOOP, TRI_WATERTIGHT = 0, 1, 2
+CWP_COPY_H2D, CWP_COPY_D2H, CWP_COPY_D2D = 1, 2, 3
 
 
 class CwpError(RuntimeError):
@@ -43,7 +44,7 @@ class FrameStats(ctypes.Structure):
                 ("t_frame_ms", ctypes.c_double), ("t_bounce_ms", ctypes.c_double), ("t_shadow_ms", ctypes.c_double),
                 ("bounce_launches", ctypes.c_int), ("shadow_launches", ctypes.c_int), ("rays_hit", ctypes.c_double),
                 ("rays_miss", ctypes.c_double), ("rays_cont", ctypes.c_double), ("shadow_unoccluded", ctypes.c_double),
-                ("camera_miss", ctypes.c_double)]
+                ("camera_miss", ctypes.c_double), ("t_shade_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -58,6 +59,50 @@ class SceneInfo(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SceneDesc(ctypes.Structure):
+    """cwp_scene_desc: a scene in the renderer's upload layout (host arrays)."""
+    _fields_ = [("variant", ctypes.c_int), ("nodes", ctypes.c_void_p), ("num_nodes", ctypes.c_size_t),
+                ("corners", ctypes.c_void_p), ("num_tris", ctypes.c_size_t), ("woop", ctypes.c_void_p),
+                ("normals", ctypes.c_void_p), ("albedo", ctypes.c_void_p), ("num_materials", ctypes.c_size_t),
+                ("tri_form", ctypes.c_int), ("root", ctypes.c_int)]
+
+
+class Options(ctypes.Structure):
+    """cwp_options: kernel and schedule options."""
+    _fields_ = [("tri_test", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int), ("fuse", ctypes.c_int),
+                ("persistent", ctypes.c_int), ("shadow_refill", ctypes.c_int), ("stack_lds", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int), ("bounce_waves", ctypes.c_int), ("time_kernels", ctypes.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib().cwp_options_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_form=0, root=0):
+    """Builds a cwp_scene_desc over numpy arrays; returns (desc, keep-alive list). tri_form 1: corners holds the
+    reference's edge form (v0, e1, e2)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
+            for a in (nodes, corners, woop, normals, albedo)]
+    n, c, w, nr, al = keep
+    per = 16 if variant == 2 else 20
+    d = SceneDesc(variant, n.ctypes.data, n.size // per, c.ctypes.data, c.size // 12,
+                  w.ctypes.data if w is not None else None, nr.ctypes.data if nr is not None else None,
+                  al.ctypes.data, al.size // 4, tri_form, root)
+    return d, keep
+
+
+def scene_check(variant, nodes, corners, albedo, woop=None, normals=None, tri_form=0, root=0):
+    """Host-side check of a scene description; returns the traversal-stack entries the tree can need."""
+    d, keep = scene_desc(variant, nodes, corners, albedo, woop, normals, tri_form, root)
+    need = ctypes.c_int(0)
+    _check(lib().cwp_scene_check(ctypes.byref(d), ctypes.byref(need)))
+    return need.value
 
 
 # declared symbols of include/cwp.h with their signatures
@@ -80,10 +125,25 @@ SIGNATURES = {
     "cwp_trace_host": ([_P, ctypes.POINTER(ctypes.c_float), _I, _I, ctypes.POINTER(ctypes.c_float)], _I),
     "cwp_frame_json": ([_P, _P, ctypes.POINTER(FrameStats), _I, _I, ctypes.c_char_p, _S], _I),
     "cwp_ctx_info": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_options_default": ([ctypes.POINTER(Options)], None),
+    "cwp_scene_check": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(_I)], _I),
+    "cwp_create_scene": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(Params), ctypes.POINTER(Options), ctypes.POINTER(_P)], _I),
     "cwp_bvh_build": ([ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), _I, _I, _I, _I, ctypes.POINTER(_P)], _I),
     "cwp_bvh_destroy": ([_P], None),
     "cwp_bvh_array": ([_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_S)], _I),
     "cwp_bvh_stats": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_bvh_from_hploc": ([ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), _I, ctypes.POINTER(ctypes.c_float), _I,
+                            _I, _I, ctypes.POINTER(_P)], _I),
+    "cwp_hploc_build_host": ([ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _I,
+                              ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_hploc_create": ([_I, ctypes.POINTER(_P)], _I),
+    "cwp_hploc_destroy": ([_P], None),
+    "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),
+    "cwp_hploc_nodes": ([_P, ctypes.POINTER(_P)], _I),
+    "cwp_hploc_to_bvh2": ([_P, _I, _P, _P], _I),
+    "cwp_device_malloc": ([_S, ctypes.POINTER(_P)], _I),
+    "cwp_device_free": ([_P], None),
+    "cwp_memcpy": ([_P, _P, _S, _I], _I),
 }
 
 _lib = None
@@ -180,10 +240,25 @@ class Renderer:
         self.setup = setup
         h = ctypes.c_void_p()
         _check(lib().cwp_create(setup.h, ctypes.byref(h)))
+        self._attach(h, setup.info["width"], setup.info["height"])
+
+    def _attach(self, h, w, hgt):
         self.h = h
+        self.w, self.hgt = w, hgt
         g, v = ctypes.c_int(), ctypes.c_double()
         _check(lib().cwp_ctx_info(self.h, ctypes.byref(g), ctypes.byref(v)))
         self.grid_blocks, self.vram_mb = g.value, v.value
+
+    @classmethod
+    def from_scene(cls, variant, nodes, corners, albedo, params, options, woop=None, normals=None, tri_form=0, root=0):
+        """Renderer over caller arrays in the upload layout (cwp_create_scene)."""
+        d, keep = scene_desc(variant, nodes, corners, albedo, woop, normals, tri_form, root)
+        h = ctypes.c_void_p()
+        _check(lib().cwp_create_scene(ctypes.byref(d), ctypes.byref(params), ctypes.byref(options), ctypes.byref(h)))
+        r = cls.__new__(cls)
+        r.setup = None
+        r._attach(h, params.w, params.h)
+        return r
 
     def render(self, spp, sample_base=0, clear=True):
         st = FrameStats()
@@ -192,7 +267,7 @@ class Renderer:
         return st.as_dict()
 
     def accum(self):
-        w, h = self.setup.info["width"], self.setup.info["height"]
+        w, h = self.w, self.hgt
         out = np.zeros((h, w, 4), dtype=np.float32)
         _check(lib().cwp_read_accum(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w * h))
         return out
@@ -220,7 +295,8 @@ class Renderer:
                 setattr(f, k, v)
             st = f
         buf = ctypes.create_string_buffer(8192)
-        _check(lib().cwp_frame_json(self.setup.h, self.h, ctypes.byref(st), rep, frame, buf, 8192))
+        sh = self.setup.h if self.setup is not None else None
+        _check(lib().cwp_frame_json(sh, self.h, ctypes.byref(st), rep, frame, buf, 8192))
         return json.loads(buf.value.decode())
 
     def close(self):
@@ -243,6 +319,10 @@ def bvh_build(vertices, triangles, variant=8, bins=16, leaf_max=4):
     _check(lib().cwp_bvh_build(v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), v.shape[0],
                                t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), t.shape[0], variant, bins, leaf_max,
                                ctypes.byref(h)))
+    return _bvh_result(h, variant)
+
+
+def _bvh_result(h, variant):
     try:
         out = {}
         nf4 = 4 if variant == 2 else 5
@@ -257,3 +337,67 @@ def bvh_build(vertices, triangles, variant=8, bins=16, leaf_max=4):
         return out
     finally:
         lib().cwp_bvh_destroy(h)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _boxes4(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 2 and a.shape[1] == 3:
+        a = np.concatenate([a, np.zeros((len(a), 1), np.float32)], 1)
+    return np.ascontiguousarray(a.reshape(-1, 4))
+
+
+def hploc_build(lo, hi):
+    """GPU H-PLOC build over host boxes ((n, 3) or (n, 4)). Returns (nodes (2n-1, 8) float32, root, gpu_ms)."""
+    lo, hi = _boxes4(lo), _boxes4(hi)
+    n = len(lo)
+    nodes = np.zeros((2 * n - 1, 8), np.float32)
+    root, ms = ctypes.c_int(), ctypes.c_double()
+    _check(lib().cwp_hploc_build_host(_fptr(lo), _fptr(hi), n, _fptr(nodes), ctypes.byref(root), ctypes.byref(ms)))
+    return nodes, root.value, ms.value
+
+
+def bvh_from_hploc(vertices, triangles, hnodes, root, variant=8, leaf_max=4):
+    """Layout (as bvh_build) from H-PLOC nodes over the mesh's triangle boxes."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    h = np.ascontiguousarray(hnodes, dtype=np.float32)
+    p = ctypes.c_void_p()
+    _check(lib().cwp_bvh_from_hploc(_fptr(v), v.shape[0], t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), t.shape[0],
+                                    _fptr(h), int(root), variant, leaf_max, ctypes.byref(p)))
+    return _bvh_result(p, variant)
+
+
+class DeviceArray:
+    """Device memory owned by libcwp (for the device-pointer entry points without a GPU array library)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().cwp_device_malloc(nbytes, ctypes.byref(p)))
+        self.ptr, self.nbytes = p.value, nbytes
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        _check(lib().cwp_memcpy(d.ptr, a.ctypes.data, a.nbytes, 1))
+        return d
+
+    def to_host(self, dtype, shape):
+        out = np.zeros(shape, dtype)
+        _check(lib().cwp_memcpy(out.ctypes.data, self.ptr, out.nbytes, 2))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cwp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

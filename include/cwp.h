@@ -7,6 +7,7 @@
  * The entry points replace the reference's statically linked C++ interfaces (paths relative to the reference repo):
  *   scene setup      src/app/scene_setup.h:12-50    parseArgs / setupScene / SceneSetup
  *   renderer         src/render/renderer.h:47-93    WavefrontRenderer::{init, render, readAccum, setParams}
+ *   scene buffers    src/render/renderer.h:11-43    RenderOptions, SceneBuffers (legacy-integrator fields)
  *   BVH build        src/accel/bvh_builder.h:9-16, src/accel/bvh_layouts.h:14-40, src/accel/bvh_tools.h:23-29
  *   traversal        src/accel/traverse.cuh:155-160 trav<V, ANY, TRI> (as a kernel over explicit rays)
  *   frame statistics src/render/renderer.h:45 FrameStats, src/app/pt.cpp:35-41 (JSON line)
@@ -58,7 +59,40 @@ typedef struct cwp_frame_stats {
     int bounce_launches, shadow_launches;
     double rays_hit, rays_miss, rays_cont, shadow_unoccluded;
     double camera_miss;               /* camera rays that miss (rays_miss includes them) */
+    double t_shade_ms;                /* split kernels: shade-kernel part of t_bounce_ms (0 when fused) */
 } cwp_frame_stats;
+
+/* A scene in the renderer's upload layout: host arrays in BVH leaf order, copied by cwp_create_scene. Replaces the
+ * legacy-integrator part of SceneBuffers (src/render/renderer.h:31-43); the layouts are those of cwp_bvh_array. */
+typedef struct cwp_scene_desc {
+    int variant;              /* 2: BVH2, 16 floats per node; 8: compressed BVH8, 20 floats (80 B) per node */
+    const float* nodes;
+    size_t num_nodes;
+    const float* corners;     /* 12 floats per triangle: (v0, original id bits), (v1, material id bits), (v2, 0) */
+    size_t num_tris;
+    const float* woop;        /* 12 floats per triangle (Woop rows), or NULL; required by tri_test 1 */
+    const float* normals;     /* 12 floats per triangle (per-corner normals), or NULL */
+    const float* albedo;      /* 4 floats per material (rgb, unused) */
+    size_t num_materials;
+    int tri_form;             /* CWP_TRIS_CORNERS, or CWP_TRIS_EDGES: the reference's (v0, id), (e1, material), (e2, 0)
+                                 (src/accel/tri_pack.cpp:6-16), turned into corners v0 + e1, v0 + e2 on upload */
+    int root;                 /* root node (0 for the layouts of cwp_bvh_build; n - 2 for cwp_hploc_to_bvh2) */
+} cwp_scene_desc;
+enum { CWP_TRIS_CORNERS = 0, CWP_TRIS_EDGES = 1 };
+
+/* Kernel and schedule options. Replaces RenderOptions (src/render/renderer.h:11-28) for the legacy integrator. */
+typedef struct cwp_options {
+    int tri_test;        /* 0 Moller-Trumbore, 1 Woop, 2 watertight */
+    int batch;           /* samples per pixel in flight per wavefront pass */
+    int bounces;         /* maximum bounce depth (camera hit = depth 0), 0..64 */
+    int fuse;            /* 1: trace and shade in one kernel; 0: split trace and shade kernels */
+    int persistent;      /* 1: split kernels whose waves refill idle lanes mid-traversal (needs fuse 0) */
+    int shadow_refill;   /* 1: the shadow kernel refills idle lanes mid-traversal */
+    int stack_lds;       /* LDS traversal-stack entries per lane: 8, 16 or 32 (deeper entries spill to HBM) */
+    int blocks_per_cu;   /* workgroups per CU of the persistent grids (0 = occupancy limit) */
+    int bounce_waves;    /* fused kernel register budget: at least this many waves per SIMD (1, 6 or 8) */
+    int time_kernels;    /* per-launch event timing (t_bounce_ms, t_shade_ms, t_shadow_ms) */
+} cwp_options;
 
 /* Scene and BVH summary of a setup. */
 typedef struct cwp_scene_info {
@@ -123,6 +157,18 @@ int cwp_trace(cwp_ctx* c, const void* rays, int n, int any_hit, void* hits);
 int cwp_trace_host(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits);
 /* One JSON object (no newline) with the reference's per-frame statistics keys (src/app/pt.cpp:35-41). */
 int cwp_frame_json(const cwp_setup* s, const cwp_ctx* c, const cwp_frame_stats* st, int rep, int frame, char* buf, size_t cap);
+/* The defaults of cwp_options: Woop test, fused trace+shade kernel at 8 waves per SIMD, one-ray-per-lane shadow
+ * kernel, 8 LDS stack entries, batch 64, 8 bounces. */
+void cwp_options_default(cwp_options* o);
+/* Checks a scene description without a GPU: array sizes, every child-node and leaf-triangle index in range, BVH8
+ * internal masks consistent with the slot metadata, material
+ * ids in range, no node with two parents and none for the root (so every traversal ends). *stack_need receives the
+ * traversal-stack entries the tree can need. cwp_create_scene runs the same
+ * check; the kernels trust every index that passes it. */
+int cwp_scene_check(const cwp_scene_desc* d, int* stack_need);
+/* Renderer over caller buffers. Replaces WavefrontRenderer::init(Params, SceneBuffers, RenderOptions)
+ * (src/render/wavefront.cu:469-502). cwp_frame_json needs a setup and is not available for such a context. */
+int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* params, const cwp_options* opts, cwp_ctx** out);
 /* Grid size (workgroups) and VRAM used by the context. */
 int cwp_ctx_info(const cwp_ctx* c, int* grid_blocks, double* vram_mb);
 
@@ -136,6 +182,37 @@ void cwp_bvh_destroy(cwp_bvh* b);
 int cwp_bvh_array(const cwp_bvh* b, int which, const void** data, size_t* count);
 /* depth of the binary tree, traversal stack need, SAH cost of the layout */
 int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stack_need, double* sah);
+/* The same outputs from H-PLOC nodes over the mesh's triangle boxes (8 floats per node, 2 nt - 1 nodes, children
+ * before parents): SAH leaf collapse (leaves <= leaf_max, 1 for variant 8), then the layout. Replaces treeFromHploc
+ * + BVH2Flat / CWBVH::build (src/accel/bvh_tools.cpp:12-38, src/app/scene_setup.cpp:76-83). Host only. */
+int cwp_bvh_from_hploc(const float* vertices, int num_vertices, const int* triangles, int num_tris, const float* hnodes,
+                       int root, int variant, int leaf_max, cwp_bvh** out);
+
+/* ---- H-PLOC GPU BVH2 builder (src/accel/hploc.h:21-40) ----
+ * Nodes: 8 floats per node, (lo.xyz, left id bits), (hi.xyz, right id bits), 2n - 1 nodes; [0, n) are the leaves in
+ * Morton order (left = primitive id, right = -1), internal node ids come from a counter (children before parents),
+ * the root is 2n - 2 (0 when n == 1). Internal node numbering depends on scheduling; the tree does not. */
+typedef struct cwp_hploc cwp_hploc;
+/* Host arrays: lo / hi 4 floats per primitive box; builds twice (the second timed with events, *gpu_ms) and copies
+ * the nodes out. Replaces hplocBuildHost. */
+int cwp_hploc_build_host(const float* lo, const float* hi, int n, float* nodes, int* root, double* gpu_ms);
+/* Stream-ordered builder over device arrays; scratch sized for `capacity` primitives (grown on demand). */
+int cwp_hploc_create(int capacity, cwp_hploc** out);
+void cwp_hploc_destroy(cwp_hploc* b);
+/* lo / hi: device pointers, 4 floats per primitive; stream: a hipStream_t or NULL (default stream). Asynchronous: the
+ * nodes (cwp_hploc_nodes, a device pointer owned by the builder) are ready when the stream gets there. */
+int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root);
+int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes);
+/* 64 B BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) of device H-PLOC nodes into `out`
+ * (device). Replaces hplocToBVH2 (src/accel/hploc.h:36). */
+int cwp_hploc_to_bvh2(const void* hnodes, int n, void* out, void* stream);
+
+/* ---- device memory for callers without a GPU array library ---- */
+enum { CWP_COPY_H2D = 1, CWP_COPY_D2H = 2, CWP_COPY_D2D = 3 };
+int cwp_device_malloc(size_t bytes, void** ptr);
+void cwp_device_free(void* ptr);
+/* synchronous copy; kind: CWP_COPY_H2D, CWP_COPY_D2H or CWP_COPY_D2D */
+int cwp_memcpy(void* dst, const void* src, size_t bytes, int kind);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 def build(quiet=True):
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wall",
-           "-o", tmp, os.path.join(HERE, "oracle.c"), "-lm", "-lpthread"]
+           "-o", tmp, os.path.join(HERE, "oracle.c"), os.path.join(HERE, "hploc.c"), "-lm", "-lpthread"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + " ".join(cmd) + "\n" + r.stdout)

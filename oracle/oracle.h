@@ -77,6 +77,20 @@ void orc_layout_free(orc_layout* l);
 void orc_pack_woop(const float* v, const int* tri, const int* order, int n, float* rows);
 void orc_pack_corners(const float* v, const int* tri, const int* mat, const int* order, int n, float* corners);
 
+/* ---- H-PLOC GPU builder and its SAH leaf collapse: src/accel/hploc.cu:11-160, src/accel/bvh_tools.cpp:12-38 ---- */
+/* lo / hi: float4 per primitive; nodes: 8 floats per node, (lo.xyz, left id bits), (hi.xyz, right id bits); leaves
+ * [0, n) in Morton order with left = primitive id, right = -1; internal nodes [n, 2n - 1) */
+void orc_hp_centroid_box(const float* lo, const float* hi, int n, float box[6]);
+uint64_t orc_expand_bits21(uint32_t v);
+void orc_hp_key_bits(int n, int* mb, int* ib);
+void orc_hp_keys(const float* lo, const float* hi, int n, const float box[6], int mb, int ib, uint64_t* keys);
+/* full build (keys, sort, leaves, H-PLOC); sorted_keys (n) may be NULL; returns the root id (2n - 2, or 0 for n = 1) */
+int orc_hp_build(const float* lo, const float* hi, int n, float* nodes, uint64_t* sorted_keys);
+/* BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) */
+void orc_hp_to_bvh2(const float* hn, int n, float* out);
+/* binary tree with SAH-collapsed leaves (<= leaf_max primitives), pre-order; prim = primitive ids */
+int orc_tree_from_hploc(const float* hn, int n, int root, int leaf_max, orc_tree* out);
+
 /* ---- traversal: src/accel/traverse.cuh ---- */
 /* rays: 8 floats (o, tmin, d, tmax); hits: 4 floats (t, u, v, leaf position bits, -1 = miss); returns 0 on success */
 int orc_trace(const orc_scene* s, const float* rays, int n, int any_hit, float* hits, int threads);

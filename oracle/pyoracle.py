@@ -61,6 +61,14 @@ def lib():
                                  ctypes.c_int, ctypes.c_int, _FP, _DP]
         L.orc_render.restype = ctypes.c_int
         L.orc_camera_ray.argtypes = [_P, ctypes.c_int, ctypes.c_uint32, _FP, _FP]
+        _U64P = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_hp_centroid_box.argtypes = [_FP, _FP, ctypes.c_int, _FP]
+        L.orc_expand_bits21.argtypes = [ctypes.c_uint32]; L.orc_expand_bits21.restype = ctypes.c_uint64
+        L.orc_hp_key_bits.argtypes = [ctypes.c_int, _IP, _IP]
+        L.orc_hp_keys.argtypes = [_FP, _FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, _U64P]
+        L.orc_hp_build.argtypes = [_FP, _FP, ctypes.c_int, _FP, _U64P]
+        L.orc_hp_to_bvh2.argtypes = [_FP, ctypes.c_int, _FP]
+        L.orc_tree_from_hploc.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcTree)]
         _lib = L
     return _lib
 
@@ -116,6 +124,81 @@ def tree_arrays(tr):
     return dict(tree=node, tree_box=box, prim=prim, depth=tr.max_depth)
 
 
+def _box4(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 2 and a.shape[1] == 3:
+        a = np.concatenate([a, np.zeros((len(a), 1), np.float32)], 1)
+    return np.ascontiguousarray(a.reshape(-1, 4))
+
+
+def hp_key_bits(n):
+    mb, ib = ctypes.c_int(), ctypes.c_int()
+    lib().orc_hp_key_bits(n, ctypes.byref(mb), ctypes.byref(ib))
+    return mb.value, ib.value
+
+
+def hp_keys(lo, hi):
+    """Centroid box and unsorted 64-bit keys of primitive boxes (lo/hi: (n, 3) or (n, 4))."""
+    lo, hi = _box4(lo), _box4(hi)
+    n = len(lo)
+    box = np.zeros(6, np.float32)
+    lib().orc_hp_centroid_box(_f(lo), _f(hi), n, _f(box))
+    mb, ib = hp_key_bits(n)
+    keys = np.zeros(n, np.uint64)
+    lib().orc_hp_keys(_f(lo), _f(hi), n, _f(box), mb, ib, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return box, keys, mb, ib
+
+
+def expand_bits21(v):
+    return lib().orc_expand_bits21(v)
+
+
+def hp_build(lo, hi):
+    """H-PLOC build: returns (nodes (2n-1, 8) float32, root, sorted keys)."""
+    lo, hi = _box4(lo), _box4(hi)
+    n = len(lo)
+    nodes = np.zeros((max(2 * n - 1, 1), 8), np.float32)
+    keys = np.zeros(n, np.uint64)
+    root = lib().orc_hp_build(_f(lo), _f(hi), n, _f(nodes), keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return nodes, root, keys
+
+
+def hp_to_bvh2(nodes, n):
+    h = np.ascontiguousarray(nodes, dtype=np.float32)
+    out = np.zeros((max(n - 1, 1), 16), np.float32)
+    lib().orc_hp_to_bvh2(_f(h), n, _f(out))
+    return out
+
+
+def tree_from_hploc(nodes, n, root, leaf_max):
+    h = np.ascontiguousarray(nodes, dtype=np.float32)
+    tr = OrcTree()
+    lib().orc_tree_from_hploc(_f(h), n, root, leaf_max, ctypes.byref(tr))
+    out = tree_arrays(tr)
+    lib().orc_tree_free(ctypes.byref(tr))
+    return out
+
+
+def hp_canonical(nodes, n, root):
+    """Canonical pre-order form of an H-PLOC tree, independent of internal node numbering: a list of tuples
+    (lo/hi bits) + ('leaf', primitive) or ('node', left index, right index) with pre-order indices."""
+    h = np.ascontiguousarray(nodes, dtype=np.float32)
+    bits = h.view(np.uint32)
+    out, st = [], [(root, -1, 0)]
+    while st:
+        x, parent, side = st.pop()
+        me = len(out)
+        if parent >= 0:
+            out[parent][2 + side] = me
+        if x < n:
+            out.append([tuple(bits[x, [0, 1, 2, 4, 5, 6]].tolist()), "leaf", int(bits[x, 3].view(np.int32)), -1])
+        else:
+            out.append([tuple(bits[x, [0, 1, 2, 4, 5, 6]].tolist()), "node", -1, -1])
+            st.append((int(h[x, 7:8].view(np.int32)[0]), me, 1))
+            st.append((int(h[x, 3:4].view(np.int32)[0]), me, 0))
+    return [tuple(r) for r in out]
+
+
 def _layout(tr, variant):
     L = OrcLayout()
     if variant == 2:
@@ -130,11 +213,28 @@ def _layout(tr, variant):
     return out
 
 
-def build(vertices, triangles, variant=8, bins=16, leaf_max=4):
-    """Full oracle build: SAH tree (leaf_max 1 for the CWBVH collapse), layout, Woop rows, corners."""
+def tri_boxes(vertices, triangles):
+    """Per-triangle boxes (n, 4) lo and hi of a mesh (exact min / max of the corners)."""
     v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
     t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
-    tr = build_sah(v, t, bins, 1 if variant == 8 else leaf_max)
+    c = v[t]
+    z = np.zeros((len(t), 1), np.float32)
+    return np.concatenate([c.min(1), z], 1), np.concatenate([c.max(1), z], 1)
+
+
+def build(vertices, triangles, variant=8, bins=16, leaf_max=4, builder="sah", hnodes=None, root=None):
+    """Full oracle build: tree (binned SAH, or the SAH leaf collapse of H-PLOC nodes -- the oracle's own H-PLOC build
+    unless hnodes/root are given), leaf_max 1 for the CWBVH collapse, then layout, Woop rows, corners."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    if builder == "hploc":
+        if hnodes is None:
+            hnodes, root, _ = hp_build(*tri_boxes(v, t))
+        tr = OrcTree()
+        h = np.ascontiguousarray(hnodes, dtype=np.float32)
+        lib().orc_tree_from_hploc(_f(h), len(t), int(root), 1 if variant == 8 else leaf_max, ctypes.byref(tr))
+    else:
+        tr = build_sah(v, t, bins, 1 if variant == 8 else leaf_max)
     try:
         out = tree_arrays(tr)
         out.update(_layout(tr, variant))

@@ -119,6 +119,37 @@ def test_render_matches_oracle(scene, tri, variant):
     assert np.mean(np.all(img == ref, axis=-1)) > 0.5
 
 
+@pytest.mark.parametrize("scene,variant", [("terrain", 8), ("boxes", 2)])
+def test_kernel_schedules_give_the_same_image(scene, variant):
+    """Fused trace+shade, split trace/shade, split with persistent refilling lanes, and the shadow kernel with and
+    without refilling compute every path with the same arithmetic in the same order per path: images and counters are
+    bit-identical, and match the oracle."""
+    print("case", scene, variant, flush=True)
+    spp, batch, bounces = 8, 8, 5
+    base = SCENES[scene] + ["--variant", "bvh%d" % variant, "--batch", str(batch), "--bounces", str(bounces),
+                            "--spp", str(spp)]
+    out = {}
+    schedules = [("fused", ["--fuse", "1", "--shadow_refill", "0"]), ("fused-refill", ["--fuse", "1"]),
+                 ("split", ["--fuse", "0", "--shadow_refill", "0"]), ("persistent", ["--persistent", "1"])]
+    for name, extra in schedules:
+        s = cwp.Setup(base + extra)
+        r = cwp.Renderer(s)
+        st = r.render(spp)
+        out[name] = (r.accum(), st)
+        assert r.frame_json(st)["fuse"] == (1 if name.startswith("fused") else 0)
+        r.close()
+    img0, st0 = out["fused"]
+    for name in ("fused-refill", "split", "persistent"):
+        img, st = out[name]
+        np.testing.assert_array_equal(img, img0, err_msg=name)
+        for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
+            assert st[k] == st0[k], (name, k)
+    assert out["split"][1]["t_shade_ms"] > 0 and out["fused"][1]["t_shade_ms"] == 0
+    sc, _ = po.scene_from_setup(cwp.Setup(base))
+    acc, _ = sc.render(cwp.Setup(base).params(), spp, batch=batch, bounces=bounces)
+    assert psnr(img0[..., :3] / spp, acc[..., :3] / spp) >= 50.0
+
+
 def test_render_progressive_and_deterministic():
     s = cwp.Setup(SCENES["boxes"] + ["--batch", "8", "--bounces", "3"])
     r = cwp.Renderer(s)
@@ -168,3 +199,21 @@ def test_frame_json_has_reference_keys():
               "avg_path_len", "vram_MB", "bvh_nodes", "bvh_bytes", "bvh_sah", "variant", "tri", "batch", "spp"):
         assert k in j
     assert j["rays_total"] == j["rays_closest"] + j["rays_shadow"]
+
+
+def test_renderer_from_caller_buffers_matches_setup_renderer():
+    """cwp_create_scene over the setup's own arrays renders the same image bit for bit as cwp_create."""
+    s = cwp.Setup(SCENES["boxes"] + ["--batch", "8", "--bounces", "4", "--spp", "8"])
+    a = cwp.Renderer(s)
+    a.render(8)
+    ref = a.accum()
+    a.close()
+    opts = cwp.Options.default(batch=8, bounces=4)
+    r = cwp.Renderer.from_scene(8, s.array(cwp.ARR_NODES), s.array(cwp.ARR_CORNERS), s.array(cwp.ARR_ALBEDO),
+                                s.params(), opts, woop=s.array(cwp.ARR_WOOP))
+    st = r.render(8)
+    np.testing.assert_array_equal(r.accum(), ref)
+    assert st["rays_closest"] > 0
+    with pytest.raises(cwp.CwpError, match="frame_json|null"):
+        r.frame_json(st)
+    r.close()

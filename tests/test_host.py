@@ -41,6 +41,10 @@ def test_version_and_error_string():
         cwp.Setup(["--mesh", "/nonexistent.obj", "--hfov", "40"])
     with pytest.raises(cwp.CwpError, match="no scene"):
         cwp.Setup([])
+    with pytest.raises(cwp.CwpError, match="--fuse"):
+        cwp.Setup(["--gen", "boxes", "--fuse", "2"])
+    with pytest.raises(cwp.CwpError, match="--persistent"):
+        cwp.Setup(["--gen", "boxes", "--persistent", "3"])
 
 
 def test_reference_example_fixtures(golden_dir):
@@ -135,3 +139,88 @@ def test_cli_reports_errors_without_gpu(tmp_path):
     r = subprocess.run([exe, "--gen", "boxes", "--spp", "10", "--batch", "4", "--width", "8", "--height", "8"],
                        capture_output=True, text=True)
     assert r.returncode == 1 and "multiple of batch" in r.stderr
+
+
+@pytest.mark.parametrize("gen,variant", [("boxes", 2), ("boxes", 8), ("terrain", 8)])
+def test_scene_check_accepts_built_scenes_and_rejects_bad_indices(gen, variant):
+    s = cwp.Setup(["--gen", gen, "--variant", "bvh%d" % variant, "--width", "8", "--height", "8"])
+    nodes = s.array(cwp.ARR_NODES).copy()
+    corners = s.array(cwp.ARR_CORNERS).copy()
+    albedo = s.array(cwp.ARR_ALBEDO).copy()
+    woop = s.array(cwp.ARR_WOOP)
+    need = cwp.scene_check(variant, nodes, corners, albedo, woop)
+    assert need == s.info["stack_need"]
+    ints = nodes.view(np.int32)
+    nn, nt = len(nodes), len(corners)
+
+    def rejected(n=nodes, c=corners, a=albedo, match=None):
+        with pytest.raises(cwp.CwpError, match=match):
+            cwp.scene_check(variant, n, c, a, woop)
+
+    if variant == 2:
+        internal = np.nonzero((ints[:, 14] == 0) & (ints[:, 12] > 0))[0]
+        leaf = np.nonzero(ints[:, 14] > 0)[0]
+        bad = ints.copy(); bad[internal[0], 12] = nn; rejected(bad.view(np.float32), match="child index")
+        bad = ints.copy(); bad[internal[1], 12] = ints[internal[0], 12]; rejected(bad.view(np.float32), match="more than one parent")
+        bad = ints.copy(); bad[internal[-1], 12] = 0; rejected(bad.view(np.float32), match="parent")
+        bad = ints.copy(); bad[leaf[0], 12] = nt - 1; bad[leaf[0], 14] = 2; rejected(bad.view(np.float32), match="triangle range")
+        bad = ints.copy(); bad[internal[0], 12] = -1; rejected(bad.view(np.float32), match="empty slot")
+    else:
+        ew = ints[:, 3].view(np.uint32)
+        with_kids = np.nonzero(ew >> 24)[0]
+        bad = ints.copy(); bad[with_kids[-1], 4] = 0; rejected(bad.view(np.float32), match="parent")
+        bad = ints.copy(); bad[with_kids[0], 4] = nn; rejected(bad.view(np.float32), match="outside the node array")
+        bad = ints.copy(); bad.view(np.uint32)[with_kids[0], 3] ^= np.uint32(0xff << 24)
+        rejected(bad.view(np.float32), match="mask")
+        meta = ints[:, 6:8].view(np.uint8).reshape(nn, 8)
+        with_leaf = np.nonzero((meta & 0x80).any(axis=1))[0]
+        bad = ints.copy(); bad[with_leaf[0], 5] = nt; rejected(bad.view(np.float32), match="triangle range")
+    bad_c = corners.copy(); bad_c.view(np.int32)[5, 1, 3] = len(albedo); rejected(c=bad_c, match="material")
+    rejected(c=corners[:0], match="required")
+
+
+def test_scene_check_accepts_the_reference_edge_form():
+    s = cwp.Setup(["--gen", "boxes", "--width", "8", "--height", "8"])
+    c = s.array(cwp.ARR_CORNERS)
+    edges = c.copy()
+    edges[:, 1, :3] = c[:, 1, :3] - c[:, 0, :3]
+    edges[:, 2, :3] = c[:, 2, :3] - c[:, 0, :3]
+    need = cwp.scene_check(8, s.array(cwp.ARR_NODES), edges, s.array(cwp.ARR_ALBEDO), tri_form=1)
+    assert need == s.info["stack_need"]
+    with pytest.raises(cwp.CwpError, match="tri_form"):
+        cwp.scene_check(8, s.array(cwp.ARR_NODES), edges, s.array(cwp.ARR_ALBEDO), tri_form=2)
+
+
+@pytest.mark.parametrize("variant", [2, 8])
+def test_bvh_from_hploc_matches_oracle_bit_exact(variant):
+    """The product's SAH leaf collapse of H-PLOC nodes and the layouts after it equal the oracle's, given the same
+    H-PLOC nodes (the oracle's H-PLOC restatement here; the GPU build is compared in the GPU tests)."""
+    s = cwp.Setup(["--gen", "terrain", "--gen_grid", "61", "--gen_boxes", "20", "--width", "8", "--height", "8"])
+    v, t = s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES)
+    hn, root, _ = po.hp_build(*po.tri_boxes(v, t))
+    p = cwp.bvh_from_hploc(v, t, hn, root, variant, 4)
+    o = po.build(v, t, variant, leaf_max=4, builder="hploc", hnodes=hn, root=root)
+    np.testing.assert_array_equal(p["tree"], o["tree"])
+    np.testing.assert_array_equal(p["tree_box"].view(np.uint32), o["tree_box"].view(np.uint32))
+    np.testing.assert_array_equal(p["nodes"].view(np.uint32), o["nodes"].view(np.uint32))
+    np.testing.assert_array_equal(p["order"], o["order"])
+    np.testing.assert_array_equal(p["woop"].view(np.uint32), o["woop"].view(np.uint32))
+    bad = hn.copy()
+    bad.view(np.int32)[len(t), 3] = 2 * len(t) - 2   # a child id not below its parent
+    with pytest.raises(cwp.CwpError, match="smaller than the parent"):
+        cwp.bvh_from_hploc(v, t, bad, root, variant, 4)
+
+
+def test_scene_check_accepts_hploc_bvh2_with_root_last():
+    """The H-PLOC BVH2 layout stores children before parents with the root last: accepted with its root index."""
+    s = cwp.Setup(["--gen", "boxes", "--width", "8", "--height", "8"])
+    v, t = s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES)
+    hn, root, _ = po.hp_build(*po.tri_boxes(v, t))
+    n = len(t)
+    bvh2 = po.hp_to_bvh2(hn, n)
+    corners = np.zeros((n, 3, 4), np.float32)   # the layout's leaves name primitives: corners in mesh order
+    corners[:, :, :3] = v[t]
+    need = cwp.scene_check(2, bvh2, corners, s.array(cwp.ARR_ALBEDO), root=n - 2)
+    assert need >= 2
+    with pytest.raises(cwp.CwpError, match="root has a parent"):
+        cwp.scene_check(2, bvh2, corners, s.array(cwp.ARR_ALBEDO), root=0)

@@ -317,3 +317,167 @@ def test_plane_with_sun_closed_form():
     full = np.round(f * 16) == 16
     assert full.sum() > 0.3 * img.size
     np.testing.assert_allclose(img[full], expect, rtol=5e-3)
+
+
+# ---------------------------------------------------------------------------------------------- H-PLOC restatement
+def test_expand_bits21_closed_form():
+    assert po.expand_bits21(0x1fffff) == 0x1249249249249249
+    assert po.expand_bits21(0) == 0
+    assert po.expand_bits21(1) == 1 and po.expand_bits21(2) == 8 and po.expand_bits21(0x100000) == 1 << 60
+    rng = np.random.default_rng(3)
+    for v in rng.integers(0, 1 << 21, 50):
+        want = 0
+        for b in range(21):
+            want |= ((int(v) >> b) & 1) << (3 * b)
+        assert po.expand_bits21(int(v)) == want
+
+
+def _interleave(x, y, z, mb):
+    m = 0
+    for b in range(mb):
+        m |= ((x >> b) & 1) << (3 * b + 2) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b)
+    return m
+
+
+def test_hploc_keys_against_direct_interleave():
+    rng = np.random.default_rng(5)
+    for n in (2, 7, 1000, 70000):
+        c = rng.random((n, 3)).astype(np.float32) * np.float32(3) - np.float32(1)
+        lo, hi = c - np.float32(0.01), c + np.float32(0.02)
+        box, keys, mb, ib = po.hp_keys(lo, hi)
+        assert ib == max(1, int(np.ceil(np.log2(n)))) and mb == min(21, (64 - ib) // 3)
+        cen = (lo + hi) * np.float32(0.5)
+        np.testing.assert_array_equal(box[:3], cen.min(0)); np.testing.assert_array_equal(box[3:], cen.max(0))
+        s = np.float32(1 << mb)
+        for i in rng.integers(0, n, 40):
+            q = []
+            for a in range(3):
+                e = np.float32(box[3 + a] - box[a])
+                t = np.float32(np.float32(cen[i, a] - box[a]) / e)
+                q.append(int(min(max(t * s, np.float32(0)), s - np.float32(1))))
+            assert int(keys[i]) == (_interleave(q[0], q[1], q[2], mb) << ib) | int(i)
+        assert len(np.unique(keys)) == n
+
+
+def _hp_invariants(lo, hi, nodes, root, n):
+    """2n - 1 nodes, every primitive once, parent box = union of children, children before parents."""
+    ints = nodes.view(np.int32)
+    assert root == (2 * n - 2 if n > 1 else 0)
+    seen = sorted(ints[:n, 3].tolist())
+    assert seen == list(range(n)) and (ints[:n, 7] == -1).all()
+    np.testing.assert_array_equal(nodes[:n, :3], lo[ints[:n, 3]]); np.testing.assert_array_equal(nodes[:n, 4:7], hi[ints[:n, 3]])
+    kids = ints[n:, [3, 7]]
+    assert (kids < np.arange(n, 2 * n - 1)[:, None]).all()
+    assert sorted(kids.ravel().tolist()) == list(range(2 * n - 2))   # every node but the root has one parent
+    np.testing.assert_array_equal(nodes[n:, :3], np.minimum(nodes[kids[:, 0], :3], nodes[kids[:, 1], :3]))
+    np.testing.assert_array_equal(nodes[n:, 4:7], np.maximum(nodes[kids[:, 0], 4:7], nodes[kids[:, 1], 4:7]))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 32, 33, 500, 20000])
+def test_hploc_tree_invariants(n):
+    rng = np.random.default_rng(n)
+    c = rng.random((n, 3)).astype(np.float32)
+    lo, hi = c - np.float32(0.005), c + rng.random((n, 3)).astype(np.float32) * np.float32(0.02)
+    nodes, root, keys = po.hp_build(lo, hi)
+    assert (np.diff(keys.astype(np.uint64)) > 0).all()
+    _hp_invariants(lo, hi, nodes, root, n)
+
+
+def _half_area(lo, hi):
+    d = (hi - lo).astype(np.float32)
+    return np.float32(np.float32(d[0] * d[1]) + np.float32(d[1] * d[2])) + np.float32(d[2] * d[0])
+
+
+@pytest.mark.parametrize("n", [5, 9, 16, 17])
+def test_hploc_small_trees_are_sequences_of_mutual_nearest_merges(n):
+    """n <= 17: no LBVH node below the root holds more than 16 primitives, so the whole build is one PLOC window at
+    the root over the leaves in Morton order. Replaying it from the Morton-ordered leaves, every
+    internal node (in id order) must join two current clusters that are each other's nearest neighbour within 8
+    positions by merged surface area (ties by position), and the survivors keep their order."""
+    rng = np.random.default_rng(100 + n)
+    c = rng.random((n, 3)).astype(np.float32)
+    lo, hi = c - np.float32(0.01), c + np.float32(0.01)
+    nodes, root, _ = po.hp_build(lo, hi)
+    ints = nodes.view(np.int32)
+    clusters = list(range(n))   # node ids in lane order
+    box = {i: (nodes[i, :3].copy(), nodes[i, 4:7].copy()) for i in range(2 * n - 1)}
+
+    def best_pairs(cl):
+        m = len(cl)
+        best = [None] * m
+        for a in range(m):
+            for r in range(1, 9):
+                b = a + r
+                if b >= m:
+                    break
+                l = np.minimum(box[cl[a]][0], box[cl[b]][0]); h = np.maximum(box[cl[a]][1], box[cl[b]][1])
+                key = (int(_half_area(l, h).view(np.uint32)), a, b)
+                for x in (a, b):
+                    if best[x] is None or key < best[x]:
+                        best[x] = key
+        return best
+
+    nid = n
+    while len(clusters) > 1:
+        best = best_pairs(clusters)
+        nn = [bk[2] if bk[1] == k else bk[1] for k, bk in enumerate(best)]
+        merged, keep = {}, []
+        for k in range(len(clusters)):
+            p = nn[k]
+            if nn[p] == k and k < p:
+                assert ints[nid, 3] == clusters[k] and ints[nid, 7] == clusters[p], (nid, k, p)
+                merged[k] = nid
+                nid += 1
+        for k in range(len(clusters)):
+            p = nn[k]
+            if nn[p] == k and k > p:
+                continue
+            keep.append(merged.get(k, clusters[k]))
+        assert merged, "an iteration must merge at least one pair"
+        clusters = keep
+    assert clusters == [root]
+
+
+def test_tree_from_hploc_sah_collapse_against_brute_force():
+    rng = np.random.default_rng(8)
+    n = 300
+    c = rng.random((n, 3)).astype(np.float32)
+    lo, hi = c - np.float32(0.01), c + np.float32(0.03)
+    nodes, root, _ = po.hp_build(lo, hi)
+    ints = nodes.view(np.int32)
+
+    def area(h):
+        d = nodes[h, 4:7] - nodes[h, :3]
+        return 2.0 * float(np.float32(np.float32(d[0] * d[1]) + np.float32(d[1] * d[2])) + np.float32(d[2] * d[0]))
+
+    memo = {}
+
+    def best(h):   # (cost, leaf?, count) by recursion from the definition
+        if h in memo:
+            return memo[h]
+        if h < n:
+            r = (area(h), True, 1)
+        else:
+            a, b = best(ints[h, 3]), best(ints[h, 7])
+            cnt = a[2] + b[2]
+            inner = area(h) + a[0] + b[0]
+            leaf = area(h) * cnt if cnt <= 4 else 1e300
+            r = (min(leaf, inner), leaf <= inner, cnt)
+        memo[h] = r
+        return r
+
+    import sys
+    sys.setrecursionlimit(10000)
+    t = po.tree_from_hploc(nodes, n, root, 4)
+    tree = t["tree"]
+    # every output leaf is a subtree that the recursion marks as leaf, and the leaves cover every primitive once
+    assert sorted(t["prim"].tolist()) == list(range(n))
+    leaves = tree[tree[:, 3] > 0]
+    assert (leaves[:, 3] <= 4).all()
+    cost_tree = 0.0
+    for k in range(len(tree)):
+        lo_, hi_ = t["tree_box"][k, :3], t["tree_box"][k, 3:]
+        d = hi_ - lo_
+        A = 2.0 * float(np.float32(np.float32(d[0] * d[1]) + np.float32(d[1] * d[2])) + np.float32(d[2] * d[0]))
+        cost_tree += A * tree[k, 3] if tree[k, 3] > 0 else A
+    assert cost_tree == pytest.approx(best(root)[0], rel=1e-12)
