@@ -43,6 +43,19 @@ def algorithmic_bytes(st, npaths):
     return 16.0 * npaths + 52.0 * cont_traced + 32.0 * cont_miss + 52.0 * st["rays_cont"] + 48.0 * st["rays_shadow"]
 
 
+def measured_traffic(kernel_tag):
+    """HBM-side bytes per launch of the dominant kernel from the committed rocprof summary (FETCH_SIZE + WRITE_SIZE,
+    each divided by its plain-copy calibration factor), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_bounce_terrain720p_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_per_launch_bytes " + kernel_tag + ":"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(args):
     """The oracle's multi-threaded CPU renderer on a bounded sample of the same workload (time-boxed)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -94,7 +107,7 @@ def main():
     ap.add_argument("--fuse", type=int, default=1, help="1: fused trace+shade kernel; 0: split trace and shade kernels")
     ap.add_argument("--persistent", type=int, default=0, help="1: split kernels with lanes refilled mid-traversal")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
-    ap.add_argument("--cpu-spp", type=int, default=16)
+    ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy_mib > 0:
@@ -161,7 +174,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
                      else "k_trace_paths + k_shade (split trace / shade)",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic("k_bounce") if args.fuse and not args.persistent else None,
+                     "traffic_source": "profiles/k_bounce_terrain720p_traffic.md (rocprofv3 FETCH_SIZE / 0.5 + WRITE_SIZE / 1.0)",
                      "bytes_per_frame": bytes_bounce / args.steps,
                      "avg_launch_ms": round(bounce_ms / max(1, launches), 4)},
         "latency": None,

@@ -69,7 +69,7 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     memset(info, 0, sizeof *info);
     info->variant = Z.variant; info->tri_test = Z.O.tri;
     info->num_tris = (int)Z.M.f.size(); info->num_verts = (int)Z.M.v.size(); info->num_materials = (int)Z.M.kd.size();
-    info->node_floats4 = Z.variant == 2 ? 4 : 5; info->num_nodes = (int)Z.nNodes;
+    info->node_floats4 = nodeFloat4s(Z.variant); info->num_nodes = (int)Z.nNodes;
     info->bvh_depth = Z.maxDepth; info->stack_need = Z.S.stackNeed; info->sah = Z.sah; info->bvh_build_s = Z.bvhS;
     info->width = Z.A.w; info->height = Z.A.h; info->spp = Z.A.spp; info->batch = Z.A.batch; info->bounces = Z.A.bounces;
     return CWP_OK;
@@ -180,12 +180,12 @@ void cwp_options_default(cwp_options* o) {
 }
 
 static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& err) {
-    if (d->variant != 2 && d->variant != 8) { err = "variant must be 2 or 8"; return false; }
+    if (!nodeFloat4s(d->variant)) { err = "variant must be 2, 4 or 8"; return false; }
     if (!d->nodes || !d->num_nodes || !d->corners || !d->num_tris || !d->albedo || !d->num_materials) {
         err = "nodes, corners and albedo are required"; return false;
     }
     if (d->num_nodes > 0x7fffffff || d->num_tris > 0x7fffffff || d->num_materials > 0x7fffffff) { err = "arrays too large"; return false; }
-    const size_t per = d->variant == 2 ? 4 : 5;
+    const size_t per = nodeFloat4s(d->variant);
     auto take = [](const float* p, size_t n4, std::vector<Vec4>& out) {
         out.resize(n4);
         if (n4) memcpy(out.data(), p, n4 * sizeof(Vec4));
@@ -256,15 +256,21 @@ static bool meshFromArrays(const float* vertices, int nv, const int* tris, int n
 // layout, triangle packs and flat tree arrays of b->tree
 static void finishBVH(const Mesh& m, int variant, cwp_bvh* b) {
     b->variant = variant;
-    if (variant == 2) {
-        layoutBVH2(b->tree, b->nodes); b->order = b->tree.prim; b->sah = sahCost(b->tree, 1, 1);
-        b->stackNeed = bvh2Depth(b->nodes) + 1;
+    if (variant == 2 || variant == 4) {
+        if (variant == 2) layoutBVH2(b->tree, b->nodes); else layoutBVH4(b->tree, b->nodes);
+        b->order = b->tree.prim; b->sah = sahCost(b->tree, 1, 1);
     } else {
         WideBVH W; collapseCWBVH(b->tree, W);
-        b->nodes.swap(W.nodes); b->order.swap(W.prim); b->sah = W.sah; b->stackNeed = W.depth + 1;
+        b->nodes.swap(W.nodes); b->order.swap(W.prim); b->sah = W.sah;
     }
     packWoop(m, b->order, b->woop);
     packVerts(m, b->order, b->corners, b->nrm);
+    SceneUpload S;   // the traversal-stack need of the layout, as the renderer will compute it
+    S.variant = variant; S.nodes = b->nodes; S.verts = b->corners;
+    for (Vec4& c : S.verts) c.w = 0.f;   // corners carry ids in w; only the material slot is checked
+    S.albedo.assign(1, v4(1.f, 1.f, 1.f, 0.f));
+    std::string err;
+    if (!checkScene(S, b->stackNeed, err)) b->stackNeed = -1;
     for (const BinNode& n : b->tree.nodes) {
         b->treeFlat.insert(b->treeFlat.end(), { n.left, n.right, n.first, n.count });
         b->treeBox.insert(b->treeBox.end(), { n.box.lo.x, n.box.lo.y, n.box.lo.z, n.box.hi.x, n.box.hi.y, n.box.hi.z });
@@ -273,7 +279,7 @@ static void finishBVH(const Mesh& m, int variant, cwp_bvh* b) {
 
 int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int variant, int bins, int leafMax, cwp_bvh** out) {
     if (!out) return fail(CWP_ERR_ARG, "cwp_bvh_build: null argument");
-    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_build: variant must be 2 or 8");
+    if (!nodeFloat4s(variant)) return fail(CWP_ERR_ARG, "cwp_bvh_build: variant must be 2, 4 or 8");
     if (bins < 2 || bins > 256 || leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_build: bins in [2,256], leaf_max in [1,32]");
     *out = nullptr;
     Mesh m; std::string err;
@@ -289,7 +295,7 @@ int cwp_bvh_build(const float* vertices, int nv, const int* tris, int nt, int va
 int cwp_bvh_from_hploc(const float* vertices, int nv, const int* tris, int nt, const float* hnodes, int root, int variant,
                        int leafMax, cwp_bvh** out) {
     if (!out || !hnodes) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: null argument");
-    if (variant != 2 && variant != 8) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: variant must be 2 or 8");
+    if (!nodeFloat4s(variant)) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: variant must be 2, 4 or 8");
     if (leafMax < 1 || leafMax > 32) return fail(CWP_ERR_ARG, "cwp_bvh_from_hploc: leaf_max in [1,32]");
     *out = nullptr;
     Mesh m; std::string err;
@@ -380,7 +386,7 @@ void cwp_bvh_destroy(cwp_bvh* b) { delete b; }
 int cwp_bvh_array(const cwp_bvh* b, int which, const void** data, size_t* count) {
     if (!b || !data || !count) return fail(CWP_ERR_ARG, "cwp_bvh_array: null argument");
     switch (which) {
-        case CWP_ARR_NODES: *data = b->nodes.data(); *count = b->nodes.size() / (b->variant == 2 ? 4 : 5); break;
+        case CWP_ARR_NODES: *data = b->nodes.data(); *count = b->nodes.size() / nodeFloat4s(b->variant); break;
         case CWP_ARR_ORDER: *data = b->order.data(); *count = b->order.size(); break;
         case CWP_ARR_WOOP: *data = b->woop.data(); *count = b->woop.size() / 3; break;
         case CWP_ARR_CORNERS: *data = b->corners.data(); *count = b->corners.size() / 3; break;

@@ -41,6 +41,7 @@ CWP_HD Vec3 vmin(Vec3 a, Vec3 b) { return v3(fminf(a.x, b.x), fminf(a.y, b.y), f
 CWP_HD Vec3 vmax(Vec3 a, Vec3 b) { return v3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 CWP_HD float maxc(Vec3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 CWP_HD float comp(Vec3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+CWP_HD float comp(const Vec4& a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : (k == 2 ? a.z : a.w)); }
 
 // bit casts
 CWP_HD float asFloat(int i) { float f; memcpy(&f, &i, 4); return f; }

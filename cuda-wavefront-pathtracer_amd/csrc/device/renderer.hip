@@ -65,11 +65,13 @@ struct DevPaths {
 
 template <int V> struct StackEntry;
 template <> struct StackEntry<2> { using type = int; };
+template <> struct StackEntry<4> { using type = int; };
 template <> struct StackEntry<8> { using type = Group; };
 
 template <int V, bool ANY, int TRI, class St>
 __device__ __forceinline__ bool traceScene(const DevScene& S, const RayPre& r, Hit& h, St& st) {
     if constexpr (V == 2) return traceBVH2<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
+    else if constexpr (V == 4) return traceBVH4<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
     else return traceBVH8<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
 }
 
@@ -283,6 +285,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
         if (i >= 0) {
             bool done;
             if constexpr (V == 2) done = stepBVH2<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else if constexpr (V == 4) done = stepBVH4<false, TRI>(S.nodes, S.tri, r, h, st, ts);
             else done = stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts);
             if (done) { P.H[i] = v4(h.t, h.u, h.v, asFloat(h.tri)); i = -1; }
         }
@@ -390,6 +393,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_refill(DevScene S, DevPath
         if (k >= 0) {
             bool done;
             if constexpr (V == 2) done = stepBVH2<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else if constexpr (V == 4) done = stepBVH4<true, TRI>(S.nodes, S.tri, r, h, st, ts);
             else done = stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts);
             if (done) {
                 if (h.tri < 0) {
@@ -454,13 +458,30 @@ __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot) 
     tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
 }
 
-// accum[pix] += sum over the K samples of the pass (path ids pix*K .. pix*K + K-1), in sample order
-__global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K) {
-    for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < N; pix += gridDim.x * blockDim.x) {
+// accum[pix] += sum over the K samples of the pass (path ids pix*K .. pix*K + K-1), in sample order. One wave per 64
+// pixels: the K samples of a pixel are contiguous, so the wave loads chunks of 16 samples of 4 pixels per instruction
+// (4 runs of 256 B) into LDS, then each lane adds its pixel's samples in order.
+constexpr int kRedChunk = 16;
+__global__ void __launch_bounds__(kWave) k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K) {
+    __shared__ float buf[kWave][3 * kRedChunk + 1];   // +1: rows 49 words apart, no bank conflicts between lanes
+    const int lane = laneId();
+    for (int p0 = blockIdx.x * kWave; p0 < N; p0 += gridDim.x * kWave) {
         float r = 0.f, g = 0.f, b = 0.f;
-        const Vec4* l = L + (size_t)pix * K;
-        for (int s = 0; s < K; s++) { const Vec4 v = l[s]; r += v.x; g += v.y; b += v.z; }
-        Vec4 a = accum[pix]; a.x += r; a.y += g; a.z += b; accum[pix] = a;
+        for (int s0 = 0; s0 < K; s0 += kRedChunk) {
+            const int cs = min(kRedChunk, K - s0);
+            for (int j = 0; j < kWave / 4; j++) {
+                const int p = 4 * j + (lane >> 4), s = lane & 15;
+                if (p0 + p < N && s < cs) {
+                    const Vec4 v = L[(size_t)(p0 + p) * K + s0 + s];
+                    buf[p][3 * s] = v.x; buf[p][3 * s + 1] = v.y; buf[p][3 * s + 2] = v.z;
+                }
+            }
+            __syncthreads();
+            for (int s = 0; s < cs; s++) { r += buf[lane][3 * s]; g += buf[lane][3 * s + 1]; b += buf[lane][3 * s + 2]; }
+            __syncthreads();
+        }
+        const int pix = p0 + lane;
+        if (pix < N) { Vec4 a = accum[pix]; a.x += r; a.y += g; a.z += b; accum[pix] = a; }
     }
 }
 
@@ -637,6 +658,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (O.fuse != 0 && O.fuse != 1) { err = "fuse must be 0 or 1"; return false; }
     if (O.persistent && O.fuse) { err = "persistent traversal needs the split kernels (fuse 0)"; return false; }
     if (S.variant == 2) pickKernels<2>(O.tri, I.sl, O.bounceWaves, I.kn);
+    else if (S.variant == 4) pickKernels<4>(O.tri, I.sl, O.bounceWaves, I.kn);
     else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
     I.bounce = O.fuse ? I.kn.bounce : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
     I.shade = O.fuse ? nullptr : k_shade;
@@ -719,7 +741,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
             I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
             if (I.O.timeKernels) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
         }
-        k_reduce<<<std::min((I.N + 255) / 256, 8192), 256, 0, I.st>>>(I.P.L, I.accum, I.N, I.K);
+        k_reduce<<<std::min((I.N + kWave - 1) / kWave, 16384), kWave, 0, I.st>>>(I.P.L, I.accum, I.N, I.K);
         k_pass_end<<<1, 64, 0, I.st>>>(I.P.cnt, I.depths, I.tot);
     }
     HIPCHK(hipGetLastError());

@@ -202,6 +202,68 @@ __device__ __forceinline__ bool traceBVH2(const Vec4* __restrict__ nodes, const 
     return h.tri >= 0;
 }
 
+// ------------------------------------------------------------------------------------------------ BVH4
+// 8 float4 per node (128 B, one cache line): lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of the 4 children (child j in
+// component j), child int4 (node index; first triangle when the count is > 0; -1 = empty slot), count int4.
+// One node visit: the 4 slabs against the hit distance at entry (near / far planes chosen by the ray's direction
+// signs: bit-identical to min / max), the hit leaves in slot order, then the internal hits that can still hold a closer
+// hit, nearest first (sorted by entry distance with a 5-comparator network; the far ones are pushed farthest first).
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool stepBVH4(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, const RayPre& r,
+                                         Hit& h, Stack& st, TravState& ts) {
+    const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
+    const Vec4* nb = nodes + (size_t)ts.node * 8;
+    const Vec4 lx = ldg4(nb), hx = ldg4(nb + 1), ly = ldg4(nb + 2), hy = ldg4(nb + 3), lz = ldg4(nb + 4), hz = ldg4(nb + 5);
+    const Vec4 cf = ldg4(nb + 6), kf = ldg4(nb + 7);
+    const Vec4 nx = r.pos & 1u ? lx : hx, fx = r.pos & 1u ? hx : lx;
+    const Vec4 ny = r.pos & 2u ? ly : hy, fy = r.pos & 2u ? hy : ly;
+    const Vec4 nz = r.pos & 4u ? lz : hz, fz = r.pos & 4u ? hz : lz;
+    const float tEntry = h.t;
+    float tn[4];
+    bool hit[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float xn = fmaf(comp(nx, j), r.ix, -odx), xf = fmaf(comp(fx, j), r.ix, -odx);
+        const float yn = fmaf(comp(ny, j), r.iy, -ody), yf = fmaf(comp(fy, j), r.iy, -ody);
+        const float zn = fmaf(comp(nz, j), r.iz, -odz), zf = fmaf(comp(fz, j), r.iz, -odz);
+        tn[j] = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
+        hit[j] = tn[j] <= fminf(fminf(xf, yf), fminf(zf, tEntry));
+    }
+    const int c[4] = { asInt(cf.x), asInt(cf.y), asInt(cf.z), asInt(cf.w) };
+    const int k[4] = { asInt(kf.x), asInt(kf.y), asInt(kf.z), asInt(kf.w) };
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (hit[j] && k[j] > 0)
+            for (int t = c[j]; t < c[j] + k[j]; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+    constexpr float kInf = 3.4e38f;
+    float ht[4];
+    int hc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) { ht[j] = hit[j] && k[j] == 0 && tn[j] <= h.t ? tn[j] : kInf; hc[j] = c[j]; }
+    auto cswap = [&](int a, int b) {
+        if (ht[b] < ht[a]) { const float tf = ht[a]; ht[a] = ht[b]; ht[b] = tf; const int ti = hc[a]; hc[a] = hc[b]; hc[b] = ti; }
+    };
+    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+    if (ht[0] == kInf) {
+        if (st.sp == 0) return true;
+        ts.node = st.pop();
+        return false;
+    }
+    if (ht[3] < kInf) st.push(hc[3]);
+    if (ht[2] < kInf) st.push(hc[2]);
+    if (ht[1] < kInf) st.push(hc[1]);
+    ts.node = hc[0];
+    return false;
+}
+
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool traceBVH4(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, int root,
+                                          const RayPre& r, Hit& h, Stack& st) {
+    TravState ts; travBegin(ts, root);
+    while (!stepBVH4<ANY, TRI>(nodes, tri, r, h, st, ts)) {}
+    return h.tri >= 0;
+}
+
 // ------------------------------------------------------------------------------------------------ compressed BVH8
 __device__ __forceinline__ uint32_t byteOf(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
 

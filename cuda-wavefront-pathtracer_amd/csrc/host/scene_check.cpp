@@ -62,6 +62,32 @@ bool checkBVH2(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::stri
     return treeDepth(G, nn, S.root, depth, err);
 }
 
+bool checkBVH4(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
+    Links G; G.first.assign(nn + 1, 0);
+    for (size_t j = 0; j < nn; j++) {
+        const Vec4* n = &S.nodes[j * 8];
+        G.first[j] = (int)G.kids.size();
+        for (int s = 0; s < 4; s++) {
+            const int c = asInt(comp(n[6], s)), k = asInt(comp(n[7], s));
+            const std::string at = "BVH4 node " + std::to_string(j) + " child " + std::to_string(s);
+            if (k < 0) return fail(err, at + ": negative triangle count");
+            if (k > 0) {
+                if (c < 0 || (size_t)c + (size_t)k > nt) return fail(err, at + ": triangle range out of bounds");
+                continue;
+            }
+            if (c == -1) {
+                for (int q = 0; q < 6; q++)
+                    if (!(comp(n[q], s) >= kEmptyBox)) return fail(err, at + ": empty slot without the empty box");
+                continue;
+            }
+            if (c < 0 || (size_t)c >= nn) return fail(err, at + ": child index out of range");
+            G.kids.push_back(c);
+        }
+    }
+    G.first[nn] = (int)G.kids.size();
+    return treeDepth(G, nn, S.root, depth, err);
+}
+
 bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
     Links G; G.first.assign(nn + 1, 0);
     for (size_t j = 0; j < nn; j++) {
@@ -94,8 +120,8 @@ bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::stri
 }  // namespace
 
 bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
-    if (S.variant != 2 && S.variant != 8) return fail(err, "variant must be 2 or 8");
-    const size_t per = S.variant == 2 ? 4 : 5;
+    const size_t per = nodeFloat4s(S.variant);
+    if (!per) return fail(err, "variant must be 2, 4 or 8");
     if (S.nodes.empty() || S.nodes.size() % per) return fail(err, "node array size is not a whole number of nodes");
     if (S.root < 0 || (size_t)S.root >= S.nodes.size() / per) return fail(err, "root node out of range");
     if (S.verts.empty() || S.verts.size() % 3) return fail(err, "corner array size is not a whole number of triangles");
@@ -110,9 +136,11 @@ bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
             return fail(err, "triangle " + std::to_string(t) + ": material id out of range");
     }
     int depth = 0;
-    if (!(S.variant == 2 ? checkBVH2(S, nn, nt, depth, err) : checkBVH8(S, nn, nt, depth, err))) return false;
-    // a traversal holds at most one deferred entry per level below the root, plus one
-    stackNeed = depth + 1;
+    const bool ok = S.variant == 2 ? checkBVH2(S, nn, nt, depth, err)
+                                   : (S.variant == 4 ? checkBVH4(S, nn, nt, depth, err) : checkBVH8(S, nn, nt, depth, err));
+    if (!ok) return false;
+    // a traversal defers at most one entry per level below the root (BVH2, BVH8 groups) or three (BVH4), plus one
+    stackNeed = (S.variant == 4 ? 3 : 1) * depth + 1;
     return true;
 }
 

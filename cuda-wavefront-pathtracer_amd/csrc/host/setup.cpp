@@ -1,6 +1,7 @@
 // host/setup.cpp - option parsing and scene setup (see host/setup.h).
 #include "host/setup.h"
 #include "device/hploc.h"
+#include "host/scene_check.h"
 #include "host/io.h"
 #include <chrono>
 #include <cmath>
@@ -79,7 +80,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         } else { err = "unknown arg " + k; return false; }
         if (!ok) return false;
     }
-    if (A.variant != "bvh2" && A.variant != "bvh8") { err = "--variant must be bvh2 or bvh8"; return false; }
+    if (A.variant != "bvh2" && A.variant != "bvh4" && A.variant != "bvh8") { err = "--variant must be bvh2, bvh4 or bvh8"; return false; }
     if (A.tri != "mt" && A.tri != "woop" && A.tri != "watertight") { err = "--tri must be mt, woop or watertight"; return false; }
     if (A.integrator != "auto" && A.integrator != "legacy") { err = "--integrator must be auto or legacy"; return false; }
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
@@ -162,7 +163,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     blackmanHarrisTable(H.ftab, kFilterTable, 1.5);
     // ---- BVH ----
     auto t0 = std::chrono::steady_clock::now();
-    const int V = Z.variant = A.variant == "bvh2" ? 2 : 8;
+    const int V = Z.variant = A.variant == "bvh2" ? 2 : (A.variant == "bvh4" ? 4 : 8);
     const int leafMax = V == 8 ? 1 : A.leaf;   // the CWBVH collapse forms its own leaves from a 1-primitive-leaf tree
     Z.reportLeaf = V == 8 ? 3 : leafMax;
     if (A.builder == "hploc") {
@@ -187,14 +188,15 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     Z.maxDepth = Z.tree.maxDepth;
     SceneUpload& S = Z.S;
     S.variant = V; S.root = 0;
-    if (V == 2) {
+    if (V == 4) {
+        layoutBVH4(Z.tree, S.nodes); S.order = Z.tree.prim; Z.sah = sahCost(Z.tree, 1, 1);
+        if (Z.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
+    } else if (V == 2) {
         layoutBVH2(Z.tree, S.nodes); S.order = Z.tree.prim; Z.sah = sahCost(Z.tree, 1, 1);
-        S.stackNeed = bvh2Depth(S.nodes) + 1;
         if (Z.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
     } else {
         WideBVH W; collapseCWBVH(Z.tree, W);
         S.nodes.swap(W.nodes); S.order.swap(W.prim); Z.sah = W.sah;
-        S.stackNeed = W.depth + 1;
     }
     std::vector<Vec4> nrm;
     packVerts(M, S.order, S.verts, S.normals);
@@ -202,7 +204,8 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     S.albedo.clear();
     for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
     Z.bvhS = msSince(t0) / 1000.0;
-    Z.nNodes = S.nodes.size() / (V == 2 ? 4 : 5);
+    Z.nNodes = S.nodes.size() / nodeFloat4s(V);
+    if (!checkScene(S, S.stackNeed, err)) return false;
     Z.nodeBytes = S.nodes.size() * 16;
     // ---- renderer options ----
     RenderOptions& O = Z.O;

@@ -90,7 +90,7 @@ def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_for
     keep = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
             for a in (nodes, corners, woop, normals, albedo)]
     n, c, w, nr, al = keep
-    per = 16 if variant == 2 else 20
+    per = {2: 16, 4: 32, 8: 20}[variant]
     d = SceneDesc(variant, n.ctypes.data, n.size // per, c.ctypes.data, c.size // 12,
                   w.ctypes.data if w is not None else None, nr.ctypes.data if nr is not None else None,
                   al.ctypes.data, al.size // 4, tri_form, root)
@@ -325,7 +325,7 @@ def bvh_build(vertices, triangles, variant=8, bins=16, leaf_max=4):
 def _bvh_result(h, variant):
     try:
         out = {}
-        nf4 = 4 if variant == 2 else 5
+        nf4 = {2: 4, 4: 8, 8: 5}[variant]
         for name, which in (("nodes", ARR_NODES), ("order", ARR_ORDER), ("woop", ARR_WOOP), ("corners", ARR_CORNERS),
                             ("tree", ARR_TREE), ("tree_box", ARR_TREE_BOX), ("prim", ARR_PRIM)):
             ptr, cnt = ctypes.c_void_p(), ctypes.c_size_t()

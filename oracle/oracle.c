@@ -289,6 +289,52 @@ int orc_layout_bvh2(const orc_tree* t, orc_layout* out) {
 }
 void orc_layout_free(orc_layout* l) { free(l->nodes); free(l->order); memset(l, 0, sizeof *l); }
 
+/* ------------------------------------------------------------------- BVH4 layout: src/accel/bvh_layouts.cpp:32-63 */
+/* children: start from the two children, then repeatedly open the internal child of largest surface area (first of
+ * equal areas) until there are 4 or only leaves remain; pre-order node ids */
+static int b4_emit(const orc_tree* t, int n, fvec* out, int level, int* depth) {
+    int id = out->n / 32;
+    fgrow(out, out->n + 32);
+    if (level > *depth) *depth = level;
+    int ch[4], nc = 2;
+    ch[0] = b2_left(t, n); ch[1] = b2_right(t, n);
+    while (nc < 4) {
+        int best = -1; float bestA = -1.f;
+        for (int j = 0; j < nc; j++)
+            if (ch[j] >= 0 && !b2_leaf(t, ch[j])) {
+                float a = box_area(tree_box(t, ch[j]));
+                if (a > bestA) { bestA = a; best = j; }
+            }
+        if (best < 0) break;
+        int nb = ch[best];
+        ch[best] = t->node[4 * nb]; ch[nc++] = t->node[4 * nb + 1];
+    }
+    int c[4], k[4];
+    box3 bx[4];
+    for (int j = 0; j < 4; j++) {
+        if (j >= nc || ch[j] < 0) { c[j] = -1; k[j] = 0; bx[j].lo = bx[j].hi = mk(3e37f, 3e37f, 3e37f); continue; }
+        bx[j] = tree_box(t, ch[j]);
+        if (b2_leaf(t, ch[j])) { c[j] = t->node[4 * ch[j] + 2]; k[j] = t->node[4 * ch[j] + 3]; }
+        else { k[j] = 0; c[j] = b4_emit(t, ch[j], out, level + 1, depth); }
+    }
+    float* o = out->p + 32 * id;
+    for (int j = 0; j < 4; j++) {
+        o[j] = bx[j].lo.x; o[4 + j] = bx[j].hi.x; o[8 + j] = bx[j].lo.y; o[12 + j] = bx[j].hi.y;
+        o[16 + j] = bx[j].lo.z; o[20 + j] = bx[j].hi.z; o[24 + j] = i2f(c[j]); o[28 + j] = i2f(k[j]);
+    }
+    return id;
+}
+int orc_layout_bvh4(const orc_tree* t, orc_layout* out) {
+    fvec v = { 0, 0, 0 };
+    int depth = 0;
+    int root = t->node[3] > 0 ? t->num_nodes : 0;
+    b4_emit(t, root, &v, 1, &depth);
+    out->num_float4 = v.n / 4; out->nodes = v.p; out->num_prims = t->num_prims;
+    out->order = (int*)malloc(sizeof(int) * t->num_prims); memcpy(out->order, t->prim, sizeof(int) * t->num_prims);
+    out->depth = depth; out->max_leaf = 0; out->sah = 0;
+    return 0;
+}
+
 /* ------------------------------------------------------- CWBVH collapse: src/accel/bvh_tools.cpp:75-168 (Ylitie 2017) */
 enum { D_LEAF = 0, D_NODE = 1, D_PREV = 2, D_DIST = 10 };
 typedef struct {
@@ -663,8 +709,53 @@ static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
     }
 }
 
+/* ---------------------------------------------------------------- BVH4 traversal: src/accel/traverse.cuh:72-107 */
+/* the 4 child slabs against the hit distance at node entry, the hit leaves in slot order, then the internal hits that
+ * can still hold a closer hit (entry distance <= current hit distance), nearest first: sorted by entry distance with
+ * the reference's 5-comparator network (ties keep the network's order), the far ones pushed farthest first */
+static int trav4(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    int stack[256], sp = 0, node = s->root;
+    const float odx = r->o.x * r->ix, ody = r->o.y * r->iy, odz = r->o.z * r->iz;
+    const float INF = 3.4e38f;
+    for (;;) {
+        const float* n = s->nodes + 32 * (size_t)node;
+        if (nv) *nv += 1;
+        const float tEntry = h->t;
+        float tn[4];
+        int hit[4], c[4], k[4];
+        for (int j = 0; j < 4; j++) {
+            float x0 = fmaf(n[j], r->ix, -odx), x1 = fmaf(n[4 + j], r->ix, -odx);
+            float y0 = fmaf(n[8 + j], r->iy, -ody), y1 = fmaf(n[12 + j], r->iy, -ody);
+            float z0 = fmaf(n[16 + j], r->iz, -odz), z1 = fmaf(n[20 + j], r->iz, -odz);
+            tn[j] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r->tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tEntry));
+            hit[j] = tn[j] <= tf;
+            c[j] = f2i(n[24 + j]); k[j] = f2i(n[28 + j]);
+        }
+        for (int j = 0; j < 4; j++)
+            if (hit[j] && k[j] > 0)
+                for (int t = c[j]; t < c[j] + k[j]; t++) { if (nt) *nt += 1; if (tri_test(s, t, r, h) && any) return 1; }
+        float ht[4];
+        int hc[4];
+        for (int j = 0; j < 4; j++) { ht[j] = hit[j] && k[j] == 0 && tn[j] <= h->t ? tn[j] : INF; hc[j] = c[j]; }
+        static const int net[5][2] = { { 0, 1 }, { 2, 3 }, { 0, 2 }, { 1, 3 }, { 1, 2 } };
+        for (int q = 0; q < 5; q++) {
+            int a = net[q][0], b = net[q][1];
+            if (ht[b] < ht[a]) { float tf = ht[a]; ht[a] = ht[b]; ht[b] = tf; int ti = hc[a]; hc[a] = hc[b]; hc[b] = ti; }
+        }
+        if (ht[0] == INF) { if (sp == 0) break; node = stack[--sp]; continue; }
+        if (ht[3] < INF) stack[sp++] = hc[3];
+        if (ht[2] < INF) stack[sp++] = hc[2];
+        if (ht[1] < INF) stack[sp++] = hc[1];
+        node = hc[0];
+    }
+    return h->tri >= 0;
+}
+
 static int trace_one(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
-    return s->variant == 2 ? trav2(s, r, h, any, nv, nt) : trav8(s, r, h, any, nv, nt);
+    if (s->variant == 2) return trav2(s, r, h, any, nv, nt);
+    if (s->variant == 4) return trav4(s, r, h, any, nv, nt);
+    return trav8(s, r, h, any, nv, nt);
 }
 
 /* --------------------------------------------------------------------------------------------- threading helper */

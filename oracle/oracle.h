@@ -32,7 +32,7 @@ typedef struct orc_params {
 } orc_params;
 
 typedef struct orc_scene {
-    int variant;            /* 2: BVH2 64 B nodes, 8: compressed BVH8 80 B nodes */
+    int variant;            /* 2: BVH2 64 B nodes, 4: BVH4 128 B nodes, 8: compressed BVH8 80 B nodes */
     int tri_test;           /* 0 Moller-Trumbore, 1 Woop rows, 2 watertight */
     const float* nodes;     /* float4 array */
     int root;
@@ -73,6 +73,7 @@ int orc_build_sah(const float* v, int nv, const int* tri, int nt, int bins, int 
 void orc_tree_free(orc_tree* t);
 int orc_layout_bvh2(const orc_tree* t, orc_layout* out);
 int orc_collapse_cwbvh(const orc_tree* t, orc_layout* out);
+int orc_layout_bvh4(const orc_tree* t, orc_layout* out);
 void orc_layout_free(orc_layout* l);
 void orc_pack_woop(const float* v, const int* tri, const int* order, int n, float* rows);
 void orc_pack_corners(const float* v, const int* tri, const int* mat, const int* order, int n, float* corners);

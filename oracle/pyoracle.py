@@ -51,6 +51,7 @@ def lib():
         L.orc_tree_free.argtypes = [ctypes.POINTER(OrcTree)]
         L.orc_layout_bvh2.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcLayout)]
         L.orc_collapse_cwbvh.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcLayout)]
+        L.orc_layout_bvh4.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcLayout)]
         L.orc_layout_free.argtypes = [ctypes.POINTER(OrcLayout)]
         L.orc_pack_woop.argtypes = [_FP, _IP, _IP, ctypes.c_int, _FP]
         L.orc_pack_corners.argtypes = [_FP, _IP, _IP, _IP, ctypes.c_int, _FP]
@@ -203,11 +204,13 @@ def _layout(tr, variant):
     L = OrcLayout()
     if variant == 2:
         lib().orc_layout_bvh2(ctypes.byref(tr), ctypes.byref(L))
+    elif variant == 4:
+        lib().orc_layout_bvh4(ctypes.byref(tr), ctypes.byref(L))
     else:
         lib().orc_collapse_cwbvh(ctypes.byref(tr), ctypes.byref(L))
     nodes = np.ctypeslib.as_array(L.nodes, shape=(L.num_float4 * 4,)).copy()
     order = np.ctypeslib.as_array(L.order, shape=(L.num_prims,)).copy()
-    out = dict(nodes=nodes.reshape(-1, 16 if variant == 2 else 20), order=order, layout_depth=L.depth,
+    out = dict(nodes=nodes.reshape(-1, {2: 16, 4: 32, 8: 20}[variant]), order=order, layout_depth=L.depth,
                max_leaf=L.max_leaf, sah=L.sah)
     lib().orc_layout_free(ctypes.byref(L))
     return out

@@ -63,7 +63,7 @@ SCENES = {
 
 
 @pytest.mark.parametrize("scene", ["boxes", "terrain"])
-@pytest.mark.parametrize("variant", [8, 2])
+@pytest.mark.parametrize("variant", [8, 4, 2])
 @pytest.mark.parametrize("tri", ["woop", "watertight", "mt"])
 def test_trace_matches_oracle(scene, variant, tri):
     print("case", scene, variant, tri, flush=True)
@@ -98,7 +98,7 @@ def test_trace_empty_and_single():
 
 
 @pytest.mark.parametrize("scene,tri,variant", [("boxes", "woop", 8), ("boxes", "watertight", 2), ("terrain", "woop", 8),
-                                               ("terrain", "watertight", 8)])
+                                               ("terrain", "watertight", 8), ("terrain", "woop", 4)])
 def test_render_matches_oracle(scene, tri, variant):
     print("case", scene, tri, variant, flush=True)
     spp, batch, bounces = 16, 8, 4
@@ -119,7 +119,7 @@ def test_render_matches_oracle(scene, tri, variant):
     assert np.mean(np.all(img == ref, axis=-1)) > 0.5
 
 
-@pytest.mark.parametrize("scene,variant", [("terrain", 8), ("boxes", 2)])
+@pytest.mark.parametrize("scene,variant", [("terrain", 8), ("boxes", 2), ("boxes", 4)])
 def test_kernel_schedules_give_the_same_image(scene, variant):
     """Fused trace+shade, split trace/shade, split with persistent refilling lanes, and the shadow kernel with and
     without refilling compute every path with the same arithmetic in the same order per path: images and counters are

