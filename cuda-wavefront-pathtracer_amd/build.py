@@ -21,7 +21,8 @@ LIB = os.path.join(PKG, "libcwp.so")
 CLI = os.path.join(PKG, "pt")
 
 HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/bvh.cpp", "host/scene_check.cpp", "host/setup.cpp", "capi.cpp"]
-HIP_SOURCES = ["device/renderer.hip", "device/hploc.hip"]
+HIP_SOURCES = ["device/renderer.hip", "device/kernels_v2.hip", "device/kernels_v4.hip", "device/kernels_v8.hip",
+               "device/hploc.hip"]
 CLI_SOURCES = ["app/pt.cpp"]
 
 INCLUDES = ["-I" + CSRC, "-I" + os.path.join(ROOT, "include")]

@@ -1,0 +1,446 @@
+// device/kernels.h - device code of the renderer: constants, path-state and scene records, shading, and the kernel
+// templates per (BVH layout V, triangle test TRI, LDS stack SL). Each layout's kernels are instantiated in exactly one
+// translation unit (kernels_v2.hip, kernels_v4.hip, kernels_v8.hip) through pickKernels<V>; renderer.hip holds the
+// layout-independent kernels and the host driver.
+//
+// Design (MI355X-first, not the reference's launch structure):
+//  * Persistent grids: every kernel launches a fixed grid of single-wave workgroups sized to the occupancy limit and
+//    walks its queue in wave-sized chunks (grid stride). Queue lengths are read from device memory, so a frame is
+//    enqueued once, with no count readback between bounces.
+//  * Queue compaction per wave: the continuation and shadow decisions are ballots; one lane reserves the wave's block
+//    of slots with a single atomic and every lane writes at its mbcnt rank (reference: per-thread atomicAdd appends,
+//    src/render/wavefront.cu:119,135).
+//  * Bounce 0 generates camera rays in registers (reference: separate k_raygen, src/render/wavefront.cu:42-58), so
+//    camera rays never round-trip through HBM.
+//  * Radiance is accumulated per path (pL) with plain read-modify-writes: each path has at most one shadow ray per
+//    bounce, so no atomics are needed; a final kernel sums the K samples of each pixel in sample order. Results are
+//    therefore independent of scheduling (the reference's float atomics into the pixel are not,
+//    src/render/wavefront.cu:71,100).
+//  * Traversal stacks live in LDS with spill to a per-thread global slab (device/traverse.h).
+// Shading restates the reference's legacy shadeOne (src/render/wavefront.cu:94-136): sky on a miss, stop past maxB,
+// geometric (and optional interpolated) normal faced to the ray, origin offset eps, sun next-event estimation by a
+// uniform cone sample with contribution T*albedo*sunE*cos/pi, cosine-weighted continuation rejected below the
+// geometric surface, throughput *= albedo.
+#pragma once
+#include "core/sampling.h"
+#include "device/renderer.h"
+#include "device/traverse.h"
+#include <hip/hip_runtime.h>
+
+namespace cwp {
+namespace dev {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 64;       // one wave per workgroup: LDS stacks are sized per wave
+static_assert(kBlock == kStackStride, "the LDS stack is strided by the workgroup size");
+constexpr int kParts = 8;        // queue partitions, one per XCD (workgroup w runs on XCD w % 8)
+// Per-depth counters: one 128-byte line per partition, so the atomics of different partitions never meet on a line.
+// Line x: [kCQueue] path-queue length of partition x, [kCShadow] shadow rays it produced, [kCMiss] its misses,
+// [kCUnocc] its unoccluded shadow rays. Misses and unoccluded rays are summed per wave and added once at kernel end.
+constexpr int kLine = 32;
+constexpr int kCntStride = kParts * kLine;
+constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
+
+struct DevScene {
+    const Vec4* nodes;    // BVH layout (BVH8: 128-byte node stride)
+    const Vec4* tri;      // traversal triangle data (Woop rows or corners)
+    const Vec4* verts;    // corners for shading
+    const Vec4* nrm;      // per-corner normals or null
+    const Vec4* albedo;   // per material
+    int root;
+};
+
+// Path state. Path id i = pixel * K + sample: the 64 lanes of a wave start as 64 samples of one pixel, and each
+// partition's slice of the id space is a band of the image.
+struct DevPaths {
+    Vec4 *O, *D, *T, *L;  // origin, direction, throughput, radiance per path id
+    Vec4* H;              // closest hit of the current bounce (t, u, v, triangle bits) when tracing and shading are split
+    int* q[2];            // ping-pong queues of path ids: depth d reads q[d & 1]
+    Vec4 *sO, *sD, *sC;   // shadow rays of the current bounce: (origin, path id bits), (dir, tmax), (contribution)
+    int* cnt;             // kCntStride counters per depth (one line per partition)
+    void* spill;          // traversal-stack spill slab, spillCap entries per thread
+    int spillCap;
+};
+
+template <int V> struct StackEntry;
+template <> struct StackEntry<2> { using type = int; };
+template <> struct StackEntry<4> { using type = int; };
+template <> struct StackEntry<8> { using type = Group; };
+
+template <int V, bool ANY, int TRI, class St>
+__device__ __forceinline__ bool traceScene(const DevScene& S, const RayPre& r, Hit& h, St& st) {
+    if constexpr (V == 2) return traceBVH2<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
+    else if constexpr (V == 4) return traceBVH4<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
+    else return traceBVH8<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
+}
+
+__device__ __forceinline__ int laneId() { return (int)__lane_id(); }
+
+// Reserves popcount(mask) consecutive slots of *counter for the lanes in mask (one atomic per wave) and returns this
+// lane's slot. Must be called by the whole wave with a wave-uniform, non-zero mask.
+__device__ __forceinline__ int waveAppend(uint64_t mask, int* counter) {
+    const int leader = __builtin_ctzll(mask);
+    int base = 0;
+    if (laneId() == leader) base = atomicAdd(counter, (int)__builtin_popcountll(mask));
+    base = __builtin_amdgcn_readlane(base, leader);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    return base + rank;
+}
+
+// adds a wave-uniform count to *counter once per wave (nothing when it is zero)
+__device__ __forceinline__ void waveAdd(int count, int* counter) {
+    if (count && laneId() == 0) atomicAdd(counter, count);
+}
+
+// counter `which` of partition `part` in a depth's counter block
+__device__ __forceinline__ int* ctr(int* cnt, int part, int which) { return cnt + part * kLine + which; }
+__device__ __forceinline__ int ctrv(const int* cnt, int part, int which) { return cnt[part * kLine + which]; }
+
+// start of partition `part` in a region laid out as the partitions' queue lengths of a depth's counter block
+__device__ __forceinline__ int partStart(const int* cnt, int part) {
+    int s = 0;
+    for (int y = 0; y < part; y++) s += ctrv(cnt, y, kCQueue);
+    return s;
+}
+
+// camera ray of path (pix, smp): filter importance sampling (Blackman-Harris table or 1 px box), pinhole or thin lens
+// (reference: src/render/wavefront.cu:43-57)
+__device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_t smp, Vec3& o, Vec3& d) {
+    const Camera& c = F.cam;
+    const int x = pix % c.w, y = pix / c.w;
+    const float u0 = rnd((uint32_t)pix, smp, 0, 0, F.seed), u1 = rnd((uint32_t)pix, smp, 0, 1, F.seed);
+    float jx, jy;
+    if (F.filterBH) { jx = tableLookup(F.ftab, kFilterTable, u0); jy = tableLookup(F.ftab, kFilterTable, u1); }
+    else { jx = u0 - 0.5f; jy = u1 - 0.5f; }
+    const float dx = ((float)x + 0.5f + jx - c.px) / c.fx, dy = -((float)y + 0.5f + jy - c.py) / c.fy;
+    const Vec3 dir = c.ax * dx + c.ay * dy - c.az;
+    d = normalize(dir);
+    o = c.pos;
+    if (c.lensR > 0.f) {
+        float lx, ly;
+        concentricDisk(rnd((uint32_t)pix, smp, 0, 2, F.seed), rnd((uint32_t)pix, smp, 0, 3, F.seed), lx, ly);
+        Vec3 pf = c.pos + dir * c.focusD;
+        o = c.pos + c.ax * (lx * c.lensR) + c.ay * (ly * c.lensR);
+        d = normalize(pf - o);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------- kernels
+// Legacy shading of one closest hit of path i (reference: shadeOne, src/render/wavefront.cu:94-136). Writes the
+// path's radiance on a miss (depth 0 also initialises it on a hit) and returns the continuation (cont: po, nd, Tn) and
+// the shadow ray (shadow: po, sdir, scon) for the caller's wave-level appends.
+struct ShadeOut { bool cont, shadow, miss; Vec3 po, nd, Tn, sdir, scon; };
+
+__device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
+                                                int i, int pix, uint32_t smp, Vec3 o, Vec3 d, Vec3 T, const Hit& h) {
+    ShadeOut out;
+    out.cont = false; out.shadow = false; out.miss = false;
+    out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po;
+    if (h.tri < 0) {
+        out.miss = true;
+        Vec3 c = T * F.sky;
+        if (depth == 0) P.L[i] = v4(c, 0.f);
+        else { Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l; }
+        return out;
+    }
+    if (depth == 0) P.L[i] = v4(0.f, 0.f, 0.f, 0.f);
+    if (depth > F.maxB) return out;
+    const Vec3 p = o + d * h.t;
+    const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
+    const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
+    Vec3 ng = normalize(cross(e1, e2)), ns = ng;
+    if (F.hasN) {
+        Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
+        Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
+        float l = length(nn);
+        if (l > 0.f) ns = nn * (1.f / l);
+        if (dot(ns, ng) < 0.f) ns = -ns;
+    }
+    if (dot(ng, d) > 0.f) { ng = -ng; ns = -ns; }
+    const Vec3 alb = xyz(S.albedo[asInt(b.w)]);
+    out.po = p + ng * F.eps;
+    const uint32_t dd = (uint32_t)depth + 1u;
+    const Vec3 l = uniformCone(F.sunTo, F.sunCos, rnd((uint32_t)pix, smp, dd, 2, F.seed), rnd((uint32_t)pix, smp, dd, 3, F.seed));
+    const float cl = dot(ns, l);
+    if (cl > 0.f && dot(ng, l) > 0.f) { out.shadow = true; out.sdir = l; out.scon = T * alb * F.sunE * (cl * kInvPi); }
+    out.nd = cosineHemisphere(ns, rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed));
+    if (dot(out.nd, ng) > 0.f) { out.cont = true; out.Tn = T * alb; }
+    return out;
+}
+
+// Wave-level appends of the shading results: continuations to partition `part` of the next queue, shadow rays to
+// partition `part` of the shadow queue. Returns the wave's number of misses.
+__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int* qOut, int i,
+                                             const ShadeOut& r) {
+    const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
+    if (mc) {
+        const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
+        if (r.cont) { qOut[pos] = i; P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, 0.f); P.T[i] = v4(r.Tn, 0.f); }
+    }
+    if (ms) {
+        const int pos = outStart + waveAppend(ms, ctr(cnt, part, kCShadow));
+        if (r.shadow) { P.sO[pos] = v4(r.po, asFloat(i)); P.sD[pos] = v4(r.sdir, 1e30f); P.sC[pos] = v4(r.scon, 0.f); }
+    }
+    return (int)__builtin_popcountll(mm);
+}
+
+// The ray of path i at this depth: its camera ray at depth 0 (recomputed from the path id), else the stored state.
+__device__ __forceinline__ void pathRay(const DevPaths& P, const FrameParams& F, int depth, int i, int smpBase, int K,
+                                        Vec3& o, Vec3& d, float& tmin) {
+    if (depth == 0) { cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
+    else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+}
+
+// Queue position of the f-th entry a wave hands out: the wave walks the chunks lw, lw + nw, lw + 2 nw, ... of its
+// partition, 64 entries each (increasing in f).
+__device__ __forceinline__ int fetchPos(uint32_t f, int lw, int nw) { return (lw + (int)(f >> 6) * nw) * kWave + (int)(f & 63u); }
+
+// idle lanes of a persistent wave are refilled once at least this many are idle (kWave: only when all are, which is
+// the plain one-ray-per-lane schedule)
+constexpr int kRefill = 16;
+
+// One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here),
+// traversal and shading fused. The queue is split into kParts partitions, one per XCD; the waves of XCD x walk
+// partition x, and append their continuations to partition x of the next queue and their shadow rays to partition x of
+// the shadow queue, so a path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue
+// starts where the depth-d partitions before x end, which bounds it by the rays partition x traced.
+// WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
+template <int V, int TRI, int SL, int WAVES>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const int* qIn = P.q[depth & 1];
+    int* qOut = P.q[(depth + 1) & 1];
+    int misses = 0;
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
+        const int k = base + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            Vec3 o, d; float tmin;
+            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+            const RayPre r = prepareRay<TRI>(o, d, tmin);
+            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
+            st.sp = 0;
+            traceScene<V, false, TRI>(S, r, h, st);
+            // throughput is fetched after the traversal so that no registers hold it across the traversal loop
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// Closest hits of the depth-`depth` queue with persistent lanes and dynamic ray fetching (after Aila & Laine 2009;
+// reference: persistentTrace, src/render/wavefront.cu:403-434): a lane whose ray finishes stays idle until at least
+// REFILL lanes of its wave are idle, then they all take the next entries of the wave's share of its partition, so
+// long rays do not leave the rest of the wave empty. Hits go to H[path id] for k_shade.
+template <int V, int TRI, int SL, int REFILL>
+__global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    const int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int* qIn = P.q[depth & 1];
+    int i = -1;                       // path id of this lane's ray, -1 = idle
+    RayPre r; Hit h; TravState ts;
+    uint32_t f = 0;                   // wave-uniform: entries of the wave's share handed out so far
+    bool more = fetchPos(0, lw, nw) < n;
+    for (;;) {
+        const uint64_t idle = __ballot(i < 0);
+        if (more && (int)__builtin_popcountll(idle) >= REFILL) {
+            if (i < 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int k = fetchPos(f + rank, lw, nw);
+                if (k < n) {
+                    i = depth == 0 ? inStart + k : qIn[inStart + k];
+                    Vec3 o, d; float tmin;
+                    pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+                    r = prepareRay<TRI>(o, d, tmin);
+                    h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
+                    travBegin(ts, S.root);
+                    st.sp = 0;
+                }
+            }
+            f += (uint32_t)__builtin_popcountll(idle);
+            more = fetchPos(f, lw, nw) < n;
+        }
+        if (__ballot(i >= 0) == 0) break;   // no ray in flight: with `more` false the wave's share is done
+        if (i >= 0) {
+            bool done;
+            if constexpr (V == 2) done = stepBVH2<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else if constexpr (V == 4) done = stepBVH4<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else done = stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+            if (done) { P.H[i] = v4(h.t, h.u, h.v, asFloat(h.tri)); i = -1; }
+        }
+    }
+}
+
+
+// Shadow rays of bounce `depth` (partition per XCD as in k_bounce): any hit; unoccluded contributions are added to
+// the path's radiance. The any-hit traversal fits 64 registers without spilling: 8 waves per SIMD.
+template <int V, int TRI, int SL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, int depth) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCShadow);
+    const int start = partStart(cnt, part);
+    int unocc = 0;
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
+        const int k = base + laneId();
+        bool vis = false;
+        if (k < n) {
+            const Vec4 O = P.sO[start + k], D = P.sD[start + k];
+            const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
+            Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
+            st.sp = 0;
+            if (!traceScene<V, true, TRI>(S, r, h, st)) {
+                vis = true;
+                const int i = asInt(O.w);
+                const Vec4 c = P.sC[start + k];
+                Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+            }
+        }
+        unocc += (int)__builtin_popcountll(__ballot(vis));
+    }
+    waveAdd(unocc, ctr(cnt, part, kCUnocc));
+}
+
+// Shadow rays with persistent lanes and dynamic ray fetching (as k_trace_paths).
+template <int V, int TRI, int SL, int REFILL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_refill(DevScene S, DevPaths P, int depth) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCShadow);
+    const int start = partStart(cnt, part);
+    int k = -1;                       // shadow-queue entry of this lane's ray, -1 = idle
+    RayPre r; Hit h; TravState ts;
+    uint32_t f = 0;
+    bool more = fetchPos(0, lw, nw) < n;
+    int unocc = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(k < 0);
+        if (more && (int)__builtin_popcountll(idle) >= REFILL) {
+            if (k < 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int kk = fetchPos(f + rank, lw, nw);
+                if (kk < n) {
+                    k = kk;
+                    const Vec4 O = P.sO[start + k], D = P.sD[start + k];
+                    r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
+                    h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
+                    travBegin(ts, S.root);
+                    st.sp = 0;
+                }
+            }
+            f += (uint32_t)__builtin_popcountll(idle);
+            more = fetchPos(f, lw, nw) < n;
+        }
+        if (__ballot(k >= 0) == 0) break;
+        if (k >= 0) {
+            bool done;
+            if constexpr (V == 2) done = stepBVH2<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else if constexpr (V == 4) done = stepBVH4<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            else done = stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts);
+            if (done) {
+                if (h.tri < 0) {
+                    unocc++;
+                    const int i = asInt(P.sO[start + k].w);
+                    const Vec4 c = P.sC[start + k];
+                    Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+                }
+                k = -1;
+            }
+        }
+    }
+    // one atomic per wave for the unoccluded count
+    int sum = unocc;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    waveAdd(sum, ctr(cnt, part, kCUnocc));
+}
+
+// Closest or any hit of explicit rays (parity and kernel-level entry point).
+template <int V, int TRI, bool ANY, int SL>
+__global__ void __launch_bounds__(kBlock, 8) k_trace(DevScene S, const Vec4* __restrict__ rays, int n, Vec4* __restrict__ hits, void* spill, int spillCap) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)spill + (size_t)gtid * spillCap, 0 };
+    for (int k = gtid; k - (int)threadIdx.x < n; k += gridDim.x * kBlock) {
+        if (k >= n) continue;
+        const Vec4 O = rays[2 * (size_t)k], D = rays[2 * (size_t)k + 1];
+        const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), O.w);
+        Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
+        st.sp = 0;
+        traceScene<V, ANY, TRI>(S, r, h, st);
+        hits[k] = v4(h.t, h.u, h.v, asFloat(h.tri));
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------- dispatch
+using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int);
+using ShadowFn = void (*)(DevScene, DevPaths, int);
+using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, void*, int);
+
+// the kernels of one (layout, triangle test, LDS stack) configuration
+struct Kernels {
+    BounceFn bounce = nullptr;                          // fused trace + shade
+    BounceFn tracePaths = nullptr, tracePlain = nullptr; // split: closest-hit trace with / without refill (then k_shade)
+    ShadowFn shadow = nullptr, shadowRefill = nullptr;
+    TraceFn traceC = nullptr, traceA = nullptr;
+};
+
+template <int V, int TRI, int SL>
+void pickSL(int waves, Kernels& k) {
+    k.bounce = waves >= 8 ? k_bounce<V, TRI, SL, 8> : (waves >= 6 ? k_bounce<V, TRI, SL, 6> : k_bounce<V, TRI, SL, 1>);
+    k.tracePaths = k_trace_paths<V, TRI, SL, kRefill>;
+    k.tracePlain = k_trace_paths<V, TRI, SL, kWave>;
+    k.shadow = k_shadow<V, TRI, SL>;
+    k.shadowRefill = k_shadow_refill<V, TRI, SL, kRefill>;
+    k.traceC = k_trace<V, TRI, false, SL>;
+    k.traceA = k_trace<V, TRI, true, SL>;
+}
+template <int V, int TRI>
+void pickTri(int sl, int waves, Kernels& k) {
+    if (sl == 8) pickSL<V, TRI, 8>(waves, k);
+    else if (sl == 32) pickSL<V, TRI, 32>(waves, k);
+    else pickSL<V, TRI, 16>(waves, k);
+}
+// the kernels of layout V for a triangle test, LDS stack size and bounce register budget (explicitly instantiated once
+// per layout in kernels_v<V>.hip)
+template <int V>
+void pickKernels(int tri, int sl, int waves, Kernels& k) {
+    if (tri == kTriMT) pickTri<V, kTriMT>(sl, waves, k);
+    else if (tri == kTriWatertight) pickTri<V, kTriWatertight>(sl, waves, k);
+    else pickTri<V, kTriWoop>(sl, waves, k);
+}
+
+extern template void pickKernels<2>(int, int, int, Kernels&);
+extern template void pickKernels<4>(int, int, int, Kernels&);
+extern template void pickKernels<8>(int, int, int, Kernels&);
+
+}  // namespace dev
+}  // namespace cwp
