@@ -14,6 +14,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cuda-wavefront-pathtracer_amd"))
 
@@ -88,6 +90,40 @@ def calibrate_copy(mib):
                       "copies": 3}), flush=True)
 
 
+def bench_instances(args):
+    """Two-level scene: 5 meshes (1.3k-200k triangles) + ground, 1,000 instances with rigid transforms redrawn every
+    frame; per frame the TLAS is rebuilt on the GPU (H-PLOC over the instance boxes) and the frame rendered at
+    1280x720, 64 spp in flight. Reports Grays/s over the frames and the TLAS build time per frame."""
+    import cwp
+    spp = 64
+    s = cwp.Setup(["--gen", "instances", "--width", "1280", "--height", "720", "--batch", str(spp), "--spp", str(spp),
+                   "--bounces", str(args.bounces), "--variant", args.variant, "--tri", args.tri, "--filter", "box",
+                   "--stack", str(args.stack)] + LIGHTING)
+    r = cwp.Renderer(s)
+    info = s.info
+    print("bench: %d meshes, %d instances, %d BLAS nodes" % (info["num_meshes"], info["num_instances"], info["num_nodes"]),
+          file=sys.stderr, flush=True)
+    r.render(spp)   # warm-up
+    tlas, frame_ms, rays = [], [], 0.0
+    for f in range(args.frames):
+        tlas.append(r.set_frame(f))
+        st = r.render(spp)
+        frame_ms.append(st["t_frame_ms"])
+        rays += st["rays_closest"] + st["rays_shadow"]
+        if f % 10 == 0:
+            print("bench: frame %d: TLAS %.3f ms, frame %.2f ms" % (f, tlas[-1], frame_ms[-1]), file=sys.stderr, flush=True)
+    tot = sum(frame_ms)
+    out = {"metric": "Grays/s and GPU TLAS rebuild ms per frame, 1,000-instance two-level scene", "value": round(rays / (tot * 1e-3) / 1e9, 4),
+           "unit": "Grays/s", "n_gpus": 1, "steps": args.frames, "warmup": 1, "ms_per_step": round(tot / args.frames, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "instances-1000-1280x720-64spp-depth%d" % args.bounces, "meshes": info["num_meshes"],
+                      "instances": info["num_instances"], "bvh": args.variant, "tri": args.tri, "frames": args.frames},
+           "tlas_ms_mean": round(float(np.mean(tlas)), 4), "tlas_ms_p50": round(float(np.percentile(tlas, 50)), 4),
+           "tlas_ms_max": round(float(np.max(tlas)), 4)}
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -108,6 +144,10 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, help="1: split kernels with lanes refilled mid-traversal")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--cpu-spp", type=int, default=40)
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "instances"],
+                    help="terrain: the diffuse frame (headline); instances: the two-level scene with the TLAS rebuilt "
+                         "on the GPU every frame (1,000 instances of 5 meshes, 720p, 64 spp, 60 frames)")
+    ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy_mib > 0:
@@ -119,6 +159,9 @@ def main():
         sys.exit(2)
 
     import cwp
+    if args.workload == "instances":
+        bench_instances(args)
+        return
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
                                 args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill))

@@ -4,6 +4,7 @@
 #include "host/bvh.h"
 #include "host/scene_check.h"
 #include "device/hploc.h"
+#include "host/serve.h"
 #include "host/setup.h"
 #include <cstddef>
 #include <cstring>
@@ -15,12 +16,14 @@ using namespace cwp;
 struct cwp_setup { SceneSetup Z; std::vector<int> triFlat; std::vector<int> tree; std::vector<float> treeBox; };
 struct cwp_ctx { Renderer R; const cwp_setup* setup = nullptr; };
 struct cwp_hploc { HplocBuilder B; };
+struct cwp_server { Server S; };
 struct cwp_bvh {
     BinaryBVH tree; std::vector<Vec4> nodes, woop, corners, nrm; std::vector<int> order, treeFlat; std::vector<float> treeBox;
     int variant = 2, stackNeed = 0; double sah = 0;
 };
 
 static_assert(sizeof(cwp_params) == sizeof(FrameParams), "cwp_params must mirror FrameParams");
+static_assert(sizeof(cwp_instance) == sizeof(InstanceRecord), "cwp_instance must mirror InstanceRecord");
 static_assert(offsetof(cwp_params, sun_to) == offsetof(FrameParams, sunTo), "cwp_params layout");
 static_assert(offsetof(cwp_params, seed) == offsetof(FrameParams, seed), "cwp_params layout");
 static_assert(offsetof(cwp_params, ftab) == offsetof(FrameParams, ftab), "cwp_params layout");
@@ -72,6 +75,8 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     info->node_floats4 = nodeFloat4s(Z.variant); info->num_nodes = (int)Z.nNodes;
     info->bvh_depth = Z.maxDepth; info->stack_need = Z.S.stackNeed; info->sah = Z.sah; info->bvh_build_s = Z.bvhS;
     info->width = Z.A.w; info->height = Z.A.h; info->spp = Z.A.spp; info->batch = Z.A.batch; info->bounces = Z.A.bounces;
+    info->num_meshes = Z.inst ? (int)Z.meshes.size() : 0; info->num_instances = Z.inst ? (int)Z.SD.instances.size() : 0;
+    info->frames = Z.frames; info->frame_start = Z.A.frameStart;
     return CWP_OK;
 }
 
@@ -97,6 +102,8 @@ int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* co
         case CWP_ARR_TREE: *data = s->tree.data(); *count = s->tree.size() / 4; break;
         case CWP_ARR_TREE_BOX: *data = s->treeBox.data(); *count = s->treeBox.size() / 6; break;
         case CWP_ARR_PRIM: *data = Z.tree.prim.data(); *count = Z.tree.prim.size(); break;
+        case CWP_ARR_MESH_ROOTS: *data = Z.S.meshRoot.data(); *count = Z.S.meshRoot.size(); break;
+        case CWP_ARR_MESH_BOXES: *data = Z.S.meshBox.data(); *count = Z.S.meshBox.size() / 2; break;
         default: return fail(CWP_ERR_ARG, "cwp_setup_array: unknown array");
     }
     return CWP_OK;
@@ -110,7 +117,45 @@ int cwp_create(const cwp_setup* s, cwp_ctx** out) {
     std::string err;
     if (!c->R.init(s->Z.S, s->Z.P, s->Z.O, err)) { delete c; return fail(CWP_ERR_DEVICE, err); }
     c->setup = s;
+    if (s->Z.inst) {
+        std::vector<InstanceRecord> rec;
+        frameInstances(s->Z, s->Z.A.frameStart, rec);
+        double ms = 0;
+        if (!c->R.setInstances(rec.data(), (int)rec.size(), ms, err)) { delete c; return fail(CWP_ERR_DEVICE, err); }
+    }
     *out = c;
+    return CWP_OK;
+}
+
+int cwp_setup_instances(const cwp_setup* s, int frame, cwp_instance* out, int cap, int* count) {
+    if (!s || !count || (cap > 0 && !out) || frame < 0) return fail(CWP_ERR_ARG, "cwp_setup_instances: bad arguments");
+    std::vector<InstanceRecord> rec;
+    if (!frameInstances(s->Z, frame, rec)) return fail(CWP_ERR_STATE, "cwp_setup_instances: the scene is not two-level");
+    *count = (int)rec.size();
+    if (cap > 0) memcpy(out, rec.data(), std::min<size_t>((size_t)cap, rec.size()) * sizeof(InstanceRecord));
+    return CWP_OK;
+}
+
+int cwp_set_instances(cwp_ctx* c, const cwp_instance* inst, int n, double* tlas_ms) {
+    if (!c || !inst || n < 1) return fail(CWP_ERR_ARG, "cwp_set_instances: bad arguments");
+    double ms = 0; std::string err;
+    if (!c->R.setInstances((const InstanceRecord*)inst, n, ms, err))
+        return fail(err.find("hip") != std::string::npos ? CWP_ERR_DEVICE : CWP_ERR_ARG, err);
+    if (tlas_ms) *tlas_ms = ms;
+    return CWP_OK;
+}
+
+int cwp_set_frame(cwp_ctx* c, int frame, double* tlas_ms) {
+    if (!c || !c->setup || frame < 0) return fail(CWP_ERR_ARG, "cwp_set_frame: needs a context made from a setup");
+    std::vector<InstanceRecord> rec;
+    if (!frameInstances(c->setup->Z, frame, rec)) return fail(CWP_ERR_STATE, "cwp_set_frame: the scene is not two-level");
+    return cwp_set_instances(c, (const cwp_instance*)rec.data(), (int)rec.size(), tlas_ms);
+}
+
+int cwp_trace_host_inst(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits, int* inst) {
+    if (!c || (n > 0 && (!rays || !hits))) return fail(CWP_ERR_ARG, "cwp_trace_host_inst: null argument");
+    std::string err;
+    if (!c->R.traceHost(rays, n, any_hit != 0, hits, err, inst)) return fail(CWP_ERR_DEVICE, err);
     return CWP_OK;
 }
 
@@ -403,6 +448,128 @@ int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stackNeed, double* sah) {
     if (depth) *depth = b->tree.maxDepth;
     if (stackNeed) *stackNeed = b->stackNeed;
     if (sah) *sah = b->sah;
+    return CWP_OK;
+}
+
+
+static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
+    o->variant = d.batch.variant; o->tri_test = d.batch.tri; o->stack_lds = d.batch.stackLds; o->bounces = d.batch.bounces;
+    o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;
+    o->batched = d.batched ? 1 : 0; o->rays_per_lane = d.batch.raysPerLane; o->use_graph = d.batch.useGraph;
+}
+
+void cwp_server_options_default(cwp_server_options* o) {
+    if (o) serverOptionsOut(ServeOptions(), o);
+}
+
+int cwp_server_get_options(const cwp_server* s, cwp_server_options* o) {
+    if (!s || !o) return fail(CWP_ERR_ARG, "cwp_server_get_options: null argument");
+    serverOptionsOut(s->S.options(), o);
+    return CWP_OK;
+}
+
+int cwp_server_create(const cwp_server_options* o, cwp_server** out) {
+    if (!o || !out) return fail(CWP_ERR_ARG, "cwp_server_create: null argument");
+    *out = nullptr;
+    ServeOptions so;
+    so.batch.variant = o->variant; so.batch.tri = o->tri_test; so.batch.stackLds = o->stack_lds; so.batch.bounces = o->bounces;
+    so.batch.maxPaths = o->max_paths; so.batch.maxJobs = o->max_jobs; so.sceneBudgetMB = o->scene_budget_mb;
+    so.batched = o->batched != 0; so.batch.raysPerLane = o->rays_per_lane; so.batch.useGraph = o->use_graph;
+    cwp_server* s = new (std::nothrow) cwp_server();
+    if (!s) return fail(CWP_ERR_ARG, "out of host memory");
+    std::string err;
+    if (!s->S.init(so, err)) { delete s; return fail(err.find("hip") != std::string::npos || err.find("device") != std::string::npos ? CWP_ERR_DEVICE : CWP_ERR_ARG, err); }
+    *out = s;
+    return CWP_OK;
+}
+
+void cwp_server_destroy(cwp_server* s) { delete s; }
+
+int cwp_server_add_scene(cwp_server* s, const cwp_setup* setup, int* scene_id, size_t* scene_bytes) {
+    if (!s || !setup || !scene_id) return fail(CWP_ERR_ARG, "cwp_server_add_scene: null argument");
+    std::string err;
+    const int id = s->S.addScene(setup->Z, err);
+    if (id < 0) return fail(CWP_ERR_ARG, err);
+    *scene_id = id;
+    if (scene_bytes) *scene_bytes = s->S.sceneBytes(id);
+    return CWP_OK;
+}
+
+int cwp_server_submit(cwp_server* s, int scene_id, int width, int height, int spp, int bounces, uint32_t seed,
+                      double arrival_s, int* job_id) {
+    if (!s || !job_id) return fail(CWP_ERR_ARG, "cwp_server_submit: null argument");
+    ServeJob J;
+    J.scene = scene_id; J.w = width; J.h = height; J.spp = spp; J.bounces = bounces; J.seed = seed; J.arrival = arrival_s;
+    std::string err;
+    const int id = s->S.submit(J, err);
+    if (id < 0) return fail(CWP_ERR_ARG, err);
+    *job_id = id;
+    return CWP_OK;
+}
+
+int cwp_server_run(cwp_server* s, int batched) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_run: null server");
+    s->S.setBatched(batched != 0);
+    std::string err;
+    if (!s->S.run(err)) return fail(CWP_ERR_DEVICE, err);
+    return CWP_OK;
+}
+
+int cwp_server_job(const cwp_server* s, int job_id, cwp_job_info* info) {
+    if (!s || !info || job_id < 0 || job_id >= s->S.numJobs()) return fail(CWP_ERR_ARG, "cwp_server_job: bad job");
+    const ServeJob& J = s->S.job(job_id);
+    info->scene = J.scene; info->width = J.w; info->height = J.h; info->spp = J.spp; info->bounces = J.bounces;
+    info->samples_per_pass = J.K; info->passes = J.passes;
+    info->arrival_s = J.arrival; info->admitted_s = J.admitted; info->finished_s = J.finished;
+    info->latency_s = J.finished >= 0 ? J.finished - J.arrival : -1;
+    return CWP_OK;
+}
+
+int cwp_server_image(const cwp_server* s, int job_id, float* rgba, size_t num_pixels) {
+    if (!s || !rgba || job_id < 0 || job_id >= s->S.numJobs()) return fail(CWP_ERR_ARG, "cwp_server_image: bad job");
+    const ServeJob& J = s->S.job(job_id);
+    if (J.finished < 0) return fail(CWP_ERR_STATE, "cwp_server_image: the job has not finished");
+    if (num_pixels < (size_t)J.w * J.h) return fail(CWP_ERR_ARG, "cwp_server_image: buffer too small");
+    memcpy(rgba, J.image.data(), J.image.size() * sizeof(float));
+    return CWP_OK;
+}
+
+int cwp_server_stats(const cwp_server* s, cwp_serve_stats* st) {
+    if (!s || !st) return fail(CWP_ERR_ARG, "cwp_server_stats: null argument");
+    const ServeStats& x = s->S.stats();
+    st->jobs = x.jobs; st->passes = x.passes; st->scenes_loaded = x.scenesLoaded; st->scenes_evicted = x.scenesEvicted;
+    st->wall_s = x.wallS; st->rays = x.rays; st->rays_per_s = x.raysPerS; st->latency_p50_s = x.latP50;
+    st->latency_p99_s = x.latP99; st->latency_mean_s = x.latMean; st->max_paths_in_flight = x.maxPathsInFlight;
+    return CWP_OK;
+}
+
+int cwp_server_clear_jobs(cwp_server* s) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_clear_jobs: null server");
+    s->S.clearJobs();
+    return CWP_OK;
+}
+
+int cwp_server_reset_jobs(cwp_server* s) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_reset_jobs: null server");
+    s->S.resetJobs();
+    return CWP_OK;
+}
+
+int cwp_server_autotune(cwp_server* s, const int* variants, int nv, const int* stacks, int ns, const int* rpl, int nr,
+                        cwp_tune_trial* trials, int max_trials, int* num_trials, cwp_server_options* chosen) {
+    if (!s || !variants || !stacks || !rpl || nv < 1 || ns < 1 || nr < 1 || (max_trials > 0 && !trials))
+        return fail(CWP_ERR_ARG, "cwp_server_autotune: bad arguments");
+    std::vector<TuneTrial> T;
+    std::string err;
+    const bool ok = s->S.autotune(std::vector<int>(variants, variants + nv), std::vector<int>(stacks, stacks + ns),
+                                  std::vector<int>(rpl, rpl + nr), T, err);
+    if (num_trials) *num_trials = (int)T.size();
+    for (int k = 0; k < (int)T.size() && k < max_trials; k++) {
+        trials[k].variant = T[k].variant; trials[k].stack_lds = T[k].stackLds; trials[k].rays_per_lane = T[k].raysPerLane;
+        trials[k].rays_per_s = T[k].raysPerS; trials[k].wall_s = T[k].wallS;
+    }
+    if (!ok) return fail(CWP_ERR_DEVICE, err);
+    if (chosen) serverOptionsOut(s->S.options(), chosen);
     return CWP_OK;
 }
 

@@ -1,0 +1,72 @@
+// device/batch.h - many render jobs in one wavefront pass (plain C++ interface; kernels in kernels.h).
+//
+// Scenes are uploaded once and stay resident until removed; a job is (scene, frame parameters, image, samples per
+// pass). One pass traces K_j samples of every pixel of every job given: the jobs' paths are concatenated into one id
+// space, so each bounce is one launch of the trace+shade kernel and one of the shadow kernel for the whole batch, and
+// small jobs fill the GPU together. Each path's arithmetic, and each pixel's summation order, are those of the job
+// rendered alone: a job's image does not depend on what it is batched with. The job table is copied to the device
+// ahead of each pass and the kernels read the pass's sizes from it, so the pass's launches (counter reset, trace+shade
+// and shadow per bounce, reduction, totals) are the same for every pass: they are captured once in a HIP graph per
+// table slot and replayed.
+#pragma once
+#include "device/renderer.h"
+#include <string>
+#include <vector>
+
+namespace cwp {
+
+struct BatchOptions {
+    int variant = 8;             // BVH layout of every scene (2, 4 or 8)
+    int tri = kTriWoop;
+    int stackLds = 8;            // 8, 16 or 32
+    size_t maxPaths = 1 << 26;   // path-state slots (paths in flight per pass, all jobs together)
+    int maxJobs = 256;           // jobs per pass
+    int bounces = 8;             // deepest bounce limit of any job
+    int raysPerLane = 1;         // 1, 2 or 4: the persistent grid is 1/raysPerLane of the resident waves, so each lane
+                                 // traces that many times more rays in sequence
+    int useGraph = 1;            // replay each pass's launches from a captured HIP graph
+};
+
+struct BatchJob {
+    int scene = -1;
+    FrameParams F;               // camera (w, h), lighting, bounce limit, seed, filter
+    int K = 1;                   // samples per pixel this pass
+    int smpBase = 0;             // sample index of the pass's first sample
+    void* accum = nullptr;       // device float4 per pixel (from allocImage)
+};
+
+class BatchRenderer {
+public:
+    BatchRenderer();
+    ~BatchRenderer();
+    BatchRenderer(const BatchRenderer&) = delete;
+    BatchRenderer& operator=(const BatchRenderer&) = delete;
+
+    bool init(const BatchOptions& O, std::string& err);
+    // uploads a checked scene (same layout as the renderer's); returns its id, or -1 with err
+    int addScene(const SceneUpload& S, std::string& err);
+    void removeScene(int id);
+    size_t sceneBytes(int id) const;
+    // device bytes a scene takes when uploaded (nodes in the device layout, triangles, normals, albedo)
+    static size_t uploadBytes(const SceneUpload& S, int tri);
+    // per-pixel accumulation images (device, zeroed)
+    bool allocImage(int nPix, void** img, std::string& err);
+    void freeImage(void* img);
+    bool clearImage(void* img, int nPix, std::string& err);
+    bool readImage(const void* img, int nPix, float* rgba, std::string& err);
+    // One pass over the given jobs, queued on the batch's stream; with `sync` it waits and st.frameMs is the pass's
+    // device time. Ray counters accumulate over passes (counters()).
+    bool pass(const std::vector<BatchJob>& jobs, bool sync, FrameStats& st, std::string& err);
+    bool synchronize(std::string& err);
+    // ray counters summed over the passes since the last reset (waits for queued passes)
+    bool counters(FrameStats& total, bool reset, std::string& err);
+    const BatchOptions& options() const;
+    double vramMB = 0;
+    int gridBlocks = 0;
+
+private:
+    struct Impl;
+    Impl* p_;
+};
+
+}  // namespace cwp

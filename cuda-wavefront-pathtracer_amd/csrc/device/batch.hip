@@ -1,0 +1,303 @@
+// device/batch.hip - BatchRenderer: resident scenes and one wavefront pass over many jobs (see batch.h).
+#include "device/batch.h"
+#include "device/kernels.h"
+#include "host/scene_check.h"
+#include <algorithm>
+#include <cstddef>
+
+namespace cwp {
+
+using namespace dev;
+
+namespace {
+
+#define HIPCHK(x)                                                                                                      \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) { err = std::string(#x) + ": " + hipGetErrorString(e_); return false; }                  \
+    } while (0)
+
+struct SceneDev {
+    bool alive = false;
+    std::vector<void*> bufs;
+    DevScene S{};
+    size_t bytes = 0;
+    int stackNeed = 0;
+};
+
+constexpr int kTableSlots = 4;   // job tables in flight (passes may be queued without host synchronisation)
+
+}  // namespace
+
+struct BatchRenderer::Impl {
+    BatchOptions O;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<SceneDev> scenes;
+    DevPaths P{};
+    std::vector<void*> allocs;
+    Kernels kn;
+    int numCU = 0, grid = 0, redGrid = 0, depths = 0, sl = 8, spillCap = 0;
+    unsigned long long *tot = nullptr, *hostTot = nullptr;
+    DevJobTable* dTable[kTableSlots] = {};
+    DevJobTable* hTable[kTableSlots] = {};
+    hipEvent_t tableDone[kTableSlots] = {};
+    bool tableUsed[kTableSlots] = {};
+    int slot = 0;
+    // one graph per table slot, valid while the path state it was captured with is (pathGen)
+    hipGraphExec_t graph[kTableSlots] = {};
+    int graphGen[kTableSlots] = {};
+    int pathGen = 1;
+
+    template <class T> bool alloc(T*& p, size_t count, std::string& err) {
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, std::max<size_t>(count * sizeof(T), 16)));
+        allocs.push_back(q); p = (T*)q;
+        return true;
+    }
+    bool fill(void* p, size_t bytes, uint32_t v, std::string& err) {
+        const size_t words = bytes / 4;
+        const int g = (int)std::max<size_t>(1, std::min<size_t>((words + 255) / 256, 4096));
+        k_fill<<<g, 256, 0, st>>>((uint32_t*)p, words, v);
+        HIPCHK(hipGetLastError());
+        return true;
+    }
+    // the per-lane spill area must hold the deepest resident scene's stack beyond the LDS entries
+    bool ensureSpill(int stackNeed, std::string& err) {
+        const int cap = stackNeed > sl ? stackNeed - sl : 0;
+        if (cap <= spillCap) return true;
+        HIPCHK(hipStreamSynchronize(st));
+        if (P.spill) { hipFree(P.spill); P.spill = nullptr; }
+        HIPCHK(hipMalloc(&P.spill, (size_t)grid * kBlock * cap * sizeof(uint64_t)));
+        spillCap = cap; P.spillCap = cap;
+        pathGen++;   // captured graphs hold the old spill pointer
+        return true;
+    }
+    // the launches of a pass reading table slot s
+    void enqueue(int s) {
+        k_pass_begin_jobs<<<1, 64, 0, st>>>(P.cnt, (depths + 1) * kCntStride, dTable[s]);
+        for (int depth = 0; depth < depths; depth++) {
+            kn.bounceJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+            kn.shadowJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+        }
+        k_reduce_jobs<<<redGrid, kWave, 0, st>>>(dTable[s], P.L);
+        k_pass_end<<<1, 64, 0, st>>>(P.cnt, depths, tot);
+    }
+    bool capture(int s, std::string& err) {
+        if (graph[s] && graphGen[s] == pathGen) return true;
+        if (graph[s]) { HIPCHK(hipGraphExecDestroy(graph[s])); graph[s] = nullptr; }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue(s);
+        const hipError_t e = hipGetLastError();
+        HIPCHK(hipStreamEndCapture(st, &g));
+        if (e != hipSuccess) { hipGraphDestroy(g); err = std::string("batch: pass capture: ") + hipGetErrorString(e); return false; }
+        const hipError_t ei = hipGraphInstantiate(&graph[s], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) { graph[s] = nullptr; err = std::string("batch: graph instantiate: ") + hipGetErrorString(ei); return false; }
+        graphGen[s] = pathGen;
+        return true;
+    }
+};
+
+BatchRenderer::BatchRenderer() : p_(new Impl) {}
+BatchRenderer::~BatchRenderer() {
+    if (!p_) return;
+    if (p_->st) hipStreamSynchronize(p_->st);
+    for (SceneDev& s : p_->scenes) for (void* b : s.bufs) hipFree(b);
+    for (void* a : p_->allocs) hipFree(a);
+    if (p_->P.spill) hipFree(p_->P.spill);
+    for (int k = 0; k < kTableSlots; k++) {
+        if (p_->graph[k]) hipGraphExecDestroy(p_->graph[k]);
+        if (p_->hTable[k]) hipHostFree(p_->hTable[k]);
+        if (p_->tableDone[k]) hipEventDestroy(p_->tableDone[k]);
+    }
+    if (p_->hostTot) hipHostFree(p_->hostTot);
+    if (p_->e0) hipEventDestroy(p_->e0);
+    if (p_->e1) hipEventDestroy(p_->e1);
+    if (p_->st) hipStreamDestroy(p_->st);
+    delete p_;
+}
+
+const BatchOptions& BatchRenderer::options() const { return p_->O; }
+
+bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
+    Impl& I = *p_;
+    if (!nodeFloat4s(O.variant)) { err = "batch: variant must be 2, 4 or 8"; return false; }
+    if (O.tri < kTriMT || O.tri > kTriWatertight) { err = "batch: unknown triangle test"; return false; }
+    if (O.stackLds != 8 && O.stackLds != 16 && O.stackLds != 32) { err = "batch: stack_lds must be 8, 16 or 32"; return false; }
+    if (O.maxJobs < 1 || O.maxJobs > kMaxJobs) { err = "batch: max jobs must be in [1, " + std::to_string(kMaxJobs) + "]"; return false; }
+    if (O.maxPaths < 1 || O.maxPaths > (size_t)0x7fffffff) { err = "batch: max paths must be in [1, 2^31)"; return false; }
+    if (O.bounces < 0 || O.bounces > 64) { err = "batch: bounces must be in [0, 64]"; return false; }
+    if (O.raysPerLane != 1 && O.raysPerLane != 2 && O.raysPerLane != 4) { err = "batch: rays per lane must be 1, 2 or 4"; return false; }
+    I.O = O; I.sl = O.stackLds;
+    std::string name;
+    if (!deviceInfo(I.numCU, name, err)) return false;
+    HIPCHK(hipSetDevice(0));
+    size_t free0 = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&free0, &total));
+    HIPCHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&I.e0)); HIPCHK(hipEventCreate(&I.e1));
+    if (O.variant == 2) pickKernels<2>(O.tri, I.sl, 8, I.kn);
+    else if (O.variant == 4) pickKernels<4>(O.tri, I.sl, 8, I.kn);
+    else pickKernels<8>(O.tri, I.sl, 8, I.kn);
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)I.kn.bounceJobs, kBlock, 0) != hipSuccess || b < 1) b = 1;
+    I.grid = std::max(kParts, (b * I.numCU / O.raysPerLane) / kParts * kParts);
+    gridBlocks = I.grid;
+    I.redGrid = (int)std::min<size_t>(16384, O.maxPaths / kWave + (size_t)O.maxJobs);
+    I.depths = O.bounces + 2;
+    const size_t n = O.maxPaths;
+    if (!I.alloc(I.P.O, n, err) || !I.alloc(I.P.D, n, err) || !I.alloc(I.P.T, n, err) || !I.alloc(I.P.L, n, err) ||
+        !I.alloc(I.P.q[0], n, err) || !I.alloc(I.P.q[1], n, err) || !I.alloc(I.P.sO, n, err) || !I.alloc(I.P.sD, n, err) ||
+        !I.alloc(I.P.sC, n, err) || !I.alloc(I.P.cnt, (size_t)(I.depths + 1) * kCntStride, err) || !I.alloc(I.tot, 8, err))
+        return false;
+    HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    if (!I.fill(I.tot, 8 * sizeof(unsigned long long), 0u, err)) return false;
+    for (int k = 0; k < kTableSlots; k++) {
+        if (!I.alloc(I.dTable[k], 1, err)) return false;
+        HIPCHK(hipHostMalloc((void**)&I.hTable[k], sizeof(DevJobTable), hipHostMallocDefault));
+        HIPCHK(hipEventCreate(&I.tableDone[k]));
+    }
+    HIPCHK(hipStreamSynchronize(I.st));
+    size_t free1 = 0;
+    HIPCHK(hipMemGetInfo(&free1, &total));
+    vramMB = (double)(free0 - free1) / 1048576.0;
+    return true;
+}
+
+int BatchRenderer::addScene(const SceneUpload& S, std::string& err) {
+    Impl& I = *p_;
+    int need = 0;
+    if (!checkScene(S, need, err)) return -1;
+    if (S.variant != I.O.variant) { err = "batch: the scene's BVH layout differs from the batch's"; return -1; }
+    if (S.inst) { err = "batch: two-level scenes are not batched"; return -1; }
+    if (I.O.tri == kTriWoop && S.woop.size() != S.verts.size()) { err = "batch: Woop rows missing"; return -1; }
+    SceneDev D;
+    auto up = [&](const std::vector<Vec4>& v, const Vec4*& dst) {
+        if (v.empty()) { dst = nullptr; return true; }
+        void* q = nullptr;
+        if (hipMalloc(&q, v.size() * sizeof(Vec4)) != hipSuccess) { err = "batch: out of device memory"; return false; }
+        D.bufs.push_back(q); D.bytes += v.size() * sizeof(Vec4);
+        if (hipMemcpy(q, v.data(), v.size() * sizeof(Vec4), hipMemcpyHostToDevice) != hipSuccess) { err = "batch: upload failed"; return false; }
+        dst = (const Vec4*)q;
+        return true;
+    };
+    std::vector<Vec4> nodes;
+    if (S.variant == 8) {   // 128-byte node stride on the device, as the renderer's
+        const size_t nn = S.nodes.size() / 5;
+        nodes.assign(nn * kBvh8Stride, v4(0.f, 0.f, 0.f, 0.f));
+        for (size_t j = 0; j < nn; j++) for (int q = 0; q < 5; q++) nodes[j * kBvh8Stride + q] = S.nodes[j * 5 + q];
+    } else nodes = S.nodes;
+    const Vec4 *dn, *dw, *dv, *dnr, *da;
+    static const std::vector<Vec4> none;
+    const bool ok = up(nodes, dn) && up(I.O.tri == kTriWoop ? S.woop : none, dw) && up(S.verts, dv) && up(S.normals, dnr) &&
+                    up(S.albedo, da);
+    if (!ok) { for (void* b : D.bufs) hipFree(b); return -1; }
+    D.S.nodes = dn; D.S.verts = dv; D.S.nrm = dnr; D.S.albedo = da; D.S.root = S.root;
+    D.S.tri = I.O.tri == kTriWoop ? dw : dv;
+    D.S.tlas = nullptr; D.S.tlasRoot = 0; D.S.inst = nullptr;
+    D.stackNeed = need; D.alive = true;
+    if (!I.ensureSpill(need, err)) { for (void* b : D.bufs) hipFree(b); return -1; }
+    for (size_t k = 0; k < I.scenes.size(); k++)
+        if (!I.scenes[k].alive) { I.scenes[k] = D; return (int)k; }
+    I.scenes.push_back(D);
+    return (int)I.scenes.size() - 1;
+}
+
+void BatchRenderer::removeScene(int id) {
+    Impl& I = *p_;
+    if (id < 0 || id >= (int)I.scenes.size() || !I.scenes[id].alive) return;
+    hipStreamSynchronize(I.st);   // queued passes may still read it
+    for (void* b : I.scenes[id].bufs) hipFree(b);
+    I.scenes[id] = SceneDev();
+}
+
+size_t BatchRenderer::uploadBytes(const SceneUpload& S, int tri) {
+    const size_t nodes = S.variant == 8 ? S.nodes.size() / 5 * kBvh8Stride : S.nodes.size();
+    return (nodes + (tri == kTriWoop ? S.woop.size() : 0) + S.verts.size() + S.normals.size() + S.albedo.size()) * sizeof(Vec4);
+}
+
+size_t BatchRenderer::sceneBytes(int id) const {
+    const Impl& I = *p_;
+    return id >= 0 && id < (int)I.scenes.size() && I.scenes[id].alive ? I.scenes[id].bytes : 0;
+}
+
+bool BatchRenderer::allocImage(int nPix, void** img, std::string& err) {
+    *img = nullptr;
+    if (nPix < 1) { err = "batch: empty image"; return false; }
+    HIPCHK(hipMalloc(img, (size_t)nPix * 16));
+    return clearImage(*img, nPix, err);
+}
+void BatchRenderer::freeImage(void* img) {
+    if (!img) return;
+    hipStreamSynchronize(p_->st);
+    hipFree(img);
+}
+bool BatchRenderer::clearImage(void* img, int nPix, std::string& err) { return p_->fill(img, (size_t)nPix * 16, 0u, err); }
+bool BatchRenderer::readImage(const void* img, int nPix, float* rgba, std::string& err) {
+    HIPCHK(hipMemcpyAsync(rgba, img, (size_t)nPix * 16, hipMemcpyDeviceToHost, p_->st));
+    HIPCHK(hipStreamSynchronize(p_->st));
+    return true;
+}
+bool BatchRenderer::synchronize(std::string& err) { HIPCHK(hipStreamSynchronize(p_->st)); return true; }
+
+bool BatchRenderer::pass(const std::vector<BatchJob>& jobs, bool sync, FrameStats& st, std::string& err) {
+    Impl& I = *p_;
+    st = FrameStats();
+    const int nj = (int)jobs.size();
+    if (nj < 1) return true;
+    if (nj > I.O.maxJobs) { err = "batch: too many jobs in one pass"; return false; }
+    // the job table of this pass, in a slot whose previous upload has completed
+    const int s = I.slot;
+    I.slot = (I.slot + 1) % kTableSlots;
+    if (I.tableUsed[s]) HIPCHK(hipEventSynchronize(I.tableDone[s]));
+    DevJobTable* T = I.hTable[s];
+    long long paths = 0;
+    int groups = 0;
+    for (int j = 0; j < nj; j++) {
+        const BatchJob& B = jobs[j];
+        if (B.scene < 0 || B.scene >= (int)I.scenes.size() || !I.scenes[B.scene].alive) { err = "batch: job " + std::to_string(j) + " names no resident scene"; return false; }
+        const SceneDev& D = I.scenes[B.scene];
+        const int N = B.F.cam.w * B.F.cam.h;
+        if (N < 1 || B.K < 1 || !B.accum || B.smpBase < 0) { err = "batch: job " + std::to_string(j) + " has no pixels, samples or image"; return false; }
+        if (B.F.hasN && !D.S.nrm) { err = "batch: job " + std::to_string(j) + " shades with normals its scene lacks"; return false; }
+        if (B.F.maxB > I.O.bounces) { err = "batch: job " + std::to_string(j) + " bounces deeper than the batch allows"; return false; }
+        DevJob& J = T->jobs[j];
+        J.S = D.S; J.F = B.F; J.accum = (Vec4*)B.accum;
+        J.pathBase = (int)paths; J.N = N; J.K = B.K; J.smpBase = B.smpBase; J.grpBase = groups;
+        paths += (long long)N * B.K;
+        groups += (N + kWave - 1) / kWave;
+        if (paths > (long long)I.O.maxPaths) { err = "batch: the pass exceeds the path-state slots"; return false; }
+    }
+    T->nJobs = nj; T->groups = groups; T->paths = paths;
+    if (I.O.useGraph && !I.capture(s, err)) return false;
+    HIPCHK(hipMemcpyAsync(I.dTable[s], T, offsetof(DevJobTable, jobs) + (size_t)nj * sizeof(DevJob), hipMemcpyHostToDevice, I.st));
+    HIPCHK(hipEventRecord(I.tableDone[s], I.st));
+    I.tableUsed[s] = true;
+    HIPCHK(hipEventRecord(I.e0, I.st));
+    if (I.O.useGraph) HIPCHK(hipGraphLaunch(I.graph[s], I.st));
+    else I.enqueue(s);
+    HIPCHK(hipEventRecord(I.e1, I.st));
+    HIPCHK(hipGetLastError());
+    if (!sync) return true;
+    HIPCHK(hipStreamSynchronize(I.st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, I.e0, I.e1));
+    st.frameMs = ms; st.traceMs = ms;
+    return true;
+}
+
+bool BatchRenderer::counters(FrameStats& t, bool reset, std::string& err) {
+    Impl& I = *p_;
+    t = FrameStats();
+    HIPCHK(hipMemcpyAsync(I.hostTot, I.tot, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, I.st));
+    HIPCHK(hipStreamSynchronize(I.st));
+    t.rays = (double)I.hostTot[0]; t.shadowRays = (double)I.hostTot[1];
+    t.missRays = (double)I.hostTot[2]; t.unoccluded = (double)I.hostTot[3]; t.contRays = (double)I.hostTot[4];
+    t.hitRays = t.rays - t.missRays; t.cameraMiss = (double)I.hostTot[5];
+    if (reset && !I.fill(I.tot, 8 * sizeof(unsigned long long), 0u, err)) return false;
+    return true;
+}
+
+}  // namespace cwp

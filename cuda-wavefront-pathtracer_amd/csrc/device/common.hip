@@ -1,0 +1,135 @@
+// device/common.hip - the kernels that do not depend on the BVH layout: split-kernel shading, pass counters, the
+// per-pixel reduction, fills (declared in kernels.h).
+#include "device/kernels.h"
+
+namespace cwp {
+namespace dev {
+
+// Shading of the depth-`depth` queue from the hits of k_trace_paths (no traversal: every lane of a wave shades).
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+    const int lane = laneId();
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const int* qIn = P.q[depth & 1];
+    int* qOut = P.q[(depth + 1) & 1];
+    int misses = 0;
+    for (int base = lw * kWave; base < n; base += nw * kWave) {
+        const int k = base + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            Vec3 o, d; float tmin;
+            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+            const Vec4 hv = P.H[i];
+            Hit h; h.t = hv.x; h.u = hv.y; h.v = hv.z; h.tri = asInt(hv.w);
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// zero the pass counters; the depth-0 queue of partition x is the x-th of kParts equal slices of the path ids
+__device__ void passBegin(int* cnt, int words, long long NK) {
+    for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = 0;
+    __syncthreads();
+    if (threadIdx.x < kParts) {
+        const long long x = threadIdx.x;
+        *ctr(cnt, (int)x, kCQueue) = (int)((x + 1) * NK / kParts - x * NK / kParts);
+    }
+}
+__global__ void k_pass_begin(int* cnt, int words, long long NK) { passBegin(cnt, words, NK); }
+// a batched pass: the path count comes from the pass's job table on the device
+__global__ void k_pass_begin_jobs(int* cnt, int words, const DevJobTable* T) { passBegin(cnt, words, T->paths); }
+
+// per-pass totals: [0] closest rays, [1] shadow rays, [2] misses, [3] unoccluded, [4] continuations,
+// [5] camera-ray misses
+__global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot) {
+    if (threadIdx.x != 0) return;
+    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = 0;
+    for (int x = 0; x < kParts; x++) m0 += (unsigned)ctrv(cnt, x, kCMiss);
+    for (int d = 0; d < depths; d++) {
+        const int* cd = cnt + d * kCntStride;
+        unsigned long long q = 0;
+        for (int x = 0; x < kParts; x++) {
+            q += (unsigned)ctrv(cd, x, kCQueue); b += (unsigned)ctrv(cd, x, kCShadow);
+            m += (unsigned)ctrv(cd, x, kCMiss); u += (unsigned)ctrv(cd, x, kCUnocc);
+        }
+        a += q; if (d > 0) c += q;
+    }
+    tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
+}
+
+// accum[pix] += sum over the K samples of the pass (path ids pix*K .. pix*K + K-1), in sample order. One wave per 64
+// pixels: the K samples of a pixel are contiguous, so the wave loads chunks of 16 samples of 4 pixels per instruction
+// (4 runs of 256 B) into LDS, then each lane adds its pixel's samples in order.
+constexpr int kRedChunk = 16;
+__global__ void __launch_bounds__(kWave) k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K) {
+    __shared__ float buf[kWave][3 * kRedChunk + 1];   // +1: rows 49 words apart, no bank conflicts between lanes
+    const int lane = laneId();
+    for (int p0 = blockIdx.x * kWave; p0 < N; p0 += gridDim.x * kWave) {
+        float r = 0.f, g = 0.f, b = 0.f;
+        for (int s0 = 0; s0 < K; s0 += kRedChunk) {
+            const int cs = min(kRedChunk, K - s0);
+            for (int j = 0; j < kWave / 4; j++) {
+                const int p = 4 * j + (lane >> 4), s = lane & 15;
+                if (p0 + p < N && s < cs) {
+                    const Vec4 v = L[(size_t)(p0 + p) * K + s0 + s];
+                    buf[p][3 * s] = v.x; buf[p][3 * s + 1] = v.y; buf[p][3 * s + 2] = v.z;
+                }
+            }
+            __syncthreads();
+            for (int s = 0; s < cs; s++) { r += buf[lane][3 * s]; g += buf[lane][3 * s + 1]; b += buf[lane][3 * s + 2]; }
+            __syncthreads();
+        }
+        const int pix = p0 + lane;
+        if (pix < N) { Vec4 a = accum[pix]; a.x += r; a.y += g; a.z += b; accum[pix] = a; }
+    }
+}
+
+__global__ void k_fill(uint32_t* p, size_t words, uint32_t v) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < words; j += (size_t)gridDim.x * blockDim.x) p[j] = v;
+}
+
+// k_reduce over the jobs of a batched pass: 64-pixel group g belongs to the last job whose first group is <= g; the
+// job's accum[pix] += its K samples of the pass, in sample order (the same sums as k_reduce for the job alone)
+__global__ void __launch_bounds__(kWave) k_reduce_jobs(const DevJobTable* __restrict__ T, const Vec4* __restrict__ L) {
+    __shared__ float buf[kWave][3 * kRedChunk + 1];
+    __shared__ int gbase[kMaxJobs];
+    const int nJobs = T->nJobs, groups = T->groups;
+    const DevJob* __restrict__ jobs = T->jobs;
+    for (int j = threadIdx.x; j < nJobs; j += kWave) gbase[j] = jobs[j].grpBase;
+    __syncthreads();
+    const int lane = laneId();
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int j = jobOf(gbase, nJobs, g);
+        const DevJob& J = jobs[j];
+        const int N = J.N, K = J.K, p0 = (g - gbase[j]) * kWave;
+        const Vec4* Lj = L + J.pathBase;
+        float r = 0.f, gg = 0.f, b = 0.f;
+        for (int s0 = 0; s0 < K; s0 += kRedChunk) {
+            const int cs = min(kRedChunk, K - s0);
+            for (int q = 0; q < kWave / 4; q++) {
+                const int p = 4 * q + (lane >> 4), s = lane & 15;
+                if (p0 + p < N && s < cs) {
+                    const Vec4 v = Lj[(size_t)(p0 + p) * K + s0 + s];
+                    buf[p][3 * s] = v.x; buf[p][3 * s + 1] = v.y; buf[p][3 * s + 2] = v.z;
+                }
+            }
+            __syncthreads();
+            for (int s = 0; s < cs; s++) { r += buf[lane][3 * s]; gg += buf[lane][3 * s + 1]; b += buf[lane][3 * s + 2]; }
+            __syncthreads();
+        }
+        const int pix = p0 + lane;
+        if (pix < N) { Vec4 a = J.accum[pix]; a.x += r; a.y += gg; a.z += b; J.accum[pix] = a; }
+    }
+}
+
+}  // namespace dev
+}  // namespace cwp

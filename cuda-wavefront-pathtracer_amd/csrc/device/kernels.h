@@ -48,6 +48,10 @@ struct DevScene {
     const Vec4* nrm;      // per-corner normals or null
     const Vec4* albedo;   // per material
     int root;
+    // two-level scenes: TLAS (BVH2 over instance ids) and its root, instance records; nodes / tri hold every BLAS
+    const Vec4* tlas;
+    int tlasRoot;
+    const InstanceRecord* inst;
 };
 
 // Path state. Path id i = pixel * K + sample: the 64 lanes of a wave start as 64 samples of one pixel, and each
@@ -72,6 +76,96 @@ __device__ __forceinline__ bool traceScene(const DevScene& S, const RayPre& r, H
     if constexpr (V == 2) return traceBVH2<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
     else if constexpr (V == 4) return traceBVH4<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
     else return traceBVH8<ANY, TRI>(S.nodes, S.tri, S.root, r, h, st);
+}
+
+// instance rows applied to a point / a vector (sums left to right), and the normal transform inv^T n
+__device__ __forceinline__ Vec3 xfRowsP(const Vec4* r, Vec3 p) {
+    return v3(r[0].x * p.x + r[0].y * p.y + r[0].z * p.z + r[0].w, r[1].x * p.x + r[1].y * p.y + r[1].z * p.z + r[1].w,
+              r[2].x * p.x + r[2].y * p.y + r[2].z * p.z + r[2].w);
+}
+__device__ __forceinline__ Vec3 xfRowsV(const Vec4* r, Vec3 p) {
+    return v3(r[0].x * p.x + r[0].y * p.y + r[0].z * p.z, r[1].x * p.x + r[1].y * p.y + r[1].z * p.z,
+              r[2].x * p.x + r[2].y * p.y + r[2].z * p.z);
+}
+__device__ __forceinline__ Vec3 xfRowsN(const Vec4* r, Vec3 n) {
+    return v3(r[0].x * n.x + r[1].x * n.y + r[2].x * n.z, r[0].y * n.x + r[1].y * n.y + r[2].y * n.z,
+              r[0].z * n.x + r[1].z * n.y + r[2].z * n.z);
+}
+
+// The BLAS of instance k for a world ray (reference: travInst's instance leaf, src/accel/traverse_inst.cuh:26-29): the
+// ray in object space (o' = inv o, d' = inv d, not normalised, so hit distances are shared with the world ray); the
+// BLAS's stack entries sit above the TLAS's (st.floor). An instance that lowers the hit distance becomes the hit
+// instance.
+template <int V, bool ANY, int TRI, class St>
+__device__ __forceinline__ void enterInstance(const DevScene& S, int k, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    const InstanceRecord* I = S.inst + k;
+    const Vec4 r[3] = { I->inv[0], I->inv[1], I->inv[2] };
+    const RayPre rb = prepareRay<TRI>(xfRowsP(r, o), xfRowsV(r, d), tmin);
+    const float before = h.t;
+    const int floor = st.floor;
+    st.floor = st.sp;
+    TravState ts; travBegin(ts, I->root);
+    if constexpr (V == 2) { while (!stepBVH2<ANY, TRI>(S.nodes, S.tri, rb, h, st, ts)) {} }
+    else if constexpr (V == 4) { while (!stepBVH4<ANY, TRI>(S.nodes, S.tri, rb, h, st, ts)) {} }
+    else { while (!stepBVH8<ANY, TRI>(S.nodes, S.tri, rb, h, st, ts)) {} }
+    st.floor = floor;
+    if (h.t < before) h.inst = k;
+}
+
+// Two-level traversal (reference: travInst, src/accel/traverse_inst.cuh:10-41): the TLAS is a BVH2 whose leaves list
+// instance ids; the TLAS's stack entries share the lane stack with the BLAS traversal above them.
+template <int V, bool ANY, int TRI, class St>
+__device__ __forceinline__ bool traceInst(const DevScene& S, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    using E = typename StackEntry<V>::type;
+    const RayPre r = prepareRay<kTriMT>(o, d, tmin);
+    const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
+    int node = S.tlasRoot;
+    for (;;) {
+        const Vec4* nb = S.tlas + (size_t)node * 4;
+        const Vec4 n0 = ldg4(nb), n1 = ldg4(nb + 1), nz = ldg4(nb + 2), mt = ldg4(nb + 3);
+        int c0 = asInt(mt.x), c1 = asInt(mt.y);
+        const int k0 = asInt(mt.z), k1 = asInt(mt.w);
+        const float a0 = fmaf(n0.x, r.ix, -odx), a1 = fmaf(n0.y, r.ix, -odx), a2 = fmaf(n0.z, r.iy, -ody), a3 = fmaf(n0.w, r.iy, -ody);
+        const float a4 = fmaf(nz.x, r.iz, -odz), a5 = fmaf(nz.y, r.iz, -odz);
+        const float b0 = fmaf(n1.x, r.ix, -odx), b1 = fmaf(n1.y, r.ix, -odx), b2 = fmaf(n1.z, r.iy, -ody), b3 = fmaf(n1.w, r.iy, -ody);
+        const float b4 = fmaf(nz.z, r.iz, -odz), b5 = fmaf(nz.w, r.iz, -odz);
+        const float t0n = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r.tmin));
+        const float t0f = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h.t));
+        const float t1n = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r.tmin));
+        const float t1f = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h.t));
+        bool h0 = t0n <= t0f, h1 = t1n <= t1f;
+        if (h0 && k0 > 0) {
+            for (int t = c0; t < c0 + k0; t++) {
+                enterInstance<V, ANY, TRI>(S, t, o, d, tmin, h, st);
+                if (ANY && h.tri >= 0) return true;
+            }
+            h0 = false;
+        }
+        if (h1 && k1 > 0) {
+            for (int t = c1; t < c1 + k1; t++) {
+                enterInstance<V, ANY, TRI>(S, t, o, d, tmin, h, st);
+                if (ANY && h.tri >= 0) return true;
+            }
+            h1 = false;
+        }
+        if (!h0 && !h1) {
+            if (st.sp == 0) break;
+            node = (int)st.pop();
+        } else if (h0 && h1) {
+            if (t1n < t0n) { const int x = c0; c0 = c1; c1 = x; }
+            st.push((E)(uint32_t)c1);
+            node = c0;
+        } else node = h0 ? c0 : c1;
+    }
+    return h.tri >= 0;
+}
+
+// closest / any hit of one ray in a single- or two-level scene
+template <int V, bool ANY, int TRI, bool INST, class St>
+__device__ __forceinline__ bool traceRay(const DevScene& S, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    st.sp = 0;
+    if constexpr (INST) return traceInst<V, ANY, TRI>(S, o, d, tmin, h, st);
+    else return traceScene<V, ANY, TRI>(S, prepareRay<TRI>(o, d, tmin), h, st);
 }
 
 __device__ __forceinline__ int laneId() { return (int)__lane_id(); }
@@ -131,6 +225,7 @@ __device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_
 // the shadow ray (shadow: po, sdir, scon) for the caller's wave-level appends.
 struct ShadeOut { bool cont, shadow, miss; Vec3 po, nd, Tn, sdir, scon; };
 
+template <bool INST = false>
 __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
                                                 int i, int pix, uint32_t smp, Vec3 o, Vec3 d, Vec3 T, const Hit& h) {
     ShadeOut out;
@@ -148,10 +243,13 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
     const Vec3 p = o + d * h.t;
     const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
     const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
-    Vec3 ng = normalize(cross(e1, e2)), ns = ng;
+    Vec4 xr[3];   // two-level scenes: object-space normals go to world space through inv^T
+    if constexpr (INST) { const InstanceRecord* I = S.inst + h.inst; xr[0] = I->inv[0]; xr[1] = I->inv[1]; xr[2] = I->inv[2]; }
+    Vec3 ng = INST ? normalize(xfRowsN(xr, cross(e1, e2))) : normalize(cross(e1, e2)), ns = ng;
     if (F.hasN) {
         Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
         Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
+        if constexpr (INST) nn = xfRowsN(xr, nn);
         float l = length(nn);
         if (l > 0.f) ns = nn * (1.f / l);
         if (dot(ns, ng) < 0.f) ns = -ns;
@@ -205,13 +303,13 @@ constexpr int kRefill = 16;
 // the shadow queue, so a path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue
 // starts where the depth-d partitions before x end, which bounds it by the rays partition x traced.
 // WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
-template <int V, int TRI, int SL, int WAVES>
+template <int V, int TRI, int SL, int WAVES, bool INST>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int lane = laneId();
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
     const int n = ctrv(cnt, part, kCQueue);
@@ -229,13 +327,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             i = depth == 0 ? inStart + k : qIn[inStart + k];
             Vec3 o, d; float tmin;
             pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
-            const RayPre r = prepareRay<TRI>(o, d, tmin);
-            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
-            st.sp = 0;
-            traceScene<V, false, TRI>(S, r, h, st);
+            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
             // throughput is fetched after the traversal so that no registers hold it across the traversal loop
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
-            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, h);
+            res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
         }
         misses += appendResults(P, cnt, part, outStart, qOut, i, res);
     }
@@ -251,7 +347,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     const int* cnt = P.cnt + depth * kCntStride;
     const int n = ctrv(cnt, part, kCQueue);
@@ -294,12 +390,16 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
 
 // Shadow rays of bounce `depth` (partition per XCD as in k_bounce): any hit; unoccluded contributions are added to
 // the path's radiance. The any-hit traversal fits 64 registers without spilling: 8 waves per SIMD.
-template <int V, int TRI, int SL>
-__global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, int depth) {
+// Two-level scenes keep the TLAS ray and the object-space ray live at once: a budget of 4 waves per SIMD (~128
+// registers) avoids spilling them.
+constexpr int kInstWaves = 4;
+
+template <int V, int TRI, int SL, bool INST>
+__global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevScene S, DevPaths P, int depth) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
     const int n = ctrv(cnt, part, kCShadow);
@@ -310,10 +410,8 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow(DevScene S, DevPaths P, in
         bool vis = false;
         if (k < n) {
             const Vec4 O = P.sO[start + k], D = P.sD[start + k];
-            const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
-            Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
-            st.sp = 0;
-            if (!traceScene<V, true, TRI>(S, r, h, st)) {
+            Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            if (!traceRay<V, true, TRI, INST>(S, xyz(O), xyz(D), 0.f, h, st)) {
                 vis = true;
                 const int i = asInt(O.w);
                 const Vec4 c = P.sC[start + k];
@@ -331,7 +429,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_refill(DevScene S, DevPath
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
     const int n = ctrv(cnt, part, kCShadow);
@@ -382,28 +480,142 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_refill(DevScene S, DevPath
     waveAdd(sum, ctr(cnt, part, kCUnocc));
 }
 
-// Closest or any hit of explicit rays (parity and kernel-level entry point).
-template <int V, int TRI, bool ANY, int SL>
-__global__ void __launch_bounds__(kBlock, 8) k_trace(DevScene S, const Vec4* __restrict__ rays, int n, Vec4* __restrict__ hits, void* spill, int spillCap) {
+// Closest or any hit of explicit rays (parity and kernel-level entry point); two-level scenes also report the hit
+// instance (-1 on a miss) when hitInst is given.
+template <int V, int TRI, bool ANY, int SL, bool INST>
+__global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_trace(DevScene S, const Vec4* __restrict__ rays, int n, Vec4* __restrict__ hits,
+                                                     int* __restrict__ hitInst, void* spill, int spillCap) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)spill + (size_t)gtid * spillCap, 0 };
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)spill + (size_t)gtid * spillCap, 0, 0 };
     for (int k = gtid; k - (int)threadIdx.x < n; k += gridDim.x * kBlock) {
         if (k >= n) continue;
         const Vec4 O = rays[2 * (size_t)k], D = rays[2 * (size_t)k + 1];
-        const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), O.w);
-        Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1;
-        st.sp = 0;
-        traceScene<V, ANY, TRI>(S, r, h, st);
+        Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+        traceRay<V, ANY, TRI, INST>(S, xyz(O), xyz(D), O.w, h, st);
         hits[k] = v4(h.t, h.u, h.v, asFloat(h.tri));
+        if (hitInst) hitInst[k] = h.tri >= 0 ? h.inst : -1;
     }
+}
+
+// ------------------------------------------------------------------------------------------------ batched jobs
+// A render job of a batched pass: its scene, frame parameters and image, and its slices of the pass's concatenated
+// path ids (path pathBase + pixel * K + sample) and 64-pixel reduction groups.
+struct DevJob {
+    DevScene S;
+    FrameParams F;
+    Vec4* accum;        // per-pixel radiance sums of the job (N float4)
+    int pathBase;       // first path id of the job in this pass
+    int N, K, smpBase;  // pixels, samples in flight, sample index of the pass's first sample
+    int grpBase;        // first 64-pixel reduction group of the job
+};
+constexpr int kMaxJobs = 256;
+
+// The job table of a pass: a header the kernels read on the device (so that a pass is a fixed launch sequence that
+// can be captured once in a graph and replayed with any jobs), then the jobs.
+struct DevJobTable {
+    int nJobs;          // jobs in the pass
+    int groups;         // 64-pixel reduction groups over all jobs
+    long long paths;    // paths over all jobs
+    int pad[12];        // 64-byte header
+    DevJob jobs[kMaxJobs];
+};
+
+// the job of path id i: the last job whose first id is <= i (bases ascending, in LDS)
+__device__ __forceinline__ int jobOf(const int* base, int n, int i) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (base[mid] <= i) lo = mid; else hi = mid; }
+    return lo;
+}
+
+// k_bounce over the paths of many jobs at once (one launch per bounce for the whole batch): each path finds its job,
+// and with it the scene it traverses and the parameters it is shaded with. A wave may hold paths of several jobs.
+template <int V, int TRI, int SL, int WAVES>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
+    __syncthreads();
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const int* qIn = P.q[depth & 1];
+    int* qOut = P.q[(depth + 1) & 1];
+    int misses = 0;
+    for (int b0 = lw * kWave; b0 < n; b0 += nw * kWave) {
+        const int k = b0 + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            const int j = jobOf(base, nJobs, i);
+            const DevJob& J = jobs[j];
+            const int li = i - base[j], Kj = J.K, pix = li / Kj;
+            const uint32_t smp = (uint32_t)(J.smpBase + li % Kj);
+            Vec3 o, d; float tmin;
+            if (depth == 0) { cameraRay(J.F, pix, smp, o, d); tmin = J.F.nearT; }
+            else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// k_shadow over the shadow rays of many jobs: the ray's path id names its job and scene
+template <int V, int TRI, int SL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
+    __syncthreads();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCShadow);
+    const int start = partStart(cnt, part);
+    int unocc = 0;
+    for (int b0 = lw * kWave; b0 < n; b0 += nw * kWave) {
+        const int k = b0 + laneId();
+        bool vis = false;
+        if (k < n) {
+            const Vec4 O = P.sO[start + k], D = P.sD[start + k];
+            const int i = asInt(O.w);
+            const DevJob& J = jobs[jobOf(base, nJobs, i)];
+            Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            if (!traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st)) {
+                vis = true;
+                const Vec4 c = P.sC[start + k];
+                Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+            }
+        }
+        unocc += (int)__builtin_popcountll(__ballot(vis));
+    }
+    waveAdd(unocc, ctr(cnt, part, kCUnocc));
 }
 
 // ------------------------------------------------------------------------------------------------------- dispatch
 using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int);
 using ShadowFn = void (*)(DevScene, DevPaths, int);
-using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, void*, int);
+using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, int*, void*, int);
+using JobsFn = void (*)(const DevJobTable*, DevPaths, int);
 
 // the kernels of one (layout, triangle test, LDS stack) configuration
 struct Kernels {
@@ -411,17 +623,29 @@ struct Kernels {
     BounceFn tracePaths = nullptr, tracePlain = nullptr; // split: closest-hit trace with / without refill (then k_shade)
     ShadowFn shadow = nullptr, shadowRefill = nullptr;
     TraceFn traceC = nullptr, traceA = nullptr;
+    // two-level scenes (fused bounce kernel only)
+    BounceFn bounceInst = nullptr;
+    ShadowFn shadowInst = nullptr;
+    TraceFn traceCInst = nullptr, traceAInst = nullptr;
+    // batched jobs
+    JobsFn bounceJobs = nullptr, shadowJobs = nullptr;
 };
 
 template <int V, int TRI, int SL>
 void pickSL(int waves, Kernels& k) {
-    k.bounce = waves >= 8 ? k_bounce<V, TRI, SL, 8> : (waves >= 6 ? k_bounce<V, TRI, SL, 6> : k_bounce<V, TRI, SL, 1>);
+    k.bounce = waves >= 8 ? k_bounce<V, TRI, SL, 8, false> : (waves >= 6 ? k_bounce<V, TRI, SL, 6, false> : k_bounce<V, TRI, SL, 1, false>);
     k.tracePaths = k_trace_paths<V, TRI, SL, kRefill>;
     k.tracePlain = k_trace_paths<V, TRI, SL, kWave>;
-    k.shadow = k_shadow<V, TRI, SL>;
+    k.shadow = k_shadow<V, TRI, SL, false>;
     k.shadowRefill = k_shadow_refill<V, TRI, SL, kRefill>;
-    k.traceC = k_trace<V, TRI, false, SL>;
-    k.traceA = k_trace<V, TRI, true, SL>;
+    k.traceC = k_trace<V, TRI, false, SL, false>;
+    k.traceA = k_trace<V, TRI, true, SL, false>;
+    k.bounceInst = k_bounce<V, TRI, SL, kInstWaves, true>;
+    k.shadowInst = k_shadow<V, TRI, SL, true>;
+    k.traceCInst = k_trace<V, TRI, false, SL, true>;
+    k.traceAInst = k_trace<V, TRI, true, SL, true>;
+    k.bounceJobs = k_bounce_jobs<V, TRI, SL, 8>;
+    k.shadowJobs = k_shadow_jobs<V, TRI, SL>;
 }
 template <int V, int TRI>
 void pickTri(int sl, int waves, Kernels& k) {
@@ -437,6 +661,15 @@ void pickKernels(int tri, int sl, int waves, Kernels& k) {
     else if (tri == kTriWatertight) pickTri<V, kTriWatertight>(sl, waves, k);
     else pickTri<V, kTriWoop>(sl, waves, k);
 }
+
+// layout-independent kernels (device/common.hip)
+__global__ void k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K);
+__global__ void k_pass_begin(int* cnt, int words, long long NK);
+__global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
+__global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);
+__global__ void k_fill(uint32_t* p, size_t words, uint32_t v);
+__global__ void k_pass_begin_jobs(int* cnt, int words, const DevJobTable* T);
+__global__ void k_reduce_jobs(const DevJobTable* __restrict__ T, const Vec4* __restrict__ L);
 
 extern template void pickKernels<2>(int, int, int, Kernels&);
 extern template void pickKernels<4>(int, int, int, Kernels&);

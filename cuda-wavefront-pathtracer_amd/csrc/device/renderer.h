@@ -46,7 +46,11 @@ struct SceneUpload {
     std::vector<Vec4> normals;    // per-corner normals (3 float4/tri) or empty
     std::vector<Vec4> albedo;     // per material (rgb, 0)
     std::vector<int> order;       // leaf position -> original triangle id
-    int stackNeed = 0;            // traversal-stack entries the tree can require
+    int stackNeed = 0;            // traversal-stack entries the tree can require (BLASes only for two-level scenes)
+    // two-level scenes: one BLAS per mesh in `nodes` (absolute indices); instances are set per frame
+    int inst = 0;
+    std::vector<int> meshRoot;    // BLAS root node per mesh
+    std::vector<Vec4> meshBox;    // object-space box per mesh: lo, hi
 };
 
 class Renderer {
@@ -62,11 +66,15 @@ public:
     bool setParams(const FrameParams& P, std::string& err);
     // Closest (anyHit = false) or any hit of n rays; rays = 2 float4 per ray (o, tmin), (d, tmax); hits = float4
     // (t, u, v, leaf position as int bits, -1 on a miss). Both are device pointers; synchronous.
-    bool trace(const void* rays, int n, bool anyHit, void* hits, std::string& err);
+    // Two-level scenes also write the hit instance (-1 on a miss) to inst (a device array here) when given.
+    bool trace(const void* rays, int n, bool anyHit, void* hits, std::string& err, int* inst = nullptr);
     // The same with host arrays, staged through device buffers owned by the renderer.
-    bool traceHost(const float* rays, int n, bool anyHit, float* hits, std::string& err);
+    bool traceHost(const float* rays, int n, bool anyHit, float* hits, std::string& err, int* inst = nullptr);
+    // Two-level scenes: the instances of the next frames (records validated against the meshes' BLAS roots); builds
+    // the TLAS on the GPU (H-PLOC over the instance boxes) and reports its device time.
+    bool setInstances(const InstanceRecord* rec, int n, double& tlasMs, std::string& err);
 
-    double vramMB = 0;
+    double vramMB = 0, tlasMs = 0;   // tlasMs: device time of the last TLAS build
     int gridBlocks = 0, numCU = 0;
     std::string gpuName;
     const RenderOptions& options() const;

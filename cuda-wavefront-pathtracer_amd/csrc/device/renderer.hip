@@ -19,6 +19,7 @@
 // geometric (and optional interpolated) normal faced to the ray, origin offset eps, sun next-event estimation by a
 // uniform cone sample with contribution T*albedo*sunE*cos/pi, cosine-weighted continuation rejected below the
 // geometric surface, throughput *= albedo.
+#include "device/hploc.h"
 #include "device/kernels.h"
 #include "host/scene_check.h"
 #include <algorithm>
@@ -30,93 +31,38 @@ using namespace dev;
 
 namespace {
 
-// Shading of the depth-`depth` queue from the hits of k_trace_paths (no traversal: every lane of a wave shades).
-__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
-    const int lane = laneId();
-    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
-    int* cnt = P.cnt + depth * kCntStride;
-    const int n = ctrv(cnt, part, kCQueue);
-    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
-    const int outStart = partStart(cnt, part);
-    const int* qIn = P.q[depth & 1];
-    int* qOut = P.q[(depth + 1) & 1];
-    int misses = 0;
-    for (int base = lw * kWave; base < n; base += nw * kWave) {
-        const int k = base + lane;
-        int i = 0;
-        ShadeOut res;
-        res.cont = false; res.shadow = false; res.miss = false;
-        if (k < n) {
-            i = depth == 0 ? inStart + k : qIn[inStart + k];
-            Vec3 o, d; float tmin;
-            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
-            const Vec4 hv = P.H[i];
-            Hit h; h.t = hv.x; h.u = hv.y; h.v = hv.z; h.tri = asInt(hv.w);
-            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
-            res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
-        }
-        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+// TLAS input: world boxes of the instances, the 8 corners of the mesh's object box through the forward rows
+// (reference: k_inst_boxes, src/render/wavefront.cu:445-451)
+__global__ void k_inst_boxes(const InstanceRecord* __restrict__ inst, const Vec4* __restrict__ meshBox, int n, Vec4* lo, Vec4* hi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const InstanceRecord& I = inst[i];
+    const Vec4 bl = meshBox[2 * I.mesh], bh = meshBox[2 * I.mesh + 1];
+    Vec3 mn = v3(3.4e38f, 3.4e38f, 3.4e38f), mx = v3(-3.4e38f, -3.4e38f, -3.4e38f);
+    for (int c = 0; c < 8; c++) {
+        const Vec3 w = xfRowsP(I.fwd, v3(c & 1 ? bh.x : bl.x, c & 2 ? bh.y : bl.y, c & 4 ? bh.z : bl.z));
+        mn = v3(w.x < mn.x ? w.x : mn.x, w.y < mn.y ? w.y : mn.y, w.z < mn.z ? w.z : mn.z);
+        mx = v3(w.x > mx.x ? w.x : mx.x, w.y > mx.y ? w.y : mx.y, w.z > mx.z ? w.z : mx.z);
     }
-    waveAdd(misses, ctr(cnt, part, kCMiss));
+    lo[i] = v4(mn, 0.f);
+    hi[i] = v4(mx, 0.f);
 }
 
-// zero the pass counters; the depth-0 queue of partition x is the x-th of kParts equal slices of the path ids
-__global__ void k_pass_begin(int* cnt, int words, long long NK) {
-    for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = 0;
-    __syncthreads();
-    if (threadIdx.x < kParts) {
-        const long long x = threadIdx.x;
-        *ctr(cnt, (int)x, kCQueue) = (int)((x + 1) * NK / kParts - x * NK / kParts);
-    }
+// parents of the H-PLOC nodes (parent[] filled with -1 first), then the deepest leaf's count of internal ancestors:
+// the TLAS traversal never holds more stack entries than that
+__global__ void k_tlas_parents(const Vec4* __restrict__ hn, int n, int* parent) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n - 1) return;
+    const int id = n + j;
+    parent[asInt(hn[2 * id].w)] = id;
+    parent[asInt(hn[2 * id + 1].w)] = id;
 }
-
-// per-pass totals: [0] closest rays, [1] shadow rays, [2] misses, [3] unoccluded, [4] continuations,
-// [5] camera-ray misses
-__global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot) {
-    if (threadIdx.x != 0) return;
-    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = 0;
-    for (int x = 0; x < kParts; x++) m0 += (unsigned)ctrv(cnt, x, kCMiss);
-    for (int d = 0; d < depths; d++) {
-        const int* cd = cnt + d * kCntStride;
-        unsigned long long q = 0;
-        for (int x = 0; x < kParts; x++) {
-            q += (unsigned)ctrv(cd, x, kCQueue); b += (unsigned)ctrv(cd, x, kCShadow);
-            m += (unsigned)ctrv(cd, x, kCMiss); u += (unsigned)ctrv(cd, x, kCUnocc);
-        }
-        a += q; if (d > 0) c += q;
-    }
-    tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
-}
-
-// accum[pix] += sum over the K samples of the pass (path ids pix*K .. pix*K + K-1), in sample order. One wave per 64
-// pixels: the K samples of a pixel are contiguous, so the wave loads chunks of 16 samples of 4 pixels per instruction
-// (4 runs of 256 B) into LDS, then each lane adds its pixel's samples in order.
-constexpr int kRedChunk = 16;
-__global__ void __launch_bounds__(kWave) k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K) {
-    __shared__ float buf[kWave][3 * kRedChunk + 1];   // +1: rows 49 words apart, no bank conflicts between lanes
-    const int lane = laneId();
-    for (int p0 = blockIdx.x * kWave; p0 < N; p0 += gridDim.x * kWave) {
-        float r = 0.f, g = 0.f, b = 0.f;
-        for (int s0 = 0; s0 < K; s0 += kRedChunk) {
-            const int cs = min(kRedChunk, K - s0);
-            for (int j = 0; j < kWave / 4; j++) {
-                const int p = 4 * j + (lane >> 4), s = lane & 15;
-                if (p0 + p < N && s < cs) {
-                    const Vec4 v = L[(size_t)(p0 + p) * K + s0 + s];
-                    buf[p][3 * s] = v.x; buf[p][3 * s + 1] = v.y; buf[p][3 * s + 2] = v.z;
-                }
-            }
-            __syncthreads();
-            for (int s = 0; s < cs; s++) { r += buf[lane][3 * s]; g += buf[lane][3 * s + 1]; b += buf[lane][3 * s + 2]; }
-            __syncthreads();
-        }
-        const int pix = p0 + lane;
-        if (pix < N) { Vec4 a = accum[pix]; a.x += r; a.y += g; a.z += b; accum[pix] = a; }
-    }
-}
-
-__global__ void k_fill(uint32_t* p, size_t words, uint32_t v) {
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < words; j += (size_t)gridDim.x * blockDim.x) p[j] = v;
+__global__ void k_tlas_depth(const int* __restrict__ parent, int n, int* depth) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = 0;
+    for (int x = parent[i]; x >= 0 && d < 2 * n; x = parent[x]) d++;   // the bound only guards against a corrupt tree
+    atomicMax(depth, d);
 }
 
 #define HIPCHK(x)                                                                                                      \
@@ -144,7 +90,16 @@ struct Renderer::Impl {
     BounceFn bounce = nullptr, shade = nullptr;   // fused: bounce; split: bounce = trace kernel, then shade
     ShadowFn shadow = nullptr;
     int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0;
-    void* stageRays = nullptr; void* stageHits = nullptr; size_t stageCap = 0;   // traceHost staging buffers
+    void* stageRays = nullptr; void* stageHits = nullptr; int* stageInst = nullptr; size_t stageCap = 0;   // traceHost staging
+    // two-level scenes: instance records, their world boxes, the TLAS (H-PLOC each frame), BLAS roots per mesh
+    bool inst = false, instSet = false;
+    int nInst = 0, instCap = 0, nMesh = 0;
+    std::vector<int> meshRoot;
+    InstanceRecord* dInst = nullptr;
+    Vec4 *dLo = nullptr, *dHi = nullptr, *dMeshBox = nullptr, *dTlas = nullptr;
+    int *dParent = nullptr, *dDepth = nullptr, *hostDepth = nullptr;
+    HplocBuilder tlasB;
+    hipEvent_t tl0 = nullptr, tl1 = nullptr;
 
     template <class T> bool alloc(T*& p, size_t count, std::string& err) {
         void* q = nullptr;
@@ -180,6 +135,12 @@ Renderer::~Renderer() {
     for (void* a : p_->allocs) hipFree(a);
     if (p_->stageRays) hipFree(p_->stageRays);
     if (p_->stageHits) hipFree(p_->stageHits);
+    if (p_->stageInst) hipFree(p_->stageInst);
+    for (void* a : { (void*)p_->dInst, (void*)p_->dLo, (void*)p_->dHi, (void*)p_->dTlas, (void*)p_->dParent })
+        if (a) hipFree(a);
+    if (p_->hostDepth) hipHostFree(p_->hostDepth);
+    if (p_->tl0) hipEventDestroy(p_->tl0);
+    if (p_->tl1) hipEventDestroy(p_->tl1);
     for (hipEvent_t e : p_->evs) hipEventDestroy(e);
     if (p_->f0) hipEventDestroy(p_->f0);
     if (p_->f1) hipEventDestroy(p_->f1);
@@ -254,9 +215,24 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (S.variant == 2) pickKernels<2>(O.tri, I.sl, O.bounceWaves, I.kn);
     else if (S.variant == 4) pickKernels<4>(O.tri, I.sl, O.bounceWaves, I.kn);
     else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
-    I.bounce = O.fuse ? I.kn.bounce : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
-    I.shade = O.fuse ? nullptr : k_shade;
-    I.shadow = (O.persistent || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
+    I.inst = S.inst != 0;
+    if (I.inst) {
+        // two-level scenes run the fused bounce kernel and the one-ray-per-lane shadow kernel
+        if (!O.fuse || O.persistent) { err = "two-level scenes need the fused kernel (fuse 1, persistent 0)"; return false; }
+        I.bounce = I.kn.bounceInst; I.shade = nullptr; I.shadow = I.kn.shadowInst;
+        I.kn.traceC = I.kn.traceCInst; I.kn.traceA = I.kn.traceAInst;
+        I.meshRoot = S.meshRoot; I.nMesh = (int)S.meshRoot.size();
+        Vec4* mb;
+        if (!I.upload(mb, S.meshBox, err)) return false;
+        I.dMeshBox = mb;
+        HIPCHK(hipEventCreate(&I.tl0)); HIPCHK(hipEventCreate(&I.tl1));
+        HIPCHK(hipHostMalloc((void**)&I.hostDepth, sizeof(int), hipHostMallocDefault));
+        if (!I.alloc(I.dDepth, 1, err)) return false;
+    } else {
+        I.bounce = O.fuse ? I.kn.bounce : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
+        I.shade = O.fuse ? nullptr : k_shade;
+        I.shadow = (O.persistent || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
+    }
     auto gridOf = [&](const void* fn) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kBlock, 0) != hipSuccess || b < 1) b = 1;
@@ -270,7 +246,9 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.maxGrid = std::max(I.gridBounce, std::max(I.gridShadow, I.gridTrace));
     gridBlocks = I.gridBounce;
 
-    // stack spill beyond the LDS part: the tree bounds the entries a traversal can hold
+    // stack spill beyond the LDS part: the tree bounds the entries a traversal can hold (two-level scenes: the
+    // deepest accepted TLAS below the deepest BLAS)
+    if (I.inst) stackNeed += kTlasMaxDepth;
     I.P.spillCap = stackNeed > I.sl ? stackNeed - I.sl : 0;
     if (I.P.spillCap > 0) {
         uint64_t* sp;
@@ -309,6 +287,7 @@ bool Renderer::setParams(const FrameParams& F, std::string& err) {
 bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string& err) {
     Impl& I = *p_;
     if (spp <= 0 || spp % I.K) { err = "spp must be a positive multiple of batch"; return false; }
+    if (I.inst && !I.instSet) { err = "two-level scene: set the instances of a frame first"; return false; }
     if (sBase < 0) { err = "sample base must be >= 0"; return false; }
     S = FrameStats();
     I.evUsed = 0;
@@ -363,32 +342,85 @@ bool Renderer::readAccum(float* rgba, size_t nPix, std::string& err) {
     return true;
 }
 
-bool Renderer::trace(const void* rays, int n, bool anyHit, void* hits, std::string& err) {
+bool Renderer::trace(const void* rays, int n, bool anyHit, void* hits, std::string& err, int* inst) {
     Impl& I = *p_;
     if (n < 0) { err = "negative ray count"; return false; }
     if (n == 0) return true;
+    if (I.inst && !I.instSet) { err = "two-level scene: set the instances of a frame first"; return false; }
     TraceFn fn = anyHit ? I.kn.traceA : I.kn.traceC;
-    fn<<<I.gridTrace, kBlock, 0, I.st>>>(I.S, (const Vec4*)rays, n, (Vec4*)hits, I.P.spill, I.P.spillCap);
+    fn<<<I.gridTrace, kBlock, 0, I.st>>>(I.S, (const Vec4*)rays, n, (Vec4*)hits, inst, I.P.spill, I.P.spillCap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(I.st));
     return true;
 }
 
-bool Renderer::traceHost(const float* rays, int n, bool anyHit, float* hits, std::string& err) {
+bool Renderer::traceHost(const float* rays, int n, bool anyHit, float* hits, std::string& err, int* inst) {
     Impl& I = *p_;
     if (n < 0) { err = "negative ray count"; return false; }
     if (n == 0) return true;
     if ((size_t)n > I.stageCap) {
         if (I.stageRays) { hipFree(I.stageRays); I.stageRays = nullptr; }
         if (I.stageHits) { hipFree(I.stageHits); I.stageHits = nullptr; }
+        if (I.stageInst) { hipFree(I.stageInst); I.stageInst = nullptr; }
         I.stageCap = 0;
         HIPCHK(hipMalloc(&I.stageRays, (size_t)n * 32));
         HIPCHK(hipMalloc(&I.stageHits, (size_t)n * 16));
+        HIPCHK(hipMalloc((void**)&I.stageInst, (size_t)n * 4));
         I.stageCap = (size_t)n;
     }
     HIPCHK(hipMemcpyAsync(I.stageRays, rays, (size_t)n * 32, hipMemcpyHostToDevice, I.st));
-    if (!trace(I.stageRays, n, anyHit, I.stageHits, err)) return false;
+    if (!trace(I.stageRays, n, anyHit, I.stageHits, err, inst ? I.stageInst : nullptr)) return false;
     HIPCHK(hipMemcpy(hits, I.stageHits, (size_t)n * 16, hipMemcpyDeviceToHost));
+    if (inst) HIPCHK(hipMemcpy(inst, I.stageInst, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return true;
+}
+
+bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, std::string& err) {
+    Impl& I = *p_;
+    if (!I.inst) { err = "setInstances: the scene is not two-level"; return false; }
+    if (n < 1 || !rec) { err = "setInstances: no instances"; return false; }
+    // the kernels trust the records: every instance must name a mesh and that mesh's BLAS root
+    for (int i = 0; i < n; i++)
+        if (rec[i].mesh < 0 || rec[i].mesh >= I.nMesh || rec[i].root != I.meshRoot[rec[i].mesh]) {
+            err = "setInstances: instance " + std::to_string(i) + " has a bad mesh or BLAS root"; return false;
+        }
+    HIPCHK(hipStreamSynchronize(I.st));   // the previous frame may still read the TLAS
+    if (n > I.instCap) {
+        for (void* a : { (void*)I.dInst, (void*)I.dLo, (void*)I.dHi, (void*)I.dTlas, (void*)I.dParent }) if (a) hipFree(a);
+        I.dInst = nullptr; I.dLo = I.dHi = I.dTlas = nullptr; I.dParent = nullptr; I.instCap = 0;
+        HIPCHK(hipMalloc((void**)&I.dInst, (size_t)n * sizeof(InstanceRecord)));
+        HIPCHK(hipMalloc((void**)&I.dLo, (size_t)n * 16));
+        HIPCHK(hipMalloc((void**)&I.dHi, (size_t)n * 16));
+        HIPCHK(hipMalloc((void**)&I.dTlas, (size_t)std::max(n - 1, 1) * 64));
+        HIPCHK(hipMalloc((void**)&I.dParent, (size_t)(2 * n - 1) * 4));
+        if (!I.tlasB.reserve(n, err)) return false;
+        I.instCap = n;
+    }
+    HIPCHK(hipMemcpyAsync(I.dInst, rec, (size_t)n * sizeof(InstanceRecord), hipMemcpyHostToDevice, I.st));
+    HIPCHK(hipEventRecord(I.tl0, I.st));
+    k_inst_boxes<<<(n + 255) / 256, 256, 0, I.st>>>(I.dInst, I.dMeshBox, n, I.dLo, I.dHi);
+    int hroot = 0;
+    if (!I.tlasB.build(I.dLo, I.dHi, n, I.st, hroot, err)) return false;
+    if (!hplocToBVH2(I.tlasB.nodes(), n, I.dTlas, I.st, err)) return false;
+    HIPCHK(hipEventRecord(I.tl1, I.st));
+    if (!I.fill(I.dDepth, 4, err)) return false;
+    k_fill<<<std::max(1, std::min((2 * n - 1 + 255) / 256, 4096)), 256, 0, I.st>>>((uint32_t*)I.dParent, (size_t)(2 * n - 1), 0xffffffffu);
+    if (n > 1) k_tlas_parents<<<(n - 1 + 255) / 256, 256, 0, I.st>>>(I.tlasB.nodes(), n, I.dParent);
+    k_tlas_depth<<<(n + 255) / 256, 256, 0, I.st>>>(I.dParent, n, I.dDepth);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(I.hostDepth, I.dDepth, 4, hipMemcpyDeviceToHost, I.st));
+    HIPCHK(hipStreamSynchronize(I.st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, I.tl0, I.tl1));
+    tlasMs = ms;
+    this->tlasMs = ms;
+    if (*I.hostDepth + 1 > kTlasMaxDepth) {
+        I.instSet = false;
+        err = "setInstances: TLAS depth " + std::to_string(*I.hostDepth) + " exceeds the traversal stack";
+        return false;
+    }
+    I.S.tlas = I.dTlas; I.S.tlasRoot = hplocBVH2Root(n); I.S.inst = I.dInst;
+    I.nInst = n; I.instSet = true;
     return true;
 }
 

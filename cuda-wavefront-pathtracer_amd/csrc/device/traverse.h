@@ -17,7 +17,7 @@
 
 namespace cwp {
 
-struct Hit { float t, u, v; int tri; };
+struct Hit { float t, u, v; int tri; int inst; };   // inst: instance of the hit in two-level scenes
 
 // Device stride of a compressed BVH8 node in float4: the 80-byte node padded to one 128-byte cache line.
 constexpr int kBvh8Stride = 8;
@@ -133,6 +133,7 @@ constexpr int kStackStride = 64;   // lanes per workgroup: one wave
 template <typename E, int SL>
 struct LaneStack {
     LdsPtr<E> lds; E* spill; int sp;
+    int floor = 0;   // entries below this belong to an enclosing traversal (the TLAS, while a BLAS is traversed)
     __device__ __forceinline__ void push(E e) {
         if (SL > 0 && sp < SL) lds[sp * kStackStride] = e; else spill[sp - SL] = e;
         sp++;
@@ -184,7 +185,7 @@ __device__ __forceinline__ bool stepBVH2(const Vec4* __restrict__ nodes, const V
         h1 = false;
     }
     if (!h0 && !h1) {
-        if (st.sp == 0) return true;
+        if (st.sp == st.floor) return true;
         ts.node = st.pop();
     } else if (h0 && h1) {
         if (t1n < t0n) { int x = c0; c0 = c1; c1 = x; }
@@ -245,7 +246,7 @@ __device__ __forceinline__ bool stepBVH4(const Vec4* __restrict__ nodes, const V
     };
     cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
     if (ht[0] == kInf) {
-        if (st.sp == 0) return true;
+        if (st.sp == st.floor) return true;
         ts.node = st.pop();
         return false;
     }
@@ -327,7 +328,7 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
         ts.gBase = (uint32_t)childBase; ts.gMask = imask; ts.gHits = hits;
     }
     while (ts.gHits == 0) {
-        if (st.sp == 0) return true;
+        if (st.sp == st.floor) return true;
         const Group e = st.pop();
         const uint32_t bits = (uint32_t)(e >> 32);
         ts.gBase = (uint32_t)e; ts.gMask = bits >> 8; ts.gHits = bits & 255u;

@@ -484,6 +484,22 @@ int appendBVH2(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBas
     }
     return base;
 }
+int appendBVH4(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase) {
+    const int base = (int)all.size() / 8;
+    for (size_t i = 0; i < blas.size(); i += 8) {
+        const Vec4 cf = blas[i + 6], kf = blas[i + 7];
+        int c[4];
+        for (int j = 0; j < 4; j++) {
+            c[j] = asInt(comp(cf, j));
+            const int k = asInt(comp(kf, j));
+            if (k > 0) c[j] += triBase; else if (c[j] >= 0) c[j] += base;
+        }
+        for (int q = 0; q < 6; q++) all.push_back(blas[i + q]);
+        all.push_back(v4(asFloat(c[0]), asFloat(c[1]), asFloat(c[2]), asFloat(c[3])));
+        all.push_back(kf);
+    }
+    return base;
+}
 int appendCWBVH(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase) {
     int base = (int)all.size() / 5;
     for (size_t i = 0; i < blas.size(); i += 5) {

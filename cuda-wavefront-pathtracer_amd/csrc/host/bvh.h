@@ -69,6 +69,7 @@ void packVerts(const Mesh& m, const std::vector<int>& order, std::vector<Vec4>& 
 
 // Appends a BLAS layout to a shared node array, offsetting child/triangle indices; returns the BLAS root node.
 int appendBVH2(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
+int appendBVH4(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
 int appendCWBVH(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
 
 }  // namespace cwp

@@ -17,25 +17,31 @@ struct Links {
     std::vector<int> first, kids;   // CSR child lists of internal children
 };
 
-bool treeDepth(const Links& G, size_t nn, int root, int& depth, std::string& err) {
+// depth of the tree under each root; every node at most one parent, roots none, roots distinct
+bool treeDepth(const Links& G, size_t nn, const std::vector<int>& roots, int& depth, std::string& err) {
     std::vector<unsigned char> parents(nn, 0);
     for (int c : G.kids) {
         if (parents[c]) return fail(err, "node " + std::to_string(c) + " has more than one parent");
         parents[c] = 1;
     }
-    if (parents[root]) return fail(err, "the root has a parent");
+    for (int r : roots) {
+        if (parents[r]) return fail(err, "the root has a parent");
+        parents[r] = 1;   // a second root at the same node is a shared tree: rejected as a second parent
+    }
     depth = 0;
-    std::vector<std::pair<int, int>> st = { { root, 1 } };
-    while (!st.empty()) {
-        const auto [j, d] = st.back(); st.pop_back();
-        depth = std::max(depth, d);
-        for (int k = G.first[j]; k < G.first[j + 1]; k++) st.push_back({ G.kids[k], d + 1 });
+    for (int r : roots) {
+        std::vector<std::pair<int, int>> st = { { r, 1 } };
+        while (!st.empty()) {
+            const auto [j, d] = st.back(); st.pop_back();
+            depth = std::max(depth, d);
+            for (int k = G.first[j]; k < G.first[j + 1]; k++) st.push_back({ G.kids[k], d + 1 });
+        }
     }
     return true;
 }
 
-bool checkBVH2(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
-    Links G; G.first.assign(nn + 1, 0);
+bool checkBVH2(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string& err) {
+    G.first.assign(nn + 1, 0);
     for (size_t j = 0; j < nn; j++) {
         const Vec4* n = &S.nodes[j * 4];
         const int c[2] = { asInt(n[3].x), asInt(n[3].y) }, k[2] = { asInt(n[3].z), asInt(n[3].w) };
@@ -59,11 +65,11 @@ bool checkBVH2(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::stri
         }
     }
     G.first[nn] = (int)G.kids.size();
-    return treeDepth(G, nn, S.root, depth, err);
+    return true;
 }
 
-bool checkBVH4(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
-    Links G; G.first.assign(nn + 1, 0);
+bool checkBVH4(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string& err) {
+    G.first.assign(nn + 1, 0);
     for (size_t j = 0; j < nn; j++) {
         const Vec4* n = &S.nodes[j * 8];
         G.first[j] = (int)G.kids.size();
@@ -85,11 +91,11 @@ bool checkBVH4(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::stri
         }
     }
     G.first[nn] = (int)G.kids.size();
-    return treeDepth(G, nn, S.root, depth, err);
+    return true;
 }
 
-bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::string& err) {
-    Links G; G.first.assign(nn + 1, 0);
+bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string& err) {
+    G.first.assign(nn + 1, 0);
     for (size_t j = 0; j < nn; j++) {
         const Vec4* n = &S.nodes[j * 5];
         const uint32_t ew = asUint(n[0].w), imask = ew >> 24;
@@ -114,7 +120,7 @@ bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, int& depth, std::stri
         for (int c = childBase; c < childBase + ni; c++) G.kids.push_back(c);
     }
     G.first[nn] = (int)G.kids.size();
-    return treeDepth(G, nn, S.root, depth, err);
+    return true;
 }
 
 }  // namespace
@@ -123,7 +129,11 @@ bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
     const size_t per = nodeFloat4s(S.variant);
     if (!per) return fail(err, "variant must be 2, 4 or 8");
     if (S.nodes.empty() || S.nodes.size() % per) return fail(err, "node array size is not a whole number of nodes");
-    if (S.root < 0 || (size_t)S.root >= S.nodes.size() / per) return fail(err, "root node out of range");
+    std::vector<int> roots = S.inst ? S.meshRoot : std::vector<int>(1, S.root);
+    if (roots.empty()) return fail(err, "no BLAS roots");
+    for (int r : roots)
+        if (r < 0 || (size_t)r >= S.nodes.size() / per) return fail(err, "root node out of range");
+    if (S.inst && S.meshBox.size() != 2 * roots.size()) return fail(err, "mesh boxes must be 2 per mesh");
     if (S.verts.empty() || S.verts.size() % 3) return fail(err, "corner array size is not a whole number of triangles");
     const size_t nn = S.nodes.size() / per, nt = S.verts.size() / 3;
     if (nn > 0x7fffffff || nt > 0x7fffffff) return fail(err, "too many nodes or triangles");
@@ -136,9 +146,10 @@ bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
             return fail(err, "triangle " + std::to_string(t) + ": material id out of range");
     }
     int depth = 0;
-    const bool ok = S.variant == 2 ? checkBVH2(S, nn, nt, depth, err)
-                                   : (S.variant == 4 ? checkBVH4(S, nn, nt, depth, err) : checkBVH8(S, nn, nt, depth, err));
-    if (!ok) return false;
+    Links G;
+    const bool ok = S.variant == 2 ? checkBVH2(S, nn, nt, G, err)
+                                   : (S.variant == 4 ? checkBVH4(S, nn, nt, G, err) : checkBVH8(S, nn, nt, G, err));
+    if (!ok || !treeDepth(G, nn, roots, depth, err)) return false;
     // a traversal defers at most one entry per level below the root (BVH2, BVH8 groups) or three (BVH4), plus one
     stackNeed = (S.variant == 4 ? 3 : 1) * depth + 1;
     return true;

@@ -2,6 +2,7 @@
 #include "host/setup.h"
 #include "device/hploc.h"
 #include "host/scene_check.h"
+#include "host/instances.h"
 #include "host/io.h"
 #include <chrono>
 #include <cmath>
@@ -52,6 +53,12 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--bins") ok = inum(A.bins);
         else if (k == "--fuse") ok = inum(A.fuse);
         else if (k == "--persistent") ok = inum(A.persistent);
+        else if (k == "--scene") ok = str(A.scene);
+        else if (k == "--flatten") ok = inum(A.flatten);
+        else if (k == "--frames") ok = inum(A.frames);
+        else if (k == "--frame_start") ok = inum(A.frameStart);
+        else if (k == "--gen_instances") ok = inum(A.genInstances);
+        else if (k == "--gen_small") ok = inum(A.genSmall);
         else if (k == "--shadow_refill") ok = inum(A.shadowRefill);
         else if (k == "--smooth") ok = inum(A.smooth);
         else if (k == "--warm") ok = inum(A.warm);
@@ -97,12 +104,133 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.bounces < 0) { err = "--bounces must be >= 0"; return false; }
     if (A.stack != 8 && A.stack != 16 && A.stack != 32) { err = "--stack must be 8, 16 or 32"; return false; }
     if (A.bounceWaves != 1 && A.bounceWaves != 6 && A.bounceWaves != 8) { err = "--bounce_waves must be 1, 6 or 8"; return false; }
-    if (!A.gen.empty() && A.gen != "boxes" && A.gen != "terrain") { err = "--gen must be boxes or terrain"; return false; }
+    if (!A.gen.empty() && A.gen != "boxes" && A.gen != "terrain" && A.gen != "instances") {
+        err = "--gen must be boxes, terrain or instances"; return false;
+    }
+    if (A.frames < 0 || A.frameStart < 0) { err = "--frames and --frame_start must be >= 0"; return false; }
     return true;
 }
 
 static double msSince(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// binary tree of one mesh: binned SAH on the host, or H-PLOC on the GPU followed by the host SAH leaf collapse
+// (reference: buildTree, src/app/scene_setup.cpp:76-83); treeMs is the build time (device time for H-PLOC), postMs
+// the host conversion
+static bool buildTree(const Mesh& M, const Args& A, int leafMax, BinaryBVH& tree, double& treeMs, double& postMs,
+                      std::string& err) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (A.builder != "hploc") {
+        buildSAH(M, A.bins, leafMax, tree);
+        treeMs = msSince(t0); postMs = 0;
+        return true;
+    }
+    const int n = (int)M.f.size();
+    std::vector<Vec4> lo(n), hi(n), hn((size_t)2 * (2 * (size_t)n - 1));
+    for (int t = 0; t < n; t++) {
+        AABB b; b.grow(M.v[M.f[t].a]); b.grow(M.v[M.f[t].b]); b.grow(M.v[M.f[t].c]);
+        lo[t] = v4(b.lo, 0.f); hi[t] = v4(b.hi, 0.f);
+    }
+    int root = 0; double gpuMs = 0;
+    if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err)) return false;
+    const auto t1 = std::chrono::steady_clock::now();
+    treeFromHploc(hn, n, root, leafMax, tree);
+    postMs = msSince(t1); treeMs = gpuMs;
+    return true;
+}
+
+bool frameInstances(const SceneSetup& Z, int f, std::vector<InstanceRecord>& out) {
+    out.clear();
+    if (!Z.inst) return false;
+    for (const InstanceDesc& I : Z.SD.instances) {
+        const Xform X = instanceXform(I, f);
+        Xform Xi; float ds;
+        invertXform(X, Xi, ds);
+        InstanceRecord R;
+        for (int r = 0; r < 3; r++) {
+            R.inv[r] = v4(Xi.m[4 * r], Xi.m[4 * r + 1], Xi.m[4 * r + 2], Xi.m[4 * r + 3]);
+            R.fwd[r] = v4(X.m[4 * r], X.m[4 * r + 1], X.m[4 * r + 2], X.m[4 * r + 3]);
+        }
+        R.root = Z.S.meshRoot[I.mesh]; R.mesh = I.mesh; R.detSign = ds; R.pad = 0;
+        out.push_back(R);
+    }
+    return true;
+}
+
+// BVH of the setup's mesh (or one BLAS per mesh of a two-level scene) in the layout Z.A.variant names, the packed
+// triangles and the GPU-ready scene Z.S; uses Z.M / Z.meshes, Z.inst, Z.P.hasN and the builder options of Z.A
+static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
+    const Args& A = Z.A;
+    const Mesh& M = Z.M;
+    auto t0 = std::chrono::steady_clock::now();
+    const int V = Z.variant = A.variant == "bvh2" ? 2 : (A.variant == "bvh4" ? 4 : 8);
+    const int leafMax = V == 8 ? 1 : A.leaf;   // the CWBVH collapse forms its own leaves from a 1-primitive-leaf tree
+    Z.reportLeaf = V == 8 ? 3 : leafMax;
+    SceneUpload& S = Z.S;
+    S = SceneUpload();
+    S.variant = V; S.root = 0; S.inst = Z.inst ? 1 : 0;
+    // one BLAS per mesh for two-level scenes (material ids already offset into the combined table), else the mesh
+    std::vector<const Mesh*> parts;
+    if (Z.inst) for (const Mesh& m : Z.meshes) parts.push_back(&m);
+    else parts.push_back(&M);
+    Z.treeMs = 0; Z.postMs = 0; Z.maxDepth = 0; Z.sah = 0;
+    for (size_t k = 0; k < parts.size(); k++) {
+        const Mesh& mk = *parts[k];
+        BinaryBVH tree;
+        double treeMs = 0, postMs = 0, sah = 0;
+        if (!buildTree(mk, A, leafMax, tree, treeMs, postMs, err)) return false;
+        std::vector<Vec4> nodes;
+        std::vector<int> order;
+        if (V == 4 || V == 2) {
+            if (tree.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
+            if (V == 4) layoutBVH4(tree, nodes); else layoutBVH2(tree, nodes);
+            order = tree.prim; sah = sahCost(tree, 1, 1);
+        } else {
+            WideBVH W; collapseCWBVH(tree, W);
+            nodes.swap(W.nodes); order.swap(W.prim); sah = W.sah;
+        }
+        const int triBase = (int)S.verts.size() / 3;
+        const int root = V == 2 ? appendBVH2(S.nodes, nodes, triBase) : (V == 4 ? appendBVH4(S.nodes, nodes, triBase)
+                                                                               : appendCWBVH(S.nodes, nodes, triBase));
+        std::vector<Vec4> verts, woop, normals;
+        packVerts(mk, order, verts, normals);
+        packWoop(mk, order, woop);
+        if (normals.empty() && Z.P.hasN) normals.assign(verts.size(), v4(0.f, 0.f, 0.f, 0.f));   // geometric normal
+        S.verts.insert(S.verts.end(), verts.begin(), verts.end());
+        S.woop.insert(S.woop.end(), woop.begin(), woop.end());
+        S.normals.insert(S.normals.end(), normals.begin(), normals.end());
+        for (int o : order) S.order.push_back(o + triBase);
+        if (Z.inst) {
+            const AABB mb = meshBox(mk);
+            S.meshRoot.push_back(root); S.meshBox.push_back(v4(mb.lo, 0.f)); S.meshBox.push_back(v4(mb.hi, 0.f));
+        }
+        if (k == 0) Z.tree = tree;
+        Z.treeMs += treeMs; Z.postMs += postMs; Z.maxDepth = std::max(Z.maxDepth, tree.maxDepth);
+        Z.sah += sah / (double)parts.size();
+    }
+    S.albedo.clear();
+    for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
+    Z.bvhS = msSince(t0) / 1000.0;
+    Z.frames = A.frames > 0 ? A.frames : (Z.inst ? Z.SD.frames : 1);
+    if (!Z.inst && Z.frames > 1) { err = "--frames > 1 needs a two-level scene (--scene or --gen instances, no --flatten)"; return false; }
+    Z.nNodes = S.nodes.size() / nodeFloat4s(V);
+    if (!checkScene(S, S.stackNeed, err)) return false;
+    Z.nodeBytes = S.nodes.size() * 16;
+    return true;
+}
+
+bool relayoutScene(const SceneSetup& Z, int variant, SceneUpload& S, std::string& err) {
+    if (Z.inst) { err = "relayout: two-level scenes keep their layout"; return false; }
+    if (!nodeFloat4s(variant)) { err = "relayout: variant must be 2, 4 or 8"; return false; }
+    SceneSetup T;
+    T.A = Z.A;
+    T.A.variant = variant == 2 ? "bvh2" : (variant == 4 ? "bvh4" : "bvh8");
+    T.M = Z.M;
+    T.P.hasN = Z.P.hasN;
+    if (!buildSceneBVH(T, err)) return false;
+    S = std::move(T.S);
+    return true;
 }
 
 bool setupScene(SceneSetup& Z, std::string& err) {
@@ -112,7 +240,56 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     identityC2W(c2w);
     bool haveC2w = false;
     // ---- scene ----
-    if (A.gen == "boxes") {
+    Z.inst = false; Z.meshes.clear(); Z.SD = SceneDesc();
+    if (A.gen == "instances" || !A.scene.empty()) {
+        std::vector<Mesh> meshes;
+        if (A.gen == "instances") {
+            makeInstanceScene(meshes, Z.SD, A.genInstances >= 0 ? A.genInstances : 1000, A.genSeed, A.genSmall != 0);
+            instanceSceneCamera(c2w); haveC2w = true; Z.sceneName = "gen:instances";
+        } else {
+            if (!loadSceneFile(A.scene, Z.SD, err)) return false;
+            for (const MeshDesc& d : Z.SD.meshes) {
+                Mesh m;
+                if (!loadMesh(d.file, m)) { err = "failed to load mesh " + d.file; return false; }
+                if (d.yup == 1) for (Vec3& p : m.v) p = v3(p.x, -p.z, p.y);   // y-up to z-up
+                if (d.yup == 1) for (Vec3& q : m.n) q = v3(q.x, -q.z, q.y);
+                meshes.push_back(m);
+            }
+            Z.sceneName = A.scene;
+        }
+        // one material table: each mesh's materials appended, its material ids offset
+        M = Mesh();
+        std::vector<int> matOff;
+        for (Mesh& m : meshes) {
+            if (m.kd.empty()) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+            int maxMat = 0; for (int t : m.fm) maxMat = std::max(maxMat, t);
+            while ((int)m.kd.size() <= maxMat) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+            matOff.push_back((int)M.kd.size());
+            for (size_t j = 0; j < m.kd.size(); j++) { M.kd.push_back(m.kd[j]); M.matName.push_back(j < m.matName.size() ? m.matName[j] : ""); }
+        }
+        if (A.flatten) {   // every instance transformed into one mesh at the first frame (reference --flatten 1)
+            for (const InstanceDesc& I : Z.SD.instances) appendTransformed(M, meshes[I.mesh], matOff[I.mesh], instanceXform(I, A.frameStart));
+        } else {
+            Z.inst = true;
+            for (size_t k = 0; k < meshes.size(); k++) {
+                Mesh& m = meshes[k];
+                if (m.fm.empty()) m.fm.assign(m.f.size(), 0);
+                for (int& t : m.fm) t += matOff[k];
+                m.kd.clear();
+            }
+            Z.meshes = meshes;
+            // the combined mesh only carries the material table and the world box of the first frame (ray offset eps)
+            for (const InstanceDesc& I : Z.SD.instances) {
+                const Xform X = instanceXform(I, A.frameStart);
+                const AABB b = meshBox(meshes[I.mesh]);
+                for (int c = 0; c < 8; c++)
+                    M.v.push_back(xfPoint(X, v3(c & 1 ? b.hi.x : b.lo.x, c & 2 ? b.hi.y : b.lo.y, c & 4 ? b.hi.z : b.lo.z)));
+            }
+            bool anyN = false;
+            for (const Mesh& m : meshes) anyN = anyN || m.hasN();
+            if (anyN) M.n.assign(M.v.size(), v3(0.f, 0.f, 1.f));
+        }
+    } else if (A.gen == "boxes") {
         makeBoxCloud(M, A.genBoxes >= 0 ? A.genBoxes : 1024, A.genSeed, 0.7f);
         boxCloudCamera(c2w); haveC2w = true; Z.sceneName = "gen:boxes";
     } else if (A.gen == "terrain") {
@@ -127,7 +304,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
         Z.sceneName = A.mesh;
     }
     if (A.synthetic && A.hfov == 0 && A.fx == 0) A.hfov = 50.f;
-    if (!A.smooth) M.n.clear();
+    if (!A.smooth) { M.n.clear(); for (Mesh& m : Z.meshes) m.n.clear(); }
     if (!A.lighting.empty() && !loadLightingFile(A.lighting, A.light, &M, err)) return false;
     if (A.albedo.x >= 0) for (auto& kd : M.kd) kd = A.albedo;
     if (M.kd.empty()) M.kd.push_back(v3(0.8f, 0.8f, 0.8f));
@@ -161,55 +338,10 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     H.maxB = A.bounces; H.seed = A.seed; H.eps = A.epsScale * Z.extent; H.nearT = A.nearT;
     H.hasN = M.hasN() ? 1 : 0; H.filterBH = A.filterBH;
     blackmanHarrisTable(H.ftab, kFilterTable, 1.5);
-    // ---- BVH ----
-    auto t0 = std::chrono::steady_clock::now();
-    const int V = Z.variant = A.variant == "bvh2" ? 2 : (A.variant == "bvh4" ? 4 : 8);
-    const int leafMax = V == 8 ? 1 : A.leaf;   // the CWBVH collapse forms its own leaves from a 1-primitive-leaf tree
-    Z.reportLeaf = V == 8 ? 3 : leafMax;
-    if (A.builder == "hploc") {
-        // GPU build over the triangle boxes, then the host SAH leaf collapse (reference: buildTree,
-        // src/app/scene_setup.cpp:76-83); treeMs is the device time, postMs the host conversion
-        const int n = (int)M.f.size();
-        std::vector<Vec4> lo(n), hi(n), hn((size_t)2 * (2 * (size_t)n - 1));
-        for (int t = 0; t < n; t++) {
-            AABB b; b.grow(M.v[M.f[t].a]); b.grow(M.v[M.f[t].b]); b.grow(M.v[M.f[t].c]);
-            lo[t] = v4(b.lo, 0.f); hi[t] = v4(b.hi, 0.f);
-        }
-        int root = 0; double gpuMs = 0;
-        if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err)) return false;
-        const auto t1 = std::chrono::steady_clock::now();
-        treeFromHploc(hn, n, root, leafMax, Z.tree);
-        Z.postMs = msSince(t1);
-        Z.treeMs = gpuMs;
-    } else {
-        buildSAH(M, A.bins, leafMax, Z.tree);
-        Z.treeMs = msSince(t0);
-    }
-    Z.maxDepth = Z.tree.maxDepth;
-    SceneUpload& S = Z.S;
-    S.variant = V; S.root = 0;
-    if (V == 4) {
-        layoutBVH4(Z.tree, S.nodes); S.order = Z.tree.prim; Z.sah = sahCost(Z.tree, 1, 1);
-        if (Z.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
-    } else if (V == 2) {
-        layoutBVH2(Z.tree, S.nodes); S.order = Z.tree.prim; Z.sah = sahCost(Z.tree, 1, 1);
-        if (Z.maxDepth > 63) { err = "BVH depth exceeds the traversal stack"; return false; }
-    } else {
-        WideBVH W; collapseCWBVH(Z.tree, W);
-        S.nodes.swap(W.nodes); S.order.swap(W.prim); Z.sah = W.sah;
-    }
-    std::vector<Vec4> nrm;
-    packVerts(M, S.order, S.verts, S.normals);
-    packWoop(M, S.order, S.woop);
-    S.albedo.clear();
-    for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
-    Z.bvhS = msSince(t0) / 1000.0;
-    Z.nNodes = S.nodes.size() / nodeFloat4s(V);
-    if (!checkScene(S, S.stackNeed, err)) return false;
-    Z.nodeBytes = S.nodes.size() * 16;
+    if (!buildSceneBVH(Z, err)) return false;
     // ---- renderer options ----
     RenderOptions& O = Z.O;
-    O.variant = V; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
+    O.variant = Z.variant; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
     O.shadowRefill = A.shadowRefill;
@@ -229,14 +361,15 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
              "\"seed\":%u,\"integrator\":\"legacy\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":0,"
-             "\"instances\":0,\"flatten\":0,\"frame\":%d,\"tlas_ms\":0.000,\"bins\":%d,\"stack_lds\":%d,"
+             "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
              rep, Z.variant, A.tri.c_str(), A.persistent, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
              Z.P.hasN, A.seed, Z.P.cam.lensR, Z.P.cam.focusD, "importance", A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
-             Z.treeMs, Z.postMs, Z.sah, frame, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
+             Z.treeMs, Z.postMs, Z.sah, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
+             A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
              st.unoccluded);
     return buf;
 }

@@ -6,6 +6,7 @@
 #include "device/renderer.h"
 #include "host/bvh.h"
 #include "host/camera.h"
+#include "host/instances.h"
 #include "host/mesh.h"
 #include <string>
 #include <vector>
@@ -14,7 +15,10 @@ namespace cwp {
 
 struct Args {
     std::string mesh, camera, out = "out", variant = "bvh8", c2wKey = "c2w_blender_zup", tri = "woop", c2w, lighting;
-    std::string gen;                 // procedural scene: "boxes" (1,024-box cloud) or "terrain" (fBm heightfield + boxes)
+    std::string gen;                 // procedural scene: "boxes" (1,024-box cloud), "terrain" (fBm heightfield + boxes) or
+                                     // "instances" (5 meshes, 1,000 instances with per-frame rigid transforms)
+    std::string scene;               // two-level scene file (meshes + instances)
+    int genInstances = -1, genSmall = 0, flatten = 0, frames = 0, frameStart = 0;
     std::string integrator = "auto", builder = "sah", collapse = "sah";
     int genBoxes = -1, genGrid = 242; uint64_t genSeed = 1;
     int fuse = -1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
@@ -42,10 +46,22 @@ struct SceneSetup {
     double bvhS = 0, treeMs = 0, postMs = 0, sah = 0;
     size_t nNodes = 0, nodeBytes = 0;
     std::string sceneName;
+    // two-level scenes: the meshes (one BLAS each, material ids into M.kd) and the instances
+    bool inst = false;
+    std::vector<Mesh> meshes;
+    SceneDesc SD;
+    int frames = 1;
 };
+
+// Instance records of frame f of a two-level scene (false if the scene is not instanced).
+bool frameInstances(const SceneSetup& Z, int f, std::vector<InstanceRecord>& out);
 
 // Runs the setup from Z.A; false with a message in err.
 bool setupScene(SceneSetup& Z, std::string& err);
+
+// The GPU-ready scene of a single-level setup in another BVH layout (2, 4 or 8), built from its mesh with its builder
+// options.
+bool relayoutScene(const SceneSetup& Z, int variant, SceneUpload& S, std::string& err);
 
 // One JSON object (no newline) with the reference's per-frame statistics keys (src/app/pt.cpp:35-41).
 std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& st, int rep, int frame);

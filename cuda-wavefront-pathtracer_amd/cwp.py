@@ -19,7 +19,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcwp.so")
 
 ARR_VERTICES, ARR_TRIANGLES, ARR_MATERIAL, ARR_ALBEDO, ARR_NODES, ARR_ORDER, ARR_WOOP, ARR_CORNERS, ARR_NORMALS, \
-    ARR_TREE, ARR_TREE_BOX, ARR_PRIM = range(12)
+    ARR_TREE, ARR_TREE_BOX, ARR_PRIM, ARR_MESH_ROOTS, ARR_MESH_BOXES = range(14)
 TRI_MT, TRI_WOOP, TRI_WATERTIGHT = 0, 1, 2
 CWP_COPY_H2D, CWP_COPY_D2H, CWP_COPY_D2D = 1, 2, 3
 
@@ -55,10 +55,18 @@ class SceneInfo(ctypes.Structure):
                 ("num_verts", ctypes.c_int), ("num_materials", ctypes.c_int), ("num_nodes", ctypes.c_int),
                 ("node_floats4", ctypes.c_int), ("bvh_depth", ctypes.c_int), ("stack_need", ctypes.c_int),
                 ("sah", ctypes.c_double), ("bvh_build_s", ctypes.c_double), ("width", ctypes.c_int),
-                ("height", ctypes.c_int), ("spp", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int)]
+                ("height", ctypes.c_int), ("spp", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int),
+                ("num_meshes", ctypes.c_int), ("num_instances", ctypes.c_int), ("frames", ctypes.c_int),
+                ("frame_start", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Instance(ctypes.Structure):
+    """cwp_instance: world-to-object rows, object-to-world rows, BLAS root, mesh, det sign."""
+    _fields_ = [("inv", ctypes.c_float * 12), ("fwd", ctypes.c_float * 12), ("root", ctypes.c_int), ("mesh", ctypes.c_int),
+                ("det_sign", ctypes.c_float), ("pad", ctypes.c_int)]
 
 
 class SceneDesc(ctypes.Structure):
@@ -82,6 +90,50 @@ class Options(ctypes.Structure):
         for k, v in kw.items():
             setattr(o, k, v)
         return o
+
+
+class ServerOptions(ctypes.Structure):
+    """cwp_server_options: the server's BVH layout, triangle test, LDS stack, bounce limit, path-state slots, jobs per
+    pass, scene budget and mode."""
+    _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("stack_lds", ctypes.c_int), ("bounces", ctypes.c_int),
+                ("max_paths", ctypes.c_size_t), ("max_jobs", ctypes.c_int), ("scene_budget_mb", ctypes.c_double),
+                ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib().cwp_server_options_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("stack_lds", ctypes.c_int), ("rays_per_lane", ctypes.c_int),
+                ("rays_per_s", ctypes.c_double), ("wall_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("scene", "width", "height", "spp", "bounces", "samples_per_pass", "passes")] + \
+               [(n, ctypes.c_double) for n in ("arrival_s", "admitted_s", "finished_s", "latency_s")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("jobs", "passes", "scenes_loaded", "scenes_evicted")] + \
+               [(n, ctypes.c_double) for n in ("wall_s", "rays", "rays_per_s", "latency_p50_s", "latency_p99_s",
+                                               "latency_mean_s", "max_paths_in_flight")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_form=0, root=0):
@@ -126,6 +178,11 @@ SIGNATURES = {
     "cwp_frame_json": ([_P, _P, ctypes.POINTER(FrameStats), _I, _I, ctypes.c_char_p, _S], _I),
     "cwp_ctx_info": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
     "cwp_options_default": ([ctypes.POINTER(Options)], None),
+    "cwp_setup_instances": ([_P, _I, ctypes.POINTER(Instance), _I, ctypes.POINTER(_I)], _I),
+    "cwp_set_instances": ([_P, ctypes.POINTER(Instance), _I, ctypes.POINTER(_D)], _I),
+    "cwp_set_frame": ([_P, _I, ctypes.POINTER(_D)], _I),
+    "cwp_trace_host_inst": ([_P, ctypes.POINTER(ctypes.c_float), _I, _I, ctypes.POINTER(ctypes.c_float),
+                             ctypes.POINTER(_I)], _I),
     "cwp_scene_check": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(_I)], _I),
     "cwp_create_scene": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(Params), ctypes.POINTER(Options), ctypes.POINTER(_P)], _I),
     "cwp_bvh_build": ([ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), _I, _I, _I, _I, ctypes.POINTER(_P)], _I),
@@ -141,6 +198,20 @@ SIGNATURES = {
     "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),
     "cwp_hploc_nodes": ([_P, ctypes.POINTER(_P)], _I),
     "cwp_hploc_to_bvh2": ([_P, _I, _P, _P], _I),
+    "cwp_server_options_default": ([ctypes.POINTER(ServerOptions)], None),
+    "cwp_server_create": ([ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)], _I),
+    "cwp_server_destroy": ([_P], None),
+    "cwp_server_add_scene": ([_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_S)], _I),
+    "cwp_server_submit": ([_P, _I, _I, _I, _I, _I, ctypes.c_uint32, _D, ctypes.POINTER(_I)], _I),
+    "cwp_server_run": ([_P, _I], _I),
+    "cwp_server_job": ([_P, _I, ctypes.POINTER(JobInfo)], _I),
+    "cwp_server_image": ([_P, _I, ctypes.POINTER(ctypes.c_float), _S], _I),
+    "cwp_server_stats": ([_P, ctypes.POINTER(ServeStats)], _I),
+    "cwp_server_clear_jobs": ([_P], _I),
+    "cwp_server_reset_jobs": ([_P], _I),
+    "cwp_server_autotune": ([_P, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I,
+                             ctypes.POINTER(TuneTrial), _I, ctypes.POINTER(_I), ctypes.POINTER(ServerOptions)], _I),
+    "cwp_server_get_options": ([_P, ctypes.POINTER(ServerOptions)], _I),
     "cwp_device_malloc": ([_S, ctypes.POINTER(_P)], _I),
     "cwp_device_free": ([_P], None),
     "cwp_memcpy": ([_P, _P, _S, _I], _I),
@@ -180,7 +251,8 @@ def device_info():
 _ARRAY_LAYOUT = {ARR_VERTICES: (np.float32, (3,)), ARR_TRIANGLES: (np.int32, (3,)), ARR_MATERIAL: (np.int32, ()),
                  ARR_ALBEDO: (np.float32, (4,)), ARR_ORDER: (np.int32, ()), ARR_WOOP: (np.float32, (3, 4)),
                  ARR_CORNERS: (np.float32, (3, 4)), ARR_NORMALS: (np.float32, (3,)), ARR_TREE: (np.int32, (4,)),
-                 ARR_TREE_BOX: (np.float32, (6,)), ARR_PRIM: (np.int32, ())}
+                 ARR_TREE_BOX: (np.float32, (6,)), ARR_PRIM: (np.int32, ()), ARR_MESH_ROOTS: (np.int32, ()),
+                 ARR_MESH_BOXES: (np.float32, (8,))}
 
 
 def _view(ptr, count, which, node_floats4=None):
@@ -220,6 +292,15 @@ class Setup:
         ptr, cnt = ctypes.c_void_p(), ctypes.c_size_t()
         _check(lib().cwp_setup_array(self.h, which, ctypes.byref(ptr), ctypes.byref(cnt)))
         return _view(ptr.value, cnt.value, which, self.info["node_floats4"])
+
+    def instances(self, frame):
+        """Instance records of a frame of a two-level scene: (n, 28) float32 (inv rows, fwd rows, root / mesh bits,
+        det sign, pad)."""
+        n = ctypes.c_int()
+        _check(lib().cwp_setup_instances(self.h, frame, None, 0, ctypes.byref(n)))
+        rec = (Instance * n.value)()
+        _check(lib().cwp_setup_instances(self.h, frame, rec, n.value, ctypes.byref(n)))
+        return np.frombuffer(rec, dtype=np.float32).reshape(n.value, 28).copy()
 
     def close(self):
         if getattr(self, "h", None):
@@ -279,6 +360,29 @@ class Renderer:
         """rays_ptr / hits_ptr: device addresses (e.g. torch.Tensor.data_ptr())."""
         _check(lib().cwp_trace(self.h, ctypes.c_void_p(rays_ptr), int(n), 1 if any_hit else 0, ctypes.c_void_p(hits_ptr)))
 
+    def set_frame(self, frame):
+        """Two-level scenes: instances of a frame of the setup, TLAS rebuilt on the GPU; returns the TLAS build ms."""
+        ms = ctypes.c_double()
+        _check(lib().cwp_set_frame(self.h, int(frame), ctypes.byref(ms)))
+        return ms.value
+
+    def set_instances(self, records):
+        """records: (n, 28) float32 instance records; returns the TLAS build ms."""
+        r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 28)
+        ms = ctypes.c_double()
+        _check(lib().cwp_set_instances(self.h, ctypes.cast(r.ctypes.data, ctypes.POINTER(Instance)), len(r), ctypes.byref(ms)))
+        return ms.value
+
+    def trace_inst(self, rays, any_hit=False):
+        """Two-level scenes: (hits (n, 4), instance ids (n,))."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((r.shape[0], 4), dtype=np.float32)
+        ids = np.zeros(r.shape[0], dtype=np.int32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(lib().cwp_trace_host_inst(self.h, r.ctypes.data_as(fp), r.shape[0], 1 if any_hit else 0,
+                                         hits.ctypes.data_as(fp), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return hits, ids
+
     def trace(self, rays, any_hit=False):
         """rays: (n, 8) float32 array (o.xyz, tmin, d.xyz, tmax). Returns (n, 4) hits (t, u, v, leaf position bits)."""
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
@@ -302,6 +406,88 @@ class Renderer:
     def close(self):
         if getattr(self, "h", None):
             lib().cwp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Render-job server (cwp_server_*): scenes from Setups, jobs with arrival times, batched or plain runs."""
+
+    def __init__(self, options=None, **kw):
+        o = options if options is not None else ServerOptions.default(**kw)
+        h = ctypes.c_void_p()
+        _check(lib().cwp_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self.options = o
+
+    def add_scene(self, setup):
+        """Returns the scene id; scene_bytes[id] is the device bytes it takes when resident."""
+        sid, nb = ctypes.c_int(), ctypes.c_size_t()
+        _check(lib().cwp_server_add_scene(self.h, setup.h, ctypes.byref(sid), ctypes.byref(nb)))
+        self.scene_bytes = getattr(self, "scene_bytes", {})
+        self.scene_bytes[sid.value] = nb.value
+        return sid.value
+
+    def submit(self, scene, width, height, spp, bounces=6, seed=1000, arrival_s=0.0):
+        jid = ctypes.c_int()
+        _check(lib().cwp_server_submit(self.h, int(scene), int(width), int(height), int(spp), int(bounces), int(seed) & 0xFFFFFFFF,
+                                       float(arrival_s), ctypes.byref(jid)))
+        return jid.value
+
+    def run(self, batched=None):
+        b = self.options.batched if batched is None else (1 if batched else 0)
+        _check(lib().cwp_server_run(self.h, b))
+        return self.stats()
+
+    def job(self, jid):
+        i = JobInfo()
+        _check(lib().cwp_server_job(self.h, int(jid), ctypes.byref(i)))
+        return i.as_dict()
+
+    def image(self, jid):
+        """Per-pixel radiance sums of a finished job, (height, width, 4) float32 (divide by spp)."""
+        j = self.job(jid)
+        out = np.zeros((j["height"], j["width"], 4), dtype=np.float32)
+        _check(lib().cwp_server_image(self.h, int(jid), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                      j["width"] * j["height"]))
+        return out
+
+    def stats(self):
+        s = ServeStats()
+        _check(lib().cwp_server_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def clear_jobs(self):
+        _check(lib().cwp_server_clear_jobs(self.h))
+
+    def reset_jobs(self):
+        _check(lib().cwp_server_reset_jobs(self.h))
+
+    def get_options(self):
+        o = ServerOptions()
+        _check(lib().cwp_server_get_options(self.h, ctypes.byref(o)))
+        return o.as_dict()
+
+    def autotune(self, variants=(4, 8), stacks=(8, 16, 32), rays_per_lane=(1, 2, 4)):
+        """Runs the submitted jobs under every combination, keeps the fastest; returns (trials, chosen options)."""
+        def arr(x):
+            return (ctypes.c_int * len(x))(*x)
+        n = len(variants) * len(stacks) * len(rays_per_lane)
+        trials = (TuneTrial * n)()
+        got, chosen = ctypes.c_int(), ServerOptions()
+        _check(lib().cwp_server_autotune(self.h, arr(variants), len(variants), arr(stacks), len(stacks), arr(rays_per_lane),
+                                         len(rays_per_lane), trials, n, ctypes.byref(got), ctypes.byref(chosen)))
+        self.options = chosen
+        return [trials[k].as_dict() for k in range(got.value)], chosen.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cwp_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -104,7 +104,17 @@ typedef struct cwp_scene_info {
     int stack_need;     /* traversal stack entries the layout can require */
     double sah, bvh_build_s;
     int width, height, spp, batch, bounces;
+    int num_meshes, num_instances, frames, frame_start;   /* two-level scenes (num_meshes = 0 otherwise) */
 } cwp_scene_info;
+
+/* Instance record of a two-level scene (112 bytes): world-to-object rows, object-to-world rows (3 x 4, row-major), BLAS
+ * root node of the mesh, mesh id, sign of det(fwd). Replaces DevInstance (src/accel/instance.h:7). */
+typedef struct cwp_instance {
+    float inv[12], fwd[12];
+    int root, mesh;
+    float det_sign;
+    int pad;
+} cwp_instance;
 
 /* Array selectors for cwp_setup_array / cwp_bvh_array (element type, count unit). */
 enum {
@@ -119,7 +129,9 @@ enum {
     CWP_ARR_NORMALS = 8,  /* float[3] per mesh vertex (empty without normals) */
     CWP_ARR_TREE = 9      /* int[4] per binary SAH node: left, right, first, count; then float[6] box via CWP_ARR_TREE_BOX */,
     CWP_ARR_TREE_BOX = 10,/* float[6] per binary SAH node: lo.xyz, hi.xyz */
-    CWP_ARR_PRIM = 11     /* int per binary-tree primitive slot: triangle id */
+    CWP_ARR_PRIM = 11,    /* int per binary-tree primitive slot: triangle id */
+    CWP_ARR_MESH_ROOTS = 12, /* int per mesh of a two-level scene: BLAS root node */
+    CWP_ARR_MESH_BOXES = 13  /* float[8] per mesh of a two-level scene: object-space box lo (xyz, 0), hi (xyz, 0) */
 };
 
 const char* cwp_last_error(void);
@@ -169,6 +181,18 @@ int cwp_scene_check(const cwp_scene_desc* d, int* stack_need);
 /* Renderer over caller buffers. Replaces WavefrontRenderer::init(Params, SceneBuffers, RenderOptions)
  * (src/render/wavefront.cu:469-502). cwp_frame_json needs a setup and is not available for such a context. */
 int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* params, const cwp_options* opts, cwp_ctx** out);
+/* Two-level scenes (--scene file or --gen instances): the instance records of animation frame `frame` (transforms
+ * evaluated in double, inverted in double; reference SceneSetup::frameData, src/app/scene_setup.cpp:274-299); *count
+ * receives the instance count, at most `cap` records are written. Host only. */
+int cwp_setup_instances(const cwp_setup* s, int frame, cwp_instance* out, int cap, int* count);
+/* Sets the instances of the next frames and rebuilds the TLAS on the GPU (H-PLOC over the instance boxes); *tlas_ms
+ * receives its device time. Every record must name a mesh and that mesh's BLAS root. Replaces setInstances
+ * (src/render/renderer.h:52). cwp_create does this for the setup's first frame. */
+int cwp_set_instances(cwp_ctx* c, const cwp_instance* inst, int n, double* tlas_ms);
+/* cwp_setup_instances + cwp_set_instances for a frame of the context's setup. */
+int cwp_set_frame(cwp_ctx* c, int frame, double* tlas_ms);
+/* cwp_trace_host that also reports the hit instance of two-level scenes (-1 on a miss). */
+int cwp_trace_host_inst(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits, int* inst);
 /* Grid size (workgroups) and VRAM used by the context. */
 int cwp_ctx_info(const cwp_ctx* c, int* grid_blocks, double* vram_mb);
 
@@ -206,6 +230,61 @@ int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes);
 /* 64 B BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) of device H-PLOC nodes into `out`
  * (device). Replaces hplocToBVH2 (src/accel/hploc.h:36). */
 int cwp_hploc_to_bvh2(const void* hnodes, int n, void* out, void* stream);
+
+/* ---- render-job server: batched multi-job passes, continuous batching, resident scenes with LRU eviction ----
+ * Jobs (scene, resolution, samples, bounce limit, seed, arrival time) are admitted in arrival order once arrived,
+ * their scene resident (within the scene budget, evicting least recently used scenes no admitted job needs) and their
+ * paths fit the free path-state slots; each step is one pass over every admitted job (one launch per bounce for all of
+ * them); finished jobs leave with their image and newly arrived ones are admitted. Plain mode: the same kernels, one
+ * job at a time. A job's image does not depend on the mode or on the other jobs. */
+typedef struct cwp_server cwp_server;
+typedef struct cwp_server_options {
+    int variant, tri_test, stack_lds;  /* every scene's BVH layout, the triangle test, LDS stack entries */
+    int bounces;                       /* deepest bounce limit of any job */
+    size_t max_paths;                  /* path-state slots (paths in flight over all admitted jobs) */
+    int max_jobs;                      /* jobs per pass (<= 256) */
+    double scene_budget_mb;            /* device memory for resident scenes */
+    int batched;                       /* 1: continuous batching; 0: plain, one job at a time */
+    int rays_per_lane;                 /* 1, 2 or 4: persistent grid = resident waves / rays_per_lane */
+    int use_graph;                     /* 1: each pass replays a captured HIP graph */
+} cwp_server_options;
+typedef struct cwp_tune_trial {
+    int variant, stack_lds, rays_per_lane;
+    double rays_per_s, wall_s;
+} cwp_tune_trial;
+typedef struct cwp_job_info {
+    int scene, width, height, spp, bounces, samples_per_pass, passes;
+    double arrival_s, admitted_s, finished_s, latency_s;   /* seconds after the start of cwp_server_run */
+} cwp_job_info;
+typedef struct cwp_serve_stats {
+    int jobs, passes, scenes_loaded, scenes_evicted;
+    double wall_s, rays, rays_per_s, latency_p50_s, latency_p99_s, latency_mean_s, max_paths_in_flight;
+} cwp_serve_stats;
+void cwp_server_options_default(cwp_server_options* o);
+int cwp_server_create(const cwp_server_options* o, cwp_server** out);
+void cwp_server_destroy(cwp_server* s);
+/* copies the setup's scene, camera and lighting (a single-level scene in any BVH layout: the server builds its own
+ * layout from the mesh when they differ); scene_bytes (optional) receives the device bytes the scene takes when
+ * resident in the server's layout, counted against the scene budget */
+int cwp_server_add_scene(cwp_server* s, const cwp_setup* setup, int* scene_id, size_t* scene_bytes);
+int cwp_server_submit(cwp_server* s, int scene_id, int width, int height, int spp, int bounces, uint32_t seed,
+                      double arrival_s, int* job_id);
+/* processes every submitted unfinished job; batched overrides the option for this run */
+int cwp_server_run(cwp_server* s, int batched);
+int cwp_server_job(const cwp_server* s, int job_id, cwp_job_info* info);
+/* per-pixel radiance sums of a finished job (divide by spp), float[4] per pixel */
+int cwp_server_image(const cwp_server* s, int job_id, float* rgba, size_t num_pixels);
+int cwp_server_stats(const cwp_server* s, cwp_serve_stats* stats);
+int cwp_server_clear_jobs(cwp_server* s);
+/* marks every job unfinished again (images dropped), e.g. to run the same stream in the other mode */
+int cwp_server_reset_jobs(cwp_server* s);
+/* Autotuner: runs the submitted jobs, batched, under every combination of BVH width (variants: 2, 4, 8), LDS stack
+ * entries (8, 16, 32) and rays per lane (1, 2, 4), keeps the fastest and leaves the jobs unfinished. trials receives
+ * up to max_trials results (num_trials = the number run); chosen (optional) receives the server's new options. */
+int cwp_server_autotune(cwp_server* s, const int* variants, int num_variants, const int* stacks, int num_stacks,
+                        const int* rays_per_lane, int num_rays_per_lane, cwp_tune_trial* trials, int max_trials,
+                        int* num_trials, cwp_server_options* chosen);
+int cwp_server_get_options(const cwp_server* s, cwp_server_options* o);
 
 /* ---- device memory for callers without a GPU array library ---- */
 enum { CWP_COPY_H2D = 1, CWP_COPY_D2H = 2, CWP_COPY_D2D = 3 };

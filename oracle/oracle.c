@@ -543,7 +543,7 @@ void orc_pack_corners(const float* v, const int* tri, const int* mat, const int*
 typedef struct {
     f3 o, d; float tmin; float ix, iy, iz; uint32_t oct; int kx, ky, kz; float Sx, Sy, Sz;
 } ray_t;
-typedef struct { float t, u, v; int tri; } hit_t;
+typedef struct { float t, u, v; int tri, inst; } hit_t;
 static float safe_rcp(float a) { const float e = 1e-20f; return 1.f / (fabsf(a) > e ? a : copysignf(e, a)); }
 static ray_t mkray(f3 o, f3 d, float tmin, int tri_test) {
     ray_t r; memset(&r, 0, sizeof r);
@@ -622,8 +622,8 @@ static int tri_test(const orc_scene* s, int t, const ray_t* r, hit_t* h) {
 }
 
 /* ------------------------------------------------------------------ BVH2 traversal: src/accel/traverse.cuh:45-70 */
-static int trav2(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
-    int stack[128], sp = 0, node = s->root;
+static int trav2(const orc_scene* s, int root, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    int stack[128], sp = 0, node = root;
     const float odx = r->o.x * r->ix, ody = r->o.y * r->iy, odz = r->o.z * r->iz;
     for (;;) {
         const float* n = s->nodes + 16 * (size_t)node;
@@ -655,10 +655,10 @@ static int trav2(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
 
 /* ---------------------------------------------------------------- BVH8 traversal: src/accel/traverse.cuh:112-152 */
 static uint32_t byte_of(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
-static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+static int trav8(const orc_scene* s, int root, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
     uint32_t stB[128], stM[128]; int sp = 0;
     uint32_t gB = 0, gM = 0, gH = 0;
-    int node = s->root;
+    int node = root;
     for (;;) {
         const float* n = s->nodes + 20 * (size_t)node;
         if (nv) *nv += 1;
@@ -713,8 +713,8 @@ static int trav8(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
 /* the 4 child slabs against the hit distance at node entry, the hit leaves in slot order, then the internal hits that
  * can still hold a closer hit (entry distance <= current hit distance), nearest first: sorted by entry distance with
  * the reference's 5-comparator network (ties keep the network's order), the far ones pushed farthest first */
-static int trav4(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
-    int stack[256], sp = 0, node = s->root;
+static int trav4(const orc_scene* s, int root, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    int stack[256], sp = 0, node = root;
     const float odx = r->o.x * r->ix, ody = r->o.y * r->iy, odz = r->o.z * r->iz;
     const float INF = 3.4e38f;
     for (;;) {
@@ -752,10 +752,83 @@ static int trav4(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* 
     return h->tri >= 0;
 }
 
+static int trace_blas(const orc_scene* s, int root, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    if (s->variant == 2) return trav2(s, root, r, h, any, nv, nt);
+    if (s->variant == 4) return trav4(s, root, r, h, any, nv, nt);
+    return trav8(s, root, r, h, any, nv, nt);
+}
+
+/* ------------------------------------------------ two-level traversal: src/accel/traverse_inst.cuh:10-41 */
+/* rows of an instance record (28 floats: inv rows 0-11, fwd rows 12-23, root, mesh, det sign, pad) applied to a point
+ * or a vector, summed left to right */
+static f3 xf_p(const float* r, f3 p) {
+    return mk(r[0] * p.x + r[1] * p.y + r[2] * p.z + r[3], r[4] * p.x + r[5] * p.y + r[6] * p.z + r[7],
+              r[8] * p.x + r[9] * p.y + r[10] * p.z + r[11]);
+}
+static f3 xf_v(const float* r, f3 p) {
+    return mk(r[0] * p.x + r[1] * p.y + r[2] * p.z, r[4] * p.x + r[5] * p.y + r[6] * p.z, r[8] * p.x + r[9] * p.y + r[10] * p.z);
+}
+static f3 xf_n(const float* r, f3 n) {   /* inv^T n */
+    return mk(r[0] * n.x + r[4] * n.y + r[8] * n.z, r[1] * n.x + r[5] * n.y + r[9] * n.z, r[2] * n.x + r[6] * n.y + r[10] * n.z);
+}
+/* TLAS: BVH2 over instance ids (leaf count = instances); an instance is entered with its object-space ray
+ * (o' = inv o, d' = inv d, not normalised: hit distances are shared); an instance that lowers the hit distance becomes
+ * the hit instance */
+static int trav_inst(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
+    int stack[128], sp = 0, node = s->tlas_root;
+    const float odx = r->o.x * r->ix, ody = r->o.y * r->iy, odz = r->o.z * r->iz;
+    for (;;) {
+        const float* n = s->tlas + 16 * (size_t)node;
+        int c[2] = { f2i(n[12]), f2i(n[13]) }, k[2] = { f2i(n[14]), f2i(n[15]) };
+        float a0 = fmaf(n[0], r->ix, -odx), a1 = fmaf(n[1], r->ix, -odx), a2 = fmaf(n[2], r->iy, -ody), a3 = fmaf(n[3], r->iy, -ody);
+        float a4 = fmaf(n[8], r->iz, -odz), a5 = fmaf(n[9], r->iz, -odz);
+        float b0 = fmaf(n[4], r->ix, -odx), b1 = fmaf(n[5], r->ix, -odx), b2 = fmaf(n[6], r->iy, -ody), b3 = fmaf(n[7], r->iy, -ody);
+        float b4 = fmaf(n[10], r->iz, -odz), b5 = fmaf(n[11], r->iz, -odz);
+        float tn[2], tf[2];
+        tn[0] = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), r->tmin));
+        tf[0] = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), h->t));
+        tn[1] = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), r->tmin));
+        tf[1] = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), h->t));
+        int hit[2] = { tn[0] <= tf[0], tn[1] <= tf[1] };
+        for (int q = 0; q < 2; q++) {
+            if (!(hit[q] && k[q] > 0)) continue;
+            for (int t = c[q]; t < c[q] + k[q]; t++) {
+                const float* I = s->inst + 28 * (size_t)t;
+                ray_t rb = mkray(xf_p(I, r->o), xf_v(I, r->d), r->tmin, s->tri_test);
+                const float before = h->t;
+                int got = trace_blas(s, f2i(I[24]), &rb, h, any, nv, nt);
+                if (h->t < before) h->inst = t;
+                if (any && got) return 1;
+            }
+            hit[q] = 0;
+        }
+        if (!hit[0] && !hit[1]) { if (sp == 0) break; node = stack[--sp]; }
+        else if (hit[0] && hit[1]) { if (tn[1] < tn[0]) { int x = c[0]; c[0] = c[1]; c[1] = x; } stack[sp++] = c[1]; node = c[0]; }
+        else node = hit[0] ? c[0] : c[1];
+    }
+    return h->tri >= 0;
+}
+
 static int trace_one(const orc_scene* s, const ray_t* r, hit_t* h, int any, double* nv, double* nt) {
-    if (s->variant == 2) return trav2(s, r, h, any, nv, nt);
-    if (s->variant == 4) return trav4(s, r, h, any, nv, nt);
-    return trav8(s, r, h, any, nv, nt);
+    if (s->tlas) return trav_inst(s, r, h, any, nv, nt);
+    return trace_blas(s, s->root, r, h, any, nv, nt);
+}
+
+void orc_inst_boxes(const float* inst, int n, const float* mesh_box, float* lo, float* hi) {
+    for (int i = 0; i < n; i++) {
+        const float* I = inst + 28 * (size_t)i;
+        const int m = f2i(I[25]);
+        const float* bl = mesh_box + 8 * (size_t)m;
+        const float* bh = bl + 4;
+        f3 mn = mk(3.4e38f, 3.4e38f, 3.4e38f), mx = mk(-3.4e38f, -3.4e38f, -3.4e38f);
+        for (int c = 0; c < 8; c++) {
+            f3 w = xf_p(I + 12, mk(c & 1 ? bh[0] : bl[0], c & 2 ? bh[1] : bl[1], c & 4 ? bh[2] : bl[2]));
+            mn = mk(w.x < mn.x ? w.x : mn.x, w.y < mn.y ? w.y : mn.y, w.z < mn.z ? w.z : mn.z);
+            mx = mk(w.x > mx.x ? w.x : mx.x, w.y > mx.y ? w.y : mx.y, w.z > mx.z ? w.z : mx.z);
+        }
+        lo[4 * i] = mn.x; lo[4 * i + 1] = mn.y; lo[4 * i + 2] = mn.z; lo[4 * i + 3] = 0.f;
+        hi[4 * i] = mx.x; hi[4 * i + 1] = mx.y; hi[4 * i + 2] = mx.z; hi[4 * i + 3] = 0.f;
+    }
 }
 
 /* --------------------------------------------------------------------------------------------- threading helper */
@@ -771,26 +844,32 @@ static void run_threads(void (*fn)(void*, int, int), void* arg, int threads) {
     free(th); free(jobs);
 }
 
-typedef struct { const orc_scene* s; const float* rays; int n, any, brute; float* hits; } trace_job;
+typedef struct { const orc_scene* s; const float* rays; int n, any, brute; float* hits; int* inst; } trace_job;
 static void trace_worker(void* a, int tid, int nth) {
     trace_job* J = (trace_job*)a;
     for (int k = tid; k < J->n; k += nth) {
         const float* R = J->rays + 8 * (size_t)k;
         ray_t r = mkray(ld3(R), ld3(R + 4), R[3], J->s->tri_test);
-        hit_t h = { R[7], 0.f, 0.f, -1 };
+        hit_t h = { R[7], 0.f, 0.f, -1, -1 };
         if (J->brute) { for (int t = 0; t < J->s->num_tris; t++) tri_test(J->s, t, &r, &h); }
         else trace_one(J->s, &r, &h, J->any, NULL, NULL);
         float* o = J->hits + 4 * (size_t)k;
         o[0] = h.t; o[1] = h.u; o[2] = h.v; o[3] = i2f(h.tri);
+        if (J->inst) J->inst[k] = h.tri >= 0 ? h.inst : -1;
     }
 }
 int orc_trace(const orc_scene* s, const float* rays, int n, int any, float* hits, int threads) {
-    trace_job J = { s, rays, n, any, 0, hits };
+    trace_job J = { s, rays, n, any, 0, hits, NULL };
+    run_threads(trace_worker, &J, threads);
+    return 0;
+}
+int orc_trace_inst(const orc_scene* s, const float* rays, int n, int any, float* hits, int* inst, int threads) {
+    trace_job J = { s, rays, n, any, 0, hits, inst };
     run_threads(trace_worker, &J, threads);
     return 0;
 }
 int orc_trace_brute(const orc_scene* s, const float* rays, int n, float* hits, int threads) {
-    trace_job J = { s, rays, n, 0, 1, hits };
+    trace_job J = { s, rays, n, 0, 1, hits, NULL };
     run_threads(trace_worker, &J, threads);
     return 0;
 }
@@ -799,7 +878,7 @@ int orc_trace_work(const orc_scene* s, const float* rays, int n, double* node_vi
     for (int k = 0; k < n; k++) {
         const float* R = rays + 8 * (size_t)k;
         ray_t r = mkray(ld3(R), ld3(R + 4), R[3], s->tri_test);
-        hit_t h = { R[7], 0.f, 0.f, -1 };
+        hit_t h = { R[7], 0.f, 0.f, -1, -1 };
         trace_one(s, &r, &h, 0, node_visits, tri_tests);
     }
     return 0;
@@ -835,18 +914,20 @@ static f3 trace_path(const orc_scene* s, const orc_params* p, int pix, uint32_t 
     const int depths = bounces + 2;
     for (int depth = 0; depth < depths; depth++) {
         ray_t r = mkray(o, d, depth == 0 ? p->near_t : 0.f, s->tri_test);
-        hit_t h = { 1e30f, 0.f, 0.f, -1 };
+        hit_t h = { 1e30f, 0.f, 0.f, -1, -1 };
         trace_one(s, &r, &h, 0, NULL, NULL);
         st[0] += 1;
         if (h.tri < 0) { st[2] += 1; L = add(L, mul(T, sky)); break; }
         if (depth > p->max_bounce) break;
         f3 ph = add(o, scl(d, h.t));
         const float* P = s->corners + 12 * h.tri;
+        const float* XI = s->tlas ? s->inst + 28 * (size_t)h.inst : NULL;   /* two-level: normals through inv^T */
         f3 e1 = sub(ld3(P + 4), ld3(P)), e2 = sub(ld3(P + 8), ld3(P));
-        f3 ng = nrm3(crs(e1, e2)), ns = ng;
+        f3 ng = XI ? nrm3(xf_n(XI, crs(e1, e2))) : nrm3(crs(e1, e2)), ns = ng;
         if (p->has_normals && s->normals) {
             const float* N = s->normals + 12 * h.tri;
             f3 nn = add(add(scl(ld3(N), 1.f - h.u - h.v), scl(ld3(N + 4), h.u)), scl(ld3(N + 8), h.v));
+            if (XI) nn = xf_n(XI, nn);
             float l = sqrtf(dot3(nn, nn));
             if (l > 0.f) ns = scl(nn, 1.f / l);
             if (dot3(ns, ng) < 0.f) ns = neg(ns);
@@ -861,7 +942,7 @@ static f3 trace_path(const orc_scene* s, const orc_params* p, int pix, uint32_t 
         if (cl > 0.f && dot3(ng, l) > 0.f) {
             f3 contrib = scl(mul(mul(T, alb), sunE), cl * 0.31830989f);
             ray_t sr = mkray(po, l, 0.f, s->tri_test);
-            hit_t sh = { 1e30f, 0.f, 0.f, -1 };
+            hit_t sh = { 1e30f, 0.f, 0.f, -1, -1 };
             st[1] += 1;
             if (!trace_one(s, &sr, &sh, 1, NULL, NULL)) { st[3] += 1; L = add(L, contrib); }
         }

@@ -41,6 +41,13 @@ typedef struct orc_scene {
     const float* normals;   /* 3 float4 per leaf position or NULL */
     const float* albedo;    /* float4 per material */
     int num_tris;
+    /* two-level scenes (tlas != NULL): BVH2 TLAS over instance ids, its root, instance records (28 floats each:
+     * world-to-object rows, object-to-world rows, BLAS root bits, mesh bits, det sign, pad); nodes then hold every
+     * BLAS with absolute indices */
+    const float* tlas;
+    int tlas_root;
+    const float* inst;
+    int num_inst;
 } orc_scene;
 
 /* binary SAH tree: node = int[4] (left, right, first, count) + float[6] box (lo, hi); prim = triangle order */
@@ -95,6 +102,11 @@ int orc_tree_from_hploc(const float* hn, int n, int root, int leaf_max, orc_tree
 /* ---- traversal: src/accel/traverse.cuh ---- */
 /* rays: 8 floats (o, tmin, d, tmax); hits: 4 floats (t, u, v, leaf position bits, -1 = miss); returns 0 on success */
 int orc_trace(const orc_scene* s, const float* rays, int n, int any_hit, float* hits, int threads);
+/* the same with the hit instance of two-level scenes (-1 on a miss) */
+int orc_trace_inst(const orc_scene* s, const float* rays, int n, int any_hit, float* hits, int* inst, int threads);
+/* world boxes of instances (TLAS input): the 8 corners of the mesh box (mesh_box: lo, hi float4 per mesh) through the
+ * object-to-world rows, min / max (reference: k_inst_boxes, src/render/wavefront.cu:445-451) */
+void orc_inst_boxes(const float* inst, int n, const float* mesh_box, float* lo, float* hi);
 /* closest hit by testing every triangle (same triangle test), for pinning the traversal */
 int orc_trace_brute(const orc_scene* s, const float* rays, int n, float* hits, int threads);
 /* node visits and triangle tests of a closest-hit traversal, summed over the rays (cache-side traffic model) */

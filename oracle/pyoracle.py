@@ -29,7 +29,8 @@ class OrcLayout(ctypes.Structure):
 
 class OrcScene(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("nodes", _FP), ("root", ctypes.c_int),
-                ("woop", _FP), ("corners", _FP), ("normals", _FP), ("albedo", _FP), ("num_tris", ctypes.c_int)]
+                ("woop", _FP), ("corners", _FP), ("normals", _FP), ("albedo", _FP), ("num_tris", ctypes.c_int),
+                ("tlas", _FP), ("tlas_root", ctypes.c_int), ("inst", _FP), ("num_inst", ctypes.c_int)]
 
 
 _lib = None
@@ -64,6 +65,8 @@ def lib():
         L.orc_camera_ray.argtypes = [_P, ctypes.c_int, ctypes.c_uint32, _FP, _FP]
         _U64P = ctypes.POINTER(ctypes.c_uint64)
         L.orc_hp_centroid_box.argtypes = [_FP, _FP, ctypes.c_int, _FP]
+        L.orc_trace_inst.argtypes = [ctypes.POINTER(OrcScene), _FP, ctypes.c_int, ctypes.c_int, _FP, _IP, ctypes.c_int]
+        L.orc_inst_boxes.argtypes = [_FP, ctypes.c_int, _FP, _FP, _FP]
         L.orc_expand_bits21.argtypes = [ctypes.c_uint32]; L.orc_expand_bits21.restype = ctypes.c_uint64
         L.orc_hp_key_bits.argtypes = [ctypes.c_int, _IP, _IP]
         L.orc_hp_keys.argtypes = [_FP, _FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, _U64P]
@@ -263,7 +266,33 @@ class Scene:
         self.albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1)
         self.normals = None if normals is None or len(normals) == 0 else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1)
         self.c = OrcScene(variant, tri_test, _f(self.nodes), root, _f(self.woop), _f(self.corners),
-                          _f(self.normals) if self.normals is not None else None, _f(self.albedo), len(self.corners) // 12)
+                          _f(self.normals) if self.normals is not None else None, _f(self.albedo), len(self.corners) // 12,
+                          None, 0, None, 0)
+
+    def set_instances(self, inst, mesh_box):
+        """Two-level scene: instance records (n, 28) float32 (world-to-object rows, object-to-world rows, root bits,
+        mesh bits, det sign, pad) and object-space mesh boxes (m, 8) (lo, hi). Builds the TLAS with the oracle's H-PLOC
+        restatement over the instance boxes; returns the TLAS BVH2 nodes."""
+        self.inst = np.ascontiguousarray(inst, dtype=np.float32).reshape(-1, 28)
+        mb = np.ascontiguousarray(mesh_box, dtype=np.float32).reshape(-1, 8)
+        n = len(self.inst)
+        lo, hi = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+        lib().orc_inst_boxes(_f(self.inst), n, _f(mb), _f(lo), _f(hi))
+        hn, root, _ = hp_build(lo, hi)
+        self.tlas = hp_to_bvh2(hn, n)
+        self.inst_boxes = (lo, hi)
+        self.c.tlas = _f(self.tlas)
+        self.c.tlas_root = n - 2 if n > 1 else 0
+        self.c.inst = _f(self.inst)
+        self.c.num_inst = n
+        return self.tlas
+
+    def trace_inst(self, rays, any_hit=False, threads=8):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        h = np.zeros((r.shape[0], 4), dtype=np.float32)
+        ids = np.zeros(r.shape[0], dtype=np.int32)
+        lib().orc_trace_inst(ctypes.byref(self.c), _f(r), r.shape[0], 1 if any_hit else 0, _f(h), _i(ids), threads)
+        return h, ids
 
     def trace(self, rays, any_hit=False, threads=8):
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -1,0 +1,97 @@
+"""GPU parity tests of two-level scenes (TLAS over instances rebuilt by the GPU H-PLOC builder each frame, BLAS per
+mesh) against the oracle's two-level traversal and renderer, and against the flattened scene."""
+import math
+
+import numpy as np
+import pytest
+
+import cwp
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ARGS = ["--gen", "instances", "--gen_small", "1", "--gen_instances", "120", "--width", "96", "--height", "64"]
+
+
+def psnr(a, b):
+    def srgb(x):
+        x = np.clip(x, 0, 1)
+        return np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1 / 2.4) - 0.055)
+    mse = float(np.mean((srgb(a.astype(np.float64)) - srgb(b.astype(np.float64))) ** 2))
+    return 99.0 if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+def oracle_scene(s, frame, tri_test=1):
+    v = s.info["variant"]
+    sc = po.Scene(v, tri_test, s.array(cwp.ARR_NODES), s.array(cwp.ARR_WOOP), s.array(cwp.ARR_CORNERS),
+                  s.array(cwp.ARR_ALBEDO))
+    sc.set_instances(s.instances(frame), s.array(cwp.ARR_MESH_BOXES))
+    return sc
+
+
+def scene_rays(s, n, seed):
+    p = s.params()
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        o, d = po.camera_ray(p, int(rng.integers(p.w * p.h)), int(rng.integers(256)))
+        out.append(np.concatenate([o, [0.0], d, [1e30]]))
+    cam = np.array(out, np.float32)
+    o = rng.uniform([-12, -12, -1], [12, 12, 3], (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rnd = np.concatenate([o, np.zeros((n, 1)), d, np.full((n, 1), 1e30)], 1).astype(np.float32)
+    return np.concatenate([cam, rnd])
+
+
+@pytest.mark.parametrize("variant,tri", [("bvh8", "woop"), ("bvh2", "watertight"), ("bvh4", "mt")])
+def test_two_level_trace_matches_oracle(variant, tri):
+    print("case", variant, tri, flush=True)
+    s = cwp.Setup(ARGS + ["--variant", variant, "--tri", tri])
+    r = cwp.Renderer(s)
+    sc = oracle_scene(s, 0, {"mt": 0, "woop": 1, "watertight": 2}[tri])
+    rays = scene_rays(s, 20000, 3)
+    g, gi = r.trace_inst(rays)
+    o, oi = sc.trace_inst(rays)
+    gt, ot = g[:, 3].view(np.int32), o[:, 3].view(np.int32)
+    assert (ot >= 0).mean() > 0.3
+    agree = ((gt == ot) & (gi == oi)).mean()
+    assert agree >= 0.9999, agree
+    both = (gt == ot) & (ot >= 0)
+    assert (np.abs(g[both, 0] - o[both, 0]) / np.abs(o[both, 0])).max() <= 1e-5
+    ga, _ = r.trace_inst(rays, any_hit=True)
+    assert ((ga[:, 3].view(np.int32) >= 0) == (gt >= 0)).mean() >= 0.9999
+
+
+def test_tlas_rebuilt_per_frame_matches_oracle():
+    s = cwp.Setup(ARGS)
+    r = cwp.Renderer(s)
+    rays = scene_rays(s, 8000, 5)
+    for f in (1, 7, 59):
+        ms = r.set_frame(f)
+        assert ms > 0
+        g, gi = r.trace_inst(rays)
+        o, oi = oracle_scene(s, f).trace_inst(rays)
+        assert ((g[:, 3].view(np.int32) == o[:, 3].view(np.int32)) & (gi == oi)).mean() >= 0.9999
+    with pytest.raises(cwp.CwpError, match="BLAS root"):
+        bad = s.instances(0)
+        bad.view(np.int32)[3, 24] += 1
+        r.set_instances(bad)
+
+
+def test_two_level_render_matches_oracle_and_flattened():
+    spp, batch, bounces = 8, 8, 4
+    extra = ["--batch", str(batch), "--bounces", str(bounces), "--spp", str(spp)]
+    s = cwp.Setup(ARGS + extra)
+    r = cwp.Renderer(s)
+    st = r.render(spp)
+    img = r.accum()[..., :3] / spp
+    j = r.frame_json(st)
+    assert j["instances"] == s.info["num_instances"] and j["tlas_ms"] > 0
+    acc, ost = oracle_scene(s, 0).render(s.params(), spp, batch=batch, bounces=bounces)
+    assert psnr(img, acc[..., :3] / spp) >= 50.0
+    assert st["rays_closest"] == pytest.approx(ost[0], rel=1e-3)
+    # the same geometry flattened into world space: equal up to transform rounding
+    f = cwp.Setup(ARGS + extra + ["--flatten", "1"])
+    rf = cwp.Renderer(f)
+    rf.render(spp)
+    assert psnr(img, rf.accum()[..., :3] / spp) >= 45.0

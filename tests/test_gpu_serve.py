@@ -1,0 +1,192 @@
+"""GPU tests of the render-job server (cwp_server_*): batched multi-job passes and continuous batching.
+
+A job's image is a pure function of the job (scene, resolution, samples, bounce limit, seed) and of its samples per
+pass: each path's arithmetic and each pixel's summation order are those of the job rendered alone. So images are
+compared bit-exact between a job alone, the same job in a mix, and the plain one-job-at-a-time mode; a job at the
+scene's own resolution is bit-identical to the single-job renderer's image at the same samples per pass, and reaches
+>= 50 dB PSNR against the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cwp
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TERRAIN = ["--gen", "terrain", "--gen_grid", "40", "--gen_boxes", "20", "--width", "64", "--height", "48"]
+BOXES = ["--gen", "boxes", "--gen_boxes", "64", "--width", "32", "--height", "32", "--seed", "7"]
+
+# (scene, width, height, spp, bounces, seed): ragged sizes, odd sample counts, different bounce limits
+JOBS = [(0, 64, 48, 4, 4, 1), (1, 32, 32, 8, 3, 2), (0, 50, 30, 6, 4, 3), (1, 64, 64, 2, 2, 4), (0, 17, 9, 5, 4, 5),
+        (1, 1, 1, 3, 4, 6), (0, 64, 48, 4, 0, 7)]
+
+
+def psnr(a, b):
+    def srgb(x):
+        x = np.clip(x, 0, 1)
+        return np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1 / 2.4) - 0.055)
+    mse = float(np.mean((srgb(a.astype(np.float64)) - srgb(b.astype(np.float64))) ** 2))
+    return 99.0 if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+def setups(variant=8):
+    return [cwp.Setup(a + ["--variant", "bvh%d" % variant, "--bounces", "4"]) for a in (TERRAIN, BOXES)]
+
+
+def run_jobs(srv, jobs, batched, arrivals=None):
+    srv.clear_jobs()
+    ids = [srv.submit(*j[:5], seed=j[5], arrival_s=(arrivals[k] if arrivals else 0.0)) for k, j in enumerate(jobs)]
+    st = srv.run(batched=batched)
+    return [srv.image(i) for i in ids], [srv.job(i) for i in ids], st
+
+
+@pytest.mark.parametrize("variant,max_paths,graph,rpl", [(8, 1 << 20, 1, 1), (8, 1 << 14, 1, 1), (8, 1 << 14, 0, 4),
+                                                          (4, 1 << 20, 1, 2), (2, 1 << 14, 1, 1)])
+def test_job_alone_equals_job_in_mix(variant, max_paths, graph, rpl):
+    """Each job alone, all jobs in one batched run, and the plain mode give bit-identical images; with 2^14 path
+    slots the larger jobs take several passes and wait for slots to free up. Passes replayed from a captured graph
+    or launched one kernel at a time, and any rays per lane, trace the same paths."""
+    print("case", variant, max_paths, graph, rpl, flush=True)
+    S = setups(variant)
+    srv = cwp.Server(variant=variant, bounces=4, max_paths=max_paths, use_graph=graph, rays_per_lane=rpl)
+    for s in S:
+        srv.add_scene(s)
+    alone = []
+    for j in JOBS:
+        img, info, _ = run_jobs(srv, [j], True)
+        alone.append(img[0])
+        assert info[0]["passes"] * info[0]["samples_per_pass"] == j[3]
+    mix, infos, st = run_jobs(srv, JOBS, True)
+    assert st["jobs"] == len(JOBS)
+    for k, (a, b) in enumerate(zip(alone, mix)):
+        np.testing.assert_array_equal(a, b, err_msg="job %d" % k)
+    plain, _, stp = run_jobs(srv, JOBS, False)
+    for k, (a, b) in enumerate(zip(alone, plain)):
+        np.testing.assert_array_equal(a, b, err_msg="plain job %d" % k)
+    assert st["rays"] == stp["rays"]             # the same paths, traced in a different grouping
+    assert stp["passes"] >= st["passes"]
+    if max_paths == 1 << 14:
+        assert max(i["passes"] for i in infos) > 1
+        assert st["max_paths_in_flight"] <= max_paths
+
+
+def test_job_matches_single_job_renderer_and_oracle():
+    """A job at the scene's own resolution, bounce limit and seed equals the single-job renderer's image at the same
+    samples per pass bit for bit, and matches the oracle's rendering at >= 50 dB."""
+    spp, bounces = 8, 4
+    base = TERRAIN + ["--bounces", str(bounces), "--spp", str(spp), "--batch", str(spp), "--seed", "1000"]
+    s = cwp.Setup(base)
+    r = cwp.Renderer(s)
+    r.render(spp)
+    ref = r.accum()
+    r.close()
+    srv = cwp.Server(variant=8, bounces=bounces, max_paths=1 << 20)
+    sid = srv.add_scene(s)
+    img, info, st = run_jobs(srv, [(sid, 64, 48, spp, bounces, 1000)], True)
+    assert info[0]["samples_per_pass"] == spp
+    np.testing.assert_array_equal(img[0], ref)
+    sc, _ = po.scene_from_setup(s)
+    acc, _ = sc.render(s.params(), spp, batch=spp, bounces=bounces)
+    assert psnr(img[0][..., :3] / spp, acc[..., :3] / spp) >= 50.0
+
+
+def test_lru_eviction_keeps_results():
+    """A scene budget that holds one scene at a time: jobs alternate scenes, scenes are evicted and re-uploaded, and
+    every image is unchanged."""
+    S = setups(8)
+    probe = cwp.Server(variant=8, bounces=4, max_paths=1 << 10)
+    for s in S:
+        probe.add_scene(s)
+    sizes = list(probe.scene_bytes.values())
+    probe.close()
+    assert max(sizes) < sum(sizes)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20, scene_budget_mb=max(sizes) * 1.01 / 1048576.0)
+    for s in S:
+        srv.add_scene(s)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in JOBS[:4]]
+    mix, _, st = run_jobs(srv, JOBS[:4], False)
+    assert st["scenes_evicted"] >= 2
+    for a, b in zip(alone, mix):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_arrivals_and_latency():
+    """Jobs arriving over time are admitted when they arrive; latency = finish - arrival; percentiles ordered."""
+    S = setups(8)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20)
+    for s in S:
+        srv.add_scene(s)
+    arr = [0.0, 0.01, 0.02, 0.05, 0.05, 0.08, 0.1]
+    imgs, infos, st = run_jobs(srv, JOBS, True, arrivals=arr)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in JOBS]
+    for a, b in zip(alone, imgs):
+        np.testing.assert_array_equal(a, b)
+    for i, a in zip(infos, arr):
+        assert i["admitted_s"] >= a and i["finished_s"] >= i["admitted_s"]
+        assert i["latency_s"] == pytest.approx(i["finished_s"] - a)
+    assert 0 < st["latency_p50_s"] <= st["latency_p99_s"]
+    assert st["wall_s"] >= 0.1 and st["rays_per_s"] > 0
+
+
+def test_server_errors():
+    S = setups(8)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 12)
+    srv.add_scene(S[0])
+    with pytest.raises(cwp.CwpError):
+        srv.submit(5, 8, 8, 1)                 # unknown scene
+    with pytest.raises(cwp.CwpError):
+        srv.submit(0, 8, 8, 1, bounces=9)      # deeper than the server's bounce limit
+    with pytest.raises(cwp.CwpError):
+        srv.submit(0, 128, 128, 1)             # more pixels than path slots
+    jid = srv.submit(0, 8, 8, 1, bounces=2)
+    with pytest.raises(cwp.CwpError):
+        srv.image(jid)                         # not finished
+    with pytest.raises(cwp.CwpError):
+        cwp.Server(variant=8, stack_lds=12)
+    with pytest.raises(cwp.CwpError):
+        cwp.Server(variant=8, rays_per_lane=3)
+
+
+def test_scene_of_another_layout_is_relaid():
+    """A bvh4 setup added to a BVH8 server is rebuilt in the server's layout from its mesh: its jobs equal the jobs of
+    the bvh8 setup of the same scene."""
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20)
+    a = srv.add_scene(cwp.Setup(TERRAIN + ["--variant", "bvh8", "--bounces", "4"]))
+    b = srv.add_scene(cwp.Setup(TERRAIN + ["--variant", "bvh4", "--bounces", "4"]))
+    assert srv.scene_bytes[a] == srv.scene_bytes[b]
+    imgs, _, _ = run_jobs(srv, [(a, 64, 48, 4, 4, 1), (b, 64, 48, 4, 4, 1)], True)
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
+def test_autotune_picks_a_configuration_and_keeps_results():
+    """The autotuner runs the mix under every (width, stack, rays per lane), keeps the fastest, leaves the jobs
+    unfinished; the next run under the chosen configuration gives the same images (bit-exact for the BVH8 layout,
+    >= 50 dB for BVH4, whose closest hits can differ only among triangles at equal distance)."""
+    S = setups(8)
+    ref_srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20)
+    for s in S:
+        ref_srv.add_scene(s)
+    ref, _, _ = run_jobs(ref_srv, JOBS, True)
+    ref_srv.close()
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20)
+    for s in S:
+        srv.add_scene(s)
+    srv.clear_jobs()
+    ids = [srv.submit(*j[:5], seed=j[5]) for j in JOBS]
+    trials, chosen = srv.autotune(variants=(4, 8), stacks=(8, 16), rays_per_lane=(1, 2))
+    assert len(trials) == 8 and all(t["rays_per_s"] > 0 for t in trials)
+    best = max(trials, key=lambda t: t["rays_per_s"])
+    assert (chosen["variant"], chosen["stack_lds"], chosen["rays_per_lane"]) == \
+        (best["variant"], best["stack_lds"], best["rays_per_lane"])
+    assert all(srv.job(i)["finished_s"] < 0 for i in ids)
+    srv.run(batched=True)
+    for i, r in zip(ids, ref):
+        img = srv.image(i)
+        if chosen["variant"] == 8:
+            np.testing.assert_array_equal(img, r)
+        else:
+            spp = srv.job(i)["spp"]
+            assert psnr(img[..., :3] / spp, r[..., :3] / spp) >= 50.0

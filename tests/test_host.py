@@ -231,3 +231,99 @@ def test_scene_check_accepts_hploc_bvh2_with_root_last():
     assert need >= 2
     with pytest.raises(cwp.CwpError, match="root has a parent"):
         cwp.scene_check(2, bvh2, corners, s.array(cwp.ARR_ALBEDO), root=0)
+
+
+def _rows(rec, which):
+    r = rec[:, 0:12] if which == "inv" else rec[:, 12:24]
+    return r.reshape(-1, 3, 4).astype(np.float64)
+
+
+def test_scene_file_instances_and_transforms(tmp_path, golden_dir):
+    """Two-level scene file (reference format): explicit matrix, parametric rotation / scale / per-frame motion,
+    per-frame matrices. Records: fwd = the transform, inv = its inverse, det sign of a mirrored instance, BLAS roots."""
+    import shutil
+    shutil.copy(os.path.join(golden_dir, "boxes.obj"), tmp_path / "boxes.obj")
+    shutil.copy(os.path.join(golden_dir, "boxes.mtl"), tmp_path / "boxes.mtl")
+    scene = {"frames": 4,
+             "meshes": [{"file": "boxes.obj"}, {"file": "boxes.obj"}],
+             "instances": [
+                 {"mesh": 0, "transform": [1, 0, 0, 10, 0, 1, 0, 0, 0, 0, 1, 0]},
+                 {"mesh": 1, "translate": [0, 5, 0], "axis": [0, 0, 1], "angle_deg": 90, "scale": 2,
+                  "translate_per_frame": [1, 0, 0], "angle_deg_per_frame": 90},
+                 {"mesh": 0, "scale": [-1, 1, 1]},
+                 {"mesh": 1, "transforms": [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], [1, 0, 0, 3, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]]}]}
+    (tmp_path / "scene.json").write_text(json.dumps(scene))
+    s = cwp.Setup(["--scene", str(tmp_path / "scene.json"), "--hfov", "40", "--c2w",
+                   "1,0,0,0,0,0,-1,-30,0,1,0,5,0,0,0,1", "--width", "16", "--height", "16"])
+    i = s.info
+    assert i["num_meshes"] == 2 and i["num_instances"] == 4 and i["frames"] == 4
+    roots = s.array(cwp.ARR_MESH_ROOTS)
+    assert len(roots) == 2 and roots[0] != roots[1]
+    r0 = s.instances(0)
+    ints = r0.view(np.int32)
+    np.testing.assert_array_equal(ints[:, 25], [0, 1, 0, 1])
+    np.testing.assert_array_equal(ints[:, 24], roots[ints[:, 25]])
+    fwd, inv = _rows(r0, "fwd"), _rows(r0, "inv")
+    assert fwd[0].tolist() == [[1, 0, 0, 10], [0, 1, 0, 0], [0, 0, 1, 0]]
+    # 90 degrees about z, scale 2, translate (0, 5, 0): x axis -> 2 y
+    np.testing.assert_allclose(fwd[1][:, :3] @ [1, 0, 0], [0, 2, 0], atol=1e-6)
+    np.testing.assert_allclose(fwd[1][:, 3], [0, 5, 0])
+    for k in range(4):   # inverse rows undo the forward rows
+        F = np.vstack([fwd[k], [0, 0, 0, 1]]); Iv = np.vstack([inv[k], [0, 0, 0, 1]])
+        np.testing.assert_allclose(Iv @ F, np.eye(4), atol=1e-5)
+    np.testing.assert_array_equal(r0[:, 26], [1, 1, -1, 1])   # mirrored instance: det sign -1
+    r2 = s.instances(2)
+    f2 = _rows(r2, "fwd")
+    np.testing.assert_allclose(f2[1][:, 3], [2, 5, 0])                       # 2 frames of (1, 0, 0) motion
+    np.testing.assert_allclose(f2[1][:, :3] @ [1, 0, 0], [0, -2, 0], atol=1e-6)   # 270 degrees
+    assert f2[3][0, 3] == 3 and _rows(s.instances(9), "fwd")[3][0, 3] == 3   # per-frame list clamps at its end
+    with pytest.raises(cwp.CwpError, match="out of range"):
+        bad = dict(scene, instances=[{"mesh": 5}])
+        (tmp_path / "bad.json").write_text(json.dumps(bad))
+        cwp.Setup(["--scene", str(tmp_path / "bad.json"), "--hfov", "40", "--c2w", "1,0,0,0,0,1,0,0,0,0,1,5,0,0,0,1"])
+
+
+def test_generated_instance_scene_and_oracle_two_level_trace_match_flattened():
+    """The generated two-level scene (5 meshes + ground, random rigid transforms per frame): the oracle's two-level
+    traversal agrees with the flattened scene (every instance transformed into world space) on hit / miss and
+    distance, and the transforms change between frames but not between runs."""
+    args = ["--gen", "instances", "--gen_small", "1", "--gen_instances", "60", "--width", "48", "--height", "32"]
+    s = cwp.Setup(args)
+    assert s.info["num_meshes"] == 6 and s.info["num_instances"] == 61
+    r0, r0b, r1 = s.instances(0), cwp.Setup(args).instances(0), s.instances(1)
+    np.testing.assert_array_equal(r0, r0b)
+    assert not np.array_equal(r0[1:, 12:24], r1[1:, 12:24])
+    fwd = _rows(r0, "fwd")[1:, :, :3]
+    np.testing.assert_allclose(np.einsum("nij,nkj->nik", fwd, fwd), np.broadcast_to(np.eye(3), fwd.shape), atol=1e-5)   # rigid
+    sc = po.Scene(8, 1, s.array(cwp.ARR_NODES), s.array(cwp.ARR_WOOP), s.array(cwp.ARR_CORNERS), s.array(cwp.ARR_ALBEDO))
+    sc.set_instances(r0, s.array(cwp.ARR_MESH_BOXES))
+    p = s.params()
+    rays = []
+    for k in range(3000):
+        o, d = po.camera_ray(p, (k * 7) % (p.w * p.h), k)
+        rays.append(np.concatenate([o, [0], d, [1e30]]))
+    rays = np.array(rays, np.float32)
+    h, ids = sc.trace_inst(rays)
+    f = cwp.Setup(args + ["--flatten", "1"])
+    fs, _ = po.scene_from_setup(f)
+    hf = fs.trace(rays)
+    gi, fi = h[:, 3].view(np.int32) >= 0, hf[:, 3].view(np.int32) >= 0
+    assert gi.mean() > 0.5 and (gi == fi).mean() >= 0.999
+    both = gi & fi
+    assert np.abs(h[both, 0] - hf[both, 0]).max() <= 1e-4 * np.abs(hf[both, 0]).max()
+    assert (ids[gi] >= 0).all() and (ids[~gi] == -1).all()
+
+
+def test_server_needs_a_device():
+    """The server's options carry the documented defaults; without a HIP device creation fails with a device error
+    (never a silent host fallback)."""
+    o = cwp.ServerOptions.default()
+    assert (o.variant, o.stack_lds, o.bounces, o.max_jobs, o.rays_per_lane, o.use_graph, o.batched) == (8, 8, 8, 256, 1, 1, 1)
+    assert o.max_paths == 1 << 26
+    try:
+        cwp.device_info()
+        pytest.skip("a HIP device is present")
+    except cwp.CwpError:
+        pass
+    with pytest.raises(cwp.CwpError, match="cwp error 3"):
+        cwp.Server(o)
