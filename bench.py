@@ -124,6 +124,135 @@ def bench_instances(args):
     print(json.dumps(out), flush=True)
 
 
+JOB_RES = (128, 256, 512, 1024)
+JOB_SPP = (4, 16, 64)
+
+
+def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000):
+    """Seeded scenes with triangle counts log-uniform in [lo_tris, hi_tris]: fBm heightfields of (G-1)^2 * 2
+    triangles, each with its own terrain seed. Returns the setup arguments of each scene."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_scenes):
+        t = float(np.exp(rng.uniform(np.log(lo_tris), np.log(hi_tris))))
+        g = int(round(np.sqrt(t / 2.0))) + 1
+        out.append(["--gen", "terrain", "--gen_grid", str(g), "--gen_boxes", "0", "--gen_seed", str(1 + k),
+                    "--width", "512", "--height", "512", "--bounces", "6", "--filter", "box"] + LIGHTING)
+    return out
+
+
+def job_stream(n_jobs, n_scenes, seed):
+    """(scene, width, height, spp, bounces, seed) of each job: scene uniform, square resolution uniform over
+    {128, 256, 512, 1024}, spp uniform over {4, 16, 64}, depth 6."""
+    rng = np.random.default_rng(seed + 1)
+    jobs = []
+    for j in range(n_jobs):
+        r = int(rng.choice(JOB_RES))
+        jobs.append((int(rng.integers(n_scenes)), r, r, int(rng.choice(JOB_SPP)), 6, 1000 + j))
+    return jobs
+
+
+def make_setups(arg_lists):
+    """Host scene setups built in parallel threads (the C ABI releases the interpreter lock)."""
+    import cwp
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(8, len(arg_lists))) as ex:
+        return list(ex.map(cwp.Setup, arg_lists))
+
+
+def run_stream(srv, jobs, batched, arrivals=None):
+    srv.clear_jobs()
+    ids = [srv.submit(*j[:5], seed=j[5], arrival_s=(arrivals[k] if arrivals is not None else 0.0))
+           for k, j in enumerate(jobs)]
+    st = srv.run(batched=batched)
+    return st, ids
+
+
+def warm_up(srv, jobs):
+    """Uploads every scene of the stream and runs a few jobs (kernels loaded, graphs captured)."""
+    srv.clear_jobs()
+    for j in jobs:
+        srv.submit(*j[:5], seed=j[5])
+    srv.preload()
+    run_stream(srv, jobs[:8], True)
+
+
+def bench_jobs(args):
+    """Mixed job stream: 64 render jobs over 8 seeded scenes (10k-2M triangles), resolutions 128^2-1024^2, 4-64 spp,
+    depth 6, all submitted at t = 0. The plain mode renders one job at a time (each job's passes alone on the GPU);
+    the batched mode runs every admitted job's paths in one trace+shade and one shadow launch per bounce, from a
+    captured graph, under the configuration the autotuner picked. Reports both rays/s, their ratio and job latency."""
+    import cwp
+    scene_args = job_stream_scenes(args.job_scenes, args.job_seed)
+    t0 = time.perf_counter()
+    setups = make_setups(scene_args)
+    tris = [s.info["num_tris"] for s in setups]
+    print("bench: %d scenes (%s tris) set up in %.1f s" % (len(setups), ",".join(str(t) for t in tris),
+                                                         time.perf_counter() - t0), file=sys.stderr, flush=True)
+    jobs = job_stream(args.jobs, len(setups), args.job_seed)
+    srv = cwp.Server(variant=8, bounces=6)
+    for s in setups:
+        srv.add_scene(s)
+    warm_up(srv, jobs)
+    plain, _ = run_stream(srv, jobs, False)
+    print("bench: plain %.3f Grays/s (%.2f s)" % (plain["rays_per_s"] / 1e9, plain["wall_s"]), file=sys.stderr, flush=True)
+    untuned, _ = run_stream(srv, jobs, True)
+    print("bench: batched (untuned) %.3f Grays/s" % (untuned["rays_per_s"] / 1e9), file=sys.stderr, flush=True)
+    trials, chosen = [], srv.get_options()
+    if args.autotune:
+        srv.clear_jobs()
+        for j in jobs[::4]:
+            srv.submit(*j[:5], seed=j[5])
+        trials, chosen = srv.autotune()
+        print("bench: autotuner chose bvh%d stack %d rays/lane %d" % (chosen["variant"], chosen["stack_lds"],
+                                                                     chosen["rays_per_lane"]), file=sys.stderr, flush=True)
+        warm_up(srv, jobs)   # scenes resident in the chosen layout
+    batched, ids = run_stream(srv, jobs, True)
+    print("bench: batched %.3f Grays/s (%.2f s)" % (batched["rays_per_s"] / 1e9, batched["wall_s"]), file=sys.stderr, flush=True)
+    srv.close()
+    return {"plain_grays": round(plain["rays_per_s"] / 1e9, 4), "batched_grays": round(batched["rays_per_s"] / 1e9, 4),
+            "batched_untuned_grays": round(untuned["rays_per_s"] / 1e9, 4),
+            "speedup": round(batched["rays_per_s"] / plain["rays_per_s"], 3),
+            "rays": batched["rays"], "plain_wall_s": round(plain["wall_s"], 3), "batched_wall_s": round(batched["wall_s"], 3),
+            "latency_s": {"plain_p50": round(plain["latency_p50_s"], 4), "plain_p99": round(plain["latency_p99_s"], 4),
+                          "batched_p50": round(batched["latency_p50_s"], 4), "batched_p99": round(batched["latency_p99_s"], 4)},
+            "passes": {"plain": plain["passes"], "batched": batched["passes"]},
+            "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"]},
+            "trials": [{"bvh": t["variant"], "stack": t["stack_lds"], "rpl": t["rays_per_lane"],
+                        "grays": round(t["rays_per_s"] / 1e9, 4)} for t in trials],
+            "workload": {"jobs": len(jobs), "scenes": len(setups), "tris": tris, "resolutions": list(JOB_RES),
+                         "spp": list(JOB_SPP), "depth": 6, "arrivals": "all at t=0", "seed": args.job_seed}}
+
+
+def bench_serve(args):
+    """Serving: the same job stream arriving as a Poisson process of rate lambda (jobs/s), continuous batching
+    (batched) against one job at a time (plain); rays/s and p50/p99 latency (finish - arrival) per rate."""
+    import cwp
+    setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed))
+    jobs = job_stream(args.jobs, len(setups), args.job_seed)
+    srv = cwp.Server(variant=8, bounces=6)
+    for s in setups:
+        srv.add_scene(s)
+    warm_up(srv, jobs)
+    rng = np.random.default_rng(args.job_seed + 2)
+    gaps = rng.exponential(1.0, len(jobs))
+    out = []
+    for lam in args.rates:
+        arr = np.cumsum(gaps) / lam
+        arr -= arr[0]
+        row = {"lambda": lam}
+        for mode, b in (("batched", True), ("plain", False)):
+            st, _ = run_stream(srv, jobs, b, arrivals=arr)
+            row[mode] = {"grays": round(st["rays_per_s"] / 1e9, 4), "p50_s": round(st["latency_p50_s"], 4),
+                         "p99_s": round(st["latency_p99_s"], 4), "wall_s": round(st["wall_s"], 3)}
+        print("bench: lambda %g: batched %.3f Grays/s p50 %.3f s p99 %.3f s | plain %.3f Grays/s p50 %.3f s p99 %.3f s"
+              % (lam, row["batched"]["grays"], row["batched"]["p50_s"], row["batched"]["p99_s"], row["plain"]["grays"],
+                 row["plain"]["p50_s"], row["plain"]["p99_s"]), file=sys.stderr, flush=True)
+        out.append(row)
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -144,9 +273,17 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, help="1: split kernels with lanes refilled mid-traversal")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--cpu-spp", type=int, default=40)
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "instances"],
-                    help="terrain: the diffuse frame (headline); instances: the two-level scene with the TLAS rebuilt "
-                         "on the GPU every frame (1,000 instances of 5 meshes, 720p, 64 spp, 60 frames)")
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve"],
+                    help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
+                         "instances: the two-level scene with the TLAS rebuilt on the GPU every frame (1,000 instances "
+                         "of 5 meshes, 720p, 64 spp, 60 frames); jobs: the mixed job stream only, batched vs plain; "
+                         "serve: the job stream with Poisson arrivals at each --rates")
+    ap.add_argument("--jobs", type=int, default=64, help="job stream: jobs")
+    ap.add_argument("--job-scenes", type=int, default=8, help="job stream: scenes (10k-2M triangles, log-uniform)")
+    ap.add_argument("--job-seed", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=1, help="job stream: run the autotuner before the batched run")
+    ap.add_argument("--rates", default="10,50,100,200,500", help="serve: Poisson arrival rates (jobs/s)")
+    ap.add_argument("--no-jobs", action="store_true", help="terrain workload without the job stream")
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -158,9 +295,20 @@ def main():
               "this build)", file=sys.stderr)
         sys.exit(2)
 
+    args.rates = [float(x) for x in args.rates.split(",") if x]
     import cwp
     if args.workload == "instances":
         bench_instances(args)
+        return
+    if args.workload in ("jobs", "serve"):
+        j = bench_jobs(args)
+        out = {"metric": "Grays/s of the mixed job stream, batched vs plain; p50/p99 job latency", "value": j["batched_grays"],
+               "unit": "Grays/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(j["batched_wall_s"] * 1e3, 3),
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": "job-stream-%d-jobs-%d-scenes-depth6" % (args.jobs, args.job_scenes)}, "jobs": j}
+        if args.workload == "serve":
+            out["serve"] = bench_serve(args)
+        print(json.dumps(out), flush=True)
         return
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
@@ -226,6 +374,11 @@ def main():
         "cpu_baseline": None,
     }
     r.close()
+    if not args.no_jobs:
+        print("bench: mixed job stream", file=sys.stderr, flush=True)
+        j = bench_jobs(args)
+        out["jobs"] = j
+        out["latency"] = dict(j["latency_s"], unit="s", speedup_batched_vs_plain=j["speedup"])
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args)

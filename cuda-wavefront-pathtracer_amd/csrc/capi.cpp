@@ -549,6 +549,13 @@ int cwp_server_clear_jobs(cwp_server* s) {
     return CWP_OK;
 }
 
+int cwp_server_preload(cwp_server* s) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_preload: null server");
+    std::string err;
+    if (!s->S.preload(err)) return fail(CWP_ERR_DEVICE, err);
+    return CWP_OK;
+}
+
 int cwp_server_reset_jobs(cwp_server* s) {
     if (!s) return fail(CWP_ERR_ARG, "cwp_server_reset_jobs: null server");
     s->S.resetJobs();

@@ -212,6 +212,7 @@ SIGNATURES = {
     "cwp_server_autotune": ([_P, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I,
                              ctypes.POINTER(TuneTrial), _I, ctypes.POINTER(_I), ctypes.POINTER(ServerOptions)], _I),
     "cwp_server_get_options": ([_P, ctypes.POINTER(ServerOptions)], _I),
+    "cwp_server_preload": ([_P], _I),
     "cwp_device_malloc": ([_S, ctypes.POINTER(_P)], _I),
     "cwp_device_free": ([_P], None),
     "cwp_memcpy": ([_P, _P, _S, _I], _I),
@@ -467,6 +468,9 @@ class Server:
 
     def reset_jobs(self):
         _check(lib().cwp_server_reset_jobs(self.h))
+
+    def preload(self):
+        _check(lib().cwp_server_preload(self.h))
 
     def get_options(self):
         o = ServerOptions()

@@ -285,6 +285,9 @@ int cwp_server_autotune(cwp_server* s, const int* variants, int num_variants, co
                         const int* rays_per_lane, int num_rays_per_lane, cwp_tune_trial* trials, int max_trials,
                         int* num_trials, cwp_server_options* chosen);
 int cwp_server_get_options(const cwp_server* s, cwp_server_options* o);
+/* uploads the scenes of the unfinished jobs, in arrival order, while they fit the scene budget (so that a run's
+ * timing excludes the first uploads) */
+int cwp_server_preload(cwp_server* s);
 
 /* ---- device memory for callers without a GPU array library ---- */
 enum { CWP_COPY_H2D = 1, CWP_COPY_D2H = 2, CWP_COPY_D2D = 3 };
