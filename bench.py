@@ -141,14 +141,14 @@ def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000):
     return out
 
 
-def job_stream(n_jobs, n_scenes, seed):
+def job_stream(n_jobs, n_scenes, seed, res=JOB_RES, spp=JOB_SPP):
     """(scene, width, height, spp, bounces, seed) of each job: scene uniform, square resolution uniform over
     {128, 256, 512, 1024}, spp uniform over {4, 16, 64}, depth 6."""
     rng = np.random.default_rng(seed + 1)
     jobs = []
     for j in range(n_jobs):
-        r = int(rng.choice(JOB_RES))
-        jobs.append((int(rng.integers(n_scenes)), r, r, int(rng.choice(JOB_SPP)), 6, 1000 + j))
+        r = int(rng.choice(res))
+        jobs.append((int(rng.integers(n_scenes)), r, r, int(rng.choice(spp)), 6, 1000 + j))
     return jobs
 
 
@@ -189,7 +189,7 @@ def bench_jobs(args):
     tris = [s.info["num_tris"] for s in setups]
     print("bench: %d scenes (%s tris) set up in %.1f s" % (len(setups), ",".join(str(t) for t in tris),
                                                          time.perf_counter() - t0), file=sys.stderr, flush=True)
-    jobs = job_stream(args.jobs, len(setups), args.job_seed)
+    jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
     srv = cwp.Server(variant=8, bounces=6)
     for s in setups:
         srv.add_scene(s)
@@ -220,8 +220,8 @@ def bench_jobs(args):
             "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"]},
             "trials": [{"bvh": t["variant"], "stack": t["stack_lds"], "rpl": t["rays_per_lane"],
                         "grays": round(t["rays_per_s"] / 1e9, 4)} for t in trials],
-            "workload": {"jobs": len(jobs), "scenes": len(setups), "tris": tris, "resolutions": list(JOB_RES),
-                         "spp": list(JOB_SPP), "depth": 6, "arrivals": "all at t=0", "seed": args.job_seed}}
+            "workload": {"jobs": len(jobs), "scenes": len(setups), "tris": tris, "resolutions": list(args.job_res),
+                         "spp": list(args.job_spp), "depth": 6, "arrivals": "all at t=0", "seed": args.job_seed}}
 
 
 def bench_serve(args):
@@ -229,7 +229,7 @@ def bench_serve(args):
     (batched) against one job at a time (plain); rays/s and p50/p99 latency (finish - arrival) per rate."""
     import cwp
     setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed))
-    jobs = job_stream(args.jobs, len(setups), args.job_seed)
+    jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
     srv = cwp.Server(variant=8, bounces=6)
     for s in setups:
         srv.add_scene(s)
@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--job-seed", type=int, default=1)
     ap.add_argument("--autotune", type=int, default=1, help="job stream: run the autotuner before the batched run")
     ap.add_argument("--rates", default="10,50,100,200,500", help="serve: Poisson arrival rates (jobs/s)")
+    ap.add_argument("--job-res", default="128,256,512,1024", help="job stream: square resolutions (uniform)")
+    ap.add_argument("--job-spp", default="4,16,64", help="job stream: samples per pixel (uniform)")
     ap.add_argument("--no-jobs", action="store_true", help="terrain workload without the job stream")
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -296,6 +298,8 @@ def main():
         sys.exit(2)
 
     args.rates = [float(x) for x in args.rates.split(",") if x]
+    args.job_res = tuple(int(x) for x in args.job_res.split(",") if x)
+    args.job_spp = tuple(int(x) for x in args.job_spp.split(",") if x)
     import cwp
     if args.workload == "instances":
         bench_instances(args)
