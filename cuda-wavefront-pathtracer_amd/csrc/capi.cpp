@@ -77,6 +77,11 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     info->width = Z.A.w; info->height = Z.A.h; info->spp = Z.A.spp; info->batch = Z.A.batch; info->bounces = Z.A.bounces;
     info->num_meshes = Z.inst ? (int)Z.meshes.size() : 0; info->num_instances = Z.inst ? (int)Z.SD.instances.size() : 0;
     info->frames = Z.frames; info->frame_start = Z.A.frameStart;
+    const SceneUpload& S = Z.S;
+    info->full = S.full; info->num_emitters = (int)S.emTri.size();
+    info->env_w = S.env ? S.env->w : 0; info->env_h = S.env ? S.env->h : 0; info->sun_on = S.sunOn;
+    info->light_mode = S.lightMode; info->rr = S.rr; info->rr_depth = S.rrDepth; info->gd_energy = S.gdEnergy;
+    info->env_uniform = S.envUniform; info->env_rot_deg = S.envRotDeg;
     return CWP_OK;
 }
 
@@ -104,6 +109,13 @@ int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* co
         case CWP_ARR_PRIM: *data = Z.tree.prim.data(); *count = Z.tree.prim.size(); break;
         case CWP_ARR_MESH_ROOTS: *data = Z.S.meshRoot.data(); *count = Z.S.meshRoot.size(); break;
         case CWP_ARR_MESH_BOXES: *data = Z.S.meshBox.data(); *count = Z.S.meshBox.size() / 2; break;
+        case CWP_ARR_MATERIALS: *data = Z.S.mats.data(); *count = Z.S.mats.size(); break;
+        case CWP_ARR_EM_TRI: *data = Z.S.emTri.data(); *count = Z.S.emTri.size(); break;
+        case CWP_ARR_EM_CDF: *data = Z.S.emCdf.data(); *count = Z.S.emCdf.size(); break;
+        case CWP_ARR_TRI_LIGHT_PDF: *data = Z.S.triLightPdf.data(); *count = Z.S.triLightPdf.size(); break;
+        case CWP_ARR_ENV_TEX: *data = Z.S.env ? (const void*)Z.S.env->texels.data() : nullptr; *count = Z.S.env ? Z.S.env->texels.size() : 0; break;
+        case CWP_ARR_ENV_MARG: *data = Z.S.env ? (const void*)Z.S.env->marg.data() : nullptr; *count = Z.S.env ? Z.S.env->marg.size() : 0; break;
+        case CWP_ARR_ENV_COND: *data = Z.S.env ? (const void*)Z.S.env->cond.data() : nullptr; *count = Z.S.env ? Z.S.env->cond.size() : 0; break;
         default: return fail(CWP_ERR_ARG, "cwp_setup_array: unknown array");
     }
     return CWP_OK;

@@ -64,6 +64,14 @@ CWP_HD float tableLookup(const float* tab, int n, float u) {
     return tab[k] * (1.f - t) + tab[k + 1] * t;
 }
 
+// uniform direction on the unit sphere
+CWP_HD Vec3 uniformSphere(float u0, float u1) {
+    const float z = 1.f - 2.f * u0;
+    const float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+    const float ph = kTwoPi * u1;
+    return v3(r * cosf(ph), r * sinf(ph), z);
+}
+
 // uniform point on the unit disk (Shirley-Chiu concentric map)
 CWP_HD void concentricDisk(float u0, float u1, float& x, float& y) {
     float a = 2.f * u0 - 1.f, b = 2.f * u1 - 1.f;

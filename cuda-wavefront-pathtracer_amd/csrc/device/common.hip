@@ -6,7 +6,8 @@ namespace cwp {
 namespace dev {
 
 // Shading of the depth-`depth` queue from the hits of k_trace_paths (no traversal: every lane of a wave shades).
-__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth, int smpBase, int K) {
+    const FrameParams& F = *Fp;
     const int lane = laneId();
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;

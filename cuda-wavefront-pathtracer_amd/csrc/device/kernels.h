@@ -22,6 +22,7 @@
 // uniform cone sample with contribution T*albedo*sunE*cos/pi, cosine-weighted continuation rejected below the
 // geometric surface, throughput *= albedo.
 #pragma once
+#include "core/bsdf.h"
 #include "core/sampling.h"
 #include "device/renderer.h"
 #include "device/traverse.h"
@@ -41,6 +42,47 @@ constexpr int kLine = 32;
 constexpr int kCntStride = kParts * kLine;
 constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
 
+// Lights and materials of the full integrator (device pointers; null / 0 for the legacy integrator)
+struct DevLights {
+    const Material* mats;        // per material id
+    const float* triLightPdf;    // per triangle (leaf position): light-selection probability / area, 0 if not emissive
+    const int* emTri;            // emissive triangles (leaf positions), in the order of their power CDF
+    const float* emCdf;          // nEm + 1 entries
+    int nEm;
+    const Vec4* envTex;          // environment radiance, row 0 = +z
+    const float* envMarg;        // envH + 1
+    const float* envCond;        // envH rows of envW + 1
+    int envW, envH;
+    float envRot;                // radians about +z
+    float pSun, pTri, pEnv;      // light-type selection probabilities of next-event estimation
+    int lightMode, rr, rrDepth, gdEnergy, envUniform, hasEnv;
+};
+
+// Fills L from the full-integrator data of S, uploading each array with up(pointer, vector) (host side). The sun is
+// always light-sampled; emitters and the environment are light-sampled unless the light mode is BSDF-only; the light
+// types that are present share the selection probability equally (reference: initFull,
+// src/render/wavefront.cu:530-533).
+template <class Up>
+bool makeDevLights(const SceneUpload& S, Up&& up, DevLights& L, std::string& err) {
+    L = DevLights{};
+    if (!S.full) return true;
+    if (!up(L.mats, S.mats, err) || !up(L.triLightPdf, S.triLightPdf, err) || !up(L.emTri, S.emTri, err) ||
+        !up(L.emCdf, S.emCdf, err))
+        return false;
+    L.nEm = (int)S.emTri.size();
+    L.hasEnv = S.env && S.env->w > 0 ? 1 : 0;
+    if (L.hasEnv) {
+        if (!up(L.envTex, S.env->texels, err) || !up(L.envMarg, S.env->marg, err) || !up(L.envCond, S.env->cond, err)) return false;
+        L.envW = S.env->w; L.envH = S.env->h;
+        L.envRot = S.envRotDeg * 3.14159265f / 180.f;
+    }
+    const bool sun = S.sunOn != 0, tri = L.nEm > 0 && S.lightMode != kLightBSDF, env = L.hasEnv && S.lightMode != kLightBSDF;
+    const int nt = (sun ? 1 : 0) + (tri ? 1 : 0) + (env ? 1 : 0);
+    L.pSun = sun ? 1.f / (float)nt : 0.f; L.pTri = tri ? 1.f / (float)nt : 0.f; L.pEnv = env ? 1.f / (float)nt : 0.f;
+    L.lightMode = S.lightMode; L.rr = S.rr; L.rrDepth = S.rrDepth; L.gdEnergy = S.gdEnergy; L.envUniform = S.envUniform;
+    return true;
+}
+
 struct DevScene {
     const Vec4* nodes;    // BVH layout (BVH8: 128-byte node stride)
     const Vec4* tri;      // traversal triangle data (Woop rows or corners)
@@ -52,6 +94,7 @@ struct DevScene {
     const Vec4* tlas;
     int tlasRoot;
     const InstanceRecord* inst;
+    DevLights Lt;         // full integrator
 };
 
 // Path state. Path id i = pixel * K + sample: the 64 lanes of a wave start as 64 samples of one pixel, and each
@@ -223,14 +266,17 @@ __device__ __forceinline__ void cameraRay(const FrameParams& F, int pix, uint32_
 // Legacy shading of one closest hit of path i (reference: shadeOne, src/render/wavefront.cu:94-136). Writes the
 // path's radiance on a miss (depth 0 also initialises it on a hit) and returns the continuation (cont: po, nd, Tn) and
 // the shadow ray (shadow: po, sdir, scon) for the caller's wave-level appends.
-struct ShadeOut { bool cont, shadow, miss; Vec3 po, nd, Tn, sdir, scon; };
+// Full-integrator shading also returns the shadow ray's own origin (so) and extent (stmax) and the continuation's
+// BSDF pdf (stored in D.w for the MIS weight of its hit).
+struct ShadeOut { bool cont, shadow, miss; Vec3 po, nd, Tn, sdir, scon, so; float stmax, pdf; };
 
 template <bool INST = false>
 __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
                                                 int i, int pix, uint32_t smp, Vec3 o, Vec3 d, Vec3 T, const Hit& h) {
     ShadeOut out;
     out.cont = false; out.shadow = false; out.miss = false;
-    out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po;
+    out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po; out.so = out.po;
+    out.stmax = 1e30f; out.pdf = 0.f;
     if (h.tri < 0) {
         out.miss = true;
         Vec3 c = T * F.sky;
@@ -257,6 +303,7 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
     if (dot(ng, d) > 0.f) { ng = -ng; ns = -ns; }
     const Vec3 alb = xyz(S.albedo[asInt(b.w)]);
     out.po = p + ng * F.eps;
+    out.so = out.po;
     const uint32_t dd = (uint32_t)depth + 1u;
     const Vec3 l = uniformCone(F.sunTo, F.sunCos, rnd((uint32_t)pix, smp, dd, 2, F.seed), rnd((uint32_t)pix, smp, dd, 3, F.seed));
     const float cl = dot(ns, l);
@@ -266,6 +313,8 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
     return out;
 }
 
+#include "device/shade_full.h"
+
 // Wave-level appends of the shading results: continuations to partition `part` of the next queue, shadow rays to
 // partition `part` of the shadow queue. Returns the wave's number of misses.
 __device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int* qOut, int i,
@@ -273,11 +322,11 @@ __device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int pa
     const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
     if (mc) {
         const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
-        if (r.cont) { qOut[pos] = i; P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, 0.f); P.T[i] = v4(r.Tn, 0.f); }
+        if (r.cont) { qOut[pos] = i; P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, r.pdf); P.T[i] = v4(r.Tn, 0.f); }
     }
     if (ms) {
         const int pos = outStart + waveAppend(ms, ctr(cnt, part, kCShadow));
-        if (r.shadow) { P.sO[pos] = v4(r.po, asFloat(i)); P.sD[pos] = v4(r.sdir, 1e30f); P.sC[pos] = v4(r.scon, 0.f); }
+        if (r.shadow) { P.sO[pos] = v4(r.so, asFloat(i)); P.sD[pos] = v4(r.sdir, r.stmax); P.sC[pos] = v4(r.scon, 0.f); }
     }
     return (int)__builtin_popcountll(mm);
 }
@@ -303,8 +352,10 @@ constexpr int kRefill = 16;
 // the shadow queue, so a path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue
 // starts where the depth-d partitions before x end, which bounds it by the rays partition x traced.
 // WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
-template <int V, int TRI, int SL, int WAVES, bool INST>
-__global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+template <int V, int TRI, int SL, int WAVES, bool INST, bool FULL = false>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth,
+                                                          int smpBase, int K) {
+    const FrameParams& F = *Fp;
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int lane = laneId();
@@ -331,7 +382,12 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
             // throughput is fetched after the traversal so that no registers hold it across the traversal loop
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
-            res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+            if constexpr (FULL) {
+                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                res = shadeFull(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
+            } else {
+                res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+            }
         }
         misses += appendResults(P, cnt, part, outStart, qOut, i, res);
     }
@@ -343,7 +399,8 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
 // REFILL lanes of its wave are idle, then they all take the next entries of the wave's share of its partition, so
 // long rays do not leave the rest of the wave empty. Hits go to H[path id] for k_shade.
 template <int V, int TRI, int SL, int REFILL>
-__global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K) {
+__global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth, int smpBase, int K) {
+    const FrameParams& F = *Fp;
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -393,6 +450,8 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
 // Two-level scenes keep the TLAS ray and the object-space ray live at once: a budget of 4 waves per SIMD (~128
 // registers) avoids spilling them.
 constexpr int kInstWaves = 4;
+// The full integrator's shading (material blend, light choice, BSDF sample) needs about twice the legacy registers.
+constexpr int kFullWaves = 4;
 
 template <int V, int TRI, int SL, bool INST>
 __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevScene S, DevPaths P, int depth) {
@@ -531,7 +590,7 @@ __device__ __forceinline__ int jobOf(const int* base, int n, int i) {
 
 // k_bounce over the paths of many jobs at once (one launch per bounce for the whole batch): each path finds its job,
 // and with it the scene it traverses and the parameters it is shaded with. A wave may hold paths of several jobs.
-template <int V, int TRI, int SL, int WAVES>
+template <int V, int TRI, int SL, int WAVES, bool FULL = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
@@ -568,7 +627,12 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
-            res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
+            if constexpr (FULL) {
+                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                res = shadeFull(J.S, P, J.F, depth, i, pix, smp, o, d, T, lastPdf, h);
+            } else {
+                res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
+            }
         }
         misses += appendResults(P, cnt, part, outStart, qOut, i, res);
     }
@@ -612,7 +676,10 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs(const DevJobTable* __
 }
 
 // ------------------------------------------------------------------------------------------------------- dispatch
-using BounceFn = void (*)(DevScene, DevPaths, FrameParams, int, int, int);
+// The frame parameters (camera, lights, filter table: ~400 bytes) are read through a pointer: as a by-value argument
+// they occupied scalar registers for the whole kernel and made the trace+shade kernel spill (68 -> 24 bytes of
+// scratch per lane, +3.6 % frame throughput on the 720p terrain frame).
+using BounceFn = void (*)(DevScene, DevPaths, const FrameParams*, int, int, int);
 using ShadowFn = void (*)(DevScene, DevPaths, int);
 using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, int*, void*, int);
 using JobsFn = void (*)(const DevJobTable*, DevPaths, int);
@@ -629,6 +696,9 @@ struct Kernels {
     TraceFn traceCInst = nullptr, traceAInst = nullptr;
     // batched jobs
     JobsFn bounceJobs = nullptr, shadowJobs = nullptr;
+    // full material integrator (fused bounce kernel; shadow rays use the legacy shadow kernels)
+    BounceFn bounceFull = nullptr;
+    JobsFn bounceJobsFull = nullptr;
 };
 
 template <int V, int TRI, int SL>
@@ -646,6 +716,8 @@ void pickSL(int waves, Kernels& k) {
     k.traceAInst = k_trace<V, TRI, true, SL, true>;
     k.bounceJobs = k_bounce_jobs<V, TRI, SL, 8>;
     k.shadowJobs = k_shadow_jobs<V, TRI, SL>;
+    k.bounceFull = k_bounce<V, TRI, SL, kFullWaves, false, true>;
+    k.bounceJobsFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true>;
 }
 template <int V, int TRI>
 void pickTri(int sl, int waves, Kernels& k) {
@@ -663,7 +735,7 @@ void pickKernels(int tri, int sl, int waves, Kernels& k) {
 }
 
 // layout-independent kernels (device/common.hip)
-__global__ void k_shade(DevScene S, DevPaths P, FrameParams F, int depth, int smpBase, int K);
+__global__ void k_shade(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth, int smpBase, int K);
 __global__ void k_pass_begin(int* cnt, int words, long long NK);
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);

@@ -1,4 +1,5 @@
-// device/renderer.h - host interface of the gfx950 wavefront renderer (legacy Lambert + sun + sky integrator).
+// device/renderer.h - host interface of the gfx950 wavefront renderer: the legacy integrator (Lambert + sun + sky) and
+// the full material integrator (OpenPBR-subset materials, emissive triangles, environment map, MIS).
 //
 // Plain C++ types only (no HIP types), so the g++-compiled host code and the C ABI can include it. Mirrors the
 // reference's WavefrontRenderer (src/render/renderer.h:47-93): init uploads the scene and allocates path state for
@@ -6,12 +7,17 @@
 // accumulation buffer, readAccum() returns per-pixel radiance sums. Errors are reported through the return value and
 // an error string, never by exiting.
 #pragma once
+#include "core/material.h"
 #include "device/traverse_types.h"
 #include "host/camera.h"
+#include "host/envmap.h"
+#include <memory>
 #include <string>
 #include <vector>
 
 namespace cwp {
+
+enum LightMode { kLightMIS = 0, kLightNEE = 1, kLightBSDF = 2 };
 
 struct RenderOptions {
     int variant = 8;        // BVH layout: 2 or 8 (compressed)
@@ -51,6 +57,19 @@ struct SceneUpload {
     int inst = 0;
     std::vector<int> meshRoot;    // BLAS root node per mesh
     std::vector<Vec4> meshBox;    // object-space box per mesh: lo, hi
+    // full material integrator (full = 0: the legacy integrator, which uses `albedo` only)
+    int full = 0;
+    std::vector<Material> mats;   // per material id
+    std::vector<float> triLightPdf;   // per leaf position: light-selection probability / area (empty: no emitters)
+    std::vector<int> emTri;       // leaf positions of the emissive triangles, in power-CDF order
+    std::vector<float> emCdf;     // emTri.size() + 1 entries
+    std::shared_ptr<const EnvMap> env;   // null: no environment map (the sky constant is used)
+    float envRotDeg = 0.f;
+    int sunOn = 1;                // the sun is light-sampled
+    int lightMode = kLightMIS;    // emitters / environment: MIS of light and BSDF sampling, light only, BSDF only
+    int rr = 0, rrDepth = 3;      // Russian roulette from this depth on
+    int gdEnergy = 1;             // glossy-diffuse energy compensation
+    int envUniform = 0;           // environment light samples uniform on the sphere (instead of by luminance)
 };
 
 class Renderer {

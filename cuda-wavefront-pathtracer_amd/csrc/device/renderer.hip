@@ -23,6 +23,7 @@
 #include "device/kernels.h"
 #include "host/scene_check.h"
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 
 namespace cwp {
@@ -76,6 +77,7 @@ __global__ void k_tlas_depth(const int* __restrict__ parent, int n, int* depth) 
 struct Renderer::Impl {
     RenderOptions O;
     FrameParams F;
+    FrameParams* dF = nullptr;    // device copy read by the bounce / shade kernels
     int N = 0, K = 1, NK = 0, depths = 0, sl = 16;
     hipStream_t st = nullptr;
     hipEvent_t f0 = nullptr, f1 = nullptr;
@@ -209,6 +211,14 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.S.nodes = dn; I.S.verts = dv; I.S.nrm = dnr; I.S.albedo = da; I.S.root = S.root;
     I.S.tri = O.tri == kTriWoop ? dw : dv;
     if (F.hasN && !dnr) { err = "hasN set without normals"; return false; }
+    auto up = [&](auto& p, const auto& v, std::string& e) -> bool {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        T* q = nullptr;
+        if (!I.upload(q, v, e)) return false;
+        p = q;
+        return true;
+    };
+    if (!makeDevLights(S, up, I.S.Lt, err)) return false;
 
     if (O.fuse != 0 && O.fuse != 1) { err = "fuse must be 0 or 1"; return false; }
     if (O.persistent && O.fuse) { err = "persistent traversal needs the split kernels (fuse 0)"; return false; }
@@ -216,7 +226,13 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     else if (S.variant == 4) pickKernels<4>(O.tri, I.sl, O.bounceWaves, I.kn);
     else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
     I.inst = S.inst != 0;
-    if (I.inst) {
+    if (S.full && (I.inst || !O.fuse || O.persistent)) {
+        err = "the full integrator runs single-level scenes with the fused kernel (fuse 1, persistent 0)"; return false;
+    }
+    if (S.full) {
+        I.bounce = I.kn.bounceFull; I.shade = nullptr;
+        I.shadow = O.shadowRefill ? I.kn.shadowRefill : I.kn.shadow;
+    } else if (I.inst) {
         // two-level scenes run the fused bounce kernel and the one-ray-per-lane shadow kernel
         if (!O.fuse || O.persistent) { err = "two-level scenes need the fused kernel (fuse 1, persistent 0)"; return false; }
         I.bounce = I.kn.bounceInst; I.shade = nullptr; I.shadow = I.kn.shadowInst;
@@ -256,6 +272,8 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.P.spill = sp;
     }
 
+    if (!I.alloc(I.dF, 1, err)) return false;
+    HIPCHK(hipMemcpy(I.dF, &I.F, sizeof(FrameParams), hipMemcpyHostToDevice));
     I.N = F.cam.w * F.cam.h;
     I.K = O.batch < 1 ? 1 : O.batch;
     const size_t NK = (size_t)I.N * I.K;
@@ -280,6 +298,8 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
 bool Renderer::setParams(const FrameParams& F, std::string& err) {
     if (F.cam.w != p_->F.cam.w || F.cam.h != p_->F.cam.h) { err = "setParams cannot change the image size"; return false; }
     if (F.hasN && !p_->S.nrm) { err = "hasN set without normals"; return false; }
+    HIPCHK(hipStreamSynchronize(p_->st));   // queued frames read the previous parameters
+    HIPCHK(hipMemcpy(p_->dF, &F, sizeof(FrameParams), hipMemcpyHostToDevice));
     p_->F = F;
     return true;
 }
@@ -301,10 +321,10 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
         for (int depth = 0; depth < I.depths; depth++) {
             hipEvent_t a = nullptr, b = nullptr;
             if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-            I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.K);
+            I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
             if (I.shade) {
                 if (I.O.timeKernels) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
-                I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.F, depth, s, I.K);
+                I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
             }
             if (I.O.timeKernels) {
                 HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });

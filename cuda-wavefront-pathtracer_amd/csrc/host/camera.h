@@ -1,6 +1,7 @@
 // host/camera.h - pinhole / thin-lens camera, lighting, and the per-frame parameter block of the legacy integrator.
 #pragma once
 #include "core/sampling.h"
+#include <string>
 
 namespace cwp {
 
@@ -18,7 +19,10 @@ struct Lighting {
     Vec3 sunTo = v3(0.3f, -0.5f, 0.8f);   // direction toward the sun (normalised by the setup)
     Vec3 sunE = v3(3.f, 3.f, 3.f);        // irradiance on a surface facing the sun
     float sunAngleDeg = 0.526f;           // angular diameter
-    Vec3 sky = v3(0.3f, 0.3f, 0.3f);      // constant sky radiance
+    Vec3 sky = v3(0.3f, 0.3f, 0.3f);      // constant sky radiance (no environment map)
+    std::string envMap;                   // equirectangular Radiance .hdr map (replaces the sky; full integrator)
+    float envStrength = 1.f;              // radiance multiplier of the map
+    float envRotDeg = 0.f;                // rotation about the world z axis
 };
 
 // Parameters of one frame of the legacy integrator, passed by value as a kernel argument (reference: the __constant__

@@ -154,12 +154,63 @@ bool loadLightingFile(const std::string& path, Lighting& L, Mesh* m, std::string
     if (keyNumber(J, "sun_angle_rad", x)) L.sunAngleDeg = (float)(x * 180.0 / M_PI);
     v = keyNumbers(J, "sky_radiance_rgb_linear");
     if (v.size() == 3) L.sky = v3((float)v[0], (float)v[1], (float)v[2]);
+    // environment map keys (reference: src/io/scene_files.cpp:22); a relative path is relative to the file
+    std::string env = J.string("env_map", "");
+    if (!env.empty()) {
+        const size_t slash = path.find_last_of('/');
+        L.envMap = env[0] == '/' || slash == std::string::npos ? env : path.substr(0, slash + 1) + env;
+    }
+    L.envStrength = (float)J.number("env_strength", L.envStrength);
+    L.envRotDeg = (float)J.number("env_rotation_deg", L.envRotDeg);
     if (m) {
         v = keyNumbers(J, "albedo_rgb_linear_by_material_index");
         if (v.size() >= 3) { m->kd.clear(); for (size_t j = 0; j + 2 < v.size(); j += 3) m->kd.push_back(v3((float)v[j], (float)v[j + 1], (float)v[j + 2])); }
         int maxMat = 0; for (int t : m->fm) maxMat = t > maxMat ? t : maxMat;
         while ((int)m->kd.size() <= maxMat) m->kd.push_back(v3(0.8f, 0.8f, 0.8f));
     }
+    return true;
+}
+
+static Vec3 vec3Or(const Json& o, const char* key, Vec3 def) {
+    const std::vector<double> v = o.numbers(key);
+    return v.size() == 3 ? v3((float)v[0], (float)v[1], (float)v[2]) : def;
+}
+
+bool loadMaterialsFile(const std::string& path, Mesh& m, std::string& err) {
+    const std::string text = readTextFile(path);
+    if (text.empty()) { err = "materials file " + path + " unreadable"; return false; }
+    Json J; std::string perr;
+    if (!parseJson(text, J, &perr)) { err = "materials file " + path + ": " + perr; return false; }
+    const Json* arr = J.get("materials");
+    if (!arr || arr->type != Json::Array) { err = "materials file " + path + ": no \"materials\" array"; return false; }
+    std::vector<Material> mats;
+    std::vector<bool> given;
+    int next = 0;
+    for (const Json& o : arr->arr) {
+        Material x = defaultMaterial();
+        x.baseColor = vec3Or(o, "base_color", x.baseColor);
+        x.metallic = (float)o.number("metallic", x.metallic);
+        x.roughness = (float)o.number("roughness", x.roughness);
+        x.ior = (float)o.number("ior", x.ior);
+        x.specularWeight = (float)o.number("specular_weight", x.specularWeight);
+        x.specularColor = vec3Or(o, "specular_color", x.specularColor);
+        x.transmission = (float)o.number("transmission", x.transmission);
+        x.emission = vec3Or(o, "emission_color", v3(1.f, 1.f, 1.f)) * (float)o.number("emission_strength", 0.0);
+        if (!(x.ior > 0.f) || x.metallic < 0.f || x.metallic > 1.f || x.transmission < 0.f || x.transmission > 1.f ||
+            x.roughness < 0.f || x.roughness > 1.f) {
+            err = "materials file " + path + ": entry " + std::to_string(next) + " out of range (ior > 0; metallic, transmission, roughness in [0, 1])";
+            return false;
+        }
+        int id = next++;
+        const std::string name = o.string("name", "");
+        for (size_t j = 0; !name.empty() && j < m.matName.size(); j++)
+            if (m.matName[j] == name) { id = (int)j; break; }
+        if ((int)mats.size() <= id) { mats.resize(id + 1, lambertMaterial(v3(0.8f, 0.8f, 0.8f))); given.resize(id + 1, false); }
+        mats[id] = x; given[id] = true;
+    }
+    for (size_t j = 0; j < mats.size(); j++)
+        if (!given[j]) mats[j] = lambertMaterial(j < m.kd.size() ? m.kd[j] : v3(0.8f, 0.8f, 0.8f));
+    m.mats = mats;
     return true;
 }
 

@@ -30,6 +30,11 @@ std::vector<double> keyNumbers(const Json& doc, const std::string& key);
 // Lighting file: sun_dir_to_light_world_zup, sun_irradiance_W_m2, sun_angle_rad, sky_radiance_rgb_linear and
 // albedo_rgb_linear_by_material_index (replaces the mesh albedo table, then padded to every used id).
 bool loadLightingFile(const std::string& path, Lighting& L, Mesh* m, std::string& err);
+// Materials file {"materials": [{name?, base_color, metallic, roughness, ior, specular_weight, specular_color,
+// transmission, emission_color, emission_strength}, ...]}: entries start from OpenPBR's defaults; an entry whose name
+// matches a mesh material name replaces that id, the others take ids in file order; ids the file does not give stay
+// Lambertian with the mesh's kd (reference: loadMaterialsFile, src/io/scene_files.cpp:28-55).
+bool loadMaterialsFile(const std::string& path, Mesh& m, std::string& err);
 // Camera file: fx/fy used only when still 0 and width/height match; else hfov_deg when still 0; c2w from key[frame]
 // (the key holds one matrix or a list of per-frame matrices); optional aperture_radius / focus_distance.
 bool loadCameraFile(const std::string& path, const std::string& key, int frame, int w, int h, float& fx, float& fy,

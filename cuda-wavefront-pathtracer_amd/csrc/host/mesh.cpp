@@ -162,4 +162,43 @@ void makeIcosphere(Mesh& m, int level, int mat) {
     for (const Tri& q : f) { m.f.push_back({ q.a + base, q.b + base, q.c + base }); m.fm.push_back(mat); }
 }
 
+void assignGeneratedMaterials(Mesh& m, int first, int n, uint64_t seed) {
+    if (m.kd.empty()) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+    std::vector<Material> mats(1 + (size_t)n);
+    Material g = defaultMaterial();   // glossy-diffuse ground / terrain
+    g.baseColor = m.kd[0]; g.roughness = 0.5f;
+    mats[0] = g;
+    const Vec3 metals[4][2] = { { v3(1.00f, 0.78f, 0.34f), v3(0.95f, 0.86f, 0.55f) },    // gold, F82 tint
+                                { v3(0.95f, 0.64f, 0.54f), v3(0.93f, 0.75f, 0.70f) },    // copper
+                                { v3(0.91f, 0.92f, 0.92f), v3(0.96f, 0.97f, 0.98f) },    // aluminium
+                                { v3(0.97f, 0.96f, 0.91f), v3(1.00f, 1.00f, 1.00f) } };  // silver
+    SplitMix r(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+    for (int b = 0; b < n; b++) {
+        Material x = defaultMaterial();
+        const float kind = r.next01(), rough = r.uniform(0.05f, 0.6f);
+        if (kind < 0.30f) {
+            const int k = (int)(r.next01() * 4.f) & 3;
+            x.metallic = 1.f; x.baseColor = metals[k][0]; x.specularColor = metals[k][1]; x.roughness = rough;
+        } else if (kind < 0.55f) {
+            x.transmission = 1.f; x.ior = 1.5f; x.roughness = rough;
+            x.baseColor = v3(r.uniform(0.8f, 1.f), r.uniform(0.8f, 1.f), r.uniform(0.8f, 1.f));
+        } else if (kind < 0.85f) {
+            x.baseColor = v3(r.uniform(0.05f, 0.9f), r.uniform(0.05f, 0.9f), r.uniform(0.05f, 0.9f)); x.roughness = rough;
+        } else {
+            x = lambertMaterial(v3(0.5f, 0.5f, 0.5f));
+            x.emission = v3(r.uniform(0.6f, 1.f), r.uniform(0.6f, 1.f), r.uniform(0.6f, 1.f)) * 8.f;
+        }
+        mats[1 + b] = x;
+    }
+    if (m.fm.empty()) m.fm.assign(m.f.size(), 0);
+    for (int b = 0; b < n; b++)
+        for (int t = 0; t < 12; t++) {
+            const size_t tri = (size_t)first + 12 * (size_t)b + (size_t)t;
+            if (tri < m.fm.size()) m.fm[tri] = 1 + b;
+        }
+    m.kd.resize(mats.size(), v3(0.8f, 0.8f, 0.8f));
+    for (size_t j = 0; j < mats.size(); j++) m.kd[j] = mats[j].baseColor;
+    m.mats = mats;
+}
+
 }  // namespace cwp

@@ -1,5 +1,6 @@
 // host/mesh.h - indexed triangle meshes, boxes, and the procedural scenes used by the CLI, the tests and the bench.
 #pragma once
+#include "core/material.h"
 #include "core/vecmath.h"
 #include <cfloat>
 #include <string>
@@ -18,6 +19,7 @@ struct Mesh {
     std::vector<int> fm;       // per-triangle material id (empty: all 0)
     std::vector<Vec3> kd;      // albedo per material id
     std::vector<std::string> matName;
+    std::vector<Material> mats; // full materials per id (from a materials file or a generator); empty: Lambert kd
     bool hasN() const { return !n.empty() && n.size() == v.size(); }
     int mat(size_t t) const { return fm.empty() ? 0 : fm[t]; }
 };
@@ -71,6 +73,13 @@ void terrainCamera(float c2w[16]);
 // nu x nv quads (reference: src/scene/synthetic.cpp:6-50; its default view: src/scene/synthetic.cpp:50).
 void makeSphereGrid(Mesh& m, int gridN, int nu, int nv, int gs);
 void sphereGridCamera(float c2w[16]);
+
+// OpenPBR-subset materials for a generated scene's boxes (triangles [first, first + 12 n)): material 0 (terrain or
+// ground) becomes glossy-diffuse; each box gets its own material drawn from the seed: rough conductor (F0 of gold,
+// copper, aluminium or silver, F82 tint, roughness U[0.05, 0.6]) 30 %, rough dielectric (IOR 1.5, roughness
+// U[0.05, 0.6]) 25 %, glossy-diffuse (random colour, roughness U[0.05, 0.6]) 30 %, emitter (Lambert 0.5, emission
+// strength 8) 15 %.
+void assignGeneratedMaterials(Mesh& m, int first, int n, uint64_t seed);
 
 // Subdivided icosahedron of radius 1 at the origin (20 * 4^level triangles).
 void makeIcosphere(Mesh& m, int level, int mat);

@@ -125,6 +125,34 @@ bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string
 
 }  // namespace
 
+// full-integrator data: a material per id, emitter and environment tables of consistent sizes and ranges
+static bool checkLights(const SceneUpload& S, size_t nt, std::string& err) {
+    if (S.mats.size() < S.albedo.size()) return fail(err, "full integrator: fewer materials than material ids");
+    if (S.lightMode < kLightMIS || S.lightMode > kLightBSDF) return fail(err, "full integrator: unknown light mode");
+    if (S.rr && S.rrDepth < 0) return fail(err, "full integrator: negative roulette depth");
+    for (const Material& m : S.mats)
+        if (!(m.ior > 0.f) || !(m.roughness >= 0.f) || !(m.metallic >= 0.f && m.metallic <= 1.f) ||
+            !(m.transmission >= 0.f && m.transmission <= 1.f))
+            return fail(err, "full integrator: material parameter out of range");
+    const size_t ne = S.emTri.size();
+    if (ne) {
+        if (S.triLightPdf.size() != nt) return fail(err, "full integrator: triLightPdf needs one entry per triangle");
+        if (S.emCdf.size() != ne + 1 || S.emCdf[0] != 0.f || S.emCdf[ne] != 1.f)
+            return fail(err, "full integrator: emitter CDF must have emitters + 1 entries from 0 to 1");
+        for (size_t e = 0; e < ne; e++) {
+            if (S.emTri[e] < 0 || (size_t)S.emTri[e] >= nt) return fail(err, "full integrator: emitter index out of range");
+            if (S.emCdf[e + 1] < S.emCdf[e]) return fail(err, "full integrator: emitter CDF decreases");
+        }
+    }
+    if (S.env && S.env->w > 0) {
+        const EnvMap& E = *S.env;
+        const size_t w = (size_t)E.w, h = (size_t)E.h;
+        if (E.h < 1 || E.texels.size() != w * h || E.marg.size() != h + 1 || E.cond.size() != h * (w + 1))
+            return fail(err, "full integrator: environment map tables have the wrong sizes");
+    }
+    return true;
+}
+
 bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
     const size_t per = nodeFloat4s(S.variant);
     if (!per) return fail(err, "variant must be 2, 4 or 8");
@@ -145,6 +173,7 @@ bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
         if (m < 0 || (size_t)m >= S.albedo.size())
             return fail(err, "triangle " + std::to_string(t) + ": material id out of range");
     }
+    if (S.full && !checkLights(S, nt, err)) return false;
     int depth = 0;
     Links G;
     const bool ok = S.variant == 2 ? checkBVH2(S, nn, nt, G, err)

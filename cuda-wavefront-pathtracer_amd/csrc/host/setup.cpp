@@ -4,6 +4,7 @@
 #include "host/scene_check.h"
 #include "host/instances.h"
 #include "host/io.h"
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -81,6 +82,17 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--sun_rgb") ok = vec(A.light.sunE);
         else if (k == "--sky") ok = vec(A.light.sky);
         else if (k == "--albedo") ok = vec(A.albedo);
+        else if (k == "--materials") ok = str(A.materials);
+        else if (k == "--env") ok = str(A.env);
+        else if (k == "--env_strength") ok = num(A.envStrength);
+        else if (k == "--env_rot") ok = num(A.envRot);
+        else if (k == "--light_sampling") ok = str(A.lightSampling);
+        else if (k == "--env_sampling") ok = str(A.envSampling);
+        else if (k == "--rr") ok = inum(A.rr);
+        else if (k == "--rr_depth") ok = inum(A.rrDepth);
+        else if (k == "--gd_energy") ok = inum(A.gdEnergy);
+        else if (k == "--gen_env") ok = str(A.genEnv);
+        else if (k == "--gen_materials") ok = inum(A.genMaterials);
         else if (k == "--devloop" || k == "--sort") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
@@ -89,7 +101,10 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     }
     if (A.variant != "bvh2" && A.variant != "bvh4" && A.variant != "bvh8") { err = "--variant must be bvh2, bvh4 or bvh8"; return false; }
     if (A.tri != "mt" && A.tri != "woop" && A.tri != "watertight") { err = "--tri must be mt, woop or watertight"; return false; }
-    if (A.integrator != "auto" && A.integrator != "legacy") { err = "--integrator must be auto or legacy"; return false; }
+    if (A.integrator != "auto" && A.integrator != "legacy" && A.integrator != "full") { err = "--integrator must be auto, legacy or full"; return false; }
+    if (A.lightSampling != "mis" && A.lightSampling != "nee" && A.lightSampling != "bsdf") { err = "--light_sampling must be mis, nee or bsdf"; return false; }
+    if (A.envSampling != "importance" && A.envSampling != "uniform") { err = "--env_sampling must be importance or uniform"; return false; }
+    if (A.rrDepth < 0) { err = "--rr_depth must be >= 0"; return false; }
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
     if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
@@ -158,6 +173,38 @@ bool frameInstances(const SceneSetup& Z, int f, std::vector<InstanceRecord>& out
     return true;
 }
 
+// Emissive triangles as lights, sampled in proportion to power = luminance(emission) * area: the CDF runs over the
+// triangles in mesh order (independent of the BVH), and triLightPdf[leaf position] = selection probability / area
+// (reference: src/app/scene_setup.cpp:225-247).
+static void emitterTables(const Mesh& M, const std::vector<Material>& mats, SceneUpload& S) {
+    const std::vector<int>& order = S.order;
+    std::vector<std::pair<int, int>> em;   // (mesh triangle, leaf position)
+    for (size_t kk = 0; kk < order.size(); kk++)
+        if (isEmissive(mats[M.mat(order[kk])])) em.push_back({ order[kk], (int)kk });
+    std::sort(em.begin(), em.end());
+    std::vector<double> pw, area;
+    double tot = 0;
+    S.emTri.clear(); S.emCdf.clear(); S.triLightPdf.clear();
+    for (const auto& e : em) {
+        const Tri& f = M.f[e.first];
+        const Vec3 e1 = M.v[f.b] - M.v[f.a], e2 = M.v[f.c] - M.v[f.a];
+        const double a = 0.5 * (double)length(cross(e1, e2));
+        const double p = (double)luminance(mats[M.mat(e.first)].emission) * a;
+        if (!(p > 0)) continue;
+        S.emTri.push_back(e.second); pw.push_back(p); area.push_back(a); tot += p;
+    }
+    if (S.emTri.empty()) return;
+    S.triLightPdf.assign(order.size(), 0.f);
+    S.emCdf.assign(S.emTri.size() + 1, 0.f);
+    double c = 0;
+    for (size_t e = 0; e < S.emTri.size(); e++) {
+        S.triLightPdf[S.emTri[e]] = (float)(pw[e] / tot / area[e]);
+        c += pw[e];
+        S.emCdf[e + 1] = (float)(c / tot);
+    }
+    S.emCdf.back() = 1.f;
+}
+
 // BVH of the setup's mesh (or one BLAS per mesh of a two-level scene) in the layout Z.A.variant names, the packed
 // triangles and the GPU-ready scene Z.S; uses Z.M / Z.meshes, Z.inst, Z.P.hasN and the builder options of Z.A
 static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
@@ -211,6 +258,13 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
     }
     S.albedo.clear();
     for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
+    if (Z.full) {
+        S.full = 1; S.mats = Z.mats; S.env = Z.env; S.envRotDeg = A.light.envRotDeg;
+        S.sunOn = luminance(A.light.sunE) > 0.f ? 1 : 0;
+        S.lightMode = A.lightSampling == "nee" ? kLightNEE : (A.lightSampling == "bsdf" ? kLightBSDF : kLightMIS);
+        S.rr = A.rr; S.rrDepth = A.rrDepth; S.gdEnergy = A.gdEnergy; S.envUniform = A.envSampling == "uniform" ? 1 : 0;
+        emitterTables(M, S.mats, S);
+    }
     Z.bvhS = msSince(t0) / 1000.0;
     Z.frames = A.frames > 0 ? A.frames : (Z.inst ? Z.SD.frames : 1);
     if (!Z.inst && Z.frames > 1) { err = "--frames > 1 needs a two-level scene (--scene or --gen instances, no --flatten)"; return false; }
@@ -228,6 +282,7 @@ bool relayoutScene(const SceneSetup& Z, int variant, SceneUpload& S, std::string
     T.A.variant = variant == 2 ? "bvh2" : (variant == 4 ? "bvh4" : "bvh8");
     T.M = Z.M;
     T.P.hasN = Z.P.hasN;
+    T.full = Z.full; T.mats = Z.mats; T.env = Z.env;
     if (!buildSceneBVH(T, err)) return false;
     S = std::move(T.S);
     return true;
@@ -290,10 +345,14 @@ bool setupScene(SceneSetup& Z, std::string& err) {
             if (anyN) M.n.assign(M.v.size(), v3(0.f, 0.f, 1.f));
         }
     } else if (A.gen == "boxes") {
-        makeBoxCloud(M, A.genBoxes >= 0 ? A.genBoxes : 1024, A.genSeed, 0.7f);
+        Z.genBoxCount = A.genBoxes >= 0 ? A.genBoxes : 1024;
+        makeBoxCloud(M, Z.genBoxCount, A.genSeed, 0.7f);
+        Z.genBoxFirst = 0;
         boxCloudCamera(c2w); haveC2w = true; Z.sceneName = "gen:boxes";
     } else if (A.gen == "terrain") {
-        makeTerrain(M, A.genGrid, 6, 0.2f, A.genBoxes >= 0 ? A.genBoxes : 200, A.genSeed, 0.8f);
+        Z.genBoxCount = A.genBoxes >= 0 ? A.genBoxes : 200;
+        makeTerrain(M, A.genGrid, 6, 0.2f, Z.genBoxCount, A.genSeed, 0.8f);
+        Z.genBoxFirst = 2 * (A.genGrid - 1) * (A.genGrid - 1);
         terrainCamera(c2w); haveC2w = true; Z.sceneName = "gen:terrain";
     } else if (A.synthetic) {
         makeSphereGrid(M, A.synthetic, 200, 100, 200);
@@ -308,8 +367,46 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     if (!A.lighting.empty() && !loadLightingFile(A.lighting, A.light, &M, err)) return false;
     if (A.albedo.x >= 0) for (auto& kd : M.kd) kd = A.albedo;
     if (M.kd.empty()) M.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+    if (!A.materials.empty() && !loadMaterialsFile(A.materials, M, err)) return false;
+    if (A.genMaterials >= 0) {
+        if (Z.genBoxFirst < 0) { err = "--gen_materials needs a --gen boxes or --gen terrain scene"; return false; }
+        assignGeneratedMaterials(M, Z.genBoxFirst, Z.genBoxCount, (uint64_t)A.genMaterials);
+    }
     { int maxMat = 0; for (int t : M.fm) maxMat = t > maxMat ? t : maxMat;
       while ((int)M.kd.size() <= maxMat) M.kd.push_back(v3(0.8f, 0.8f, 0.8f)); }
+    // the material table: materials-file / generated entries, else Lambertian kd; --albedo overrides base colours
+    Z.mats.clear();
+    for (size_t j = 0; j < std::max(M.kd.size(), M.mats.size()); j++) {
+        Material x = j < M.mats.size() ? M.mats[j] : lambertMaterial(M.kd[j]);
+        if (A.albedo.x >= 0) x.baseColor = A.albedo;
+        Z.mats.push_back(x);
+    }
+    while (M.kd.size() < Z.mats.size()) M.kd.push_back(Z.mats[M.kd.size()].baseColor);
+    if (!A.env.empty()) A.light.envMap = A.env;
+    if (A.envStrength >= 0.f) A.light.envStrength = A.envStrength;
+    if (A.envRot > -1e29f) A.light.envRotDeg = A.envRot;
+    bool needFull = A.rr != 0 || !A.light.envMap.empty() || !A.genEnv.empty();
+    for (const Material& x : Z.mats) needFull = needFull || !isLambert(x) || isEmissive(x);
+    Z.full = A.integrator == "full" || (A.integrator == "auto" && needFull);
+    if (Z.full && Z.inst) { err = "the full integrator renders single-level scenes (use --flatten 1)"; return false; }
+    Z.env.reset();
+    if (Z.full && (!A.light.envMap.empty() || !A.genEnv.empty())) {
+        std::vector<float> rgb;
+        int ew = 0, eh = 0;
+        if (!A.genEnv.empty()) {
+            double g[4] = { 2048, 1024, 2, 1 };
+            int ng = 0;
+            std::string s = A.genEnv; for (auto& ch : s) if (ch == ',') ch = ' ';
+            const char* q = s.c_str(); char* e;
+            while (ng < 4) { const double x = strtod(q, &e); if (e == q) break; g[ng++] = x; q = e; }
+            if (ng < 3 || g[0] < 2 || g[1] < 1 || g[0] > 16384 || g[1] > 16384 || g[2] < 0) { err = "--gen_env needs W,H,SIGMA[,SEED]"; return false; }
+            ew = (int)g[0]; eh = (int)g[1];
+            makeLogNormalEnv(ew, eh, 0.f, (float)g[2], (uint64_t)g[3], rgb);
+        } else if (!loadHDR(A.light.envMap, ew, eh, rgb, err)) return false;
+        std::shared_ptr<EnvMap> E = std::make_shared<EnvMap>();
+        buildEnvMap(rgb, ew, eh, A.light.envStrength, *E);
+        Z.env = E;
+    }
     AABB sb = meshBox(M);
     Z.extent = maxc(sb.extent());
     // ---- camera ----
@@ -359,7 +456,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"bvh_build_s\":%.2f,\"w\":%d,\"h\":%d,\"spp\":%d,\"bounces\":%d,\"rays_closest\":%.0f,\"rays_shadow\":%.0f,"
              "\"rays_total\":%.0f,\"t_trace_ms\":%.2f,\"t_frame_ms\":%.2f,\"mrays_s_trace\":%.1f,\"mrays_s_e2e\":%.1f,"
              "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
-             "\"seed\":%u,\"integrator\":\"legacy\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
+             "\"seed\":%u,\"integrator\":\"%s\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":0,"
              "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
@@ -367,7 +464,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
-             Z.P.hasN, A.seed, Z.P.cam.lensR, Z.P.cam.focusD, "importance", A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
+             Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
              Z.treeMs, Z.postMs, Z.sah, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
              A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
              st.unoccluded);

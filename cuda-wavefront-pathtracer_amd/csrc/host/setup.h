@@ -8,6 +8,7 @@
 #include "host/camera.h"
 #include "host/instances.h"
 #include "host/mesh.h"
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,12 @@ struct Args {
     int fuse = -1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
     int persistent = 0, shadowRefill = 0, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
     int bounceWaves = 8;
+    // full material integrator (reference: scene_setup.h:20-22)
+    std::string materials, env, lightSampling = "mis", envSampling = "importance";
+    float envStrength = -1.f, envRot = -1e30f;   // unset: the lighting file's (or 1, 0)
+    int rr = 0, rrDepth = 3, gdEnergy = 1;
+    std::string genEnv;              // generated environment "W,H,SIGMA[,SEED]": log-normal luminance (mu 0)
+    int genMaterials = -1;           // >= 0: seed of the generated OpenPBR materials of a --gen scene's boxes
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);
@@ -46,6 +53,11 @@ struct SceneSetup {
     double bvhS = 0, treeMs = 0, postMs = 0, sah = 0;
     size_t nNodes = 0, nodeBytes = 0;
     std::string sceneName;
+    // full material integrator: the material table and the environment map (shared by every layout of the scene)
+    bool full = false;
+    std::vector<Material> mats;
+    std::shared_ptr<const EnvMap> env;
+    int genBoxFirst = -1, genBoxCount = 0;   // generated scenes: triangle range of the boxes (12 each)
     // two-level scenes: the meshes (one BLAS each, material ids into M.kd) and the instances
     bool inst = false;
     std::vector<Mesh> meshes;

@@ -19,7 +19,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcwp.so")
 
 ARR_VERTICES, ARR_TRIANGLES, ARR_MATERIAL, ARR_ALBEDO, ARR_NODES, ARR_ORDER, ARR_WOOP, ARR_CORNERS, ARR_NORMALS, \
-    ARR_TREE, ARR_TREE_BOX, ARR_PRIM, ARR_MESH_ROOTS, ARR_MESH_BOXES = range(14)
+    ARR_TREE, ARR_TREE_BOX, ARR_PRIM, ARR_MESH_ROOTS, ARR_MESH_BOXES, ARR_MATERIALS, ARR_EM_TRI, ARR_EM_CDF, \
+    ARR_TRI_LIGHT_PDF, ARR_ENV_TEX, ARR_ENV_MARG, ARR_ENV_COND = range(21)
 TRI_MT, TRI_WOOP, TRI_WATERTIGHT = 0, 1, 2
 CWP_COPY_H2D, CWP_COPY_D2H, CWP_COPY_D2D = 1, 2, 3
 
@@ -57,7 +58,10 @@ class SceneInfo(ctypes.Structure):
                 ("sah", ctypes.c_double), ("bvh_build_s", ctypes.c_double), ("width", ctypes.c_int),
                 ("height", ctypes.c_int), ("spp", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int),
                 ("num_meshes", ctypes.c_int), ("num_instances", ctypes.c_int), ("frames", ctypes.c_int),
-                ("frame_start", ctypes.c_int)]
+                ("frame_start", ctypes.c_int), ("full", ctypes.c_int), ("num_emitters", ctypes.c_int),
+                ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("sun_on", ctypes.c_int), ("light_mode", ctypes.c_int),
+                ("rr", ctypes.c_int), ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int),
+                ("env_uniform", ctypes.c_int), ("env_rot_deg", ctypes.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -253,7 +257,9 @@ _ARRAY_LAYOUT = {ARR_VERTICES: (np.float32, (3,)), ARR_TRIANGLES: (np.int32, (3,
                  ARR_ALBEDO: (np.float32, (4,)), ARR_ORDER: (np.int32, ()), ARR_WOOP: (np.float32, (3, 4)),
                  ARR_CORNERS: (np.float32, (3, 4)), ARR_NORMALS: (np.float32, (3,)), ARR_TREE: (np.int32, (4,)),
                  ARR_TREE_BOX: (np.float32, (6,)), ARR_PRIM: (np.int32, ()), ARR_MESH_ROOTS: (np.int32, ()),
-                 ARR_MESH_BOXES: (np.float32, (8,))}
+                 ARR_MESH_BOXES: (np.float32, (8,)), ARR_MATERIALS: (np.float32, (16,)), ARR_EM_TRI: (np.int32, ()),
+                 ARR_EM_CDF: (np.float32, ()), ARR_TRI_LIGHT_PDF: (np.float32, ()), ARR_ENV_TEX: (np.float32, (4,)),
+                 ARR_ENV_MARG: (np.float32, ()), ARR_ENV_COND: (np.float32, ())}
 
 
 def _view(ptr, count, which, node_floats4=None):

@@ -105,6 +105,11 @@ typedef struct cwp_scene_info {
     double sah, bvh_build_s;
     int width, height, spp, batch, bounces;
     int num_meshes, num_instances, frames, frame_start;   /* two-level scenes (num_meshes = 0 otherwise) */
+    /* full material integrator (full = 0: legacy Lambert + sun + sky) */
+    int full, num_emitters, env_w, env_h, sun_on;
+    int light_mode;     /* 0 MIS, 1 light sampling only, 2 BSDF sampling only */
+    int rr, rr_depth, gd_energy, env_uniform;
+    float env_rot_deg;
 } cwp_scene_info;
 
 /* Instance record of a two-level scene (112 bytes): world-to-object rows, object-to-world rows (3 x 4, row-major), BLAS
@@ -131,7 +136,16 @@ enum {
     CWP_ARR_TREE_BOX = 10,/* float[6] per binary SAH node: lo.xyz, hi.xyz */
     CWP_ARR_PRIM = 11,    /* int per binary-tree primitive slot: triangle id */
     CWP_ARR_MESH_ROOTS = 12, /* int per mesh of a two-level scene: BLAS root node */
-    CWP_ARR_MESH_BOXES = 13  /* float[8] per mesh of a two-level scene: object-space box lo (xyz, 0), hi (xyz, 0) */
+    CWP_ARR_MESH_BOXES = 13, /* float[8] per mesh of a two-level scene: object-space box lo (xyz, 0), hi (xyz, 0) */
+    /* full material integrator */
+    CWP_ARR_MATERIALS = 14,  /* float[16] per material: base rgb, metallic, specular rgb, roughness, emission rgb, ior,
+                                specular weight, transmission, 0, 0 */
+    CWP_ARR_EM_TRI = 15,     /* int per emitter: leaf position, in power-CDF order */
+    CWP_ARR_EM_CDF = 16,     /* float per emitter + 1: power CDF */
+    CWP_ARR_TRI_LIGHT_PDF = 17, /* float per leaf position: light-selection probability / area */
+    CWP_ARR_ENV_TEX = 18,    /* float[4] per environment texel (radiance x strength), row 0 = +z */
+    CWP_ARR_ENV_MARG = 19,   /* float per environment row + 1: marginal CDF */
+    CWP_ARR_ENV_COND = 20    /* float per (row, column + 1): conditional CDFs */
 };
 
 const char* cwp_last_error(void);

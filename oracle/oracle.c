@@ -954,6 +954,540 @@ static f3 trace_path(const orc_scene* s, const orc_params* p, int pix, uint32_t 
     return L;
 }
 
+/* ============================================================================================================
+ * Full material integrator: the reference's shadeFull (src/render/wavefront.cu:223-356) with its BSDFs
+ * (src/render/bsdf.h:1-220) and light tables (src/scene/environment.cpp:7-37, src/app/scene_setup.cpp:225-247).
+ * The path's radiance is summed in the order the product adds it: emission of each hit, then that vertex's
+ * unoccluded light sample.
+ * ============================================================================================================ */
+#define ORC_PI 3.14159265f
+#define ORC_TWO_PI 6.2831853f
+#define ORC_INV_PI 0.31830989f
+#define ORC_DELTA_ALPHA 1e-3f
+#define ORC_ONE_MINUS 0.99999994f
+
+typedef struct { f3 base; float metallic; f3 spec; float rough; f3 emis; float ior; float ws, transm; } orc_mat;
+static orc_mat ld_mat(const float* m) {
+    orc_mat r;
+    r.base = ld3(m); r.metallic = m[3]; r.spec = ld3(m + 4); r.rough = m[7]; r.emis = ld3(m + 8); r.ior = m[11];
+    r.ws = m[12]; r.transm = m[13];
+    return r;
+}
+static float lum3(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+static int emissive(const orc_mat* m) { return m->emis.x > 0.f || m->emis.y > 0.f || m->emis.z > 0.f; }
+
+/* ---- GGX, Smith height-correlated G2, visible normals (Walter 2007, Heitz 2014, Dupuy & Benyoub 2023) */
+static float g_d(float c, float a2) { if (c <= 0.f) return 0.f; float c2 = c * c; float d = c2 * (a2 - 1.f) + 1.f; return a2 / (ORC_PI * d * d); }
+static float g_lambda(float c, float a2) { float c2 = c * c; if (c2 < 1e-12f) return 1e12f; return 0.5f * (sqrtf(1.f + a2 * (1.f - c2) / c2) - 1.f); }
+static float g_g1(float c, float a2) { return 1.f / (1.f + g_lambda(c, a2)); }
+static float g_g2(float co, float ci, float a2) { return 1.f / (1.f + g_lambda(co, a2) + g_lambda(ci, a2)); }
+static float g_vpdf(f3 wo, f3 m, float a2) { float om = dot3(wo, m); if (om <= 0.f || m.z <= 0.f) return 0.f; return g_g1(wo.z, a2) * om * g_d(m.z, a2) / wo.z; }
+static f3 g_vsample(f3 wo, float a, float u1, float u2) {
+    f3 ws = nrm3(mk(a * wo.x, a * wo.y, wo.z));
+    float ph = ORC_TWO_PI * u1;
+    float z = (1.f - u2) * (1.f + ws.z) - ws.z;
+    float sn = sqrtf(fminf(fmaxf(1.f - z * z, 0.f), 1.f));
+    f3 h = add(mk(sn * cosf(ph), sn * sinf(ph), z), ws);
+    return nrm3(mk(a * h.x, a * h.y, fmaxf(h.z, 0.f)));
+}
+static float fr_diel(float ci, float eta) {
+    float s2t = (1.f - ci * ci) / (eta * eta);
+    if (s2t >= 1.f) return 1.f;
+    float ct = sqrtf(1.f - s2t);
+    float rs = (ci - eta * ct) / (ci + eta * ct), rp = (eta * ci - ct) / (eta * ci + ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+static f3 fr_f82(f3 F0, f3 tint, float mu) {
+    mu = fminf(fmaxf(mu, 0.f), 1.f);
+    float m = 1.f - mu, m2 = m * m, m5 = m2 * m2 * m, m6 = m5 * m;
+    const float mb = 1.f / 7.f, mb5 = 0.46266436f, mb6 = 0.39656945f;
+    f3 one = mk(1.f, 1.f, 1.f);
+    f3 Fs = add(F0, scl(sub(one, F0), m5));
+    f3 Fsb = add(F0, scl(sub(one, F0), mb5));
+    f3 b = scl(mul(Fsb, sub(one, tint)), 1.f / (mb * mb6));
+    f3 F = sub(Fs, scl(b, mu * m6));
+    return mk(fminf(fmaxf(F.x, 0.f), 1.f), fminf(fmaxf(F.y, 0.f), 1.f), fminf(fmaxf(F.z, 0.f), 1.f));
+}
+
+/* ---- lobes; local frame, wo.z > 0 */
+typedef struct { f3 f; float pdf; } bval;
+typedef struct { f3 base; float ws, eta, a, a2; int delta; float Fo, pSpec, kd; } gd_t;
+static gd_t gd_make(const orc_mat* m, f3 wo, int energy, float ux1, float ux2) {
+    gd_t g;
+    g.base = m->base; g.ws = m->ws; g.eta = m->ior; g.a = m->rough * m->rough; g.a2 = g.a * g.a; g.delta = g.a < ORC_DELTA_ALPHA;
+    g.Fo = g.ws > 0.f ? g.ws * fr_diel(wo.z, g.eta) : 0.f;
+    float pd = (1.f - g.Fo) * lum3(g.base);
+    g.pSpec = g.Fo + pd > 0.f ? g.Fo / (g.Fo + pd) : 0.f;
+    g.kd = 1.f - g.Fo;
+    if (energy && g.ws > 0.f && !g.delta) {
+        f3 mm = g_vsample(wo, g.a, ux1, ux2);
+        float om = dot3(wo, mm);
+        f3 wr = sub(scl(mm, 2.f * om), wo);
+        float X = (om > 0.f && wr.z > 0.f) ? g.ws * fr_diel(om, g.eta) * g_g2(wo.z, wr.z, g.a2) / g_g1(wo.z, g.a2) : 0.f;
+        g.kd = 1.f - X;
+    }
+    g.kd = fmaxf(g.kd, 0.f);
+    return g;
+}
+static bval gd_eval(const gd_t* g, f3 wo, f3 wi) {
+    bval r = { { 0.f, 0.f, 0.f }, 0.f };
+    if (wi.z <= 0.f) return r;
+    r.f = scl(g->base, g->kd * ORC_INV_PI * 1.f);
+    r.pdf = (1.f - g->pSpec) * wi.z * ORC_INV_PI;
+    if (!g->delta && g->Fo > 0.f) {
+        f3 m = nrm3(add(wo, wi));
+        float om = dot3(wo, m);
+        if (om > 0.f) {
+            float F = g->ws * fr_diel(om, g->eta);
+            float sp = F * g_d(m.z, g->a2) * g_g2(wo.z, wi.z, g->a2) / (4.f * wo.z * wi.z);
+            r.f = add(r.f, mk(sp, sp, sp));
+            r.pdf += g->pSpec * g_vpdf(wo, m, g->a2) / (4.f * om);
+        }
+    }
+    return r;
+}
+static int gd_sample(const gd_t* g, f3 wo, float u, float u1, float u2, f3* wi, f3* w, float* pdf) {
+    if (u < g->pSpec) {
+        if (g->delta) { *wi = mk(-wo.x, -wo.y, wo.z); float s = g->Fo / g->pSpec; *w = mk(s, s, s); *pdf = 0.f; return 1; }
+        f3 m = g_vsample(wo, g->a, u1, u2);
+        *wi = sub(scl(m, 2.f * dot3(wo, m)), wo);
+    } else {
+        *wi = cos_hemi(mk(0.f, 0.f, 1.f), u1, u2);
+    }
+    if (wi->z <= 0.f) return 0;
+    bval e = gd_eval(g, wo, *wi);
+    if (e.pdf <= 0.f) return 0;
+    *w = scl(e.f, wi->z / e.pdf); *pdf = e.pdf;
+    return 1;
+}
+static bval cd_eval(const orc_mat* mt, f3 wo, f3 wi) {
+    bval r = { { 0.f, 0.f, 0.f }, 0.f };
+    float a = mt->rough * mt->rough, a2 = a * a;
+    if (a < ORC_DELTA_ALPHA || wi.z <= 0.f) return r;
+    f3 m = nrm3(add(wo, wi));
+    float om = dot3(wo, m);
+    if (om <= 0.f) return r;
+    r.f = scl(fr_f82(mt->base, mt->spec, om), mt->ws * g_d(m.z, a2) * g_g2(wo.z, wi.z, a2) / (4.f * wo.z * wi.z));
+    r.pdf = g_vpdf(wo, m, a2) / (4.f * om);
+    return r;
+}
+static int cd_sample(const orc_mat* mt, f3 wo, float u1, float u2, f3* wi, f3* w, float* pdf) {
+    float a = mt->rough * mt->rough, a2 = a * a;
+    if (a < ORC_DELTA_ALPHA) { *wi = mk(-wo.x, -wo.y, wo.z); *w = scl(fr_f82(mt->base, mt->spec, wo.z), mt->ws); *pdf = 0.f; return 1; }
+    f3 m = g_vsample(wo, a, u1, u2);
+    float om = dot3(wo, m);
+    *wi = sub(scl(m, 2.f * om), wo);
+    if (wi->z <= 0.f || om <= 0.f) return 0;
+    *w = scl(fr_f82(mt->base, mt->spec, om), mt->ws * g_g2(wo.z, wi->z, a2) / g_g1(wo.z, a2));
+    *pdf = g_vpdf(wo, m, a2) / (4.f * om);
+    return 1;
+}
+static bval dl_eval(const orc_mat* mt, float eta, f3 wo, f3 wi) {
+    bval r = { { 0.f, 0.f, 0.f }, 0.f };
+    float a = mt->rough * mt->rough, a2 = a * a;
+    if (a < ORC_DELTA_ALPHA) return r;
+    if (wi.z > 0.f) {
+        f3 m = nrm3(add(wo, wi));
+        float om = dot3(wo, m);
+        if (om <= 0.f) return r;
+        float F = fr_diel(om, eta);
+        float sp = F * g_d(m.z, a2) * g_g2(wo.z, wi.z, a2) / (4.f * wo.z * wi.z);
+        r.f = mk(sp, sp, sp); r.pdf = F * g_vpdf(wo, m, a2) / (4.f * om);
+    } else if (wi.z < 0.f) {
+        f3 m = nrm3(add(wo, scl(wi, eta)));
+        if (m.z < 0.f) m = neg(m);
+        float om = dot3(wo, m), im = dot3(wi, m);
+        if (om <= 0.f || im >= 0.f) return r;
+        float F = fr_diel(om, eta);
+        if (F >= 1.f) return r;
+        float den = om + eta * im, jac = eta * eta * (-im) / (den * den);
+        float sp = (1.f - F) * g_d(m.z, a2) * g_g2(wo.z, -wi.z, a2) * om * jac / (wo.z * -wi.z);
+        r.f = scl(mt->base, sp); r.pdf = (1.f - F) * g_vpdf(wo, m, a2) * jac;
+    }
+    return r;
+}
+static int dl_sample(const orc_mat* mt, float eta, f3 wo, float u, float u1, float u2, f3* wi, f3* w, float* pdf) {
+    float a = mt->rough * mt->rough;
+    int delta = a < ORC_DELTA_ALPHA;
+    f3 m = delta ? mk(0.f, 0.f, 1.f) : g_vsample(wo, a, u1, u2);
+    float om = dot3(wo, m);
+    if (om <= 0.f) return 0;
+    float F = fr_diel(om, eta);
+    if (u < F) {
+        *wi = sub(scl(m, 2.f * om), wo);
+        if (wi->z <= 0.f) return 0;
+        if (delta) { *w = mk(1.f, 1.f, 1.f); *pdf = 0.f; return 1; }
+    } else {
+        float ct = sqrtf(fmaxf(1.f - (1.f - om * om) / (eta * eta), 0.f));
+        *wi = add(scl(wo, -1.f / eta), scl(m, om / eta - ct));
+        if (wi->z >= 0.f) return 0;
+        *wi = nrm3(*wi);
+        if (delta) { *w = mt->base; *pdf = 0.f; return 1; }
+    }
+    bval e = dl_eval(mt, eta, wo, *wi);
+    if (e.pdf <= 0.f) return 0;
+    *w = scl(e.f, fabsf(wi->z) / e.pdf); *pdf = e.pdf;
+    return 1;
+}
+static void base_w(const orc_mat* m, float* wM, float* wT, float* wD) {
+    *wM = m->metallic; *wT = (1.f - m->metallic) * m->transm; *wD = (1.f - m->metallic) * (1.f - m->transm);
+}
+static int has_non_delta(const orc_mat* m) {
+    float wM, wT, wD; base_w(m, &wM, &wT, &wD);
+    int rough = m->rough * m->rough >= ORC_DELTA_ALPHA;
+    return (wD > 0.f && (lum3(m->base) > 0.f || rough)) || ((wM > 0.f || wT > 0.f) && rough);
+}
+static bval mat_eval(const orc_mat* m, f3 wo, f3 wi, float eta, int gde, float ux1, float ux2) {
+    float wM, wT, wD; base_w(m, &wM, &wT, &wD);
+    bval r = { { 0.f, 0.f, 0.f }, 0.f };
+    if (wM > 0.f) { bval e = cd_eval(m, wo, wi); r.f = add(r.f, scl(e.f, wM)); r.pdf += wM * e.pdf; }
+    if (wT > 0.f) { bval e = dl_eval(m, eta, wo, wi); r.f = add(r.f, scl(e.f, wT)); r.pdf += wT * e.pdf; }
+    if (wD > 0.f) { gd_t g = gd_make(m, wo, gde, ux1, ux2); bval e = gd_eval(&g, wo, wi); r.f = add(r.f, scl(e.f, wD)); r.pdf += wD * e.pdf; }
+    return r;
+}
+static int mat_sample(const orc_mat* m, f3 wo, float eta, int gde, float ux1, float ux2, float u, float u1, float u2, f3* wi, f3* w, float* pdf) {
+    float wM, wT, wD; base_w(m, &wM, &wT, &wD);
+    int ok;
+    if (u < wM) ok = cd_sample(m, wo, u1, u2, wi, w, pdf);
+    else if (u < wM + wT) { float uu = (u - wM) / wT; ok = dl_sample(m, eta, wo, fminf(uu, ORC_ONE_MINUS), u1, u2, wi, w, pdf); }
+    else { float uu = wD > 0.f ? (u - wM - wT) / wD : 0.f; gd_t g = gd_make(m, wo, gde, ux1, ux2); ok = gd_sample(&g, wo, fminf(uu, ORC_ONE_MINUS), u1, u2, wi, w, pdf); }
+    if (!ok) return 0;
+    if (*pdf <= 0.f) return 1;
+    if (wM == 1.f || wT == 1.f || wD == 1.f) return 1;
+    bval e = mat_eval(m, wo, *wi, eta, gde, ux1, ux2);
+    if (e.pdf <= 0.f) return 0;
+    *w = scl(e.f, fabsf(wi->z) / e.pdf); *pdf = e.pdf;
+    return 1;
+}
+static float mis2(float a, float b) { float a2 = a * a; return a2 / (a2 + b * b); }
+
+void orc_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gde, float ux1, float ux2, float f[3], float* pdf) {
+    orc_mat m = ld_mat(mat);
+    bval e = mat_eval(&m, ld3(wo), ld3(wi), eta, gde, ux1, ux2);
+    f[0] = e.f.x; f[1] = e.f.y; f[2] = e.f.z; *pdf = e.pdf;
+}
+int orc_bsdf_sample(const float* mat, const float wo[3], float eta, int gde, float ux1, float ux2, float u, float u1, float u2,
+                    float wi[3], float weight[3], float* pdf) {
+    orc_mat m = ld_mat(mat);
+    f3 a, w; float p = 0.f;
+    int ok = mat_sample(&m, ld3(wo), eta, gde, ux1, ux2, u, u1, u2, &a, &w, &p);
+    wi[0] = a.x; wi[1] = a.y; wi[2] = a.z; weight[0] = w.x; weight[1] = w.y; weight[2] = w.z; *pdf = p;
+    return ok;
+}
+static uint64_t sm_next(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float sm_u(uint64_t* s) { float f = (float)(sm_next(s) >> 40) * (1.0f / 16777216.0f); return f < 1.f ? f : ORC_ONE_MINUS; }
+void orc_bsdf_sample_batch(const float* mat, const float wo[3], float eta, int gde, int n, uint64_t seed, float* wi, float* weight,
+                           float* pdf, int* ok) {
+    orc_mat m = ld_mat(mat);
+    uint64_t st = seed;
+    for (int k = 0; k < n; k++) {
+        float u = sm_u(&st), u1 = sm_u(&st), u2 = sm_u(&st), x1 = sm_u(&st), x2 = sm_u(&st);
+        f3 a = { 0, 0, 0 }, w = { 0, 0, 0 }; float p = 0.f;
+        ok[k] = mat_sample(&m, ld3(wo), eta, gde, x1, x2, u, u1, u2, &a, &w, &p);
+        wi[3 * k] = a.x; wi[3 * k + 1] = a.y; wi[3 * k + 2] = a.z;
+        weight[3 * k] = w.x; weight[3 * k + 1] = w.y; weight[3 * k + 2] = w.z; pdf[k] = p;
+    }
+}
+void orc_bsdf_pdf_batch(const float* mat, const float wo[3], float eta, int gde, const float* wi, int n, float* pdf) {
+    orc_mat m = ld_mat(mat);
+    for (int k = 0; k < n; k++) pdf[k] = mat_eval(&m, ld3(wo), ld3(wi + 3 * k), eta, gde, 0.5f, 0.5f).pdf;
+}
+float orc_fresnel_dielectric(float ci, float eta) { return fr_diel(ci, eta); }
+void orc_fresnel_f82(const float f0[3], const float tint[3], float mu, float out[3]) {
+    f3 r = fr_f82(ld3(f0), ld3(tint), mu); out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float orc_ggx_d(float c, float a2) { return g_d(c, a2); }
+float orc_ggx_g2(float co, float ci, float a2) { return g_g2(co, ci, a2); }
+double orc_quad_albedo(int kind, float rough, float eta, const float wo3[3]) {
+    const double a = (double)rough * rough, a2 = a * a;
+    const int NU = 1024, NV = 256;
+    f3 wo = ld3(wo3);
+    double sum = 0;
+    for (int i = 0; i < NU; i++)
+        for (int j = 0; j < NV; j++) {
+            /* m ~ D(m) cos(theta_m) by the GGX inverse CDF; u1 = 1 - s^2 removes the endpoint singularity */
+            double sq = (i + 0.5) / NU, u1 = 1.0 - sq * sq, u2 = (j + 0.5) / NV, jac = 2.0 * sq;
+            double t2 = a2 * u1 / (1.0 - u1), ct = 1.0 / sqrt(1.0 + t2), st = sqrt(fmax(0.0, 1 - ct * ct)), ph = 2 * M_PI * u2;
+            f3 m = mk((float)(st * cos(ph)), (float)(st * sin(ph)), (float)ct);
+            double om = dot3(wo, m);
+            if (om <= 0) continue;
+            double g = om / (wo.z * ct);
+            f3 wr = sub(scl(m, (float)(2 * om)), wo);
+            double val = 0;
+            if (kind == 0) { if (wr.z > 0) val = g_g2(wo.z, wr.z, (float)a2); }
+            else if (kind == 1) {
+                double F = fr_diel((float)om, eta);
+                if (wr.z > 0) val += F * g_g2(wo.z, wr.z, (float)a2);
+                if (F < 1) {
+                    double cT = sqrt(fmax(0.0, 1 - (1 - om * om) / ((double)eta * eta)));
+                    f3 wt = add(scl(wo, -1.f / eta), scl(m, (float)(om / eta - cT)));
+                    if (wt.z < 0) val += (1 - F) * g_g2(wo.z, -nrm3(wt).z, (float)a2);
+                }
+            } else { if (wr.z > 0) val = fr_diel((float)om, eta) * g_g2(wo.z, wr.z, (float)a2); }
+            sum += g * val * jac;
+        }
+    return sum / ((double)NU * NV);
+}
+
+/* ---- environment: equirectangular, row 0 = +z, u = 0.5 - atan2(y, x) / 2pi (src/scene/environment.h:1-5) */
+void orc_env_build(const float* rgb, int w, int h, float strength, float* tex, float* marg, float* cond) {
+    double* row = (double*)malloc(sizeof(double) * (w + 1));
+    double* rs = (double*)malloc(sizeof(double) * h);
+    double* mc = (double*)malloc(sizeof(double) * (h + 1));
+    mc[0] = 0.0;
+    for (int y = 0; y < h; y++) {
+        double st = sin(M_PI * (y + 0.5) / h);
+        row[0] = 0.0;
+        for (int x = 0; x < w; x++) {
+            const float* c = rgb + 3 * ((size_t)y * w + x);
+            f3 L = scl(mk(c[0], c[1], c[2]), strength);
+            float* t = tex + 4 * ((size_t)y * w + x);
+            t[0] = L.x; t[1] = L.y; t[2] = L.z; t[3] = 0.f;
+            double lum = 0.2126 * L.x + 0.7152 * L.y + 0.0722 * L.z;
+            row[x + 1] = row[x] + (lum > 0 ? lum : 0.0) * st;
+        }
+        rs[y] = row[w];
+        float* cr = cond + (size_t)y * (w + 1);
+        for (int x = 0; x <= w; x++) cr[x] = rs[y] > 0 ? (float)(row[x] / rs[y]) : (float)x / (float)w;
+        cr[w] = 1.f;
+        mc[y + 1] = mc[y] + rs[y];
+    }
+    for (int y = 0; y <= h; y++) marg[y] = mc[h] > 0 ? (float)(mc[y] / mc[h]) : (float)y / (float)h;
+    marg[h] = 1.f;
+    free(row); free(rs); free(mc);
+}
+static int cdf_find(const float* cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (cdf[mid] <= u) lo = mid; else hi = mid; }
+    return lo;
+}
+static void env_uv(const orc_scene* s, f3 d, float* u, float* v) {
+    *u = 0.5f - (atan2f(d.y, d.x) - s->env_rot) * (1.f / ORC_TWO_PI);
+    *u -= floorf(*u);
+    *v = acosf(fminf(fmaxf(d.z, -1.f), 1.f)) * ORC_INV_PI;
+}
+static f3 env_dir(const orc_scene* s, float u, float v) {
+    float ph = (0.5f - u) * ORC_TWO_PI + s->env_rot, th = v * ORC_PI, st = sinf(th);
+    return mk(st * cosf(ph), st * sinf(ph), cosf(th));
+}
+static void env_texel(const orc_scene* s, f3 d, int* x, int* y, float* v) {
+    float u; env_uv(s, d, &u, v);
+    *x = (int)(u * (float)s->env_w); if (*x > s->env_w - 1) *x = s->env_w - 1;
+    *y = (int)(*v * (float)s->env_h); if (*y > s->env_h - 1) *y = s->env_h - 1;
+}
+static f3 env_rad(const orc_scene* s, f3 d) { int x, y; float v; env_texel(s, d, &x, &y, &v); return ld3(s->env_tex + 4 * ((size_t)y * s->env_w + x)); }
+static float env_pdf_texel(const orc_scene* s, int x, int y, float v) {
+    const float* cr = s->env_cond + (size_t)y * (s->env_w + 1);
+    float puv = (s->env_marg[y + 1] - s->env_marg[y]) * (cr[x + 1] - cr[x]) * (float)s->env_w * (float)s->env_h;
+    float st = sinf(v * ORC_PI);
+    return st > 0.f ? puv / (2.f * ORC_PI * ORC_PI * st) : 0.f;
+}
+static float env_pdf_dir(const orc_scene* s, f3 d) {
+    if (s->env_uniform) return 0.25f * ORC_INV_PI;
+    int x, y; float v; env_texel(s, d, &x, &y, &v); return env_pdf_texel(s, x, y, v);
+}
+float orc_env_pdf(const orc_scene* s, const float d[3]) { return env_pdf_dir(s, ld3(d)); }
+/* luminance sample from (ue, u2, u3): direction, texel and pdf (0 if none) */
+static float env_sample(const orc_scene* s, float ue, float u2, float u3, f3* l, int* tx, int* ty) {
+    int y = cdf_find(s->env_marg, s->env_h, ue);
+    const float* cr = s->env_cond + (size_t)y * (s->env_w + 1);
+    int x = cdf_find(cr, s->env_w, u2);
+    float v = ((float)y + u3) / (float)s->env_h, u = ((float)x + 0.5f) / (float)s->env_w;
+    float w0 = cr[x + 1] - cr[x];
+    if (w0 > 0.f) u = ((float)x + fminf((u2 - cr[x]) / w0, ORC_ONE_MINUS)) / (float)s->env_w;
+    *l = env_dir(s, u, v); *tx = x; *ty = y;
+    return env_pdf_texel(s, x, y, v);
+}
+void orc_env_sample_batch(const orc_scene* s, int n, uint64_t seed, float* dirs, float* pdf) {
+    uint64_t st = seed;
+    for (int k = 0; k < n; k++) {
+        float ue = sm_u(&st), u2 = sm_u(&st), u3 = sm_u(&st);
+        f3 l; int x, y;
+        pdf[k] = env_sample(s, ue, u2, u3, &l, &x, &y);
+        dirs[3 * k] = l.x; dirs[3 * k + 1] = l.y; dirs[3 * k + 2] = l.z;
+    }
+}
+
+/* ---- emissive triangles by power, in mesh order (src/app/scene_setup.cpp:225-247) */
+typedef struct { int tri, pos; } em_pair;
+static int em_cmp(const void* a, const void* b) { int x = ((const em_pair*)a)->tri, y = ((const em_pair*)b)->tri; return (x > y) - (x < y); }
+int orc_emitter_tables(const float* corners, const int* order, int nt, const float* mats, int* em_tri, float* em_cdf, float* tri_light_pdf) {
+    em_pair* em = (em_pair*)malloc(sizeof(em_pair) * (nt > 0 ? nt : 1));
+    int ne = 0;
+    for (int k = 0; k < nt; k++) {
+        orc_mat m = ld_mat(mats + 16 * f2i(corners[12 * (size_t)k + 7]));
+        if (emissive(&m)) { em[ne].tri = order[k]; em[ne].pos = k; ne++; }
+    }
+    qsort(em, ne, sizeof(em_pair), em_cmp);
+    double* pw = (double*)malloc(sizeof(double) * (ne + 1));
+    double* ar = (double*)malloc(sizeof(double) * (ne + 1));
+    double tot = 0;
+    int n = 0;
+    for (int e = 0; e < ne; e++) {
+        const float* C = corners + 12 * (size_t)em[e].pos;
+        f3 e1 = sub(ld3(C + 4), ld3(C)), e2 = sub(ld3(C + 8), ld3(C));
+        f3 cr = crs(e1, e2);
+        double a = 0.5 * (double)sqrtf(dot3(cr, cr));
+        orc_mat m = ld_mat(mats + 16 * f2i(C[7]));
+        double p = (double)lum3(m.emis) * a;
+        if (!(p > 0)) continue;
+        em_tri[n] = em[e].pos; pw[n] = p; ar[n] = a; tot += p; n++;
+    }
+    for (int k = 0; k < nt; k++) tri_light_pdf[k] = 0.f;
+    em_cdf[0] = 0.f;
+    double c = 0;
+    for (int e = 0; e < n; e++) {
+        tri_light_pdf[em_tri[e]] = (float)(pw[e] / tot / ar[e]);
+        c += pw[e];
+        em_cdf[e + 1] = (float)(c / tot);
+    }
+    if (n) em_cdf[n] = 1.f;
+    free(em); free(pw); free(ar);
+    return n;
+}
+
+/* ---- one path of the full integrator */
+static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint32_t smp, int bounces, double* st) {
+    float o3[3], d3[3];
+    orc_camera_ray(p, pix, smp, o3, d3);
+    f3 o = ld3(o3), d = ld3(d3), T = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
+    const f3 sky = ld3(p->sky), sunE = ld3(p->sun_e), sunTo = ld3(p->sun_to);
+    const uint32_t seed = p->seed;
+    float lastPdf = 0.f;
+    const int depths = bounces + 2;
+    for (int depth = 0; depth < depths; depth++) {
+        ray_t r = mkray(o, d, depth == 0 ? p->near_t : 0.f, s->tri_test);
+        hit_t h = { 1e30f, 0.f, 0.f, -1, -1 };
+        trace_one(s, &r, &h, 0, NULL, NULL);
+        st[0] += 1;
+        if (h.tri < 0) {
+            st[2] += 1;
+            f3 Lm;
+            if (s->env_w > 0) {
+                Lm = env_rad(s, d);
+                if (lastPdf > 0.f) {
+                    if (s->light_mode == 1) Lm = mk(0.f, 0.f, 0.f);
+                    else if (s->light_mode == 0) Lm = scl(Lm, mis2(lastPdf, s->p_env * env_pdf_dir(s, d)));
+                }
+            } else Lm = sky;
+            L = add(L, mul(T, Lm));
+            break;
+        }
+        f3 ph = add(o, scl(d, h.t));
+        const float* P = s->corners + 12 * h.tri;
+        f3 e1 = sub(ld3(P + 4), ld3(P)), e2 = sub(ld3(P + 8), ld3(P));
+        f3 ng0 = nrm3(crs(e1, e2)), ns = ng0;
+        if (p->has_normals && s->normals) {
+            const float* N = s->normals + 12 * h.tri;
+            f3 nn = add(add(scl(ld3(N), 1.f - h.u - h.v), scl(ld3(N + 4), h.u)), scl(ld3(N + 8), h.v));
+            float l = sqrtf(dot3(nn, nn));
+            if (l > 0.f) ns = scl(nn, 1.f / l);
+            if (dot3(ns, ng0) < 0.f) ns = neg(ns);
+        }
+        orc_mat M = ld_mat(s->mats + 16 * f2i(P[7]));
+        int front = dot3(ng0, d) < 0.f;
+        f3 ng = front ? ng0 : neg(ng0);
+        if (!front) ns = neg(ns);
+        if (emissive(&M)) {
+            float w = 1.f;
+            if (lastPdf > 0.f) {
+                if (s->light_mode == 1) w = 0.f;
+                else if (s->light_mode == 0) {
+                    float cl = fabsf(dot3(ng0, d));
+                    float pl = cl > 0.f ? s->p_tri * s->tri_light_pdf[h.tri] * h.t * h.t / cl : 0.f;
+                    w = mis2(lastPdf, pl);
+                }
+            }
+            L = add(L, scl(mul(T, M.emis), w));
+        }
+        if (depth > p->max_bounce) break;
+        uint32_t dd = (uint32_t)depth + 1u;
+        if (s->rr && depth >= s->rr_depth) {
+            float mx = fmaxf(T.x, fmaxf(T.y, T.z));
+            float qs = fminf(fmaxf(mx, 0.05f), 1.f);
+            if (orc_rng4((uint32_t)pix, smp, dd, 6, seed) >= qs) break;
+            T = scl(T, 1.f / qs);
+        }
+        f3 wo = neg(d);
+        if (dot3(ns, wo) <= 1e-6f) ns = ng;
+        f3 tx, ty; onb(ns, &tx, &ty);
+        f3 woL = mk(dot3(wo, tx), dot3(wo, ty), dot3(wo, ns));
+        float eta = front ? M.ior : 1.f / M.ior;
+        f3 poUp = add(ph, scl(ng, p->eps)), poDn = sub(ph, scl(ng, p->eps));
+        float ux1 = orc_rng4((uint32_t)pix, smp, dd, 7, seed), ux2 = orc_rng4((uint32_t)pix, smp, dd, 7, seed ^ 0x68E31DA4u);
+        if (has_non_delta(&M)) {
+            float us = orc_rng4((uint32_t)pix, smp, dd, 5, seed);
+            float u2 = orc_rng4((uint32_t)pix, smp, dd, 2, seed), u3 = orc_rng4((uint32_t)pix, smp, dd, 3, seed);
+            f3 l = mk(0.f, 0.f, 1.f), Le = mk(0.f, 0.f, 0.f);
+            float pl = 0.f, tmax = 1e30f;
+            int isSun = 0, ok = 0, side = 0;
+            if (us < s->p_sun) { l = cone(sunTo, p->sun_cos, u2, u3); Le = sunE; pl = s->p_sun; isSun = 1; ok = 1; }
+            else if (us < s->p_sun + s->p_tri) {
+                float ut = fminf((us - s->p_sun) / s->p_tri, ORC_ONE_MINUS);
+                int kt = s->em_tri[cdf_find(s->em_cdf, s->num_em, ut)];
+                const float* A = s->corners + 12 * (size_t)kt;
+                f3 f1 = sub(ld3(A + 4), ld3(A)), f2 = sub(ld3(A + 8), ld3(A));
+                float su = sqrtf(u2);
+                f3 pL = add(add(ld3(A), scl(f1, su * (1.f - u3))), scl(f2, su * u3));
+                f3 nL = nrm3(crs(f1, f2));
+                side = dot3(ng, sub(pL, ph)) > 0.f ? 1 : -1;
+                f3 org = side > 0 ? poUp : poDn;
+                f3 dv = sub(pL, org);
+                float dist2 = dot3(dv, dv), dist = sqrtf(dist2);
+                if (dist > 0.f) {
+                    l = scl(dv, 1.f / dist);
+                    float cl = fabsf(dot3(nL, l));
+                    if (cl > 0.f) {
+                        pl = s->p_tri * s->tri_light_pdf[kt] * dist2 / cl;
+                        Le = ld_mat(s->mats + 16 * f2i(A[7])).emis;
+                        tmax = dist * (1.f - 1e-3f);
+                        ok = 1;
+                    }
+                }
+            } else if (s->p_env > 0.f && s->env_uniform) {
+                float z = 1.f - 2.f * u2, rr = sqrtf(fmaxf(0.f, 1.f - z * z)), phi = ORC_TWO_PI * u3;
+                l = mk(rr * cosf(phi), rr * sinf(phi), z); pl = s->p_env * (0.25f * ORC_INV_PI); Le = env_rad(s, l); ok = 1;
+            } else if (s->p_env > 0.f) {
+                float ue = fminf((us - s->p_sun - s->p_tri) / s->p_env, ORC_ONE_MINUS);
+                int x, y;
+                float pe = env_sample(s, ue, u2, u3, &l, &x, &y);
+                if (pe > 0.f) { pl = s->p_env * pe; Le = ld3(s->env_tex + 4 * ((size_t)y * s->env_w + x)); ok = 1; }
+            }
+            if (ok) {
+                f3 lL = mk(dot3(l, tx), dot3(l, ty), dot3(l, ns));
+                float gl = dot3(ng, l);
+                if (lL.z != 0.f && (gl > 0.f) == (lL.z > 0.f)) {
+                    bval e = mat_eval(&M, woL, lL, eta, s->gd_energy, ux1, ux2);
+                    float w = (isSun || s->light_mode == 1) ? 1.f : mis2(pl, e.pdf);
+                    f3 con = scl(mul(mul(T, e.f), Le), fabsf(lL.z) * w / pl);
+                    if (con.x > 0.f || con.y > 0.f || con.z > 0.f) {
+                        f3 so = (side ? side > 0 : gl > 0.f) ? poUp : poDn;
+                        ray_t sr = mkray(so, l, 0.f, s->tri_test);
+                        hit_t sh = { tmax, 0.f, 0.f, -1, -1 };
+                        st[1] += 1;
+                        if (!trace_one(s, &sr, &sh, 1, NULL, NULL)) { st[3] += 1; L = add(L, con); }
+                    }
+                }
+            }
+        }
+        f3 wiL, wgt; float pdf = 0.f;
+        if (!mat_sample(&M, woL, eta, s->gd_energy, ux1, ux2, orc_rng4((uint32_t)pix, smp, dd, 4, seed),
+                        orc_rng4((uint32_t)pix, smp, dd, 0, seed), orc_rng4((uint32_t)pix, smp, dd, 1, seed), &wiL, &wgt, &pdf))
+            break;
+        f3 wi = nrm3(add(add(scl(tx, wiL.x), scl(ty, wiL.y)), scl(ns, wiL.z)));
+        float gi = dot3(ng, wi);
+        if (gi == 0.f || (gi > 0.f) != (wiL.z > 0.f)) break;
+        T = mul(T, wgt);
+        if (!(fmaxf(T.x, fmaxf(T.y, T.z)) > 0.f)) break;
+        o = gi > 0.f ? poUp : poDn; d = wi; lastPdf = pdf;
+    }
+    return L;
+}
+
 typedef struct { const orc_scene* s; const orc_params* p; int spp, sbase, batch, bounces; float* accum; double st[64][4]; } render_job;
 static void render_worker(void* a, int tid, int nth) {
     render_job* J = (render_job*)a;
@@ -964,7 +1498,8 @@ static void render_worker(void* a, int tid, int nth) {
         for (int s0 = J->sbase; s0 < J->sbase + J->spp; s0 += J->batch) {
             float r = 0.f, g = 0.f, b = 0.f;
             for (int s = s0; s < s0 + J->batch; s++) {
-                f3 L = trace_path(J->s, J->p, pix, (uint32_t)s, J->bounces, st);
+                f3 L = J->s->full ? trace_path_full(J->s, J->p, pix, (uint32_t)s, J->bounces, st)
+                                  : trace_path(J->s, J->p, pix, (uint32_t)s, J->bounces, st);
                 r += L.x; g += L.y; b += L.z;
             }
             acc[0] += r; acc[1] += g; acc[2] += b;

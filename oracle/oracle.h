@@ -48,6 +48,23 @@ typedef struct orc_scene {
     int tlas_root;
     const float* inst;
     int num_inst;
+    /* full material integrator (full != 0): materials (16 floats each: base rgb, metallic, specular rgb, roughness,
+     * emission rgb, ior, specular weight, transmission, 0, 0), emitter tables (orc_emitter_tables), environment map
+     * (orc_env_build; env_w = 0: none, the sky constant is used), light-selection probabilities and options */
+    int full;
+    const float* mats;
+    int num_mats;
+    const float* tri_light_pdf;
+    const int* em_tri;
+    const float* em_cdf;
+    int num_em;
+    const float* env_tex;
+    const float* env_marg;
+    const float* env_cond;
+    int env_w, env_h;
+    float env_rot;
+    float p_sun, p_tri, p_env;
+    int light_mode, rr, rr_depth, gd_energy, env_uniform;
 } orc_scene;
 
 /* binary SAH tree: node = int[4] (left, right, first, count) + float[6] box (lo, hi); prim = triangle order */
@@ -116,6 +133,38 @@ int orc_trace_work(const orc_scene* s, const float* rays, int n, double* node_vi
 /* Renders spp samples (passes of `batch` samples; per-path radiance summed per pass in sample order) into accum
  * (float4 per pixel, += unless clear). stats[0..3] += closest rays, shadow rays, misses, unoccluded shadow rays.
  * Pixels rows are split over `threads` threads; each pixel is owned by one thread. */
+/* ---- full material integrator: src/render/bsdf.h, src/render/wavefront.cu:177-356, src/scene/environment.cpp,
+ * src/app/scene_setup.cpp:225-247 ---- */
+/* f (3) and pdf of a material (16 floats) for local directions wo, wi (z = shading normal) */
+void orc_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gd_energy, float ux1, float ux2,
+                   float f[3], float* pdf);
+/* returns 1 and wi, weight = f|cos|/pdf, pdf (0: delta) on success */
+int orc_bsdf_sample(const float* mat, const float wo[3], float eta, int gd_energy, float ux1, float ux2, float u, float u1,
+                    float u2, float wi[3], float weight[3], float* pdf);
+/* n samples with a seeded splitmix64 stream (5 numbers per sample: u, u1, u2, ux1, ux2): wi, weight, pdf, ok */
+void orc_bsdf_sample_batch(const float* mat, const float wo[3], float eta, int gd_energy, int n, uint64_t seed, float* wi,
+                           float* weight, float* pdf, int* ok);
+/* n evaluations (ux1 = ux2 = 0.5): pdf only */
+void orc_bsdf_pdf_batch(const float* mat, const float wo[3], float eta, int gd_energy, const float* wi, int n, float* pdf);
+float orc_fresnel_dielectric(float cos_i, float eta);
+void orc_fresnel_f82(const float f0[3], const float tint[3], float mu, float out[3]);
+float orc_ggx_d(float cos_m, float a2);
+float orc_ggx_g2(float cos_o, float cos_i, float a2);
+/* deterministic directional albedo by quadrature over microfacet normals: kind 0 conductor F = 1, 1 dielectric (R + T),
+ * 2 the glossy-diffuse specular lobe alone (F = Fdiel) */
+double orc_quad_albedo(int kind, float rough, float eta, const float wo[3]);
+/* environment distribution of w x h rgb texels (row 0 = +z): texels rgba * strength, marginal (h + 1) and conditional
+ * (h rows of w + 1) CDFs over luminance * sin(theta) */
+void orc_env_build(const float* rgb, int w, int h, float strength, float* tex, float* marg, float* cond);
+/* solid-angle pdf of the environment's luminance sampling in direction d (s->env_* set) */
+float orc_env_pdf(const orc_scene* s, const float d[3]);
+/* n luminance samples of the environment (u pairs from a seeded stream): directions and their pdf */
+void orc_env_sample_batch(const orc_scene* s, int n, uint64_t seed, float* dirs, float* pdf);
+/* emissive triangles by power (mesh order = order[] ascending); returns the emitter count; em_tri (n), em_cdf (n + 1),
+ * tri_light_pdf (num_tris) */
+int orc_emitter_tables(const float* corners, const int* order, int num_tris, const float* mats, int* em_tri, float* em_cdf,
+                       float* tri_light_pdf);
+
 int orc_render(const orc_scene* s, const orc_params* p, int spp, int sample_base, int batch, int bounces, int threads,
                int clear, float* accum, double* stats);
 /* the camera ray of (pixel, sample): origin and direction */

@@ -30,7 +30,12 @@ class OrcLayout(ctypes.Structure):
 class OrcScene(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("nodes", _FP), ("root", ctypes.c_int),
                 ("woop", _FP), ("corners", _FP), ("normals", _FP), ("albedo", _FP), ("num_tris", ctypes.c_int),
-                ("tlas", _FP), ("tlas_root", ctypes.c_int), ("inst", _FP), ("num_inst", ctypes.c_int)]
+                ("tlas", _FP), ("tlas_root", ctypes.c_int), ("inst", _FP), ("num_inst", ctypes.c_int),
+                ("full", ctypes.c_int), ("mats", _FP), ("num_mats", ctypes.c_int), ("tri_light_pdf", _FP), ("em_tri", _IP),
+                ("em_cdf", _FP), ("num_em", ctypes.c_int), ("env_tex", _FP), ("env_marg", _FP), ("env_cond", _FP),
+                ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("env_rot", ctypes.c_float), ("p_sun", ctypes.c_float),
+                ("p_tri", ctypes.c_float), ("p_env", ctypes.c_float), ("light_mode", ctypes.c_int), ("rr", ctypes.c_int),
+                ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int), ("env_uniform", ctypes.c_int)]
 
 
 _lib = None
@@ -73,6 +78,22 @@ def lib():
         L.orc_hp_build.argtypes = [_FP, _FP, ctypes.c_int, _FP, _U64P]
         L.orc_hp_to_bvh2.argtypes = [_FP, ctypes.c_int, _FP]
         L.orc_tree_from_hploc.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcTree)]
+        _F = ctypes.c_float
+        L.orc_bsdf_eval.argtypes = [_FP, _FP, _FP, _F, ctypes.c_int, _F, _F, _FP, _FP]
+        L.orc_bsdf_sample.argtypes = [_FP, _FP, _F, ctypes.c_int, _F, _F, _F, _F, _F, _FP, _FP, _FP]
+        L.orc_bsdf_sample.restype = ctypes.c_int
+        L.orc_bsdf_sample_batch.argtypes = [_FP, _FP, _F, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _FP, _FP, _FP, _IP]
+        L.orc_bsdf_pdf_batch.argtypes = [_FP, _FP, _F, ctypes.c_int, _FP, ctypes.c_int, _FP]
+        L.orc_fresnel_dielectric.argtypes = [_F, _F]; L.orc_fresnel_dielectric.restype = _F
+        L.orc_fresnel_f82.argtypes = [_FP, _FP, _F, _FP]
+        L.orc_ggx_d.argtypes = [_F, _F]; L.orc_ggx_d.restype = _F
+        L.orc_ggx_g2.argtypes = [_F, _F, _F]; L.orc_ggx_g2.restype = _F
+        L.orc_quad_albedo.argtypes = [ctypes.c_int, _F, _F, _FP]; L.orc_quad_albedo.restype = ctypes.c_double
+        L.orc_env_build.argtypes = [_FP, ctypes.c_int, ctypes.c_int, _F, _FP, _FP, _FP]
+        L.orc_env_pdf.argtypes = [ctypes.POINTER(OrcScene), _FP]; L.orc_env_pdf.restype = _F
+        L.orc_env_sample_batch.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_int, ctypes.c_uint64, _FP, _FP]
+        L.orc_emitter_tables.argtypes = [_FP, _IP, ctypes.c_int, _FP, _IP, _FP, _FP]
+        L.orc_emitter_tables.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -269,6 +290,36 @@ class Scene:
                           _f(self.normals) if self.normals is not None else None, _f(self.albedo), len(self.corners) // 12,
                           None, 0, None, 0)
 
+    def set_full(self, mats, order, sun_on=1, light_mode=0, rr=0, rr_depth=3, gd_energy=1, env_uniform=0, env_tex=None,
+                 env_w=0, env_h=0, env_rot_deg=0.0):
+        """Full material integrator: materials (m, 16), the leaf order (leaf position -> mesh triangle) for the
+        emitter CDF, options, and an optional environment (texels (h, w, 4) or (w*h, 4), radiance x strength). The
+        emitter tables and the environment CDFs are built by the oracle; the light types present share the selection
+        probability equally, the sun always, emitters and environment unless light_mode is 2 (BSDF only)."""
+        self.mats = np.ascontiguousarray(mats, dtype=np.float32).reshape(-1, 16)
+        nt = len(self.corners) // 12
+        self.em_tri, self.em_cdf, self.tri_light_pdf = emitter_tables(self.corners.reshape(-1, 3, 4), order, self.mats)
+        c = self.c
+        c.full = 1; c.mats = _f(self.mats); c.num_mats = len(self.mats)
+        c.num_em = len(self.em_tri)
+        c.em_tri = _i(self.em_tri); c.em_cdf = _f(self.em_cdf); c.tri_light_pdf = _f(self.tri_light_pdf)
+        if env_tex is not None and env_w > 0:
+            self.env_tex, self.env_marg, self.env_cond = env_build(np.asarray(env_tex, np.float32).reshape(-1, 4)[:, :3].reshape(env_h, env_w, 3), 1.0)
+            c.env_tex = _f(self.env_tex); c.env_marg = _f(self.env_marg); c.env_cond = _f(self.env_cond)
+            c.env_w, c.env_h = env_w, env_h
+            c.env_rot = np.float32(env_rot_deg) * np.float32(3.14159265) / np.float32(180.0)
+        else:
+            c.env_w = c.env_h = 0
+        tri = c.num_em > 0 and light_mode != 2
+        env = c.env_w > 0 and light_mode != 2
+        nt_types = int(bool(sun_on)) + int(tri) + int(env)
+        p = np.float32(1.0) / np.float32(nt_types) if nt_types else np.float32(0)
+        c.p_sun = p if sun_on else 0.0
+        c.p_tri = p if tri else 0.0
+        c.p_env = p if env else 0.0
+        c.light_mode, c.rr, c.rr_depth, c.gd_energy, c.env_uniform = light_mode, rr, rr_depth, gd_energy, env_uniform
+        assert nt == len(self.tri_light_pdf)
+
     def set_instances(self, inst, mesh_box):
         """Two-level scene: instance records (n, 28) float32 (world-to-object rows, object-to-world rows, root bits,
         mesh bits, det sign, pad) and object-space mesh boxes (m, 8) (lo, hi). Builds the TLAS with the oracle's H-PLOC
@@ -328,6 +379,55 @@ class Scene:
         return accum, stats
 
 
+def emitter_tables(corners, order, mats):
+    """Emissive triangles by power in mesh order: (em_tri leaf positions, em_cdf, tri_light_pdf per leaf position)."""
+    c = np.ascontiguousarray(corners, dtype=np.float32)
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    m = np.ascontiguousarray(mats, dtype=np.float32)
+    nt = len(o)
+    et, ec, tp = np.zeros(max(nt, 1), np.int32), np.zeros(nt + 1, np.float32), np.zeros(nt, np.float32)
+    n = lib().orc_emitter_tables(_f(c), _i(o), nt, _f(m), _i(et), _f(ec), _f(tp))
+    return et[:n].copy(), ec[:n + 1].copy() if n else np.zeros(0, np.float32), tp
+
+
+def env_build(rgb, strength):
+    """Environment texels (w*h, 4), marginal (h + 1) and conditional (h, w + 1) CDFs of an (h, w, 3) image."""
+    a = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = a.shape[:2]
+    tex, marg, cond = np.zeros((w * h, 4), np.float32), np.zeros(h + 1, np.float32), np.zeros((h, w + 1), np.float32)
+    lib().orc_env_build(_f(a), w, h, strength, _f(tex), _f(marg), _f(cond))
+    return tex, marg, cond
+
+
+def bsdf_eval(mat, wo, wi, eta, gd_energy=1, ux1=0.5, ux2=0.5):
+    m = np.ascontiguousarray(mat, np.float32); a = np.asarray(wo, np.float32); b = np.asarray(wi, np.float32)
+    f = np.zeros(3, np.float32); p = np.zeros(1, np.float32)
+    lib().orc_bsdf_eval(_f(m), _f(a), _f(b), eta, gd_energy, ux1, ux2, _f(f), _f(p))
+    return f, float(p[0])
+
+
+def bsdf_sample_batch(mat, wo, eta, n, seed, gd_energy=1):
+    m = np.ascontiguousarray(mat, np.float32); a = np.asarray(wo, np.float32)
+    wi, w, p, ok = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32)
+    lib().orc_bsdf_sample_batch(_f(m), _f(a), eta, gd_energy, n, seed, _f(wi), _f(w), _f(p), _i(ok))
+    return wi, w, p, ok
+
+
+def bsdf_pdf_batch(mat, wo, eta, wi, gd_energy=1):
+    m = np.ascontiguousarray(mat, np.float32); a = np.asarray(wo, np.float32)
+    d = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    p = np.zeros(len(d), np.float32)
+    lib().orc_bsdf_pdf_batch(_f(m), _f(a), eta, gd_energy, _f(d), len(d), _f(p))
+    return p
+
+
+def material(base=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.5, specular_weight=1.0,
+             specular_color=(1, 1, 1), transmission=0.0, emission=(0, 0, 0)):
+    """16-float material record (OpenPBR defaults)."""
+    return np.array(list(base) + [metallic] + list(specular_color) + [roughness] + list(emission) + [ior] +
+                    [specular_weight, transmission, 0, 0], dtype=np.float32)
+
+
 def camera_ray(params, pix, smp):
     o = np.zeros(3, dtype=np.float32); d = np.zeros(3, dtype=np.float32)
     lib().orc_camera_ray(ctypes.cast(ctypes.byref(params), _P), pix, smp, _f(o), _f(d))
@@ -349,4 +449,9 @@ def scene_from_setup(setup, tri_test=None, bins=16):
                            _i(np.ascontiguousarray(mats, dtype=np.int32)), _i(order), len(order), _f(corners))
     b["corners"] = corners
     tt = info["tri_test"] if tri_test is None else tri_test
-    return Scene(info["variant"], tt, b["nodes"], b["woop"], corners, alb), b
+    sc = Scene(info["variant"], tt, b["nodes"], b["woop"], corners, alb)
+    if info.get("full"):
+        env = setup.array(cwp.ARR_ENV_TEX) if info["env_w"] > 0 else None
+        sc.set_full(setup.array(cwp.ARR_MATERIALS), order, info["sun_on"], info["light_mode"], info["rr"], info["rr_depth"],
+                    info["gd_energy"], info["env_uniform"], env, info["env_w"], info["env_h"], info["env_rot_deg"])
+    return sc, b

@@ -1,0 +1,92 @@
+"""GPU parity of the full material integrator (OpenPBR-subset materials, emissive triangles sampled by power, the
+environment map sampled by luminance, MIS, Russian roulette) against the oracle, through the C ABI.
+
+Bars: images >= 50 dB PSNR against the oracle (sRGB, clip [0, 1]) at the same samples; ray counters within 1 %
+(a path may take another branch where a device transcendental differs from the host's in the last place); the
+closed forms the oracle is pinned on (tests/test_oracle_full.py) hold on the GPU too.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cwp
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TERRAIN = ["--gen", "terrain", "--gen_grid", "60", "--gen_boxes", "24", "--gen_materials", "5", "--width", "64",
+           "--height", "48", "--filter", "box"]
+BOXES = ["--gen", "boxes", "--gen_boxes", "96", "--gen_materials", "7", "--width", "56", "--height", "56"]
+ENV = ["--gen_env", "128,64,2,3"]
+
+
+def psnr(a, b):
+    def srgb(x):
+        x = np.clip(x, 0, 1)
+        return np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1 / 2.4) - 0.055)
+    mse = float(np.mean((srgb(a.astype(np.float64)) - srgb(b.astype(np.float64))) ** 2))
+    return 99.0 if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+CASES = [
+    ("terrain-materials-bvh8", TERRAIN + ["--variant", "bvh8"]),
+    ("terrain-env-mis-bvh8", TERRAIN + ENV + ["--variant", "bvh8"]),
+    ("terrain-env-nee", TERRAIN + ENV + ["--variant", "bvh8", "--light_sampling", "nee"]),
+    ("terrain-env-bsdf", TERRAIN + ENV + ["--variant", "bvh8", "--light_sampling", "bsdf"]),
+    ("terrain-env-rr-bvh2-watertight", TERRAIN + ENV + ["--variant", "bvh2", "--tri", "watertight", "--rr", "1", "--rr_depth", "2"]),
+    ("boxes-uniform-env-bvh4", BOXES + ["--gen_env", "64,32,1,2", "--env_sampling", "uniform", "--variant", "bvh4", "--tri", "mt"]),
+    ("terrain-plain-mix-bh-filter", TERRAIN + ENV + ["--variant", "bvh8", "--gd_energy", "0", "--filter", "bh"]),
+]
+
+
+@pytest.mark.parametrize("name,args", CASES, ids=[c[0] for c in CASES])
+def test_full_render_matches_oracle(name, args):
+    print("case", name, flush=True)
+    spp, bounces = 32, 4
+    s = cwp.Setup(args + ["--bounces", str(bounces), "--spp", str(spp), "--batch", str(spp)])
+    assert s.info["full"] == 1
+    r = cwp.Renderer(s)
+    st = r.render(spp)
+    img = r.accum()[..., :3] / spp
+    assert r.frame_json(st)["integrator"] == "full"
+    r.close()
+    sc, _ = po.scene_from_setup(s)
+    acc, ost = sc.render(s.params(), spp, batch=spp, bounces=bounces)
+    ref = acc[..., :3] / spp
+    assert np.isfinite(img).all()
+    assert psnr(img, ref) >= 50.0, (name, psnr(img, ref))
+    assert st["rays_closest"] == pytest.approx(ost[0], rel=1e-2)
+    assert st["rays_shadow"] == pytest.approx(ost[1], rel=1e-2)
+    assert st["shadow_unoccluded"] == pytest.approx(ost[3], rel=1e-2)
+
+
+def test_full_kernel_is_deterministic_and_progressive():
+    """the same image twice, and 2 x 8 samples accumulated equal 16 samples in two passes of 8"""
+    s = cwp.Setup(TERRAIN + ENV + ["--bounces", "4", "--spp", "16", "--batch", "8"])
+    r = cwp.Renderer(s)
+    r.render(16)
+    a = r.accum()
+    r.render(16)
+    np.testing.assert_array_equal(r.accum(), a)
+    r.render(8, sample_base=0)
+    r.render(8, sample_base=8, clear=False)
+    np.testing.assert_array_equal(r.accum(), a)
+
+
+def test_full_plane_closed_form_on_gpu(tmp_path):
+    """a Lambert plane of albedo a under a constant sky L, no sun: a sample that hits the plane is a * L (to float
+    rounding), one that misses is L, so each pixel is L (1 - f (1 - a)) with f = hits / 8"""
+    obj = tmp_path / "plane.obj"
+    obj.write_text("v -1000 -1000 0\nv 1000 -1000 0\nv 1000 1000 0\nv -1000 1000 0\nf 1 2 3\nf 1 3 4\n")
+    a, sky = 0.6, (0.4, 0.5, 0.6)
+    s = cwp.Setup(["--mesh", str(obj), "--c2w", "1,0,0,0,0,0.7071068,-0.7071068,-2,0,0.7071068,0.7071068,1,0,0,0,1",
+                   "--width", "48", "--height", "48", "--albedo", str(a), str(a), str(a), "--sky", *map(str, sky),
+                   "--sun_rgb", "0", "0", "0", "--filter", "box", "--batch", "8", "--spp", "8", "--integrator", "full",
+                   "--bounces", "0"])
+    r = cwp.Renderer(s)
+    r.render(8)
+    img = r.accum()[..., :3] / 8
+    f = (1.0 - img / np.array(sky, np.float32)) / (1.0 - a)
+    np.testing.assert_allclose(f * 8, np.round(f * 8), atol=2e-3)
+    assert np.isclose(f[..., 0], 1.0, atol=1e-4).sum() > 0.3 * 48 * 48
