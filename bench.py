@@ -128,16 +128,23 @@ JOB_RES = (128, 256, 512, 1024)
 JOB_SPP = (4, 16, 64)
 
 
-def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000):
+def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000, full=False):
     """Seeded scenes with triangle counts log-uniform in [lo_tris, hi_tris]: fBm heightfields of (G-1)^2 * 2
-    triangles, each with its own terrain seed. Returns the setup arguments of each scene."""
+    triangles, each with its own terrain seed. With `full`, 64 boxes on each terrain carry OpenPBR-subset materials
+    (conductors, rough dielectrics, glossy-diffuse, emitters) and each scene has a 2048 x 1024 log-normal (sigma 2)
+    environment map. Returns the setup arguments of each scene."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_scenes):
         t = float(np.exp(rng.uniform(np.log(lo_tris), np.log(hi_tris))))
         g = int(round(np.sqrt(t / 2.0))) + 1
-        out.append(["--gen", "terrain", "--gen_grid", str(g), "--gen_boxes", "0", "--gen_seed", str(1 + k),
-                    "--width", "512", "--height", "512", "--bounces", "6", "--filter", "box"] + LIGHTING)
+        a = ["--gen", "terrain", "--gen_grid", str(g), "--gen_seed", str(1 + k), "--width", "512", "--height", "512",
+             "--bounces", "6", "--filter", "box"] + LIGHTING
+        if full:
+            a += ["--gen_boxes", "64", "--gen_materials", str(1 + k), "--gen_env", "2048,1024,2,%d" % (1 + k)]
+        else:
+            a += ["--gen_boxes", "0"]
+        out.append(a)
     return out
 
 
@@ -183,14 +190,14 @@ def bench_jobs(args):
     the batched mode runs every admitted job's paths in one trace+shade and one shadow launch per bounce, from a
     captured graph, under the configuration the autotuner picked. Reports both rays/s, their ratio and job latency."""
     import cwp
-    scene_args = job_stream_scenes(args.job_scenes, args.job_seed)
+    scene_args = job_stream_scenes(args.job_scenes, args.job_seed, full=bool(args.full))
     t0 = time.perf_counter()
     setups = make_setups(scene_args)
     tris = [s.info["num_tris"] for s in setups]
     print("bench: %d scenes (%s tris) set up in %.1f s" % (len(setups), ",".join(str(t) for t in tris),
                                                          time.perf_counter() - t0), file=sys.stderr, flush=True)
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6)
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full))
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)
@@ -228,9 +235,9 @@ def bench_serve(args):
     """Serving: the same job stream arriving as a Poisson process of rate lambda (jobs/s), continuous batching
     (batched) against one job at a time (plain); rays/s and p50/p99 latency (finish - arrival) per rate."""
     import cwp
-    setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed))
+    setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed, full=bool(args.full)))
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6)
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full))
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)
@@ -286,6 +293,8 @@ def main():
     ap.add_argument("--job-res", default="128,256,512,1024", help="job stream: square resolutions (uniform)")
     ap.add_argument("--job-spp", default="4,16,64", help="job stream: samples per pixel (uniform)")
     ap.add_argument("--no-jobs", action="store_true", help="terrain workload without the job stream")
+    ap.add_argument("--full", type=int, default=0, help="job stream / serving: the full material integrator "
+                    "(OpenPBR-subset materials, emitters, 2048 x 1024 log-normal environment, MIS)")
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -309,7 +318,9 @@ def main():
         out = {"metric": "Grays/s of the mixed job stream, batched vs plain; p50/p99 job latency", "value": j["batched_grays"],
                "unit": "Grays/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(j["batched_wall_s"] * 1e3, 3),
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-               "config": {"workload": "job-stream-%d-jobs-%d-scenes-depth6" % (args.jobs, args.job_scenes)}, "jobs": j}
+               "config": {"workload": "job-stream-%d-jobs-%d-scenes-depth6%s" % (args.jobs, args.job_scenes,
+                                                                               "-openpbr-env-mis" if args.full else "")},
+               "jobs": j}
         if args.workload == "serve":
             out["serve"] = bench_serve(args)
         print(json.dumps(out), flush=True)

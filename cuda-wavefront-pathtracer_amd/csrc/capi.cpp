@@ -134,6 +134,14 @@ int cwp_create(const cwp_setup* s, cwp_ctx** out) {
         frameInstances(s->Z, s->Z.A.frameStart, rec);
         double ms = 0;
         if (!c->R.setInstances(rec.data(), (int)rec.size(), ms, err)) { delete c; return fail(CWP_ERR_DEVICE, err); }
+        if (s->Z.full) {
+            std::vector<Vec4> em;
+            std::vector<float> cdf, mlp;
+            frameEmitters(s->Z, s->Z.A.frameStart, em, cdf, mlp);
+            if (!c->R.setEmitters(em.data(), (int)(em.size() / 3), cdf.data(), mlp.data(), (int)mlp.size(), err)) {
+                delete c; return fail(CWP_ERR_DEVICE, err);
+            }
+        }
     }
     *out = c;
     return CWP_OK;
@@ -161,7 +169,36 @@ int cwp_set_frame(cwp_ctx* c, int frame, double* tlas_ms) {
     if (!c || !c->setup || frame < 0) return fail(CWP_ERR_ARG, "cwp_set_frame: needs a context made from a setup");
     std::vector<InstanceRecord> rec;
     if (!frameInstances(c->setup->Z, frame, rec)) return fail(CWP_ERR_STATE, "cwp_set_frame: the scene is not two-level");
-    return cwp_set_instances(c, (const cwp_instance*)rec.data(), (int)rec.size(), tlas_ms);
+    const int rc = cwp_set_instances(c, (const cwp_instance*)rec.data(), (int)rec.size(), tlas_ms);
+    if (rc != CWP_OK || !c->setup->Z.full) return rc;
+    std::vector<Vec4> em;
+    std::vector<float> cdf, mlp;
+    frameEmitters(c->setup->Z, frame, em, cdf, mlp);
+    return cwp_set_emitters(c, (const float*)em.data(), (int)(em.size() / 3), cdf.data(), mlp.data(), (int)mlp.size());
+}
+
+int cwp_set_emitters(cwp_ctx* c, const float* em_tris, int num_emitters, const float* em_cdf, const float* mat_light_pdf,
+                     int num_materials) {
+    if (!c) return fail(CWP_ERR_ARG, "cwp_set_emitters: null context");
+    std::string err;
+    if (!c->R.setEmitters((const Vec4*)em_tris, num_emitters, em_cdf, mat_light_pdf, num_materials, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_setup_emitters(const cwp_setup* s, int frame, float* em_tris, int cap, int* num_emitters, float* em_cdf,
+                       float* mat_light_pdf, int mat_cap) {
+    if (!s || !num_emitters || frame < 0) return fail(CWP_ERR_ARG, "cwp_setup_emitters: bad argument");
+    std::vector<Vec4> em;
+    std::vector<float> cdf, mlp;
+    frameEmitters(s->Z, frame, em, cdf, mlp);
+    *num_emitters = (int)(em.size() / 3);
+    if (!em_tris) return CWP_OK;   // count only
+    if (cap < *num_emitters || !em_cdf || (mat_light_pdf && mat_cap < (int)mlp.size()))
+        return fail(CWP_ERR_ARG, "cwp_setup_emitters: buffers too small");
+    memcpy(em_tris, em.data(), em.size() * sizeof(Vec4));
+    memcpy(em_cdf, cdf.data(), cdf.size() * sizeof(float));
+    if (mat_light_pdf) memcpy(mat_light_pdf, mlp.data(), mlp.size() * sizeof(float));
+    return CWP_OK;
 }
 
 int cwp_trace_host_inst(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits, int* inst) {
@@ -233,7 +270,7 @@ void cwp_options_default(cwp_options* o) {
     const RenderOptions d;
     o->tri_test = d.tri; o->batch = d.batch; o->bounces = d.bounces; o->fuse = d.fuse; o->persistent = d.persistent;
     o->shadow_refill = d.shadowRefill; o->stack_lds = 8; o->blocks_per_cu = d.blocksPerCU; o->bounce_waves = d.bounceWaves;
-    o->time_kernels = d.timeKernels;
+    o->time_kernels = d.timeKernels; o->graph = d.graph;
 }
 
 static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& err) {
@@ -279,7 +316,7 @@ int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* p, const cwp_opt
     RenderOptions O;
     O.variant = d->variant; O.tri = o->tri_test; O.batch = o->batch; O.bounces = o->bounces; O.fuse = o->fuse;
     O.persistent = o->persistent; O.shadowRefill = o->shadow_refill; O.stackLds = o->stack_lds;
-    O.blocksPerCU = o->blocks_per_cu; O.bounceWaves = o->bounce_waves; O.timeKernels = o->time_kernels;
+    O.blocksPerCU = o->blocks_per_cu; O.bounceWaves = o->bounce_waves; O.timeKernels = o->time_kernels; O.graph = o->graph;
     if (O.batch < 1) return fail(CWP_ERR_ARG, "batch must be >= 1");
     if (O.fuse != 0 && O.fuse != 1) return fail(CWP_ERR_ARG, "fuse must be 0 or 1");
     if (O.persistent && O.fuse) return fail(CWP_ERR_ARG, "persistent needs fuse 0");
@@ -468,6 +505,7 @@ static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
     o->variant = d.batch.variant; o->tri_test = d.batch.tri; o->stack_lds = d.batch.stackLds; o->bounces = d.batch.bounces;
     o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;
     o->batched = d.batched ? 1 : 0; o->rays_per_lane = d.batch.raysPerLane; o->use_graph = d.batch.useGraph;
+    o->full = d.batch.full;
 }
 
 void cwp_server_options_default(cwp_server_options* o) {
@@ -487,6 +525,7 @@ int cwp_server_create(const cwp_server_options* o, cwp_server** out) {
     so.batch.variant = o->variant; so.batch.tri = o->tri_test; so.batch.stackLds = o->stack_lds; so.batch.bounces = o->bounces;
     so.batch.maxPaths = o->max_paths; so.batch.maxJobs = o->max_jobs; so.sceneBudgetMB = o->scene_budget_mb;
     so.batched = o->batched != 0; so.batch.raysPerLane = o->rays_per_lane; so.batch.useGraph = o->use_graph;
+    so.batch.full = o->full;
     cwp_server* s = new (std::nothrow) cwp_server();
     if (!s) return fail(CWP_ERR_ARG, "out of host memory");
     std::string err;

@@ -25,6 +25,7 @@ struct BatchOptions {
     int raysPerLane = 1;         // 1, 2 or 4: the persistent grid is 1/raysPerLane of the resident waves, so each lane
                                  // traces that many times more rays in sequence
     int useGraph = 1;            // replay each pass's launches from a captured HIP graph
+    int full = 0;                // 1: the full material integrator (every scene full); 0: the legacy integrator
 };
 
 struct BatchJob {

@@ -4,6 +4,7 @@
 #include "host/scene_check.h"
 #include <algorithm>
 #include <cstddef>
+#include <type_traits>
 
 namespace cwp {
 
@@ -37,7 +38,7 @@ struct BatchRenderer::Impl {
     DevPaths P{};
     std::vector<void*> allocs;
     Kernels kn;
-    int numCU = 0, grid = 0, redGrid = 0, depths = 0, sl = 8, spillCap = 0;
+    int numCU = 0, grid = 0, gridFull = 0, redGrid = 0, depths = 0, sl = 8, spillCap = 0;
     unsigned long long *tot = nullptr, *hostTot = nullptr;
     DevJobTable* dTable[kTableSlots] = {};
     DevJobTable* hTable[kTableSlots] = {};
@@ -68,7 +69,7 @@ struct BatchRenderer::Impl {
         if (cap <= spillCap) return true;
         HIPCHK(hipStreamSynchronize(st));
         if (P.spill) { hipFree(P.spill); P.spill = nullptr; }
-        HIPCHK(hipMalloc(&P.spill, (size_t)grid * kBlock * cap * sizeof(uint64_t)));
+        HIPCHK(hipMalloc(&P.spill, (size_t)std::max(grid, gridFull) * kBlock * cap * sizeof(uint64_t)));
         spillCap = cap; P.spillCap = cap;
         pathGen++;   // captured graphs hold the old spill pointer
         return true;
@@ -77,7 +78,7 @@ struct BatchRenderer::Impl {
     void enqueue(int s) {
         k_pass_begin_jobs<<<1, 64, 0, st>>>(P.cnt, (depths + 1) * kCntStride, dTable[s]);
         for (int depth = 0; depth < depths; depth++) {
-            kn.bounceJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+            (O.full ? kn.bounceJobsFull : kn.bounceJobs)<<<O.full ? gridFull : grid, kBlock, 0, st>>>(dTable[s], P, depth);
             kn.shadowJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
         }
         k_reduce_jobs<<<redGrid, kWave, 0, st>>>(dTable[s], P.L);
@@ -144,6 +145,9 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     int b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)I.kn.bounceJobs, kBlock, 0) != hipSuccess || b < 1) b = 1;
     I.grid = std::max(kParts, (b * I.numCU / O.raysPerLane) / kParts * kParts);
+    int bf = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bf, (const void*)I.kn.bounceJobsFull, kBlock, 0) != hipSuccess || bf < 1) bf = 1;
+    I.gridFull = std::max(kParts, (bf * I.numCU / O.raysPerLane) / kParts * kParts);
     gridBlocks = I.grid;
     I.redGrid = (int)std::min<size_t>(16384, O.maxPaths / kWave + (size_t)O.maxJobs);
     I.depths = O.bounces + 2;
@@ -172,6 +176,7 @@ int BatchRenderer::addScene(const SceneUpload& S, std::string& err) {
     if (!checkScene(S, need, err)) return -1;
     if (S.variant != I.O.variant) { err = "batch: the scene's BVH layout differs from the batch's"; return -1; }
     if (S.inst) { err = "batch: two-level scenes are not batched"; return -1; }
+    if ((S.full != 0) != (I.O.full != 0)) { err = "batch: the scene's integrator differs from the batch's"; return -1; }
     if (I.O.tri == kTriWoop && S.woop.size() != S.verts.size()) { err = "batch: Woop rows missing"; return -1; }
     SceneDev D;
     auto up = [&](const std::vector<Vec4>& v, const Vec4*& dst) {
@@ -193,7 +198,17 @@ int BatchRenderer::addScene(const SceneUpload& S, std::string& err) {
     static const std::vector<Vec4> none;
     const bool ok = up(nodes, dn) && up(I.O.tri == kTriWoop ? S.woop : none, dw) && up(S.verts, dv) && up(S.normals, dnr) &&
                     up(S.albedo, da);
-    if (!ok) { for (void* b : D.bufs) hipFree(b); return -1; }
+    auto upAny = [&](auto& p, const auto& v, std::string& e) -> bool {
+        if (v.empty()) { p = nullptr; return true; }
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        void* q = nullptr;
+        if (hipMalloc(&q, v.size() * sizeof(T)) != hipSuccess) { e = "batch: out of device memory"; return false; }
+        D.bufs.push_back(q); D.bytes += v.size() * sizeof(T);
+        if (hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) { e = "batch: upload failed"; return false; }
+        p = (T*)q;
+        return true;
+    };
+    if (!ok || !makeDevLights(S, upAny, D.S.Lt, err)) { for (void* b : D.bufs) hipFree(b); return -1; }
     D.S.nodes = dn; D.S.verts = dv; D.S.nrm = dnr; D.S.albedo = da; D.S.root = S.root;
     D.S.tri = I.O.tri == kTriWoop ? dw : dv;
     D.S.tlas = nullptr; D.S.tlasRoot = 0; D.S.inst = nullptr;
@@ -215,7 +230,12 @@ void BatchRenderer::removeScene(int id) {
 
 size_t BatchRenderer::uploadBytes(const SceneUpload& S, int tri) {
     const size_t nodes = S.variant == 8 ? S.nodes.size() / 5 * kBvh8Stride : S.nodes.size();
-    return (nodes + (tri == kTriWoop ? S.woop.size() : 0) + S.verts.size() + S.normals.size() + S.albedo.size()) * sizeof(Vec4);
+    size_t b = (nodes + (tri == kTriWoop ? S.woop.size() : 0) + S.verts.size() + S.normals.size() + S.albedo.size()) * sizeof(Vec4);
+    if (S.full) {
+        b += S.mats.size() * sizeof(Material) + (S.triLightPdf.size() + S.emTri.size() + S.emCdf.size()) * 4;
+        if (S.env) b += S.env->texels.size() * sizeof(Vec4) + (S.env->marg.size() + S.env->cond.size()) * 4;
+    }
+    return b;
 }
 
 size_t BatchRenderer::sceneBytes(int id) const {

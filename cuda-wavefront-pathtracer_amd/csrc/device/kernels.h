@@ -56,6 +56,10 @@ struct DevLights {
     float envRot;                // radians about +z
     float pSun, pTri, pEnv;      // light-type selection probabilities of next-event estimation
     int lightMode, rr, rrDepth, gdEnergy, envUniform, hasEnv;
+    // two-level scenes: this frame's world-space emitters (3 float4 each: (v0, leaf bits), (e1, material bits),
+    // (e2, 0)) and per material the light pdf per unit area (emTri / triLightPdf are unused)
+    const Vec4* emTris;
+    const float* matLightPdf;
 };
 
 // Fills L from the full-integrator data of S, uploading each array with up(pointer, vector) (host side). The sun is
@@ -66,10 +70,17 @@ template <class Up>
 bool makeDevLights(const SceneUpload& S, Up&& up, DevLights& L, std::string& err) {
     L = DevLights{};
     if (!S.full) return true;
-    if (!up(L.mats, S.mats, err) || !up(L.triLightPdf, S.triLightPdf, err) || !up(L.emTri, S.emTri, err) ||
-        !up(L.emCdf, S.emCdf, err))
-        return false;
-    L.nEm = (int)S.emTri.size();
+    if (!up(L.mats, S.mats, err)) return false;
+    if (S.inst) {   // per-frame emitters: buffers of this scene's size, filled for each frame
+        L.nEm = S.numEmInst;
+        if (L.nEm > 0 && (!up(L.emTris, std::vector<Vec4>(3 * (size_t)L.nEm), err) ||
+                          !up(L.emCdf, std::vector<float>((size_t)L.nEm + 1), err) ||
+                          !up(L.matLightPdf, std::vector<float>(S.mats.size()), err)))
+            return false;
+    } else {
+        if (!up(L.triLightPdf, S.triLightPdf, err) || !up(L.emTri, S.emTri, err) || !up(L.emCdf, S.emCdf, err)) return false;
+        L.nEm = (int)S.emTri.size();
+    }
     L.hasEnv = S.env && S.env->w > 0 ? 1 : 0;
     if (L.hasEnv) {
         if (!up(L.envTex, S.env->texels, err) || !up(L.envMarg, S.env->marg, err) || !up(L.envCond, S.env->cond, err)) return false;
@@ -384,7 +395,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
             if constexpr (FULL) {
                 const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
-                res = shadeFull(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
+                res = shadeFull<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
             }
@@ -629,7 +640,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
             const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
             if constexpr (FULL) {
                 const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
-                res = shadeFull(J.S, P, J.F, depth, i, pix, smp, o, d, T, lastPdf, h);
+                res = shadeFull<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
             }
@@ -697,7 +708,7 @@ struct Kernels {
     // batched jobs
     JobsFn bounceJobs = nullptr, shadowJobs = nullptr;
     // full material integrator (fused bounce kernel; shadow rays use the legacy shadow kernels)
-    BounceFn bounceFull = nullptr;
+    BounceFn bounceFull = nullptr, bounceFullInst = nullptr;
     JobsFn bounceJobsFull = nullptr;
 };
 
@@ -717,6 +728,7 @@ void pickSL(int waves, Kernels& k) {
     k.bounceJobs = k_bounce_jobs<V, TRI, SL, 8>;
     k.shadowJobs = k_shadow_jobs<V, TRI, SL>;
     k.bounceFull = k_bounce<V, TRI, SL, kFullWaves, false, true>;
+    k.bounceFullInst = k_bounce<V, TRI, SL, kFullWaves, true, true>;
     k.bounceJobsFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true>;
 }
 template <int V, int TRI>

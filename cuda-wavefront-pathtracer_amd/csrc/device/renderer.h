@@ -31,6 +31,7 @@ struct RenderOptions {
     int fuse = 1;           // 1: trace and shade in one kernel; 0: a closest-hit trace kernel, then a shade kernel
     int persistent = 0;     // 1: split kernels whose waves refill idle lanes with new rays mid-traversal (implies fuse 0)
     int shadowRefill = 0;   // shadow kernel: 1 = refill idle lanes mid-traversal, 0 = one ray per lane per round
+    int graph = 0;          // 1: a frame is captured once into a HIP graph and replayed (per-kernel timing off)
 };
 
 struct FrameStats {
@@ -70,6 +71,8 @@ struct SceneUpload {
     int rr = 0, rrDepth = 3;      // Russian roulette from this depth on
     int gdEnergy = 1;             // glossy-diffuse energy compensation
     int envUniform = 0;           // environment light samples uniform on the sphere (instead of by luminance)
+    int numEmInst = 0;            // two-level scenes: emissive triangles over all instances (set per frame)
+    std::vector<int> meshTris;    // two-level scenes: first leaf position and triangle count per mesh
 };
 
 class Renderer {
@@ -92,6 +95,9 @@ public:
     // Two-level scenes: the instances of the next frames (records validated against the meshes' BLAS roots); builds
     // the TLAS on the GPU (H-PLOC over the instance boxes) and reports its device time.
     bool setInstances(const InstanceRecord* rec, int n, double& tlasMs, std::string& err);
+    // Two-level full-integrator scenes: the frame's world-space emitters (3 float4 each), their power CDF (nEm + 1)
+    // and the per-material light pdf per unit area; nEm must be the scene's instanced emitter count.
+    bool setEmitters(const Vec4* emTris, int nEm, const float* emCdf, const float* matLightPdf, int nMats, std::string& err);
 
     double vramMB = 0, tlasMs = 0;   // tlasMs: device time of the last TLAS build
     int gridBlocks = 0, numCU = 0;

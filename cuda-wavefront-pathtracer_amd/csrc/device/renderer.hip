@@ -78,6 +78,14 @@ struct Renderer::Impl {
     RenderOptions O;
     FrameParams F;
     FrameParams* dF = nullptr;    // device copy read by the bounce / shade kernels
+    // captured frame (graph mode): the launch parameters baked into it and the scene state it was captured with
+    hipGraphExec_t graph = nullptr;
+    // full integrator on two-level scenes: the frame's emitters must be set with the instances
+    bool full = false;
+    int numEmInst = 0, numMats = 0;
+    bool emSet = false;
+    int gSpp = 0, gBase = 0, sceneGen = 0, gGen = -1;
+    bool gClear = false;
     int N = 0, K = 1, NK = 0, depths = 0, sl = 16;
     hipStream_t st = nullptr;
     hipEvent_t f0 = nullptr, f1 = nullptr;
@@ -134,6 +142,7 @@ Renderer::Renderer() : p_(new Impl) {}
 Renderer::~Renderer() {
     if (!p_) return;
     if (p_->st) hipStreamSynchronize(p_->st);
+    if (p_->graph) hipGraphExecDestroy(p_->graph);
     for (void* a : p_->allocs) hipFree(a);
     if (p_->stageRays) hipFree(p_->stageRays);
     if (p_->stageHits) hipFree(p_->stageHits);
@@ -226,16 +235,19 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     else if (S.variant == 4) pickKernels<4>(O.tri, I.sl, O.bounceWaves, I.kn);
     else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
     I.inst = S.inst != 0;
-    if (S.full && (I.inst || !O.fuse || O.persistent)) {
-        err = "the full integrator runs single-level scenes with the fused kernel (fuse 1, persistent 0)"; return false;
+    if (S.full && (!O.fuse || O.persistent)) {
+        err = "the full integrator runs the fused kernel (fuse 1, persistent 0)"; return false;
     }
-    if (S.full) {
+    I.full = S.full != 0;
+    I.numEmInst = S.inst ? S.numEmInst : 0;
+    I.numMats = (int)S.mats.size();
+    if (S.full && !I.inst) {
         I.bounce = I.kn.bounceFull; I.shade = nullptr;
         I.shadow = O.shadowRefill ? I.kn.shadowRefill : I.kn.shadow;
     } else if (I.inst) {
         // two-level scenes run the fused bounce kernel and the one-ray-per-lane shadow kernel
         if (!O.fuse || O.persistent) { err = "two-level scenes need the fused kernel (fuse 1, persistent 0)"; return false; }
-        I.bounce = I.kn.bounceInst; I.shade = nullptr; I.shadow = I.kn.shadowInst;
+        I.bounce = S.full ? I.kn.bounceFullInst : I.kn.bounceInst; I.shade = nullptr; I.shadow = I.kn.shadowInst;
         I.kn.traceC = I.kn.traceCInst; I.kn.traceA = I.kn.traceAInst;
         I.meshRoot = S.meshRoot; I.nMesh = (int)S.meshRoot.size();
         Vec4* mb;
@@ -308,36 +320,63 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     Impl& I = *p_;
     if (spp <= 0 || spp % I.K) { err = "spp must be a positive multiple of batch"; return false; }
     if (I.inst && !I.instSet) { err = "two-level scene: set the instances of a frame first"; return false; }
+    if (I.inst && I.full && I.numEmInst > 0 && !I.emSet) { err = "two-level scene: set the emitters of a frame first"; return false; }
     if (sBase < 0) { err = "sample base must be >= 0"; return false; }
     S = FrameStats();
     I.evUsed = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv;
-    if (clear && !I.fill(I.accum, (size_t)I.N * 16, err)) return false;
-    if (!I.fill(I.tot, 8 * sizeof(unsigned long long), err)) return false;
-    HIPCHK(hipEventRecord(I.f0, I.st));
-    const int words = (I.depths + 1) * kCntStride;
-    for (int s = sBase; s < sBase + spp; s += I.K) {
-        k_pass_begin<<<1, 64, 0, I.st>>>(I.P.cnt, words, I.NK);
-        for (int depth = 0; depth < I.depths; depth++) {
-            hipEvent_t a = nullptr, b = nullptr;
-            if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-            I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
-            if (I.shade) {
-                if (I.O.timeKernels) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
-                I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
+    const bool timed = I.O.timeKernels && !I.O.graph;
+    // the frame's launches: clears (by kernels, never memset nodes), then per pass the counter reset, trace+shade
+    // and shadow per bounce, the reduction and the totals
+    auto enqueue = [&]() -> bool {
+        if (clear && !I.fill(I.accum, (size_t)I.N * 16, err)) return false;
+        if (!I.fill(I.tot, 8 * sizeof(unsigned long long), err)) return false;
+        const int words = (I.depths + 1) * kCntStride;
+        for (int s = sBase; s < sBase + spp; s += I.K) {
+            k_pass_begin<<<1, 64, 0, I.st>>>(I.P.cnt, words, I.NK);
+            for (int depth = 0; depth < I.depths; depth++) {
+                hipEvent_t a = nullptr, b = nullptr;
+                if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
+                I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
+                if (I.shade) {
+                    if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
+                    I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
+                }
+                if (timed) {
+                    HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });
+                    if (I.shade) shadeEv.back().second = b;
+                }
+                if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
+                I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
+                if (timed) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
             }
-            if (I.O.timeKernels) {
-                HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });
-                if (I.shade) shadeEv.back().second = b;
-            }
-            if (I.O.timeKernels) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-            I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
-            if (I.O.timeKernels) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
+            k_reduce<<<std::min((I.N + kWave - 1) / kWave, 16384), kWave, 0, I.st>>>(I.P.L, I.accum, I.N, I.K);
+            k_pass_end<<<1, 64, 0, I.st>>>(I.P.cnt, I.depths, I.tot);
         }
-        k_reduce<<<std::min((I.N + kWave - 1) / kWave, 16384), kWave, 0, I.st>>>(I.P.L, I.accum, I.N, I.K);
-        k_pass_end<<<1, 64, 0, I.st>>>(I.P.cnt, I.depths, I.tot);
+        HIPCHK(hipGetLastError());
+        return true;
+    };
+    if (I.O.graph) {
+        // one captured graph per (spp, sample base, clear) and scene state; re-captured when any of them changes
+        if (!I.graph || I.gSpp != spp || I.gBase != sBase || I.gClear != clear || I.gGen != I.sceneGen) {
+            if (I.graph) { HIPCHK(hipGraphExecDestroy(I.graph)); I.graph = nullptr; }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(I.st, hipStreamCaptureModeThreadLocal));
+            const bool ok = enqueue();
+            const hipError_t ec = hipStreamEndCapture(I.st, &g);
+            if (!ok) { if (g) hipGraphDestroy(g); return false; }
+            HIPCHK(ec);
+            const hipError_t ei = hipGraphInstantiate(&I.graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ei != hipSuccess) { I.graph = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(ei); return false; }
+            I.gSpp = spp; I.gBase = sBase; I.gClear = clear; I.gGen = I.sceneGen;
+        }
+        HIPCHK(hipEventRecord(I.f0, I.st));
+        HIPCHK(hipGraphLaunch(I.graph, I.st));
+    } else {
+        HIPCHK(hipEventRecord(I.f0, I.st));
+        if (!enqueue()) return false;
     }
-    HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(I.f1, I.st));
     HIPCHK(hipMemcpyAsync(I.hostTot, I.tot, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, I.st));
     HIPCHK(hipEventSynchronize(I.f1));
@@ -395,6 +434,29 @@ bool Renderer::traceHost(const float* rays, int n, bool anyHit, float* hits, std
     return true;
 }
 
+bool Renderer::setEmitters(const Vec4* emTris, int nEm, const float* emCdf, const float* matLightPdf, int nMats,
+                           std::string& err) {
+    Impl& I = *p_;
+    if (!I.inst || !I.full) { err = "setEmitters: the scene is not a two-level full-integrator scene"; return false; }
+    if (nEm != I.numEmInst || nMats != I.numMats) { err = "setEmitters: the scene has " + std::to_string(I.numEmInst) +
+        " instanced emitters and " + std::to_string(I.numMats) + " materials"; return false; }
+    if (nEm == 0) { I.emSet = true; return true; }
+    if (!emTris || !emCdf || !matLightPdf) { err = "setEmitters: null array"; return false; }
+    // the kernels trust the material ids and the CDF
+    if (emCdf[0] != 0.f || emCdf[nEm] != 1.f) { err = "setEmitters: the CDF must run from 0 to 1"; return false; }
+    for (int e = 0; e < nEm; e++) {
+        const int m = asInt(emTris[3 * e + 1].w);
+        if (m < 0 || m >= nMats) { err = "setEmitters: emitter " + std::to_string(e) + " has a material id out of range"; return false; }
+        if (emCdf[e + 1] < emCdf[e]) { err = "setEmitters: the CDF decreases"; return false; }
+    }
+    HIPCHK(hipStreamSynchronize(I.st));   // queued frames read the previous frame's emitters
+    HIPCHK(hipMemcpy((void*)I.S.Lt.emTris, emTris, (size_t)nEm * 3 * sizeof(Vec4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void*)I.S.Lt.emCdf, emCdf, ((size_t)nEm + 1) * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void*)I.S.Lt.matLightPdf, matLightPdf, (size_t)nMats * sizeof(float), hipMemcpyHostToDevice));
+    I.emSet = true;
+    return true;
+}
+
 bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, std::string& err) {
     Impl& I = *p_;
     if (!I.inst) { err = "setInstances: the scene is not two-level"; return false; }
@@ -440,6 +502,7 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
         return false;
     }
     I.S.tlas = I.dTlas; I.S.tlasRoot = hplocBVH2Root(n); I.S.inst = I.dInst;
+    I.sceneGen++;   // captured frames hold the scene record by value
     I.nInst = n; I.instSet = true;
     return true;
 }

@@ -69,7 +69,10 @@ __device__ __forceinline__ void addRadiance(const DevPaths& P, int i, int depth,
 }
 
 // Shades the closest hit (or miss) of path i; lastPdf = pdf of the BSDF sample that made the ray (0: camera ray or a
-// delta lobe, whose emission hits count fully).
+// delta lobe, whose emission hits count fully). INST: the hit lies in instance h.inst: the triangle is in object
+// space, normals go to world space through inv^T and keep the geometric winding by sign(det); emitters are the
+// frame's world-space list and an emitter's area pdf is its material's.
+template <bool INST>
 __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
                                               int i, int pix, uint32_t smp, Vec3 o, Vec3 d, Vec3 T, float lastPdf,
                                               const Hit& h) {
@@ -96,11 +99,18 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     const Vec3 p = o + d * h.t;
     const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
     const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
-    const Vec3 ng0 = normalize(cross(e1, e2));
+    Vec4 xr[3];
+    float dsg = 1.f;
+    if constexpr (INST) {
+        const InstanceRecord* I = S.inst + h.inst;
+        xr[0] = I->inv[0]; xr[1] = I->inv[1]; xr[2] = I->inv[2]; dsg = I->detSign;
+    }
+    const Vec3 ng0 = INST ? normalize(xfRowsN(xr, cross(e1, e2))) * dsg : normalize(cross(e1, e2));
     Vec3 ns = ng0;
     if (F.hasN) {
         const Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
-        const Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
+        Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
+        if constexpr (INST) nn = xfRowsN(xr, nn);
         const float l = length(nn);
         if (l > 0.f) ns = nn * (1.f / l);
         if (dot(ns, ng0) < 0.f) ns = -ns;
@@ -118,7 +128,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
             if (E.lightMode == kLightNEE) w = 0.f;
             else if (E.lightMode == kLightMIS) {
                 const float cl = fabsf(dot(ng0, d));
-                const float pa = E.triLightPdf[h.tri];
+                const float pa = INST ? E.matLightPdf[mid] : E.triLightPdf[h.tri];
                 const float pl = cl > 0.f ? E.pTri * pa * h.t * h.t / cl : 0.f;
                 w = misPower(lastPdf, pl);
             }
@@ -153,9 +163,18 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
             l = uniformCone(F.sunTo, F.sunCos, u2, u3); Le = F.sunE; pl = E.pSun; isSun = true; ok = true;
         } else if (us < E.pSun + E.pTri) {
             const float ut = fminf((us - E.pSun) / E.pTri, kOneMinusEps);
-            const int kt = E.emTri[cdfFind(E.emCdf, E.nEm, ut)];
-            const Vec4 a0 = S.verts[3 * kt], a1 = S.verts[3 * kt + 1], a2 = S.verts[3 * kt + 2];
-            const Vec3 f1 = xyz(a1) - xyz(a0), f2 = xyz(a2) - xyz(a0);
+            const int em = cdfFind(E.emCdf, E.nEm, ut);
+            Vec4 a0, a1;
+            Vec3 f1, f2;
+            int kt;
+            if constexpr (INST) {
+                a0 = E.emTris[3 * em]; a1 = E.emTris[3 * em + 1];
+                f1 = xyz(a1); f2 = xyz(E.emTris[3 * em + 2]); kt = asInt(a0.w);
+            } else {
+                kt = E.emTri[em];
+                a0 = S.verts[3 * kt]; a1 = S.verts[3 * kt + 1];
+                f1 = xyz(a1) - xyz(a0); f2 = xyz(S.verts[3 * kt + 2]) - xyz(a0);
+            }
             const float su = sqrtf(u2);
             const Vec3 pL = xyz(a0) + f1 * (su * (1.f - u3)) + f2 * (su * u3);
             const Vec3 nL = normalize(cross(f1, f2));
@@ -168,7 +187,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
                 const float cl = fabsf(dot(nL, l));
                 if (cl > 0.f) {
                     const int lm = asInt(a1.w);
-                    pl = E.pTri * E.triLightPdf[kt] * dist2 / cl;
+                    pl = E.pTri * (INST ? E.matLightPdf[lm] : E.triLightPdf[kt]) * dist2 / cl;
                     Le = loadMaterial(E.mats, lm).emission;
                     tmax = dist * (1.f - 1e-3f);
                     ok = true;

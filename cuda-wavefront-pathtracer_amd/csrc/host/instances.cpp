@@ -58,6 +58,8 @@ bool loadSceneFile(const std::string& path, SceneDesc& S, std::string& err) {
     }
     for (const Json& m : ms->arr) {
         MeshDesc d; d.file = rel(m.string("file", "")); d.yup = (int)m.number("yup_to_zup", -1);
+        const std::string mf = m.string("materials", "");
+        if (!mf.empty()) d.materials = rel(mf);
         if (d.file.empty()) { err = "scene file: mesh without \"file\""; return false; }
         S.meshes.push_back(d);
     }

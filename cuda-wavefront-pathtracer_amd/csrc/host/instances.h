@@ -39,7 +39,7 @@ struct InstanceDesc {
     uint64_t randomKey = 0;
     Vec3 spanLo = v3(0, 0, 0), spanHi = v3(0, 0, 0);   // translation range of random instances
 };
-struct MeshDesc { std::string file; int yup = -1; };
+struct MeshDesc { std::string file; int yup = -1; std::string materials; };   // materials: per-mesh materials file
 struct SceneDesc { std::vector<MeshDesc> meshes; std::vector<InstanceDesc> instances; int frames = 1; };
 
 bool loadSceneFile(const std::string& path, SceneDesc& S, std::string& err);

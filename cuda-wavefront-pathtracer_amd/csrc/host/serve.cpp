@@ -54,9 +54,13 @@ bool Server::ensureLayout(HostScene& H, std::string& err) {
 int Server::addScene(const SceneSetup& Z, std::string& err) {
     if (Z.inst) { err = "serve: two-level scenes are not served"; return -1; }
     if (!nodeFloat4s(Z.variant)) { err = "serve: the setup has no BVH"; return -1; }
+    if (Z.full && !O_.batch.full) { err = "serve: the scene needs the full integrator; the server runs the legacy one"; return -1; }
     HostScene H;
     std::shared_ptr<SceneSetup> Zc = std::make_shared<SceneSetup>(Z);
-    H.lay[layIndex(Z.variant)] = std::move(Zc->S);
+    // a legacy setup on a full-integrator server renders its Lambert materials with the full integrator: its layout
+    // is rebuilt with the full-integrator tables
+    if (O_.batch.full && !Z.full) Zc->full = true;
+    else H.lay[layIndex(Z.variant)] = std::move(Zc->S);
     Zc->S = SceneUpload(); Zc->tree = BinaryBVH();   // the copy keeps the mesh and options only
     H.Z = Zc;
     H.P = Z.P;

@@ -93,6 +93,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--gd_energy") ok = inum(A.gdEnergy);
         else if (k == "--gen_env") ok = str(A.genEnv);
         else if (k == "--gen_materials") ok = inum(A.genMaterials);
+        else if (k == "--graph") ok = inum(A.graph);
         else if (k == "--devloop" || k == "--sort") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
@@ -152,6 +153,40 @@ static bool buildTree(const Mesh& M, const Args& A, int leafMax, BinaryBVH& tree
     const auto t1 = std::chrono::steady_clock::now();
     treeFromHploc(hn, n, root, leafMax, tree);
     postMs = msSince(t1); treeMs = gpuMs;
+    return true;
+}
+
+bool frameEmitters(const SceneSetup& Z, int f, std::vector<Vec4>& emTris, std::vector<float>& emCdf,
+                   std::vector<float>& matLightPdf) {
+    emTris.clear(); emCdf.clear(); matLightPdf.assign(Z.S.mats.size(), 0.f);
+    if (!Z.inst || !Z.full || Z.S.numEmInst == 0) return false;
+    // leaf position of every mesh triangle: S.order maps leaf -> mesh triangle offset + local triangle
+    std::vector<int> posOf(Z.S.order.size(), 0);
+    for (size_t kk = 0; kk < Z.S.order.size(); kk++) posOf[Z.S.order[kk]] = (int)kk;
+    std::vector<double> pw;
+    double tot = 0;
+    for (const InstanceDesc& I : Z.SD.instances) {
+        const Xform X = instanceXform(I, f);
+        const Mesh& m = Z.meshes[I.mesh];
+        const int off = Z.S.meshTris[2 * I.mesh];
+        for (size_t t = 0; t < m.f.size(); t++) {
+            const int mi = m.mat(t);
+            if (!isEmissive(Z.S.mats[mi])) continue;
+            const Vec3 w0 = xfPoint(X, m.v[m.f[t].a]), w1 = xfPoint(X, m.v[m.f[t].b]), w2 = xfPoint(X, m.v[m.f[t].c]);
+            const Vec3 e1 = w1 - w0, e2 = w2 - w0;
+            const double area = 0.5 * (double)length(cross(e1, e2));
+            emTris.push_back(v4(w0, asFloat(posOf[off + (int)t]))); emTris.push_back(v4(e1, asFloat(mi))); emTris.push_back(v4(e2, 0.f));
+            double p = (double)luminance(Z.S.mats[mi].emission) * area;
+            p = p > 0 ? p : 0;
+            pw.push_back(p); tot += p;
+        }
+    }
+    emCdf.assign(pw.size() + 1, 0.f);
+    double c = 0;
+    for (size_t e = 0; e < pw.size(); e++) { c += pw[e]; emCdf[e + 1] = (float)(c / tot); }
+    emCdf.back() = 1.f;
+    for (size_t j = 0; j < Z.S.mats.size(); j++)
+        if (isEmissive(Z.S.mats[j])) matLightPdf[j] = (float)(luminance(Z.S.mats[j].emission) / tot);
     return true;
 }
 
@@ -251,6 +286,7 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
         if (Z.inst) {
             const AABB mb = meshBox(mk);
             S.meshRoot.push_back(root); S.meshBox.push_back(v4(mb.lo, 0.f)); S.meshBox.push_back(v4(mb.hi, 0.f));
+            S.meshTris.push_back(triBase); S.meshTris.push_back((int)mk.f.size());
         }
         if (k == 0) Z.tree = tree;
         Z.treeMs += treeMs; Z.postMs += postMs; Z.maxDepth = std::max(Z.maxDepth, tree.maxDepth);
@@ -263,7 +299,15 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
         S.sunOn = luminance(A.light.sunE) > 0.f ? 1 : 0;
         S.lightMode = A.lightSampling == "nee" ? kLightNEE : (A.lightSampling == "bsdf" ? kLightBSDF : kLightMIS);
         S.rr = A.rr; S.rrDepth = A.rrDepth; S.gdEnergy = A.gdEnergy; S.envUniform = A.envSampling == "uniform" ? 1 : 0;
-        emitterTables(M, S.mats, S);
+        if (Z.inst) {   // emitters are per frame in world space; here only their count
+            S.numEmInst = 0;
+            for (const InstanceDesc& I : Z.SD.instances) {
+                const Mesh& m = Z.meshes[I.mesh];
+                for (size_t t = 0; t < m.f.size(); t++) if (isEmissive(S.mats[m.mat(t)])) S.numEmInst++;
+            }
+        } else {
+            emitterTables(M, S.mats, S);
+        }
     }
     Z.bvhS = msSince(t0) / 1000.0;
     Z.frames = A.frames > 0 ? A.frames : (Z.inst ? Z.SD.frames : 1);
@@ -306,6 +350,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
             for (const MeshDesc& d : Z.SD.meshes) {
                 Mesh m;
                 if (!loadMesh(d.file, m)) { err = "failed to load mesh " + d.file; return false; }
+                if (!d.materials.empty() && !loadMaterialsFile(d.materials, m, err)) return false;
                 if (d.yup == 1) for (Vec3& p : m.v) p = v3(p.x, -p.z, p.y);   // y-up to z-up
                 if (d.yup == 1) for (Vec3& q : m.n) q = v3(q.x, -q.z, q.y);
                 meshes.push_back(m);
@@ -315,12 +360,18 @@ bool setupScene(SceneSetup& Z, std::string& err) {
         // one material table: each mesh's materials appended, its material ids offset
         M = Mesh();
         std::vector<int> matOff;
+        bool anyMats = false;
+        for (const Mesh& m : meshes) anyMats = anyMats || !m.mats.empty();
         for (Mesh& m : meshes) {
             if (m.kd.empty()) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
             int maxMat = 0; for (int t : m.fm) maxMat = std::max(maxMat, t);
             while ((int)m.kd.size() <= maxMat) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+            while (m.kd.size() < m.mats.size()) m.kd.push_back(m.mats[m.kd.size()].baseColor);
             matOff.push_back((int)M.kd.size());
-            for (size_t j = 0; j < m.kd.size(); j++) { M.kd.push_back(m.kd[j]); M.matName.push_back(j < m.matName.size() ? m.matName[j] : ""); }
+            for (size_t j = 0; j < m.kd.size(); j++) {
+                M.kd.push_back(m.kd[j]); M.matName.push_back(j < m.matName.size() ? m.matName[j] : "");
+                if (anyMats) M.mats.push_back(j < m.mats.size() ? m.mats[j] : lambertMaterial(m.kd[j]));
+            }
         }
         if (A.flatten) {   // every instance transformed into one mesh at the first frame (reference --flatten 1)
             for (const InstanceDesc& I : Z.SD.instances) appendTransformed(M, meshes[I.mesh], matOff[I.mesh], instanceXform(I, A.frameStart));
@@ -388,7 +439,6 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     bool needFull = A.rr != 0 || !A.light.envMap.empty() || !A.genEnv.empty();
     for (const Material& x : Z.mats) needFull = needFull || !isLambert(x) || isEmissive(x);
     Z.full = A.integrator == "full" || (A.integrator == "auto" && needFull);
-    if (Z.full && Z.inst) { err = "the full integrator renders single-level scenes (use --flatten 1)"; return false; }
     Z.env.reset();
     if (Z.full && (!A.light.envMap.empty() || !A.genEnv.empty())) {
         std::vector<float> rgb;
@@ -442,6 +492,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
     O.shadowRefill = A.shadowRefill;
+    O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;
 }
 
@@ -457,7 +508,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"rays_total\":%.0f,\"t_trace_ms\":%.2f,\"t_frame_ms\":%.2f,\"mrays_s_trace\":%.1f,\"mrays_s_e2e\":%.1f,"
              "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
              "\"seed\":%u,\"integrator\":\"%s\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
-             "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":0,"
+             "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":%d,"
              "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
              rep, Z.variant, A.tri.c_str(), A.persistent, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
@@ -465,7 +516,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
              Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
-             Z.treeMs, Z.postMs, Z.sah, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
+             Z.treeMs, Z.postMs, Z.sah, R.options().graph, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
              A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
              st.unoccluded);
     return buf;

@@ -31,6 +31,7 @@ struct Args {
     int rr = 0, rrDepth = 3, gdEnergy = 1;
     std::string genEnv;              // generated environment "W,H,SIGMA[,SEED]": log-normal luminance (mu 0)
     int genMaterials = -1;           // >= 0: seed of the generated OpenPBR materials of a --gen scene's boxes
+    int graph = -1;                  // frames replayed from a captured HIP graph (-1: on for the full integrator)
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);
@@ -67,6 +68,12 @@ struct SceneSetup {
 
 // Instance records of frame f of a two-level scene (false if the scene is not instanced).
 bool frameInstances(const SceneSetup& Z, int f, std::vector<InstanceRecord>& out);
+// World-space emissive triangles of frame f of a two-level full-integrator scene, 3 float4 each: (v0, leaf position
+// bits), (e1, material bits), (e2, 0), in instance order then mesh order; their power CDF; and per material the
+// light pdf per unit area luminance(emission) / total power (reference: frameData, src/app/scene_setup.cpp:273-299).
+// False if the scene has no instanced emitters.
+bool frameEmitters(const SceneSetup& Z, int f, std::vector<Vec4>& emTris, std::vector<float>& emCdf,
+                   std::vector<float>& matLightPdf);
 
 // Runs the setup from Z.A; false with a message in err.
 bool setupScene(SceneSetup& Z, std::string& err);

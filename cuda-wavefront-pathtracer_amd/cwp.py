@@ -85,7 +85,8 @@ class Options(ctypes.Structure):
     """cwp_options: kernel and schedule options."""
     _fields_ = [("tri_test", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int), ("fuse", ctypes.c_int),
                 ("persistent", ctypes.c_int), ("shadow_refill", ctypes.c_int), ("stack_lds", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("bounce_waves", ctypes.c_int), ("time_kernels", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("bounce_waves", ctypes.c_int), ("time_kernels", ctypes.c_int),
+                ("graph", ctypes.c_int)]
 
     @classmethod
     def default(cls, **kw):
@@ -101,7 +102,8 @@ class ServerOptions(ctypes.Structure):
     pass, scene budget and mode."""
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("stack_lds", ctypes.c_int), ("bounces", ctypes.c_int),
                 ("max_paths", ctypes.c_size_t), ("max_jobs", ctypes.c_int), ("scene_budget_mb", ctypes.c_double),
-                ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("full", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -202,6 +204,10 @@ SIGNATURES = {
     "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),
     "cwp_hploc_nodes": ([_P, ctypes.POINTER(_P)], _I),
     "cwp_hploc_to_bvh2": ([_P, _I, _P, _P], _I),
+    "cwp_set_emitters": ([_P, ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(ctypes.c_float),
+                          ctypes.POINTER(ctypes.c_float), _I], _I),
+    "cwp_setup_emitters": ([_P, _I, ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float),
+                            ctypes.POINTER(ctypes.c_float), _I], _I),
     "cwp_server_options_default": ([ctypes.POINTER(ServerOptions)], None),
     "cwp_server_create": ([ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)], _I),
     "cwp_server_destroy": ([_P], None),
@@ -309,6 +315,18 @@ class Setup:
         _check(lib().cwp_setup_instances(self.h, frame, rec, n.value, ctypes.byref(n)))
         return np.frombuffer(rec, dtype=np.float32).reshape(n.value, 28).copy()
 
+    def emitters(self, frame):
+        """Two-level full-integrator scenes: (world emitters (n, 3, 4), power CDF (n + 1), per-material light pdf)."""
+        n = ctypes.c_int()
+        _check(lib().cwp_setup_emitters(self.h, frame, None, 0, ctypes.byref(n), None, None, 0))
+        em = np.zeros((n.value, 3, 4), np.float32)
+        cdf = np.zeros(n.value + 1, np.float32)
+        mlp = np.zeros(self.info["num_materials"], np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(lib().cwp_setup_emitters(self.h, frame, em.ctypes.data_as(fp), n.value, ctypes.byref(n), cdf.ctypes.data_as(fp),
+                                        mlp.ctypes.data_as(fp), len(mlp)))
+        return em, cdf, mlp
+
     def close(self):
         if getattr(self, "h", None):
             lib().cwp_setup_destroy(self.h)
@@ -379,6 +397,13 @@ class Renderer:
         ms = ctypes.c_double()
         _check(lib().cwp_set_instances(self.h, ctypes.cast(r.ctypes.data, ctypes.POINTER(Instance)), len(r), ctypes.byref(ms)))
         return ms.value
+
+    def set_emitters(self, em_tris, em_cdf, mat_light_pdf):
+        fp = ctypes.POINTER(ctypes.c_float)
+        e = np.ascontiguousarray(em_tris, np.float32).reshape(-1, 3, 4)
+        c = np.ascontiguousarray(em_cdf, np.float32)
+        m = np.ascontiguousarray(mat_light_pdf, np.float32)
+        _check(lib().cwp_set_emitters(self.h, e.ctypes.data_as(fp), len(e), c.ctypes.data_as(fp), m.ctypes.data_as(fp), len(m)))
 
     def trace_inst(self, rays, any_hit=False):
         """Two-level scenes: (hits (n, 4), instance ids (n,))."""

@@ -92,6 +92,7 @@ typedef struct cwp_options {
     int blocks_per_cu;   /* workgroups per CU of the persistent grids (0 = occupancy limit) */
     int bounce_waves;    /* fused kernel register budget: at least this many waves per SIMD (1, 6 or 8) */
     int time_kernels;    /* per-launch event timing (t_bounce_ms, t_shade_ms, t_shadow_ms) */
+    int graph;           /* 1: each frame replays a captured HIP graph (per-launch timing off) */
 } cwp_options;
 
 /* Scene and BVH summary of a setup. */
@@ -203,8 +204,19 @@ int cwp_setup_instances(const cwp_setup* s, int frame, cwp_instance* out, int ca
  * receives its device time. Every record must name a mesh and that mesh's BLAS root. Replaces setInstances
  * (src/render/renderer.h:52). cwp_create does this for the setup's first frame. */
 int cwp_set_instances(cwp_ctx* c, const cwp_instance* inst, int n, double* tlas_ms);
-/* cwp_setup_instances + cwp_set_instances for a frame of the context's setup. */
+/* cwp_setup_instances + cwp_set_instances (and, for the full integrator, cwp_setup_emitters + cwp_set_emitters) for a
+ * frame of the context's setup. */
 int cwp_set_frame(cwp_ctx* c, int frame, double* tlas_ms);
+/* Two-level full-integrator scenes: the frame's world-space emissive triangles (3 float[4] each: (v0, leaf position
+ * bits), (e1, material bits), (e2, 0)), their power CDF (num_emitters + 1) and per material the light pdf per unit
+ * area; num_emitters must equal the scene's instanced emitter count (reference: the emTris / emCdf / matLightPdf
+ * arguments of setInstances, src/render/renderer.h:52). */
+int cwp_set_emitters(cwp_ctx* c, const float* em_tris, int num_emitters, const float* em_cdf, const float* mat_light_pdf,
+                     int num_materials);
+/* the setup's emitters of a frame (em_tris = NULL: count only; em_tris: cap emitters, em_cdf: num + 1 floats,
+ * mat_light_pdf: mat_cap floats or NULL) (reference: frameData, src/app/scene_setup.cpp:273-299) */
+int cwp_setup_emitters(const cwp_setup* s, int frame, float* em_tris, int cap, int* num_emitters, float* em_cdf,
+                       float* mat_light_pdf, int mat_cap);
 /* cwp_trace_host that also reports the hit instance of two-level scenes (-1 on a miss). */
 int cwp_trace_host_inst(cwp_ctx* c, const float* rays, int n, int any_hit, float* hits, int* inst);
 /* Grid size (workgroups) and VRAM used by the context. */
@@ -261,6 +273,7 @@ typedef struct cwp_server_options {
     int batched;                       /* 1: continuous batching; 0: plain, one job at a time */
     int rays_per_lane;                 /* 1, 2 or 4: persistent grid = resident waves / rays_per_lane */
     int use_graph;                     /* 1: each pass replays a captured HIP graph */
+    int full;                          /* 1: full material integrator (legacy setups render their Lambert materials) */
 } cwp_server_options;
 typedef struct cwp_tune_trial {
     int variant, stack_lds, rays_per_lane;

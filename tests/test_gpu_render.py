@@ -217,3 +217,28 @@ def test_renderer_from_caller_buffers_matches_setup_renderer():
     with pytest.raises(cwp.CwpError, match="frame_json|null"):
         r.frame_json(st)
     r.close()
+
+
+@pytest.mark.parametrize("full", [0, 1])
+def test_graph_frames_equal_stream_frames(full):
+    """frames replayed from a captured graph equal frames launched one kernel at a time, bit for bit, on every
+    replay, with other work (a different frame, a parameter change) between the replays"""
+    args = SCENES["terrain"] + ["--batch", "8", "--bounces", "4", "--spp", "16"]
+    if full:
+        args += ["--gen_materials", "3", "--gen_env", "64,32,2,1"]
+    ref = cwp.Renderer(cwp.Setup(args + ["--graph", "0"]))
+    ref.render(16)
+    a = ref.accum()
+    ref.render(8, sample_base=16)
+    b = ref.accum()
+    g = cwp.Renderer(cwp.Setup(args + ["--graph", "1"]))
+    for rep in range(3):
+        st = g.render(16)
+        np.testing.assert_array_equal(g.accum(), a, err_msg="replay %d" % rep)
+        assert g.frame_json(st)["graph"] == 1 and st["t_bounce_ms"] == 0
+        g.render(8, sample_base=16)            # another captured frame in between
+        np.testing.assert_array_equal(g.accum(), b)
+    p = g.setup.params()
+    g.set_params(p)                            # parameters are read through a pointer: the graphs stay valid
+    g.render(16)
+    np.testing.assert_array_equal(g.accum(), a)

@@ -190,3 +190,47 @@ def test_autotune_picks_a_configuration_and_keeps_results():
         else:
             spp = srv.job(i)["spp"]
             assert psnr(img[..., :3] / spp, r[..., :3] / spp) >= 50.0
+
+
+def test_full_integrator_jobs_alone_equal_mix():
+    """full-integrator server: materials, emitters and an environment; each job alone equals the job in the mix and in
+    plain mode, and a job at the scene's resolution equals the single-job renderer's image"""
+    mats = ["--gen_materials", "4", "--gen_env", "64,32,2,5"]
+    S = [cwp.Setup(TERRAIN + mats + ["--bounces", "4"]), cwp.Setup(BOXES + ["--gen_materials", "6", "--bounces", "4"])]
+    assert all(s.info["full"] for s in S)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16, full=1)
+    for s in S:
+        srv.add_scene(s)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in JOBS]
+    mix, _, _ = run_jobs(srv, JOBS, True)
+    plain, _, _ = run_jobs(srv, JOBS, False)
+    for k in range(len(JOBS)):
+        np.testing.assert_array_equal(alone[k], mix[k], err_msg="job %d" % k)
+        np.testing.assert_array_equal(alone[k], plain[k], err_msg="plain job %d" % k)
+    spp = 8
+    s = cwp.Setup(TERRAIN + mats + ["--bounces", "4", "--spp", str(spp), "--batch", str(spp), "--seed", "1000"])
+    r = cwp.Renderer(s)
+    r.render(spp)
+    ref = r.accum()
+    r.close()
+    srv2 = cwp.Server(variant=8, bounces=4, max_paths=1 << 20, full=1)
+    sid = srv2.add_scene(s)
+    img, info, _ = run_jobs(srv2, [(sid, 64, 48, spp, 4, 1000)], True)
+    np.testing.assert_array_equal(img[0], ref)
+
+
+def test_legacy_scene_on_full_server_and_full_scene_on_legacy_server():
+    full = cwp.Setup(TERRAIN + ["--gen_materials", "4", "--bounces", "4"])
+    legacy = cwp.Setup(BOXES + ["--bounces", "4"])
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16)
+    with pytest.raises(cwp.CwpError):
+        srv.add_scene(full)
+    fsrv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16, full=1)
+    sid = fsrv.add_scene(legacy)
+    img, _, _ = run_jobs(fsrv, [(sid, 32, 32, 8, 3, 2)], True)
+    # the full integrator on Lambert materials estimates the same image as the legacy one (different sampling
+    # arithmetic, same expectation): compare with the legacy server at 8 spp loosely
+    lsrv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16)
+    lid = lsrv.add_scene(legacy)
+    limg, _, _ = run_jobs(lsrv, [(lid, 32, 32, 8, 3, 2)], True)
+    assert psnr(img[0][..., :3] / 8, limg[0][..., :3] / 8) >= 40.0
