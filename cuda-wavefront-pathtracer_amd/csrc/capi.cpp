@@ -78,7 +78,7 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     info->num_meshes = Z.inst ? (int)Z.meshes.size() : 0; info->num_instances = Z.inst ? (int)Z.SD.instances.size() : 0;
     info->frames = Z.frames; info->frame_start = Z.A.frameStart;
     const SceneUpload& S = Z.S;
-    info->full = S.full; info->num_emitters = (int)S.emTri.size();
+    info->full = S.full; info->num_emitters = S.inst ? S.numEmInst : (int)S.emTri.size();
     info->env_w = S.env ? S.env->w : 0; info->env_h = S.env ? S.env->h : 0; info->sun_on = S.sunOn;
     info->light_mode = S.lightMode; info->rr = S.rr; info->rr_depth = S.rrDepth; info->gd_energy = S.gdEnergy;
     info->env_uniform = S.envUniform; info->env_rot_deg = S.envRotDeg;
@@ -115,6 +115,7 @@ int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* co
         case CWP_ARR_TRI_LIGHT_PDF: *data = Z.S.triLightPdf.data(); *count = Z.S.triLightPdf.size(); break;
         case CWP_ARR_ENV_TEX: *data = Z.S.env ? (const void*)Z.S.env->texels.data() : nullptr; *count = Z.S.env ? Z.S.env->texels.size() : 0; break;
         case CWP_ARR_ENV_MARG: *data = Z.S.env ? (const void*)Z.S.env->marg.data() : nullptr; *count = Z.S.env ? Z.S.env->marg.size() : 0; break;
+        case CWP_ARR_MESH_TRIS: *data = Z.S.meshTris.data(); *count = Z.S.meshTris.size() / 2; break;
         case CWP_ARR_ENV_COND: *data = Z.S.env ? (const void*)Z.S.env->cond.data() : nullptr; *count = Z.S.env ? Z.S.env->cond.size() : 0; break;
         default: return fail(CWP_ERR_ARG, "cwp_setup_array: unknown array");
     }

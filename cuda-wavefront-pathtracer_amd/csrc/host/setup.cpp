@@ -345,6 +345,19 @@ bool setupScene(SceneSetup& Z, std::string& err) {
         if (A.gen == "instances") {
             makeInstanceScene(meshes, Z.SD, A.genInstances >= 0 ? A.genInstances : 1000, A.genSeed, A.genSmall != 0);
             instanceSceneCamera(c2w); haveC2w = true; Z.sceneName = "gen:instances";
+            if (A.genMaterials >= 0) {   // one OpenPBR material per mesh: glossy-diffuse, conductor, dielectric, emitter
+                SplitMix r((uint64_t)A.genMaterials * 0x9E3779B97F4A7C15ull + 11);
+                for (size_t k = 0; k < meshes.size(); k++) {
+                    Material x = defaultMaterial();
+                    x.baseColor = meshes[k].kd.empty() ? v3(0.8f, 0.8f, 0.8f) : meshes[k].kd[0];
+                    x.roughness = r.uniform(0.05f, 0.6f);
+                    const int kind = k + 1 == meshes.size() ? 0 : (int)(k % 4);
+                    if (kind == 1) { x.metallic = 1.f; x.specularColor = v3(0.95f, 0.86f, 0.55f); }
+                    else if (kind == 2) { x.transmission = 1.f; x.baseColor = v3(0.95f, 0.95f, 1.f); }
+                    else if (kind == 3) { x = lambertMaterial(v3(0.5f, 0.5f, 0.5f)); x.emission = x.baseColor * 6.f; }
+                    meshes[k].mats.assign(1, x);
+                }
+            }
         } else {
             if (!loadSceneFile(A.scene, Z.SD, err)) return false;
             for (const MeshDesc& d : Z.SD.meshes) {
@@ -419,8 +432,8 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     if (A.albedo.x >= 0) for (auto& kd : M.kd) kd = A.albedo;
     if (M.kd.empty()) M.kd.push_back(v3(0.8f, 0.8f, 0.8f));
     if (!A.materials.empty() && !loadMaterialsFile(A.materials, M, err)) return false;
-    if (A.genMaterials >= 0) {
-        if (Z.genBoxFirst < 0) { err = "--gen_materials needs a --gen boxes or --gen terrain scene"; return false; }
+    if (A.genMaterials >= 0 && A.gen != "instances") {
+        if (Z.genBoxFirst < 0) { err = "--gen_materials needs a --gen boxes, terrain or instances scene"; return false; }
         assignGeneratedMaterials(M, Z.genBoxFirst, Z.genBoxCount, (uint64_t)A.genMaterials);
     }
     { int maxMat = 0; for (int t : M.fm) maxMat = t > maxMat ? t : maxMat;

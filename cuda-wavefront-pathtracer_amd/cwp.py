@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG, "libcwp.so")
 
 ARR_VERTICES, ARR_TRIANGLES, ARR_MATERIAL, ARR_ALBEDO, ARR_NODES, ARR_ORDER, ARR_WOOP, ARR_CORNERS, ARR_NORMALS, \
     ARR_TREE, ARR_TREE_BOX, ARR_PRIM, ARR_MESH_ROOTS, ARR_MESH_BOXES, ARR_MATERIALS, ARR_EM_TRI, ARR_EM_CDF, \
-    ARR_TRI_LIGHT_PDF, ARR_ENV_TEX, ARR_ENV_MARG, ARR_ENV_COND = range(21)
+    ARR_TRI_LIGHT_PDF, ARR_ENV_TEX, ARR_ENV_MARG, ARR_ENV_COND, ARR_MESH_TRIS = range(22)
 TRI_MT, TRI_WOOP, TRI_WATERTIGHT = 0, 1, 2
 CWP_COPY_H2D, CWP_COPY_D2H, CWP_COPY_D2D = 1, 2, 3
 
@@ -265,7 +265,7 @@ _ARRAY_LAYOUT = {ARR_VERTICES: (np.float32, (3,)), ARR_TRIANGLES: (np.int32, (3,
                  ARR_TREE_BOX: (np.float32, (6,)), ARR_PRIM: (np.int32, ()), ARR_MESH_ROOTS: (np.int32, ()),
                  ARR_MESH_BOXES: (np.float32, (8,)), ARR_MATERIALS: (np.float32, (16,)), ARR_EM_TRI: (np.int32, ()),
                  ARR_EM_CDF: (np.float32, ()), ARR_TRI_LIGHT_PDF: (np.float32, ()), ARR_ENV_TEX: (np.float32, (4,)),
-                 ARR_ENV_MARG: (np.float32, ()), ARR_ENV_COND: (np.float32, ())}
+                 ARR_ENV_MARG: (np.float32, ()), ARR_ENV_COND: (np.float32, ()), ARR_MESH_TRIS: (np.int32, (2,))}
 
 
 def _view(ptr, count, which, node_floats4=None):

@@ -146,7 +146,8 @@ enum {
     CWP_ARR_TRI_LIGHT_PDF = 17, /* float per leaf position: light-selection probability / area */
     CWP_ARR_ENV_TEX = 18,    /* float[4] per environment texel (radiance x strength), row 0 = +z */
     CWP_ARR_ENV_MARG = 19,   /* float per environment row + 1: marginal CDF */
-    CWP_ARR_ENV_COND = 20    /* float per (row, column + 1): conditional CDFs */
+    CWP_ARR_ENV_COND = 20,   /* float per (row, column + 1): conditional CDFs */
+    CWP_ARR_MESH_TRIS = 21   /* int[2] per mesh of a two-level scene: first leaf position, triangle count */
 };
 
 const char* cwp_last_error(void);

@@ -1350,6 +1350,64 @@ int orc_emitter_tables(const float* corners, const int* order, int nt, const flo
     return n;
 }
 
+/* ---- two-level scenes: world emitters per frame (src/app/scene_setup.cpp:273-299) */
+int orc_inst_emitters(const float* inst, int n_inst, const int* mesh_tris, const float* corners, const int* order,
+                      const float* mats, int num_mats, float* em_tris, float* em_cdf, float* mlp) {
+    int n = 0, nl = 0;
+    double tot = 0;
+    double* pw = NULL;
+    for (int i = 0; i < n_inst; i++) {   /* leaves of all meshes: the largest first + count */
+        const int mesh = f2i(inst[28 * (size_t)i + 25]);
+        if (mesh_tris[2 * mesh] + mesh_tris[2 * mesh + 1] > nl) nl = mesh_tris[2 * mesh] + mesh_tris[2 * mesh + 1];
+    }
+    int* pos_of = (int*)malloc(sizeof(int) * (nl > 0 ? nl : 1));
+    for (int k = 0; k < nl; k++) pos_of[order[k]] = k;
+    for (int pass = 0; pass < 2; pass++) {
+        n = 0; tot = 0;
+        for (int i = 0; i < n_inst; i++) {
+            const float* R = inst + 28 * (size_t)i;
+            const int mesh = f2i(R[25]);
+            const int first = mesh_tris[2 * mesh], cnt = mesh_tris[2 * mesh + 1];
+            /* the mesh's triangles in mesh order: leaf of local triangle t = the leaf whose order entry is first + t */
+            for (int t = 0; t < cnt; t++) {
+                const int leaf = pos_of[first + t];
+                const float* Cn = corners + 12 * (size_t)leaf;
+                const int mi = f2i(Cn[7]);
+                orc_mat m = ld_mat(mats + 16 * mi);
+                if (!emissive(&m)) continue;
+                if (pass == 1) {
+                    f3 w0 = xf_p(R + 12, ld3(Cn)), w1 = xf_p(R + 12, ld3(Cn + 4)), w2 = xf_p(R + 12, ld3(Cn + 8));
+                    f3 e1 = sub(w1, w0), e2 = sub(w2, w0);
+                    f3 cr = crs(e1, e2);
+                    double area = 0.5 * (double)sqrtf(dot3(cr, cr));
+                    float* E = em_tris + 12 * (size_t)n;
+                    E[0] = w0.x; E[1] = w0.y; E[2] = w0.z; E[3] = i2f(leaf);
+                    E[4] = e1.x; E[5] = e1.y; E[6] = e1.z; E[7] = i2f(mi);
+                    E[8] = e2.x; E[9] = e2.y; E[10] = e2.z; E[11] = 0.f;
+                    double p = (double)lum3(m.emis) * area;
+                    pw[n] = p > 0 ? p : 0; tot += pw[n];
+                }
+                n++;
+            }
+        }
+        if (pass == 0) {
+            if (!em_tris) { free(pos_of); return n; }
+            pw = (double*)malloc(sizeof(double) * (n + 1));
+        }
+    }
+    em_cdf[0] = 0.f;
+    double c = 0;
+    for (int e = 0; e < n; e++) { c += pw[e]; em_cdf[e + 1] = (float)(c / tot); }
+    if (n) em_cdf[n] = 1.f;
+    for (int j = 0; j < num_mats; j++) {
+        orc_mat m = ld_mat(mats + 16 * j);
+        mlp[j] = emissive(&m) && n ? (float)(lum3(m.emis) / tot) : 0.f;
+    }
+    free(pw);
+    free(pos_of);
+    return n;
+}
+
 /* ---- one path of the full integrator */
 static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint32_t smp, int bounces, double* st) {
     float o3[3], d3[3];
@@ -1379,11 +1437,13 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
         }
         f3 ph = add(o, scl(d, h.t));
         const float* P = s->corners + 12 * h.tri;
+        const float* XI = s->tlas ? s->inst + 28 * (size_t)h.inst : NULL;   /* two-level: normals through inv^T */
         f3 e1 = sub(ld3(P + 4), ld3(P)), e2 = sub(ld3(P + 8), ld3(P));
-        f3 ng0 = nrm3(crs(e1, e2)), ns = ng0;
+        f3 ng0 = XI ? scl(nrm3(xf_n(XI, crs(e1, e2))), XI[26]) : nrm3(crs(e1, e2)), ns = ng0;
         if (p->has_normals && s->normals) {
             const float* N = s->normals + 12 * h.tri;
             f3 nn = add(add(scl(ld3(N), 1.f - h.u - h.v), scl(ld3(N + 4), h.u)), scl(ld3(N + 8), h.v));
+            if (XI) nn = xf_n(XI, nn);
             float l = sqrtf(dot3(nn, nn));
             if (l > 0.f) ns = scl(nn, 1.f / l);
             if (dot3(ns, ng0) < 0.f) ns = neg(ns);
@@ -1398,7 +1458,8 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
                 if (s->light_mode == 1) w = 0.f;
                 else if (s->light_mode == 0) {
                     float cl = fabsf(dot3(ng0, d));
-                    float pl = cl > 0.f ? s->p_tri * s->tri_light_pdf[h.tri] * h.t * h.t / cl : 0.f;
+                    float pa = XI ? s->mat_light_pdf[f2i(P[7])] : s->tri_light_pdf[h.tri];
+                    float pl = cl > 0.f ? s->p_tri * pa * h.t * h.t / cl : 0.f;
                     w = mis2(lastPdf, pl);
                 }
             }
@@ -1428,9 +1489,12 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
             if (us < s->p_sun) { l = cone(sunTo, p->sun_cos, u2, u3); Le = sunE; pl = s->p_sun; isSun = 1; ok = 1; }
             else if (us < s->p_sun + s->p_tri) {
                 float ut = fminf((us - s->p_sun) / s->p_tri, ORC_ONE_MINUS);
-                int kt = s->em_tri[cdf_find(s->em_cdf, s->num_em, ut)];
-                const float* A = s->corners + 12 * (size_t)kt;
-                f3 f1 = sub(ld3(A + 4), ld3(A)), f2 = sub(ld3(A + 8), ld3(A));
+                int em = cdf_find(s->em_cdf, s->num_em, ut);
+                const float* A;
+                f3 f1, f2;
+                int kt = -1;
+                if (s->tlas) { A = s->em_tris + 12 * (size_t)em; f1 = ld3(A + 4); f2 = ld3(A + 8); }
+                else { kt = s->em_tri[em]; A = s->corners + 12 * (size_t)kt; f1 = sub(ld3(A + 4), ld3(A)); f2 = sub(ld3(A + 8), ld3(A)); }
                 float su = sqrtf(u2);
                 f3 pL = add(add(ld3(A), scl(f1, su * (1.f - u3))), scl(f2, su * u3));
                 f3 nL = nrm3(crs(f1, f2));
@@ -1442,7 +1506,7 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
                     l = scl(dv, 1.f / dist);
                     float cl = fabsf(dot3(nL, l));
                     if (cl > 0.f) {
-                        pl = s->p_tri * s->tri_light_pdf[kt] * dist2 / cl;
+                        pl = s->p_tri * (s->tlas ? s->mat_light_pdf[f2i(A[7])] : s->tri_light_pdf[kt]) * dist2 / cl;
                         Le = ld_mat(s->mats + 16 * f2i(A[7])).emis;
                         tmax = dist * (1.f - 1e-3f);
                         ok = 1;

@@ -65,6 +65,10 @@ typedef struct orc_scene {
     float env_rot;
     float p_sun, p_tri, p_env;
     int light_mode, rr, rr_depth, gd_energy, env_uniform;
+    /* two-level scenes with the full integrator: the frame's world emitters (orc_inst_emitters) and per-material
+     * light pdf per unit area */
+    const float* em_tris;
+    const float* mat_light_pdf;
 } orc_scene;
 
 /* binary SAH tree: node = int[4] (left, right, first, count) + float[6] box (lo, hi); prim = triangle order */
@@ -164,6 +168,13 @@ void orc_env_sample_batch(const orc_scene* s, int n, uint64_t seed, float* dirs,
  * tri_light_pdf (num_tris) */
 int orc_emitter_tables(const float* corners, const int* order, int num_tris, const float* mats, int* em_tri, float* em_cdf,
                        float* tri_light_pdf);
+
+/* two-level scenes: world-space emissive triangles of the given instance records (28 floats each), in instance order
+ * then mesh triangle order (mesh_tris: first leaf and count per mesh; order: leaf -> first leaf of its mesh + local
+ * triangle), 12 floats each (v0, leaf bits, e1, material bits, e2, 0); power CDF (n + 1); mat_light_pdf (num_mats).
+ * Call with em_tris = NULL to get the count. */
+int orc_inst_emitters(const float* inst, int n_inst, const int* mesh_tris, const float* corners, const int* order,
+                      const float* mats, int num_mats, float* em_tris, float* em_cdf, float* mat_light_pdf);
 
 int orc_render(const orc_scene* s, const orc_params* p, int spp, int sample_base, int batch, int bounces, int threads,
                int clear, float* accum, double* stats);

@@ -35,7 +35,8 @@ class OrcScene(ctypes.Structure):
                 ("em_cdf", _FP), ("num_em", ctypes.c_int), ("env_tex", _FP), ("env_marg", _FP), ("env_cond", _FP),
                 ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("env_rot", ctypes.c_float), ("p_sun", ctypes.c_float),
                 ("p_tri", ctypes.c_float), ("p_env", ctypes.c_float), ("light_mode", ctypes.c_int), ("rr", ctypes.c_int),
-                ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int), ("env_uniform", ctypes.c_int)]
+                ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int), ("env_uniform", ctypes.c_int),
+                ("em_tris", _FP), ("mat_light_pdf", _FP)]
 
 
 _lib = None
@@ -94,6 +95,8 @@ def lib():
         L.orc_env_sample_batch.argtypes = [ctypes.POINTER(OrcScene), ctypes.c_int, ctypes.c_uint64, _FP, _FP]
         L.orc_emitter_tables.argtypes = [_FP, _IP, ctypes.c_int, _FP, _IP, _FP, _FP]
         L.orc_emitter_tables.restype = ctypes.c_int
+        L.orc_inst_emitters.argtypes = [_FP, ctypes.c_int, _IP, _FP, _IP, _FP, ctypes.c_int, _FP, _FP, _FP]
+        L.orc_inst_emitters.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -319,6 +322,29 @@ class Scene:
         c.p_env = p if env else 0.0
         c.light_mode, c.rr, c.rr_depth, c.gd_energy, c.env_uniform = light_mode, rr, rr_depth, gd_energy, env_uniform
         assert nt == len(self.tri_light_pdf)
+
+    def set_inst_emitters(self, mesh_tris, order):
+        """Two-level full-integrator scene (after set_instances and set_full): the frame's world emitters, computed
+        by the oracle from the instance records and the BLAS corners; returns (em_tris (n, 3, 4), em_cdf, mlp)."""
+        L = lib()
+        self.mesh_tris = np.ascontiguousarray(mesh_tris, np.int32).reshape(-1)
+        self.order = np.ascontiguousarray(order, np.int32)
+        args = (_f(self.inst), len(self.inst), _i(self.mesh_tris), _f(self.corners), _i(self.order), _f(self.mats), len(self.mats))
+        n = L.orc_inst_emitters(*args, None, None, None)
+        self.em_tris = np.zeros((max(n, 1), 3, 4), np.float32)
+        self.em_cdf = np.zeros(n + 1, np.float32)
+        self.mlp = np.zeros(len(self.mats), np.float32)
+        L.orc_inst_emitters(*args, _f(self.em_tris), _f(self.em_cdf), _f(self.mlp))
+        c = self.c
+        c.em_tris = _f(self.em_tris); c.mat_light_pdf = _f(self.mlp); c.em_cdf = _f(self.em_cdf); c.num_em = n
+        tri = n > 0 and c.light_mode != 2
+        env = c.env_w > 0 and c.light_mode != 2
+        k = int(c.p_sun > 0) + int(tri) + int(env)
+        p = np.float32(1.0) / np.float32(k) if k else np.float32(0)
+        c.p_tri = p if tri else 0.0
+        c.p_env = p if env else 0.0
+        c.p_sun = p if c.p_sun > 0 else 0.0
+        return self.em_tris[:n], self.em_cdf, self.mlp
 
     def set_instances(self, inst, mesh_box):
         """Two-level scene: instance records (n, 28) float32 (world-to-object rows, object-to-world rows, root bits,
