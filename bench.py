@@ -58,6 +58,23 @@ def measured_traffic(kernel_tag):
     return None
 
 
+def measured_utilisation(kernel_tag):
+    """VALU busy, VALU lane utilisation and L2 hit rate (percent) of the dominant kernel from the committed rocprof
+    counter summary, or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "k_bounce_terrain720p_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("| " + kernel_tag):
+                c = [x.strip() for x in line.strip().strip("|").split("|")]
+                pct = lambda x: float(x.replace("%", "").strip())
+                return {"valu_busy_pct": pct(c[3]), "valu_lane_util_pct": pct(c[4]), "l2_hit_pct": pct(c[5]),
+                        "source": "profiles/k_bounce_terrain720p_counters.md (rocprofv3 SQ_ACTIVE_INST_VALU, "
+                                  "SQ_THREAD_CYCLES_VALU, TCC_HIT/MISS, GRBM_GUI_ACTIVE)"}
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 def cpu_baseline(args):
     """The oracle's multi-threaded CPU renderer on a bounded sample of the same workload (time-boxed)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -384,6 +401,8 @@ def main():
                      "traffic": measured_traffic("k_bounce") if args.fuse and not args.persistent else None,
                      "traffic_source": "profiles/k_bounce_terrain720p_traffic.md (rocprofv3 FETCH_SIZE / 0.5 + WRITE_SIZE / 1.0)",
                      "bytes_per_frame": bytes_bounce / args.steps,
+                     "utilisation": measured_utilisation("k_bounce<8, 1, 8, 8, false, false>") if args.fuse and not args.persistent
+                     and args.variant == "bvh8" else None,
                      "avg_launch_ms": round(bounce_ms / max(1, launches), 4)},
         "latency": None,
         "cpu_baseline": None,
