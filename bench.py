@@ -145,6 +145,24 @@ JOB_RES = (128, 256, 512, 1024)
 JOB_SPP = (4, 16, 64)
 
 
+def bench_reference_config(args):
+    """The same frame in the configuration of the reference's published diffuse benchmark (4 bounces, BVH2 and the
+    compressed BVH8, Woop triangles, fused kernel, 256 spp in flight at 1280x720): Grays/s of one frame after a
+    warm-up, for a like-for-like reading next to the reference's numbers."""
+    import cwp
+    out = {"bounces": 4, "spp": 256, "width": 1280, "height": 720}
+    for variant in ("bvh2", "bvh8"):
+        s = cwp.Setup(workload_args(1280, 720, 256, 4, "woop", variant, args.stack))
+        r = cwp.Renderer(s)
+        r.render(256)
+        st = r.render(256)
+        out[variant + "_grays"] = round((st["rays_closest"] + st["rays_shadow"]) / (st["t_frame_ms"] * 1e-3) / 1e9, 4)
+        out[variant + "_ms"] = round(st["t_frame_ms"], 3)
+        r.close()
+        print("bench: reference config %s: %.3f Grays/s" % (variant, out[variant + "_grays"]), file=sys.stderr, flush=True)
+    return out
+
+
 def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000, full=False):
     """Seeded scenes with triangle counts log-uniform in [lo_tris, hi_tris]: fBm heightfields of (G-1)^2 * 2
     triangles, each with its own terrain seed. With `full`, 64 boxes on each terrain carry OpenPBR-subset materials
@@ -294,7 +312,8 @@ def main():
     ap.add_argument("--bounce-waves", type=int, default=8)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=1, help="1: fused trace+shade kernel; 0: split trace and shade kernels")
-    ap.add_argument("--persistent", type=int, default=0, help="1: split kernels with lanes refilled mid-traversal")
+    ap.add_argument("--persistent", type=int, default=0, help="lanes refilled with new rays mid-traversal: 1 split "
+                    "trace / shade kernels, 2 the fused kernel")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve"],
@@ -310,6 +329,8 @@ def main():
     ap.add_argument("--job-res", default="128,256,512,1024", help="job stream: square resolutions (uniform)")
     ap.add_argument("--job-spp", default="4,16,64", help="job stream: samples per pixel (uniform)")
     ap.add_argument("--no-jobs", action="store_true", help="terrain workload without the job stream")
+    ap.add_argument("--no-reference-config", action="store_true",
+                    help="terrain workload without the frames in the reference's published configuration")
     ap.add_argument("--full", type=int, default=0, help="job stream / serving: the full material integrator "
                     "(OpenPBR-subset materials, emitters, 2048 x 1024 log-normal environment, MIS)")
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
@@ -388,14 +409,14 @@ def main():
                    "width": args.width, "height": args.height, "spp": args.batch, "batch": args.batch,
                    "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
                    "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"],
-                   "kernels": "persistent" if args.persistent else ("fused" if args.fuse else "split"),
+                   "kernels": {1: "persistent-split", 2: "persistent-fused"}.get(args.persistent, "fused" if args.fuse else "split"),
                    "shadow_refill": args.shadow_refill},
         "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
         "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
         "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shade_ms": round(last["t_shade_ms"], 3),
         "t_shadow_ms": round(last["t_shadow_ms"], 3),
         "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
-                     else "k_trace_paths + k_shade (split trace / shade)",
+                     else ("k_bounce_refill (fused, refilled lanes)" if args.persistent == 2 else "k_trace_paths + k_shade (split trace / shade)"),
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic("k_bounce") if args.fuse and not args.persistent else None,
@@ -408,6 +429,8 @@ def main():
         "cpu_baseline": None,
     }
     r.close()
+    if not args.no_reference_config:
+        out["reference_config"] = bench_reference_config(args)
     if not args.no_jobs:
         print("bench: mixed job stream", file=sys.stderr, flush=True)
         j = bench_jobs(args)

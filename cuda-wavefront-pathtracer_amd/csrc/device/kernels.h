@@ -356,6 +356,8 @@ __device__ __forceinline__ int fetchPos(uint32_t f, int lw, int nw) { return (lw
 // idle lanes of a persistent wave are refilled once at least this many are idle (kWave: only when all are, which is
 // the plain one-ray-per-lane schedule)
 constexpr int kRefill = 16;
+// fused persistent bounce: the done lanes shade together once this many are done
+constexpr int kShadeBatch = 32;
 
 // One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here),
 // traversal and shading fused. The queue is split into kParts partitions, one per XCD; the waves of XCD x walk
@@ -455,6 +457,82 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
     }
 }
 
+
+// The fused trace+shade bounce with persistent lanes: a lane whose traversal is over waits (keeping its hit) until
+// REFILL lanes of the wave are done (or no lane traces any more), then the done lanes shade together, append their
+// results, and take new rays from the partition's queue, while the other lanes keep their traversal state. A wave
+// therefore does not idle until its longest ray is done, and shading still runs on many lanes at once. Each path's
+// arithmetic is the one of k_bounce (same image); only the order of the appends differs.
+template <int V, int TRI, int SL, int WAVES, bool FULL, int REFILL>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_refill(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp,
+                                                                 int depth, int smpBase, int K) {
+    const FrameParams& F = *Fp;
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const int* qIn = P.q[depth & 1];
+    int* qOut = P.q[(depth + 1) & 1];
+    int i = -1;                       // path id of this lane's ray, -1 = idle
+    bool tracing = false;
+    RayPre r; Hit h; TravState ts;
+    uint32_t f = 0;                   // wave-uniform: entries of the wave's share handed out so far
+    bool more = fetchPos(0, lw, nw) < n;
+    int misses = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(i < 0);
+        if (more && idle) {
+            if (i < 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int k = fetchPos(f + rank, lw, nw);
+                if (k < n) {
+                    i = depth == 0 ? inStart + k : qIn[inStart + k];
+                    Vec3 o, d; float tmin;
+                    pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+                    r = prepareRay<TRI>(o, d, tmin);
+                    h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+                    travBegin(ts, S.root);
+                    st.sp = 0;
+                    tracing = true;
+                }
+            }
+            f += (uint32_t)__builtin_popcountll(idle);
+            more = fetchPos(f, lw, nw) < n;
+        }
+        if (__ballot(i >= 0) == 0) break;   // no ray in flight: with `more` false the wave's share is done
+        for (;;) {
+            if (tracing) {
+                bool done;
+                if constexpr (V == 2) done = stepBVH2<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+                else if constexpr (V == 4) done = stepBVH4<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+                else done = stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts);
+                if (done) tracing = false;
+            }
+            const uint64_t tr = __ballot(tracing);
+            if (tr == 0 || (more && (int)__builtin_popcountll(__ballot(i >= 0 && !tracing)) >= REFILL)) break;
+        }
+        const bool sh = i >= 0 && !tracing;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (sh) {
+            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            if constexpr (FULL) {
+                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                res = shadeFull<false>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, lastPdf, h);
+            } else {
+                res = shadeLegacy<false>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, h);
+            }
+        }
+        misses += appendResults(P, cnt, part, outStart, qOut, sh ? i : 0, res);
+        if (sh) i = -1;
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
 
 // Shadow rays of bounce `depth` (partition per XCD as in k_bounce): any hit; unoccluded contributions are added to
 // the path's radiance. The any-hit traversal fits 64 registers without spilling: 8 waves per SIMD.
@@ -709,6 +787,8 @@ struct Kernels {
     JobsFn bounceJobs = nullptr, shadowJobs = nullptr;
     // full material integrator (fused bounce kernel; shadow rays use the legacy shadow kernels)
     BounceFn bounceFull = nullptr, bounceFullInst = nullptr;
+    // fused trace + shade with persistent lanes (legacy and full integrator)
+    BounceFn bounceRefill = nullptr, bounceRefillFull = nullptr;
     JobsFn bounceJobsFull = nullptr;
 };
 
@@ -729,6 +809,9 @@ void pickSL(int waves, Kernels& k) {
     k.shadowJobs = k_shadow_jobs<V, TRI, SL>;
     k.bounceFull = k_bounce<V, TRI, SL, kFullWaves, false, true>;
     k.bounceFullInst = k_bounce<V, TRI, SL, kFullWaves, true, true>;
+    k.bounceRefill = waves >= 8 ? k_bounce_refill<V, TRI, SL, 8, false, kShadeBatch>
+                                : (waves >= 6 ? k_bounce_refill<V, TRI, SL, 6, false, kShadeBatch> : k_bounce_refill<V, TRI, SL, 4, false, kShadeBatch>);
+    k.bounceRefillFull = k_bounce_refill<V, TRI, SL, kFullWaves, true, kShadeBatch>;
     k.bounceJobsFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true>;
 }
 template <int V, int TRI>

@@ -230,19 +230,21 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (!makeDevLights(S, up, I.S.Lt, err)) return false;
 
     if (O.fuse != 0 && O.fuse != 1) { err = "fuse must be 0 or 1"; return false; }
-    if (O.persistent && O.fuse) { err = "persistent traversal needs the split kernels (fuse 0)"; return false; }
+    if (O.persistent < 0 || O.persistent > 2) { err = "persistent must be 0, 1 (split kernels) or 2 (fused)"; return false; }
+    if (O.persistent == 1 && O.fuse) { err = "persistent traversal 1 runs the split kernels (fuse 0)"; return false; }
+    if (O.persistent == 2 && !O.fuse) { err = "persistent traversal 2 runs the fused kernel (fuse 1)"; return false; }
     if (S.variant == 2) pickKernels<2>(O.tri, I.sl, O.bounceWaves, I.kn);
     else if (S.variant == 4) pickKernels<4>(O.tri, I.sl, O.bounceWaves, I.kn);
     else pickKernels<8>(O.tri, I.sl, O.bounceWaves, I.kn);
     I.inst = S.inst != 0;
-    if (S.full && (!O.fuse || O.persistent)) {
-        err = "the full integrator runs the fused kernel (fuse 1, persistent 0)"; return false;
+    if (S.full && (!O.fuse || O.persistent == 1)) {
+        err = "the full integrator runs the fused kernel (fuse 1, persistent 0 or 2)"; return false;
     }
     I.full = S.full != 0;
     I.numEmInst = S.inst ? S.numEmInst : 0;
     I.numMats = (int)S.mats.size();
     if (S.full && !I.inst) {
-        I.bounce = I.kn.bounceFull; I.shade = nullptr;
+        I.bounce = O.persistent == 2 ? I.kn.bounceRefillFull : I.kn.bounceFull; I.shade = nullptr;
         I.shadow = O.shadowRefill ? I.kn.shadowRefill : I.kn.shadow;
     } else if (I.inst) {
         // two-level scenes run the fused bounce kernel and the one-ray-per-lane shadow kernel
@@ -257,9 +259,9 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         HIPCHK(hipHostMalloc((void**)&I.hostDepth, sizeof(int), hipHostMallocDefault));
         if (!I.alloc(I.dDepth, 1, err)) return false;
     } else {
-        I.bounce = O.fuse ? I.kn.bounce : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
+        I.bounce = O.fuse ? (O.persistent == 2 ? I.kn.bounceRefill : I.kn.bounce) : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
         I.shade = O.fuse ? nullptr : k_shade;
-        I.shadow = (O.persistent || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
+        I.shadow = (O.persistent == 1 || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
     }
     auto gridOf = [&](const void* fn) {
         int b = 0;

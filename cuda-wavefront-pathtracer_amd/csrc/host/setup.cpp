@@ -109,10 +109,13 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
     if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
-    if (A.persistent != 0 && A.persistent != 1) { err = "--persistent must be 0 or 1"; return false; }
+    if (A.persistent < 0 || A.persistent > 2) {
+        err = "--persistent must be 0, 1 (split kernels with refilled lanes) or 2 (fused kernel with refilled lanes)"; return false;
+    }
     if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
     if (A.fuse < 0) A.fuse = 1;
-    if (A.persistent) A.fuse = 0;   // persistent traversal runs in the split trace kernel
+    if (A.persistent == 1) A.fuse = 0;   // persistent traversal 1 runs in the split trace kernel
+    if (A.persistent == 2) A.fuse = 1;   // 2: the fused trace+shade kernel with refilled lanes
     if (A.bins < 2 || A.bins > 256) { err = "--bins must be in [2, 256]"; return false; }
     if (A.leaf < 1 || A.leaf > 32) { err = "--leaf must be in [1, 32]"; return false; }
     if (A.w < 1 || A.h < 1) { err = "--width/--height must be positive"; return false; }

@@ -86,7 +86,7 @@ typedef struct cwp_options {
     int batch;           /* samples per pixel in flight per wavefront pass */
     int bounces;         /* maximum bounce depth (camera hit = depth 0), 0..64 */
     int fuse;            /* 1: trace and shade in one kernel; 0: split trace and shade kernels */
-    int persistent;      /* 1: split kernels whose waves refill idle lanes mid-traversal (needs fuse 0) */
+    int persistent;      /* waves refill idle lanes with new rays: 1 split kernels (needs fuse 0), 2 fused kernel */
     int shadow_refill;   /* 1: the shadow kernel refills idle lanes mid-traversal */
     int stack_lds;       /* LDS traversal-stack entries per lane: 8, 16 or 32 (deeper entries spill to HBM) */
     int blocks_per_cu;   /* workgroups per CU of the persistent grids (0 = occupancy limit) */

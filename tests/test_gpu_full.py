@@ -61,6 +61,20 @@ def test_full_render_matches_oracle(name, args):
     assert st["shadow_unoccluded"] == pytest.approx(ost[3], rel=1e-2)
 
 
+def test_full_persistent_kernel_gives_the_same_image():
+    """the fused kernel with refilled lanes computes every path with the same arithmetic: the same image, counters"""
+    base = TERRAIN + ENV + ["--bounces", "4", "--spp", "8", "--batch", "8"]
+    out = []
+    for extra in ([], ["--persistent", "2"]):
+        r = cwp.Renderer(cwp.Setup(base + extra))
+        st = r.render(8)
+        out.append((r.accum(), st))
+        r.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in ("rays_closest", "rays_shadow", "rays_miss", "shadow_unoccluded"):
+        assert out[0][1][k] == out[1][1][k], k
+
+
 def test_full_kernel_is_deterministic_and_progressive():
     """the same image twice, and 2 x 8 samples accumulated equal 16 samples in two passes of 8"""
     s = cwp.Setup(TERRAIN + ENV + ["--bounces", "4", "--spp", "16", "--batch", "8"])

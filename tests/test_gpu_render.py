@@ -130,7 +130,8 @@ def test_kernel_schedules_give_the_same_image(scene, variant):
                             "--spp", str(spp)]
     out = {}
     schedules = [("fused", ["--fuse", "1", "--shadow_refill", "0"]), ("fused-refill", ["--fuse", "1"]),
-                 ("split", ["--fuse", "0", "--shadow_refill", "0"]), ("persistent", ["--persistent", "1"])]
+                 ("split", ["--fuse", "0", "--shadow_refill", "0"]), ("persistent", ["--persistent", "1"]),
+                 ("fused-persistent", ["--persistent", "2"])]
     for name, extra in schedules:
         s = cwp.Setup(base + extra)
         r = cwp.Renderer(s)
@@ -139,7 +140,7 @@ def test_kernel_schedules_give_the_same_image(scene, variant):
         assert r.frame_json(st)["fuse"] == (1 if name.startswith("fused") else 0)
         r.close()
     img0, st0 = out["fused"]
-    for name in ("fused-refill", "split", "persistent"):
+    for name in ("fused-refill", "split", "persistent", "fused-persistent"):
         img, st = out[name]
         np.testing.assert_array_equal(img, img0, err_msg=name)
         for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
