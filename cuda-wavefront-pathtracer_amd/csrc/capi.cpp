@@ -81,7 +81,7 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     info->full = S.full; info->num_emitters = S.inst ? S.numEmInst : (int)S.emTri.size();
     info->env_w = S.env ? S.env->w : 0; info->env_h = S.env ? S.env->h : 0; info->sun_on = S.sunOn;
     info->light_mode = S.lightMode; info->rr = S.rr; info->rr_depth = S.rrDepth; info->gd_energy = S.gdEnergy;
-    info->env_uniform = S.envUniform; info->env_rot_deg = S.envRotDeg;
+    info->env_uniform = S.envUniform; info->env_rot_deg = S.envRotDeg; info->bvh_cache = Z.bvhCacheHit;
     return CWP_OK;
 }
 

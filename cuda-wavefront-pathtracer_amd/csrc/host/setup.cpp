@@ -1,6 +1,7 @@
 // host/setup.cpp - option parsing and scene setup (see host/setup.h).
 #include "host/setup.h"
 #include "device/hploc.h"
+#include "host/bvhcache.h"
 #include "host/scene_check.h"
 #include "host/instances.h"
 #include "host/io.h"
@@ -94,6 +95,8 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--gen_env") ok = str(A.genEnv);
         else if (k == "--gen_materials") ok = inum(A.genMaterials);
         else if (k == "--graph") ok = inum(A.graph);
+        else if (k == "--bvh_cache") ok = str(A.bvhCache);
+        else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--devloop" || k == "--sort") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
@@ -260,7 +263,19 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
     if (Z.inst) for (const Mesh& m : Z.meshes) parts.push_back(&m);
     else parts.push_back(&M);
     Z.treeMs = 0; Z.postMs = 0; Z.maxDepth = 0; Z.sah = 0;
-    for (size_t k = 0; k < parts.size(); k++) {
+    // a cached build of the same meshes with the same layout and builder options replaces the build
+    Z.bvhCacheHit = -1;
+    uint64_t cacheKey = 0;
+    std::string cachePath;
+    if (!A.bvhCache.empty()) {
+        cacheKey = bvhCacheKey(parts, V, A.builder, leafMax, A.bins, Z.P.hasN);
+        cachePath = bvhCachePath(A.bvhCache, cacheKey);
+        BvhCacheEntry E;
+        Z.bvhCacheHit = loadBvhCache(cachePath, cacheKey, S, E) && S.variant == V && S.inst == (Z.inst ? 1 : 0) ? 1 : 0;
+        if (Z.bvhCacheHit) { Z.tree = std::move(E.tree); Z.maxDepth = E.maxDepth; Z.sah = E.sah; }
+        else { S = SceneUpload(); S.variant = V; S.root = 0; S.inst = Z.inst ? 1 : 0; }
+    }
+    for (size_t k = 0; Z.bvhCacheHit != 1 && k < parts.size(); k++) {
         const Mesh& mk = *parts[k];
         BinaryBVH tree;
         double treeMs = 0, postMs = 0, sah = 0;
@@ -294,6 +309,12 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
         if (k == 0) Z.tree = tree;
         Z.treeMs += treeMs; Z.postMs += postMs; Z.maxDepth = std::max(Z.maxDepth, tree.maxDepth);
         Z.sah += sah / (double)parts.size();
+    }
+    if (Z.bvhCacheHit == 0) {
+        BvhCacheEntry E;
+        E.tree = Z.tree; E.maxDepth = Z.maxDepth; E.sah = Z.sah;
+        std::string werr;
+        if (!saveBvhCache(cachePath, cacheKey, S, E, werr)) fprintf(stderr, "warning: %s\n", werr.c_str());
     }
     S.albedo.clear();
     for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
@@ -469,10 +490,12 @@ bool setupScene(SceneSetup& Z, std::string& err) {
             ew = (int)g[0]; eh = (int)g[1];
             makeLogNormalEnv(ew, eh, 0.f, (float)g[2], (uint64_t)g[3], rgb);
         } else if (!loadHDR(A.light.envMap, ew, eh, rgb, err)) return false;
+        if (!A.envOut.empty() && !writeHDR(A.envOut, ew, eh, rgb, err)) return false;   // radiance before the strength
         std::shared_ptr<EnvMap> E = std::make_shared<EnvMap>();
         buildEnvMap(rgb, ew, eh, A.light.envStrength, *E);
         Z.env = E;
     }
+    if (!A.envOut.empty() && !Z.env) { err = "--env_out needs an environment map (--env or --gen_env) of the full integrator"; return false; }
     AABB sb = meshBox(M);
     Z.extent = maxc(sb.extent());
     // ---- camera ----
@@ -526,7 +549,8 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"seed\":%u,\"integrator\":\"%s\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":%d,"
              "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
-             "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f}",
+             "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f,"
+             "\"bvh_cache\":\"%s\"}",
              rep, Z.variant, A.tri.c_str(), A.persistent, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
@@ -534,7 +558,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
              Z.treeMs, Z.postMs, Z.sah, R.options().graph, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
              A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
-             st.unoccluded);
+             st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"));
     return buf;
 }
 

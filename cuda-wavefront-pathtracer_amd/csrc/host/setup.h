@@ -32,6 +32,8 @@ struct Args {
     std::string genEnv;              // generated environment "W,H,SIGMA[,SEED]": log-normal luminance (mu 0)
     int genMaterials = -1;           // >= 0: seed of the generated OpenPBR materials of a --gen scene's boxes
     int graph = -1;                  // frames replayed from a captured HIP graph (-1: on for the full integrator)
+    std::string bvhCache;            // directory of built-scene cache files (empty: no cache)
+    std::string envOut;              // write the environment map in use (e.g. a generated one) as a Radiance HDR file
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);
@@ -59,6 +61,7 @@ struct SceneSetup {
     std::vector<Material> mats;
     std::shared_ptr<const EnvMap> env;
     int genBoxFirst = -1, genBoxCount = 0;   // generated scenes: triangle range of the boxes (12 each)
+    int bvhCacheHit = -1;            // -1: no cache directory, 0: built (and saved), 1: loaded from the cache
     // two-level scenes: the meshes (one BLAS each, material ids into M.kd) and the instances
     bool inst = false;
     std::vector<Mesh> meshes;

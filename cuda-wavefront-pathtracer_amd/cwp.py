@@ -61,7 +61,8 @@ class SceneInfo(ctypes.Structure):
                 ("frame_start", ctypes.c_int), ("full", ctypes.c_int), ("num_emitters", ctypes.c_int),
                 ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("sun_on", ctypes.c_int), ("light_mode", ctypes.c_int),
                 ("rr", ctypes.c_int), ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int),
-                ("env_uniform", ctypes.c_int), ("env_rot_deg", ctypes.c_float)]
+                ("env_uniform", ctypes.c_int), ("env_rot_deg", ctypes.c_float),
+                ("bvh_cache", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -111,6 +111,7 @@ typedef struct cwp_scene_info {
     int light_mode;     /* 0 MIS, 1 light sampling only, 2 BSDF sampling only */
     int rr, rr_depth, gd_energy, env_uniform;
     float env_rot_deg;
+    int bvh_cache;      /* -1 no cache directory (--bvh_cache), 0 built and saved, 1 loaded from the cache */
 } cwp_scene_info;
 
 /* Instance record of a two-level scene (112 bytes): world-to-object rows, object-to-world rows (3 x 4, row-major), BLAS
