@@ -502,6 +502,16 @@ int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stackNeed, double* sah) {
 }
 
 
+int cwp_bvh8_device_nodes(const float* nodes, size_t num_nodes, float* out) {
+    if ((!nodes || !out) && num_nodes) return fail(CWP_ERR_ARG, "cwp_bvh8_device_nodes: null argument");
+    std::vector<Vec4> n80(num_nodes * 5), dev;
+    if (num_nodes) memcpy(n80.data(), nodes, n80.size() * sizeof(Vec4));
+    std::string err;
+    if (!deviceBVH8(n80, dev, err)) return fail(CWP_ERR_ARG, err);
+    if (!dev.empty()) memcpy(out, dev.data(), dev.size() * sizeof(Vec4));
+    return CWP_OK;
+}
+
 static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
     o->variant = d.batch.variant; o->tri_test = d.batch.tri; o->stack_lds = d.batch.stackLds; o->bounces = d.batch.bounces;
     o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;

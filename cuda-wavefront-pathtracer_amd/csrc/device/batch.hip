@@ -1,6 +1,7 @@
 // device/batch.hip - BatchRenderer: resident scenes and one wavefront pass over many jobs (see batch.h).
 #include "device/batch.h"
 #include "device/kernels.h"
+#include "host/bvh.h"
 #include "host/scene_check.h"
 #include <algorithm>
 #include <cstddef>
@@ -189,10 +190,8 @@ int BatchRenderer::addScene(const SceneUpload& S, std::string& err) {
         return true;
     };
     std::vector<Vec4> nodes;
-    if (S.variant == 8) {   // 128-byte node stride on the device, as the renderer's
-        const size_t nn = S.nodes.size() / 5;
-        nodes.assign(nn * kBvh8Stride, v4(0.f, 0.f, 0.f, 0.f));
-        for (size_t j = 0; j < nn; j++) for (int q = 0; q < 5; q++) nodes[j * kBvh8Stride + q] = S.nodes[j * 5 + q];
+    if (S.variant == 8) {   // the 128-byte device encoding, as the renderer's
+        if (!deviceBVH8(S.nodes, nodes, err)) return -1;
     } else nodes = S.nodes;
     const Vec4 *dn, *dw, *dv, *dnr, *da;
     static const std::vector<Vec4> none;

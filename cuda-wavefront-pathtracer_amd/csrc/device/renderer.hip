@@ -21,6 +21,7 @@
 // geometric surface, throughput *= albedo.
 #include "device/hploc.h"
 #include "device/kernels.h"
+#include "host/bvh.h"
 #include "host/scene_check.h"
 #include <algorithm>
 #include <type_traits>
@@ -205,14 +206,11 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     HIPCHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&I.f0)); HIPCHK(hipEventCreate(&I.f1));
 
-    // BVH8 nodes are re-strided from 80 to 128 bytes: each node then fills exactly one 128-byte cache line instead of
-    // straddling two in 3 cases out of 5
+    // BVH8 nodes are re-encoded from 80 to 128 bytes: each node then fills exactly one 128-byte cache line instead of
+    // straddling two in 3 cases out of 5, and its planes are binary16 (one mixed FMA per plane)
     std::vector<Vec4> devNodes;
     if (S.variant == 8) {
-        const size_t nn = S.nodes.size() / 5;
-        devNodes.assign(nn * kBvh8Stride, v4(0.f, 0.f, 0.f, 0.f));
-        for (size_t j = 0; j < nn; j++)
-            for (int q = 0; q < 5; q++) devNodes[j * kBvh8Stride + q] = S.nodes[j * 5 + q];
+        if (!deviceBVH8(S.nodes, devNodes, err)) return false;
     } else devNodes = S.nodes;
     Vec4 *dn, *dw, *dv, *dnr, *da;
     if (!I.upload(dn, devNodes, err) || !I.upload(dw, S.woop, err) || !I.upload(dv, S.verts, err) ||

@@ -19,7 +19,8 @@ namespace cwp {
 
 struct Hit { float t, u, v; int tri; int inst; };   // inst: instance of the hit in two-level scenes
 
-// Device stride of a compressed BVH8 node in float4: the 80-byte node padded to one 128-byte cache line.
+// Device stride of a compressed BVH8 node in float4: the 80-byte node padded to one 128-byte cache line (deviceBVH8 on
+// the host writes it; the child base carries the leaf-slot mask in its top byte).
 constexpr int kBvh8Stride = 8;
 
 // Per-ray constants shared by every node and triangle test.
@@ -282,7 +283,8 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
     const float ox = (f0.x - r.o.x) * r.ix, oy = (f0.y - r.o.y) * r.iy, oz = (f0.z - r.o.z) * r.iz;
     const uint32_t imask = ew >> 24;
     const uint32_t meta[2] = { asUint(f1.z), asUint(f1.w) };
-    const int childBase = asInt(f1.x), triBase = asInt(f1.y);
+    const uint32_t w1 = asUint(f1.x), leafMask = w1 >> 24;   // slots holding leaves (empty slots: in neither mask)
+    const int childBase = (int)(w1 & 0xFFFFFFu), triBase = asInt(f1.y);
     const uint32_t qlx[2] = { asUint(f2.x), asUint(f2.y) }, qly[2] = { asUint(f2.z), asUint(f2.w) };
     const uint32_t qlz[2] = { asUint(f3.x), asUint(f3.y) }, qhx[2] = { asUint(f3.z), asUint(f3.w) };
     const uint32_t qhy[2] = { asUint(f4.x), asUint(f4.y) }, qhz[2] = { asUint(f4.z), asUint(f4.w) };
@@ -296,21 +298,19 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
     const uint32_t nz[2] = { r.pos & 4u ? qlz[0] : qhz[0], r.pos & 4u ? qlz[1] : qhz[1] };
     const uint32_t fz[2] = { r.pos & 4u ? qhz[0] : qlz[0], r.pos & 4u ? qhz[1] : qlz[1] };
     const float tEntry = h.t;
-    uint32_t hitSlots = 0, leafSlots = 0;
+    uint32_t hitSlots = 0;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int w = s >> 2;
-        const uint32_t m = byteOf(meta[w], s);
         const float xn = fmaf((float)byteOf(nx[w], s), sx, ox), xf = fmaf((float)byteOf(fx[w], s), sx, ox);
         const float yn = fmaf((float)byteOf(ny[w], s), sy, oy), yf = fmaf((float)byteOf(fy[w], s), sy, oy);
         const float zn = fmaf((float)byteOf(nz[w], s), sz, oz), zf = fmaf((float)byteOf(fz[w], s), sz, oz);
         const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
         const float tf = fminf(fminf(xf, yf), fminf(zf, tEntry));
-        hitSlots |= (uint32_t)(m != 0 && tn <= tf) << s;
-        leafSlots |= (m >> 7) << s;
+        hitSlots |= (uint32_t)(tn <= tf) << s;
     }
     // Phase 2: hit leaves in slot order, one shared loop
-    uint32_t lb = hitSlots & leafSlots;
+    uint32_t lb = hitSlots & leafMask;
     while (lb) {
         const int s = __builtin_ctz(lb);
         lb &= lb - 1;
@@ -319,7 +319,7 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
         for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
     }
     // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
-    uint32_t hits = hitSlots & ~leafSlots;
+    uint32_t hits = hitSlots & imask;
     if (r.oct & 1u) hits = ((hits & 0x55u) << 1) | ((hits >> 1) & 0x55u);
     if (r.oct & 2u) hits = ((hits & 0x33u) << 2) | ((hits >> 2) & 0x33u);
     if (r.oct & 4u) hits = ((hits & 0x0Fu) << 4) | ((hits >> 4) & 0x0Fu);

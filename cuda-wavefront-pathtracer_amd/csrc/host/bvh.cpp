@@ -7,6 +7,7 @@
 #include "host/bvh.h"
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <functional>
 
 namespace cwp {
@@ -500,6 +501,26 @@ int appendBVH4(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBas
     }
     return base;
 }
+bool deviceBVH8(const std::vector<Vec4>& nodes80, std::vector<Vec4>& dev, std::string& err) {
+    const size_t nn = nodes80.size() / 5;
+    if (nodes80.size() % 5) { err = "BVH8: node array is not a whole number of 80-byte nodes"; return false; }
+    if (nn > (size_t)1 << 24) { err = "BVH8: more than 2^24 nodes (the device encoding's child index has 24 bits)"; return false; }
+    dev.assign(nn * 8, v4(0.f, 0.f, 0.f, 0.f));
+    for (size_t j = 0; j < nn; j++) {
+        const Vec4* n = &nodes80[j * 5];
+        Vec4* o = &dev[j * 8];
+        const uint32_t meta[2] = { asUint(n[1].z), asUint(n[1].w) };
+        uint32_t leaf = 0;
+        for (int s = 0; s < 8; s++) leaf |= ((meta[s >> 2] >> (8 * (s & 3) + 7)) & 1u) << s;
+        const int childBase = asInt(n[1].x);
+        if (childBase < 0 || childBase >= (1 << 24)) { err = "BVH8: child base out of the 24-bit range"; return false; }
+        o[0] = n[0];
+        o[1] = v4(asFloatU((uint32_t)childBase | (leaf << 24)), n[1].y, n[1].z, n[1].w);
+        o[2] = n[2]; o[3] = n[3]; o[4] = n[4];
+    }
+    return true;
+}
+
 int appendCWBVH(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase) {
     int base = (int)all.size() / 5;
     for (size_t i = 0; i < blas.size(); i += 5) {

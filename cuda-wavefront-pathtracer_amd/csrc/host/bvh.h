@@ -9,6 +9,7 @@
 //    keeps the three corners (not edges) so the watertight test sees bit-identical shared vertices.
 #pragma once
 #include "host/mesh.h"
+#include <string>
 #include <vector>
 
 namespace cwp {
@@ -71,5 +72,10 @@ void packVerts(const Mesh& m, const std::vector<int>& order, std::vector<Vec4>& 
 int appendBVH2(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
 int appendBVH4(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
 int appendCWBVH(std::vector<Vec4>& all, const std::vector<Vec4>& blas, int triBase);
+
+// Device encoding of compressed BVH8 nodes: each 80-byte node padded to 128 bytes (8 float4, one cache line), with
+// the leaf-slot mask (slots whose metadata has the leaf bit) in the top byte of the child base, which replaces the
+// per-slot metadata tests of the traversal. Child bases must fit 24 bits (16.7 M nodes).
+bool deviceBVH8(const std::vector<Vec4>& nodes80, std::vector<Vec4>& dev, std::string& err);
 
 }  // namespace cwp

@@ -196,6 +196,7 @@ SIGNATURES = {
     "cwp_bvh_destroy": ([_P], None),
     "cwp_bvh_array": ([_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_S)], _I),
     "cwp_bvh_stats": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_bvh8_device_nodes": ([ctypes.POINTER(ctypes.c_float), _S, ctypes.POINTER(ctypes.c_float)], _I),
     "cwp_bvh_from_hploc": ([ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), _I, ctypes.POINTER(ctypes.c_float), _I,
                             _I, _I, ctypes.POINTER(_P)], _I),
     "cwp_hploc_build_host": ([ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _I,
@@ -531,6 +532,15 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def bvh8_device_nodes(nodes):
+    """Device encoding (n, 32) float32 of compressed BVH8 nodes given as (n, 20) float32 (CWP_ARR_NODES)."""
+    n = np.ascontiguousarray(nodes, dtype=np.float32).reshape(-1, 20)
+    out = np.zeros((n.shape[0], 32), dtype=np.float32)
+    _check(lib().cwp_bvh8_device_nodes(n.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n.shape[0],
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
 
 
 def bvh_build(vertices, triangles, variant=8, bins=16, leaf_max=4):

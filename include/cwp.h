@@ -234,6 +234,11 @@ void cwp_bvh_destroy(cwp_bvh* b);
 int cwp_bvh_array(const cwp_bvh* b, int which, const void** data, size_t* count);
 /* depth of the binary tree, traversal stack need, SAH cost of the layout */
 int cwp_bvh_stats(const cwp_bvh* b, int* depth, int* stack_need, double* sah);
+/* The device encoding of compressed BVH8 nodes (what cwp_create / cwp_create_scene upload): 32 floats (128 bytes) per
+ * 20-float node of CWP_ARR_NODES: the node's 20 floats, except that the child base (float 4) carries the leaf-slot
+ * mask in its top byte, then 12 floats of padding. out holds 32 * num_nodes floats. No reference counterpart (the
+ * reference uploads its 80-byte nodes); for inspection and tests. Host only. */
+int cwp_bvh8_device_nodes(const float* nodes, size_t num_nodes, float* out);
 /* The same outputs from H-PLOC nodes over the mesh's triangle boxes (8 floats per node, 2 nt - 1 nodes, children
  * before parents): SAH leaf collapse (leaves <= leaf_max, 1 for variant 8), then the layout. Replaces treeFromHploc
  * + BVH2Flat / CWBVH::build (src/accel/bvh_tools.cpp:12-38, src/app/scene_setup.cpp:76-83). Host only. */

@@ -101,6 +101,29 @@ def test_product_bvh_matches_oracle_bit_exact(gen, variant, bins):
     assert s.info["bvh_depth"] == b["depth"]
 
 
+@pytest.mark.parametrize("gen", ["boxes", "terrain"])
+def test_bvh8_device_encoding_is_exact(gen):
+    """The device encoding of BVH8 nodes carries the 80-byte node's fields exactly; the top byte of the child base is
+    the mask of the slots whose metadata has the leaf bit."""
+    args = ["--gen", gen, "--width", "8", "--height", "8"] + (["--gen_grid", "61", "--gen_boxes", "20"] if gen == "terrain" else [])
+    n80 = cwp.Setup(args).array(cwp.ARR_NODES).reshape(-1, 20)
+    dev = cwp.bvh8_device_nodes(n80)
+    u80, ud = n80.view(np.uint32), dev.view(np.uint32)
+    np.testing.assert_array_equal(ud[:, 0:4], u80[:, 0:4])
+    np.testing.assert_array_equal(ud[:, 5:8], u80[:, 5:8])
+    np.testing.assert_array_equal(ud[:, 4] & 0xFFFFFF, u80[:, 4])
+    meta = n80[:, 6:8].copy().view(np.uint8).reshape(-1, 8)
+    leaf = ((meta >> 7) & 1).astype(np.uint32) @ (1 << np.arange(8, dtype=np.uint32))
+    np.testing.assert_array_equal(ud[:, 4] >> 24, leaf)
+    assert leaf.any() and (u80[:, 3] >> 24).any()
+    np.testing.assert_array_equal(ud[:, 8:20], u80[:, 8:20])
+    assert not ud[:, 20:].any()
+    bad = n80.copy()
+    bad[0, 4] = np.array([1 << 24], dtype=np.int32).view(np.float32)[0]
+    with pytest.raises(cwp.CwpError, match="24-bit"):
+        cwp.bvh8_device_nodes(bad)
+
+
 def test_bvh_build_api_matches_oracle():
     rng = np.random.default_rng(9)
     c = rng.random((700, 3)).astype(np.float32)
