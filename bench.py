@@ -141,6 +141,42 @@ def bench_instances(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_hploc(args):
+    """H-PLOC GPU build over the triangle boxes of heightfield meshes the size of the reference's published builds
+    (2.02 M and 2.80 M triangles): device time of one build with the boxes resident (hipEvents, median of 3 builds), and
+    the host post-processing to a CWBVH layout (SAH leaf collapse + 8-wide collapse + triangle packing)."""
+    import cwp
+    rows = []
+    for g in (1005, 1184):
+        rng = np.random.default_rng(g)
+        xs = np.linspace(0.0, 1.0, g, dtype=np.float32)
+        X, Y = np.meshgrid(xs, xs, indexing="xy")
+        Z = (0.05 * rng.standard_normal((g, g))).astype(np.float32)
+        v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+        i = np.arange(g * g, dtype=np.int32).reshape(g, g)
+        a, b, c, d = i[:-1, :-1].ravel(), i[:-1, 1:].ravel(), i[1:, :-1].ravel(), i[1:, 1:].ravel()
+        t = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.int32)
+        corners = v[t]
+        lo, hi = corners.min(1), corners.max(1)
+        ms = []
+        for _ in range(3):
+            nodes, root, gpu_ms = cwp.hploc_build(lo, hi)
+            ms.append(gpu_ms)
+        t0 = time.perf_counter()
+        bv = cwp.bvh_from_hploc(v, t, nodes, root, 8)
+        post = (time.perf_counter() - t0) * 1e3
+        rows.append({"tris": int(len(t)), "gpu_build_ms": round(float(np.median(ms)), 3), "gpu_build_ms_all": [round(x, 3) for x in ms],
+                     "host_post_ms": round(post, 1), "cwbvh_nodes": int(len(bv["nodes"]))})
+        print("bench: H-PLOC %d tris: %.3f ms on the GPU, %.0f ms host post-processing" % (len(t), rows[-1]["gpu_build_ms"], post),
+              file=sys.stderr, flush=True)
+    out = {"metric": "H-PLOC GPU BVH build ms (boxes resident), host post-processing ms", "value": rows[-1]["gpu_build_ms"],
+           "unit": "ms", "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": rows[-1]["gpu_build_ms"], "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "hploc-heightfield-2.02M-2.80M", "builds": rows},
+           "reference_published_h800_ms": {"2.02M": 1.29, "2.80M": 1.74, "host_post_ms": "226-404"}}
+    print(json.dumps(out), flush=True)
+
+
 JOB_RES = (128, 256, 512, 1024)
 JOB_SPP = (4, 16, 64)
 
@@ -316,11 +352,11 @@ def main():
                     "trace / shade kernels, 2 the fused kernel")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--cpu-spp", type=int, default=40)
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve"],
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve", "hploc"],
                     help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
                          "instances: the two-level scene with the TLAS rebuilt on the GPU every frame (1,000 instances "
                          "of 5 meshes, 720p, 64 spp, 60 frames); jobs: the mixed job stream only, batched vs plain; "
-                         "serve: the job stream with Poisson arrivals at each --rates")
+                         "serve: the job stream with Poisson arrivals at each --rates; hploc: GPU BVH build times")
     ap.add_argument("--jobs", type=int, default=64, help="job stream: jobs")
     ap.add_argument("--job-scenes", type=int, default=8, help="job stream: scenes (10k-2M triangles, log-uniform)")
     ap.add_argument("--job-seed", type=int, default=1)
@@ -350,6 +386,9 @@ def main():
     import cwp
     if args.workload == "instances":
         bench_instances(args)
+        return
+    if args.workload == "hploc":
+        bench_hploc(args)
         return
     if args.workload in ("jobs", "serve"):
         j = bench_jobs(args)

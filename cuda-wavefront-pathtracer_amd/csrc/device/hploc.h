@@ -35,6 +35,8 @@ public:
     bool build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err);
     const Vec4* nodes() const { return nodes_; }   // device, 2 float4 per node
     const uint64_t* sortedKeys() const { return keysA_; }
+    // device counter of allocated node ids: 2n - 1 after a consistent build of n > 1 primitives
+    const int* idCounter() const { return counter_; }
     void release();
     int capacity() const { return cap_; }
 
@@ -43,6 +45,7 @@ private:
     Vec4* nodes_ = nullptr;
     uint64_t *keysA_ = nullptr, *keysB_ = nullptr;
     int *clusters_ = nullptr, *bounds_ = nullptr, *counter_ = nullptr, *counts_ = nullptr;
+    Vec4* cboxes_ = nullptr;   // box of the cluster in each slot of clusters_ (2 float4)
     uint32_t* cbox_ = nullptr;
     int tiles_ = 0;
 };

@@ -49,13 +49,34 @@ __device__ __forceinline__ float halfArea(Vec3 lo, Vec3 hi) {
 }
 __device__ __forceinline__ Vec3 shfl3(Vec3 v, int src) { return v3(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)); }
 
+// Data handed between waves of the climb (cluster ids, node boxes) is read and written with relaxed device-scope
+// atomic accesses: on gfx950 these carry the device-coherence bit, so a wave on one XCD sees what a wave on another
+// XCD wrote once the hand-over atomic orders them, without the whole-L2 write-back and invalidate of an agent-scope
+// fence (the L2s of the 8 XCDs are not coherent with each other). Workgroup-scope fences keep the compiler and the
+// memory counters in order around the hand-over.
+constexpr int kWaitVmcnt0 = 0x0F70;   // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ int ldC(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void stC(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ Vec3 ldBox3(const Vec4* p) {   // xyz of a node record
+    const uint64_t a = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t z = __hip_atomic_load((const uint32_t*)p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v3(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)), __uint_as_float(z));
+}
+__device__ __forceinline__ void stNode(Vec4* p, Vec4 v) {
+    const uint64_t a = (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32);
+    const uint64_t b = (uint64_t)__float_as_uint(v.z) | ((uint64_t)__float_as_uint(v.w) << 32);
+    __hip_atomic_store((uint64_t*)p, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((uint64_t*)p + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------------------------------------------ keys
 __global__ void k_hp_init(uint32_t* cbox, int* counter, int n) {
     if (threadIdx.x < 3) { cbox[threadIdx.x] = 0xffffffffu; cbox[3 + threadIdx.x] = 0u; }
     if (threadIdx.x == 0) *counter = n;
 }
 
-__global__ void k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
+__global__ void __launch_bounds__(256) k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
+    __shared__ uint32_t red[4][6];
     uint32_t a[3] = { 0xffffffffu, 0xffffffffu, 0xffffffffu }, b[3] = { 0u, 0u, 0u };
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const Vec4 l = lo[i], h = hi[i];
@@ -67,8 +88,16 @@ __global__ void k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict_
             a[k] = min(a[k], (uint32_t)__shfl_xor((int)a[k], off));
             b[k] = max(b[k], (uint32_t)__shfl_xor((int)b[k], off));
         }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 3; k++) { atomicMin(&cbox[k], a[k]); atomicMax(&cbox[3 + k], b[k]); }
+        for (int k = 0; k < 3; k++) { red[w][k] = a[k]; red[w][3 + k] = b[k]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {   // one atomic per workgroup and bound
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) v = k < 3 ? min(v, red[q][k]) : max(v, red[q][k]);
+        if (k < 3) atomicMin(&cbox[k], v); else atomicMax(&cbox[k], v);
+    }
 }
 
 // key = Morton code of the box centroid (mb bits per axis, x << 2 | y << 1 | z) << ib | primitive index
@@ -138,31 +167,33 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_count(const uint64_t* __restr
     counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// exclusive scan of counts[0, total) in place (one workgroup of 1024 threads)
-__global__ void __launch_bounds__(1024) k_rs_scan(int* counts, int total) {
-    __shared__ int part[1024];
-    const int per = (total + 1023) / 1024;
-    const int b = threadIdx.x * per, e = min(b + per, total);
-    int s = 0;
-    for (int j = b; j < e; j++) s += counts[j];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {   // inclusive Hillis-Steele scan of the partial sums
-        const int y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += y;
-        __syncthreads();
+// per digit (one workgroup per digit row of counts): exclusive scan of the row's tile counts in place, row total to
+// totals[digit]
+__global__ void __launch_bounds__(kSortBlock) k_rs_rowscan(int* counts, int tiles, int* totals) {
+    __shared__ int tmp[4];
+    int* row = counts + (size_t)blockIdx.x * tiles;
+    int carry = 0;
+    for (int b = 0; b < tiles; b += kSortBlock) {
+        const int j = b + (int)threadIdx.x;
+        const int v = j < tiles ? row[j] : 0;
+        int total;
+        const int pre = blockScan256(v, tmp, total);
+        if (j < tiles) row[j] = carry + pre;
+        carry += total;
     }
-    int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-    for (int j = b; j < e; j++) { const int c = counts[j]; counts[j] = run; run += c; }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-// stable scatter of one tile by the digit at `shift`, at the offsets scanned from k_rs_count
+// stable scatter of one tile by the digit at `shift`: the tile's start for digit d is the exclusive scan of the digit
+// totals (computed here by every workgroup) plus the row prefix of k_rs_rowscan
 __global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int n,
-                                                           int shift, const int* __restrict__ offsets, int tiles) {
-    __shared__ int base[256], running[256], wh[4][256];
+                                                           int shift, const int* __restrict__ offsets, int tiles,
+                                                           const int* __restrict__ totals) {
+    __shared__ int base[256], running[256], wh[4][256], tmp[4];
     const int t = threadIdx.x, w = t >> 6;
-    base[t] = offsets[t * tiles + blockIdx.x];
+    int all;
+    const int digitStart = blockScan256(totals[t], tmp, all);
+    base[t] = digitStart + offsets[t * tiles + blockIdx.x];
     running[t] = 0;
     const int tile = blockIdx.x * kTile;
     for (int j = 0; j < kSortItems; j++) {
@@ -230,7 +261,7 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_single(uint64_t* a, uint64_t*
 
 // ------------------------------------------------------------------------------------------------ H-PLOC
 __global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, const uint64_t* __restrict__ keys,
-                            int n, int ib, Vec4* nodes, int* C, int* bounds) {
+                            int n, int ib, Vec4* nodes, int* C, Vec4* CB, int* bounds) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = (int)(keys[i] & ((1ull << ib) - 1ull));
@@ -238,35 +269,64 @@ __global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict_
     nodes[2 * i] = v4(xyz(l), asFloat(p));
     nodes[2 * i + 1] = v4(xyz(h), asFloat(-1));
     C[i] = i;
+    CB[2 * i] = v4(xyz(l), 0.f);
+    CB[2 * i + 1] = v4(xyz(h), 0.f);
     bounds[i] = -1;
 }
 
+// Per half-wave LDS of a merge: compaction buffers (ids, boxes) and the nodes created by the merge, which get their
+// global ids only when the merge is done (one counter reservation per merge instead of one per iteration).
+struct MergeLds {
+    int id[32];
+    float box[32 * 6];
+    float node[32 * 8];   // lo xyz, hi xyz, left id bits, right id bits
+};
+
 // PLOC on the clusters of LBVH node [L, R] split at s, run by one half-wave (32 lanes; both halves call together, a
-// half with on = false idles). Its clusters sit at the start of each child's range in C (-1 = empty slot): up to 16
-// from each side, compacted into lanes in order. Each iteration: every lane finds the pair of smallest merged
-// half-area among the lanes within kR of it (pairs ordered by (area bits, lower lane, upper lane), so the global best
-// pair is mutual and every iteration merges); mutual pairs merge into a new node (lower lane = left child, id from the
-// counter in lane order); survivors are compacted in lane order. Stops at kT clusters (1 at the root); the survivors
-// are written back to the start of [L, R].
-__device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, int* C, int* counter, int* shId, float* shB) {
+// half with on = false idles). Its clusters sit at the start of each child's range in C (ids) and CB (boxes) (-1 =
+// empty slot): up to 16 from each side, compacted into lanes in order. Each iteration: every lane finds the pair of
+// smallest merged half-area among the lanes within kR of it (pairs ordered by (area bits, lower lane, upper lane), so
+// the global best pair is mutual and every iteration merges); mutual pairs merge into a new node (lower lane = left
+// child), numbered locally (-2 - k) in lane order; survivors are compacted in lane order. Stops at kT clusters (1 at
+// the root). Then the merge's nodes take consecutive global ids from the counter (local order, so children stay
+// below parents and the root's last merge is node 2n - 2) and the survivors go back to the start of [L, R].
+__device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, int* C, Vec4* CB, int* counter,
+                          MergeLds& sh, int cap) {
     const int lane = (int)threadIdx.x & 63, hl = lane & 31, hb = lane & 32;
     const uint64_t ltMask = (1ull << lane) - 1ull, halfMask = 0xffffffffull << hb;
     int nL = 0, nR = 0;
     if (on) { nL = min(s - L + 1, kT); nR = min(R - s, kT); }
     const int slot = hl < nL ? L + hl : (hl < nL + nR ? s + 1 + (hl - nL) : -1);
-    int id = slot >= 0 ? C[slot] : -1;
+    int id = -1;
+    Vec3 lo = v3(0.f, 0.f, 0.f), hi = lo;
+    if (slot >= 0) {   // id and box of the slot, read together
+        id = ldC(C + slot);
+        lo = ldBox3(CB + 2 * slot); hi = ldBox3(CB + 2 * slot + 1);
+    }
     const uint64_t vm = __ballot(id >= 0) & halfMask;
     int num = (int)__builtin_popcountll(vm);
-    if (id >= 0) shId[__builtin_popcountll(vm & ltMask)] = id;
+    if (id >= 0) {
+        const int dd = (int)__builtin_popcountll(vm & ltMask);
+        sh.id[dd] = id;
+        float* b = sh.box + 6 * dd;
+        b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
+    }
     __syncthreads();
-    id = hl < num ? shId[hl] : -1;
+    id = -1;
+    if (hl < num) {
+        id = sh.id[hl];
+        const float* b = sh.box + 6 * hl;
+        lo = v3(b[0], b[1], b[2]); hi = v3(b[3], b[4], b[5]);
+    }
     __syncthreads();
-    Vec3 lo = v3(0.f, 0.f, 0.f), hi = lo;
-    if (id >= 0) { lo = xyz(nodes[2 * id]); hi = xyz(nodes[2 * id + 1]); }
     const int target = root ? 1 : kT;
+    int made = 0;   // nodes this half created so far (uniform per half)
     while (__ballot(num > target)) {
         const bool act = num > target;
         uint64_t best = ~0ull;
+        // not unrolled further: the fully unrolled search holds every neighbour's box at once (102 registers, 4 waves
+        // per SIMD); two radii per trip keep the kernel at 8 waves
+#pragma unroll 2
         for (int r = 1; r <= kR; r++) {
             const Vec3 olo = shfl3(lo, hb + min(hl + r, 31)), ohi = shfl3(hi, hb + min(hl + r, 31));
             uint64_t key = ~0ull;
@@ -284,54 +344,73 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, 
         const Vec3 plo = shfl3(lo, hb + nn), phi = shfl3(hi, hb + nn);
         const int pid = __shfl(id, hb + nn);
         const uint64_t lm = __ballot(lead) & halfMask;
-        int base = 0;
-        if (hl == 0 && lm) base = atomicAdd(counter, (int)__builtin_popcountll(lm));
-        base = __shfl(base, hb);
         if (lead) {
-            const int nid = base + (int)__builtin_popcountll(lm & ltMask);
+            const int k = made + (int)__builtin_popcountll(lm & ltMask);
             lo = vmn(lo, plo); hi = vmx(hi, phi);
-            nodes[2 * nid] = v4(lo, asFloat(id));
-            nodes[2 * nid + 1] = v4(hi, asFloat(pid));
-            id = nid;
+            float* q = sh.node + 8 * k;
+            q[0] = lo.x; q[1] = lo.y; q[2] = lo.z; q[3] = hi.x; q[4] = hi.y; q[5] = hi.z;
+            q[6] = __int_as_float(id); q[7] = __int_as_float(pid);
+            id = -2 - k;
         }
+        made += (int)__builtin_popcountll(lm);
         const bool keep = act && hl < num && !(mutual && hl > nn);
         const uint64_t km = __ballot(keep) & halfMask;
         if (keep) {
             const int dd = (int)__builtin_popcountll(km & ltMask);
-            shId[dd] = id;
-            float* b = shB + 6 * dd;
+            sh.id[dd] = id;
+            float* b = sh.box + 6 * dd;
             b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
         }
         __syncthreads();
         if (act) {
             num = (int)__builtin_popcountll(km);
             if (hl < num) {
-                id = shId[hl];
-                const float* b = shB + 6 * hl;
+                id = sh.id[hl];
+                const float* b = sh.box + 6 * hl;
                 lo = v3(b[0], b[1], b[2]); hi = v3(b[3], b[4], b[5]);
             } else id = -1;
         }
         __syncthreads();
     }
-    if (slot >= 0) C[slot] = -1;
+    // global ids: one reservation for both halves' nodes
+    const int made0 = __shfl(made, 0), made1 = __shfl(made, 32);
+    int base = 0;
+    if (lane == 0 && made0 + made1 > 0) base = atomicAdd(counter, made0 + made1);
+    base = __shfl(base, 0) + (hb ? made0 : 0);
+    auto global = [&](int x) { return x >= 0 ? x : base + (-2 - x); };
+    if (hl < made && base + hl < cap) {   // a consistent build allocates exactly cap - n internal nodes
+        const float* q = sh.node + 8 * hl;
+        stNode(nodes + 2 * (base + hl), v4(q[0], q[1], q[2], asFloat(global(__float_as_int(q[6])))));
+        stNode(nodes + 2 * (base + hl) + 1, v4(q[3], q[4], q[5], asFloat(global(__float_as_int(q[7])))));
+    }
+    if (slot >= 0) stC(C + slot, -1);
     __syncthreads();
-    if (on && hl < num) C[L + hl] = id;
+    if (on && hl < num) {
+        const int g = global(id);
+        stC(C + L + hl, g < cap ? g : -1);
+        stNode(CB + 2 * (L + hl), v4(lo, 0.f));
+        stNode(CB + 2 * (L + hl) + 1, v4(hi, 0.f));
+    }
+    __syncthreads();
 }
 
 // One lane per sorted primitive climbs the implicit LBVH: the parent of range [L, R] is [L, R'] (this range is the
 // left child, split at R) when L == 0 or the key delta at R is smaller than at L - 1, else [L', R] (split at L - 1).
 // The lane exchanges its range end into bounds[split]; the first lane to arrive stops, the second takes the union
 // range and, when it holds more than kT clusters or is the root, the wave runs the PLOC merge on it, two ranges at a
-// time. Fences order the cluster and node writes before the hand-over and the reads after it, across XCDs.
-__global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __restrict__ keys, int* C, int* bounds, int* counter, int n) {
-    __shared__ int shId[2][32];
-    __shared__ float shB[2][32 * 6];
+// time. Cluster and node records cross waves (and XCDs) through device-coherent atomic accesses; fences order them
+// before the hand-over and after it.
+__global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __restrict__ keys, int* C, Vec4* CB, int* bounds,
+                                              int* counter, int n) {
+    __shared__ MergeLds sh[2];
     const int lane = (int)threadIdx.x, half = lane >> 5;
     const int i = blockIdx.x * 64 + lane;
     bool active = i < n;
     int L = i, R = i;
     while (__ballot(active)) {
-        __threadfence();   // release: this wave's earlier merges before the hand-overs below
+        // this wave's earlier merges before the hand-overs: every store performed (vmcnt 0) at the coherence point
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         int s = 0;
         bool mrg = false;
         if (active) {
@@ -346,7 +425,7 @@ __global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __res
             if (prev == -1) active = false;   // first to arrive: the sibling's lane continues
             else mrg = (R - L + 1 > kT) || (L == 0 && R == n - 1);
         }
-        __threadfence();   // acquire: the sibling subtree's clusters and nodes
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // the sibling subtree's clusters and nodes after them
         uint64_t mm = __ballot(mrg);
         while (mm) {
             const int a = __builtin_ctzll(mm);
@@ -356,7 +435,7 @@ __global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __res
             const int src = half == 0 ? a : b;
             const int sl = src < 0 ? 0 : src;
             const int l = __shfl(L, sl), r = __shfl(R, sl), sp = __shfl(s, sl);
-            plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, nodes, C, counter, shId[half], shB[half]);
+            plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, nodes, C, CB, counter, sh[half], 2 * n - 1);
         }
         if (active && L == 0 && R == n - 1) active = false;
     }
@@ -404,7 +483,8 @@ HplocBuilder::~HplocBuilder() { release(); }
 void HplocBuilder::release() {
     if (!cap_) return;
     hipFree(nodes_); hipFree(keysA_); hipFree(keysB_); hipFree(clusters_); hipFree(bounds_); hipFree(counter_);
-    hipFree(cbox_); hipFree(counts_);
+    hipFree(cbox_); hipFree(counts_); hipFree(cboxes_);
+    cboxes_ = nullptr;
     nodes_ = nullptr; keysA_ = keysB_ = nullptr; clusters_ = bounds_ = counter_ = counts_ = nullptr; cbox_ = nullptr;
     cap_ = 0; tiles_ = 0;
 }
@@ -418,10 +498,11 @@ bool HplocBuilder::reserve(int n, std::string& err) {
     HCHK(hipMalloc(&keysA_, (size_t)n * 8));
     HCHK(hipMalloc(&keysB_, (size_t)n * 8));
     HCHK(hipMalloc(&clusters_, (size_t)n * 4));
+    HCHK(hipMalloc(&cboxes_, (size_t)n * 2 * sizeof(Vec4)));
     HCHK(hipMalloc(&bounds_, (size_t)n * 4));
     HCHK(hipMalloc(&counter_, 4));
     HCHK(hipMalloc(&cbox_, 24));
-    HCHK(hipMalloc(&counts_, (size_t)256 * tiles_ * 4));
+    HCHK(hipMalloc(&counts_, (size_t)256 * (tiles_ + 1) * 4));   // digit x tile counts, then the 256 digit totals
     cap_ = n;
     return true;
 }
@@ -433,7 +514,7 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     hipStream_t st = (hipStream_t)stream;
     const int g = (n + 255) / 256;
     k_hp_init<<<1, 64, 0, st>>>(cbox_, counter_, n);
-    k_hp_bounds<<<std::min(g, 2048), 256, 0, st>>>(lo, hi, n, cbox_);
+    k_hp_bounds<<<std::min(g, 512), 256, 0, st>>>(lo, hi, n, cbox_);
     int ib = 1;
     while ((1ll << ib) < n) ib++;
     const int mb = (64 - ib) / 3 < 21 ? (64 - ib) / 3 : 21;
@@ -448,14 +529,14 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
         uint64_t *src = keysA_, *dst = keysB_;
         for (int p = 0; p < passes; p++) {
             k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, 8 * p, counts_, tiles);
-            k_rs_scan<<<1, 1024, 0, st>>>(counts_, 256 * tiles);
-            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, 8 * p, counts_, tiles);
+            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, tiles, counts_ + 256 * (size_t)tiles);
+            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, 8 * p, counts_, tiles, counts_ + 256 * (size_t)tiles);
             std::swap(src, dst);
         }
         sorted = src;
     }
-    k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, bounds_);
-    if (n > 1) k_hploc<<<(n + 63) / 64, 64, 0, st>>>(nodes_, sorted, clusters_, bounds_, counter_, n);
+    k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_);
+    if (n > 1) k_hploc<<<(n + 63) / 64, 64, 0, st>>>(nodes_, sorted, clusters_, cboxes_, bounds_, counter_, n);
     HCHK(hipGetLastError());
     root = n > 1 ? 2 * n - 2 : 0;
     return true;
@@ -488,6 +569,9 @@ bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& 
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
         gpuMs = ms;
+        int ids = 0;
+        if (hipMemcpy(&ids, B.idCounter(), 4, hipMemcpyDeviceToHost) != hipSuccess) { err = "H-PLOC: download failed"; break; }
+        if (n > 1 && ids != 2 * n - 1) { err = "H-PLOC: inconsistent build (" + std::to_string(ids) + " node ids for " + std::to_string(n) + " primitives)"; break; }
         if (hipMemcpy(nodesOut, B.nodes(), (size_t)2 * (2 * (size_t)n - 1) * 16, hipMemcpyDeviceToHost) != hipSuccess) {
             err = "H-PLOC: download failed"; break;
         }
