@@ -3,6 +3,7 @@
 #include "device/renderer.h"
 #include "host/bvh.h"
 #include "host/scene_check.h"
+#include "device/collapse.h"
 #include "device/hploc.h"
 #include "host/serve.h"
 #include "host/setup.h"
@@ -456,6 +457,34 @@ int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes) {
 int cwp_hploc_to_bvh2(const void* hnodes, int n, void* out, void* stream) {
     std::string err;
     if (!hplocToBVH2((const Vec4*)hnodes, n, (Vec4*)out, stream, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_cwbvh_from_hploc(const void* hnodes, int n, void* nodes, int* num_nodes, void* prims, int* depth, void* stream) {
+    if (!hnodes || !nodes || !prims || !num_nodes || n < 1) return fail(CWP_ERR_ARG, "cwp_cwbvh_from_hploc: bad arguments");
+    // scratch kept between calls of a thread; never destroyed (the device memory goes with the process, and a
+    // destructor at exit could run after the HIP runtime has shut down)
+    static thread_local CwbvhCollapser* collapser = new CwbvhCollapser();
+    std::string err;
+    int d = 0;
+    if (!collapser->run((const Vec4*)hnodes, n, (Vec4*)nodes, (int*)prims, stream, *num_nodes, d, err)) return fail(CWP_ERR_DEVICE, err);
+    if (depth) *depth = d;
+    return CWP_OK;
+}
+
+int cwp_cwbvh_from_hploc_host(const float* hnodes, int n, float* nodes, int* num_nodes, int* prims, int* depth, double* gpu_ms) {
+    if (!hnodes || !nodes || !prims || !num_nodes || n < 1) return fail(CWP_ERR_ARG, "cwp_cwbvh_from_hploc_host: bad arguments");
+    std::vector<Vec4> out;
+    std::vector<int> pr;
+    int d = 0;
+    double ms = 0;
+    std::string err;
+    if (!cwbvhFromHplocHost((const Vec4*)hnodes, n, out, pr, d, ms, err)) return fail(CWP_ERR_DEVICE, err);
+    memcpy(nodes, out.data(), out.size() * sizeof(Vec4));
+    memcpy(prims, pr.data(), pr.size() * sizeof(int));
+    *num_nodes = (int)(out.size() / 5);
+    if (depth) *depth = d;
+    if (gpu_ms) *gpu_ms = ms;
     return CWP_OK;
 }
 

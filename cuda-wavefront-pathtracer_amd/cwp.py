@@ -206,6 +206,9 @@ SIGNATURES = {
     "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),
     "cwp_hploc_nodes": ([_P, ctypes.POINTER(_P)], _I),
     "cwp_hploc_to_bvh2": ([_P, _I, _P, _P], _I),
+    "cwp_cwbvh_from_hploc": ([_P, _I, _P, ctypes.POINTER(_I), _P, ctypes.POINTER(_I), _P], _I),
+    "cwp_cwbvh_from_hploc_host": ([ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I),
+                                   ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
     "cwp_set_emitters": ([_P, ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(ctypes.c_float),
                           ctypes.POINTER(ctypes.c_float), _I], _I),
     "cwp_setup_emitters": ([_P, _I, ctypes.POINTER(ctypes.c_float), _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float),
@@ -590,6 +593,18 @@ def hploc_build(lo, hi):
     root, ms = ctypes.c_int(), ctypes.c_double()
     _check(lib().cwp_hploc_build_host(_fptr(lo), _fptr(hi), n, _fptr(nodes), ctypes.byref(root), ctypes.byref(ms)))
     return nodes, root.value, ms.value
+
+
+def cwbvh_from_hploc_gpu(hnodes, n):
+    """Compressed BVH8 from H-PLOC nodes on the GPU. Returns (nodes (m, 20) float32, order (n,) int32, depth, gpu_ms)."""
+    h = np.ascontiguousarray(hnodes, dtype=np.float32).reshape(-1, 8)
+    assert h.shape[0] == 2 * n - 1
+    nodes = np.zeros((n, 20), np.float32)
+    prims = np.zeros(n, np.int32)
+    m, d, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().cwp_cwbvh_from_hploc_host(_fptr(h), n, _fptr(nodes), ctypes.byref(m), prims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           ctypes.byref(d), ctypes.byref(ms)))
+    return nodes[:m.value].copy(), prims, d.value, ms.value
 
 
 def bvh_from_hploc(vertices, triangles, hnodes, root, variant=8, leaf_max=4):

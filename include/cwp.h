@@ -263,6 +263,15 @@ int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes);
 /* 64 B BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) of device H-PLOC nodes into `out`
  * (device). Replaces hplocToBVH2 (src/accel/hploc.h:36). */
 int cwp_hploc_to_bvh2(const void* hnodes, int n, void* out, void* stream);
+/* Compressed BVH8 (80-byte nodes, CWP_ARR_NODES layout) from H-PLOC nodes of n primitives, on the device: the
+ * collapse of cwp_bvh_from_hploc with variant 8 (one primitive per binary leaf, SAH-optimal 8-wide collapse, octant
+ * slot order, quantised child boxes), bit-identical to it. hnodes, nodes (room for 20 floats per primitive) and prims
+ * (n ints: the leaf primitive order, i.e. CWP_ARR_ORDER) are device pointers; the call synchronises with `stream`
+ * once per level of the wide tree. Replaces treeFromHploc + CWBVH::build on the host (src/accel/bvh_tools.cpp:12-168,
+ * src/app/scene_setup.cpp:76-83). */
+int cwp_cwbvh_from_hploc(const void* hnodes, int n, void* nodes, int* num_nodes, void* prims, int* depth, void* stream);
+/* Host convenience for the same: uploads, collapses (timed with events: gpu_ms), downloads. */
+int cwp_cwbvh_from_hploc_host(const float* hnodes, int n, float* nodes, int* num_nodes, int* prims, int* depth, double* gpu_ms);
 
 /* ---- render-job server: batched multi-job passes, continuous batching, resident scenes with LRU eviction ----
  * Jobs (scene, resolution, samples, bounce limit, seed, arrival time) are admitted in arrival order once arrived,
