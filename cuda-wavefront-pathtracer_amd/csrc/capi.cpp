@@ -479,7 +479,8 @@ int cwp_cwbvh_from_hploc_host(const float* hnodes, int n, float* nodes, int* num
     int d = 0;
     double ms = 0;
     std::string err;
-    if (!cwbvhFromHplocHost((const Vec4*)hnodes, n, out, pr, d, ms, err)) return fail(CWP_ERR_DEVICE, err);
+    float sah = 0.f;
+    if (!cwbvhFromHplocHost((const Vec4*)hnodes, n, out, pr, d, ms, sah, err)) return fail(CWP_ERR_DEVICE, err);
     memcpy(nodes, out.data(), out.size() * sizeof(Vec4));
     memcpy(prims, pr.data(), pr.size() * sizeof(int));
     *num_nodes = (int)(out.size() / 5);

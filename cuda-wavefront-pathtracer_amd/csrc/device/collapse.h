@@ -36,18 +36,24 @@ public:
     // primOut: device, n ints. Runs on `stream` and synchronises with it once per wide-tree level. numNodes: the
     // wide nodes written; depth: wide-tree levels.
     bool run(const Vec4* hnodes, int n, Vec4* nodesOut, int* primOut, void* stream, int& numNodes, int& depth, std::string& err);
+    // the root's cost as one wide root (after run): the collapse's SAH is this over the root box's area
+    float rootCost() const { return rootCost_; }
+    bool rootIsLeaf() const { return rootLeaf_; }
     void release();
 
 private:
     bool reserve(int n, std::string& err);
     int cap_ = 0;
     void* mem_ = nullptr;   // one allocation holding every scratch array
+    float rootCost_ = 0.f;
+    bool rootLeaf_ = false;
     size_t bytes_ = 0;
 };
 
 // Host convenience: uploads the H-PLOC nodes, collapses on the device (timed with events), downloads the 80-byte
 // nodes and the primitive order.
+// sah: the collapse's SAH cost normalised by the root area (WideBVH::sah of the host collapse).
 bool cwbvhFromHplocHost(const Vec4* hnodes, int n, std::vector<Vec4>& nodes, std::vector<int>& prim, int& depth,
-                        double& gpuMs, std::string& err);
+                        double& gpuMs, float& sah, std::string& err);
 
 }  // namespace cwp

@@ -303,6 +303,12 @@ __global__ void k_cb_root(int n, Scratch W) {
         if (e_ != hipSuccess) { err = std::string(#x) + ": " + hipGetErrorString(e_); return false; }                  \
     } while (0)
 
+float hostArea(Vec3 lo, Vec3 hi) {
+    if (hi.x < lo.x) return 0.f;
+    const Vec3 d = hi - lo;
+    return 2.f * (d.x * d.y + d.y * d.z + d.z * d.x);
+}
+
 Scratch carve(void* mem, int n) {
     const size_t N = 2 * (size_t)n - 1, M = (size_t)n;
     char* p = (char*)mem;
@@ -364,8 +370,10 @@ bool CwbvhCollapser::run(const Vec4* hn, int n, Vec4* nodesOut, int* primOut, vo
     // root as a single leaf: the root's table chose a leaf (at most pMax primitives), or the root is a primitive
     unsigned char rootDec[2] = { 0, 0 };
     CCHK(hipMemcpyAsync(rootDec, W.dec + 8 * (size_t)(N - 1), 2, hipMemcpyDeviceToHost, st));
+    CCHK(hipMemcpyAsync(&rootCost_, W.cost + 8 * (size_t)(N - 1) + 1, 4, hipMemcpyDeviceToHost, st));
     CCHK(hipStreamSynchronize(st));
     const int rootLeaf = n == 1 || rootDec[1] == kDecLeaf;
+    rootLeaf_ = rootLeaf != 0;
     std::vector<int> starts = { 0 };
     int end = 1;
     while (end > starts.back()) {
@@ -391,7 +399,7 @@ bool CwbvhCollapser::run(const Vec4* hn, int n, Vec4* nodesOut, int* primOut, vo
 }
 
 bool cwbvhFromHplocHost(const Vec4* hnodes, int n, std::vector<Vec4>& nodes, std::vector<int>& prim, int& depth,
-                        double& gpuMs, std::string& err) {
+                        double& gpuMs, float& sah, std::string& err) {
     if (n < 1) { err = "CWBVH collapse: no primitives"; return false; }
     const size_t N = 2 * (size_t)n - 1;
     Vec4 *dH = nullptr, *dN = nullptr;
@@ -413,6 +421,12 @@ bool cwbvhFromHplocHost(const Vec4* hnodes, int n, std::vector<Vec4>& nodes, std
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
         gpuMs = ms;
+        {   // the host collapse's SAH: the root's cost (an internal wide root; or a leaf root plus one node) over its area
+            const Vec4 rl = hnodes[2 * (2 * (size_t)n - 2)], rh = hnodes[2 * (2 * (size_t)n - 2) + 1];
+            const float Ar = hostArea(xyz(rl), xyz(rh));
+            const bool leafRoot = n == 1 || C.rootIsLeaf();
+            sah = Ar > 0 ? (leafRoot ? (Ar * (float)n * kCPrim + Ar * kCNode) / Ar : C.rootCost() / Ar) : 0.f;
+        }
         nodes.resize((size_t)nn * 5); prim.resize((size_t)n);
         if (hipMemcpy(nodes.data(), dN, nodes.size() * sizeof(Vec4), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(prim.data(), dP, prim.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) { err = "CWBVH collapse: download failed"; break; }

@@ -1,5 +1,6 @@
 // host/setup.cpp - option parsing and scene setup (see host/setup.h).
 #include "host/setup.h"
+#include "device/collapse.h"
 #include "device/hploc.h"
 #include "host/bvhcache.h"
 #include "host/scene_check.h"
@@ -96,6 +97,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--gen_materials") ok = inum(A.genMaterials);
         else if (k == "--graph") ok = inum(A.graph);
         else if (k == "--bvh_cache") ok = str(A.bvhCache);
+        else if (k == "--gpu_collapse") ok = inum(A.gpuCollapse);
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--devloop" || k == "--sort") {
             int v = 0; ok = inum(v);
@@ -141,7 +143,7 @@ static double msSince(std::chrono::steady_clock::time_point t0) {
 // (reference: buildTree, src/app/scene_setup.cpp:76-83); treeMs is the build time (device time for H-PLOC), postMs
 // the host conversion
 static bool buildTree(const Mesh& M, const Args& A, int leafMax, BinaryBVH& tree, double& treeMs, double& postMs,
-                      std::string& err) {
+                      std::string& err, WideBVH* wide = nullptr) {
     const auto t0 = std::chrono::steady_clock::now();
     if (A.builder != "hploc") {
         buildSAH(M, A.bins, leafMax, tree);
@@ -158,6 +160,11 @@ static bool buildTree(const Mesh& M, const Args& A, int leafMax, BinaryBVH& tree
     if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err)) return false;
     const auto t1 = std::chrono::steady_clock::now();
     treeFromHploc(hn, n, root, leafMax, tree);
+    if (wide) {   // the compressed BVH8 collapse on the device (bit-identical to collapseCWBVH of the tree above)
+        int depth = 0; double collapseMs = 0; float sah = 0.f;
+        if (!cwbvhFromHplocHost(hn.data(), n, wide->nodes, wide->prim, depth, collapseMs, sah, err)) return false;
+        wide->sah = sah; wide->depth = depth;
+    }
     postMs = msSince(t1); treeMs = gpuMs;
     return true;
 }
@@ -279,7 +286,9 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
         const Mesh& mk = *parts[k];
         BinaryBVH tree;
         double treeMs = 0, postMs = 0, sah = 0;
-        if (!buildTree(mk, A, leafMax, tree, treeMs, postMs, err)) return false;
+        const bool gpuWide = V == 8 && A.builder == "hploc" && A.gpuCollapse;
+        WideBVH W;
+        if (!buildTree(mk, A, leafMax, tree, treeMs, postMs, err, gpuWide ? &W : nullptr)) return false;
         std::vector<Vec4> nodes;
         std::vector<int> order;
         if (V == 4 || V == 2) {
@@ -287,7 +296,7 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
             if (V == 4) layoutBVH4(tree, nodes); else layoutBVH2(tree, nodes);
             order = tree.prim; sah = sahCost(tree, 1, 1);
         } else {
-            WideBVH W; collapseCWBVH(tree, W);
+            if (!gpuWide) collapseCWBVH(tree, W);
             nodes.swap(W.nodes); order.swap(W.prim); sah = W.sah;
         }
         const int triBase = (int)S.verts.size() / 3;

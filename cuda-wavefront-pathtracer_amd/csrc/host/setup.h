@@ -33,6 +33,7 @@ struct Args {
     int genMaterials = -1;           // >= 0: seed of the generated OpenPBR materials of a --gen scene's boxes
     int graph = -1;                  // frames replayed from a captured HIP graph (-1: on for the full integrator)
     std::string bvhCache;            // directory of built-scene cache files (empty: no cache)
+    int gpuCollapse = 1;             // --builder hploc with bvh8: the CWBVH collapse on the GPU (bit-identical to the host's)
     std::string envOut;              // write the environment map in use (e.g. a generated one) as a Radiance HDR file
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;

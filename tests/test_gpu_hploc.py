@@ -81,12 +81,14 @@ def test_hploc_device_api_and_bvh2_layout():
         cwp.lib().cwp_hploc_destroy(h)
 
 
-@pytest.mark.parametrize("variant", [8, 2])
-def test_builder_hploc_scene_matches_oracle_and_renders(variant):
-    """--builder hploc: GPU H-PLOC over the triangle boxes, host SAH leaf collapse, layout. The layout equals the
-    oracle's build from the oracle's own H-PLOC restatement, bit for bit, and the image matches the oracle's."""
+@pytest.mark.parametrize("variant,gpu_collapse", [(8, 1), (8, 0), (2, 1)])
+def test_builder_hploc_scene_matches_oracle_and_renders(variant, gpu_collapse):
+    """--builder hploc: GPU H-PLOC over the triangle boxes, then the layout: for BVH8 the CWBVH collapse on the GPU
+    (default) or on the host, for BVH2 the host SAH leaf collapse. The layout equals the oracle's build from the
+    oracle's own H-PLOC restatement, bit for bit, and the image matches the oracle's."""
+    print("case builder hploc bvh%d gpu_collapse %d" % (variant, gpu_collapse), flush=True)
     args = ["--gen", "boxes", "--width", "64", "--height", "64", "--builder", "hploc", "--variant", "bvh%d" % variant,
-            "--batch", "8", "--spp", "8", "--bounces", "3"]
+            "--batch", "8", "--spp", "8", "--bounces", "3", "--gpu_collapse", str(gpu_collapse)]
     s = cwp.Setup(args)
     v, t = s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES)
     o = po.build(v, t, variant, leaf_max=4, builder="hploc")
@@ -97,7 +99,7 @@ def test_builder_hploc_scene_matches_oracle_and_renders(variant):
     img = r.accum()
     j = r.frame_json(st)
     assert j["builder"] == "hploc" and j["bvh_tree_ms"] > 0
-    sref = cwp.Setup([a for a in args if a not in ("--builder", "hploc")])
+    sref = cwp.Setup([a for a in args if a not in ("--builder", "hploc")])   # (--gpu_collapse is ignored with sah)
     ref = cwp.Renderer(sref)
     ref.render(8)
     # a different tree, the same nearest hits: the images agree (ties between coincident faces may pick another face)
