@@ -2,8 +2,8 @@
 //
 // Same contract as the reference's HplocBuilder (src/accel/hploc.h:21-40): primitive boxes in, 2n - 1 nodes of two
 // float4 out, (lo.xyz, left id bits), (hi.xyz, right id bits). Nodes [0, n) are the leaves in Morton order
-// (left = primitive id, right = -1); internal nodes [n, 2n - 1) come from an atomic counter, so every child id is
-// smaller than its parent's and the root is node 2n - 2 (node 0 when n == 1).
+// (left = primitive id, right = -1); internal nodes [n, 2n - 1) are numbered by height (longest path to a leaf) after
+// the climb, so every child id is smaller than its parent's and the root is node 2n - 2 (node 0 when n == 1).
 //
 // Steps (all stream-ordered on the caller's stream, no host synchronisation):
 //  1. centroid box: wave reductions + one order-preserving integer atomic per wave and axis;
@@ -15,7 +15,10 @@
 //     second continues with the union), and each LBVH node whose range exceeds 16 clusters (or the root) is reduced
 //     by PLOC iterations (nearest neighbour by merged surface area within 8 positions; mutual pairs merge) to 16
 //     clusters (1 at the root). A wave64 runs two such merges at once, one per half-wave of 32 lanes: the merge
-//     window is 2 x 16 clusters, as in the reference's 32-lane merge, so the trees are the same.
+//     window is 2 x 16 clusters, as in the reference's 32-lane merge, so the trees are the same. New nodes take
+//     build-time slots from per-wave chunks of one counter (one atomic per chunk rather than per merge: a single
+//     address hit by every merge of the grid serialises at the memory-side atomic unit);
+//  5. renumbering: the internal nodes sorted by height (two radix passes), children before parents.
 #pragma once
 #include "core/vecmath.h"
 #include <string>
@@ -35,8 +38,8 @@ public:
     bool build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err);
     const Vec4* nodes() const { return nodes_; }   // device, 2 float4 per node
     const uint64_t* sortedKeys() const { return keysA_; }
-    // device counter of allocated node ids: 2n - 1 after a consistent build of n > 1 primitives
-    const int* idCounter() const { return counter_; }
+    // device status of the last build: 0 when the renumbered tree is consistent
+    const int* status() const { return status_; }
     void release();
     int capacity() const { return cap_; }
 
@@ -45,7 +48,11 @@ private:
     Vec4* nodes_ = nullptr;
     uint64_t *keysA_ = nullptr, *keysB_ = nullptr;
     int *clusters_ = nullptr, *bounds_ = nullptr, *counter_ = nullptr, *counts_ = nullptr;
-    Vec4* cboxes_ = nullptr;   // box of the cluster in each slot of clusters_ (2 float4)
+    Vec4* cboxes_ = nullptr;   // box of the cluster in each slot of clusters_ (2 float4), height in the first w
+    Vec4* tnodes_ = nullptr;   // internal nodes in build-time slots, renumbered into nodes_ after the climb
+    int *theight_ = nullptr, *newId_ = nullptr, *status_ = nullptr;
+    uint64_t *rkA_ = nullptr, *rkB_ = nullptr;
+    int tcap_ = 0;
     uint32_t* cbox_ = nullptr;
     int tiles_ = 0;
 };

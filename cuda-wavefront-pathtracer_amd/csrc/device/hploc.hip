@@ -62,6 +62,12 @@ __device__ __forceinline__ Vec3 ldBox3(const Vec4* p) {   // xyz of a node recor
     const uint32_t z = __hip_atomic_load((const uint32_t*)p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return v3(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)), __uint_as_float(z));
 }
+__device__ __forceinline__ Vec4 ldBox4(const Vec4* p) {   // a cluster box record with its w component
+    const uint64_t a = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load((const uint64_t*)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)), __uint_as_float((uint32_t)b),
+              __uint_as_float((uint32_t)(b >> 32)));
+}
 __device__ __forceinline__ void stNode(Vec4* p, Vec4 v) {
     const uint64_t a = (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32);
     const uint64_t b = (uint64_t)__float_as_uint(v.z) | ((uint64_t)__float_as_uint(v.w) << 32);
@@ -70,9 +76,9 @@ __device__ __forceinline__ void stNode(Vec4* p, Vec4 v) {
 }
 
 // ------------------------------------------------------------------------------------------------ keys
-__global__ void k_hp_init(uint32_t* cbox, int* counter, int n) {
+__global__ void k_hp_init(uint32_t* cbox, int* counter, int* status) {
     if (threadIdx.x < 3) { cbox[threadIdx.x] = 0xffffffffu; cbox[3 + threadIdx.x] = 0u; }
-    if (threadIdx.x == 0) *counter = n;
+    if (threadIdx.x == 0) { *counter = 0; *status = 0; }
 }
 
 __global__ void __launch_bounds__(256) k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
@@ -277,44 +283,54 @@ __global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict_
 // Per half-wave LDS of a merge: compaction buffers (ids, boxes) and the nodes created by the merge, which get their
 // global ids only when the merge is done (one counter reservation per merge instead of one per iteration).
 struct MergeLds {
-    int id[32];
+    int id[32], ht[32];
     float box[32 * 6];
     float node[32 * 8];   // lo xyz, hi xyz, left id bits, right id bits
+    int nodeHt[32];
 };
 
+// Build-time node ids: leaves keep [0, n); a merge's nodes get temporary slots n + t from its wave's chunk of the
+// temporary array (chunks of kChunk slots from one counter, so the counter sees one atomic per chunk instead of one
+// per merge); every node carries its height (leaves 0), and a renumbering pass after the climb orders the internal
+// nodes by height, which keeps every child below its parent and puts the root, the one node of largest height, last.
+constexpr int kChunk = 64;
+struct Chunk { int next, end; };   // wave-uniform
+
 // PLOC on the clusters of LBVH node [L, R] split at s, run by one half-wave (32 lanes; both halves call together, a
-// half with on = false idles). Its clusters sit at the start of each child's range in C (ids) and CB (boxes) (-1 =
-// empty slot): up to 16 from each side, compacted into lanes in order. Each iteration: every lane finds the pair of
-// smallest merged half-area among the lanes within kR of it (pairs ordered by (area bits, lower lane, upper lane), so
-// the global best pair is mutual and every iteration merges); mutual pairs merge into a new node (lower lane = left
-// child), numbered locally (-2 - k) in lane order; survivors are compacted in lane order. Stops at kT clusters (1 at
-// the root). Then the merge's nodes take consecutive global ids from the counter (local order, so children stay
-// below parents and the root's last merge is node 2n - 2) and the survivors go back to the start of [L, R].
-__device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, int* C, Vec4* CB, int* counter,
-                          MergeLds& sh, int cap) {
+// half with on = false idles). Its clusters sit at the start of each child's range in C (ids) and CB (boxes, height in
+// the first record's w) (-1 = empty slot): up to 16 from each side, compacted into lanes in order. Each iteration:
+// every lane finds the pair of smallest merged half-area among the lanes within kR of it (pairs ordered by (area bits,
+// lower lane, upper lane), so the global best pair is mutual and every iteration merges); mutual pairs merge into a
+// new node (lower lane = left child), numbered locally (-2 - k) in lane order; survivors are compacted in lane order.
+// Stops at kT clusters (1 at the root). Then the merge's nodes take consecutive temporary slots of the wave's chunk
+// and the survivors go back to the start of [L, R].
+__device__ void plocMerge(bool on, int L, int R, int s, bool root, int n, Vec4* tnodes, int* theight, int tcap, int* C,
+                          Vec4* CB, int* counter, Chunk& ck, MergeLds& sh) {
     const int lane = (int)threadIdx.x & 63, hl = lane & 31, hb = lane & 32;
     const uint64_t ltMask = (1ull << lane) - 1ull, halfMask = 0xffffffffull << hb;
     int nL = 0, nR = 0;
     if (on) { nL = min(s - L + 1, kT); nR = min(R - s, kT); }
     const int slot = hl < nL ? L + hl : (hl < nL + nR ? s + 1 + (hl - nL) : -1);
-    int id = -1;
+    int id = -1, ht = 0;
     Vec3 lo = v3(0.f, 0.f, 0.f), hi = lo;
-    if (slot >= 0) {   // id and box of the slot, read together
+    if (slot >= 0) {   // id, box and height of the slot, read together
         id = ldC(C + slot);
-        lo = ldBox3(CB + 2 * slot); hi = ldBox3(CB + 2 * slot + 1);
+        const Vec4 b0 = ldBox4(CB + 2 * slot);
+        lo = xyz(b0); ht = __float_as_int(b0.w);
+        hi = ldBox3(CB + 2 * slot + 1);
     }
     const uint64_t vm = __ballot(id >= 0) & halfMask;
     int num = (int)__builtin_popcountll(vm);
     if (id >= 0) {
         const int dd = (int)__builtin_popcountll(vm & ltMask);
-        sh.id[dd] = id;
+        sh.id[dd] = id; sh.ht[dd] = ht;
         float* b = sh.box + 6 * dd;
         b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
     }
     __syncthreads();
     id = -1;
     if (hl < num) {
-        id = sh.id[hl];
+        id = sh.id[hl]; ht = sh.ht[hl];
         const float* b = sh.box + 6 * hl;
         lo = v3(b[0], b[1], b[2]); hi = v3(b[3], b[4], b[5]);
     }
@@ -342,14 +358,16 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, 
         const bool mutual = act && hl < num && nn != hl && nnn == hl;
         const bool lead = mutual && hl < nn;
         const Vec3 plo = shfl3(lo, hb + nn), phi = shfl3(hi, hb + nn);
-        const int pid = __shfl(id, hb + nn);
+        const int pid = __shfl(id, hb + nn), pht = __shfl(ht, hb + nn);
         const uint64_t lm = __ballot(lead) & halfMask;
         if (lead) {
             const int k = made + (int)__builtin_popcountll(lm & ltMask);
             lo = vmn(lo, plo); hi = vmx(hi, phi);
+            ht = 1 + max(ht, pht);
             float* q = sh.node + 8 * k;
             q[0] = lo.x; q[1] = lo.y; q[2] = lo.z; q[3] = hi.x; q[4] = hi.y; q[5] = hi.z;
             q[6] = __int_as_float(id); q[7] = __int_as_float(pid);
+            sh.nodeHt[k] = ht;
             id = -2 - k;
         }
         made += (int)__builtin_popcountll(lm);
@@ -357,7 +375,7 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, 
         const uint64_t km = __ballot(keep) & halfMask;
         if (keep) {
             const int dd = (int)__builtin_popcountll(km & ltMask);
-            sh.id[dd] = id;
+            sh.id[dd] = id; sh.ht[dd] = ht;
             float* b = sh.box + 6 * dd;
             b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
         }
@@ -365,30 +383,40 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, 
         if (act) {
             num = (int)__builtin_popcountll(km);
             if (hl < num) {
-                id = sh.id[hl];
+                id = sh.id[hl]; ht = sh.ht[hl];
                 const float* b = sh.box + 6 * hl;
                 lo = v3(b[0], b[1], b[2]); hi = v3(b[3], b[4], b[5]);
             } else id = -1;
         }
         __syncthreads();
     }
-    // global ids: one reservation for both halves' nodes
-    const int made0 = __shfl(made, 0), made1 = __shfl(made, 32);
-    int base = 0;
-    if (lane == 0 && made0 + made1 > 0) base = atomicAdd(counter, made0 + made1);
-    base = __shfl(base, 0) + (hb ? made0 : 0);
-    auto global = [&](int x) { return x >= 0 ? x : base + (-2 - x); };
-    if (hl < made && base + hl < cap) {   // a consistent build allocates exactly cap - n internal nodes
+    // temporary slots for both halves' nodes: the rest of the wave's chunk, then a new chunk (kChunk >= 2 x 31 covers
+    // the remainder of any merge); slots follow the halves' local order
+    const int made0 = __shfl(made, 0), made1 = __shfl(made, 32), total = made0 + made1;
+    const int a0 = ck.next, r0 = min(total, ck.end - ck.next);
+    int a1 = 0;
+    if (total > r0) {
+        int b = 0;
+        if (lane == 0) b = atomicAdd(counter, kChunk);
+        b = __shfl(b, 0);
+        a1 = b; ck.end = b + kChunk; ck.next = b + (total - r0);
+    } else ck.next += total;
+    const int first = hb ? made0 : 0;
+    auto slotOf = [&](int k) { const int g = first + k; return g < r0 ? a0 + g : a1 + (g - r0); };
+    auto ref = [&](int x) { return x >= -1 ? x : n + slotOf(-2 - x); };   // leaf id, n + slot, or -1
+    if (hl < made && slotOf(hl) < tcap) {   // node records are read only after the climb: plain stores
+        const int t = slotOf(hl);
         const float* q = sh.node + 8 * hl;
-        stNode(nodes + 2 * (base + hl), v4(q[0], q[1], q[2], asFloat(global(__float_as_int(q[6])))));
-        stNode(nodes + 2 * (base + hl) + 1, v4(q[3], q[4], q[5], asFloat(global(__float_as_int(q[7])))));
+        tnodes[2 * (size_t)t] = v4(q[0], q[1], q[2], asFloat(ref(__float_as_int(q[6]))));
+        tnodes[2 * (size_t)t + 1] = v4(q[3], q[4], q[5], asFloat(ref(__float_as_int(q[7]))));
+        theight[t] = sh.nodeHt[hl];
     }
     if (slot >= 0) stC(C + slot, -1);
     __syncthreads();
     if (on && hl < num) {
-        const int g = global(id);
-        stC(C + L + hl, g < cap ? g : -1);
-        stNode(CB + 2 * (L + hl), v4(lo, 0.f));
+        const int g = ref(id);
+        stC(C + L + hl, g - n < tcap ? g : -1);
+        stNode(CB + 2 * (L + hl), v4(lo, __int_as_float(ht)));
         stNode(CB + 2 * (L + hl) + 1, v4(hi, 0.f));
     }
     __syncthreads();
@@ -400,9 +428,10 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, Vec4* nodes, 
 // range and, when it holds more than kT clusters or is the root, the wave runs the PLOC merge on it, two ranges at a
 // time. Cluster and node records cross waves (and XCDs) through device-coherent atomic accesses; fences order them
 // before the hand-over and after it.
-__global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __restrict__ keys, int* C, Vec4* CB, int* bounds,
-                                              int* counter, int n) {
+__global__ void __launch_bounds__(64) k_hploc(Vec4* tnodes, int* theight, int tcap, const uint64_t* __restrict__ keys, int* C,
+                                              Vec4* CB, int* bounds, int* counter, int n) {
     __shared__ MergeLds sh[2];
+    Chunk ck{ 0, 0 };
     const int lane = (int)threadIdx.x, half = lane >> 5;
     const int i = blockIdx.x * 64 + lane;
     bool active = i < n;
@@ -435,10 +464,45 @@ __global__ void __launch_bounds__(64) k_hploc(Vec4* nodes, const uint64_t* __res
             const int src = half == 0 ? a : b;
             const int sl = src < 0 ? 0 : src;
             const int l = __shfl(L, sl), r = __shfl(R, sl), sp = __shfl(s, sl);
-            plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, nodes, C, CB, counter, sh[half], 2 * n - 1);
+            plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, n, tnodes, theight, tcap, C, CB, counter, ck, sh[half]);
         }
         if (active && L == 0 && R == n - 1) active = false;
     }
+}
+
+// ------------------------------------------------------------------------------------------------ renumbering
+// sort keys of the temporary slots: height (holes: 0xffff) above the slot index
+__global__ void k_rn_fill(int* theight, int tcap) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < tcap) theight[t] = -1;
+}
+__global__ void k_rn_keys(const int* __restrict__ theight, int used, uint64_t* keys) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= used) return;
+    const int h = theight[t];
+    keys[t] = ((uint64_t)(uint32_t)(h < 0 ? 0xffff : min(h, 0xfffe)) << 32) | (uint64_t)(uint32_t)t;
+}
+// final ids: the p-th slot in height order becomes node n + p; the n - 1 nodes must be exactly the non-hole slots
+__global__ void k_rn_map(const uint64_t* __restrict__ sorted, int used, int n, int* newId, int* status) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= used) return;
+    const uint32_t h = (uint32_t)(sorted[p] >> 32), t = (uint32_t)sorted[p];
+    const bool hole = h == 0xffff;
+    if (hole != (p >= n - 1)) atomicOr(status, 1);
+    if (!hole && p < n - 1) newId[t] = n + p;
+}
+__global__ void k_rn_write(const Vec4* __restrict__ tnodes, const int* __restrict__ theight, const int* __restrict__ newId,
+                           int used, int n, Vec4* nodes, int* status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= used || theight[t] < 0) return;
+    const int f = newId[t];
+    if (f < n || f > 2 * n - 2) { atomicOr(status, 2); return; }
+    Vec4 a = tnodes[2 * (size_t)t], b = tnodes[2 * (size_t)t + 1];
+    const int l = asInt(a.w), r = asInt(b.w);
+    const int fl = l < n ? l : (l - n < used ? newId[l - n] : -1), fr = r < n ? r : (r - n < used ? newId[r - n] : -1);
+    if (fl < 0 || fr < 0 || fl >= f || fr >= f) atomicOr(status, 4);
+    a.w = asFloat(fl); b.w = asFloat(fr);
+    nodes[2 * (size_t)f] = a; nodes[2 * (size_t)f + 1] = b;
 }
 
 __global__ void k_hp_to_bvh2(const Vec4* __restrict__ hn, int n, Vec4* out) {
@@ -484,7 +548,8 @@ void HplocBuilder::release() {
     if (!cap_) return;
     hipFree(nodes_); hipFree(keysA_); hipFree(keysB_); hipFree(clusters_); hipFree(bounds_); hipFree(counter_);
     hipFree(cbox_); hipFree(counts_); hipFree(cboxes_);
-    cboxes_ = nullptr;
+    hipFree(tnodes_); hipFree(theight_); hipFree(rkA_); hipFree(rkB_); hipFree(newId_); hipFree(status_);
+    cboxes_ = nullptr; tnodes_ = nullptr; theight_ = newId_ = status_ = nullptr; rkA_ = rkB_ = nullptr; tcap_ = 0;
     nodes_ = nullptr; keysA_ = keysB_ = nullptr; clusters_ = bounds_ = counter_ = counts_ = nullptr; cbox_ = nullptr;
     cap_ = 0; tiles_ = 0;
 }
@@ -502,7 +567,17 @@ bool HplocBuilder::reserve(int n, std::string& err) {
     HCHK(hipMalloc(&bounds_, (size_t)n * 4));
     HCHK(hipMalloc(&counter_, 4));
     HCHK(hipMalloc(&cbox_, 24));
-    HCHK(hipMalloc(&counts_, (size_t)256 * (tiles_ + 1) * 4));   // digit x tile counts, then the 256 digit totals
+    // temporary node slots: the n - 1 internal nodes plus at most one partly used chunk per climbing wave
+    tcap_ = (n - 1) + kChunk * ((n + 63) / 64 + 1);
+    const int rtiles = (tcap_ + kTile - 1) / kTile;
+    HCHK(hipMalloc(&tnodes_, (size_t)tcap_ * 2 * sizeof(Vec4)));
+    HCHK(hipMalloc(&theight_, (size_t)tcap_ * 4));
+    HCHK(hipMalloc(&rkA_, (size_t)tcap_ * 8));
+    HCHK(hipMalloc(&rkB_, (size_t)tcap_ * 8));
+    HCHK(hipMalloc(&newId_, (size_t)tcap_ * 4));
+    HCHK(hipMalloc(&status_, 4));
+    // digit x tile counts, then the 256 digit totals (for the key sort and the renumbering sort)
+    HCHK(hipMalloc(&counts_, (size_t)256 * (std::max(tiles_, rtiles) + 1) * 4));
     cap_ = n;
     return true;
 }
@@ -513,7 +588,8 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     if (!reserve(n, err)) return false;
     hipStream_t st = (hipStream_t)stream;
     const int g = (n + 255) / 256;
-    k_hp_init<<<1, 64, 0, st>>>(cbox_, counter_, n);
+    k_hp_init<<<1, 64, 0, st>>>(cbox_, counter_, status_);
+    k_rn_fill<<<(tcap_ + 255) / 256, 256, 0, st>>>(theight_, tcap_);
     k_hp_bounds<<<std::min(g, 512), 256, 0, st>>>(lo, hi, n, cbox_);
     int ib = 1;
     while ((1ll << ib) < n) ib++;
@@ -536,7 +612,21 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
         sorted = src;
     }
     k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_);
-    if (n > 1) k_hploc<<<(n + 63) / 64, 64, 0, st>>>(nodes_, sorted, clusters_, cboxes_, bounds_, counter_, n);
+    if (n > 1) {
+        k_hploc<<<(n + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, clusters_, cboxes_, bounds_, counter_, n);
+        // internal nodes in height order: two 8-bit passes over the heights of every temporary slot (holes last)
+        const int T = tcap_, rt = (T + kTile - 1) / kTile;
+        k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_);
+        uint64_t *src = rkA_, *dst = rkB_;
+        for (int p = 0; p < 2; p++) {
+            k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt);
+            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt);
+            k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt);
+            std::swap(src, dst);
+        }
+        k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(src, T, n, newId_, status_);
+        k_rn_write<<<(T + 255) / 256, 256, 0, st>>>(tnodes_, theight_, newId_, T, n, nodes_, status_);
+    }
     HCHK(hipGetLastError());
     root = n > 1 ? 2 * n - 2 : 0;
     return true;
@@ -569,9 +659,9 @@ bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& 
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
         gpuMs = ms;
-        int ids = 0;
-        if (hipMemcpy(&ids, B.idCounter(), 4, hipMemcpyDeviceToHost) != hipSuccess) { err = "H-PLOC: download failed"; break; }
-        if (n > 1 && ids != 2 * n - 1) { err = "H-PLOC: inconsistent build (" + std::to_string(ids) + " node ids for " + std::to_string(n) + " primitives)"; break; }
+        int status = 0;
+        if (hipMemcpy(&status, B.status(), 4, hipMemcpyDeviceToHost) != hipSuccess) { err = "H-PLOC: download failed"; break; }
+        if (status) { err = "H-PLOC: inconsistent build (status " + std::to_string(status) + ")"; break; }
         if (hipMemcpy(nodesOut, B.nodes(), (size_t)2 * (2 * (size_t)n - 1) * 16, hipMemcpyDeviceToHost) != hipSuccess) {
             err = "H-PLOC: download failed"; break;
         }
