@@ -53,6 +53,7 @@ private:
     int *theight_ = nullptr, *newId_ = nullptr, *status_ = nullptr;
     uint64_t *rkA_ = nullptr, *rkB_ = nullptr;
     int tcap_ = 0;
+    int *rangeL_ = nullptr, *rangeR_ = nullptr;   // LBVH node ranges by split position
     uint32_t* cbox_ = nullptr;
     int tiles_ = 0;
 };

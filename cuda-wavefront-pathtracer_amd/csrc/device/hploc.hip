@@ -277,7 +277,7 @@ __global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict_
     C[i] = i;
     CB[2 * i] = v4(xyz(l), 0.f);
     CB[2 * i + 1] = v4(xyz(h), 0.f);
-    bounds[i] = -1;
+    bounds[i] = 0;   // arrival counter of the LBVH node split at i
 }
 
 // Per half-wave LDS of a merge: compaction buffers (ids, boxes) and the nodes created by the merge, which get their
@@ -422,40 +422,58 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, int n, Vec4* 
     __syncthreads();
 }
 
-// One lane per sorted primitive climbs the implicit LBVH: the parent of range [L, R] is [L, R'] (this range is the
-// left child, split at R) when L == 0 or the key delta at R is smaller than at L - 1, else [L', R] (split at L - 1).
-// The lane exchanges its range end into bounds[split]; the first lane to arrive stops, the second takes the union
-// range and, when it holds more than kT clusters or is the root, the wave runs the PLOC merge on it, two ranges at a
-// time. Cluster and node records cross waves (and XCDs) through device-coherent atomic accesses; fences order them
-// before the hand-over and after it.
-__global__ void __launch_bounds__(64) k_hploc(Vec4* tnodes, int* theight, int tcap, const uint64_t* __restrict__ keys, int* C,
-                                              Vec4* CB, int* bounds, int* counter, int n) {
+// The LBVH over the sorted keys as a binary radix tree (Karras 2012): internal node i finds its range [min(i, j),
+// max(i, j)] and its split g by binary searches on the common-prefix length of its keys (keys are unique). The
+// node is stored under its split position g (the bottom-up climb's hand-over position: the range is the union of
+// [L, g] and [g + 1, R]); the tree is the one the bottom-up rule builds (extend towards the neighbour with the
+// longer common prefix).
+__device__ __forceinline__ int prefixLen(const uint64_t* __restrict__ keys, int n, int i, int j) {
+    return (j < 0 || j > n - 1) ? -1 : (int)__builtin_clzll(keys[i] ^ keys[j]);
+}
+__global__ void k_lbvh_ranges(const uint64_t* __restrict__ keys, int n, int* rangeL, int* rangeR) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = prefixLen(keys, n, i, i + 1) - prefixLen(keys, n, i, i - 1) > 0 ? 1 : -1;
+    const int dmin = prefixLen(keys, n, i, i - d);
+    int lmax = 2;
+    while (prefixLen(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (prefixLen(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = prefixLen(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (prefixLen(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int g = i + s * d + min(d, 0);
+    rangeL[g] = min(i, j); rangeR[g] = max(i, j);
+}
+
+// H-PLOC over the LBVH nodes that merge: those whose range holds more than kT clusters, and the root. One lane per
+// split position; a lane starts at a merge node none of whose children merge (its clusters are the leaves of its
+// range), merges it, then climbs to the parent (extend towards the neighbour with the longer common prefix). A parent
+// with two merging children is handed over through an arrival counter: the first lane stops, the second merges it.
+// Nodes whose ranges hold at most kT clusters need no work and no hand-over. The wave runs the merges of its lanes two
+// at a time (one per half-wave). Cluster and node records cross waves (and XCDs) through device-coherent atomic
+// accesses; a vmcnt(0) wait orders them before the arrival.
+__global__ void __launch_bounds__(64) k_hploc(Vec4* tnodes, int* theight, int tcap, const uint64_t* __restrict__ keys,
+                                              const int* __restrict__ rangeL, const int* __restrict__ rangeR, int* C,
+                                              Vec4* CB, int* arrive, int* counter, int n) {
     __shared__ MergeLds sh[2];
     Chunk ck{ 0, 0 };
     const int lane = (int)threadIdx.x, half = lane >> 5;
-    const int i = blockIdx.x * 64 + lane;
-    bool active = i < n;
-    int L = i, R = i;
+    const int g = blockIdx.x * 64 + lane;
+    int s = g, L = 0, R = 0;
+    bool active = false;
+    if (g < n - 1) {
+        L = rangeL[g]; R = rangeR[g];
+        const bool root = L == 0 && R == n - 1;
+        active = (R - L + 1 > kT || root) && g - L + 1 <= kT && R - g <= kT;   // merges, no merging child
+    }
     while (__ballot(active)) {
-        // this wave's earlier merges before the hand-overs: every store performed (vmcnt 0) at the coherence point
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        int s = 0;
-        bool mrg = false;
-        if (active) {
-            int prev;
-            if (L == 0 || (R != n - 1 && (keys[R] ^ keys[R + 1]) < (keys[L - 1] ^ keys[L]))) {
-                s = R; prev = atomicExch(&bounds[R], L);
-                if (prev != -1) R = prev;
-            } else {
-                s = L - 1; prev = atomicExch(&bounds[L - 1], R);
-                if (prev != -1) L = prev;
-            }
-            if (prev == -1) active = false;   // first to arrive: the sibling's lane continues
-            else mrg = (R - L + 1 > kT) || (L == 0 && R == n - 1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // the sibling subtree's clusters and nodes after them
-        uint64_t mm = __ballot(mrg);
+        uint64_t mm = __ballot(active);
         while (mm) {
             const int a = __builtin_ctzll(mm);
             mm &= mm - 1;
@@ -466,7 +484,20 @@ __global__ void __launch_bounds__(64) k_hploc(Vec4* tnodes, int* theight, int tc
             const int l = __shfl(L, sl), r = __shfl(R, sl), sp = __shfl(s, sl);
             plocMerge(src >= 0, l, r, sp, l == 0 && r == n - 1, n, tnodes, theight, tcap, C, CB, counter, ck, sh[half]);
         }
-        if (active && L == 0 && R == n - 1) active = false;
+        // this wave's merges performed (vmcnt 0) at the coherence point before the arrivals below
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        if (active) {
+            if (L == 0 && R == n - 1) active = false;   // the root is done
+            else {
+                const int p = (L == 0 || (R != n - 1 && (keys[R] ^ keys[R + 1]) < (keys[L - 1] ^ keys[L]))) ? R : L - 1;
+                const int pl = rangeL[p], pr = rangeR[p];
+                const int need = (p - pl + 1 > kT ? 1 : 0) + (pr - p > kT ? 1 : 0);   // merging children of p
+                if (need == 2 && atomicAdd(&arrive[p], 1) == 0) active = false;   // the sibling's lane merges p
+                s = p; L = pl; R = pr;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // the sibling's clusters after its arrival
     }
 }
 
@@ -549,6 +580,8 @@ void HplocBuilder::release() {
     hipFree(nodes_); hipFree(keysA_); hipFree(keysB_); hipFree(clusters_); hipFree(bounds_); hipFree(counter_);
     hipFree(cbox_); hipFree(counts_); hipFree(cboxes_);
     hipFree(tnodes_); hipFree(theight_); hipFree(rkA_); hipFree(rkB_); hipFree(newId_); hipFree(status_);
+    hipFree(rangeL_); hipFree(rangeR_);
+    rangeL_ = rangeR_ = nullptr;
     cboxes_ = nullptr; tnodes_ = nullptr; theight_ = newId_ = status_ = nullptr; rkA_ = rkB_ = nullptr; tcap_ = 0;
     nodes_ = nullptr; keysA_ = keysB_ = nullptr; clusters_ = bounds_ = counter_ = counts_ = nullptr; cbox_ = nullptr;
     cap_ = 0; tiles_ = 0;
@@ -568,7 +601,7 @@ bool HplocBuilder::reserve(int n, std::string& err) {
     HCHK(hipMalloc(&counter_, 4));
     HCHK(hipMalloc(&cbox_, 24));
     // temporary node slots: the n - 1 internal nodes plus at most one partly used chunk per climbing wave
-    tcap_ = (n - 1) + kChunk * ((n + 63) / 64 + 1);
+    tcap_ = (n - 1) + kChunk * ((n + 62) / 64 + 1);
     const int rtiles = (tcap_ + kTile - 1) / kTile;
     HCHK(hipMalloc(&tnodes_, (size_t)tcap_ * 2 * sizeof(Vec4)));
     HCHK(hipMalloc(&theight_, (size_t)tcap_ * 4));
@@ -576,6 +609,8 @@ bool HplocBuilder::reserve(int n, std::string& err) {
     HCHK(hipMalloc(&rkB_, (size_t)tcap_ * 8));
     HCHK(hipMalloc(&newId_, (size_t)tcap_ * 4));
     HCHK(hipMalloc(&status_, 4));
+    HCHK(hipMalloc(&rangeL_, (size_t)n * 4));
+    HCHK(hipMalloc(&rangeR_, (size_t)n * 4));
     // digit x tile counts, then the 256 digit totals (for the key sort and the renumbering sort)
     HCHK(hipMalloc(&counts_, (size_t)256 * (std::max(tiles_, rtiles) + 1) * 4));
     cap_ = n;
@@ -613,7 +648,9 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     }
     k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_);
     if (n > 1) {
-        k_hploc<<<(n + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, clusters_, cboxes_, bounds_, counter_, n);
+        k_lbvh_ranges<<<(n - 1 + 255) / 256, 256, 0, st>>>(sorted, n, rangeL_, rangeR_);
+        k_hploc<<<(n - 1 + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, rangeL_, rangeR_, clusters_, cboxes_, bounds_,
+                                                  counter_, n);
         // internal nodes in height order: two 8-bit passes over the heights of every temporary slot (holes last)
         const int T = tcap_, rt = (T + kTile - 1) / kTile;
         k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_);
