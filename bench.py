@@ -27,22 +27,23 @@ LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3
 
 
 def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=8, fuse=1,
-                  persistent=0, shadow_refill=0):
+                  persistent=0, shadow_refill=0, sort=0):
     return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
             "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack),
             "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves), "--fuse", str(fuse),
-            "--persistent", str(persistent), "--shadow_refill", str(shadow_refill)] + LIGHTING
+            "--persistent", str(persistent), "--shadow_refill", str(shadow_refill), "--sort", str(sort)] + LIGHTING
 
 
-def algorithmic_bytes(st, npaths):
+def algorithmic_bytes(st, npaths, sort=0):
     """Compulsory HBM bytes of one frame's trace+shade (bounce) kernels, from the frame's ray counters.
 
     Per camera ray: 16 B radiance write. Per continuation ray traced: 4 B queue read + 48 B origin/direction/throughput
     read. Per continuation-ray miss: 32 B radiance read-modify-write. Per continuation produced: 4 B queue write + 48 B
-    path-state write. Per shadow ray produced: 48 B shadow record write."""
+    path-state write (+ 1 B direction-bucket key with --sort 1). Per shadow ray produced: 48 B shadow record write."""
     cont_traced = st["rays_closest"] - npaths
     cont_miss = st["rays_miss"] - st["camera_miss"]
-    return 16.0 * npaths + 52.0 * cont_traced + 32.0 * cont_miss + 52.0 * st["rays_cont"] + 48.0 * st["rays_shadow"]
+    return (16.0 * npaths + 52.0 * cont_traced + 32.0 * cont_miss + (53.0 if sort else 52.0) * st["rays_cont"] +
+            48.0 * st["rays_shadow"])
 
 
 def measured_traffic(kernel_tag):
@@ -359,6 +360,7 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, help="lanes refilled with new rays mid-traversal: 1 split "
                     "trace / shade kernels, 2 the fused kernel")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
+    ap.add_argument("--sort", type=int, default=0, help="1: group each bounce's continuations by direction bucket")
     ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve", "hploc"],
                     help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
@@ -412,7 +414,8 @@ def main():
         return
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
-                                args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill))
+                                args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill,
+                                args.sort))
     r = cwp.Renderer(s)
     info = s.info
     npaths = args.width * args.height * args.batch
@@ -434,7 +437,7 @@ def main():
     grays = rays / (frame_ms * 1e-3) / 1e9
     bounce_ms = sum(f["t_bounce_ms"] for f in frames)
     launches = sum(f["bounce_launches"] for f in frames)
-    bytes_bounce = sum(algorithmic_bytes(f, npaths) for f in frames)
+    bytes_bounce = sum(algorithmic_bytes(f, npaths, args.sort) for f in frames)
     achieved = bytes_bounce / (bounce_ms * 1e-3) / 1e9
     last = frames[-1]
     out = {
@@ -457,11 +460,11 @@ def main():
                    "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
                    "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"],
                    "kernels": {1: "persistent-split", 2: "persistent-fused"}.get(args.persistent, "fused" if args.fuse else "split"),
-                   "shadow_refill": args.shadow_refill},
+                   "shadow_refill": args.shadow_refill, "sort": args.sort},
         "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
         "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
         "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shade_ms": round(last["t_shade_ms"], 3),
-        "t_shadow_ms": round(last["t_shadow_ms"], 3),
+        "t_shadow_ms": round(last["t_shadow_ms"], 3), "t_sort_ms": round(last["t_sort_ms"], 3),
         "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
                      else ("k_bounce_refill (fused, refilled lanes)" if args.persistent == 2 else "k_trace_paths + k_shade (split trace / shade)"),
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

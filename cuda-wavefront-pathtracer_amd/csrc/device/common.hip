@@ -36,6 +36,80 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, const 
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
 
+// Continuation sort. The continuations partition x appended at depth `depth` occupy [start, start + n) of P.qs / P.qk
+// (start: where partition x's depth-`depth` queue began; the same positions the unsorted append would use in
+// P.q[(depth + 1) & 1]). The workgroups of partition x (blockIdx % kParts, so they run on that partition's XCD) walk
+// its tiles of kSortTile entries. k_qcount adds each tile's bucket histogram (LDS atomics) into the partition's bucket
+// sizes; k_qscatter gives bucket b the range after buckets 0..b-1, reserves each tile's share of it with one atomic per
+// bucket and writes the path ids there, tile entries in the order of their LDS ranks. Which entries share a wave
+// afterwards depends on scheduling; the paths' results do not (every path keeps its own state and radiance).
+__device__ __forceinline__ void sortRange(const DevPaths& P, int depth, int part, int& start, int& n) {
+    const int* cnt = P.cnt + depth * kCntStride;
+    start = partStart(cnt, part);
+    n = ctrv(cnt + kCntStride, part, kCQueue);
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_qcount(DevPaths P, int depth) {
+    __shared__ int h[kBuckets];
+    const int t = (int)threadIdx.x;
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int start, n;
+    sortRange(P, depth, part, start, n);
+    if (t < kBuckets) h[t] = 0;
+    __syncthreads();
+    for (int tile = lw; tile * kSortTile < n; tile += nw) {
+        const int k0 = tile * kSortTile + t;
+#pragma unroll
+        for (int j = 0; j < kSortPer; j++) {
+            const int k = k0 + j * kSortBlock;
+            if (k < n) atomicAdd(&h[P.qk[start + k] & (kBuckets - 1)], 1);
+        }
+    }
+    __syncthreads();
+    if (t < kBuckets && h[t]) atomicAdd(P.bcnt + depth * kSortWords + part * 2 * kBuckets + t, h[t]);
+}
+
+__global__ void __launch_bounds__(kSortBlock) k_qscatter(DevPaths P, int depth) {
+    __shared__ int h[kBuckets], first[kBuckets], base[kBuckets];
+    const int t = (int)threadIdx.x;
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int start, n;
+    sortRange(P, depth, part, start, n);
+    int* size = P.bcnt + depth * kSortWords + part * 2 * kBuckets;
+    int* cursor = size + kBuckets;
+    int* qOut = P.q[(depth + 1) & 1];
+    if (t < kBuckets) {
+        // exclusive prefix of the bucket sizes over the first wave's 64 lanes
+        const int c = size[t];
+        int s = c;
+#pragma unroll
+        for (int o = 1; o < kBuckets; o <<= 1) {
+            const int y = __shfl_up(s, o, kBuckets);
+            if (t >= o) s += y;
+        }
+        first[t] = start + s - c;
+    }
+    for (int tile = lw; tile * kSortTile < n; tile += nw) {
+        if (t < kBuckets) h[t] = 0;
+        __syncthreads();
+        const int k0 = tile * kSortTile + t;
+        int id[kSortPer], key[kSortPer], rank[kSortPer];
+#pragma unroll
+        for (int j = 0; j < kSortPer; j++) {
+            const int k = k0 + j * kSortBlock;
+            key[j] = -1; id[j] = 0; rank[j] = 0;
+            if (k < n) { key[j] = P.qk[start + k] & (kBuckets - 1); id[j] = P.qs[start + k]; rank[j] = atomicAdd(&h[key[j]], 1); }
+        }
+        __syncthreads();
+        if (t < kBuckets && h[t]) base[t] = first[t] + atomicAdd(cursor + t, h[t]);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSortPer; j++)
+            if (key[j] >= 0) qOut[base[key[j]] + rank[j]] = id[j];
+        __syncthreads();   // h and base are reused by the next tile
+    }
+}
+
 // zero the pass counters; the depth-0 queue of partition x is the x-th of kParts equal slices of the path ids
 __device__ void passBegin(int* cnt, int words, long long NK) {
     for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = 0;

@@ -118,7 +118,30 @@ struct DevPaths {
     int* cnt;             // kCntStride counters per depth (one line per partition)
     void* spill;          // traversal-stack spill slab, spillCap entries per thread
     int spillCap;
+    // continuation sort (null when off): continuations are appended to qs with their direction bucket in qk, and
+    // k_qcount / k_qscatter move them into q[(d + 1) & 1] grouped by bucket within each partition; bcnt holds per
+    // depth and partition the bucket sizes and fill cursors (kSortWords ints per depth)
+    int* qs = nullptr;
+    uint8_t* qk = nullptr;
+    int* bcnt = nullptr;
 };
+
+// Continuation sort: 64 direction buckets, an 8 x 8 grid over the octahedral map of the unit sphere (upper hemisphere
+// in the central square, the lower one folded onto the corners), so that a wave of one bucket shares a narrow cone of
+// directions instead of the whole cosine lobe of one pixel's samples.
+constexpr int kBuckets = 64;
+constexpr int kSortWords = kParts * 2 * kBuckets;   // per depth: [partition][0: size, 1: cursor][bucket]
+__device__ __forceinline__ int dirBucket(Vec3 d) {
+    const float a = 1.f / (fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
+    float u = d.x * a, v = d.y * a;
+    if (d.z < 0.f) {
+        const float fu = copysignf(1.f - fabsf(v), u), fv = copysignf(1.f - fabsf(u), v);
+        u = fu; v = fv;
+    }
+    // fmaxf first: a NaN component lands in bucket 0
+    const int iu = (int)fminf(fmaxf((u + 1.f) * 4.f, 0.f), 7.f), iv = (int)fminf(fmaxf((v + 1.f) * 4.f, 0.f), 7.f);
+    return iu + 8 * iv;
+}
 
 template <int V> struct StackEntry;
 template <> struct StackEntry<2> { using type = int; };
@@ -333,7 +356,11 @@ __device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int pa
     const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
     if (mc) {
         const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
-        if (r.cont) { qOut[pos] = i; P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, r.pdf); P.T[i] = v4(r.Tn, 0.f); }
+        if (r.cont) {
+            if (P.qk) { P.qs[pos] = i; P.qk[pos] = (uint8_t)dirBucket(r.nd); }
+            else qOut[pos] = i;
+            P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, r.pdf); P.T[i] = v4(r.Tn, 0.f);
+        }
     }
     if (ms) {
         const int pos = outStart + waveAppend(ms, ctr(cnt, part, kCShadow));
@@ -832,6 +859,11 @@ void pickKernels(int tri, int sl, int waves, Kernels& k) {
 // layout-independent kernels (device/common.hip)
 __global__ void k_shade(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth, int smpBase, int K);
 __global__ void k_pass_begin(int* cnt, int words, long long NK);
+// continuation sort of the depth-(depth + 1) queue (P.qs, P.qk) into P.q[(depth + 1) & 1], per partition: bucket
+// sizes, then the scatter. Grids of kSortBlock-thread workgroups, a multiple of kParts.
+constexpr int kSortBlock = 256, kSortPer = 16, kSortTile = kSortBlock * kSortPer;
+__global__ void k_qcount(DevPaths P, int depth);
+__global__ void k_qscatter(DevPaths P, int depth);
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);
 __global__ void k_fill(uint32_t* p, size_t words, uint32_t v);

@@ -32,6 +32,7 @@ struct RenderOptions {
     int persistent = 0;     // 1: split kernels whose waves refill idle lanes with new rays mid-traversal (implies fuse 0)
     int shadowRefill = 0;   // shadow kernel: 1 = refill idle lanes mid-traversal, 0 = one ray per lane per round
     int graph = 0;          // 1: a frame is captured once into a HIP graph and replayed (per-kernel timing off)
+    int sort = 0;           // 1: between bounces, group each partition's continuations by direction bucket
 };
 
 struct FrameStats {
@@ -42,6 +43,7 @@ struct FrameStats {
     double hitRays = 0, contRays = 0, missRays = 0, unoccluded = 0;   // event counters (bytes-per-ray accounting)
     double cameraMiss = 0;                                             // camera rays that miss
     double shadeMs = 0;                  // split kernels: the shade kernels' part of bounceMs
+    double sortMs = 0;                   // continuation sort kernels (per-kernel timing on)
 };
 
 // Scene data ready for upload, in BVH leaf order.

@@ -100,7 +100,7 @@ struct Renderer::Impl {
     Kernels kn;
     BounceFn bounce = nullptr, shade = nullptr;   // fused: bounce; split: bounce = trace kernel, then shade
     ShadowFn shadow = nullptr;
-    int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0;
+    int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0, gridSort = 0;
     void* stageRays = nullptr; void* stageHits = nullptr; int* stageInst = nullptr; size_t stageCap = 0;   // traceHost staging
     // two-level scenes: instance records, their world boxes, the TLAS (H-PLOC each frame), BLAS roots per mesh
     bool inst = false, instSet = false;
@@ -125,6 +125,8 @@ struct Renderer::Impl {
         HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return true;
     }
+    // pass counters: kCntStride per depth (+1), then the continuation sort's bucket counters per depth
+    size_t counterWords() const { return (size_t)(depths + 1) * kCntStride + (O.sort ? (size_t)depths * kSortWords : 0); }
     hipEvent_t nextEvent() {
         if (evUsed == evs.size()) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) return nullptr; evs.push_back(e); }
         return evs[evUsed++];
@@ -272,6 +274,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.gridShadow = gridOf((const void*)I.shadow);
     I.gridTrace = std::max(gridOf((const void*)I.kn.traceC), gridOf((const void*)I.kn.traceA));
     I.maxGrid = std::max(I.gridBounce, std::max(I.gridShadow, I.gridTrace));
+    I.gridSort = std::max(kParts, 4 * numCU / kParts * kParts);
     gridBlocks = I.gridBounce;
 
     // stack spill beyond the LDS part: the tree bounds the entries a traversal can hold (two-level scenes: the
@@ -294,10 +297,14 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.depths = O.bounces + 2;
     if (!I.alloc(I.P.O, NK, err) || !I.alloc(I.P.D, NK, err) || !I.alloc(I.P.T, NK, err) || !I.alloc(I.P.L, NK, err) ||
         !I.alloc(I.P.q[0], NK, err) || !I.alloc(I.P.q[1], NK, err) || !I.alloc(I.P.sO, NK, err) ||
-        !I.alloc(I.P.sD, NK, err) || !I.alloc(I.P.sC, NK, err) || !I.alloc(I.P.cnt, (size_t)(I.depths + 1) * kCntStride, err) ||
+        !I.alloc(I.P.sD, NK, err) || !I.alloc(I.P.sC, NK, err) || !I.alloc(I.P.cnt, I.counterWords(), err) ||
         !I.alloc(I.accum, (size_t)I.N, err) || !I.alloc(I.tot, 8, err))
         return false;
     if (!O.fuse && !I.alloc(I.P.H, NK, err)) return false;
+    if (O.sort) {
+        if (!I.alloc(I.P.qs, NK, err) || !I.alloc(I.P.qk, NK, err)) return false;
+        I.P.bcnt = I.P.cnt + (size_t)(I.depths + 1) * kCntStride;
+    }
     HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     if (!I.fill(I.accum, (size_t)I.N * 16, err)) return false;
     HIPCHK(hipStreamSynchronize(I.st));
@@ -324,14 +331,14 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     if (sBase < 0) { err = "sample base must be >= 0"; return false; }
     S = FrameStats();
     I.evUsed = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv, sortEv;
     const bool timed = I.O.timeKernels && !I.O.graph;
     // the frame's launches: clears (by kernels, never memset nodes), then per pass the counter reset, trace+shade
     // and shadow per bounce, the reduction and the totals
     auto enqueue = [&]() -> bool {
         if (clear && !I.fill(I.accum, (size_t)I.N * 16, err)) return false;
         if (!I.fill(I.tot, 8 * sizeof(unsigned long long), err)) return false;
-        const int words = (I.depths + 1) * kCntStride;
+        const int words = (int)I.counterWords();
         for (int s = sBase; s < sBase + spp; s += I.K) {
             k_pass_begin<<<1, 64, 0, I.st>>>(I.P.cnt, words, I.NK);
             for (int depth = 0; depth < I.depths; depth++) {
@@ -349,6 +356,12 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                 I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
                 if (timed) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
+                if (I.P.qk && depth + 1 < I.depths) {
+                    if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
+                    k_qcount<<<I.gridSort, kSortBlock, 0, I.st>>>(I.P, depth);
+                    k_qscatter<<<I.gridSort, kSortBlock, 0, I.st>>>(I.P, depth);
+                    if (timed) { HIPCHK(hipEventRecord(b, I.st)); sortEv.push_back({ a, b }); }
+                }
             }
             k_reduce<<<std::min((I.N + kWave - 1) / kWave, 16384), kWave, 0, I.st>>>(I.P.L, I.accum, I.N, I.K);
             k_pass_end<<<1, 64, 0, I.st>>>(I.P.cnt, I.depths, I.tot);
@@ -387,6 +400,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     for (auto& e : bounceEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.bounceMs += ms; S.bounceLaunches++; }
     for (auto& e : shadowEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadowMs += ms; S.shadowLaunches++; }
     for (auto& e : shadeEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadeMs += ms; }
+    for (auto& e : sortEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.sortMs += ms; }
     S.traceMs = I.O.timeKernels ? S.bounceMs + S.shadowMs : S.frameMs;
     S.rays = (double)I.hostTot[0]; S.shadowRays = (double)I.hostTot[1];
     S.missRays = (double)I.hostTot[2]; S.unoccluded = (double)I.hostTot[3]; S.contRays = (double)I.hostTot[4];

@@ -99,7 +99,8 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--bvh_cache") ok = str(A.bvhCache);
         else if (k == "--gpu_collapse") ok = inum(A.gpuCollapse);
         else if (k == "--env_out") ok = str(A.envOut);
-        else if (k == "--devloop" || k == "--sort") {
+        else if (k == "--sort") ok = inum(A.sort);
+        else if (k == "--devloop") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
         } else { err = "unknown arg " + k; return false; }
@@ -118,6 +119,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         err = "--persistent must be 0, 1 (split kernels with refilled lanes) or 2 (fused kernel with refilled lanes)"; return false;
     }
     if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
+    if (A.sort != 0 && A.sort != 1) { err = "--sort must be 0 or 1"; return false; }
     if (A.fuse < 0) A.fuse = 1;
     if (A.persistent == 1) A.fuse = 0;   // persistent traversal 1 runs in the split trace kernel
     if (A.persistent == 2) A.fuse = 1;   // 2: the fused trace+shade kernel with refilled lanes
@@ -539,7 +541,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.variant = Z.variant; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
-    O.shadowRefill = A.shadowRefill;
+    O.shadowRefill = A.shadowRefill; O.sort = A.sort;
     O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;
 }
@@ -550,7 +552,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
     const int N = A.w * A.h;
     char buf[4096];
     snprintf(buf, sizeof buf,
-             "{\"rep\":%d,\"variant\":\"bvh%d\",\"tri\":\"%s\",\"persistent\":%d,\"devloop\":0,\"sort\":0,\"fuse\":%d,\"leaf\":%d,"
+             "{\"rep\":%d,\"variant\":\"bvh%d\",\"tri\":\"%s\",\"persistent\":%d,\"devloop\":0,\"sort\":%d,\"fuse\":%d,\"leaf\":%d,"
              "\"batch\":%d,\"minb\":0,\"scene\":\"%s\",\"tris\":%zu,\"bvh_nodes\":%zu,\"bvh_bytes\":%zu,\"bvh_depth\":%d,"
              "\"bvh_build_s\":%.2f,\"w\":%d,\"h\":%d,\"spp\":%d,\"bounces\":%d,\"rays_closest\":%.0f,\"rays_shadow\":%.0f,"
              "\"rays_total\":%.0f,\"t_trace_ms\":%.2f,\"t_frame_ms\":%.2f,\"mrays_s_trace\":%.1f,\"mrays_s_e2e\":%.1f,"
@@ -559,15 +561,15 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":%d,"
              "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f,"
-             "\"bvh_cache\":\"%s\"}",
-             rep, Z.variant, A.tri.c_str(), A.persistent, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
+             "\"bvh_cache\":\"%s\",\"t_sort_ms\":%.3f}",
+             rep, Z.variant, A.tri.c_str(), A.persistent, A.sort, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
              Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
              Z.treeMs, Z.postMs, Z.sah, R.options().graph, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
              A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
-             st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"));
+             st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"), st.sortMs);
     return buf;
 }
 

@@ -25,6 +25,7 @@ struct Args {
     int fuse = -1, leaf = 4, bins = 16, batch = 64, synthetic = 0, frame = 0, w = 512, h = 512, spp = 256, bounces = 8;
     int persistent = 0, shadowRefill = 0, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
     int bounceWaves = 8;
+    int sort = 0;                    // continuations grouped by direction bucket between bounces (same image)
     // full material integrator (reference: scene_setup.h:20-22)
     std::string materials, env, lightSampling = "mis", envSampling = "importance";
     float envStrength = -1.f, envRot = -1e30f;   // unset: the lighting file's (or 1, 0)

@@ -60,6 +60,7 @@ typedef struct cwp_frame_stats {
     double rays_hit, rays_miss, rays_cont, shadow_unoccluded;
     double camera_miss;               /* camera rays that miss (rays_miss includes them) */
     double t_shade_ms;                /* split kernels: shade-kernel part of t_bounce_ms (0 when fused) */
+    double t_sort_ms;                 /* continuation-sort kernels (options.sort; 0 when off or untimed) */
 } cwp_frame_stats;
 
 /* A scene in the renderer's upload layout: host arrays in BVH leaf order, copied by cwp_create_scene. Replaces the
@@ -93,6 +94,8 @@ typedef struct cwp_options {
     int bounce_waves;    /* fused kernel register budget: at least this many waves per SIMD (1, 6 or 8) */
     int time_kernels;    /* per-launch event timing (t_bounce_ms, t_shade_ms, t_shadow_ms) */
     int graph;           /* 1: each frame replays a captured HIP graph (per-launch timing off) */
+    int sort;            /* 1: each bounce's continuations are grouped by direction bucket before they are traced
+                            (reference option --sort, src/render/wavefront.cu:717; a different sort, same results) */
 } cwp_options;
 
 /* Scene and BVH summary of a setup. */

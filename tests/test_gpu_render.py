@@ -151,6 +151,48 @@ def test_kernel_schedules_give_the_same_image(scene, variant):
     assert psnr(img0[..., :3] / spp, acc[..., :3] / spp) >= 50.0
 
 
+SORT_CASES = [
+    ("terrain-bvh8", SCENES["terrain"] + ["--variant", "bvh8"]),
+    ("boxes-bvh2", SCENES["boxes"] + ["--variant", "bvh2"]),
+    ("terrain-split", SCENES["terrain"] + ["--fuse", "0"]),
+    ("terrain-refill", SCENES["terrain"] + ["--persistent", "2"]),
+    ("boxes-full", SCENES["boxes"] + ["--gen_materials", "3", "--gen_env", "64,32,2,1", "--graph", "0"]),
+    ("instances", ["--gen", "instances", "--gen_small", "1", "--gen_instances", "40", "--width", "96", "--height", "64"]),
+    ("terrain-graph", SCENES["terrain"] + ["--graph", "1"]),
+]
+
+
+@pytest.mark.parametrize("name,args", SORT_CASES, ids=[c[0] for c in SORT_CASES])
+def test_sorted_continuations_give_the_same_image(name, args):
+    """--sort 1 regroups each bounce's continuations by direction bucket before they are traced. Every path keeps its
+    own state and radiance, so the image and every counter are bit-identical to the unsorted schedule (a lost, doubled
+    or misplaced path id would change both), on the first frame and on a second frame that reuses the counters."""
+    print("case", name, flush=True)
+    spp = 16
+    base = args + ["--batch", "8", "--bounces", "6", "--spp", str(spp)]
+    out = {}
+    for sort in (0, 1):
+        r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort)]))
+        frames = []
+        for f in range(2):
+            if name == "instances":
+                r.set_frame(f)
+            st = r.render(spp, sample_base=f * spp)
+            frames.append((r.accum(), st))
+        out[sort] = frames
+        j = r.frame_json(frames[-1][1])
+        assert j["sort"] == sort
+        timed = "--graph" not in args or "0" == args[args.index("--graph") + 1]
+        assert (j["t_sort_ms"] > 0) == (sort == 1 and timed)
+        r.close()
+    for f in range(2):
+        (a, sa), (b, sb) = out[0][f], out[1][f]
+        np.testing.assert_array_equal(b, a, err_msg="frame %d" % f)
+        for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
+            assert sa[k] == sb[k], (f, k)
+    assert out[0][0][1]["rays_cont"] > 0
+
+
 def test_render_progressive_and_deterministic():
     s = cwp.Setup(SCENES["boxes"] + ["--batch", "8", "--bounces", "3"])
     r = cwp.Renderer(s)

@@ -45,6 +45,12 @@ def test_version_and_error_string():
         cwp.Setup(["--gen", "boxes", "--fuse", "2"])
     with pytest.raises(cwp.CwpError, match="--persistent"):
         cwp.Setup(["--gen", "boxes", "--persistent", "3"])
+    with pytest.raises(cwp.CwpError, match="--sort"):
+        cwp.Setup(["--gen", "boxes", "--sort", "2"])
+    with pytest.raises(cwp.CwpError, match="experimental"):
+        cwp.Setup(["--gen", "boxes", "--devloop", "1"])
+    cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0"])
+    assert cwp.Options.default().sort == 0
 
 
 def test_reference_example_fixtures(golden_dir):
