@@ -78,7 +78,7 @@ __device__ __forceinline__ void stNode(Vec4* p, Vec4 v) {
 // ------------------------------------------------------------------------------------------------ keys
 __global__ void k_hp_init(uint32_t* cbox, int* counter, int* status) {
     if (threadIdx.x < 3) { cbox[threadIdx.x] = 0xffffffffu; cbox[3 + threadIdx.x] = 0u; }
-    if (threadIdx.x == 0) { *counter = 0; *status = 0; }
+    if (threadIdx.x == 0) { *counter = 0; status[0] = 0; status[1] = 0; }
 }
 
 __global__ void __launch_bounds__(256) k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
@@ -160,7 +160,10 @@ __device__ __forceinline__ int blockScan256(int v, int* tmp, int& total) {
 }
 
 // digit histogram of one tile: counts[digit * tiles + tile]
-__global__ void __launch_bounds__(kSortBlock) k_rs_count(const uint64_t* __restrict__ keys, int n, int shift, int* counts, int tiles) {
+// gate (optional): the pass runs only if *gate is non-zero (a pass whose need is known only on the device)
+__global__ void __launch_bounds__(kSortBlock) k_rs_count(const uint64_t* __restrict__ keys, int n, int shift, int* counts, int tiles,
+                                                         const int* gate = nullptr) {
+    if (gate && *gate == 0) return;
     __shared__ int hist[256];
     hist[threadIdx.x] = 0;
     __syncthreads();
@@ -175,7 +178,8 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_count(const uint64_t* __restr
 
 // per digit (one workgroup per digit row of counts): exclusive scan of the row's tile counts in place, row total to
 // totals[digit]
-__global__ void __launch_bounds__(kSortBlock) k_rs_rowscan(int* counts, int tiles, int* totals) {
+__global__ void __launch_bounds__(kSortBlock) k_rs_rowscan(int* counts, int tiles, int* totals, const int* gate = nullptr) {
+    if (gate && *gate == 0) return;
     __shared__ int tmp[4];
     int* row = counts + (size_t)blockIdx.x * tiles;
     int carry = 0;
@@ -194,7 +198,8 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_rowscan(int* counts, int tile
 // totals (computed here by every workgroup) plus the row prefix of k_rs_rowscan
 __global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int n,
                                                            int shift, const int* __restrict__ offsets, int tiles,
-                                                           const int* __restrict__ totals) {
+                                                           const int* __restrict__ totals, const int* gate = nullptr) {
+    if (gate && *gate == 0) return;
     __shared__ int base[256], running[256], wh[4][256], tmp[4];
     const int t = threadIdx.x, w = t >> 6;
     int all;
@@ -222,15 +227,15 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const uint64_t* __res
     }
 }
 
-// whole sort in one workgroup (small n): every pass counts, scans and scatters a <-> b; the result ends in a if
-// `passes` is even, else in b
-__global__ void __launch_bounds__(kSortBlock) k_rs_single(uint64_t* a, uint64_t* b, int n, int passes) {
+// whole sort in one workgroup (small n): every pass (digits at shift0, shift0 + 8, ...) counts, scans and scatters
+// a <-> b; the result ends in a if `passes` is even, else in b
+__global__ void __launch_bounds__(kSortBlock) k_rs_single(uint64_t* a, uint64_t* b, int n, int shift0, int passes) {
     __shared__ int hist[256], running[256], wh[4][256], tmp[4];
     const int t = threadIdx.x, w = t >> 6;
     uint64_t* src = a;
     uint64_t* dst = b;
     for (int p = 0; p < passes; p++) {
-        const int shift = 8 * p;
+        const int shift = shift0 + 8 * p;
         hist[t] = 0;
         __syncthreads();
         for (int idx = t; idx < n; idx += kSortBlock) atomicAdd(&hist[(uint32_t)(src[idx] >> shift) & 255u], 1);
@@ -507,16 +512,20 @@ __global__ void k_rn_fill(int* theight, int tcap) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < tcap) theight[t] = -1;
 }
-__global__ void k_rn_keys(const int* __restrict__ theight, int used, uint64_t* keys) {
+__global__ void k_rn_keys(const int* __restrict__ theight, int used, uint64_t* keys, int* tall) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= used) return;
     const int h = theight[t];
+    if (h >= 255) atomicOr(tall, 1);   // the low height byte alone no longer orders the nodes (holes are 0xffff)
     keys[t] = ((uint64_t)(uint32_t)(h < 0 ? 0xffff : min(h, 0xfffe)) << 32) | (uint64_t)(uint32_t)t;
 }
-// final ids: the p-th slot in height order becomes node n + p; the n - 1 nodes must be exactly the non-hole slots
-__global__ void k_rn_map(const uint64_t* __restrict__ sorted, int used, int n, int* newId, int* status) {
+// final ids: the p-th slot in height order becomes node n + p; the n - 1 nodes must be exactly the non-hole slots.
+// The order is in `one` after the pass over the low height byte, in `two` when the high byte needed a pass too.
+__global__ void k_rn_map(const uint64_t* __restrict__ one, const uint64_t* __restrict__ two, const int* tall, int used,
+                         int n, int* newId, int* status) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= used) return;
+    const uint64_t* sorted = *tall ? two : one;
     const uint32_t h = (uint32_t)(sorted[p] >> 32), t = (uint32_t)sorted[p];
     const bool hole = h == 0xffff;
     if (hole != (p >= n - 1)) atomicOr(status, 1);
@@ -608,7 +617,7 @@ bool HplocBuilder::reserve(int n, std::string& err) {
     HCHK(hipMalloc(&rkA_, (size_t)tcap_ * 8));
     HCHK(hipMalloc(&rkB_, (size_t)tcap_ * 8));
     HCHK(hipMalloc(&newId_, (size_t)tcap_ * 4));
-    HCHK(hipMalloc(&status_, 4));
+    HCHK(hipMalloc(&status_, 8));   // [0] error bits, [1] a height reached 255
     HCHK(hipMalloc(&rangeL_, (size_t)n * 4));
     HCHK(hipMalloc(&rangeR_, (size_t)n * 4));
     // digit x tile counts, then the 256 digit totals (for the key sort and the renumbering sort)
@@ -630,18 +639,20 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     while ((1ll << ib) < n) ib++;
     const int mb = (64 - ib) / 3 < 21 ? (64 - ib) / 3 : 21;
     k_hp_keys<<<g, 256, 0, st>>>(lo, hi, n, cbox_, keysA_, mb, ib);
-    const int passes = (3 * mb + ib + 7) / 8;
+    // The keys start in primitive order and their low ib bits are the primitive index, so a stable sort on the Morton
+    // bits alone leaves equal codes in index order: the order of the full keys, with ib bits fewer to sort.
+    const int passes = (3 * mb + 7) / 8;
     const uint64_t* sorted = keysA_;
     if (n <= kSmallSort) {
-        k_rs_single<<<1, kSortBlock, 0, st>>>(keysA_, keysB_, n, passes);
+        k_rs_single<<<1, kSortBlock, 0, st>>>(keysA_, keysB_, n, ib, passes);
         sorted = (passes & 1) ? keysB_ : keysA_;
     } else {
         const int tiles = (n + kTile - 1) / kTile;
         uint64_t *src = keysA_, *dst = keysB_;
         for (int p = 0; p < passes; p++) {
-            k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, 8 * p, counts_, tiles);
+            k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, ib + 8 * p, counts_, tiles);
             k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, tiles, counts_ + 256 * (size_t)tiles);
-            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, 8 * p, counts_, tiles, counts_ + 256 * (size_t)tiles);
+            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, ib + 8 * p, counts_, tiles, counts_ + 256 * (size_t)tiles);
             std::swap(src, dst);
         }
         sorted = src;
@@ -651,17 +662,18 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
         k_lbvh_ranges<<<(n - 1 + 255) / 256, 256, 0, st>>>(sorted, n, rangeL_, rangeR_);
         k_hploc<<<(n - 1 + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, rangeL_, rangeR_, clusters_, cboxes_, bounds_,
                                                   counter_, n);
-        // internal nodes in height order: two 8-bit passes over the heights of every temporary slot (holes last)
+        // internal nodes in height order: 8-bit passes over the heights of every temporary slot (holes last); the pass
+        // over the high byte runs only when a height reaches 255 (k_rn_keys sets status_[1])
         const int T = tcap_, rt = (T + kTile - 1) / kTile;
-        k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_);
-        uint64_t *src = rkA_, *dst = rkB_;
+        k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_, status_ + 1);
         for (int p = 0; p < 2; p++) {
-            k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt);
-            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt);
-            k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt);
-            std::swap(src, dst);
+            uint64_t *src = p ? rkB_ : rkA_, *dst = p ? rkA_ : rkB_;
+            const int* gate = p ? status_ + 1 : nullptr;
+            k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt, gate);
+            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt, gate);
+            k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt, gate);
         }
-        k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(src, T, n, newId_, status_);
+        k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkA_, status_ + 1, T, n, newId_, status_);
         k_rn_write<<<(T + 255) / 256, 256, 0, st>>>(tnodes_, theight_, newId_, T, n, nodes_, status_);
     }
     HCHK(hipGetLastError());
