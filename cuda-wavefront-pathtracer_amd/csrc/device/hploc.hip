@@ -15,6 +15,7 @@ constexpr int kSortBlock = 256; // radix sort workgroup: 4 waves, one thread per
 constexpr int kSortItems = 16;  // keys per thread per pass
 constexpr int kTile = kSortBlock * kSortItems;
 constexpr int kSmallSort = 16384;   // up to this many keys are sorted by one workgroup (no inter-workgroup passes)
+constexpr int kRankSort = 4096;     // up to this many (unique) keys are placed by their rank in one launch
 
 // order-preserving float <-> uint mapping, for min / max through integer atomics
 __device__ __forceinline__ uint32_t f2o(float f) {
@@ -76,9 +77,12 @@ __device__ __forceinline__ void stNode(Vec4* p, Vec4 v) {
 }
 
 // ------------------------------------------------------------------------------------------------ keys
-__global__ void k_hp_init(uint32_t* cbox, int* counter, int* status) {
-    if (threadIdx.x < 3) { cbox[threadIdx.x] = 0xffffffffu; cbox[3 + threadIdx.x] = 0u; }
-    if (threadIdx.x == 0) { *counter = 0; status[0] = 0; status[1] = 0; }
+// build state reset; the grid also marks every temporary slot empty (height -1)
+__global__ void k_hp_init(uint32_t* cbox, int* counter, int* status, int* theight, int tcap) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3) { cbox[t] = 0xffffffffu; cbox[3 + t] = 0u; }
+    if (t == 0) { *counter = 0; status[0] = 0; status[1] = 0; }
+    if (t < tcap) theight[t] = -1;
 }
 
 __global__ void __launch_bounds__(256) k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
@@ -225,6 +229,25 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_scatter(const uint64_t* __res
         running[t] += wh[0][t] + wh[1][t] + wh[2][t] + wh[3][t];
         __syncthreads();
     }
+}
+
+// Sort of at most kRankSort unique keys in one launch: every key's position is the number of keys below it. Each
+// workgroup ranks its 256 keys against all n, staged through LDS 256 at a time (every lane reads the same entry:
+// broadcasts). Padding entries are ~0, above every key (keys use at most 63 bits).
+__global__ void __launch_bounds__(kSortBlock) k_rank_sort(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int n) {
+    __shared__ uint64_t tile[kSortBlock];
+    const int i = blockIdx.x * kSortBlock + (int)threadIdx.x;
+    const uint64_t key = i < n ? in[i] : ~0ull;
+    int rank = 0;
+    for (int b = 0; b < n; b += kSortBlock) {
+        const int j = b + (int)threadIdx.x;
+        tile[threadIdx.x] = j < n ? in[j] : ~0ull;
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < kSortBlock; k++) rank += tile[k] < key ? 1 : 0;
+        __syncthreads();
+    }
+    if (i < n) out[rank] = key;
 }
 
 // whole sort in one workgroup (small n): every pass (digits at shift0, shift0 + 8, ...) counts, scans and scatters
@@ -508,10 +531,6 @@ __global__ void __launch_bounds__(64) k_hploc(Vec4* tnodes, int* theight, int tc
 
 // ------------------------------------------------------------------------------------------------ renumbering
 // sort keys of the temporary slots: height (holes: 0xffff) above the slot index
-__global__ void k_rn_fill(int* theight, int tcap) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < tcap) theight[t] = -1;
-}
 __global__ void k_rn_keys(const int* __restrict__ theight, int used, uint64_t* keys, int* tall) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= used) return;
@@ -632,8 +651,7 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     if (!reserve(n, err)) return false;
     hipStream_t st = (hipStream_t)stream;
     const int g = (n + 255) / 256;
-    k_hp_init<<<1, 64, 0, st>>>(cbox_, counter_, status_);
-    k_rn_fill<<<(tcap_ + 255) / 256, 256, 0, st>>>(theight_, tcap_);
+    k_hp_init<<<(tcap_ + 255) / 256, 256, 0, st>>>(cbox_, counter_, status_, theight_, tcap_);
     k_hp_bounds<<<std::min(g, 512), 256, 0, st>>>(lo, hi, n, cbox_);
     int ib = 1;
     while ((1ll << ib) < n) ib++;
@@ -643,7 +661,10 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
     // bits alone leaves equal codes in index order: the order of the full keys, with ib bits fewer to sort.
     const int passes = (3 * mb + 7) / 8;
     const uint64_t* sorted = keysA_;
-    if (n <= kSmallSort) {
+    if (n <= kRankSort) {
+        k_rank_sort<<<(n + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(keysA_, keysB_, n);
+        sorted = keysB_;
+    } else if (n <= kSmallSort) {
         k_rs_single<<<1, kSortBlock, 0, st>>>(keysA_, keysB_, n, ib, passes);
         sorted = (passes & 1) ? keysB_ : keysA_;
     } else {
@@ -664,16 +685,22 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
                                                   counter_, n);
         // internal nodes in height order: 8-bit passes over the heights of every temporary slot (holes last); the pass
         // over the high byte runs only when a height reaches 255 (k_rn_keys sets status_[1])
+        // (few slots: the keys are unique, so one ranking launch orders them)
         const int T = tcap_, rt = (T + kTile - 1) / kTile;
         k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_, status_ + 1);
-        for (int p = 0; p < 2; p++) {
-            uint64_t *src = p ? rkB_ : rkA_, *dst = p ? rkA_ : rkB_;
-            const int* gate = p ? status_ + 1 : nullptr;
-            k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt, gate);
-            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt, gate);
-            k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt, gate);
+        if (T <= kRankSort) {
+            k_rank_sort<<<(T + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(rkA_, rkB_, T);
+            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkB_, status_ + 1, T, n, newId_, status_);
+        } else {
+            for (int p = 0; p < 2; p++) {
+                uint64_t *src = p ? rkB_ : rkA_, *dst = p ? rkA_ : rkB_;
+                const int* gate = p ? status_ + 1 : nullptr;
+                k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt, gate);
+                k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt, gate);
+                k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt, gate);
+            }
+            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkA_, status_ + 1, T, n, newId_, status_);
         }
-        k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkA_, status_ + 1, T, n, newId_, status_);
         k_rn_write<<<(T + 255) / 256, 256, 0, st>>>(tnodes_, theight_, newId_, T, n, nodes_, status_);
     }
     HCHK(hipGetLastError());
