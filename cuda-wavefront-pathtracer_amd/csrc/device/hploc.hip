@@ -88,10 +88,22 @@ __global__ void k_hp_init(uint32_t* cbox, int* counter, int* status, int* theigh
 __global__ void __launch_bounds__(256) k_hp_bounds(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, int n, uint32_t* cbox) {
     __shared__ uint32_t red[4][6];
     uint32_t a[3] = { 0xffffffffu, 0xffffffffu, 0xffffffffu }, b[3] = { 0u, 0u, 0u };
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Vec4 l = lo[i], h = hi[i];
-        const uint32_t c[3] = { f2o((l.x + h.x) * 0.5f), f2o((l.y + h.y) * 0.5f), f2o((l.z + h.z) * 0.5f) };
-        for (int k = 0; k < 3; k++) { a[k] = min(a[k], c[k]); b[k] = max(b[k], c[k]); }
+    // four boxes per thread per round: their loads are issued together (the rounds are latency-bound; more
+    // workgroups instead would put more atomics on the six bounds)
+    const int S = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * S) {
+        Vec4 l[4], h[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * S;
+            if (i < n) { l[u] = lo[i]; h[u] = hi[i]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + u * S >= n) break;
+            const uint32_t c[3] = { f2o((l[u].x + h[u].x) * 0.5f), f2o((l[u].y + h[u].y) * 0.5f), f2o((l[u].z + h[u].z) * 0.5f) };
+            for (int k = 0; k < 3; k++) { a[k] = min(a[k], c[k]); b[k] = max(b[k], c[k]); }
+        }
     }
     for (int k = 0; k < 3; k++)
         for (int off = 32; off > 0; off >>= 1) {
@@ -294,10 +306,8 @@ __global__ void __launch_bounds__(kSortBlock) k_rs_single(uint64_t* a, uint64_t*
 }
 
 // ------------------------------------------------------------------------------------------------ H-PLOC
-__global__ void k_hp_leaves(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, const uint64_t* __restrict__ keys,
-                            int n, int ib, Vec4* nodes, int* C, Vec4* CB, int* bounds) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void hpLeaf(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, const uint64_t* __restrict__ keys,
+                                       int i, int ib, Vec4* nodes, int* C, Vec4* CB, int* bounds) {
     const int p = (int)(keys[i] & ((1ull << ib) - 1ull));
     const Vec4 l = lo[p], h = hi[p];
     nodes[2 * i] = v4(xyz(l), asFloat(p));
@@ -458,9 +468,7 @@ __device__ void plocMerge(bool on, int L, int R, int s, bool root, int n, Vec4* 
 __device__ __forceinline__ int prefixLen(const uint64_t* __restrict__ keys, int n, int i, int j) {
     return (j < 0 || j > n - 1) ? -1 : (int)__builtin_clzll(keys[i] ^ keys[j]);
 }
-__global__ void k_lbvh_ranges(const uint64_t* __restrict__ keys, int n, int* rangeL, int* rangeR) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
+__device__ __forceinline__ void lbvhRange(const uint64_t* __restrict__ keys, int n, int i, int* rangeL, int* rangeR) {
     const int d = prefixLen(keys, n, i, i + 1) - prefixLen(keys, n, i, i - 1) > 0 ? 1 : -1;
     const int dmin = prefixLen(keys, n, i, i - d);
     int lmax = 2;
@@ -477,6 +485,15 @@ __global__ void k_lbvh_ranges(const uint64_t* __restrict__ keys, int n, int* ran
     } while (t > 1);
     const int g = i + s * d + min(d, 0);
     rangeL[g] = min(i, j); rangeR[g] = max(i, j);
+}
+
+// Leaf records (nodes, cluster slots, arrival counters) and the LBVH ranges in one launch: the leaves' gathers and
+// stores and the ranges' chains of dependent key loads overlap.
+__global__ void k_hp_leaves_ranges(const Vec4* __restrict__ lo, const Vec4* __restrict__ hi, const uint64_t* __restrict__ keys,
+                                   int n, int ib, Vec4* nodes, int* C, Vec4* CB, int* bounds, int* rangeL, int* rangeR) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hpLeaf(lo, hi, keys, i, ib, nodes, C, CB, bounds);
+    if (i < n - 1) lbvhRange(keys, n, i, rangeL, rangeR);
 }
 
 // H-PLOC over the LBVH nodes that merge: those whose range holds more than kT clusters, and the root. One lane per
@@ -678,9 +695,8 @@ bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, in
         }
         sorted = src;
     }
-    k_hp_leaves<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_);
+    k_hp_leaves_ranges<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_, rangeL_, rangeR_);
     if (n > 1) {
-        k_lbvh_ranges<<<(n - 1 + 255) / 256, 256, 0, st>>>(sorted, n, rangeL_, rangeR_);
         k_hploc<<<(n - 1 + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, rangeL_, rangeR_, clusters_, cboxes_, bounds_,
                                                   counter_, n);
         // internal nodes in height order: 8-bit passes over the heights of every temporary slot (holes last); the pass
