@@ -10,7 +10,8 @@
 //  2. the collapse's cost tables bottom-up: one lane per leaf climbs to the root, the second lane to reach a node
 //     computes it from both children (cost of the subtree as 1..7 wide-tree roots, decisions, the split of 8 roots),
 //     with device-coherent hand-over as in the H-PLOC climb;
-//  3. the wide nodes top-down, one launch per level: each node's children (the decisions' roots), the greedy octant
+//  3. the wide nodes top-down, one launch per level (its range read from device memory; the host checks once per 8
+//     levels): each node's children (the decisions' roots), the greedy octant
 //     slot assignment, leaf / internal flags;
 //  4. the host's numbering: its job stack visits a node's internal children in decreasing slot order and appends
 //     each node's children (and leaf primitives) when it pops the node, so child bases and triangle bases are prefix
@@ -33,7 +34,7 @@ public:
     CwbvhCollapser& operator=(const CwbvhCollapser&) = delete;
 
     // hnodes: device H-PLOC nodes of n primitives; nodesOut: device, room for 5 float4 per node (at most n nodes);
-    // primOut: device, n ints. Runs on `stream` and synchronises with it once per wide-tree level. numNodes: the
+    // primOut: device, n ints. Runs on `stream` and synchronises with it after the checks and once per 8 wide-tree levels. numNodes: the
     // wide nodes written; depth: wide-tree levels.
     bool run(const Vec4* hnodes, int n, Vec4* nodesOut, int* primOut, void* stream, int& numNodes, int& depth, std::string& err);
     // the root's cost as one wide root (after run): the collapse's SAH is this over the root box's area

@@ -14,6 +14,8 @@ constexpr unsigned char kDecLeaf = 0, kDecNode = 1, kDecPrev = 2, kDecDist = 10;
 constexpr float kInf = 3e38f;
 constexpr int kWaitVmcnt0 = 0x0F70;      // s_waitcnt vmcnt(0)
 
+constexpr int kLevelBatch = 8;   // wide levels launched between host checks
+
 // scratch arrays carved out of one allocation (binary nodes: N = 2n - 1; wide nodes: at most n)
 struct Scratch {
     int* parent; int* pcount; int* flag; int* pcnt;     // per binary node: parent, parent count, climb arrivals, primitives
@@ -22,6 +24,7 @@ struct Scratch {
     int* wBin; int* wSlot; int* wChild; int* wNInt; int* wLp; int* wT; int* wTP; int* wS; int* wP; int* wIdx;
     unsigned char* wLeaf;
     int* ctl;                                           // [0] error, [1] wide node count, [2] max leaf primitives
+    int* lv;                                            // level starts: level l is wide nodes [lv[l], lv[l + 1])
 };
 
 __device__ __forceinline__ Vec3 boxLo(const Vec4* hn, int h) { return xyz(hn[2 * h]); }
@@ -139,15 +142,12 @@ __device__ int collectRoots(const Vec4* hn, int n, const unsigned char* dec, int
     return m;
 }
 
-// One level of wide nodes [start, end): children (decision roots), greedy octant slot assignment (repeatedly the
-// (child, free slot) pair of lowest -dot(child centre - node centre, slot direction)), leaf flags, and the next
-// level's records appended at ctl[1].
-__global__ void __launch_bounds__(256) k_cb_level(const Vec4* __restrict__ hn, int n, int start, int end, int rootLeaf, int cap,
-                                                  Scratch W) {
-    const int X = start + blockIdx.x * blockDim.x + threadIdx.x;
-    if (X >= end) return;
+// One wide node X of a level: children (decision roots), greedy octant slot assignment (repeatedly the (child, free
+// slot) pair of lowest -dot(child centre - node centre, slot direction)), leaf flags, and the next level's records
+// appended at ctl[1]. The root is a single leaf when its table chose a leaf (at most pMax primitives) or n == 1.
+__device__ void cbLevelNode(const Vec4* __restrict__ hn, int n, int X, int cap, const Scratch& W) {
     const int b = W.wBin[X];
-    const bool asLeaf = X == 0 && rootLeaf;
+    const bool asLeaf = X == 0 && (n == 1 || W.dec[8 * (size_t)(2 * n - 2) + 1] == kDecLeaf);
     int ch[8], nc = 0;
     if (asLeaf) ch[nc++] = b;
     else {
@@ -204,6 +204,19 @@ __global__ void __launch_bounds__(256) k_cb_level(const Vec4* __restrict__ hn, i
             ord++;
         }
     }
+}
+
+// Level `l` of the wide tree, its range read from lv (the host does not wait for it): a grid-stride walk. Nothing runs
+// after an error (ctl[0]), and the range is clamped to the wide-node capacity.
+__global__ void __launch_bounds__(256) k_cb_level(const Vec4* __restrict__ hn, int n, int l, int cap, Scratch W) {
+    if (W.ctl[0]) return;
+    const int start = W.lv[l], end = min(W.lv[l + 1], cap);
+    for (int X = start + blockIdx.x * blockDim.x + threadIdx.x; X < end; X += gridDim.x * blockDim.x)
+        cbLevelNode(hn, n, X, cap, W);
+}
+// after level l: level l + 1 ends where the appends of level l ended
+__global__ void k_cb_snap(int l, Scratch W) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) W.lv[l + 2] = W.ctl[1];
 }
 
 // subtree totals of internal children and leaf primitives, one level at a time from the deepest
@@ -295,6 +308,7 @@ __global__ void k_cb_root(int n, Scratch W) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     W.wBin[0] = 2 * n - 2; W.wS[0] = 0; W.wP[0] = 0; W.wIdx[0] = 0;
     W.ctl[1] = 1;
+    W.lv[0] = 0; W.lv[1] = 1;
 }
 
 #define CCHK(x)                                                                                                        \
@@ -320,12 +334,13 @@ Scratch carve(void* mem, int n) {
     W.wBin = (int*)take(M * 4); W.wSlot = (int*)take(M * 32); W.wChild = (int*)take(M * 32); W.wNInt = (int*)take(M * 4);
     W.wLp = (int*)take(M * 4); W.wT = (int*)take(M * 4); W.wTP = (int*)take(M * 4); W.wS = (int*)take(M * 4);
     W.wP = (int*)take(M * 4); W.wIdx = (int*)take(M * 4); W.wLeaf = (unsigned char*)take(M);
+    W.lv = (int*)take((M + 2 + kLevelBatch) * 4);   // snapshots run up to a batch past the last level
     return W;
 }
 size_t scratchBytes(int n) {
     const size_t N = 2 * (size_t)n - 1, M = (size_t)n;
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(64) + 4 * r(N * 4) + r(N * 32) + r(N * 8) + r(M * 4) + 2 * r(M * 32) + 7 * r(M * 4) + r(M) + 256;
+    return r(64) + 4 * r(N * 4) + r(N * 32) + r(N * 8) + r(M * 4) + 2 * r(M * 32) + 7 * r(M * 4) + r(M) + r((M + 2 + kLevelBatch) * 4) + 256;
 }
 
 }  // namespace
@@ -367,26 +382,33 @@ bool CwbvhCollapser::run(const Vec4* hn, int n, Vec4* nodesOut, int* primOut, vo
     if (ctl[0]) { err = "CWBVH collapse: the H-PLOC nodes are not a tree over [0, n) (code " + std::to_string(ctl[0]) + ")"; return false; }
     k_cb_dp<<<(n + 255) / 256, 256, 0, st>>>(hn, n, W);
     k_cb_root<<<1, 64, 0, st>>>(n, W);
-    // root as a single leaf: the root's table chose a leaf (at most pMax primitives), or the root is a primitive
-    unsigned char rootDec[2] = { 0, 0 };
-    CCHK(hipMemcpyAsync(rootDec, W.dec + 8 * (size_t)(N - 1), 2, hipMemcpyDeviceToHost, st));
-    CCHK(hipMemcpyAsync(&rootCost_, W.cost + 8 * (size_t)(N - 1) + 1, 4, hipMemcpyDeviceToHost, st));
-    CCHK(hipStreamSynchronize(st));
-    const int rootLeaf = n == 1 || rootDec[1] == kDecLeaf;
-    rootLeaf_ = rootLeaf != 0;
-    std::vector<int> starts = { 0 };
-    int end = 1;
-    while (end > starts.back()) {
-        const int s0 = starts.back();
-        k_cb_level<<<(end - s0 + 255) / 256, 256, 0, st>>>(hn, n, s0, end, rootLeaf, n, W);
+    // The wide levels, kLevelBatch launches between host checks: each level reads its range from lv, and a snapshot
+    // after it records where the next level ends. The loop stops once the next level is empty (at most n levels:
+    // every non-empty level holds a wide node; levels after an empty one are empty and launch no work).
+    const int gL = std::max(1, std::min((n + 255) / 256, 2048));
+    int l = 0, lvPair[2] = { 0, 1 };
+    while (lvPair[1] > lvPair[0]) {
+        if (l >= n) { err = "CWBVH collapse: wide tree deeper than its primitive count"; return false; }
+        for (int k = 0; k < kLevelBatch; k++, l++) {
+            k_cb_level<<<gL, 256, 0, st>>>(hn, n, l, n, W);
+            k_cb_snap<<<1, 64, 0, st>>>(l, W);
+        }
         CCHK(hipMemcpyAsync(ctl, W.ctl, 12, hipMemcpyDeviceToHost, st));
         CCHK(hipStreamSynchronize(st));
         if (ctl[0]) { err = "CWBVH collapse: more wide nodes than primitives (code " + std::to_string(ctl[0]) + ")"; return false; }
-        starts.push_back(end);
-        end = ctl[1];
+        // levels 0 .. l - 1 are done; lv[l], lv[l + 1] bound level l (lv[l + 1]: the snapshot after level l - 1)
+        CCHK(hipMemcpy(lvPair, W.lv + l, 8, hipMemcpyDeviceToHost));
     }
-    // starts: level boundaries (the last entry is the total)
-    const int levels = (int)starts.size() - 1;
+    // levels: the non-empty ones (a prefix of 0 .. l - 1); starts: their boundaries (the last entry is the total)
+    std::vector<int> lvAll(l + 2);
+    CCHK(hipMemcpy(lvAll.data(), W.lv, (size_t)(l + 2) * 4, hipMemcpyDeviceToHost));
+    int levels = 0;
+    while (levels <= l && lvAll[levels + 1] > lvAll[levels]) levels++;
+    std::vector<int> starts(lvAll.begin(), lvAll.begin() + levels + 1);
+    unsigned char rootDec[2] = { 0, 0 };
+    CCHK(hipMemcpy(rootDec, W.dec + 8 * (size_t)(N - 1), 2, hipMemcpyDeviceToHost));
+    CCHK(hipMemcpy(&rootCost_, W.cost + 8 * (size_t)(N - 1) + 1, 4, hipMemcpyDeviceToHost));
+    rootLeaf_ = n == 1 || rootDec[1] == kDecLeaf;
     for (int l = levels - 1; l >= 0; l--)
         k_cb_totals<<<(starts[l + 1] - starts[l] + 255) / 256, 256, 0, st>>>(starts[l], starts[l + 1], W);
     for (int l = 0; l < levels; l++)
