@@ -24,9 +24,11 @@ def tris_of(lo, hi):
     return v, t
 
 
-def compare(lo, hi):
+def compare(lo, hi, g=None):
     n = len(lo)
-    g, root, _ = cwp.hploc_build(lo, hi)
+    if g is None:
+        g, root, _ = cwp.hploc_build(lo, hi)
+    root = 2 * n - 2 if n > 1 else 0
     v, t = tris_of(lo, hi)
     host = cwp.bvh_from_hploc(v, t, g, root, 8)
     nodes, order, depth, ms = cwp.cwbvh_from_hploc_gpu(g, n)
@@ -66,3 +68,30 @@ def test_collapse_rejects_a_malformed_tree():
     bad.view(np.int32)[70, 3] = 70   # an internal node naming itself as a child
     with pytest.raises(cwp.CwpError, match="not a tree"):
         cwp.cwbvh_from_hploc_gpu(bad, 64)
+
+
+def caterpillar(n):
+    """H-PLOC-layout nodes of a chain: internal node n + k joins the previous internal node (leaf 0 for k = 0) and leaf
+    k + 1; leaves are boxes along x, so the tree is as deep as it is large"""
+    lo = np.stack([np.arange(n), np.zeros(n), np.zeros(n)], 1).astype(np.float32)
+    hi = lo + np.float32(0.5)
+    g = np.zeros((2 * n - 1, 8), np.float32)
+    gi = g.view(np.int32)
+    g[:n, 0:3] = lo; g[:n, 4:7] = hi
+    gi[:n, 3] = np.arange(n); gi[:n, 7] = -1
+    for k in range(n - 1):
+        a, b = (0 if k == 0 else n + k - 1), k + 1
+        g[n + k, 0:3] = np.minimum(g[a, 0:3], g[b, 0:3]); g[n + k, 4:7] = np.maximum(g[a, 4:7], g[b, 4:7])
+        gi[n + k, 3] = a; gi[n + k, 7] = b
+    return lo, hi, g
+
+
+@pytest.mark.parametrize("n", [300, 1000])
+def test_collapse_matches_host_deep_chain(n):
+    """a binary chain collapses into a wide tree tens of levels deep: the device level loop runs several batches of
+    levels between its host checks and must still give the host's layout"""
+    print("case collapse chain n=%d" % n, flush=True)
+    lo, hi, g = caterpillar(n)
+    compare(lo, hi, g)
+    _, _, depth, _ = cwp.cwbvh_from_hploc_gpu(g, n)
+    assert depth > 8
