@@ -4,7 +4,7 @@ Host sources compile with g++, kernels with hipcc for gfx950 only. Both use -ffp
 are written explicitly in the sources, so host and device evaluate the same floating-point operations. Packed FP32
 instructions are disabled for device code.
 
-    python cuda-wavefront-pathtracer_amd/build.py [--jobs N] [--quiet]
+    python cuda-wavefront-pathtracer_amd/build.py [--jobs N] [--quiet] [--incremental]
 """
 import argparse
 import os
@@ -48,19 +48,40 @@ def _run(cmd, quiet):
         print(r.stdout, flush=True)
 
 
-def build(jobs=8, quiet=False):
+def _headers_mtime():
+    t = 0.0
+    for d in (CSRC, os.path.join(ROOT, "include")):
+        for base, _, files in os.walk(d):
+            for f in files:
+                if f.endswith(".h"):
+                    t = max(t, os.path.getmtime(os.path.join(base, f)))
+    return t
+
+
+def _stale(src, obj, headers):
+    """incremental builds: an object is rebuilt when it is older than its source or than any header"""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return t < os.path.getmtime(src) or t < headers
+
+
+def build(jobs=8, quiet=False, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     hc = hipcc()
     steps = []
     objs = []
+    headers = _headers_mtime() if incremental else 0.0
     for s in HOST_SOURCES + CLI_SOURCES:
         o = os.path.join(BUILD, s.replace("/", "_") + ".o")
-        steps.append(["g++"] + HOST_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, s), "-o", o])
+        if not incremental or _stale(os.path.join(CSRC, s), o, headers):
+            steps.append(["g++"] + HOST_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, s), "-o", o])
         if s not in CLI_SOURCES:
             objs.append(o)
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s.replace("/", "_") + ".o")
-        steps.append([hc] + HIP_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, s), "-o", o])
+        if not incremental or _stale(os.path.join(CSRC, s), o, headers):
+            steps.append([hc] + HIP_FLAGS + INCLUDES + ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for f in [ex.submit(_run, c, quiet) for c in steps]:
@@ -78,9 +99,10 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="rebuild only objects older than their source or a header")
     a = ap.parse_args()
     try:
-        print(build(a.jobs, a.quiet))
+        print(build(a.jobs, a.quiet, a.incremental))
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

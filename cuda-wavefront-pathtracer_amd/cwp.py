@@ -612,6 +612,10 @@ def bvh_from_hploc(vertices, triangles, hnodes, root, variant=8, leaf_max=4):
     v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
     t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
     h = np.ascontiguousarray(hnodes, dtype=np.float32)
+    # the C entry point reads 2 * (2n - 1) float4 of H-PLOC nodes for the n triangles
+    if h.size != (2 * t.shape[0] - 1) * 8:
+        raise ValueError("bvh_from_hploc: %d floats of H-PLOC nodes for %d triangles (expected %d)"
+                         % (h.size, t.shape[0], (2 * t.shape[0] - 1) * 8))
     p = ctypes.c_void_p()
     _check(lib().cwp_bvh_from_hploc(_fptr(v), v.shape[0], t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), t.shape[0],
                                     _fptr(h), int(root), variant, leaf_max, ctypes.byref(p)))
