@@ -209,6 +209,19 @@ int BatchRenderer::addScene(const SceneUpload& S, std::string& err) {
     };
     if (!ok || !makeDevLights(S, upAny, D.S.Lt, err)) { for (void* b : D.bufs) hipFree(b); return -1; }
     D.S.nodes = dn; D.S.verts = dv; D.S.nrm = dnr; D.S.albedo = da; D.S.root = S.root;
+    {   // per-triangle shading record (cross(e1, e2), material id), computed on the device from the corners
+        const int nt = (int)(S.verts.size() / 3);
+        void* q = nullptr;
+        if (hipMalloc(&q, (size_t)std::max(nt, 1) * sizeof(Vec4)) != hipSuccess) {
+            err = "batch: out of device memory"; for (void* b : D.bufs) hipFree(b); return -1;
+        }
+        D.bufs.push_back(q); D.bytes += (size_t)std::max(nt, 1) * sizeof(Vec4);
+        if (nt > 0) k_tri_geo<<<(nt + 255) / 256, 256, 0, I.st>>>(dv, nt, (Vec4*)q);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(I.st) != hipSuccess) {
+            err = "batch: shading-record kernel failed"; for (void* b : D.bufs) hipFree(b); return -1;
+        }
+        D.S.geo = (const Vec4*)q;
+    }
     D.S.tri = I.O.tri == kTriWoop ? dw : dv;
     D.S.tlas = nullptr; D.S.tlasRoot = 0; D.S.inst = nullptr;
     D.stackNeed = need; D.alive = true;
@@ -229,7 +242,8 @@ void BatchRenderer::removeScene(int id) {
 
 size_t BatchRenderer::uploadBytes(const SceneUpload& S, int tri) {
     const size_t nodes = S.variant == 8 ? S.nodes.size() / 5 * kBvh8Stride : S.nodes.size();
-    size_t b = (nodes + (tri == kTriWoop ? S.woop.size() : 0) + S.verts.size() + S.normals.size() + S.albedo.size()) * sizeof(Vec4);
+    size_t b = (nodes + (tri == kTriWoop ? S.woop.size() : 0) + S.verts.size() + S.verts.size() / 3 + S.normals.size() +
+                S.albedo.size()) * sizeof(Vec4);
     if (S.full) {
         b += S.mats.size() * sizeof(Material) + (S.triLightPdf.size() + S.emTri.size() + S.emCdf.size()) * 4;
         if (S.env) b += S.env->texels.size() * sizeof(Vec4) + (S.env->marg.size() + S.env->cond.size()) * 4;

@@ -168,6 +168,13 @@ __global__ void __launch_bounds__(kWave) k_reduce(const Vec4* __restrict__ L, Ve
     }
 }
 
+__global__ void k_tri_geo(const Vec4* __restrict__ verts, int nt, Vec4* __restrict__ geo) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const Vec4 a = verts[3 * t], b = verts[3 * t + 1], c = verts[3 * t + 2];
+    geo[t] = v4(cross(xyz(b) - xyz(a), xyz(c) - xyz(a)), b.w);
+}
+
 __global__ void k_fill(uint32_t* p, size_t words, uint32_t v) {
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < words; j += (size_t)gridDim.x * blockDim.x) p[j] = v;
 }

@@ -601,6 +601,10 @@ __global__ void k_hp_to_bvh2(const Vec4* __restrict__ hn, int n, Vec4* out) {
     int c[2], k[2];
     for (int q = 0; q < 2; q++) {
         const int x = ch[q];
+        if (x < 0 || x >= 2 * n - 1) {   // only an inconsistent build (reported by the builder's status) gets here
+            cl[q] = v4(E, E, E, asFloat(-1)); chh[q] = v4(E, E, E, asFloat(-1)); c[q] = -1; k[q] = 0;
+            continue;
+        }
         cl[q] = hn[2 * x]; chh[q] = hn[2 * x + 1];
         if (x < n) { c[q] = asInt(cl[q].w); k[q] = 1; } else { c[q] = x - n; k[q] = 0; }
     }

@@ -97,7 +97,8 @@ bool makeDevLights(const SceneUpload& S, Up&& up, DevLights& L, std::string& err
 struct DevScene {
     const Vec4* nodes;    // BVH layout (BVH8: 128-byte node stride)
     const Vec4* tri;      // traversal triangle data (Woop rows or corners)
-    const Vec4* verts;    // corners for shading
+    const Vec4* verts;    // corners (full-integrator shading, Moller-Trumbore and watertight tests)
+    const Vec4* geo;      // per triangle: cross(e1, e2) (unnormalised geometric normal), material id bits
     const Vec4* nrm;      // per-corner normals or null
     const Vec4* albedo;   // per material
     int root;
@@ -321,11 +322,12 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
     if (depth == 0) P.L[i] = v4(0.f, 0.f, 0.f, 0.f);
     if (depth > F.maxB) return out;
     const Vec3 p = o + d * h.t;
-    const Vec4 a = S.verts[3 * h.tri], b = S.verts[3 * h.tri + 1], c = S.verts[3 * h.tri + 2];
-    const Vec3 e1 = xyz(b) - xyz(a), e2 = xyz(c) - xyz(a);
+    // one 16-byte record per triangle instead of its 48 bytes of corners: cross(e1, e2) is computed once per scene by
+    // k_tri_geo with the operations this shader would apply, so the normal is bit-identical
+    const Vec4 g = S.geo[h.tri];
     Vec4 xr[3];   // two-level scenes: object-space normals go to world space through inv^T
     if constexpr (INST) { const InstanceRecord* I = S.inst + h.inst; xr[0] = I->inv[0]; xr[1] = I->inv[1]; xr[2] = I->inv[2]; }
-    Vec3 ng = INST ? normalize(xfRowsN(xr, cross(e1, e2))) : normalize(cross(e1, e2)), ns = ng;
+    Vec3 ng = INST ? normalize(xfRowsN(xr, xyz(g))) : normalize(xyz(g)), ns = ng;
     if (F.hasN) {
         Vec3 n0 = xyz(S.nrm[3 * h.tri]), n1 = xyz(S.nrm[3 * h.tri + 1]), n2 = xyz(S.nrm[3 * h.tri + 2]);
         Vec3 nn = n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v;
@@ -335,7 +337,7 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
         if (dot(ns, ng) < 0.f) ns = -ns;
     }
     if (dot(ng, d) > 0.f) { ng = -ng; ns = -ns; }
-    const Vec3 alb = xyz(S.albedo[asInt(b.w)]);
+    const Vec3 alb = xyz(S.albedo[asInt(g.w)]);
     out.po = p + ng * F.eps;
     out.so = out.po;
     const uint32_t dd = (uint32_t)depth + 1u;
@@ -868,6 +870,8 @@ __global__ void k_qscatter(DevPaths P, int depth);
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);
 __global__ void k_fill(uint32_t* p, size_t words, uint32_t v);
+// per triangle of a scene's corner array: (cross(e1, e2), material id bits of the second corner)
+__global__ void k_tri_geo(const Vec4* __restrict__ verts, int nt, Vec4* __restrict__ geo);
 __global__ void k_pass_begin_jobs(int* cnt, int words, const DevJobTable* T);
 __global__ void k_reduce_jobs(const DevJobTable* __restrict__ T, const Vec4* __restrict__ L);
 

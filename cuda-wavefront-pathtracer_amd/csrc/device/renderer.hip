@@ -56,8 +56,10 @@ __global__ void k_tlas_parents(const Vec4* __restrict__ hn, int n, int* parent) 
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n - 1) return;
     const int id = n + j;
-    parent[asInt(hn[2 * id].w)] = id;
-    parent[asInt(hn[2 * id + 1].w)] = id;
+    const int a = asInt(hn[2 * id].w), b = asInt(hn[2 * id + 1].w);
+    // child ids outside the tree only come from an inconsistent build, which the builder's status reports
+    if (a >= 0 && a < 2 * n - 1) parent[a] = id;
+    if (b >= 0 && b < 2 * n - 1) parent[b] = id;
 }
 __global__ void k_tlas_depth(const int* __restrict__ parent, int n, int* depth) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,6 +200,8 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (O.stackLds != 8 && O.stackLds != 16 && O.stackLds != 32) { err = "stack_lds must be 8, 16 or 32"; return false; }
     if (O.bounceWaves != 1 && O.bounceWaves != 6 && O.bounceWaves != 8) { err = "bounce_waves must be 1, 6 or 8"; return false; }
     if (O.bounces < 0 || O.bounces > 64) { err = "bounces must be in [0, 64]"; return false; }
+    // a frame launches bounces + 2 depths: a deeper path limit would end paths early without an error
+    if (F.maxB > O.bounces) { err = "max_bounce (" + std::to_string(F.maxB) + ") exceeds bounces (" + std::to_string(O.bounces) + ")"; return false; }
     I.O = O; I.F = F;
     I.sl = O.stackLds == 8 ? 8 : (O.stackLds == 32 ? 32 : 16);
     I.O.stackLds = I.sl;
@@ -218,6 +222,14 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (!I.upload(dn, devNodes, err) || !I.upload(dw, S.woop, err) || !I.upload(dv, S.verts, err) ||
         !I.upload(dnr, S.normals, err) || !I.upload(da, S.albedo, err)) return false;
     I.S.nodes = dn; I.S.verts = dv; I.S.nrm = dnr; I.S.albedo = da; I.S.root = S.root;
+    {
+        const int nt = (int)(S.verts.size() / 3);
+        Vec4* dg;
+        if (!I.alloc(dg, (size_t)std::max(nt, 1), err)) return false;
+        if (nt > 0) k_tri_geo<<<(nt + 255) / 256, 256, 0, I.st>>>(dv, nt, dg);
+        HIPCHK(hipGetLastError());
+        I.S.geo = dg;
+    }
     I.S.tri = O.tri == kTriWoop ? dw : dv;
     if (F.hasN && !dnr) { err = "hasN set without normals"; return false; }
     auto up = [&](auto& p, const auto& v, std::string& e) -> bool {
@@ -256,7 +268,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         if (!I.upload(mb, S.meshBox, err)) return false;
         I.dMeshBox = mb;
         HIPCHK(hipEventCreate(&I.tl0)); HIPCHK(hipEventCreate(&I.tl1));
-        HIPCHK(hipHostMalloc((void**)&I.hostDepth, sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&I.hostDepth, 2 * sizeof(int), hipHostMallocDefault));   // TLAS depth, build status
         if (!I.alloc(I.dDepth, 1, err)) return false;
     } else {
         I.bounce = O.fuse ? (O.persistent == 2 ? I.kn.bounceRefill : I.kn.bounce) : (O.persistent ? I.kn.tracePaths : I.kn.tracePlain);
@@ -317,6 +329,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
 bool Renderer::setParams(const FrameParams& F, std::string& err) {
     if (F.cam.w != p_->F.cam.w || F.cam.h != p_->F.cam.h) { err = "setParams cannot change the image size"; return false; }
     if (F.hasN && !p_->S.nrm) { err = "hasN set without normals"; return false; }
+    if (F.maxB > p_->O.bounces) { err = "max_bounce (" + std::to_string(F.maxB) + ") exceeds bounces (" + std::to_string(p_->O.bounces) + ")"; return false; }
     HIPCHK(hipStreamSynchronize(p_->st));   // queued frames read the previous parameters
     HIPCHK(hipMemcpy(p_->dF, &F, sizeof(FrameParams), hipMemcpyHostToDevice));
     p_->F = F;
@@ -481,6 +494,10 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
             err = "setInstances: instance " + std::to_string(i) + " has a bad mesh or BLAS root"; return false;
         }
     HIPCHK(hipStreamSynchronize(I.st));   // the previous frame may still read the TLAS
+    // from here on the previous instance set is gone: a failure below leaves the scene without instances (render then
+    // reports an error), and frames captured with the old TLAS are never replayed
+    I.instSet = false;
+    I.sceneGen++;
     if (n > I.instCap) {
         for (void* a : { (void*)I.dInst, (void*)I.dLo, (void*)I.dHi, (void*)I.dTlas, (void*)I.dParent }) if (a) hipFree(a);
         I.dInst = nullptr; I.dLo = I.dHi = I.dTlas = nullptr; I.dParent = nullptr; I.instCap = 0;
@@ -505,7 +522,12 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
     k_tlas_depth<<<(n + 255) / 256, 256, 0, I.st>>>(I.dParent, n, I.dDepth);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(I.hostDepth, I.dDepth, 4, hipMemcpyDeviceToHost, I.st));
+    HIPCHK(hipMemcpyAsync(I.hostDepth + 1, I.tlasB.status(), 4, hipMemcpyDeviceToHost, I.st));
     HIPCHK(hipStreamSynchronize(I.st));
+    if (I.hostDepth[1] != 0) {
+        err = "setInstances: inconsistent TLAS build (status " + std::to_string(I.hostDepth[1]) + ")";
+        return false;
+    }
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, I.tl0, I.tl1));
     tlasMs = ms;

@@ -308,14 +308,20 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
         const float tf = fminf(fminf(xf, yf), fminf(zf, tEntry));
         hitSlots |= (uint32_t)(tn <= tf) << s;
     }
-    // Phase 2: hit leaves in slot order, one shared loop
-    uint32_t lb = hitSlots & leafMask;
+    // Phase 2: the triangles of the hit leaves as one bit set relative to the node's triangle base (a node's leaves
+    // hold their triangles at increasing offsets in slot order, at most 24 in all), then one flat loop over its bits:
+    // the wave iterates as often as its lane with the most triangles, not once per leaf and again per triangle.
+    uint32_t lb = hitSlots & leafMask, tm = 0;
     while (lb) {
         const int s = __builtin_ctz(lb);
         lb &= lb - 1;
         const uint32_t m = byteOf(s < 4 ? meta[0] : meta[1], s);
-        const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
-        for (int t = first; t < first + cnt; t++) if (triTest<TRI>(tri, t, r, h) && ANY) return true;
+        tm |= ((2u << ((m >> 5) & 3u)) - 1u) << (m & 31u);
+    }
+    while (tm) {
+        const int j = __builtin_ctz(tm);
+        tm &= tm - 1;
+        if (triTest<TRI>(tri, triBase + j, r, h) && ANY) return true;
     }
     // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
     uint32_t hits = hitSlots & imask;

@@ -110,6 +110,9 @@ bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string
                 const int first = triBase + (int)(m & 31u), cnt = (int)((m >> 5) & 3u) + 1;
                 if (triBase < 0 || first < 0 || (size_t)first + (size_t)cnt > nt)
                     return fail(err, at + " slot " + std::to_string(s) + ": triangle range out of bounds");
+                // the traversal gathers a node's leaf triangles into one 32-bit set relative to the triangle base
+                if ((m & 31u) + (uint32_t)cnt > 32u)
+                    return fail(err, at + " slot " + std::to_string(s) + ": triangle offset beyond the node's 32 triangles");
             } else internal |= 1u << s;
         }
         if (internal != imask) return fail(err, at + ": internal-child mask disagrees with the slot metadata");
