@@ -1,5 +1,6 @@
 // capi.cpp - the extern "C" entry points of libcwp.so (declared in include/cwp.h).
 #include "cwp.h"
+#include "core/bsdf.h"
 #include "device/renderer.h"
 #include "host/bvh.h"
 #include "host/scene_check.h"
@@ -542,6 +543,31 @@ int cwp_bvh8_device_nodes(const float* nodes, size_t num_nodes, float* out) {
     std::string err;
     if (!deviceBVH8(n80, dev, err)) return fail(CWP_ERR_ARG, err);
     if (!dev.empty()) memcpy(out, dev.data(), dev.size() * sizeof(Vec4));
+    return CWP_OK;
+}
+
+static Material materialOf(const float* m) {
+    Material M;
+    static_assert(sizeof(Material) == 16 * sizeof(float), "material record");
+    memcpy(&M, m, sizeof(Material));
+    return M;
+}
+int cwp_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gd_energy, float ux1, float ux2,
+                  float f[3], float* pdf) {
+    if (!mat || !wo || !wi || !f || !pdf) return fail(CWP_ERR_ARG, "cwp_bsdf_eval: null argument");
+    const Vec3 o = v3(wo[0], wo[1], wo[2]);
+    const BsdfValue e = evalResponse(surfaceResponse(materialOf(mat), o, eta, gd_energy, ux1, ux2), o, v3(wi[0], wi[1], wi[2]));
+    f[0] = e.f.x; f[1] = e.f.y; f[2] = e.f.z; *pdf = e.pdf;
+    return CWP_OK;
+}
+int cwp_bsdf_sample(const float* mat, const float wo[3], float eta, int gd_energy, float ux1, float ux2, float u, float u1,
+                    float u2, float wi[3], float weight[3], float* pdf, int* ok) {
+    if (!mat || !wo || !wi || !weight || !pdf || !ok) return fail(CWP_ERR_ARG, "cwp_bsdf_sample: null argument");
+    const Vec3 o = v3(wo[0], wo[1], wo[2]);
+    Vec3 a = v3(0.f, 0.f, 0.f), w = v3(0.f, 0.f, 0.f);
+    float p = 0.f;
+    *ok = sampleResponse(surfaceResponse(materialOf(mat), o, eta, gd_energy, ux1, ux2), o, u, u1, u2, a, w, p) ? 1 : 0;
+    wi[0] = a.x; wi[1] = a.y; wi[2] = a.z; weight[0] = w.x; weight[1] = w.y; weight[2] = w.z; *pdf = p;
     return CWP_OK;
 }
 

@@ -153,8 +153,9 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     I.redGrid = (int)std::min<size_t>(16384, O.maxPaths / kWave + (size_t)O.maxJobs);
     I.depths = O.bounces + 2;
     const size_t n = O.maxPaths;
-    if (!I.alloc(I.P.O, n, err) || !I.alloc(I.P.D, n, err) || !I.alloc(I.P.T, n, err) || !I.alloc(I.P.L, n, err) ||
-        !I.alloc(I.P.q[0], n, err) || !I.alloc(I.P.q[1], n, err) || !I.alloc(I.P.sO, n, err) || !I.alloc(I.P.sD, n, err) ||
+    if (!I.alloc(I.P.O[0], n, err) || !I.alloc(I.P.D[0], n, err) || !I.alloc(I.P.T[0], n, err) ||
+        !I.alloc(I.P.O[1], n, err) || !I.alloc(I.P.D[1], n, err) || !I.alloc(I.P.T[1], n, err) || !I.alloc(I.P.L, n, err) ||
+        !I.alloc(I.P.sO, n, err) || !I.alloc(I.P.sD, n, err) ||
         !I.alloc(I.P.sC, n, err) || !I.alloc(I.P.cnt, (size_t)(I.depths + 1) * kCntStride, err) || !I.alloc(I.tot, 8, err))
         return false;
     HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));

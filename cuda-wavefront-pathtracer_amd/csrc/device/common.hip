@@ -14,8 +14,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, const 
     const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const int* qIn = P.q[depth & 1];
-    int* qOut = P.q[(depth + 1) & 1];
     int misses = 0;
     for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + lane;
@@ -23,26 +21,27 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, DevPaths P, const 
         ShadeOut res;
         res.cont = false; res.shadow = false; res.miss = false;
         if (k < n) {
-            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            const int slot = inStart + k;
             Vec3 o, d; float tmin;
-            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
-            const Vec4 hv = P.H[i];
+            pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
+            const Vec4 hv = P.H[slot];
             Hit h; h.t = hv.x; h.u = hv.y; h.v = hv.z; h.tri = asInt(hv.w);
-            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            const Vec3 T = pathThroughput(P, depth, slot);
             res = shadeLegacy(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
         }
-        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+        misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
 
-// Continuation sort. The continuations partition x appended at depth `depth` occupy [start, start + n) of P.qs / P.qk
-// (start: where partition x's depth-`depth` queue began; the same positions the unsorted append would use in
-// P.q[(depth + 1) & 1]). The workgroups of partition x (blockIdx % kParts, so they run on that partition's XCD) walk
-// its tiles of kSortTile entries. k_qcount adds each tile's bucket histogram (LDS atomics) into the partition's bucket
-// sizes; k_qscatter gives bucket b the range after buckets 0..b-1, reserves each tile's share of it with one atomic per
-// bucket and writes the path ids there, tile entries in the order of their LDS ranks. Which entries share a wave
-// afterwards depends on scheduling; the paths' results do not (every path keeps its own state and radiance).
+// Continuation sort. The continuations partition x appended at depth `depth` occupy [start, start + n) of the staging
+// rays P.stO / stD / stT and their keys P.qk (start: where partition x's depth-`depth` queue began; the same slots the
+// unsorted append would use in ray set (depth + 1) & 1). The workgroups of partition x (blockIdx % kParts, so they run
+// on that partition's XCD) walk its tiles of kSortTile entries. k_qcount adds each tile's bucket histogram (LDS
+// atomics) into the partition's bucket sizes; k_qscatter gives bucket b the range after buckets 0..b-1, reserves each
+// tile's share of it with one atomic per bucket and moves the rays there, tile entries in the order of their LDS ranks.
+// Which entries share a wave afterwards depends on scheduling; the paths' results do not (every path keeps its own
+// state and radiance).
 __device__ __forceinline__ void sortRange(const DevPaths& P, int depth, int part, int& start, int& n) {
     const int* cnt = P.cnt + depth * kCntStride;
     start = partStart(cnt, part);
@@ -77,7 +76,7 @@ __global__ void __launch_bounds__(kSortBlock) k_qscatter(DevPaths P, int depth) 
     sortRange(P, depth, part, start, n);
     int* size = P.bcnt + depth * kSortWords + part * 2 * kBuckets;
     int* cursor = size + kBuckets;
-    int* qOut = P.q[(depth + 1) & 1];
+    const int set = (depth + 1) & 1;
     if (t < kBuckets) {
         // exclusive prefix of the bucket sizes over the first wave's 64 lanes
         const int c = size[t];
@@ -93,19 +92,22 @@ __global__ void __launch_bounds__(kSortBlock) k_qscatter(DevPaths P, int depth) 
         if (t < kBuckets) h[t] = 0;
         __syncthreads();
         const int k0 = tile * kSortTile + t;
-        int id[kSortPer], key[kSortPer], rank[kSortPer];
+        int key[kSortPer], rank[kSortPer];
 #pragma unroll
         for (int j = 0; j < kSortPer; j++) {
             const int k = k0 + j * kSortBlock;
-            key[j] = -1; id[j] = 0; rank[j] = 0;
-            if (k < n) { key[j] = P.qk[start + k] & (kBuckets - 1); id[j] = P.qs[start + k]; rank[j] = atomicAdd(&h[key[j]], 1); }
+            key[j] = -1; rank[j] = 0;
+            if (k < n) { key[j] = P.qk[start + k] & (kBuckets - 1); rank[j] = atomicAdd(&h[key[j]], 1); }
         }
         __syncthreads();
         if (t < kBuckets && h[t]) base[t] = first[t] + atomicAdd(cursor + t, h[t]);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kSortPer; j++)
-            if (key[j] >= 0) qOut[base[key[j]] + rank[j]] = id[j];
+            if (key[j] >= 0) {
+                const int from = start + k0 + j * kSortBlock, to = base[key[j]] + rank[j];
+                P.O[set][to] = P.stO[from]; P.D[set][to] = P.stD[from]; P.T[set][to] = P.stT[from];
+            }
         __syncthreads();   // h and base are reused by the next tile
     }
 }

@@ -111,18 +111,22 @@ struct DevScene {
 
 // Path state. Path id i = pixel * K + sample: the 64 lanes of a wave start as 64 samples of one pixel, and each
 // partition's slice of the id space is a band of the image.
+// The rays of depth d >= 1 are stored at their queue slot, not at their path id: set d & 1 holds them, the slot is the
+// position the depth-(d-1) kernel appended them at, and the path id rides in O.w. A wave's appends are one contiguous
+// block (base + mbcnt), so the next bounce reads its 64 rays as whole cache lines, with no id-queue load in front of
+// them; only the radiance L stays indexed by path id.
 struct DevPaths {
-    Vec4 *O, *D, *T, *L;  // origin, direction, throughput, radiance per path id
-    Vec4* H;              // closest hit of the current bounce (t, u, v, triangle bits) when tracing and shading are split
-    int* q[2];            // ping-pong queues of path ids: depth d reads q[d & 1]
-    Vec4 *sO, *sD, *sC;   // shadow rays of the current bounce: (origin, path id bits), (dir, tmax), (contribution)
-    int* cnt;             // kCntStride counters per depth (one line per partition)
-    void* spill;          // traversal-stack spill slab, spillCap entries per thread
+    Vec4 *O[2], *D[2], *T[2];  // continuation rays by queue slot: (origin, path id bits), (direction, BSDF pdf), throughput
+    Vec4* L;                   // radiance per path id
+    Vec4* H;                   // closest hit by queue slot (t, u, v, triangle bits) when tracing and shading are split
+    Vec4 *sO, *sD, *sC;        // shadow rays of the current bounce: (origin, path id bits), (dir, tmax), (contribution)
+    int* cnt;                  // kCntStride counters per depth (one line per partition)
+    void* spill;               // traversal-stack spill slab, spillCap entries per thread
     int spillCap;
-    // continuation sort (null when off): continuations are appended to qs with their direction bucket in qk, and
-    // k_qcount / k_qscatter move them into q[(d + 1) & 1] grouped by bucket within each partition; bcnt holds per
-    // depth and partition the bucket sizes and fill cursors (kSortWords ints per depth)
-    int* qs = nullptr;
+    // continuation sort (null when off): continuations are appended to the staging rays stO / stD / stT with their
+    // direction bucket in qk, and k_qcount / k_qscatter move them into set (d + 1) & 1 grouped by bucket within each
+    // partition; bcnt holds per depth and partition the bucket sizes and fill cursors (kSortWords ints per depth)
+    Vec4 *stO = nullptr, *stD = nullptr, *stT = nullptr;
     uint8_t* qk = nullptr;
     int* bcnt = nullptr;
 };
@@ -351,17 +355,18 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
 
 #include "device/shade_full.h"
 
-// Wave-level appends of the shading results: continuations to partition `part` of the next queue, shadow rays to
-// partition `part` of the shadow queue. Returns the wave's number of misses.
-__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int* qOut, int i,
+// Wave-level appends of the shading results: continuations to partition `part` of the next queue (ray set
+// (depth + 1) & 1, or the sort's staging rays), shadow rays to partition `part` of the shadow queue. Returns the wave's
+// number of misses.
+__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int depth, int i,
                                              const ShadeOut& r) {
     const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
     if (mc) {
         const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
         if (r.cont) {
-            if (P.qk) { P.qs[pos] = i; P.qk[pos] = (uint8_t)dirBucket(r.nd); }
-            else qOut[pos] = i;
-            P.O[i] = v4(r.po, 0.f); P.D[i] = v4(r.nd, r.pdf); P.T[i] = v4(r.Tn, 0.f);
+            const Vec4 o = v4(r.po, asFloat(i)), d = v4(r.nd, r.pdf), t = v4(r.Tn, 0.f);
+            if (P.qk) { P.stO[pos] = o; P.stD[pos] = d; P.stT[pos] = t; P.qk[pos] = (uint8_t)dirBucket(r.nd); }
+            else { const int s = (depth + 1) & 1; P.O[s][pos] = o; P.D[s][pos] = d; P.T[s][pos] = t; }
         }
     }
     if (ms) {
@@ -371,11 +376,22 @@ __device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int pa
     return (int)__builtin_popcountll(mm);
 }
 
-// The ray of path i at this depth: its camera ray at depth 0 (recomputed from the path id), else the stored state.
-__device__ __forceinline__ void pathRay(const DevPaths& P, const FrameParams& F, int depth, int i, int smpBase, int K,
-                                        Vec3& o, Vec3& d, float& tmin) {
-    if (depth == 0) { cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
-    else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+// The ray at queue slot `slot` of this depth and its path id i: at depth 0 the slot is the path id and the camera ray
+// is computed from it; deeper, the stored ray of set depth & 1.
+__device__ __forceinline__ void pathRay(const DevPaths& P, const FrameParams& F, int depth, int slot, int smpBase, int K,
+                                        int& i, Vec3& o, Vec3& d, float& tmin) {
+    if (depth == 0) { i = slot; cameraRay(F, i / K, (uint32_t)(smpBase + i % K), o, d); tmin = F.nearT; }
+    else {
+        const Vec4 O = P.O[depth & 1][slot];
+        i = asInt(O.w); o = xyz(O); d = xyz(P.D[depth & 1][slot]); tmin = 0.f;
+    }
+}
+// throughput and last BSDF pdf of the ray at queue slot `slot` (camera rays: 1 and 0)
+__device__ __forceinline__ Vec3 pathThroughput(const DevPaths& P, int depth, int slot) {
+    return depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[depth & 1][slot]);
+}
+__device__ __forceinline__ float pathLastPdf(const DevPaths& P, int depth, int slot) {
+    return depth == 0 ? 0.f : P.D[depth & 1][slot].w;
 }
 
 // Queue position of the f-th entry a wave hands out: the wave walks the chunks lw, lw + nw, lw + 2 nw, ... of its
@@ -408,8 +424,6 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
     const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const int* qIn = P.q[depth & 1];
-    int* qOut = P.q[(depth + 1) & 1];
     int misses = 0;
     for (int base = lw * kWave; base < n; base += nw * kWave) {
         const int k = base + lane;
@@ -417,21 +431,21 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
         ShadeOut res;
         res.cont = false; res.shadow = false; res.miss = false;
         if (k < n) {
-            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            const int slot = inStart + k;
             Vec3 o, d; float tmin;
-            pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+            pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
             // throughput is fetched after the traversal so that no registers hold it across the traversal loop
-            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            const Vec3 T = pathThroughput(P, depth, slot);
             if constexpr (FULL) {
-                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                const float lastPdf = pathLastPdf(P, depth, slot);
                 res = shadeFull<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
             }
         }
-        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+        misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
@@ -439,7 +453,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
 // Closest hits of the depth-`depth` queue with persistent lanes and dynamic ray fetching (after Aila & Laine 2009;
 // reference: persistentTrace, src/render/wavefront.cu:403-434): a lane whose ray finishes stays idle until at least
 // REFILL lanes of its wave are idle, then they all take the next entries of the wave's share of its partition, so
-// long rays do not leave the rest of the wave empty. Hits go to H[path id] for k_shade.
+// long rays do not leave the rest of the wave empty. Hits go to H[queue slot] for k_shade.
 template <int V, int TRI, int SL, int REFILL>
 __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth, int smpBase, int K) {
     const FrameParams& F = *Fp;
@@ -451,8 +465,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
     const int* cnt = P.cnt + depth * kCntStride;
     const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
-    const int* qIn = P.q[depth & 1];
-    int i = -1;                       // path id of this lane's ray, -1 = idle
+    int i = -1;                       // queue slot of this lane's ray, -1 = idle
     RayPre r; Hit h; TravState ts;
     uint32_t f = 0;                   // wave-uniform: entries of the wave's share handed out so far
     bool more = fetchPos(0, lw, nw) < n;
@@ -463,9 +476,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_trace_paths(DevScene S, DevPaths 
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const int k = fetchPos(f + rank, lw, nw);
                 if (k < n) {
-                    i = depth == 0 ? inStart + k : qIn[inStart + k];
-                    Vec3 o, d; float tmin;
-                    pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+                    i = inStart + k;
+                    int pid; Vec3 o, d; float tmin;
+                    pathRay(P, F, depth, i, smpBase, K, pid, o, d, tmin);
                     r = prepareRay<TRI>(o, d, tmin);
                     h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1;
                     travBegin(ts, S.root);
@@ -505,24 +518,23 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_refill(DevScene S, Dev
     const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const int* qIn = P.q[depth & 1];
-    int* qOut = P.q[(depth + 1) & 1];
-    int i = -1;                       // path id of this lane's ray, -1 = idle
+    int slot = -1;                    // queue slot of this lane's ray, -1 = idle
+    int i = 0;                        // its path id
     bool tracing = false;
     RayPre r; Hit h; TravState ts;
     uint32_t f = 0;                   // wave-uniform: entries of the wave's share handed out so far
     bool more = fetchPos(0, lw, nw) < n;
     int misses = 0;
     for (;;) {
-        const uint64_t idle = __ballot(i < 0);
+        const uint64_t idle = __ballot(slot < 0);
         if (more && idle) {
-            if (i < 0) {
+            if (slot < 0) {
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const int k = fetchPos(f + rank, lw, nw);
                 if (k < n) {
-                    i = depth == 0 ? inStart + k : qIn[inStart + k];
+                    slot = inStart + k;
                     Vec3 o, d; float tmin;
-                    pathRay(P, F, depth, i, smpBase, K, o, d, tmin);
+                    pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
                     r = prepareRay<TRI>(o, d, tmin);
                     h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
                     travBegin(ts, S.root);
@@ -533,7 +545,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_refill(DevScene S, Dev
             f += (uint32_t)__builtin_popcountll(idle);
             more = fetchPos(f, lw, nw) < n;
         }
-        if (__ballot(i >= 0) == 0) break;   // no ray in flight: with `more` false the wave's share is done
+        if (__ballot(slot >= 0) == 0) break;   // no ray in flight: with `more` false the wave's share is done
         for (;;) {
             if (tracing) {
                 bool done;
@@ -543,22 +555,22 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_refill(DevScene S, Dev
                 if (done) tracing = false;
             }
             const uint64_t tr = __ballot(tracing);
-            if (tr == 0 || (more && (int)__builtin_popcountll(__ballot(i >= 0 && !tracing)) >= REFILL)) break;
+            if (tr == 0 || (more && (int)__builtin_popcountll(__ballot(slot >= 0 && !tracing)) >= REFILL)) break;
         }
-        const bool sh = i >= 0 && !tracing;
+        const bool sh = slot >= 0 && !tracing;
         ShadeOut res;
         res.cont = false; res.shadow = false; res.miss = false;
         if (sh) {
-            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            const Vec3 T = pathThroughput(P, depth, slot);
             if constexpr (FULL) {
-                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                const float lastPdf = pathLastPdf(P, depth, slot);
                 res = shadeFull<false>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<false>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), r.o, r.d, T, h);
             }
         }
-        misses += appendResults(P, cnt, part, outStart, qOut, sh ? i : 0, res);
-        if (sh) i = -1;
+        misses += appendResults(P, cnt, part, outStart, depth, sh ? i : 0, res);
+        if (sh) slot = -1;
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
@@ -726,8 +738,6 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
     const int n = ctrv(cnt, part, kCQueue);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const int* qIn = P.q[depth & 1];
-    int* qOut = P.q[(depth + 1) & 1];
     int misses = 0;
     for (int b0 = lw * kWave; b0 < n; b0 += nw * kWave) {
         const int k = b0 + lane;
@@ -735,25 +745,28 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
         ShadeOut res;
         res.cont = false; res.shadow = false; res.miss = false;
         if (k < n) {
-            i = depth == 0 ? inStart + k : qIn[inStart + k];
+            const int slot = inStart + k;
+            Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
+            if (depth == 0) i = slot;
+            else { Or = P.O[depth & 1][slot]; i = asInt(Or.w); }
             const int j = jobOf(base, nJobs, i);
             const DevJob& J = jobs[j];
             const int li = i - base[j], Kj = J.K, pix = li / Kj;
             const uint32_t smp = (uint32_t)(J.smpBase + li % Kj);
             Vec3 o, d; float tmin;
             if (depth == 0) { cameraRay(J.F, pix, smp, o, d); tmin = J.F.nearT; }
-            else { o = xyz(P.O[i]); d = xyz(P.D[i]); tmin = 0.f; }
+            else { o = xyz(Or); d = xyz(P.D[depth & 1][slot]); tmin = 0.f; }
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
-            const Vec3 T = depth == 0 ? v3(1.f, 1.f, 1.f) : xyz(P.T[i]);
+            const Vec3 T = pathThroughput(P, depth, slot);
             if constexpr (FULL) {
-                const float lastPdf = depth == 0 ? 0.f : P.D[i].w;
+                const float lastPdf = pathLastPdf(P, depth, slot);
                 res = shadeFull<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
             }
         }
-        misses += appendResults(P, cnt, part, outStart, qOut, i, res);
+        misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }

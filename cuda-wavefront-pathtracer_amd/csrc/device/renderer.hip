@@ -307,14 +307,18 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (NK > (size_t)0x7fffffff) { err = "width * height * batch exceeds 2^31 paths"; return false; }
     I.NK = (int)NK;
     I.depths = O.bounces + 2;
-    if (!I.alloc(I.P.O, NK, err) || !I.alloc(I.P.D, NK, err) || !I.alloc(I.P.T, NK, err) || !I.alloc(I.P.L, NK, err) ||
-        !I.alloc(I.P.q[0], NK, err) || !I.alloc(I.P.q[1], NK, err) || !I.alloc(I.P.sO, NK, err) ||
+    // ray sets by queue slot: depth d >= 1 reads set d & 1 and appends to the other (set 0 is read from depth 2 on,
+    // depth 1 reads set 1)
+    if (!I.alloc(I.P.O[0], NK, err) || !I.alloc(I.P.D[0], NK, err) || !I.alloc(I.P.T[0], NK, err) ||
+        !I.alloc(I.P.O[1], NK, err) || !I.alloc(I.P.D[1], NK, err) || !I.alloc(I.P.T[1], NK, err) ||
+        !I.alloc(I.P.L, NK, err) || !I.alloc(I.P.sO, NK, err) ||
         !I.alloc(I.P.sD, NK, err) || !I.alloc(I.P.sC, NK, err) || !I.alloc(I.P.cnt, I.counterWords(), err) ||
         !I.alloc(I.accum, (size_t)I.N, err) || !I.alloc(I.tot, 8, err))
         return false;
     if (!O.fuse && !I.alloc(I.P.H, NK, err)) return false;
     if (O.sort) {
-        if (!I.alloc(I.P.qs, NK, err) || !I.alloc(I.P.qk, NK, err)) return false;
+        if (!I.alloc(I.P.stO, NK, err) || !I.alloc(I.P.stD, NK, err) || !I.alloc(I.P.stT, NK, err) ||
+            !I.alloc(I.P.qk, NK, err)) return false;
         I.P.bcnt = I.P.cnt + (size_t)(I.depths + 1) * kCntStride;
     }
     HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));

@@ -152,7 +152,9 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     const Vec3 poUp = p + ng * F.eps, poDn = p - ng * F.eps;
     const float ux1 = rnd((uint32_t)pix, smp, dd, 7, F.seed), ux2 = rnd((uint32_t)pix, smp, dd, 7, F.seed ^ 0x68E31DA4u);
     // ---- next-event estimation: one light sample
-    if (bsdfHasNonDelta(M)) {
+    // the material's response is rebuilt where it is used (light sample, BSDF sample), so it holds no registers across
+    // the light-selection code
+    if (hasDensity(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2))) {
         const float us = rnd((uint32_t)pix, smp, dd, 5, F.seed);
         const float u2 = rnd((uint32_t)pix, smp, dd, 2, F.seed), u3 = rnd((uint32_t)pix, smp, dd, 3, F.seed);
         Vec3 l = v3(0.f, 0.f, 1.f), Le = v3(0.f, 0.f, 0.f);
@@ -212,7 +214,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
             const Vec3 lL = v3(dot(l, tx), dot(l, ty), dot(l, ns));
             const float gl = dot(ng, l);
             if (lL.z != 0.f && (gl > 0.f) == (lL.z > 0.f)) {
-                const BsdfValue e = bsdfEval(M, woL, lL, eta, E.gdEnergy, ux1, ux2);
+                const BsdfValue e = evalResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2), woL, lL);
                 const float w = (isSun || E.lightMode == kLightNEE) ? 1.f : misPower(pl, e.pdf);
                 const Vec3 con = T * e.f * Le * (fabsf(lL.z) * w / pl);
                 if (con.x > 0.f || con.y > 0.f || con.z > 0.f) {
@@ -226,8 +228,8 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     // ---- BSDF sample: the continuation
     Vec3 wiL, wgt;
     float pdf;
-    if (!bsdfSample(M, woL, eta, E.gdEnergy, ux1, ux2, rnd((uint32_t)pix, smp, dd, 4, F.seed),
-                    rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed), wiL, wgt, pdf))
+    if (!sampleResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2), woL, rnd((uint32_t)pix, smp, dd, 4, F.seed),
+                        rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed), wiL, wgt, pdf))
         return out;
     const Vec3 wi = normalize(tx * wiL.x + ty * wiL.y + ns * wiL.z);
     const float gi = dot(ng, wi);

@@ -166,6 +166,7 @@ def scene_check(variant, nodes, corners, albedo, woop=None, normals=None, tri_fo
 
 # declared symbols of include/cwp.h with their signatures
 _P, _I, _D, _S = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+_F, _FP = ctypes.c_float, ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "cwp_last_error": ([], ctypes.c_char_p),
     "cwp_version": ([], _I),
@@ -231,6 +232,8 @@ SIGNATURES = {
     "cwp_device_malloc": ([_S, ctypes.POINTER(_P)], _I),
     "cwp_device_free": ([_P], None),
     "cwp_memcpy": ([_P, _P, _S, _I], _I),
+    "cwp_bsdf_eval": ([_FP, _FP, _FP, _F, _I, _F, _F, _FP, _FP], _I),
+    "cwp_bsdf_sample": ([_FP, _FP, _F, _I, _F, _F, _F, _F, _F, _FP, _FP, _FP, ctypes.POINTER(_I)], _I),
 }
 
 _lib = None
@@ -535,6 +538,30 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def bsdf_eval(mat, wo, wi, eta, gd_energy=1, ux1=0.5, ux2=0.5):
+    """f (3,) and pdf of the full integrator's material response (the kernels' code, run on the host)."""
+    m = np.ascontiguousarray(mat, dtype=np.float32).reshape(16)
+    a = np.ascontiguousarray(wo, dtype=np.float32).reshape(3)
+    b = np.ascontiguousarray(wi, dtype=np.float32).reshape(3)
+    f = np.zeros(3, np.float32)
+    p = np.zeros(1, np.float32)
+    _check(lib().cwp_bsdf_eval(_fptr(m), _fptr(a), _fptr(b), eta, gd_energy, ux1, ux2, _fptr(f), _fptr(p)))
+    return f, float(p[0])
+
+
+def bsdf_sample(mat, wo, eta, u, u1, u2, gd_energy=1, ux1=0.5, ux2=0.5):
+    """(ok, wi (3,), weight (3,), pdf) of one sample of the material response; pdf 0 marks a delta sample."""
+    m = np.ascontiguousarray(mat, dtype=np.float32).reshape(16)
+    a = np.ascontiguousarray(wo, dtype=np.float32).reshape(3)
+    wi = np.zeros(3, np.float32)
+    w = np.zeros(3, np.float32)
+    p = np.zeros(1, np.float32)
+    ok = ctypes.c_int(0)
+    _check(lib().cwp_bsdf_sample(_fptr(m), _fptr(a), eta, gd_energy, ux1, ux2, u, u1, u2, _fptr(wi), _fptr(w), _fptr(p),
+                                 ctypes.byref(ok)))
+    return bool(ok.value), wi, w, float(p[0])
 
 
 def bvh8_device_nodes(nodes):

@@ -335,6 +335,17 @@ int cwp_server_get_options(const cwp_server* s, cwp_server_options* o);
  * timing excludes the first uploads) */
 int cwp_server_preload(cwp_server* s);
 
+/* ---- material response on the host (the code the full-integrator kernels run, compiled for the CPU) ----
+ * mat: the 64-byte material record (16 floats: baseColor, metallic, specularColor, roughness, emission, ior,
+ * specularWeight, transmission, 2 unused). Local frame z = shading normal, wo.z > 0. eta = index ratio of the
+ * dielectric base for this side; gd_energy / ux1 / ux2 = glossy-diffuse energy compensation and its random numbers.
+ * Replace the reference's bsdfEval / bsdfSample (src/render/bsdf.h:196-220) for host-side checks. */
+int cwp_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gd_energy, float ux1, float ux2,
+                  float f[3], float* pdf);
+/* returns CWP_OK and sets *ok (1: a direction was sampled; pdf 0 marks a delta sample) */
+int cwp_bsdf_sample(const float* mat, const float wo[3], float eta, int gd_energy, float ux1, float ux2, float u, float u1,
+                    float u2, float wi[3], float weight[3], float* pdf, int* ok);
+
 /* ---- device memory for callers without a GPU array library ---- */
 enum { CWP_COPY_H2D = 1, CWP_COPY_D2H = 2, CWP_COPY_D2D = 3 };
 int cwp_device_malloc(size_t bytes, void** ptr);
