@@ -3,7 +3,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cctype>
 #include <cstring>
+#include <algorithm>
 #include <map>
 
 namespace cwp {
@@ -27,89 +29,159 @@ std::vector<double> Json::numbers(const std::string& key) const {
 }
 
 namespace {
-struct Parser {
-    const std::string& s; size_t i = 0; std::string err;
-    explicit Parser(const std::string& t) : s(t) {}
-    void ws() { while (i < s.size() && (s[i] == ' ' || s[i] == '\t' || s[i] == '\n' || s[i] == '\r')) i++; }
-    bool fail(const char* m) { if (err.empty()) err = std::string(m) + " at offset " + std::to_string(i); return false; }
-    static void utf8(std::string& out, unsigned cp) {
-        if (cp < 0x80) out += (char)cp;
-        else if (cp < 0x800) { out += (char)(0xC0 | (cp >> 6)); out += (char)(0x80 | (cp & 63)); }
-        else { out += (char)(0xE0 | (cp >> 12)); out += (char)(0x80 | ((cp >> 6) & 63)); out += (char)(0x80 | (cp & 63)); }
-    }
-    bool string(std::string& out) {
-        if (i >= s.size() || s[i] != '"') return fail("expected string");
-        i++;
-        while (i < s.size() && s[i] != '"') {
-            char c = s[i++];
-            if (c != '\\') { out += c; continue; }
-            if (i >= s.size()) break;
-            char e = s[i++];
-            switch (e) {
-                case 'n': out += '\n'; break; case 't': out += '\t'; break; case 'r': out += '\r'; break;
-                case 'b': out += '\b'; break; case 'f': out += '\f'; break;
-                case 'u': if (i + 4 <= s.size()) { utf8(out, (unsigned)strtoul(s.substr(i, 4).c_str(), nullptr, 16)); i += 4; } break;
-                default: out += e;
-            }
-        }
-        if (i >= s.size()) return fail("unterminated string");
-        i++;
-        return true;
-    }
-    bool value(Json& v, int depth) {
-        if (depth > 200) return fail("nesting too deep");
-        ws();
-        if (i >= s.size()) return fail("unexpected end");
-        char c = s[i];
-        if (c == '{') {
-            v.type = Json::Object; i++; ws();
-            if (i < s.size() && s[i] == '}') { i++; return true; }
-            for (;;) {
-                ws();
-                std::pair<std::string, Json> kv;
-                if (!string(kv.first)) return fail("expected key");
-                ws();
-                if (i >= s.size() || s[i] != ':') return fail("expected ':'");
-                i++;
-                if (!value(kv.second, depth + 1)) return false;
-                v.obj.push_back(std::move(kv));
-                ws();
-                if (i < s.size() && s[i] == ',') { i++; continue; }
-                if (i < s.size() && s[i] == '}') { i++; return true; }
-                return fail("expected ',' or '}'");
-            }
-        }
-        if (c == '[') {
-            v.type = Json::Array; i++; ws();
-            if (i < s.size() && s[i] == ']') { i++; return true; }
-            for (;;) {
-                Json e;
-                if (!value(e, depth + 1)) return false;
-                v.arr.push_back(std::move(e));
-                ws();
-                if (i < s.size() && s[i] == ',') { i++; continue; }
-                if (i < s.size() && s[i] == ']') { i++; return true; }
-                return fail("expected ',' or ']'");
-            }
-        }
-        if (c == '"') { v.type = Json::String; return string(v.str); }
-        if (s.compare(i, 4, "true") == 0) { v.type = Json::Bool; v.b = true; i += 4; return true; }
-        if (s.compare(i, 5, "false") == 0) { v.type = Json::Bool; v.b = false; i += 5; return true; }
-        if (s.compare(i, 4, "null") == 0) { v.type = Json::Null; i += 4; return true; }
-        char* e; double x = strtod(s.c_str() + i, &e);
-        if (e == s.c_str() + i) return fail("unexpected character");
-        v.type = Json::Number; v.num = x; i = (size_t)(e - s.c_str());
-        return true;
-    }
+
+// JSON in two stages: a lexer that turns the text into tokens (strings unescaped, numbers converted), and an assembler
+// that builds the tree with an explicit stack of the containers still open, so nesting depth costs no native stack.
+struct Token {
+    enum Kind { OpenObj, CloseObj, OpenArr, CloseArr, Colon, Comma, Str, Num, True, False, Null, End, Bad } kind = Bad;
+    size_t at = 0;
+    std::string text;
+    double num = 0;
 };
+
+class JsonLexer {
+public:
+    explicit JsonLexer(const std::string& s) : s_(s) {}
+    Token next() {
+        while (pos_ < s_.size() && (s_[pos_] == ' ' || s_[pos_] == '\t' || s_[pos_] == '\n' || s_[pos_] == '\r')) pos_++;
+        Token t;
+        t.at = pos_;
+        if (pos_ >= s_.size()) { t.kind = Token::End; return t; }
+        const char c = s_[pos_];
+        switch (c) {
+            case '{': pos_++; t.kind = Token::OpenObj; return t;
+            case '}': pos_++; t.kind = Token::CloseObj; return t;
+            case '[': pos_++; t.kind = Token::OpenArr; return t;
+            case ']': pos_++; t.kind = Token::CloseArr; return t;
+            case ':': pos_++; t.kind = Token::Colon; return t;
+            case ',': pos_++; t.kind = Token::Comma; return t;
+            case '"': return stringToken(t);
+            default: break;
+        }
+        static const struct { const char* word; Token::Kind kind; } words[] = { { "true", Token::True }, { "false", Token::False },
+                                                                                { "null", Token::Null } };
+        for (const auto& w : words) {
+            const size_t n = strlen(w.word);
+            if (s_.compare(pos_, n, w.word) == 0) { pos_ += n; t.kind = w.kind; return t; }
+        }
+        const char* b = s_.c_str() + pos_;
+        char* e = nullptr;
+        const double v = strtod(b, &e);
+        if (e == b) { t.kind = Token::Bad; t.text = "unexpected character"; return t; }
+        pos_ += (size_t)(e - b);
+        t.kind = Token::Num; t.num = v;
+        return t;
+    }
+
+private:
+    static void appendUtf8(std::string& out, uint32_t cp) {
+        if (cp < 0x80) { out += (char)cp; return; }
+        if (cp < 0x800) { out += (char)(0xC0 | (cp >> 6)); out += (char)(0x80 | (cp & 0x3F)); return; }
+        out += (char)(0xE0 | (cp >> 12)); out += (char)(0x80 | ((cp >> 6) & 0x3F)); out += (char)(0x80 | (cp & 0x3F));
+    }
+    Token stringToken(Token& t) {
+        pos_++;   // opening quote
+        for (;;) {
+            if (pos_ >= s_.size()) { t.kind = Token::Bad; t.text = "unterminated string"; return t; }
+            const char c = s_[pos_++];
+            if (c == '"') break;
+            if (c != '\\') { t.text += c; continue; }
+            if (pos_ >= s_.size()) { t.kind = Token::Bad; t.text = "unterminated string"; return t; }
+            const char e = s_[pos_++];
+            const char* from = "nrtbf";
+            const char* to = "\n\r\t\b\f";
+            if (const char* k = strchr(from, e); k && e) { t.text += to[k - from]; continue; }
+            if (e == 'u') {
+                uint32_t cp = 0;
+                int digits = 0;
+                for (; digits < 4 && pos_ < s_.size() && isxdigit((unsigned char)s_[pos_]); digits++, pos_++)
+                    cp = cp * 16 + (uint32_t)(isdigit((unsigned char)s_[pos_]) ? s_[pos_] - '0' : (tolower((unsigned char)s_[pos_]) - 'a' + 10));
+                if (digits == 4) appendUtf8(t.text, cp);
+                continue;
+            }
+            t.text += e;   // \" \\ \/ and anything else: the character itself
+        }
+        t.kind = Token::Str;
+        return t;
+    }
+    const std::string& s_;
+    size_t pos_ = 0;
+};
+
+bool jsonError(std::string* err, const char* what, size_t at) {
+    if (err) *err = std::string(what) + " at offset " + std::to_string(at);
+    return false;
+}
+
 }  // namespace
 
 bool parseJson(const std::string& text, Json& out, std::string* err) {
-    Parser p(text); out = Json();
-    bool ok = p.value(out, 0);
-    if (ok) { p.ws(); if (p.i != text.size()) ok = p.fail("trailing characters"); }
-    if (!ok && err) *err = p.err;
-    return ok;
+    out = Json();
+    JsonLexer lex(text);
+    // each open container, and for an object the key whose value comes next
+    struct Open { Json* v; std::string key; };
+    std::vector<Open> open;
+    bool done = false;                 // the top-level value is complete
+    enum Expect { Value, KeyOrClose, Key, ColonNext, CommaOrClose, ValueOrClose } want = Value;
+    for (;;) {
+        Token t = lex.next();
+        if (done) {
+            if (t.kind != Token::End) return jsonError(err, "trailing characters", t.at);
+            return true;
+        }
+        if (t.kind == Token::Bad) return jsonError(err, t.text.c_str(), t.at);
+        if (t.kind == Token::End) return jsonError(err, "unexpected end", t.at);
+        if (open.size() > 200) return jsonError(err, "nesting too deep", t.at);
+        // the slot the next value goes to
+        auto place = [&]() -> Json* {
+            if (open.empty()) return &out;
+            Open& o = open.back();
+            if (o.v->type == Json::Array) { o.v->arr.emplace_back(); return &o.v->arr.back(); }
+            o.v->obj.emplace_back(o.key, Json());
+            return &o.v->obj.back().second;
+        };
+        auto closeOne = [&]() {
+            open.pop_back();
+            if (open.empty()) done = true;
+            want = CommaOrClose;
+        };
+        switch (want) {
+            case KeyOrClose:
+            case Key:
+                if (want == KeyOrClose && t.kind == Token::CloseObj) { closeOne(); continue; }
+                if (t.kind != Token::Str) return jsonError(err, "expected key", t.at);
+                open.back().key = t.text;
+                want = ColonNext;
+                continue;
+            case ColonNext:
+                if (t.kind != Token::Colon) return jsonError(err, "expected ':'", t.at);
+                want = Value;
+                continue;
+            case CommaOrClose: {
+                const bool inObj = open.back().v->type == Json::Object;
+                if (t.kind == Token::Comma) { want = inObj ? Key : Value; continue; }
+                if (t.kind == (inObj ? Token::CloseObj : Token::CloseArr)) { closeOne(); continue; }
+                return jsonError(err, inObj ? "expected ',' or '}'" : "expected ',' or ']'", t.at);
+            }
+            case ValueOrClose:
+                if (t.kind == Token::CloseArr) { closeOne(); continue; }
+                [[fallthrough]];
+            case Value: {
+                Json* v = nullptr;
+                switch (t.kind) {
+                    case Token::OpenObj: v = place(); v->type = Json::Object; open.push_back({ v }); want = KeyOrClose; continue;
+                    case Token::OpenArr: v = place(); v->type = Json::Array; open.push_back({ v }); want = ValueOrClose; continue;
+                    case Token::Str: v = place(); v->type = Json::String; v->str = t.text; break;
+                    case Token::Num: v = place(); v->type = Json::Number; v->num = t.num; break;
+                    case Token::True: case Token::False: v = place(); v->type = Json::Bool; v->b = t.kind == Token::True; break;
+                    case Token::Null: v = place(); v->type = Json::Null; break;
+                    default: return jsonError(err, "unexpected character", t.at);
+                }
+                if (open.empty()) done = true; else want = CommaOrClose;
+                continue;
+            }
+        }
+    }
 }
 
 std::string readTextFile(const std::string& path) {
@@ -236,170 +308,249 @@ bool loadCameraFile(const std::string& path, const std::string& key, int frame, 
     return true;
 }
 
-// ------------------------------------------------------------------------------------------------------------- OBJ
-static std::string dirOf(const std::string& p) {
-    size_t k = p.find_last_of("/\\");
+// ------------------------------------------------------------------------------------------------------ text meshes
+namespace {
+
+std::string dirOf(const std::string& p) {
+    const size_t k = p.find_last_of("/\\");
     return k == std::string::npos ? std::string(".") : p.substr(0, k);
 }
-static void loadMTL(const std::string& path, Mesh& m, std::map<std::string, int>& ids) {
-    FILE* fp = fopen(path.c_str(), "rb");
-    if (!fp) { fprintf(stderr, "warn: mtl %s not found\n", path.c_str()); return; }
-    char line[1024]; int cur = -1;
-    while (fgets(line, sizeof line, fp)) {
-        if (strncmp(line, "newmtl", 6) == 0) {
-            char name[512];
-            if (sscanf(line + 6, "%511s", name) == 1) {
-                cur = (int)m.kd.size(); ids[name] = cur; m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
-                m.matName.resize(m.kd.size()); m.matName[cur] = name;
-            }
-        } else if (cur >= 0 && strncmp(line, "Kd", 2) == 0) {
-            float r, g, b;
-            if (sscanf(line + 2, "%f %f %f", &r, &g, &b) == 3) m.kd[cur] = v3(r, g, b);
+
+// A text file read line by line, each line split at blanks into words (no word longer than the line).
+class WordReader {
+public:
+    explicit WordReader(const std::string& path) : fp_(fopen(path.c_str(), "rb")) {}
+    ~WordReader() { if (fp_) fclose(fp_); }
+    bool ok() const { return fp_ != nullptr; }
+    // the words of the next line (false at end of file)
+    bool line(std::vector<std::string>& words) {
+        words.clear();
+        std::string cur;
+        int c;
+        bool any = false;
+        while ((c = fgetc(fp_)) != EOF) {
+            any = true;
+            if (c == '\n') break;
+            if (c == ' ' || c == '\t' || c == '\r') { if (!cur.empty()) { words.push_back(cur); cur.clear(); } continue; }
+            cur += (char)c;
         }
+        if (!cur.empty()) words.push_back(cur);
+        return any;
     }
-    fclose(fp);
+private:
+    FILE* fp_;
+};
+
+bool parseFloats(const std::vector<std::string>& w, size_t from, float* out, int n) {
+    if (w.size() < from + (size_t)n) return false;
+    for (int k = 0; k < n; k++) {
+        char* e = nullptr;
+        out[k] = strtof(w[from + k].c_str(), &e);
+        if (e == w[from + k].c_str()) return false;
+    }
+    return true;
 }
 
-// OBJ faces are fanned from their first corner; "v/t/n" normals are scattered onto the vertices; usemtl selects an
-// MTL material by name (unknown names -> material 0) (reference: src/io/mesh_loader.cpp:14-62).
-bool loadOBJ(const std::string& path, Mesh& m) {
-    FILE* fp = fopen(path.c_str(), "rb");
-    if (!fp) return false;
-    std::vector<Vec3> vn; std::map<std::string, int> ids; int curMat = 0;
-    if (m.kd.empty()) { m.kd.push_back(v3(0.8f, 0.8f, 0.8f)); m.matName.push_back(""); }
-    std::vector<char> line(1 << 16);
-    std::vector<int> vi, ni;
-    while (fgets(line.data(), (int)line.size(), fp)) {
-        char* s = line.data();
-        if (s[0] == 'v' && s[1] == ' ') { float x, y, z; if (sscanf(s + 2, "%f %f %f", &x, &y, &z) == 3) m.v.push_back(v3(x, y, z)); }
-        else if (s[0] == 'v' && s[1] == 'n') { float x, y, z; if (sscanf(s + 2, "%f %f %f", &x, &y, &z) == 3) vn.push_back(v3(x, y, z)); }
-        else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
-            vi.clear(); ni.clear();
-            char* p = s + 1;
-            while (*p) {
-                while (*p == ' ' || *p == '\t') p++;
-                if (*p == '\n' || *p == '\r' || *p == 0) break;
-                int a = atoi(p), b = 0; bool hasNrm = false;
-                char* q = p; while (*q && *q != ' ' && *q != '\t' && *q != '\n' && *q != '\r') q++;
-                char* sl = (char*)memchr(p, '/', q - p);
-                if (sl) { char* sl2 = (char*)memchr(sl + 1, '/', q - sl - 1); if (sl2 && sl2 + 1 < q) { b = atoi(sl2 + 1); hasNrm = true; } }
-                vi.push_back(a > 0 ? a - 1 : (int)m.v.size() + a);
-                ni.push_back(hasNrm ? (b > 0 ? b - 1 : (int)vn.size() + b) : -1);
-                p = q;
-            }
-            for (size_t k = 1; k + 1 < vi.size(); k++) { m.f.push_back({ vi[0], vi[k], vi[k + 1] }); m.fm.push_back(curMat); }
-            for (size_t k = 0; k < vi.size(); k++)
-                if (ni[k] >= 0 && ni[k] < (int)vn.size() && vi[k] >= 0 && vi[k] < (int)m.v.size()) {
-                    if (m.n.size() != m.v.size()) m.n.resize(m.v.size(), v3(0, 0, 0));
-                    m.n[vi[k]] = vn[ni[k]];
-                }
-        } else if (strncmp(s, "mtllib", 6) == 0) {
-            char name[1024]; if (sscanf(s + 6, "%1023s", name) == 1) loadMTL(dirOf(path) + "/" + name, m, ids);
-        } else if (strncmp(s, "usemtl", 6) == 0) {
-            char name[1024]; curMat = 0;
-            if (sscanf(s + 6, "%1023s", name) == 1) { auto it = ids.find(name); if (it != ids.end()) curMat = it->second; }
+// MTL: "newmtl name" opens a material (diffuse 0.8 until a "Kd r g b" line sets it)
+void readMTL(const std::string& path, Mesh& m, std::map<std::string, int>& byName) {
+    WordReader in(path);
+    if (!in.ok()) { fprintf(stderr, "warn: mtl %s not found\n", path.c_str()); return; }
+    std::vector<std::string> w;
+    int cur = -1;
+    while (in.line(w)) {
+        if (w.empty()) continue;
+        if (w[0] == "newmtl" && w.size() >= 2) {
+            cur = (int)m.kd.size();
+            byName[w[1]] = cur;
+            m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+            m.matName.resize(m.kd.size());
+            m.matName[cur] = w[1];
+        } else if (w[0] == "Kd" && cur >= 0) {
+            float c[3];
+            if (parseFloats(w, 1, c, 3)) m.kd[cur] = v3(c[0], c[1], c[2]);
         }
     }
-    fclose(fp);
-    if (!m.n.empty()) {
-        bool any = false; for (const Vec3& x : m.n) if (length(x) > 0) { any = true; break; }
-        if (!any) m.n.clear();
+}
+
+// one OBJ index (1-based, or negative: counted back from the end of the list so far); 0 or garbage gives an index
+// outside the list, which the mesh check rejects
+int objIndex(const std::string& s, int listSize) {
+    const int k = atoi(s.c_str());
+    return k > 0 ? k - 1 : listSize + k;
+}
+
+}  // namespace
+
+// OBJ: "v x y z", "vn x y z", faces "f a b c ..." whose corners are "v", "v/t", "v//n" or "v/t/n" (fanned from the
+// first corner into triangles), "mtllib file" (relative to the OBJ), "usemtl name" (unknown names: material 0). A
+// corner's normal is stored on its vertex; a mesh whose normals are all zero has none (reference behaviour:
+// src/io/mesh_loader.cpp:14-62).
+bool loadOBJ(const std::string& path, Mesh& m) {
+    WordReader in(path);
+    if (!in.ok()) return false;
+    if (m.kd.empty()) { m.kd.push_back(v3(0.8f, 0.8f, 0.8f)); m.matName.push_back(""); }
+    std::vector<Vec3> normals;
+    std::map<std::string, int> byName;
+    int material = 0;
+    std::vector<std::string> w;
+    std::vector<int> cv, cn;   // corners of the current face: vertex, normal (-1: none)
+    while (in.line(w)) {
+        if (w.empty()) continue;
+        const std::string& k = w[0];
+        float x[3];
+        if (k == "v") { if (parseFloats(w, 1, x, 3)) m.v.push_back(v3(x[0], x[1], x[2])); }
+        else if (k == "vn") { if (parseFloats(w, 1, x, 3)) normals.push_back(v3(x[0], x[1], x[2])); }
+        else if (k == "f") {
+            cv.clear(); cn.clear();
+            for (size_t j = 1; j < w.size(); j++) {
+                const std::string& c = w[j];
+                const size_t s1 = c.find('/'), s2 = s1 == std::string::npos ? s1 : c.find('/', s1 + 1);
+                cv.push_back(objIndex(c, (int)m.v.size()));
+                const bool hasN = s2 != std::string::npos && s2 + 1 < c.size();
+                cn.push_back(hasN ? objIndex(c.substr(s2 + 1), (int)normals.size()) : -1);
+            }
+            for (size_t j = 2; j < cv.size(); j++) { m.f.push_back({ cv[0], cv[j - 1], cv[j] }); m.fm.push_back(material); }
+            for (size_t j = 0; j < cv.size(); j++) {
+                if (cn[j] < 0 || cn[j] >= (int)normals.size() || cv[j] < 0 || cv[j] >= (int)m.v.size()) continue;
+                if (m.n.size() != m.v.size()) m.n.resize(m.v.size(), v3(0.f, 0.f, 0.f));
+                m.n[cv[j]] = normals[cn[j]];
+            }
+        } else if (k == "mtllib" && w.size() >= 2) readMTL(dirOf(path) + "/" + w[1], m, byName);
+        else if (k == "usemtl") {
+            const auto it = w.size() >= 2 ? byName.find(w[1]) : byName.end();
+            material = it == byName.end() ? 0 : it->second;
+        }
     }
+    bool anyNormal = false;
+    for (const Vec3& x : m.n) anyNormal = anyNormal || length(x) > 0.f;
+    if (!anyNormal) m.n.clear();
     return !m.f.empty();
 }
 
 // ------------------------------------------------------------------------------------------------------------- PLY
 namespace {
-struct PlyProp { std::string name, type, ltype, itype; bool list = false; };
-struct PlyElem { std::string name; long count = 0; std::vector<PlyProp> props; };
-int plySize(const std::string& t) {
-    if (t == "char" || t == "uchar" || t == "int8" || t == "uint8") return 1;
-    if (t == "short" || t == "ushort" || t == "int16" || t == "uint16") return 2;
-    if (t == "double" || t == "float64") return 8;
-    return 4;
+
+enum class PlyType { I8, U8, I16, U16, I32, U32, F32, F64 };
+bool plyType(const std::string& t, PlyType& out) {
+    static const struct { const char* a; const char* b; PlyType t; } names[] = {
+        { "char", "int8", PlyType::I8 }, { "uchar", "uint8", PlyType::U8 }, { "short", "int16", PlyType::I16 },
+        { "ushort", "uint16", PlyType::U16 }, { "int", "int32", PlyType::I32 }, { "uint", "uint32", PlyType::U32 },
+        { "float", "float32", PlyType::F32 }, { "double", "float64", PlyType::F64 } };
+    for (const auto& n : names) if (t == n.a || t == n.b) { out = n.t; return true; }
+    return false;
 }
-double plyBin(FILE* fp, const std::string& t) {
-    unsigned char b[8] = { 0 }; int n = plySize(t);
-    if (fread(b, 1, n, fp) != (size_t)n) return 0;
-    if (t == "char" || t == "int8") return (double)(int8_t)b[0];
-    if (t == "uchar" || t == "uint8") return (double)b[0];
-    if (t == "short" || t == "int16") { int16_t x; memcpy(&x, b, 2); return x; }
-    if (t == "ushort" || t == "uint16") { uint16_t x; memcpy(&x, b, 2); return x; }
-    if (t == "int" || t == "int32") { int32_t x; memcpy(&x, b, 4); return x; }
-    if (t == "uint" || t == "uint32") { uint32_t x; memcpy(&x, b, 4); return x; }
-    if (t == "float" || t == "float32") { float x; memcpy(&x, b, 4); return x; }
-    if (t == "double" || t == "float64") { double x; memcpy(&x, b, 8); return x; }
-    return 0;
-}
+
+struct PlyProperty { std::string name; bool list = false; PlyType count = PlyType::U8, value = PlyType::F32; };
+struct PlyElement { std::string name; long n = 0; std::vector<PlyProperty> props; };
+
+// scalar values from the body of a PLY file, ASCII words or little-endian binary
+class PlyBody {
+public:
+    PlyBody(FILE* fp, bool ascii) : fp_(fp), ascii_(ascii) {}
+    double read(PlyType t) {
+        if (ascii_) { double x = 0; return fscanf(fp_, "%lf", &x) == 1 ? x : 0.0; }
+        unsigned char b[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        static const int bytes[] = { 1, 1, 2, 2, 4, 4, 4, 8 };
+        const int n = bytes[(int)t];
+        if (fread(b, 1, (size_t)n, fp_) != (size_t)n) return 0.0;
+        switch (t) {
+            case PlyType::I8: return (double)(int8_t)b[0];
+            case PlyType::U8: return (double)b[0];
+            case PlyType::I16: { int16_t x; memcpy(&x, b, 2); return x; }
+            case PlyType::U16: { uint16_t x; memcpy(&x, b, 2); return x; }
+            case PlyType::I32: { int32_t x; memcpy(&x, b, 4); return x; }
+            case PlyType::U32: { uint32_t x; memcpy(&x, b, 4); return x; }
+            case PlyType::F32: { float x; memcpy(&x, b, 4); return x; }
+            case PlyType::F64: { double x; memcpy(&x, b, 8); return x; }
+        }
+        return 0.0;
+    }
+    // a list property: its values (or nothing when `keep` is null)
+    void list(const PlyProperty& p, std::vector<double>* keep) {
+        const long n = (long)read(p.count);
+        if (keep) keep->clear();
+        for (long k = 0; k < n; k++) { const double x = read(p.value); if (keep) keep->push_back(x); }
+    }
+private:
+    FILE* fp_;
+    bool ascii_;
+};
+
 }  // namespace
 
-// ASCII or binary little-endian PLY: vertex x y z [nx ny nz], face vertex_indices (fanned) [material_index]
-// (reference: src/io/mesh_loader.cpp:64-157).
+// PLY, ASCII or binary little-endian: vertex x y z [nx ny nz]; face vertex_indices (or vertex_index), fanned from the
+// first corner, with an optional material_index; other elements are skipped (reference behaviour:
+// src/io/mesh_loader.cpp:64-157).
 bool loadPLY(const std::string& path, Mesh& m) {
     FILE* fp = fopen(path.c_str(), "rb");
     if (!fp) return false;
-    char line[1024]; bool ascii = false, ok = false; std::vector<PlyElem> elems;
-    if (!fgets(line, sizeof line, fp) || strncmp(line, "ply", 3) != 0) { fclose(fp); return false; }
-    while (fgets(line, sizeof line, fp)) {
-        if (strncmp(line, "format", 6) == 0) {
-            ascii = strstr(line, "ascii") != nullptr;
-            if (strstr(line, "big_endian")) { fprintf(stderr, "PLY big endian unsupported\n"); fclose(fp); return false; }
-        } else if (strncmp(line, "element", 7) == 0) {
-            PlyElem e; char name[256]; long c;
-            if (sscanf(line + 7, "%255s %ld", name, &c) == 2) { e.name = name; e.count = c; elems.push_back(e); }
-        } else if (strncmp(line, "property", 8) == 0 && !elems.empty()) {
-            PlyProp p; char a[64], b[64], c[64];
-            int got = sscanf(line + 8, "%63s %63s %63s", a, b, c);
-            if (got >= 2) {
-                if (strcmp(a, "list") == 0) { p.list = true; p.ltype = b; p.itype = got >= 3 ? c : ""; char d[64]; if (sscanf(line + 8, "%*s %*s %*s %63s", d) == 1) p.name = d; }
-                else { p.type = a; p.name = b; }
+    std::vector<PlyElement> elems;
+    bool ascii = false, header = false;
+    {
+        char buf[1024];
+        if (!fgets(buf, sizeof buf, fp) || strncmp(buf, "ply", 3) != 0) { fclose(fp); return false; }
+        while (fgets(buf, sizeof buf, fp)) {
+            std::vector<std::string> w;
+            for (char* tok = strtok(buf, " \t\r\n"); tok; tok = strtok(nullptr, " \t\r\n")) w.push_back(tok);
+            if (w.empty()) continue;
+            if (w[0] == "end_header") { header = true; break; }
+            if (w[0] == "format" && w.size() >= 2) {
+                if (w[1] == "binary_big_endian") { fprintf(stderr, "PLY big endian unsupported\n"); fclose(fp); return false; }
+                ascii = w[1] == "ascii";
+            } else if (w[0] == "element" && w.size() >= 3) {
+                PlyElement e; e.name = w[1]; e.n = atol(w[2].c_str());
+                elems.push_back(e);
+            } else if (w[0] == "property" && !elems.empty()) {
+                PlyProperty p;
+                if (w.size() >= 5 && w[1] == "list") {
+                    p.list = true;
+                    if (!plyType(w[2], p.count) || !plyType(w[3], p.value)) { fclose(fp); return false; }
+                    p.name = w[4];
+                } else if (w.size() >= 3) {
+                    if (!plyType(w[1], p.value)) { fclose(fp); return false; }
+                    p.name = w[2];
+                } else continue;
                 elems.back().props.push_back(p);
             }
-        } else if (strncmp(line, "end_header", 10) == 0) { ok = true; break; }
+        }
     }
-    if (!ok) { fclose(fp); return false; }
+    if (!header) { fclose(fp); return false; }
     if (m.kd.empty()) { m.kd.push_back(v3(0.8f, 0.8f, 0.8f)); m.matName.push_back(""); }
-    auto rd = [&](const std::string& t) { double x = 0; if (ascii) { if (fscanf(fp, "%lf", &x) != 1) x = 0; } else x = plyBin(fp, t); return x; };
-    for (auto& e : elems) {
+    PlyBody body(fp, ascii);
+    std::vector<double> vals, idx;
+    for (const PlyElement& e : elems) {
+        auto column = [&](const char* name) {
+            for (size_t k = 0; k < e.props.size(); k++) if (!e.props[k].list && e.props[k].name == name) return (int)k;
+            return -1;
+        };
         if (e.name == "vertex") {
-            int ix = -1, iy = -1, iz = -1, inx = -1, iny = -1, inz = -1;
-            for (size_t k = 0; k < e.props.size(); k++) {
-                const std::string& nm = e.props[k].name;
-                if (nm == "x") ix = (int)k; else if (nm == "y") iy = (int)k; else if (nm == "z") iz = (int)k;
-                else if (nm == "nx") inx = (int)k; else if (nm == "ny") iny = (int)k; else if (nm == "nz") inz = (int)k;
-            }
-            if (ix < 0 || iy < 0 || iz < 0) { fclose(fp); return false; }
-            bool hasNrm = inx >= 0 && iny >= 0 && inz >= 0;
-            std::vector<double> vals(e.props.size());
-            for (long i = 0; i < e.count; i++) {
+            const int cx = column("x"), cy = column("y"), cz = column("z"), nx = column("nx"), ny = column("ny"), nz = column("nz");
+            if (cx < 0 || cy < 0 || cz < 0) { fclose(fp); return false; }
+            const bool withN = nx >= 0 && ny >= 0 && nz >= 0;
+            vals.assign(e.props.size(), 0.0);
+            for (long r = 0; r < e.n; r++) {
                 for (size_t k = 0; k < e.props.size(); k++) {
-                    if (e.props[k].list) { int cnt = (int)rd(e.props[k].ltype); for (int j = 0; j < cnt; j++) rd(e.props[k].itype); vals[k] = 0; }
-                    else vals[k] = rd(e.props[k].type);
+                    if (e.props[k].list) { body.list(e.props[k], nullptr); vals[k] = 0.0; }
+                    else vals[k] = body.read(e.props[k].value);
                 }
-                m.v.push_back(v3((float)vals[ix], (float)vals[iy], (float)vals[iz]));
-                if (hasNrm) m.n.push_back(v3((float)vals[inx], (float)vals[iny], (float)vals[inz]));
+                m.v.push_back(v3((float)vals[cx], (float)vals[cy], (float)vals[cz]));
+                if (withN) m.n.push_back(v3((float)vals[nx], (float)vals[ny], (float)vals[nz]));
             }
         } else if (e.name == "face") {
-            std::vector<int> idx; int maxMat = 0;
-            for (long i = 0; i < e.count; i++) {
-                int mat = 0;
-                for (const PlyProp& p : e.props) {
-                    if (p.list) {
-                        int cnt = (int)rd(p.ltype);
-                        bool isIdx = p.name == "vertex_indices" || p.name == "vertex_index";
-                        if (isIdx) { idx.resize(cnt); for (int j = 0; j < cnt; j++) idx[j] = (int)rd(p.itype); }
-                        else for (int j = 0; j < cnt; j++) rd(p.itype);
-                    } else { double x = rd(p.type); if (p.name == "material_index") mat = (int)x; }
+            int topMaterial = 0;
+            for (long r = 0; r < e.n; r++) {
+                int material = 0;
+                for (const PlyProperty& p : e.props) {
+                    if (p.list) body.list(p, p.name == "vertex_indices" || p.name == "vertex_index" ? &idx : nullptr);
+                    else { const double x = body.read(p.value); if (p.name == "material_index") material = (int)x; }
                 }
-                maxMat = mat > maxMat ? mat : maxMat;
-                for (size_t k = 1; k + 1 < idx.size(); k++) { m.f.push_back({ idx[0], idx[k], idx[k + 1] }); m.fm.push_back(mat); }
+                topMaterial = std::max(topMaterial, material);
+                for (size_t j = 2; j < idx.size(); j++) { m.f.push_back({ (int)idx[0], (int)idx[j - 1], (int)idx[j] }); m.fm.push_back(material); }
             }
-            while ((int)m.kd.size() <= maxMat) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
+            while ((int)m.kd.size() <= topMaterial) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
         } else {
-            for (long i = 0; i < e.count; i++)
-                for (const PlyProp& p : e.props) {
-                    if (p.list) { int cnt = (int)rd(p.ltype); for (int j = 0; j < cnt; j++) rd(p.itype); } else rd(p.type);
-                }
+            for (long r = 0; r < e.n; r++)
+                for (const PlyProperty& p : e.props) { if (p.list) body.list(p, nullptr); else body.read(p.value); }
         }
     }
     fclose(fp);
