@@ -25,6 +25,7 @@ struct cwp_bvh {
 };
 
 static_assert(sizeof(cwp_params) == sizeof(FrameParams), "cwp_params must mirror FrameParams");
+static_assert(CWP_MORTON_ADAPTIVE == kMortonAdaptive && CWP_MORTON_30 == kMorton30, "Morton modes");
 static_assert(sizeof(cwp_instance) == sizeof(InstanceRecord), "cwp_instance must mirror InstanceRecord");
 static_assert(offsetof(cwp_params, sun_to) == offsetof(FrameParams, sunTo), "cwp_params layout");
 static_assert(offsetof(cwp_params, seed) == offsetof(FrameParams, seed), "cwp_params layout");
@@ -422,13 +423,17 @@ int cwp_bvh_from_hploc(const float* vertices, int nv, const int* tris, int nt, c
     return CWP_OK;
 }
 
-int cwp_hploc_build_host(const float* lo, const float* hi, int n, float* nodes, int* root, double* gpu_ms) {
+int cwp_hploc_build_host_mode(const float* lo, const float* hi, int n, int morton_bits, float* nodes, int* root, double* gpu_ms) {
     if (!lo || !hi || !nodes || n < 1) return fail(CWP_ERR_ARG, "cwp_hploc_build_host: bad arguments");
+    if (!validMortonMode(morton_bits)) return fail(CWP_ERR_ARG, "cwp_hploc_build_host: morton_bits must be 0 (adaptive) or 30");
     int r = 0; double ms = 0; std::string err;
-    if (!hplocBuildHost((const Vec4*)lo, (const Vec4*)hi, n, (Vec4*)nodes, r, ms, err)) return fail(CWP_ERR_DEVICE, err);
+    if (!hplocBuildHost((const Vec4*)lo, (const Vec4*)hi, n, (Vec4*)nodes, r, ms, err, morton_bits)) return fail(CWP_ERR_DEVICE, err);
     if (root) *root = r;
     if (gpu_ms) *gpu_ms = ms;
     return CWP_OK;
+}
+int cwp_hploc_build_host(const float* lo, const float* hi, int n, float* nodes, int* root, double* gpu_ms) {
+    return cwp_hploc_build_host_mode(lo, hi, n, CWP_MORTON_ADAPTIVE, nodes, root, gpu_ms);
 }
 
 int cwp_hploc_create(int capacity, cwp_hploc** out) {
@@ -444,12 +449,16 @@ int cwp_hploc_create(int capacity, cwp_hploc** out) {
 
 void cwp_hploc_destroy(cwp_hploc* b) { delete b; }
 
-int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root) {
+int cwp_hploc_build_mode(cwp_hploc* b, const void* lo, const void* hi, int n, int morton_bits, void* stream, int* root) {
     if (!b || !lo || !hi || n < 1) return fail(CWP_ERR_ARG, "cwp_hploc_build: bad arguments");
+    if (!validMortonMode(morton_bits)) return fail(CWP_ERR_ARG, "cwp_hploc_build: morton_bits must be 0 (adaptive) or 30");
     int r = 0; std::string err;
-    if (!b->B.build((const Vec4*)lo, (const Vec4*)hi, n, stream, r, err)) return fail(CWP_ERR_DEVICE, err);
+    if (!b->B.build((const Vec4*)lo, (const Vec4*)hi, n, stream, r, err, morton_bits)) return fail(CWP_ERR_DEVICE, err);
     if (root) *root = r;
     return CWP_OK;
+}
+int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root) {
+    return cwp_hploc_build_mode(b, lo, hi, n, CWP_MORTON_ADAPTIVE, stream, root);
 }
 
 int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes) {

@@ -25,6 +25,30 @@
 
 namespace cwp {
 
+// Device scratch of one build for up to `cap` primitives (the pointers of a HplocBuilder's allocations, or of a
+// fixed static array for the allocation-free entry point).
+struct HplocBuffers {
+    int cap = 0;
+    Vec4* nodes = nullptr;                        // 2n - 1 output nodes
+    uint64_t *keysA = nullptr, *keysB = nullptr;
+    int *clusters = nullptr, *bounds = nullptr, *counter = nullptr, *counts = nullptr;
+    Vec4* cboxes = nullptr;                       // box of the cluster in each slot of clusters (2 float4), height in w
+    Vec4* tnodes = nullptr;                       // internal nodes in build-time slots, renumbered into nodes
+    int *theight = nullptr, *newId = nullptr, *status = nullptr;
+    uint64_t *rkA = nullptr, *rkB = nullptr;
+    int tcap = 0;
+    int *rangeL = nullptr, *rangeR = nullptr;     // LBVH node ranges by split position
+    uint32_t* cbox = nullptr;
+    // bytes of each buffer for `cap` primitives, in the order of the fields above (nodes excluded when the caller
+    // provides them); used both to allocate and to carve a static array
+    static void sizes(int cap, bool withNodes, size_t bytes[18], int& tcap);
+};
+
+// Morton code modes: kMortonAdaptive = min(21, (64 - ceil(log2 n)) / 3) bits per axis (the reference's keys,
+// src/accel/hploc.cu:181-183); kMorton30 = the classic 30-bit code, 10 bits per axis (the index keeps keys unique).
+enum : int { kMortonAdaptive = 0, kMorton30 = 30 };
+inline bool validMortonMode(int m) { return m == kMortonAdaptive || m == kMorton30; }
+
 class HplocBuilder {
 public:
     HplocBuilder() = default;
@@ -35,28 +59,21 @@ public:
     bool reserve(int n, std::string& err);
     // lo / hi: n float4 per primitive box (device). stream: a hipStream_t (nullptr = default stream). Returns the root
     // node id through `root`; the nodes are ready when the stream reaches this point.
-    bool build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err);
-    const Vec4* nodes() const { return nodes_; }   // device, 2 float4 per node
-    const uint64_t* sortedKeys() const { return keysA_; }
+    bool build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err, int morton = kMortonAdaptive);
+    const Vec4* nodes() const { return b_.nodes; }   // device, 2 float4 per node
+    const uint64_t* sortedKeys() const { return b_.keysA; }
     // device status of the last build: 0 when the renumbered tree is consistent
-    const int* status() const { return status_; }
+    const int* status() const { return b_.status; }
     void release();
-    int capacity() const { return cap_; }
+    int capacity() const { return b_.cap; }
 
 private:
-    int cap_ = 0;
-    Vec4* nodes_ = nullptr;
-    uint64_t *keysA_ = nullptr, *keysB_ = nullptr;
-    int *clusters_ = nullptr, *bounds_ = nullptr, *counter_ = nullptr, *counts_ = nullptr;
-    Vec4* cboxes_ = nullptr;   // box of the cluster in each slot of clusters_ (2 float4), height in the first w
-    Vec4* tnodes_ = nullptr;   // internal nodes in build-time slots, renumbered into nodes_ after the climb
-    int *theight_ = nullptr, *newId_ = nullptr, *status_ = nullptr;
-    uint64_t *rkA_ = nullptr, *rkB_ = nullptr;
-    int tcap_ = 0;
-    int *rangeL_ = nullptr, *rangeR_ = nullptr;   // LBVH node ranges by split position
-    uint32_t* cbox_ = nullptr;
-    int tiles_ = 0;
+    HplocBuffers b_;
 };
+
+// One build on the given scratch (capacity >= n): every launch on `stream`, no allocation, no host synchronisation.
+bool hplocEnqueue(const HplocBuffers& B, const Vec4* lo, const Vec4* hi, int n, void* stream, int morton, int& root,
+                  std::string& err);
 
 // BVH2 layout (4 float4 per node, reference layout of src/accel/bvh_layouts.h:12-13) of H-PLOC nodes on the device:
 // internal node j <- H-PLOC node n + j, root n - 2; a leaf child becomes (primitive id, count 1). n == 1: node 0 holds
@@ -65,6 +82,7 @@ bool hplocToBVH2(const Vec4* hnodes, int n, Vec4* out, void* stream, std::string
 inline int hplocBVH2Root(int n) { return n > 1 ? n - 2 : 0; }
 
 // Host convenience: uploads the boxes, builds (once to warm up, once timed with events), downloads 2n - 1 nodes.
-bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err);
+bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err,
+                    int morton = kMortonAdaptive);
 
 }  // namespace cwp

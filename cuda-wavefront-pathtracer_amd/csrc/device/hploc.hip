@@ -622,106 +622,123 @@ __global__ void k_hp_to_bvh2(const Vec4* __restrict__ hn, int n, Vec4* out) {
 
 }  // namespace
 
+void HplocBuffers::sizes(int cap, bool withNodes, size_t bytes[18], int& tc) {
+    const size_t n = (size_t)cap;
+    // temporary node slots: the n - 1 internal nodes plus at most one partly used chunk per climbing wave
+    tc = (cap - 1) + kChunk * ((cap + 62) / 64 + 1);
+    const int tiles = (cap + kTile - 1) / kTile, rtiles = (tc + kTile - 1) / kTile;
+    const size_t T = (size_t)tc;
+    const size_t b[18] = { withNodes ? 2 * (2 * n - 1) * sizeof(Vec4) : 0,   // nodes
+                           n * 8, n * 8,                                      // keysA, keysB
+                           n * 4, n * 4, 4,                                   // clusters, bounds, counter
+                           (size_t)256 * (std::max(tiles, rtiles) + 1) * 4,   // counts: digit x tile, then 256 totals
+                           n * 2 * sizeof(Vec4),                              // cboxes
+                           T * 2 * sizeof(Vec4), T * 4, T * 4, 8,             // tnodes, theight, newId, status
+                           T * 8, T * 8,                                      // rkA, rkB
+                           n * 4, n * 4,                                      // rangeL, rangeR
+                           24, 0 };                                           // cbox
+    for (int k = 0; k < 18; k++) bytes[k] = b[k];
+}
+
 HplocBuilder::~HplocBuilder() { release(); }
 
 void HplocBuilder::release() {
-    if (!cap_) return;
-    hipFree(nodes_); hipFree(keysA_); hipFree(keysB_); hipFree(clusters_); hipFree(bounds_); hipFree(counter_);
-    hipFree(cbox_); hipFree(counts_); hipFree(cboxes_);
-    hipFree(tnodes_); hipFree(theight_); hipFree(rkA_); hipFree(rkB_); hipFree(newId_); hipFree(status_);
-    hipFree(rangeL_); hipFree(rangeR_);
-    rangeL_ = rangeR_ = nullptr;
-    cboxes_ = nullptr; tnodes_ = nullptr; theight_ = newId_ = status_ = nullptr; rkA_ = rkB_ = nullptr; tcap_ = 0;
-    nodes_ = nullptr; keysA_ = keysB_ = nullptr; clusters_ = bounds_ = counter_ = counts_ = nullptr; cbox_ = nullptr;
-    cap_ = 0; tiles_ = 0;
+    if (!b_.cap) return;
+    for (void* p : { (void*)b_.nodes, (void*)b_.keysA, (void*)b_.keysB, (void*)b_.clusters, (void*)b_.bounds, (void*)b_.counter,
+                     (void*)b_.counts, (void*)b_.cboxes, (void*)b_.tnodes, (void*)b_.theight, (void*)b_.newId, (void*)b_.status,
+                     (void*)b_.rkA, (void*)b_.rkB, (void*)b_.rangeL, (void*)b_.rangeR, (void*)b_.cbox })
+        if (p) hipFree(p);
+    b_ = HplocBuffers();
 }
 
 bool HplocBuilder::reserve(int n, std::string& err) {
-    if (n <= cap_) return true;
+    if (n <= b_.cap) return true;
     if (n < 1 || n > (1 << 30)) { err = "H-PLOC: primitive count out of range"; return false; }
     release();
-    tiles_ = (n + kTile - 1) / kTile;
-    HCHK(hipMalloc(&nodes_, (size_t)2 * (2 * (size_t)n - 1) * sizeof(Vec4)));
-    HCHK(hipMalloc(&keysA_, (size_t)n * 8));
-    HCHK(hipMalloc(&keysB_, (size_t)n * 8));
-    HCHK(hipMalloc(&clusters_, (size_t)n * 4));
-    HCHK(hipMalloc(&cboxes_, (size_t)n * 2 * sizeof(Vec4)));
-    HCHK(hipMalloc(&bounds_, (size_t)n * 4));
-    HCHK(hipMalloc(&counter_, 4));
-    HCHK(hipMalloc(&cbox_, 24));
-    // temporary node slots: the n - 1 internal nodes plus at most one partly used chunk per climbing wave
-    tcap_ = (n - 1) + kChunk * ((n + 62) / 64 + 1);
-    const int rtiles = (tcap_ + kTile - 1) / kTile;
-    HCHK(hipMalloc(&tnodes_, (size_t)tcap_ * 2 * sizeof(Vec4)));
-    HCHK(hipMalloc(&theight_, (size_t)tcap_ * 4));
-    HCHK(hipMalloc(&rkA_, (size_t)tcap_ * 8));
-    HCHK(hipMalloc(&rkB_, (size_t)tcap_ * 8));
-    HCHK(hipMalloc(&newId_, (size_t)tcap_ * 4));
-    HCHK(hipMalloc(&status_, 8));   // [0] error bits, [1] a height reached 255
-    HCHK(hipMalloc(&rangeL_, (size_t)n * 4));
-    HCHK(hipMalloc(&rangeR_, (size_t)n * 4));
-    // digit x tile counts, then the 256 digit totals (for the key sort and the renumbering sort)
-    HCHK(hipMalloc(&counts_, (size_t)256 * (std::max(tiles_, rtiles) + 1) * 4));
-    cap_ = n;
+    size_t bytes[18];
+    int tc = 0;
+    HplocBuffers::sizes(n, true, bytes, tc);
+    void* p[17] = {};
+    for (int k = 0; k < 17; k++) {
+        if (hipMalloc(&p[k], std::max<size_t>(bytes[k], 16)) != hipSuccess) {
+            for (int j = 0; j < k; j++) hipFree(p[j]);
+            err = "H-PLOC: out of device memory";
+            return false;
+        }
+    }
+    HplocBuffers B;
+    B.nodes = (Vec4*)p[0]; B.keysA = (uint64_t*)p[1]; B.keysB = (uint64_t*)p[2]; B.clusters = (int*)p[3]; B.bounds = (int*)p[4];
+    B.counter = (int*)p[5]; B.counts = (int*)p[6]; B.cboxes = (Vec4*)p[7]; B.tnodes = (Vec4*)p[8]; B.theight = (int*)p[9];
+    B.newId = (int*)p[10]; B.status = (int*)p[11]; B.rkA = (uint64_t*)p[12]; B.rkB = (uint64_t*)p[13]; B.rangeL = (int*)p[14];
+    B.rangeR = (int*)p[15]; B.cbox = (uint32_t*)p[16];
+    B.tcap = tc; B.cap = n;
+    b_ = B;
     return true;
 }
 
-bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err) {
+bool HplocBuilder::build(const Vec4* lo, const Vec4* hi, int n, void* stream, int& root, std::string& err, int morton) {
     if (n < 1) { err = "H-PLOC: no primitives"; return false; }
     if (!lo || !hi) { err = "H-PLOC: null box arrays"; return false; }
     if (!reserve(n, err)) return false;
+    return hplocEnqueue(b_, lo, hi, n, stream, morton, root, err);
+}
+
+bool hplocEnqueue(const HplocBuffers& B, const Vec4* lo, const Vec4* hi, int n, void* stream, int morton, int& root,
+                  std::string& err) {
+    if (n < 1 || n > B.cap) { err = "H-PLOC: primitive count outside the scratch capacity"; return false; }
+    if (!validMortonMode(morton)) { err = "H-PLOC: Morton mode must be 0 (adaptive) or 30"; return false; }
     hipStream_t st = (hipStream_t)stream;
     const int g = (n + 255) / 256;
-    k_hp_init<<<(tcap_ + 255) / 256, 256, 0, st>>>(cbox_, counter_, status_, theight_, tcap_);
-    k_hp_bounds<<<std::min(g, 512), 256, 0, st>>>(lo, hi, n, cbox_);
+    k_hp_init<<<(B.tcap + 255) / 256, 256, 0, st>>>(B.cbox, B.counter, B.status, B.theight, B.tcap);
+    k_hp_bounds<<<std::min(g, 512), 256, 0, st>>>(lo, hi, n, B.cbox);
     int ib = 1;
     while ((1ll << ib) < n) ib++;
-    const int mb = (64 - ib) / 3 < 21 ? (64 - ib) / 3 : 21;
-    k_hp_keys<<<g, 256, 0, st>>>(lo, hi, n, cbox_, keysA_, mb, ib);
+    const int mb = morton == kMorton30 ? 10 : ((64 - ib) / 3 < 21 ? (64 - ib) / 3 : 21);
+    k_hp_keys<<<g, 256, 0, st>>>(lo, hi, n, B.cbox, B.keysA, mb, ib);
     // The keys start in primitive order and their low ib bits are the primitive index, so a stable sort on the Morton
     // bits alone leaves equal codes in index order: the order of the full keys, with ib bits fewer to sort.
     const int passes = (3 * mb + 7) / 8;
-    const uint64_t* sorted = keysA_;
+    const uint64_t* sorted = B.keysA;
     if (n <= kRankSort) {
-        k_rank_sort<<<(n + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(keysA_, keysB_, n);
-        sorted = keysB_;
+        k_rank_sort<<<(n + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(B.keysA, B.keysB, n);
+        sorted = B.keysB;
     } else if (n <= kSmallSort) {
-        k_rs_single<<<1, kSortBlock, 0, st>>>(keysA_, keysB_, n, ib, passes);
-        sorted = (passes & 1) ? keysB_ : keysA_;
+        k_rs_single<<<1, kSortBlock, 0, st>>>(B.keysA, B.keysB, n, ib, passes);
+        sorted = (passes & 1) ? B.keysB : B.keysA;
     } else {
         const int tiles = (n + kTile - 1) / kTile;
-        uint64_t *src = keysA_, *dst = keysB_;
+        uint64_t *src = B.keysA, *dst = B.keysB;
         for (int p = 0; p < passes; p++) {
-            k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, ib + 8 * p, counts_, tiles);
-            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, tiles, counts_ + 256 * (size_t)tiles);
-            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, ib + 8 * p, counts_, tiles, counts_ + 256 * (size_t)tiles);
+            k_rs_count<<<tiles, kSortBlock, 0, st>>>(src, n, ib + 8 * p, B.counts, tiles);
+            k_rs_rowscan<<<256, kSortBlock, 0, st>>>(B.counts, tiles, B.counts + 256 * (size_t)tiles);
+            k_rs_scatter<<<tiles, kSortBlock, 0, st>>>(src, dst, n, ib + 8 * p, B.counts, tiles, B.counts + 256 * (size_t)tiles);
             std::swap(src, dst);
         }
         sorted = src;
     }
-    k_hp_leaves_ranges<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, nodes_, clusters_, cboxes_, bounds_, rangeL_, rangeR_);
+    k_hp_leaves_ranges<<<g, 256, 0, st>>>(lo, hi, sorted, n, ib, B.nodes, B.clusters, B.cboxes, B.bounds, B.rangeL, B.rangeR);
     if (n > 1) {
-        k_hploc<<<(n - 1 + 63) / 64, 64, 0, st>>>(tnodes_, theight_, tcap_, sorted, rangeL_, rangeR_, clusters_, cboxes_, bounds_,
-                                                  counter_, n);
+        k_hploc<<<(n - 1 + 63) / 64, 64, 0, st>>>(B.tnodes, B.theight, B.tcap, sorted, B.rangeL, B.rangeR, B.clusters, B.cboxes,
+                                                  B.bounds, B.counter, n);
         // internal nodes in height order: 8-bit passes over the heights of every temporary slot (holes last); the pass
-        // over the high byte runs only when a height reaches 255 (k_rn_keys sets status_[1])
+        // over the high byte runs only when a height reaches 255 (k_rn_keys sets status[1])
         // (few slots: the keys are unique, so one ranking launch orders them)
-        const int T = tcap_, rt = (T + kTile - 1) / kTile;
-        k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(theight_, T, rkA_, status_ + 1);
+        const int T = B.tcap, rt = (T + kTile - 1) / kTile;
+        k_rn_keys<<<(T + 255) / 256, 256, 0, st>>>(B.theight, T, B.rkA, B.status + 1);
         if (T <= kRankSort) {
-            k_rank_sort<<<(T + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(rkA_, rkB_, T);
-            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkB_, status_ + 1, T, n, newId_, status_);
+            k_rank_sort<<<(T + kSortBlock - 1) / kSortBlock, kSortBlock, 0, st>>>(B.rkA, B.rkB, T);
+            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(B.rkB, B.rkB, B.status + 1, T, n, B.newId, B.status);
         } else {
             for (int p = 0; p < 2; p++) {
-                uint64_t *src = p ? rkB_ : rkA_, *dst = p ? rkA_ : rkB_;
-                const int* gate = p ? status_ + 1 : nullptr;
-                k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, counts_, rt, gate);
-                k_rs_rowscan<<<256, kSortBlock, 0, st>>>(counts_, rt, counts_ + 256 * (size_t)rt, gate);
-                k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, counts_, rt, counts_ + 256 * (size_t)rt, gate);
+                uint64_t *src = p ? B.rkB : B.rkA, *dst = p ? B.rkA : B.rkB;
+                const int* gate = p ? B.status + 1 : nullptr;
+                k_rs_count<<<rt, kSortBlock, 0, st>>>(src, T, 32 + 8 * p, B.counts, rt, gate);
+                k_rs_rowscan<<<256, kSortBlock, 0, st>>>(B.counts, rt, B.counts + 256 * (size_t)rt, gate);
+                k_rs_scatter<<<rt, kSortBlock, 0, st>>>(src, dst, T, 32 + 8 * p, B.counts, rt, B.counts + 256 * (size_t)rt, gate);
             }
-            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(rkB_, rkA_, status_ + 1, T, n, newId_, status_);
+            k_rn_map<<<(T + 255) / 256, 256, 0, st>>>(B.rkB, B.rkA, B.status + 1, T, n, B.newId, B.status);
         }
-        k_rn_write<<<(T + 255) / 256, 256, 0, st>>>(tnodes_, theight_, newId_, T, n, nodes_, status_);
+        k_rn_write<<<(T + 255) / 256, 256, 0, st>>>(B.tnodes, B.theight, B.newId, T, n, B.nodes, B.status);
     }
     HCHK(hipGetLastError());
     root = n > 1 ? 2 * n - 2 : 0;
@@ -736,7 +753,7 @@ bool hplocToBVH2(const Vec4* hnodes, int n, Vec4* out, void* stream, std::string
     return true;
 }
 
-bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err) {
+bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& root, double& gpuMs, std::string& err, int morton) {
     if (n < 1) { err = "H-PLOC: no primitives"; return false; }
     Vec4 *dLo = nullptr, *dHi = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -747,9 +764,9 @@ bool hplocBuildHost(const Vec4* lo, const Vec4* hi, int n, Vec4* nodesOut, int& 
         if (hipMemcpy(dLo, lo, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(dHi, hi, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess) { err = "H-PLOC: upload failed"; break; }
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { err = "H-PLOC: event creation failed"; break; }
-        if (!B.build(dLo, dHi, n, nullptr, root, err)) break;   // first build: loads the kernels (not timed)
+        if (!B.build(dLo, dHi, n, nullptr, root, err, morton)) break;   // first build: loads the kernels (not timed)
         hipEventRecord(e0, nullptr);
-        if (!B.build(dLo, dHi, n, nullptr, root, err)) break;
+        if (!B.build(dLo, dHi, n, nullptr, root, err, morton)) break;
         hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) { err = "H-PLOC: build failed"; break; }
         float ms = 0;

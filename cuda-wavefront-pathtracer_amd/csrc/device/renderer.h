@@ -33,6 +33,7 @@ struct RenderOptions {
     int shadowRefill = 0;   // shadow kernel: 1 = refill idle lanes mid-traversal, 0 = one ray per lane per round
     int graph = 0;          // 1: a frame is captured once into a HIP graph and replayed (per-kernel timing off)
     int sort = 0;           // 1: between bounces, group each partition's continuations by direction bucket
+    int mortonBits = 0;     // two-level scenes: Morton keys of the per-frame TLAS build (0 adaptive, 30: 10 bits per axis)
 };
 
 struct FrameStats {

@@ -200,6 +200,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     if (O.stackLds != 8 && O.stackLds != 16 && O.stackLds != 32) { err = "stack_lds must be 8, 16 or 32"; return false; }
     if (O.bounceWaves != 1 && O.bounceWaves != 6 && O.bounceWaves != 8) { err = "bounce_waves must be 1, 6 or 8"; return false; }
     if (O.bounces < 0 || O.bounces > 64) { err = "bounces must be in [0, 64]"; return false; }
+    if (!validMortonMode(O.mortonBits)) { err = "morton_bits must be 0 (adaptive) or 30"; return false; }
     // a frame launches bounces + 2 depths: a deeper path limit would end paths early without an error
     if (F.maxB > O.bounces) { err = "max_bounce (" + std::to_string(F.maxB) + ") exceeds bounces (" + std::to_string(O.bounces) + ")"; return false; }
     I.O = O; I.F = F;
@@ -517,7 +518,7 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
     HIPCHK(hipEventRecord(I.tl0, I.st));
     k_inst_boxes<<<(n + 255) / 256, 256, 0, I.st>>>(I.dInst, I.dMeshBox, n, I.dLo, I.dHi);
     int hroot = 0;
-    if (!I.tlasB.build(I.dLo, I.dHi, n, I.st, hroot, err)) return false;
+    if (!I.tlasB.build(I.dLo, I.dHi, n, I.st, hroot, err, I.O.mortonBits)) return false;
     if (!hplocToBVH2(I.tlasB.nodes(), n, I.dTlas, I.st, err)) return false;
     HIPCHK(hipEventRecord(I.tl1, I.st));
     if (!I.fill(I.dDepth, 4, err)) return false;

@@ -98,6 +98,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--graph") ok = inum(A.graph);
         else if (k == "--bvh_cache") ok = str(A.bvhCache);
         else if (k == "--gpu_collapse") ok = inum(A.gpuCollapse);
+        else if (k == "--morton_bits") ok = inum(A.mortonBits);
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--sort") ok = inum(A.sort);
         else if (k == "--devloop") {
@@ -113,6 +114,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.envSampling != "importance" && A.envSampling != "uniform") { err = "--env_sampling must be importance or uniform"; return false; }
     if (A.rrDepth < 0) { err = "--rr_depth must be >= 0"; return false; }
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
+    if (A.mortonBits != 0 && A.mortonBits != 30) { err = "--morton_bits must be 0 (adaptive) or 30"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
     if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
     if (A.persistent < 0 || A.persistent > 2) {
@@ -159,7 +161,7 @@ static bool buildTree(const Mesh& M, const Args& A, int leafMax, BinaryBVH& tree
         lo[t] = v4(b.lo, 0.f); hi[t] = v4(b.hi, 0.f);
     }
     int root = 0; double gpuMs = 0;
-    if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err)) return false;
+    if (!hplocBuildHost(lo.data(), hi.data(), n, hn.data(), root, gpuMs, err, A.mortonBits)) return false;
     const auto t1 = std::chrono::steady_clock::now();
     treeFromHploc(hn, n, root, leafMax, tree);
     if (wide) {   // the compressed BVH8 collapse on the device (bit-identical to collapseCWBVH of the tree above)
@@ -277,7 +279,9 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
     uint64_t cacheKey = 0;
     std::string cachePath;
     if (!A.bvhCache.empty()) {
-        cacheKey = bvhCacheKey(parts, V, A.builder, leafMax, A.bins, Z.P.hasN);
+        // the Morton mode changes H-PLOC trees: it is part of the builder's identity
+        const std::string builderId = A.builder == "hploc" && A.mortonBits ? A.builder + std::to_string(A.mortonBits) : A.builder;
+        cacheKey = bvhCacheKey(parts, V, builderId, leafMax, A.bins, Z.P.hasN);
         cachePath = bvhCachePath(A.bvhCache, cacheKey);
         BvhCacheEntry E;
         Z.bvhCacheHit = loadBvhCache(cachePath, cacheKey, S, E) && S.variant == V && S.inst == (Z.inst ? 1 : 0) ? 1 : 0;
@@ -541,7 +545,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.variant = Z.variant; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
-    O.shadowRefill = A.shadowRefill; O.sort = A.sort;
+    O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.mortonBits = A.mortonBits;
     O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;
 }

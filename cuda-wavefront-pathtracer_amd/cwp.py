@@ -202,6 +202,8 @@ SIGNATURES = {
                             _I, _I, ctypes.POINTER(_P)], _I),
     "cwp_hploc_build_host": ([ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _I,
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_hploc_build_host_mode": ([_FP, _FP, _I, _I, _FP, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
+    "cwp_hploc_build_mode": ([_P, _P, _P, _I, _I, _P, ctypes.POINTER(_I)], _I),
     "cwp_hploc_create": ([_I, ctypes.POINTER(_P)], _I),
     "cwp_hploc_destroy": ([_P], None),
     "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),
@@ -612,13 +614,18 @@ def _boxes4(a):
     return np.ascontiguousarray(a.reshape(-1, 4))
 
 
-def hploc_build(lo, hi):
-    """GPU H-PLOC build over host boxes ((n, 3) or (n, 4)). Returns (nodes (2n-1, 8) float32, root, gpu_ms)."""
+MORTON_ADAPTIVE, MORTON_30 = 0, 30
+
+
+def hploc_build(lo, hi, morton_bits=MORTON_ADAPTIVE):
+    """GPU H-PLOC build over host boxes ((n, 3) or (n, 4)). Returns (nodes (2n-1, 8) float32, root, gpu_ms).
+    morton_bits: MORTON_ADAPTIVE (the reference's keys) or MORTON_30 (10 bits per axis)."""
     lo, hi = _boxes4(lo), _boxes4(hi)
     n = len(lo)
     nodes = np.zeros((2 * n - 1, 8), np.float32)
     root, ms = ctypes.c_int(), ctypes.c_double()
-    _check(lib().cwp_hploc_build_host(_fptr(lo), _fptr(hi), n, _fptr(nodes), ctypes.byref(root), ctypes.byref(ms)))
+    _check(lib().cwp_hploc_build_host_mode(_fptr(lo), _fptr(hi), n, morton_bits, _fptr(nodes), ctypes.byref(root),
+                                           ctypes.byref(ms)))
     return nodes, root.value, ms.value
 
 

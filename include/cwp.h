@@ -253,15 +253,23 @@ int cwp_bvh_from_hploc(const float* vertices, int num_vertices, const int* trian
  * Morton order (left = primitive id, right = -1), internal node ids come from a counter (children before parents),
  * the root is 2n - 2 (0 when n == 1). Internal node numbering depends on scheduling; the tree does not. */
 typedef struct cwp_hploc cwp_hploc;
+/* Morton code of the keys: CWP_MORTON_ADAPTIVE = min(21, (64 - ceil(log2 n)) / 3) bits per axis, the reference's keys
+ * (src/accel/hploc.cu:181-183); CWP_MORTON_30 = 10 bits per axis (30-bit codes). The primitive index below the code
+ * keeps every key unique either way. */
+enum { CWP_MORTON_ADAPTIVE = 0, CWP_MORTON_30 = 30 };
 /* Host arrays: lo / hi 4 floats per primitive box; builds twice (the second timed with events, *gpu_ms) and copies
- * the nodes out. Replaces hplocBuildHost. */
+ * the nodes out. Replaces hplocBuildHost (adaptive keys); the _mode variant takes the Morton mode. */
 int cwp_hploc_build_host(const float* lo, const float* hi, int n, float* nodes, int* root, double* gpu_ms);
+int cwp_hploc_build_host_mode(const float* lo, const float* hi, int n, int morton_bits, float* nodes, int* root,
+                              double* gpu_ms);
 /* Stream-ordered builder over device arrays; scratch sized for `capacity` primitives (grown on demand). */
 int cwp_hploc_create(int capacity, cwp_hploc** out);
 void cwp_hploc_destroy(cwp_hploc* b);
 /* lo / hi: device pointers, 4 floats per primitive; stream: a hipStream_t or NULL (default stream). Asynchronous: the
  * nodes (cwp_hploc_nodes, a device pointer owned by the builder) are ready when the stream gets there. */
 int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root);
+/* as cwp_hploc_build, with the Morton mode (CWP_MORTON_ADAPTIVE or CWP_MORTON_30) */
+int cwp_hploc_build_mode(cwp_hploc* b, const void* lo, const void* hi, int n, int morton_bits, void* stream, int* root);
 int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes);
 /* 64 B BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) of device H-PLOC nodes into `out`
  * (device). Replaces hplocToBVH2 (src/accel/hploc.h:36). */

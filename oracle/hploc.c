@@ -51,12 +51,15 @@ uint64_t orc_expand_bits21(uint32_t v) {
     return r;
 }
 
-void orc_hp_key_bits(int n, int* mb, int* ib) {
+void orc_hp_key_bits_mode(int n, int morton, int* mb, int* ib) {
     int b = 1;
     while ((1ll << b) < n) b++;
     *ib = b;
-    *mb = (64 - b) / 3 < 21 ? (64 - b) / 3 : 21;
+    /* adaptive (the reference, src/accel/hploc.cu:181-183): as many bits per axis as fit beside the index, at most 21;
+       morton == 30: the classic 30-bit code, 10 bits per axis */
+    *mb = morton == 30 ? 10 : ((64 - b) / 3 < 21 ? (64 - b) / 3 : 21);
 }
+void orc_hp_key_bits(int n, int* mb, int* ib) { orc_hp_key_bits_mode(n, 0, mb, ib); }
 
 void orc_hp_keys(const float* lo, const float* hi, int n, const float box[6], int mb, int ib, uint64_t* keys) {
     const float s = (float)(1u << mb), smax = s - 1.f;
@@ -157,11 +160,14 @@ static void hp_merge(int L, int R, int s, int root, float* nodes, int* C, int* c
 }
 
 int orc_hp_build(const float* plo, const float* phi, int n, float* nodes, uint64_t* sorted_keys) {
+    return orc_hp_build_mode(plo, phi, n, 0, nodes, sorted_keys);
+}
+int orc_hp_build_mode(const float* plo, const float* phi, int n, int morton, float* nodes, uint64_t* sorted_keys) {
     if (n < 1) return -1;
     float box[6];
     int mb, ib;
     orc_hp_centroid_box(plo, phi, n, box);
-    orc_hp_key_bits(n, &mb, &ib);
+    orc_hp_key_bits_mode(n, morton, &mb, &ib);
     uint64_t* k = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
     orc_hp_keys(plo, phi, n, box, mb, ib, k);
     qsort(k, (size_t)n, sizeof(uint64_t), cmp_u64);

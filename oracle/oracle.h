@@ -112,9 +112,12 @@ void orc_pack_corners(const float* v, const int* tri, const int* mat, const int*
 void orc_hp_centroid_box(const float* lo, const float* hi, int n, float box[6]);
 uint64_t orc_expand_bits21(uint32_t v);
 void orc_hp_key_bits(int n, int* mb, int* ib);
+/* morton: 0 = adaptive bits per axis (the reference), 30 = 10 bits per axis */
+void orc_hp_key_bits_mode(int n, int morton, int* mb, int* ib);
 void orc_hp_keys(const float* lo, const float* hi, int n, const float box[6], int mb, int ib, uint64_t* keys);
 /* full build (keys, sort, leaves, H-PLOC); sorted_keys (n) may be NULL; returns the root id (2n - 2, or 0 for n = 1) */
 int orc_hp_build(const float* lo, const float* hi, int n, float* nodes, uint64_t* sorted_keys);
+int orc_hp_build_mode(const float* lo, const float* hi, int n, int morton, float* nodes, uint64_t* sorted_keys);
 /* BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) */
 void orc_hp_to_bvh2(const float* hn, int n, float* out);
 /* binary tree with SAH-collapsed leaves (<= leaf_max primitives), pre-order; prim = primitive ids */

@@ -77,6 +77,8 @@ def lib():
         L.orc_hp_key_bits.argtypes = [ctypes.c_int, _IP, _IP]
         L.orc_hp_keys.argtypes = [_FP, _FP, ctypes.c_int, _FP, ctypes.c_int, ctypes.c_int, _U64P]
         L.orc_hp_build.argtypes = [_FP, _FP, ctypes.c_int, _FP, _U64P]
+        L.orc_hp_build_mode.argtypes = [_FP, _FP, ctypes.c_int, ctypes.c_int, _FP, _U64P]
+        L.orc_hp_key_bits_mode.argtypes = [ctypes.c_int, ctypes.c_int, _IP, _IP]
         L.orc_hp_to_bvh2.argtypes = [_FP, ctypes.c_int, _FP]
         L.orc_tree_from_hploc.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcTree)]
         _F = ctypes.c_float
@@ -159,19 +161,20 @@ def _box4(a):
     return np.ascontiguousarray(a.reshape(-1, 4))
 
 
-def hp_key_bits(n):
+def hp_key_bits(n, morton_bits=0):
+    """(bits per axis, index bits) of the keys; morton_bits 0 = adaptive (the reference), 30 = 10 bits per axis"""
     mb, ib = ctypes.c_int(), ctypes.c_int()
-    lib().orc_hp_key_bits(n, ctypes.byref(mb), ctypes.byref(ib))
+    lib().orc_hp_key_bits_mode(n, morton_bits, ctypes.byref(mb), ctypes.byref(ib))
     return mb.value, ib.value
 
 
-def hp_keys(lo, hi):
+def hp_keys(lo, hi, morton_bits=0):
     """Centroid box and unsorted 64-bit keys of primitive boxes (lo/hi: (n, 3) or (n, 4))."""
     lo, hi = _box4(lo), _box4(hi)
     n = len(lo)
     box = np.zeros(6, np.float32)
     lib().orc_hp_centroid_box(_f(lo), _f(hi), n, _f(box))
-    mb, ib = hp_key_bits(n)
+    mb, ib = hp_key_bits(n, morton_bits)
     keys = np.zeros(n, np.uint64)
     lib().orc_hp_keys(_f(lo), _f(hi), n, _f(box), mb, ib, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return box, keys, mb, ib
@@ -181,13 +184,14 @@ def expand_bits21(v):
     return lib().orc_expand_bits21(v)
 
 
-def hp_build(lo, hi):
+def hp_build(lo, hi, morton_bits=0):
     """H-PLOC build: returns (nodes (2n-1, 8) float32, root, sorted keys)."""
     lo, hi = _box4(lo), _box4(hi)
     n = len(lo)
     nodes = np.zeros((max(2 * n - 1, 1), 8), np.float32)
     keys = np.zeros(n, np.uint64)
-    root = lib().orc_hp_build(_f(lo), _f(hi), n, _f(nodes), keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    root = lib().orc_hp_build_mode(_f(lo), _f(hi), n, morton_bits, _f(nodes),
+                                   keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return nodes, root, keys
 
 

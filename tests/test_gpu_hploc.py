@@ -43,6 +43,32 @@ def test_hploc_matches_oracle(n):
     assert ms > 0
 
 
+@pytest.mark.parametrize("n", [1, 2, 100, 4097, 16385, 70000])
+def test_hploc_30bit_matches_oracle(n):
+    print("case hploc30 n=%d" % n, flush=True)
+    lo, hi = boxes(n, n + 7)
+    g, groot, _ = cwp.hploc_build(lo, hi, cwp.MORTON_30)
+    o, oroot, _ = po.hp_build(lo, hi, 30)
+    check_tree(g, groot, lo, hi)
+    assert po.hp_canonical(g, n, groot) == po.hp_canonical(o, n, oroot)
+    np.testing.assert_array_equal(g[:n].view(np.uint32), o[:n].view(np.uint32))
+
+
+@pytest.mark.parametrize("morton", [0, 30])
+def test_hploc_200k_triangle_mesh_matches_oracle(morton):
+    """The largest BLAS of the two-level configuration (about 200k triangles), both Morton modes."""
+    print("case hploc mesh 200k morton=%d" % morton, flush=True)
+    s = cwp.Setup(["--gen", "terrain", "--gen_grid", "317", "--gen_boxes", "0", "--width", "8", "--height", "8"])
+    lo, hi = po.tri_boxes(s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES))
+    n = len(lo)
+    assert n == 2 * 316 * 316
+    g, groot, _ = cwp.hploc_build(lo, hi, morton)
+    o, oroot, _ = po.hp_build(lo, hi, morton)
+    check_tree(g, groot, lo, hi)
+    assert po.hp_canonical(g, n, groot) == po.hp_canonical(o, n, oroot)
+    np.testing.assert_array_equal(g[:n].view(np.uint32), o[:n].view(np.uint32))
+
+
 def test_hploc_on_mesh_triangles_and_duplicate_centroids():
     s = cwp.Setup(["--gen", "terrain", "--gen_grid", "121", "--gen_boxes", "50", "--width", "8", "--height", "8"])
     lo, hi = po.tri_boxes(s.array(cwp.ARR_VERTICES), s.array(cwp.ARR_TRIANGLES))

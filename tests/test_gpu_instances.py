@@ -78,6 +78,29 @@ def test_tlas_rebuilt_per_frame_matches_oracle():
         r.set_instances(bad)
 
 
+@pytest.mark.parametrize("morton", [0, 30])
+def test_full_size_two_level_scene_matches_oracle(morton):
+    """The two-level configuration at full size: 5 meshes of 1.3k-200k triangles and a ground, 1,001 instances with
+    their transforms redrawn per frame, the TLAS rebuilt on the GPU with adaptive or 30-bit Morton keys; closest-hit
+    triangle and instance ids of 16,000 rays against the oracle's two-level traversal at 3 frames."""
+    print("case full-size two-level morton=%d" % morton, flush=True)
+    s = cwp.Setup(["--gen", "instances", "--width", "96", "--height", "64", "--morton_bits", str(morton)])
+    assert s.info["num_instances"] >= 1000
+    r = cwp.Renderer(s)
+    rays = scene_rays(s, 8000, 11)
+    for f in (0, 17, 59):
+        assert r.set_frame(f) > 0
+        g, gi = r.trace_inst(rays)
+        o, oi = oracle_scene(s, f).trace_inst(rays)
+        gt, ot = g[:, 3].view(np.int32), o[:, 3].view(np.int32)
+        assert (ot >= 0).mean() > 0.3
+        agree = ((gt == ot) & (gi == oi)).mean()
+        assert agree >= 0.9999, (f, agree)
+        both = (gt == ot) & (ot >= 0)
+        assert (np.abs(g[both, 0] - o[both, 0]) / np.abs(o[both, 0])).max() <= 1e-5
+    r.close()
+
+
 def test_two_level_render_matches_oracle_and_flattened():
     spp, batch, bounces = 8, 8, 4
     extra = ["--batch", str(batch), "--bounces", str(bounces), "--spp", str(spp)]

@@ -49,7 +49,9 @@ def test_version_and_error_string():
         cwp.Setup(["--gen", "boxes", "--sort", "2"])
     with pytest.raises(cwp.CwpError, match="experimental"):
         cwp.Setup(["--gen", "boxes", "--devloop", "1"])
-    cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0"])
+    with pytest.raises(cwp.CwpError, match="--morton_bits"):
+        cwp.Setup(["--gen", "boxes", "--morton_bits", "12"])
+    cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0", "--morton_bits", "30"])
     assert cwp.Options.default().sort == 0
 
 

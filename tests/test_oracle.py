@@ -393,6 +393,31 @@ def test_hploc_keys_against_direct_interleave():
         assert len(np.unique(keys)) == n
 
 
+def test_hploc_30bit_keys_closed_form():
+    """30-bit mode: 10 bits per axis whatever n is; the corners of the centroid box map to code 0 and 2^30 - 1, a
+    centroid at 0.5 of the box on every axis to the code of (512, 512, 512), and the key order is (code, index)."""
+    rng = np.random.default_rng(9)
+    for n in (3, 1000, 70000):
+        c = rng.random((n, 3)).astype(np.float32)
+        c[0] = 0.0; c[1] = 1.0; c[2] = 0.5   # box corners and its centre (the other centroids lie inside [0, 1))
+        lo, hi = c, c
+        box, keys, mb, ib = po.hp_keys(lo, hi, 30)
+        assert mb == 10 and ib == max(1, int(np.ceil(np.log2(n))))
+        code = keys >> np.uint64(ib)
+        assert int(code[0]) == 0 and int(code[1]) == (1 << 30) - 1
+        assert int(code[2]) == 7 << 27   # 512 = bit 9 on each axis: bits 29, 28, 27
+        assert int(code.max()) < 1 << 30
+        np.testing.assert_array_equal(keys & np.uint64((1 << ib) - 1), np.arange(n, dtype=np.uint64))
+        # the adaptive mode refines the same order: its codes shifted down to 10 bits per axis are the 30-bit codes
+        _, ka, mba, iba = po.hp_keys(lo, hi, 0)
+        assert mba > 10
+        np.testing.assert_array_equal((ka >> np.uint64(iba)) >> np.uint64(3 * (mba - 10)), code)
+        for i in rng.integers(0, n, 50):
+            q = [int(min(max(np.float32(c[i, a] - box[a]) / np.float32(box[3 + a] - box[a]) * np.float32(1024), 0), 1023))
+                 for a in range(3)]
+            assert int(code[i]) == _interleave(q[0], q[1], q[2], 10)
+
+
 def _hp_invariants(lo, hi, nodes, root, n):
     """2n - 1 nodes, every primitive once, parent box = union of children, children before parents."""
     ints = nodes.view(np.int32)
