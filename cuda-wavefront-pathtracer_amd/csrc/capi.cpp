@@ -26,6 +26,7 @@ struct cwp_bvh {
 
 static_assert(sizeof(cwp_params) == sizeof(FrameParams), "cwp_params must mirror FrameParams");
 static_assert(CWP_MORTON_ADAPTIVE == kMortonAdaptive && CWP_MORTON_30 == kMorton30, "Morton modes");
+static_assert(CWP_HPLOC_STATIC_MAX == kHplocStaticMax, "allocation-free build limit");
 static_assert(sizeof(cwp_instance) == sizeof(InstanceRecord), "cwp_instance must mirror InstanceRecord");
 static_assert(offsetof(cwp_params, sun_to) == offsetof(FrameParams, sunTo), "cwp_params layout");
 static_assert(offsetof(cwp_params, seed) == offsetof(FrameParams, seed), "cwp_params layout");
@@ -459,6 +460,18 @@ int cwp_hploc_build_mode(cwp_hploc* b, const void* lo, const void* hi, int n, in
 }
 int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root) {
     return cwp_hploc_build_mode(b, lo, hi, n, CWP_MORTON_ADAPTIVE, stream, root);
+}
+
+int cwp_hploc_build_static(const void* lo, const void* hi, int n, int morton_bits, void* nodes, void* stream, int* root) {
+    if (!lo || !hi || !nodes) return fail(CWP_ERR_ARG, "cwp_hploc_build_static: null argument");
+    if (n < 1 || n > CWP_HPLOC_STATIC_MAX)
+        return fail(CWP_ERR_ARG, "cwp_hploc_build_static: " + std::to_string(n) + " primitives; the fixed scratch holds 1 to " +
+                                     std::to_string(CWP_HPLOC_STATIC_MAX));
+    if (!validMortonMode(morton_bits)) return fail(CWP_ERR_ARG, "cwp_hploc_build_static: morton_bits must be 0 (adaptive) or 30");
+    int r = 0; std::string err;
+    if (!hplocBuildStatic((const Vec4*)lo, (const Vec4*)hi, n, morton_bits, (Vec4*)nodes, stream, r, err)) return fail(CWP_ERR_DEVICE, err);
+    if (root) *root = r;
+    return CWP_OK;
 }
 
 int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes) {

@@ -75,6 +75,13 @@ private:
 bool hplocEnqueue(const HplocBuffers& B, const Vec4* lo, const Vec4* hi, int n, void* stream, int morton, int& root,
                   std::string& err);
 
+// The same build with no allocation: the scratch is a fixed static device array (16 MiB per GPU, held until the
+// process ends) for at most kHplocStaticMax primitives; `nodes` is the caller's device buffer of 2n - 1 nodes. All
+// calls on one GPU share the array: calls that overlap on a GPU (other streams or threads) race.
+constexpr int kHplocStaticMax = 1 << 16;
+bool hplocBuildStatic(const Vec4* lo, const Vec4* hi, int n, int morton, Vec4* nodes, void* stream, int& root,
+                      std::string& err);
+
 // BVH2 layout (4 float4 per node, reference layout of src/accel/bvh_layouts.h:12-13) of H-PLOC nodes on the device:
 // internal node j <- H-PLOC node n + j, root n - 2; a leaf child becomes (primitive id, count 1). n == 1: node 0 holds
 // the leaf and an empty slot (root 0). Reference: src/accel/hploc.cu:138-160.

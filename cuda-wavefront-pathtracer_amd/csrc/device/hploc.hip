@@ -2,6 +2,7 @@
 // 2024) and the reference's description of its variant (src/accel/hploc.h:1-11): Morton-keyed LBVH climbed bottom-up
 // (Apetrei 2014), PLOC merges of 2 x 16 clusters within a radius of 8 positions.
 #include "device/hploc.h"
+#include <mutex>
 #include <hip/hip_runtime.h>
 #include <algorithm>
 
@@ -743,6 +744,61 @@ bool hplocEnqueue(const HplocBuffers& B, const Vec4* lo, const Vec4* hi, int n, 
     HCHK(hipGetLastError());
     root = n > 1 ? 2 * n - 2 : 0;
     return true;
+}
+
+// ------------------------------------------------------------------------------------------------ allocation-free build
+// The scratch of hplocBuildStatic: one fixed device array per GPU (the module's copy on each device), carved into the
+// build's buffers on every call. Host code never names the array itself: it uses the per-device address that
+// hipGetSymbolAddress returns for it.
+namespace {
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;   // holds the buffers of kHplocStaticMax primitives
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+unsigned char* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+bool hplocBuildStatic(const Vec4* lo, const Vec4* hi, int n, int morton, Vec4* nodes, void* stream, int& root,
+                      std::string& err) {
+    if (n < 1 || n > kHplocStaticMax) {
+        err = "H-PLOC (no allocation): " + std::to_string(n) + " primitives; the fixed scratch holds 1 to " +
+              std::to_string(kHplocStaticMax);
+        return false;
+    }
+    if (!lo || !hi || !nodes) { err = "H-PLOC (no allocation): null array"; return false; }
+    int dev = 0;
+    HCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) { err = "H-PLOC (no allocation): device index out of range"; return false; }
+    unsigned char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratchMu);
+        if (!g_scratchPtr[dev]) {   // once per device, with that device current
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                err = std::string("H-PLOC (no allocation): scratch address: ") + hipGetErrorString(e);
+                return false;
+            }
+            g_scratchPtr[dev] = (unsigned char*)p;
+        }
+        base = g_scratchPtr[dev];
+    }
+    size_t bytes[18];
+    int tc = 0;
+    HplocBuffers::sizes(n, false, bytes, tc);
+    size_t off[18], at = 0;
+    for (int k = 0; k < 18; k++) { off[k] = at; at += (bytes[k] + 255) & ~(size_t)255; }
+    if (at > kStaticScratchBytes) { err = "H-PLOC (no allocation): scratch too small for this build"; return false; }
+    HplocBuffers B;
+    B.cap = n; B.tcap = tc;
+    B.nodes = nodes;
+    B.keysA = (uint64_t*)(base + off[1]); B.keysB = (uint64_t*)(base + off[2]);
+    B.clusters = (int*)(base + off[3]); B.bounds = (int*)(base + off[4]); B.counter = (int*)(base + off[5]);
+    B.counts = (int*)(base + off[6]); B.cboxes = (Vec4*)(base + off[7]); B.tnodes = (Vec4*)(base + off[8]);
+    B.theight = (int*)(base + off[9]); B.newId = (int*)(base + off[10]); B.status = (int*)(base + off[11]);
+    B.rkA = (uint64_t*)(base + off[12]); B.rkB = (uint64_t*)(base + off[13]);
+    B.rangeL = (int*)(base + off[14]); B.rangeR = (int*)(base + off[15]); B.cbox = (uint32_t*)(base + off[16]);
+    return hplocEnqueue(B, lo, hi, n, stream, morton, root, err);
 }
 
 bool hplocToBVH2(const Vec4* hnodes, int n, Vec4* out, void* stream, std::string& err) {

@@ -204,6 +204,7 @@ SIGNATURES = {
                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
     "cwp_hploc_build_host_mode": ([_FP, _FP, _I, _I, _FP, ctypes.POINTER(_I), ctypes.POINTER(_D)], _I),
     "cwp_hploc_build_mode": ([_P, _P, _P, _I, _I, _P, ctypes.POINTER(_I)], _I),
+    "cwp_hploc_build_static": ([_P, _P, _I, _I, _P, _P, ctypes.POINTER(_I)], _I),
     "cwp_hploc_create": ([_I, ctypes.POINTER(_P)], _I),
     "cwp_hploc_destroy": ([_P], None),
     "cwp_hploc_build": ([_P, _P, _P, _I, _P, ctypes.POINTER(_I)], _I),

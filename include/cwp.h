@@ -270,6 +270,15 @@ void cwp_hploc_destroy(cwp_hploc* b);
 int cwp_hploc_build(cwp_hploc* b, const void* lo, const void* hi, int n, void* stream, int* root);
 /* as cwp_hploc_build, with the Morton mode (CWP_MORTON_ADAPTIVE or CWP_MORTON_30) */
 int cwp_hploc_build_mode(cwp_hploc* b, const void* lo, const void* hi, int n, int morton_bits, void* stream, int* root);
+/* No counterpart in the reference. An H-PLOC build that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid allocation): the same kernels as
+ * cwp_hploc_build, with their scratch in a fixed static device array (16 MiB per GPU, held until the process ends) for
+ * at most CWP_HPLOC_STATIC_MAX primitives (larger n is rejected with CWP_ERR_ARG). lo / hi: device arrays (4 floats
+ * per box); nodes: the caller's device buffer of 2n - 1 nodes (8 floats each); asynchronous on `stream` like
+ * cwp_hploc_build. Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU (other streams or
+ * threads) race. */
+enum { CWP_HPLOC_STATIC_MAX = 65536 };
+int cwp_hploc_build_static(const void* lo, const void* hi, int n, int morton_bits, void* nodes, void* stream, int* root);
 int cwp_hploc_nodes(const cwp_hploc* b, const void** nodes);
 /* 64 B BVH2 layout (16 floats per node, max(n - 1, 1) nodes, root n - 2 or 0) of device H-PLOC nodes into `out`
  * (device). Replaces hplocToBVH2 (src/accel/hploc.h:36). */
