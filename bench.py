@@ -76,12 +76,38 @@ def measured_utilisation(kernel_tag):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# The plain method on the CPU: 1,024 seeded boxes + ground (12,290 triangles), Lambert albedo 0.7, sun toward
+# (0.3, 0.8, 0.5) with irradiance 5 and sky 0.5, pinhole 256x256, 16 spp, 4 bounces, binned-SAH BVH2 (32 bins, leaf <= 4)
+PLAIN_CPU_ARGS = ["--gen", "boxes", "--width", "256", "--height", "256", "--spp", "16", "--batch", "16", "--bounces", "4",
+                  "--variant", "bvh2", "--bins", "32", "--sun_to", "0.3", "0.8", "0.5", "--sun_rgb", "5", "5", "5",
+                  "--sky", "0.5", "0.5", "0.5", "--filter", "box"]
+
+
 def cpu_baseline(args):
-    """The oracle's multi-threaded CPU renderer on a bounded sample of the same workload (time-boxed)."""
+    """The oracle's multi-threaded CPU renderer (kind "port"): the plain CPU configuration in full, and a bounded sample
+    (reduced spp) of the headline terrain frame. Threads: the host cores a GPU of this machine is given (16 per GPU on
+    the measurement boxes; the visible core count is reported beside it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cwp
     import pyoracle as po
-    threads = max(1, min(16, os.cpu_count() or 1))
+    visible = os.cpu_count() or 1
+    threads = max(1, min(args.cpu_threads, visible))
+    s = cwp.Setup(PLAIN_CPU_ARGS)
+    sc, _ = po.scene_from_setup(s)
+    t0 = time.perf_counter()
+    _, st = sc.render(s.params(), 16, batch=16, bounces=4, threads=threads)
+    dt1 = time.perf_counter() - t0
+    rays1 = float(st[0] + st[1])
     spp = args.cpu_spp
     s = cwp.Setup(workload_args(args.width, args.height, spp, args.bounces, args.tri, args.variant, args.stack))
     sc, _ = po.scene_from_setup(s)
@@ -91,7 +117,11 @@ def cpu_baseline(args):
     rays = float(st[0] + st[1])
     return {"value": rays / dt / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
             "sample": "oracle CPU renderer, same scene/camera/lighting, %dx%d, %d spp, %d bounces, %d threads: "
-                      "%.0f rays in %.1f s" % (args.width, args.height, spp, args.bounces, threads, rays, dt)}
+                      "%.0f rays in %.1f s" % (args.width, args.height, spp, args.bounces, threads, rays, dt),
+            "cpu_model": cpu_model(), "cpus_visible": visible,
+            "plain_cpu_config": {"value": rays1 / dt1 / 1e9, "unit": "Grays/s", "frame_s": round(dt1, 3), "rays": rays1,
+                                 "workload": "1,024 boxes + ground (12,290 tris), 256x256, 16 spp, 4 bounces, BVH2 "
+                                             "32-bin SAH leaf <= 4, albedo 0.7, sun 5 + sky 0.5 (in full)"}}
 
 
 def calibrate_copy(mib):
@@ -108,7 +138,7 @@ def calibrate_copy(mib):
                       "copies": 3}), flush=True)
 
 
-def bench_instances(args):
+def bench_instances(args, frames=None, quiet=False):
     """Two-level scene: 5 meshes (1.3k-200k triangles) + ground, 1,000 instances with rigid transforms redrawn every
     frame; per frame the TLAS is rebuilt on the GPU (H-PLOC over the instance boxes) and the frame rendered at
     1280x720, 64 spp in flight. Reports Grays/s over the frames and the TLAS build time per frame."""
@@ -116,14 +146,15 @@ def bench_instances(args):
     spp = 64
     s = cwp.Setup(["--gen", "instances", "--width", "1280", "--height", "720", "--batch", str(spp), "--spp", str(spp),
                    "--bounces", str(args.bounces), "--variant", args.variant, "--tri", args.tri, "--filter", "box",
-                   "--stack", str(args.stack)] + LIGHTING)
+                   "--stack", str(args.stack), "--morton_bits", str(args.tlas_morton_bits)] + LIGHTING)
     r = cwp.Renderer(s)
     info = s.info
     print("bench: %d meshes, %d instances, %d BLAS nodes" % (info["num_meshes"], info["num_instances"], info["num_nodes"]),
           file=sys.stderr, flush=True)
     r.render(spp)   # warm-up
     tlas, frame_ms, rays = [], [], 0.0
-    for f in range(args.frames):
+    frames = args.frames if frames is None else frames
+    for f in range(frames):
         tlas.append(r.set_frame(f))
         st = r.render(spp)
         frame_ms.append(st["t_frame_ms"])
@@ -132,17 +163,20 @@ def bench_instances(args):
             print("bench: frame %d: TLAS %.3f ms, frame %.2f ms" % (f, tlas[-1], frame_ms[-1]), file=sys.stderr, flush=True)
     tot = sum(frame_ms)
     out = {"metric": "Grays/s and GPU TLAS rebuild ms per frame, 1,000-instance two-level scene", "value": round(rays / (tot * 1e-3) / 1e9, 4),
-           "unit": "Grays/s", "n_gpus": 1, "steps": args.frames, "warmup": 1, "ms_per_step": round(tot / args.frames, 3),
+           "unit": "Grays/s", "n_gpus": 1, "steps": frames, "warmup": 1, "ms_per_step": round(tot / frames, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "instances-1000-1280x720-64spp-depth%d" % args.bounces, "meshes": info["num_meshes"],
-                      "instances": info["num_instances"], "bvh": args.variant, "tri": args.tri, "frames": args.frames},
+                      "instances": info["num_instances"], "bvh": args.variant, "tri": args.tri, "frames": frames,
+                      "tlas_morton_bits": args.tlas_morton_bits},
            "tlas_ms_mean": round(float(np.mean(tlas)), 4), "tlas_ms_p50": round(float(np.percentile(tlas, 50)), 4),
            "tlas_ms_max": round(float(np.max(tlas)), 4)}
     r.close()
-    print(json.dumps(out), flush=True)
+    if not quiet:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_hploc(args):
+def bench_hploc(args, quiet=False):
     """H-PLOC GPU build over the triangle boxes of heightfield meshes the size of the reference's published builds
     (2.02 M and 2.80 M triangles): device time of one build with the boxes resident (hipEvents, median of 3 builds), and
     the host post-processing to a CWBVH layout (SAH leaf collapse + 8-wide collapse + triangle packing)."""
@@ -161,7 +195,7 @@ def bench_hploc(args):
         lo, hi = corners.min(1), corners.max(1)
         ms = []
         for _ in range(3):
-            nodes, root, gpu_ms = cwp.hploc_build(lo, hi)
+            nodes, root, gpu_ms = cwp.hploc_build(lo, hi, args.hploc_morton_bits)
             ms.append(gpu_ms)
         t0 = time.perf_counter()
         bv = cwp.bvh_from_hploc(v, t, nodes, root, 8)
@@ -181,9 +215,11 @@ def bench_hploc(args):
     out = {"metric": "H-PLOC GPU BVH build ms (boxes resident); CWBVH collapse ms on the GPU and on the host", "value": rows[-1]["gpu_build_ms"],
            "unit": "ms", "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": rows[-1]["gpu_build_ms"], "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "hploc-heightfield-2.02M-2.80M", "builds": rows},
+           "config": {"workload": "hploc-heightfield-2.02M-2.80M", "builds": rows, "morton_bits": args.hploc_morton_bits},
            "reference_published_h800_ms": {"2.02M": 1.29, "2.80M": 1.74, "host_post_ms": "226-404"}}
-    print(json.dumps(out), flush=True)
+    if not quiet:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 JOB_RES = (128, 256, 512, 1024)
@@ -206,6 +242,21 @@ def bench_reference_config(args):
         r.close()
         print("bench: reference config %s: %.3f Grays/s" % (variant, out[variant + "_grays"]), file=sys.stderr, flush=True)
     return out
+
+
+def bench_variant_frame(args, width, height, tri):
+    """One frame of the terrain workload at another resolution or triangle test (8 bounces, BVH8, 256 spp in flight),
+    after a warm-up frame: Grays/s."""
+    import cwp
+    s = cwp.Setup(workload_args(width, height, 256, args.bounces, tri, "bvh8", args.stack))
+    r = cwp.Renderer(s)
+    r.render(256)
+    st = r.render(256)
+    r.close()
+    g = (st["rays_closest"] + st["rays_shadow"]) / (st["t_frame_ms"] * 1e-3) / 1e9
+    print("bench: %dx%d %s: %.3f Grays/s" % (width, height, tri, g), file=sys.stderr, flush=True)
+    return {"grays": round(g, 4), "ms": round(st["t_frame_ms"], 3), "width": width, "height": height, "tri": tri,
+            "bvh": "bvh8", "bounces": args.bounces, "spp": 256}
 
 
 def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000, full=False):
@@ -380,6 +431,14 @@ def main():
     ap.add_argument("--full", type=int, default=0, help="job stream / serving: the full material integrator "
                     "(OpenPBR-subset materials, emitters, 2048 x 1024 log-normal environment, MIS)")
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
+    ap.add_argument("--tlas-morton-bits", type=int, default=30, help="two-level scene: Morton keys of the per-frame TLAS "
+                    "(30: 10 bits per axis, as the two-level configuration asks; 0: the reference's adaptive keys)")
+    ap.add_argument("--hploc-morton-bits", type=int, default=0, help="hploc workload: Morton keys (0 adaptive, 30)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="CPU baseline threads (the host cores of one GPU)")
+    ap.add_argument("--no-extras", action="store_true", help="terrain workload: only the headline frame, the reference "
+                    "configuration and the job stream (no resolution / triangle-test variants, serving sweep, H-PLOC "
+                    "builds or two-level frames)")
+    ap.add_argument("--serve-jobs", type=int, default=32, help="terrain workload: jobs per rate of the serving sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy_mib > 0:
@@ -481,11 +540,29 @@ def main():
     r.close()
     if not args.no_reference_config:
         out["reference_config"] = bench_reference_config(args)
+    if not args.no_extras:
+        out["variants"] = {"watertight_1280x720": bench_variant_frame(args, 1280, 720, "watertight"),
+                           "woop_512x512": bench_variant_frame(args, 512, 512, "woop")}
     if not args.no_jobs:
         print("bench: mixed job stream", file=sys.stderr, flush=True)
         j = bench_jobs(args)
         out["jobs"] = j
         out["latency"] = dict(j["latency_s"], unit="s", speedup_batched_vs_plain=j["speedup"])
+    if not args.no_extras:
+        print("bench: serving sweep (full integrator)", file=sys.stderr, flush=True)
+        sargs = argparse.Namespace(**vars(args))
+        sargs.full, sargs.jobs = 1, args.serve_jobs
+        out["serve"] = {"integrator": "full (OpenPBR subset, emitters, 2048x1024 log-normal environment, MIS)",
+                        "jobs_per_rate": sargs.jobs, "rates": bench_serve(sargs)}
+        print("bench: H-PLOC builds", file=sys.stderr, flush=True)
+        h = bench_hploc(args, quiet=True)
+        out["hploc"] = {"builds": h["config"]["builds"], "morton_bits": args.hploc_morton_bits,
+                        "reference_published_h800_ms": h["reference_published_h800_ms"]}
+        print("bench: two-level scene", file=sys.stderr, flush=True)
+        inst = bench_instances(args, frames=10, quiet=True)
+        out["instances"] = {"grays": inst["value"], "ms_per_frame": inst["ms_per_step"], "frames": 10,
+                            "tlas_ms_mean": inst["tlas_ms_mean"], "tlas_ms_p50": inst["tlas_ms_p50"],
+                            "config": inst["config"]}
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args)

@@ -597,7 +597,7 @@ static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
     o->variant = d.batch.variant; o->tri_test = d.batch.tri; o->stack_lds = d.batch.stackLds; o->bounces = d.batch.bounces;
     o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;
     o->batched = d.batched ? 1 : 0; o->rays_per_lane = d.batch.raysPerLane; o->use_graph = d.batch.useGraph;
-    o->full = d.batch.full;
+    o->full = d.batch.full; o->rolling = d.rolling ? 1 : 0;
 }
 
 void cwp_server_options_default(cwp_server_options* o) {
@@ -617,7 +617,7 @@ int cwp_server_create(const cwp_server_options* o, cwp_server** out) {
     so.batch.variant = o->variant; so.batch.tri = o->tri_test; so.batch.stackLds = o->stack_lds; so.batch.bounces = o->bounces;
     so.batch.maxPaths = o->max_paths; so.batch.maxJobs = o->max_jobs; so.sceneBudgetMB = o->scene_budget_mb;
     so.batched = o->batched != 0; so.batch.raysPerLane = o->rays_per_lane; so.batch.useGraph = o->use_graph;
-    so.batch.full = o->full;
+    so.batch.full = o->full; so.rolling = o->rolling != 0;
     cwp_server* s = new (std::nothrow) cwp_server();
     if (!s) return fail(CWP_ERR_ARG, "out of host memory");
     std::string err;
@@ -683,6 +683,48 @@ int cwp_server_stats(const cwp_server* s, cwp_serve_stats* st) {
     st->jobs = x.jobs; st->passes = x.passes; st->scenes_loaded = x.scenesLoaded; st->scenes_evicted = x.scenesEvicted;
     st->wall_s = x.wallS; st->rays = x.rays; st->rays_per_s = x.raysPerS; st->latency_p50_s = x.latP50;
     st->latency_p99_s = x.latP99; st->latency_mean_s = x.latMean; st->max_paths_in_flight = x.maxPathsInFlight;
+    st->steps = x.steps;
+    return CWP_OK;
+}
+
+int cwp_server_start(cwp_server* s) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_start: null server");
+    std::string err;
+    if (!s->S.start(err)) return fail(CWP_ERR_STATE, err);
+    return CWP_OK;
+}
+
+int cwp_job_submit(cwp_server* s, int scene_id, int width, int height, int spp, int bounces, uint32_t seed, int* job_id) {
+    if (!s || !job_id) return fail(CWP_ERR_ARG, "cwp_job_submit: null argument");
+    ServeJob J;
+    J.scene = scene_id; J.w = width; J.h = height; J.spp = spp; J.bounces = bounces; J.seed = seed;
+    std::string err;
+    const int id = s->S.submitLive(J, err);
+    if (id < 0) return fail(s->S.running() ? CWP_ERR_ARG : CWP_ERR_STATE, err);
+    *job_id = id;
+    return CWP_OK;
+}
+
+int cwp_job_poll(const cwp_server* s, int job_id, int* state) {
+    if (!s || !state) return fail(CWP_ERR_ARG, "cwp_job_poll: null argument");
+    *state = (int)s->S.poll(job_id);
+    return CWP_OK;
+}
+
+int cwp_job_result(const cwp_server* s, int job_id, float* rgba, size_t num_pixels) {
+    if (!s || !rgba) return fail(CWP_ERR_ARG, "cwp_job_result: null argument");
+    std::vector<float> img;
+    std::string err;
+    if (!s->S.result(job_id, img, err)) return fail(CWP_ERR_STATE, err);
+    if (num_pixels * 4 < img.size()) return fail(CWP_ERR_ARG, "cwp_job_result: buffer too small");
+    memcpy(rgba, img.data(), img.size() * sizeof(float));
+    return CWP_OK;
+}
+
+int cwp_server_stop(cwp_server* s) {
+    if (!s) return fail(CWP_ERR_ARG, "cwp_server_stop: null server");
+    std::string err;
+    if (!s->S.stop(err)) return fail(err.find("not started") != std::string::npos ? CWP_ERR_STATE : CWP_ERR_DEVICE, err);
     return CWP_OK;
 }
 

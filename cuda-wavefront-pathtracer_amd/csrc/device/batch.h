@@ -36,6 +36,16 @@ struct BatchJob {
     void* accum = nullptr;       // device float4 per pixel (from allocImage)
 };
 
+// A live job of a rolling batch (see rollStep): its scene, parameters, samples per pixel of this round of samples
+// (K, from sample index smpBase), image, and the path ids it owns, [pathBase, pathBase + w h K).
+struct RollJob {
+    int scene = -1;
+    FrameParams F;
+    int K = 1, smpBase = 0;
+    void* accum = nullptr;
+    int pathBase = 0;
+};
+
 class BatchRenderer {
 public:
     BatchRenderer();
@@ -59,6 +69,22 @@ public:
     // device time. Ray counters accumulate over passes (counters()).
     bool pass(const std::vector<BatchJob>& jobs, bool sync, FrameStats& st, std::string& err);
     bool synchronize(std::string& err);
+    // Rolling batches: steps instead of passes. Each step is one trace+shade launch and one shadow launch over every
+    // live path of every live job (whatever its depth) plus the camera rays of the jobs admitted at that step, so a job
+    // starts at the next launch rather than the next pass. A job admitted at step s has traced every ray at step
+    // s + bounces + 1 and is reduced after that step (flag it done there); its path ids are free for the next step.
+    // rollBegin clears the step counters; rollStep queues one step (no host synchronisation except when the step-table
+    // ring wraps) and returns its id; stepFinished tells whether a step has completed on the device.
+    bool rollBegin(std::string& err);
+    bool rollStep(const std::vector<RollJob>& live, const std::vector<char>& isNew, const std::vector<char>& isDone,
+                  long long& stepId, std::string& err);
+    bool stepFinished(long long stepId) const;
+    bool stepSlotFree() const;   // rollStep would not wait for an earlier step
+    // an image copy on the batch's copy stream, which does not wait for queued steps (the caller makes sure the image
+    // is final: the step that reduced it has finished); and a clear queued on the batch's stream, ahead of the step
+    // that first writes the image
+    bool fetchImage(const void* img, int nPix, float* rgba, std::string& err);
+    bool clearImageAsync(void* img, int nPix, std::string& err);
     // ray counters summed over the passes since the last reset (waits for queued passes)
     bool counters(FrameStats& total, bool reset, std::string& err);
     const BatchOptions& options() const;

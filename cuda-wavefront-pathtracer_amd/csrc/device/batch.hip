@@ -28,12 +28,14 @@ struct SceneDev {
 };
 
 constexpr int kTableSlots = 4;   // job tables in flight (passes may be queued without host synchronisation)
+constexpr int kRollSlots = 8;    // step tables in flight of a rolling batch (even: a slot always holds steps of one parity)
 
 }  // namespace
 
 struct BatchRenderer::Impl {
     BatchOptions O;
     hipStream_t st = nullptr;
+    hipStream_t cst = nullptr;   // image read-back of rolling batches (does not wait for queued steps)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<SceneDev> scenes;
     DevPaths P{};
@@ -50,6 +52,14 @@ struct BatchRenderer::Impl {
     hipGraphExec_t graph[kTableSlots] = {};
     int graphGen[kTableSlots] = {};
     int pathGen = 1;
+    // rolling batches: step tables, their completion events and graphs (one per slot)
+    int gridRoll = 0, gridRollFull = 0;
+    DevStepTable* dStep[kRollSlots] = {};
+    DevStepTable* hStep[kRollSlots] = {};
+    hipEvent_t stepDone[kRollSlots] = {};
+    hipGraphExec_t rollGraph[kRollSlots] = {};
+    int rollGraphGen[kRollSlots] = {};
+    long long steps = 0;   // steps queued since rollBegin
 
     template <class T> bool alloc(T*& p, size_t count, std::string& err) {
         void* q = nullptr;
@@ -70,7 +80,8 @@ struct BatchRenderer::Impl {
         if (cap <= spillCap) return true;
         HIPCHK(hipStreamSynchronize(st));
         if (P.spill) { hipFree(P.spill); P.spill = nullptr; }
-        HIPCHK(hipMalloc(&P.spill, (size_t)std::max(grid, gridFull) * kBlock * cap * sizeof(uint64_t)));
+        HIPCHK(hipMalloc(&P.spill, (size_t)std::max(std::max(grid, gridFull), std::max(gridRoll, gridRollFull)) * kBlock * cap *
+                                   sizeof(uint64_t)));
         spillCap = cap; P.spillCap = cap;
         pathGen++;   // captured graphs hold the old spill pointer
         return true;
@@ -84,6 +95,29 @@ struct BatchRenderer::Impl {
         }
         k_reduce_jobs<<<redGrid, kWave, 0, st>>>(dTable[s], P.L);
         k_pass_end<<<1, 64, 0, st>>>(P.cnt, depths, tot);
+    }
+    // the launches of a rolling step reading step-table slot s (step parity = s & 1)
+    void enqueueStep(int s, int step) {
+        k_step_begin<<<1, 64, 0, st>>>(dStep[s], P.cnt, step);
+        (O.full ? kn.bounceRollFull : kn.bounceRoll)<<<O.full ? gridRollFull : gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        kn.shadowRoll<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        k_reduce_roll<<<redGrid, kWave, 0, st>>>(dStep[s], P.L);
+        k_step_end<<<1, 64, 0, st>>>(P.cnt, step, tot);
+    }
+    bool captureStep(int s, std::string& err) {
+        if (rollGraph[s] && rollGraphGen[s] == pathGen) return true;
+        if (rollGraph[s]) { HIPCHK(hipGraphExecDestroy(rollGraph[s])); rollGraph[s] = nullptr; }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueueStep(s, s);
+        const hipError_t e = hipGetLastError();
+        HIPCHK(hipStreamEndCapture(st, &g));
+        if (e != hipSuccess) { hipGraphDestroy(g); err = std::string("batch: step capture: ") + hipGetErrorString(e); return false; }
+        const hipError_t ei = hipGraphInstantiate(&rollGraph[s], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) { rollGraph[s] = nullptr; err = std::string("batch: graph instantiate: ") + hipGetErrorString(ei); return false; }
+        rollGraphGen[s] = pathGen;
+        return true;
     }
     bool capture(int s, std::string& err) {
         if (graph[s] && graphGen[s] == pathGen) return true;
@@ -114,9 +148,15 @@ BatchRenderer::~BatchRenderer() {
         if (p_->hTable[k]) hipHostFree(p_->hTable[k]);
         if (p_->tableDone[k]) hipEventDestroy(p_->tableDone[k]);
     }
+    for (int k = 0; k < kRollSlots; k++) {
+        if (p_->rollGraph[k]) hipGraphExecDestroy(p_->rollGraph[k]);
+        if (p_->hStep[k]) hipHostFree(p_->hStep[k]);
+        if (p_->stepDone[k]) hipEventDestroy(p_->stepDone[k]);
+    }
     if (p_->hostTot) hipHostFree(p_->hostTot);
     if (p_->e0) hipEventDestroy(p_->e0);
     if (p_->e1) hipEventDestroy(p_->e1);
+    if (p_->cst) { hipStreamSynchronize(p_->cst); hipStreamDestroy(p_->cst); }
     if (p_->st) hipStreamDestroy(p_->st);
     delete p_;
 }
@@ -139,6 +179,7 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     size_t free0 = 0, total = 0;
     HIPCHK(hipMemGetInfo(&free0, &total));
     HIPCHK(hipStreamCreateWithFlags(&I.st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&I.cst, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&I.e0)); HIPCHK(hipEventCreate(&I.e1));
     if (O.variant == 2) pickKernels<2>(O.tri, I.sl, 8, I.kn);
     else if (O.variant == 4) pickKernels<4>(O.tri, I.sl, 8, I.kn);
@@ -149,6 +190,11 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     int bf = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bf, (const void*)I.kn.bounceJobsFull, kBlock, 0) != hipSuccess || bf < 1) bf = 1;
     I.gridFull = std::max(kParts, (bf * I.numCU / O.raysPerLane) / kParts * kParts);
+    int br = 0, brf = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&br, (const void*)I.kn.bounceRoll, kBlock, 0) != hipSuccess || br < 1) br = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&brf, (const void*)I.kn.bounceRollFull, kBlock, 0) != hipSuccess || brf < 1) brf = 1;
+    I.gridRoll = std::max(kParts, (br * I.numCU / O.raysPerLane) / kParts * kParts);
+    I.gridRollFull = std::max(kParts, (brf * I.numCU / O.raysPerLane) / kParts * kParts);
     gridBlocks = I.grid;
     I.redGrid = (int)std::min<size_t>(16384, O.maxPaths / kWave + (size_t)O.maxJobs);
     I.depths = O.bounces + 2;
@@ -164,6 +210,11 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
         if (!I.alloc(I.dTable[k], 1, err)) return false;
         HIPCHK(hipHostMalloc((void**)&I.hTable[k], sizeof(DevJobTable), hipHostMallocDefault));
         HIPCHK(hipEventCreate(&I.tableDone[k]));
+    }
+    for (int k = 0; k < kRollSlots; k++) {
+        if (!I.alloc(I.dStep[k], 1, err)) return false;
+        HIPCHK(hipHostMalloc((void**)&I.hStep[k], sizeof(DevStepTable), hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&I.stepDone[k], hipEventDisableTiming));
     }
     HIPCHK(hipStreamSynchronize(I.st));
     size_t free1 = 0;
@@ -321,6 +372,79 @@ bool BatchRenderer::pass(const std::vector<BatchJob>& jobs, bool sync, FrameStat
     st.frameMs = ms; st.traceMs = ms;
     return true;
 }
+
+bool BatchRenderer::rollBegin(std::string& err) {
+    Impl& I = *p_;
+    // both counter blocks of the step ring start empty (no continuations before the first step)
+    if (!I.fill(I.P.cnt, (size_t)2 * kCntStride * sizeof(int), 0u, err)) return false;
+    I.steps = 0;
+    return true;
+}
+
+bool BatchRenderer::rollStep(const std::vector<RollJob>& live, const std::vector<char>& isNew,
+                             const std::vector<char>& isDone, long long& stepId, std::string& err) {
+    Impl& I = *p_;
+    const int nj = (int)live.size();
+    if (nj > I.O.maxJobs) { err = "batch: too many live jobs in one step"; return false; }
+    if (isNew.size() != live.size() || isDone.size() != live.size()) { err = "batch: step flags do not match the jobs"; return false; }
+    const long long step = I.steps;
+    const int s = (int)(step % kRollSlots);
+    if (step >= kRollSlots) HIPCHK(hipEventSynchronize(I.stepDone[s]));   // the slot's previous step is done
+    DevStepTable* T = I.hStep[s];
+    // jobs by path base (the kernels find a path's job by binary search); new and done lists in that order
+    std::vector<int> order(nj);
+    for (int k = 0; k < nj; k++) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return live[a].pathBase < live[b].pathBase; });
+    long long cam = 0;
+    int groups = 0, nNew = 0, nDone = 0;
+    long long lastEnd = 0;
+    for (int r = 0; r < nj; r++) {
+        const RollJob& B = live[order[r]];
+        if (B.scene < 0 || B.scene >= (int)I.scenes.size() || !I.scenes[B.scene].alive) { err = "batch: a live job names no resident scene"; return false; }
+        const SceneDev& D = I.scenes[B.scene];
+        const int N = B.F.cam.w * B.F.cam.h;
+        if (N < 1 || B.K < 1 || !B.accum || B.smpBase < 0 || B.pathBase < 0) { err = "batch: a live job has no pixels, samples, image or paths"; return false; }
+        if (B.pathBase < lastEnd || (size_t)B.pathBase + (size_t)N * B.K > I.O.maxPaths) { err = "batch: live jobs' path ranges overlap or exceed the slots"; return false; }
+        if (B.F.hasN && !D.S.nrm) { err = "batch: a live job shades with normals its scene lacks"; return false; }
+        if (B.F.maxB > I.O.bounces) { err = "batch: a live job bounces deeper than the batch allows"; return false; }
+        lastEnd = (long long)B.pathBase + (long long)N * B.K;
+        DevJob& J = T->jobs[r];
+        J.S = D.S; J.F = B.F; J.accum = (Vec4*)B.accum;
+        J.pathBase = B.pathBase; J.N = N; J.K = B.K; J.smpBase = B.smpBase; J.grpBase = 0;
+        if (isNew[order[r]]) { T->newJob[nNew] = r; T->camFirst[nNew] = (int)cam; nNew++; cam += (long long)N * B.K; }
+        if (isDone[order[r]]) { T->doneJob[nDone] = r; T->doneGrp[nDone] = groups; nDone++; groups += (N + kWave - 1) / kWave; }
+    }
+    T->nJobs = nj; T->nNew = nNew; T->nDone = nDone; T->doneGroups = groups; T->camPaths = cam;
+    if (I.O.useGraph && !I.captureStep(s, err)) return false;
+    HIPCHK(hipMemcpyAsync(I.dStep[s], T, offsetof(DevStepTable, jobs) + (size_t)nj * sizeof(DevJob), hipMemcpyHostToDevice, I.st));
+    if (I.O.useGraph) HIPCHK(hipGraphLaunch(I.rollGraph[s], I.st));
+    else I.enqueueStep(s, s);
+    HIPCHK(hipEventRecord(I.stepDone[s], I.st));
+    HIPCHK(hipGetLastError());
+    stepId = step;
+    I.steps++;
+    return true;
+}
+
+bool BatchRenderer::stepFinished(long long stepId) const {
+    const Impl& I = *p_;
+    if (stepId < I.steps - kRollSlots) return true;   // its slot was reused, which waited for it
+    return hipEventQuery(I.stepDone[stepId % kRollSlots]) == hipSuccess;
+}
+
+bool BatchRenderer::stepSlotFree() const {
+    const Impl& I = *p_;
+    return I.steps < kRollSlots || hipEventQuery(I.stepDone[I.steps % kRollSlots]) == hipSuccess;
+}
+
+bool BatchRenderer::fetchImage(const void* img, int nPix, float* rgba, std::string& err) {
+    Impl& I = *p_;
+    HIPCHK(hipMemcpyAsync(rgba, img, (size_t)nPix * 16, hipMemcpyDeviceToHost, I.cst));
+    HIPCHK(hipStreamSynchronize(I.cst));
+    return true;
+}
+
+bool BatchRenderer::clearImageAsync(void* img, int nPix, std::string& err) { return p_->fill(img, (size_t)nPix * 16, 0u, err); }
 
 bool BatchRenderer::counters(FrameStats& t, bool reset, std::string& err) {
     Impl& I = *p_;

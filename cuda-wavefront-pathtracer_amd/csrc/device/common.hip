@@ -215,5 +215,65 @@ __global__ void __launch_bounds__(kWave) k_reduce_jobs(const DevJobTable* __rest
     }
 }
 
+
+// ------------------------------------------------------------------------------------------------ rolling batches
+__global__ void k_step_begin(const DevStepTable* T, int* cnt, int step) {
+    int* blk = cnt + (step & 1) * kCntStride;
+    const int* prev = cnt + ((step + 1) & 1) * kCntStride;
+    for (int j = threadIdx.x; j < kCntStride; j += blockDim.x) blk[j] = 0;
+    __syncthreads();
+    if (threadIdx.x < kParts) {
+        const long long x = threadIdx.x, C = T->camPaths;
+        const int cam = (int)((x + 1) * C / kParts - x * C / kParts);
+        *ctr(blk, (int)x, kCIn) = ctrv(prev, (int)x, kCQueue) + cam;
+    }
+}
+
+// totals of step `step`: [0] rays traced, [1] shadow rays, [2] misses, [3] unoccluded, [4] continuations traced,
+// [5] camera-ray misses (the layout of k_pass_end)
+__global__ void k_step_end(const int* cnt, int step, unsigned long long* tot) {
+    if (threadIdx.x != 0) return;
+    const int* blk = cnt + (step & 1) * kCntStride;
+    const int* prev = cnt + ((step + 1) & 1) * kCntStride;
+    unsigned long long a = 0, b = 0, m = 0, u = 0, c = 0, m0 = 0;
+    for (int x = 0; x < kParts; x++) {
+        a += (unsigned)ctrv(blk, x, kCIn); b += (unsigned)ctrv(blk, x, kCShadow); m += (unsigned)ctrv(blk, x, kCMiss);
+        u += (unsigned)ctrv(blk, x, kCUnocc); c += (unsigned)ctrv(prev, x, kCQueue); m0 += (unsigned)ctrv(blk, x, kCMissCam);
+    }
+    tot[0] += a; tot[1] += b; tot[2] += m; tot[3] += u; tot[4] += c; tot[5] += m0;
+}
+
+// the jobs done after this step: accum[pix] += the K samples of each pixel, in sample order (as k_reduce_jobs)
+__global__ void __launch_bounds__(kWave) k_reduce_roll(const DevStepTable* __restrict__ T, const Vec4* __restrict__ L) {
+    __shared__ float buf[kWave][3 * kRedChunk + 1];
+    __shared__ int gbase[kMaxJobs];
+    const int nDone = T->nDone, groups = T->doneGroups;
+    for (int q = threadIdx.x; q < nDone; q += kWave) gbase[q] = T->doneGrp[q];
+    __syncthreads();
+    const int lane = laneId();
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int q = jobOf(gbase, nDone, g);
+        const DevJob& J = T->jobs[T->doneJob[q]];
+        const int N = J.N, K = J.K, p0 = (g - gbase[q]) * kWave;
+        const Vec4* Lj = L + J.pathBase;
+        float r = 0.f, gg = 0.f, b = 0.f;
+        for (int s0 = 0; s0 < K; s0 += kRedChunk) {
+            const int cs = min(kRedChunk, K - s0);
+            for (int w = 0; w < kWave / 4; w++) {
+                const int p = 4 * w + (lane >> 4), s = lane & 15;
+                if (p0 + p < N && s < cs) {
+                    const Vec4 v = Lj[(size_t)(p0 + p) * K + s0 + s];
+                    buf[p][3 * s] = v.x; buf[p][3 * s + 1] = v.y; buf[p][3 * s + 2] = v.z;
+                }
+            }
+            __syncthreads();
+            for (int s = 0; s < cs; s++) { r += buf[lane][3 * s]; gg += buf[lane][3 * s + 1]; b += buf[lane][3 * s + 2]; }
+            __syncthreads();
+        }
+        const int pix = p0 + lane;
+        if (pix < N) { Vec4 a = J.accum[pix]; a.x += r; a.y += gg; a.z += b; J.accum[pix] = a; }
+    }
+}
+
 }  // namespace dev
 }  // namespace cwp

@@ -41,6 +41,9 @@ constexpr int kParts = 8;        // queue partitions, one per XCD (workgroup w r
 constexpr int kLine = 32;
 constexpr int kCntStride = kParts * kLine;
 constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
+// rolling batches (steps instead of passes) also keep per partition and step: [kCIn] rays the step traces (the previous
+// step's continuations + the step's camera rays), [kCMissCam] camera-ray misses
+constexpr int kCIn = 4, kCMissCam = 5;
 
 // Lights and materials of the full integrator (device pointers; null / 0 for the legacy integrator)
 struct DevLights {
@@ -272,10 +275,11 @@ __device__ __forceinline__ void waveAdd(int count, int* counter) {
 __device__ __forceinline__ int* ctr(int* cnt, int part, int which) { return cnt + part * kLine + which; }
 __device__ __forceinline__ int ctrv(const int* cnt, int part, int which) { return cnt[part * kLine + which]; }
 
-// start of partition `part` in a region laid out as the partitions' queue lengths of a depth's counter block
-__device__ __forceinline__ int partStart(const int* cnt, int part) {
+// start of partition `part` in a region laid out as the partitions' counts `which` of a counter block (default: the
+// queue lengths of a depth)
+__device__ __forceinline__ int partStart(const int* cnt, int part, int which = kCQueue) {
     int s = 0;
-    for (int y = 0; y < part; y++) s += ctrv(cnt, y, kCQueue);
+    for (int y = 0; y < part; y++) s += ctrv(cnt, y, which);
     return s;
 }
 
@@ -355,25 +359,31 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
 
 #include "device/shade_full.h"
 
-// Wave-level appends of the shading results: continuations to partition `part` of the next queue (ray set
-// (depth + 1) & 1, or the sort's staging rays), shadow rays to partition `part` of the shadow queue. Returns the wave's
-// number of misses.
-__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int depth, int i,
-                                             const ShadeOut& r) {
+// Wave-level appends of the shading results: continuations (with their next depth in T.w) to ray set `outSet` at
+// outStart + the wave's block from *contCtr (or to the sort's staging rays), shadow rays at outStart + the block from
+// *shadowCtr. Returns the wave's number of misses.
+__device__ __forceinline__ int appendTo(const DevPaths& P, int* contCtr, int* shadowCtr, int outStart, int outSet,
+                                        int nextDepth, int i, const ShadeOut& r) {
     const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
     if (mc) {
-        const int pos = outStart + waveAppend(mc, ctr(cnt + kCntStride, part, kCQueue));
+        const int pos = outStart + waveAppend(mc, contCtr);
         if (r.cont) {
-            const Vec4 o = v4(r.po, asFloat(i)), d = v4(r.nd, r.pdf), t = v4(r.Tn, 0.f);
+            const Vec4 o = v4(r.po, asFloat(i)), d = v4(r.nd, r.pdf), t = v4(r.Tn, asFloat(nextDepth));
             if (P.qk) { P.stO[pos] = o; P.stD[pos] = d; P.stT[pos] = t; P.qk[pos] = (uint8_t)dirBucket(r.nd); }
-            else { const int s = (depth + 1) & 1; P.O[s][pos] = o; P.D[s][pos] = d; P.T[s][pos] = t; }
+            else { P.O[outSet][pos] = o; P.D[outSet][pos] = d; P.T[outSet][pos] = t; }
         }
     }
     if (ms) {
-        const int pos = outStart + waveAppend(ms, ctr(cnt, part, kCShadow));
+        const int pos = outStart + waveAppend(ms, shadowCtr);
         if (r.shadow) { P.sO[pos] = v4(r.so, asFloat(i)); P.sD[pos] = v4(r.sdir, r.stmax); P.sC[pos] = v4(r.scon, 0.f); }
     }
     return (int)__builtin_popcountll(mm);
+}
+// the appends of a depth-ordered pass: continuations to partition `part` of the depth + 1 queue (ray set
+// (depth + 1) & 1), shadow rays to partition `part` of the depth's shadow queue
+__device__ __forceinline__ int appendResults(const DevPaths& P, int* cnt, int part, int outStart, int depth, int i,
+                                             const ShadeOut& r) {
+    return appendTo(P, ctr(cnt + kCntStride, part, kCQueue), ctr(cnt, part, kCShadow), outStart, (depth + 1) & 1, depth + 1, i, r);
 }
 
 // The ray at queue slot `slot` of this depth and its path id i: at depth 0 the slot is the path id and the camera ray
@@ -807,6 +817,138 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs(const DevJobTable* __
     waveAdd(unocc, ctr(cnt, part, kCUnocc));
 }
 
+// ------------------------------------------------------------------------------------------------ rolling batches
+// A rolling batch runs STEPS rather than passes: every step is one trace+shade launch and one shadow launch over every
+// live path of every live job, whatever its depth, plus the camera rays of the jobs admitted at that step. A path
+// carries its depth (T.w), so its arithmetic, random numbers and radiance are those of the job rendered alone; a job
+// admitted at step s has traced all its rays at step s + bounces + 1 and is reduced right after that step. New jobs
+// therefore join the very next launch instead of waiting for a pass to end, and the deep bounces of old jobs share
+// launches with the camera rays of new ones (no per-pass tail of nearly empty launches).
+//
+// Step s reads its continuations from ray set s & 1 (appended by step s - 1) and appends to set (s + 1) & 1; its
+// counters are block s & 1 of a ring of two. Partition x's input is the previous step's continuations of partition x,
+// then its share of the step's camera rays; it appends at the start given by the prefix of the partitions' inputs.
+struct DevStepTable {
+    int nJobs;            // live jobs, path bases ascending
+    int nNew;             // jobs whose camera rays this step traces
+    int nDone;            // jobs reduced after this step
+    int doneGroups;       // 64-pixel reduction groups over the done jobs
+    long long camPaths;   // camera rays of this step
+    int pad[10];          // 64-byte header
+    int newJob[kMaxJobs];    // new jobs (indices into jobs), in camera-index order
+    int camFirst[kMaxJobs];  // first camera index of each new job (ascending)
+    int doneJob[kMaxJobs];   // done jobs (indices into jobs)
+    int doneGrp[kMaxJobs];   // first reduction group of each done job (ascending)
+    DevJob jobs[kMaxJobs];
+};
+
+// the new job (index into newJob / camFirst) of camera index c: the last one whose first index is <= c
+__device__ __forceinline__ int camJobOf(const DevStepTable* __restrict__ T, int c) {
+    int lo = 0, hi = T->nNew;
+    while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (T->camFirst[mid] <= c) lo = mid; else hi = mid; }
+    return lo;
+}
+
+template <int V, int TRI, int SL, int WAVES, bool FULL = false>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTable* __restrict__ T, DevPaths P, int step) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
+    __syncthreads();
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* blk = P.cnt + (step & 1) * kCntStride;
+    const int* prev = P.cnt + ((step + 1) & 1) * kCntStride;
+    const int n = ctrv(blk, part, kCIn);
+    const int nCont = ctrv(prev, part, kCQueue);
+    const int contStart = partStart(prev, part, kCIn);
+    const int outStart = partStart(blk, part, kCIn);
+    const int camLo = (int)((long long)part * T->camPaths / kParts);
+    const int inSet = step & 1, outSet = (step + 1) & 1;
+    int misses = 0, camMisses = 0;
+    for (int b0 = lw * kWave; b0 < n; b0 += nw * kWave) {
+        const int k = b0 + lane;
+        int i = 0, depth = 0;
+        bool cam = false;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            int j, slot = 0;
+            Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
+            if (k < nCont) {
+                slot = contStart + k;
+                Or = P.O[inSet][slot];
+                i = asInt(Or.w);
+                j = jobOf(base, nJobs, i);
+            } else {
+                cam = true;
+                const int c = camLo + (k - nCont);
+                const int q = camJobOf(T, c);
+                j = T->newJob[q];
+                i = base[j] + (c - T->camFirst[q]);
+            }
+            const DevJob& J = jobs[j];
+            const int li = i - base[j], Kj = J.K, pix = li / Kj;
+            const uint32_t smp = (uint32_t)(J.smpBase + li % Kj);
+            Vec3 o, d; float tmin;
+            if (cam) { cameraRay(J.F, pix, smp, o, d); tmin = J.F.nearT; }
+            else { o = xyz(Or); d = xyz(P.D[inSet][slot]); tmin = 0.f; }
+            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
+            Vec3 Tp = v3(1.f, 1.f, 1.f);
+            float lastPdf = 0.f;
+            if (!cam) { const Vec4 Tv = P.T[inSet][slot]; Tp = xyz(Tv); depth = asInt(Tv.w); lastPdf = P.D[inSet][slot].w; }
+            if constexpr (FULL) res = shadeFull<false>(J.S, P, J.F, depth, i, pix, smp, o, d, Tp, lastPdf, h);
+            else res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, Tp, h);
+        }
+        camMisses += (int)__builtin_popcountll(__ballot(cam && res.miss));
+        misses += appendTo(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
+    }
+    waveAdd(misses, ctr(blk, part, kCMiss));
+    waveAdd(camMisses, ctr(blk, part, kCMissCam));
+}
+
+// the step's shadow rays (as k_shadow_jobs, counters of block step & 1)
+template <int V, int TRI, int SL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_roll(const DevStepTable* __restrict__ T, DevPaths P, int step) {
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kBlock : 1];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
+    __syncthreads();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* blk = P.cnt + (step & 1) * kCntStride;
+    const int n = ctrv(blk, part, kCShadow);
+    const int start = partStart(blk, part, kCIn);
+    int unocc = 0;
+    for (int b0 = lw * kWave; b0 < n; b0 += nw * kWave) {
+        const int k = b0 + laneId();
+        bool vis = false;
+        if (k < n) {
+            const Vec4 O = P.sO[start + k], D = P.sD[start + k];
+            const int i = asInt(O.w);
+            const DevJob& J = jobs[jobOf(base, nJobs, i)];
+            Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            if (!traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st)) {
+                vis = true;
+                const Vec4 c = P.sC[start + k];
+                Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+            }
+        }
+        unocc += (int)__builtin_popcountll(__ballot(vis));
+    }
+    waveAdd(unocc, ctr(blk, part, kCUnocc));
+}
+
 // ------------------------------------------------------------------------------------------------------- dispatch
 // The frame parameters (camera, lights, filter table: ~400 bytes) are read through a pointer: as a by-value argument
 // they occupied scalar registers for the whole kernel and made the trace+shade kernel spill (68 -> 24 bytes of
@@ -815,6 +957,7 @@ using BounceFn = void (*)(DevScene, DevPaths, const FrameParams*, int, int, int)
 using ShadowFn = void (*)(DevScene, DevPaths, int);
 using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, int*, void*, int);
 using JobsFn = void (*)(const DevJobTable*, DevPaths, int);
+using RollFn = void (*)(const DevStepTable*, DevPaths, int);
 
 // the kernels of one (layout, triangle test, LDS stack) configuration
 struct Kernels {
@@ -833,6 +976,8 @@ struct Kernels {
     // fused trace + shade with persistent lanes (legacy and full integrator)
     BounceFn bounceRefill = nullptr, bounceRefillFull = nullptr;
     JobsFn bounceJobsFull = nullptr;
+    // rolling batches
+    RollFn bounceRoll = nullptr, bounceRollFull = nullptr, shadowRoll = nullptr;
 };
 
 template <int V, int TRI, int SL>
@@ -856,6 +1001,9 @@ void pickSL(int waves, Kernels& k) {
                                 : (waves >= 6 ? k_bounce_refill<V, TRI, SL, 6, false, kShadeBatch> : k_bounce_refill<V, TRI, SL, 4, false, kShadeBatch>);
     k.bounceRefillFull = k_bounce_refill<V, TRI, SL, kFullWaves, true, kShadeBatch>;
     k.bounceJobsFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true>;
+    k.bounceRoll = k_bounce_roll<V, TRI, SL, 8>;
+    k.bounceRollFull = k_bounce_roll<V, TRI, SL, kFullWaves, true>;
+    k.shadowRoll = k_shadow_roll<V, TRI, SL>;
 }
 template <int V, int TRI>
 void pickTri(int sl, int waves, Kernels& k) {
@@ -887,6 +1035,11 @@ __global__ void k_fill(uint32_t* p, size_t words, uint32_t v);
 __global__ void k_tri_geo(const Vec4* __restrict__ verts, int nt, Vec4* __restrict__ geo);
 __global__ void k_pass_begin_jobs(int* cnt, int words, const DevJobTable* T);
 __global__ void k_reduce_jobs(const DevJobTable* __restrict__ T, const Vec4* __restrict__ L);
+// rolling batches: the step's counter block (zeroed, then each partition's input = the previous step's continuations
+// + its share of the camera rays), the reduction of the jobs done after the step, the step's totals
+__global__ void k_step_begin(const DevStepTable* T, int* cnt, int step);
+__global__ void k_reduce_roll(const DevStepTable* __restrict__ T, const Vec4* __restrict__ L);
+__global__ void k_step_end(const int* cnt, int step, unsigned long long* tot);
 
 extern template void pickKernels<2>(int, int, int, Kernels&);
 extern template void pickKernels<4>(int, int, int, Kernels&);

@@ -1,20 +1,31 @@
 // host/serve.h - a render-job server over the batched renderer: continuous batching with admission by path-state
 // slots and scene memory, resident scenes with LRU eviction, and per-job latency.
 //
-// Jobs (scene, resolution, samples, bounce limit, seed, arrival time) are admitted in arrival order once they have
-// arrived, their scene is resident (uploading it, and evicting the least recently used scenes no admitted job needs,
-// within the scene budget) and the path-state slots their passes need are free. Every step runs one pass over all
-// admitted jobs (each traces K_j more samples per pixel); a job whose samples are done leaves with its image, and the
-// jobs that arrived meanwhile are admitted at the next step. The plain mode runs the same kernels one job at a time in
-// arrival order. A job's image is the same in either mode and in any mix.
+// Jobs (scene, resolution, samples, bounce limit, seed, arrival time) are admitted once they have arrived, their scene
+// is resident (uploading it, and evicting the least recently used scenes no live job needs, within the scene budget)
+// and a contiguous range of path ids for K_j samples of every pixel is free. Batched mode is ROLLING (continuous
+// batching at bounce granularity): the device runs steps, each one trace+shade and one shadow launch over every live
+// path of every live job plus the camera rays of the jobs admitted at that step, so an arrival joins the very next
+// launch; admission is first fit in arrival order (a job that does not fit yet does not block smaller ones behind it,
+// and a job that waits longer than the starvation limit stops later arrivals from overtaking it); the host never waits
+// for a step, it only polls which steps have finished. A job whose samples are done leaves with its image and frees
+// its path ids for the next step. The pass mode (rolling off) admits between whole multi-bounce passes; the plain mode
+// runs the same kernels one job at a time in arrival order. A job's image is the same in every mode and in any mix.
+// start() runs the rolling loop on a worker thread: submit() / poll() / result() may then be called from other threads
+// while jobs are on the GPU; stop() lets the queued jobs finish and joins the worker.
 //
 // The autotuner runs the submitted jobs under each combination of BVH width, LDS stack entries and rays per lane,
 // keeps the fastest (rays per second over the whole mix) and leaves the jobs unfinished for the next run.
 #pragma once
 #include "device/batch.h"
 #include "host/setup.h"
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace cwp {
@@ -23,6 +34,8 @@ struct ServeOptions {
     BatchOptions batch;          // layout, triangle test, LDS stack, path-state slots, jobs per pass, bounce limit
     double sceneBudgetMB = 16384;// device memory for resident scenes
     bool batched = true;         // false: plain, one job at a time
+    bool rolling = true;         // batched mode: admission at every step (true) or between whole passes (false)
+    double starveS = 0.25;       // rolling admission: a job waiting longer than this stops later jobs from overtaking it
 };
 
 struct ServeJob {
@@ -40,8 +53,12 @@ struct TuneTrial {
     double raysPerS = 0, wallS = 0;
 };
 
+// job states as poll() reports them
+enum JobState : int { kJobQueued = 0, kJobRunning = 1, kJobDone = 2 };
+
 struct ServeStats {
     int jobs = 0, passes = 0, scenesLoaded = 0, scenesEvicted = 0;
+    long long steps = 0;         // rolling steps (each one trace+shade and one shadow launch)
     double wallS = 0, rays = 0, raysPerS = 0, latP50 = 0, latP99 = 0, latMean = 0, maxPathsInFlight = 0;
 };
 
@@ -53,6 +70,14 @@ public:
     int addScene(const SceneSetup& Z, std::string& err);
     int submit(const ServeJob& J, std::string& err);
     bool run(std::string& err);   // processes every submitted, unfinished job
+    // live serving: the rolling loop on a worker thread; submitLive stamps the arrival with the current time
+    bool start(std::string& err);
+    int submitLive(ServeJob J, std::string& err);
+    JobState poll(int id) const;
+    bool result(int id, std::vector<float>& rgba, std::string& err) const;
+    bool stop(std::string& err);  // waits until every submitted job is done, then joins the worker
+    bool running() const { return worker_.joinable(); }
+    ~Server();
     const ServeJob& job(int id) const { return jobs_[id]; }
     int numJobs() const { return (int)jobs_.size(); }
     size_t sceneBytes(int id) const;
@@ -85,14 +110,23 @@ private:
         double lastUse = 0;
     };
     static int layIndex(int variant) { return variant == 2 ? 0 : (variant == 4 ? 1 : 2); }
+    bool runPasses(std::string& err);
+    bool runRolling(bool live, std::string& err);
     bool ensureLayout(HostScene& H, std::string& err);
     bool makeResident(int s, const std::vector<int>& pinned, double now, std::string& err);
     ServeOptions O_;
     std::unique_ptr<BatchRenderer> R_;
     std::vector<HostScene> scenes_;
-    std::vector<ServeJob> jobs_;
+    std::deque<ServeJob> jobs_;   // a deque: live submissions append without moving the jobs the worker holds
     ServeStats stats_;
     size_t residentBytes_ = 0;
+    // live serving
+    mutable std::mutex mu_;
+    std::condition_variable cv_;
+    std::thread worker_;
+    bool stopReq_ = false, workerOk_ = true;
+    std::string workerErr_;
+    std::chrono::steady_clock::time_point liveT0_;
 };
 
 }  // namespace cwp

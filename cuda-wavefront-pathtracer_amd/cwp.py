@@ -104,7 +104,7 @@ class ServerOptions(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("stack_lds", ctypes.c_int), ("bounces", ctypes.c_int),
                 ("max_paths", ctypes.c_size_t), ("max_jobs", ctypes.c_int), ("scene_budget_mb", ctypes.c_double),
                 ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("full", ctypes.c_int)]
+                ("full", ctypes.c_int), ("rolling", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -137,7 +137,7 @@ class JobInfo(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("jobs", "passes", "scenes_loaded", "scenes_evicted")] + \
                [(n, ctypes.c_double) for n in ("wall_s", "rays", "rays_per_s", "latency_p50_s", "latency_p99_s",
-                                               "latency_mean_s", "max_paths_in_flight")]
+                                               "latency_mean_s", "max_paths_in_flight")] + [("steps", ctypes.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -236,6 +236,11 @@ SIGNATURES = {
     "cwp_device_free": ([_P], None),
     "cwp_memcpy": ([_P, _P, _S, _I], _I),
     "cwp_bsdf_eval": ([_FP, _FP, _FP, _F, _I, _F, _F, _FP, _FP], _I),
+    "cwp_server_start": ([_P], _I),
+    "cwp_job_submit": ([_P, _I, _I, _I, _I, _I, ctypes.c_uint32, ctypes.POINTER(_I)], _I),
+    "cwp_job_poll": ([_P, _I, ctypes.POINTER(_I)], _I),
+    "cwp_job_result": ([_P, _I, ctypes.POINTER(ctypes.c_float), _S], _I),
+    "cwp_server_stop": ([_P], _I),
     "cwp_bsdf_sample": ([_FP, _FP, _F, _I, _F, _F, _F, _F, _F, _FP, _FP, _FP, ctypes.POINTER(_I)], _I),
 }
 
@@ -250,7 +255,10 @@ def lib():
             raise CwpError("libcwp.so not found at %s: build it with cuda-wavefront-pathtracer_amd/build.py" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
-            f = getattr(L, name)
+            try:
+                f = getattr(L, name)
+            except AttributeError:   # an older build: calling this entry point fails, the others work
+                continue
             f.argtypes = args
             f.restype = res
         _lib = L
@@ -504,6 +512,31 @@ class Server:
         s = ServeStats()
         _check(lib().cwp_server_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    # live serving: submissions while other jobs are on the GPU
+    JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+    def start(self):
+        _check(lib().cwp_server_start(self.h))
+
+    def submit_live(self, scene, width, height, spp, bounces=6, seed=1000):
+        jid = ctypes.c_int()
+        _check(lib().cwp_job_submit(self.h, int(scene), int(width), int(height), int(spp), int(bounces),
+                                    int(seed) & 0xFFFFFFFF, ctypes.byref(jid)))
+        return jid.value
+
+    def poll(self, jid):
+        st = ctypes.c_int()
+        _check(lib().cwp_job_poll(self.h, int(jid), ctypes.byref(st)))
+        return st.value
+
+    def result(self, jid, width, height):
+        out = np.zeros((height, width, 4), dtype=np.float32)
+        _check(lib().cwp_job_result(self.h, int(jid), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), width * height))
+        return out
+
+    def stop(self):
+        _check(lib().cwp_server_stop(self.h))
 
     def clear_jobs(self):
         _check(lib().cwp_server_clear_jobs(self.h))

@@ -310,6 +310,9 @@ typedef struct cwp_server_options {
     int rays_per_lane;                 /* 1, 2 or 4: persistent grid = resident waves / rays_per_lane */
     int use_graph;                     /* 1: each pass replays a captured HIP graph */
     int full;                          /* 1: full material integrator (legacy setups render their Lambert materials) */
+    int rolling;                       /* batched mode: 1 = admission at every step (one trace+shade and one shadow
+                                          launch over all live paths of all live jobs, whatever their depth, plus the
+                                          camera rays of the jobs admitted at that step); 0 = between whole passes */
 } cwp_server_options;
 typedef struct cwp_tune_trial {
     int variant, stack_lds, rays_per_lane;
@@ -320,8 +323,9 @@ typedef struct cwp_job_info {
     double arrival_s, admitted_s, finished_s, latency_s;   /* seconds after the start of cwp_server_run */
 } cwp_job_info;
 typedef struct cwp_serve_stats {
-    int jobs, passes, scenes_loaded, scenes_evicted;
+    int jobs, passes, scenes_loaded, scenes_evicted;   /* passes: rounds of K samples admitted */
     double wall_s, rays, rays_per_s, latency_p50_s, latency_p99_s, latency_mean_s, max_paths_in_flight;
+    long long steps;                                   /* rolling steps queued */
 } cwp_serve_stats;
 void cwp_server_options_default(cwp_server_options* o);
 int cwp_server_create(const cwp_server_options* o, cwp_server** out);
@@ -348,6 +352,18 @@ int cwp_server_autotune(cwp_server* s, const int* variants, int num_variants, co
                         const int* rays_per_lane, int num_rays_per_lane, cwp_tune_trial* trials, int max_trials,
                         int* num_trials, cwp_server_options* chosen);
 int cwp_server_get_options(const cwp_server* s, cwp_server_options* o);
+/* Live serving. No counterpart in the reference, which renders one job per process. cwp_server_start runs the rolling
+ * loop on a worker thread of the library; from then on cwp_job_submit (the arrival is the time of the call),
+ * cwp_job_poll and cwp_job_result may be called from any thread while other jobs are on the GPU; cwp_server_stop
+ * waits until every submitted job is done and joins the worker. cwp_server_run / _autotune / _preload and the
+ * per-job reads cwp_server_job / _image are for a stopped server. */
+enum { CWP_JOB_QUEUED = 0, CWP_JOB_RUNNING = 1, CWP_JOB_DONE = 2 };
+int cwp_server_start(cwp_server* s);
+int cwp_job_submit(cwp_server* s, int scene_id, int width, int height, int spp, int bounces, uint32_t seed, int* job_id);
+int cwp_job_poll(const cwp_server* s, int job_id, int* state);
+/* per-pixel radiance sums of a done job (divide by spp), float[4] per pixel; CWP_ERR_STATE while it is not done */
+int cwp_job_result(const cwp_server* s, int job_id, float* rgba, size_t num_pixels);
+int cwp_server_stop(cwp_server* s);
 /* uploads the scenes of the unfinished jobs, in arrival order, while they fit the scene budget (so that a run's
  * timing excludes the first uploads) */
 int cwp_server_preload(cwp_server* s);

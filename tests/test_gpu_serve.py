@@ -43,15 +43,18 @@ def run_jobs(srv, jobs, batched, arrivals=None):
     return [srv.image(i) for i in ids], [srv.job(i) for i in ids], st
 
 
-@pytest.mark.parametrize("variant,max_paths,graph,rpl", [(8, 1 << 20, 1, 1), (8, 1 << 14, 1, 1), (8, 1 << 14, 0, 4),
-                                                          (4, 1 << 20, 1, 2), (2, 1 << 14, 1, 1)])
-def test_job_alone_equals_job_in_mix(variant, max_paths, graph, rpl):
+@pytest.mark.parametrize("variant,max_paths,graph,rpl,rolling", [(8, 1 << 20, 1, 1, 1), (8, 1 << 14, 1, 1, 1),
+                                                                  (8, 1 << 14, 0, 4, 1), (4, 1 << 20, 1, 2, 1),
+                                                                  (2, 1 << 14, 1, 1, 1), (8, 1 << 14, 1, 1, 0),
+                                                                  (8, 1 << 20, 0, 1, 0)])
+def test_job_alone_equals_job_in_mix(variant, max_paths, graph, rpl, rolling):
     """Each job alone, all jobs in one batched run, and the plain mode give bit-identical images; with 2^14 path
-    slots the larger jobs take several passes and wait for slots to free up. Passes replayed from a captured graph
-    or launched one kernel at a time, and any rays per lane, trace the same paths."""
-    print("case", variant, max_paths, graph, rpl, flush=True)
+    slots the larger jobs take several rounds of samples and wait for slots to free up. Rolling steps (paths of every
+    depth and new camera rays in one launch) and whole passes, launches replayed from a captured graph or issued one
+    kernel at a time, and any rays per lane, trace the same paths."""
+    print("case", variant, max_paths, graph, rpl, rolling, flush=True)
     S = setups(variant)
-    srv = cwp.Server(variant=variant, bounces=4, max_paths=max_paths, use_graph=graph, rays_per_lane=rpl)
+    srv = cwp.Server(variant=variant, bounces=4, max_paths=max_paths, use_graph=graph, rays_per_lane=rpl, rolling=rolling)
     for s in S:
         srv.add_scene(s)
     alone = []
@@ -234,3 +237,58 @@ def test_legacy_scene_on_full_server_and_full_scene_on_legacy_server():
     lid = lsrv.add_scene(legacy)
     limg, _, _ = run_jobs(lsrv, [(lid, 32, 32, 8, 3, 2)], True)
     assert psnr(img[0][..., :3] / 8, limg[0][..., :3] / 8) >= 40.0
+
+
+def test_rolling_admits_arrivals_at_the_next_step():
+    """A long job (several rounds of samples) is running when a small job arrives: with rolling steps the small job
+    starts at the next launch and is done long before the long one; both images equal their images alone."""
+    S = setups(8)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 18)
+    for s in S:
+        srv.add_scene(s)
+    big, small = (0, 256, 256, 64, 4, 11), (1, 16, 16, 4, 3, 12)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in (big, small)]
+    imgs, infos, st = run_jobs(srv, [big, small], True, arrivals=[0.0, 0.01])
+    np.testing.assert_array_equal(imgs[0], alone[0])
+    np.testing.assert_array_equal(imgs[1], alone[1])
+    assert infos[0]["passes"] > 1
+    assert infos[1]["finished_s"] < infos[0]["finished_s"]
+    assert infos[1]["latency_s"] < 0.5 * infos[0]["latency_s"], infos
+    assert st["steps"] > 0
+
+
+def test_live_submissions_from_threads_while_jobs_run():
+    """cwp_server_start / cwp_job_submit / cwp_job_poll / cwp_job_result: two threads submit jobs while others are on
+    the GPU; every job reaches DONE and its image equals the job rendered alone."""
+    import threading
+    import time
+    S = setups(8)
+    srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16)
+    for s in S:
+        srv.add_scene(s)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in JOBS]
+    srv.clear_jobs()
+    srv.start()
+    ids = [None] * len(JOBS)
+
+    def submit(ks):
+        for k in ks:
+            j = JOBS[k]
+            ids[k] = srv.submit_live(*j[:5], seed=j[5])
+            time.sleep(0.003)
+    th = [threading.Thread(target=submit, args=(list(range(0, len(JOBS), 2)),)),
+          threading.Thread(target=submit, args=(list(range(1, len(JOBS), 2)),))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    deadline = time.time() + 60
+    while any(srv.poll(i) != srv.JOB_DONE for i in ids) and time.time() < deadline:
+        time.sleep(0.002)
+    states = [srv.poll(i) for i in ids]
+    srv.stop()
+    assert all(x == srv.JOB_DONE for x in states), states
+    for k, j in enumerate(JOBS):
+        np.testing.assert_array_equal(srv.result(ids[k], j[1], j[2]), alone[k], err_msg="job %d" % k)
+    with pytest.raises(cwp.CwpError):
+        srv.submit_live(0, 8, 8, 1)   # stopped
