@@ -341,9 +341,10 @@ def bench_jobs(args):
         srv.clear_jobs()
         for j in jobs[::4]:
             srv.submit(*j[:5], seed=j[5])
-        trials, chosen = srv.autotune()
-        print("bench: autotuner chose bvh%d stack %d rays/lane %d" % (chosen["variant"], chosen["stack_lds"],
-                                                                     chosen["rays_per_lane"]), file=sys.stderr, flush=True)
+        trials, chosen = srv.autotune(compacts=(0, 1))
+        print("bench: autotuner chose bvh%d stack %d rays/lane %d compaction %s" % (
+            chosen["variant"], chosen["stack_lds"], chosen["rays_per_lane"], ("wave", "lane")[chosen["compact"]]),
+            file=sys.stderr, flush=True)
         warm_up(srv, jobs)   # scenes resident in the chosen layout
     batched, ids = run_stream(srv, jobs, True)
     print("bench: batched %.3f Grays/s (%.2f s)" % (batched["rays_per_s"] / 1e9, batched["wall_s"]), file=sys.stderr, flush=True)
@@ -355,9 +356,11 @@ def bench_jobs(args):
             "latency_s": {"plain_p50": round(plain["latency_p50_s"], 4), "plain_p99": round(plain["latency_p99_s"], 4),
                           "batched_p50": round(batched["latency_p50_s"], 4), "batched_p99": round(batched["latency_p99_s"], 4)},
             "passes": {"plain": plain["passes"], "batched": batched["passes"]},
-            "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"]},
+            "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"],
+                       "compaction": ("wave", "lane")[chosen["compact"]]},
             "trials": [{"bvh": t["variant"], "stack": t["stack_lds"], "rpl": t["rays_per_lane"],
-                        "grays": round(t["rays_per_s"] / 1e9, 4)} for t in trials],
+                        "compaction": ("wave", "lane")[t["compact"]], "grays": round(t["rays_per_s"] / 1e9, 4)}
+                       for t in trials],
             "workload": {"jobs": len(jobs), "scenes": len(setups), "tris": tris, "resolutions": list(args.job_res),
                          "spp": list(args.job_spp), "depth": 6, "arrivals": "all at t=0", "seed": args.job_seed}}
 

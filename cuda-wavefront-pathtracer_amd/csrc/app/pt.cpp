@@ -1,6 +1,8 @@
 // app/pt.cpp - batch renderer CLI: set up the scene, render on the GPU, print one JSON line of statistics per timed
 // frame, write <out>.png (sRGB) and <out>.pfm (linear radiance). Same flags and JSON keys as the reference's pt
-// (src/app/pt.cpp:13-53): an untimed warm-up render of `--warm` samples, then `--reps` timed renders of `--spp`.
+// (src/app/pt.cpp:13-53): for each of the scene's frames (two-level scenes: instance transforms and emitters of that
+// frame, the TLAS rebuilt on the GPU), an untimed warm-up render of `--warm` samples before the first, then `--reps`
+// timed renders of `--spp`. The image files hold the last frame.
 #include "device/renderer.h"
 #include "host/io.h"
 #include "host/setup.h"
@@ -21,16 +23,33 @@ int main(int argc, char** argv) {
     if (!R.init(Z.S, Z.P, Z.O, err)) { fprintf(stderr, "renderer: %s\n", err.c_str()); return 2; }
     fprintf(stderr, "device %s: %d CUs, grid %d workgroups, %.0f MB\n", R.gpuName.c_str(), R.numCU, R.gridBlocks, R.vramMB);
     const int K = A.batch < 1 ? 1 : A.batch;
-    if (A.warm > 0) {
-        FrameStats W;
-        if (!R.render(((A.warm + K - 1) / K) * K, 0, true, W, err)) { fprintf(stderr, "render: %s\n", err.c_str()); return 2; }
-        fprintf(stderr, "warmup: %.1f ms, %.1f Mrays/s\n", W.frameMs, (W.rays + W.shadowRays) / (W.frameMs * 1e3));
-    }
-    for (int r = 0; r < A.reps; r++) {
-        FrameStats st;
-        if (!R.render(A.spp, 0, true, st, err)) { fprintf(stderr, "render: %s\n", err.c_str()); return 2; }
-        printf("%s\n", frameJson(Z, R, st, r, A.frame).c_str());
-        fflush(stdout);
+    std::vector<InstanceRecord> inst;
+    std::vector<Vec4> emTris;
+    std::vector<float> emCdf, matLightPdf;
+    for (int fi = 0; fi < Z.frames; fi++) {
+        const int f = A.frameStart + fi;
+        if (Z.inst) {
+            double tlasMs = 0;
+            if (!frameInstances(Z, f, inst) || !R.setInstances(inst.data(), (int)inst.size(), tlasMs, err)) {
+                fprintf(stderr, "frame %d: %s\n", f, err.c_str()); return 2;
+            }
+            if (Z.full) {
+                frameEmitters(Z, f, emTris, emCdf, matLightPdf);
+                if (!R.setEmitters(emTris.data(), (int)(emTris.size() / 3), emCdf.data(), matLightPdf.data(),
+                                   (int)matLightPdf.size(), err)) { fprintf(stderr, "frame %d: %s\n", f, err.c_str()); return 2; }
+            }
+        }
+        if (fi == 0 && A.warm > 0) {
+            FrameStats W;
+            if (!R.render(((A.warm + K - 1) / K) * K, 0, true, W, err)) { fprintf(stderr, "render: %s\n", err.c_str()); return 2; }
+            fprintf(stderr, "warmup: %.1f ms, %.1f Mrays/s\n", W.frameMs, (W.rays + W.shadowRays) / (W.frameMs * 1e3));
+        }
+        for (int r = 0; r < A.reps; r++) {
+            FrameStats st;
+            if (!R.render(A.spp, 0, true, st, err)) { fprintf(stderr, "render: %s\n", err.c_str()); return 2; }
+            printf("%s\n", frameJson(Z, R, st, r, f).c_str());
+            fflush(stdout);
+        }
     }
     const int N = A.w * A.h;
     std::vector<float> acc((size_t)N * 4), lin((size_t)N * 3);

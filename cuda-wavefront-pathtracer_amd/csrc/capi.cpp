@@ -3,6 +3,7 @@
 #include "core/bsdf.h"
 #include "device/renderer.h"
 #include "host/bvh.h"
+#include "host/envmap.h"
 #include "host/scene_check.h"
 #include "device/collapse.h"
 #include "device/hploc.h"
@@ -25,6 +26,7 @@ struct cwp_bvh {
 };
 
 static_assert(sizeof(cwp_params) == sizeof(FrameParams), "cwp_params must mirror FrameParams");
+static_assert(CWP_LIGHT_MIS == kLightMIS && CWP_LIGHT_NEE == kLightNEE && CWP_LIGHT_BSDF == kLightBSDF, "light modes");
 static_assert(CWP_MORTON_ADAPTIVE == kMortonAdaptive && CWP_MORTON_30 == kMorton30, "Morton modes");
 static_assert(CWP_HPLOC_STATIC_MAX == kHplocStaticMax, "allocation-free build limit");
 static_assert(sizeof(cwp_instance) == sizeof(InstanceRecord), "cwp_instance must mirror InstanceRecord");
@@ -302,7 +304,44 @@ static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& e
     if (d->woop) take(d->woop, d->num_tris * 3, S.woop); else S.woop.clear();
     if (d->normals) take(d->normals, d->num_tris * 3, S.normals); else S.normals.clear();
     take(d->albedo, d->num_materials, S.albedo);
+    if (d->mats) {   // the full material integrator
+        S.full = 1;
+        S.mats.resize(d->num_materials);
+        memcpy(S.mats.data(), d->mats, d->num_materials * sizeof(Material));
+        if (d->tri_light_pdf) {
+            if (!d->em_tri || !d->em_cdf || !d->num_emitters) { err = "emitters: tri_light_pdf needs em_tri, em_cdf and num_emitters"; return false; }
+            S.triLightPdf.assign(d->tri_light_pdf, d->tri_light_pdf + d->num_tris);
+            S.emTri.assign(d->em_tri, d->em_tri + d->num_emitters);
+            S.emCdf.assign(d->em_cdf, d->em_cdf + d->num_emitters + 1);
+        }
+        if (d->env_texels) {
+            if (d->env_w < 1 || d->env_h < 1) { err = "environment: env_w and env_h must be positive"; return false; }
+            std::vector<float> rgb((size_t)d->env_w * d->env_h * 3);
+            for (size_t t = 0; t < (size_t)d->env_w * d->env_h; t++)
+                for (int k = 0; k < 3; k++) rgb[3 * t + k] = d->env_texels[4 * t + k];
+            auto E = std::make_shared<EnvMap>();
+            buildEnvMap(rgb, d->env_w, d->env_h, 1.f, *E);
+            S.env = E;
+        }
+        S.envRotDeg = d->env_rot_deg; S.sunOn = d->sun_on; S.lightMode = d->light_mode;
+        S.rr = d->rr; S.rrDepth = d->rr_depth; S.gdEnergy = d->gd_energy; S.envUniform = d->env_uniform;
+        if (S.lightMode < CWP_LIGHT_MIS || S.lightMode > CWP_LIGHT_BSDF) { err = "light_mode must be MIS, NEE or BSDF"; return false; }
+    }
     return checkScene(S, S.stackNeed, err);
+}
+
+int cwp_set_materials(cwp_ctx* c, const float* mats, int num_materials) {
+    if (!c || !mats || num_materials < 1) return fail(CWP_ERR_ARG, "cwp_set_materials: bad arguments");
+    std::string err;
+    if (!c->R.setMaterials((const Material*)mats, num_materials, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_set_environment(cwp_ctx* c, const float* texels, size_t num_texels, float env_rot_deg, int sun_on) {
+    if (!c) return fail(CWP_ERR_ARG, "cwp_set_environment: null context");
+    std::string err;
+    if (!c->R.setEnvironment((const Vec4*)texels, num_texels, env_rot_deg, sun_on != 0, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
 }
 
 int cwp_scene_check(const cwp_scene_desc* d, int* stack_need) {
@@ -597,7 +636,7 @@ static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
     o->variant = d.batch.variant; o->tri_test = d.batch.tri; o->stack_lds = d.batch.stackLds; o->bounces = d.batch.bounces;
     o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;
     o->batched = d.batched ? 1 : 0; o->rays_per_lane = d.batch.raysPerLane; o->use_graph = d.batch.useGraph;
-    o->full = d.batch.full; o->rolling = d.rolling ? 1 : 0;
+    o->full = d.batch.full; o->rolling = d.rolling ? 1 : 0; o->compact = d.batch.compact;
 }
 
 void cwp_server_options_default(cwp_server_options* o) {
@@ -617,7 +656,7 @@ int cwp_server_create(const cwp_server_options* o, cwp_server** out) {
     so.batch.variant = o->variant; so.batch.tri = o->tri_test; so.batch.stackLds = o->stack_lds; so.batch.bounces = o->bounces;
     so.batch.maxPaths = o->max_paths; so.batch.maxJobs = o->max_jobs; so.sceneBudgetMB = o->scene_budget_mb;
     so.batched = o->batched != 0; so.batch.raysPerLane = o->rays_per_lane; so.batch.useGraph = o->use_graph;
-    so.batch.full = o->full; so.rolling = o->rolling != 0;
+    so.batch.full = o->full; so.rolling = o->rolling != 0; so.batch.compact = o->compact;
     cwp_server* s = new (std::nothrow) cwp_server();
     if (!s) return fail(CWP_ERR_ARG, "out of host memory");
     std::string err;
@@ -748,16 +787,20 @@ int cwp_server_reset_jobs(cwp_server* s) {
 }
 
 int cwp_server_autotune(cwp_server* s, const int* variants, int nv, const int* stacks, int ns, const int* rpl, int nr,
-                        cwp_tune_trial* trials, int max_trials, int* num_trials, cwp_server_options* chosen) {
-    if (!s || !variants || !stacks || !rpl || nv < 1 || ns < 1 || nr < 1 || (max_trials > 0 && !trials))
+                        const int* compacts, int nc, cwp_tune_trial* trials, int max_trials, int* num_trials,
+                        cwp_server_options* chosen) {
+    if (!s || !variants || !stacks || !rpl || nv < 1 || ns < 1 || nr < 1 || (compacts && nc < 1) ||
+        (max_trials > 0 && !trials))
         return fail(CWP_ERR_ARG, "cwp_server_autotune: bad arguments");
     std::vector<TuneTrial> T;
     std::string err;
+    const std::vector<int> cm = compacts ? std::vector<int>(compacts, compacts + nc) : std::vector<int>{0};
     const bool ok = s->S.autotune(std::vector<int>(variants, variants + nv), std::vector<int>(stacks, stacks + ns),
-                                  std::vector<int>(rpl, rpl + nr), T, err);
+                                  std::vector<int>(rpl, rpl + nr), cm, T, err);
     if (num_trials) *num_trials = (int)T.size();
     for (int k = 0; k < (int)T.size() && k < max_trials; k++) {
         trials[k].variant = T[k].variant; trials[k].stack_lds = T[k].stackLds; trials[k].rays_per_lane = T[k].raysPerLane;
+        trials[k].compact = T[k].compact;
         trials[k].rays_per_s = T[k].raysPerS; trials[k].wall_s = T[k].wallS;
     }
     if (!ok) return fail(CWP_ERR_DEVICE, err);

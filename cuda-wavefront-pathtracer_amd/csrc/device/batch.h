@@ -26,6 +26,7 @@ struct BatchOptions {
                                  // traces that many times more rays in sequence
     int useGraph = 1;            // replay each pass's launches from a captured HIP graph
     int full = 0;                // 1: the full material integrator (every scene full); 0: the legacy integrator
+    int compact = 0;             // queue compaction of rolling steps: 0 per wave (ballot, one atomic), 1 per lane
 };
 
 struct BatchJob {

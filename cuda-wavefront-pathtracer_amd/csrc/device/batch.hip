@@ -99,7 +99,8 @@ struct BatchRenderer::Impl {
     // the launches of a rolling step reading step-table slot s (step parity = s & 1)
     void enqueueStep(int s, int step) {
         k_step_begin<<<1, 64, 0, st>>>(dStep[s], P.cnt, step);
-        (O.full ? kn.bounceRollFull : kn.bounceRoll)<<<O.full ? gridRollFull : gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        const RollFn b = O.compact ? (O.full ? kn.bounceRollFullLane : kn.bounceRollLane) : (O.full ? kn.bounceRollFull : kn.bounceRoll);
+        b<<<O.full ? gridRollFull : gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
         kn.shadowRoll<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
         k_reduce_roll<<<redGrid, kWave, 0, st>>>(dStep[s], P.L);
         k_step_end<<<1, 64, 0, st>>>(P.cnt, step, tot);
@@ -172,6 +173,7 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     if (O.maxPaths < 1 || O.maxPaths > (size_t)0x7fffffff) { err = "batch: max paths must be in [1, 2^31)"; return false; }
     if (O.bounces < 0 || O.bounces > 64) { err = "batch: bounces must be in [0, 64]"; return false; }
     if (O.raysPerLane != 1 && O.raysPerLane != 2 && O.raysPerLane != 4) { err = "batch: rays per lane must be 1, 2 or 4"; return false; }
+    if (O.compact != 0 && O.compact != 1) { err = "batch: compaction must be 0 (per wave) or 1 (per lane)"; return false; }
     I.O = O; I.sl = O.stackLds;
     std::string name;
     if (!deviceInfo(I.numCU, name, err)) return false;

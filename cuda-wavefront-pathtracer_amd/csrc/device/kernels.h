@@ -359,11 +359,24 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
 
 #include "device/shade_full.h"
 
-// Wave-level appends of the shading results: continuations (with their next depth in T.w) to ray set `outSet` at
-// outStart + the wave's block from *contCtr (or to the sort's staging rays), shadow rays at outStart + the block from
-// *shadowCtr. Returns the wave's number of misses.
+// Appends of the shading results: continuations (with their next depth in T.w) to ray set `outSet` at outStart +
+// a slot from *contCtr (or to the sort's staging rays), shadow rays at outStart + a slot from *shadowCtr. Compaction
+// granularity: the wave (LANE false: one ballot and one atomic per wave, slots base + mbcnt) or the lane (LANE true:
+// one atomic per appending lane, the reference's scheme). Returns the wave's number of misses.
+template <bool LANE = false>
 __device__ __forceinline__ int appendTo(const DevPaths& P, int* contCtr, int* shadowCtr, int outStart, int outSet,
                                         int nextDepth, int i, const ShadeOut& r) {
+    if constexpr (LANE) {
+        if (r.cont) {
+            const int pos = outStart + atomicAdd(contCtr, 1);
+            P.O[outSet][pos] = v4(r.po, asFloat(i)); P.D[outSet][pos] = v4(r.nd, r.pdf); P.T[outSet][pos] = v4(r.Tn, asFloat(nextDepth));
+        }
+        if (r.shadow) {
+            const int pos = outStart + atomicAdd(shadowCtr, 1);
+            P.sO[pos] = v4(r.so, asFloat(i)); P.sD[pos] = v4(r.sdir, r.stmax); P.sC[pos] = v4(r.scon, 0.f);
+        }
+        return (int)__builtin_popcountll(__ballot(r.miss));
+    }
     const uint64_t mc = __ballot(r.cont), ms = __ballot(r.shadow), mm = __ballot(r.miss);
     if (mc) {
         const int pos = outStart + waveAppend(mc, contCtr);
@@ -849,7 +862,7 @@ __device__ __forceinline__ int camJobOf(const DevStepTable* __restrict__ T, int 
     return lo;
 }
 
-template <int V, int TRI, int SL, int WAVES, bool FULL = false>
+template <int V, int TRI, int SL, int WAVES, bool FULL = false, bool LANE = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTable* __restrict__ T, DevPaths P, int step) {
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
@@ -907,7 +920,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTabl
             else res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, Tp, h);
         }
         camMisses += (int)__builtin_popcountll(__ballot(cam && res.miss));
-        misses += appendTo(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
+        misses += appendTo<LANE>(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
     }
     waveAdd(misses, ctr(blk, part, kCMiss));
     waveAdd(camMisses, ctr(blk, part, kCMissCam));
@@ -978,6 +991,7 @@ struct Kernels {
     JobsFn bounceJobsFull = nullptr;
     // rolling batches
     RollFn bounceRoll = nullptr, bounceRollFull = nullptr, shadowRoll = nullptr;
+    RollFn bounceRollLane = nullptr, bounceRollFullLane = nullptr;   // per-lane compaction
 };
 
 template <int V, int TRI, int SL>
@@ -1003,6 +1017,8 @@ void pickSL(int waves, Kernels& k) {
     k.bounceJobsFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true>;
     k.bounceRoll = k_bounce_roll<V, TRI, SL, 8>;
     k.bounceRollFull = k_bounce_roll<V, TRI, SL, kFullWaves, true>;
+    k.bounceRollLane = k_bounce_roll<V, TRI, SL, 8, false, true>;
+    k.bounceRollFullLane = k_bounce_roll<V, TRI, SL, kFullWaves, true, true>;
     k.shadowRoll = k_shadow_roll<V, TRI, SL>;
 }
 template <int V, int TRI>

@@ -101,6 +101,11 @@ public:
     // Two-level full-integrator scenes: the frame's world-space emitters (3 float4 each), their power CDF (nEm + 1)
     // and the per-material light pdf per unit area; nEm must be the scene's instanced emitter count.
     bool setEmitters(const Vec4* emTris, int nEm, const float* emCdf, const float* matLightPdf, int nMats, std::string& err);
+    // runtime updates (reference: setMaterials / setEnvironment, src/render/renderer.h:59-63): same material count
+    // (legacy: the albedo is the base colour; full: emission unchanged); environment texels of the same size or null,
+    // rotation in degrees, sun light-sampled or not (light-type probabilities recomputed)
+    bool setMaterials(const Material* m, int n, std::string& err);
+    bool setEnvironment(const Vec4* texels, size_t count, float envRotDeg, bool sunOn, std::string& err);
 
     double vramMB = 0, tlasMs = 0;   // tlasMs: device time of the last TLAS build
     int gridBlocks = 0, numCU = 0;

@@ -22,10 +22,12 @@
 #include "device/hploc.h"
 #include "device/kernels.h"
 #include "host/bvh.h"
+#include "host/envmap.h"
 #include "host/scene_check.h"
 #include <algorithm>
 #include <type_traits>
 #include <cstdio>
+#include <cstring>
 
 namespace cwp {
 
@@ -85,7 +87,8 @@ struct Renderer::Impl {
     hipGraphExec_t graph = nullptr;
     // full integrator on two-level scenes: the frame's emitters must be set with the instances
     bool full = false;
-    int numEmInst = 0, numMats = 0;
+    int numEmInst = 0, numMats = 0, nAlbedo = 0;
+    std::vector<Material> hostMats;   // the full integrator's materials as uploaded (emission must not change)
     bool emSet = false;
     int gSpp = 0, gBase = 0, sceneGen = 0, gGen = -1;
     bool gClear = false;
@@ -172,7 +175,10 @@ bool deviceInfo(int& numCU, std::string& name, std::string& err) {
     if (n < 1) { err = "no HIP device"; return false; }
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, 0));
-    numCU = prop.multiProcessorCount; name = prop.name;
+    numCU = prop.multiProcessorCount;
+    name = prop.name;
+    if (name.empty()) name = prop.gcnArchName;   // some runtimes leave the marketing name empty
+    if (name.empty()) name = "gfx (unnamed)";
     return true;
 }
 
@@ -256,6 +262,8 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.full = S.full != 0;
     I.numEmInst = S.inst ? S.numEmInst : 0;
     I.numMats = (int)S.mats.size();
+    I.hostMats = S.mats;
+    I.nAlbedo = (int)S.albedo.size();
     if (S.full && !I.inst) {
         I.bounce = O.persistent == 2 ? I.kn.bounceRefillFull : I.kn.bounceFull; I.shade = nullptr;
         I.shadow = O.shadowRefill ? I.kn.shadowRefill : I.kn.shadow;
@@ -463,6 +471,60 @@ bool Renderer::traceHost(const float* rays, int n, bool anyHit, float* hits, std
     if (!trace(I.stageRays, n, anyHit, I.stageHits, err, inst ? I.stageInst : nullptr)) return false;
     HIPCHK(hipMemcpy(hits, I.stageHits, (size_t)n * 16, hipMemcpyDeviceToHost));
     if (inst) HIPCHK(hipMemcpy(inst, I.stageInst, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return true;
+}
+
+bool Renderer::setMaterials(const Material* m, int n, std::string& err) {
+    Impl& I = *p_;
+    if (!m || n < 1) { err = "setMaterials: no materials"; return false; }
+    if (n != (I.full ? I.numMats : I.nAlbedo)) {
+        err = "setMaterials: the scene has " + std::to_string(I.full ? I.numMats : I.nAlbedo) + " materials"; return false;
+    }
+    for (int k = 0; k < n; k++) {
+        const Material& x = m[k];
+        if (!(x.ior > 0.f) || !(x.metallic >= 0.f && x.metallic <= 1.f) || !(x.transmission >= 0.f && x.transmission <= 1.f) ||
+            !(x.roughness >= 0.f && x.roughness <= 1.f)) { err = "setMaterials: material " + std::to_string(k) + " out of range"; return false; }
+        if (I.full && memcmp(&x.emission, &I.hostMats[k].emission, sizeof(Vec3)) != 0) {
+            err = "setMaterials: material " + std::to_string(k) + " changes its emission (the light tables depend on it)"; return false;
+        }
+    }
+    std::vector<Vec4> alb(n);
+    for (int k = 0; k < n; k++) alb[k] = v4(m[k].baseColor, 0.f);
+    HIPCHK(hipStreamSynchronize(I.st));   // queued frames read the previous materials
+    HIPCHK(hipMemcpy((void*)I.S.albedo, alb.data(), (size_t)n * sizeof(Vec4), hipMemcpyHostToDevice));
+    if (I.full) {
+        HIPCHK(hipMemcpy((void*)I.S.Lt.mats, m, (size_t)n * sizeof(Material), hipMemcpyHostToDevice));
+        I.hostMats.assign(m, m + n);
+    }
+    return true;
+}
+
+bool Renderer::setEnvironment(const Vec4* texels, size_t count, float envRotDeg, bool sunOn, std::string& err) {
+    Impl& I = *p_;
+    DevLights& L = I.S.Lt;
+    if (!I.full) { err = "setEnvironment: the scene renders with the legacy integrator"; return false; }
+    EnvMap E;
+    if (texels) {
+        if (!L.hasEnv) { err = "setEnvironment: the scene has no environment map"; return false; }
+        if (count != (size_t)L.envW * L.envH) {
+            err = "setEnvironment: the environment has " + std::to_string((size_t)L.envW * L.envH) + " texels"; return false;
+        }
+        std::vector<float> rgb(count * 3);
+        for (size_t t = 0; t < count; t++) { rgb[3 * t] = texels[t].x; rgb[3 * t + 1] = texels[t].y; rgb[3 * t + 2] = texels[t].z; }
+        buildEnvMap(rgb, L.envW, L.envH, 1.f, E);   // texels and the sampling distribution built from them
+    }
+    HIPCHK(hipStreamSynchronize(I.st));   // queued frames read the previous environment
+    if (texels) {
+        HIPCHK(hipMemcpy((void*)L.envTex, E.texels.data(), E.texels.size() * sizeof(Vec4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)L.envMarg, E.marg.data(), E.marg.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void*)L.envCond, E.cond.data(), E.cond.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    L.envRot = envRotDeg * 3.14159265f / 180.f;
+    // light-type selection: equal shares of the light types present (the rule of the scene's creation)
+    const bool tri = L.nEm > 0 && L.lightMode != kLightBSDF, env = L.hasEnv && L.lightMode != kLightBSDF;
+    const int nt = (sunOn ? 1 : 0) + (tri ? 1 : 0) + (env ? 1 : 0);
+    L.pSun = sunOn && nt ? 1.f / (float)nt : 0.f; L.pTri = tri ? 1.f / (float)nt : 0.f; L.pEnv = env ? 1.f / (float)nt : 0.f;
+    I.sceneGen++;   // captured frames hold the scene record by value
     return true;
 }
 

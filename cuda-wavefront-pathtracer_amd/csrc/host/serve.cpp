@@ -149,9 +149,10 @@ bool Server::preload(std::string& err) {
 }
 
 bool Server::autotune(const std::vector<int>& variants, const std::vector<int>& stacks, const std::vector<int>& raysPerLane,
-                      std::vector<TuneTrial>& trials, std::string& err) {
+                      const std::vector<int>& compacts, std::vector<TuneTrial>& trials, std::string& err) {
     trials.clear();
-    if (variants.empty() || stacks.empty() || raysPerLane.empty()) { err = "autotune: empty search space"; return false; }
+    if (running()) { err = "autotune: the live worker is running"; return false; }
+    if (variants.empty() || stacks.empty() || raysPerLane.empty() || compacts.empty()) { err = "autotune: empty search space"; return false; }
     const bool batched = O_.batched;
     O_.batched = true;
     const BatchOptions base = O_.batch;
@@ -159,19 +160,21 @@ bool Server::autotune(const std::vector<int>& variants, const std::vector<int>& 
     best.raysPerS = -1;
     for (int v : variants)
         for (int sl : stacks)
-            for (int r : raysPerLane) {
-                BatchOptions B = base;
-                B.variant = v; B.stackLds = sl; B.raysPerLane = r;
-                resetJobs();
-                if (!reconfigure(B, err) || !preload(err) || !run(err)) { O_.batched = batched; return false; }
-                TuneTrial t;
-                t.variant = v; t.stackLds = sl; t.raysPerLane = r; t.raysPerS = stats_.raysPerS; t.wallS = stats_.wallS;
-                trials.push_back(t);
-                if (t.raysPerS > best.raysPerS) best = t;
-            }
+            for (int r : raysPerLane)
+                for (int cm : compacts) {
+                    BatchOptions B = base;
+                    B.variant = v; B.stackLds = sl; B.raysPerLane = r; B.compact = cm;
+                    resetJobs();
+                    if (!reconfigure(B, err) || !preload(err) || !run(err)) { O_.batched = batched; return false; }
+                    TuneTrial t;
+                    t.variant = v; t.stackLds = sl; t.raysPerLane = r; t.compact = cm;
+                    t.raysPerS = stats_.raysPerS; t.wallS = stats_.wallS;
+                    trials.push_back(t);
+                    if (t.raysPerS > best.raysPerS) best = t;
+                }
     O_.batched = batched;
     BatchOptions B = base;
-    B.variant = best.variant; B.stackLds = best.stackLds; B.raysPerLane = best.raysPerLane;
+    B.variant = best.variant; B.stackLds = best.stackLds; B.raysPerLane = best.raysPerLane; B.compact = best.compact;
     resetJobs();
     stats_ = ServeStats();
     return reconfigure(B, err);

@@ -14,8 +14,9 @@
 // start() runs the rolling loop on a worker thread: submit() / poll() / result() may then be called from other threads
 // while jobs are on the GPU; stop() lets the queued jobs finish and joins the worker.
 //
-// The autotuner runs the submitted jobs under each combination of BVH width, LDS stack entries and rays per lane,
-// keeps the fastest (rays per second over the whole mix) and leaves the jobs unfinished for the next run.
+// The autotuner runs the submitted jobs under each combination of BVH width, LDS stack entries, rays per lane and
+// queue-compaction granularity (per wave or per lane), keeps the fastest (rays per second over the whole mix) and
+// leaves the jobs unfinished for the next run.
 #pragma once
 #include "device/batch.h"
 #include "host/setup.h"
@@ -49,7 +50,7 @@ struct ServeJob {
 };
 
 struct TuneTrial {
-    int variant = 8, stackLds = 8, raysPerLane = 1;
+    int variant = 8, stackLds = 8, raysPerLane = 1, compact = 0;
     double raysPerS = 0, wallS = 0;
 };
 
@@ -91,7 +92,7 @@ public:
     // uploads the scenes of the unfinished jobs in arrival order while they fit the budget
     bool preload(std::string& err);
     bool autotune(const std::vector<int>& variants, const std::vector<int>& stacks, const std::vector<int>& raysPerLane,
-                  std::vector<TuneTrial>& trials, std::string& err);
+                  const std::vector<int>& compacts, std::vector<TuneTrial>& trials, std::string& err);
     // frame parameters of a job (the scene's camera and lighting at the job's resolution, bounces and seed)
     FrameParams jobParams(const ServeJob& J) const;
     // samples per pass of a job: all its samples if they fit a quarter of the path-state slots, else the largest

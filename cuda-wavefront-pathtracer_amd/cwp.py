@@ -79,7 +79,12 @@ class SceneDesc(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("nodes", ctypes.c_void_p), ("num_nodes", ctypes.c_size_t),
                 ("corners", ctypes.c_void_p), ("num_tris", ctypes.c_size_t), ("woop", ctypes.c_void_p),
                 ("normals", ctypes.c_void_p), ("albedo", ctypes.c_void_p), ("num_materials", ctypes.c_size_t),
-                ("tri_form", ctypes.c_int), ("root", ctypes.c_int)]
+                ("tri_form", ctypes.c_int), ("root", ctypes.c_int),
+                ("mats", ctypes.c_void_p), ("tri_light_pdf", ctypes.c_void_p), ("em_tri", ctypes.c_void_p),
+                ("em_cdf", ctypes.c_void_p), ("num_emitters", ctypes.c_size_t), ("env_texels", ctypes.c_void_p),
+                ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("env_rot_deg", ctypes.c_float), ("sun_on", ctypes.c_int),
+                ("light_mode", ctypes.c_int), ("rr", ctypes.c_int), ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int),
+                ("env_uniform", ctypes.c_int)]
 
 
 class Options(ctypes.Structure):
@@ -104,7 +109,7 @@ class ServerOptions(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("stack_lds", ctypes.c_int), ("bounces", ctypes.c_int),
                 ("max_paths", ctypes.c_size_t), ("max_jobs", ctypes.c_int), ("scene_budget_mb", ctypes.c_double),
                 ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("full", ctypes.c_int), ("rolling", ctypes.c_int)]
+                ("full", ctypes.c_int), ("rolling", ctypes.c_int), ("compact", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -120,7 +125,7 @@ class ServerOptions(ctypes.Structure):
 
 class TuneTrial(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("stack_lds", ctypes.c_int), ("rays_per_lane", ctypes.c_int),
-                ("rays_per_s", ctypes.c_double), ("wall_s", ctypes.c_double)]
+                ("compact", ctypes.c_int), ("rays_per_s", ctypes.c_double), ("wall_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -143,9 +148,11 @@ class ServeStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_form=0, root=0):
+def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_form=0, root=0, full=None):
     """Builds a cwp_scene_desc over numpy arrays; returns (desc, keep-alive list). tri_form 1: corners holds the
-    reference's edge form (v0, e1, e2)."""
+    reference's edge form (v0, e1, e2). full: a dict of the full integrator's fields (mats (m, 16) float32,
+    tri_light_pdf, em_tri, em_cdf, env_texels (h, w, 4), env_rot_deg, sun_on, light_mode, rr, rr_depth, gd_energy,
+    env_uniform), or None for a legacy-integrator scene."""
     keep = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
             for a in (nodes, corners, woop, normals, albedo)]
     n, c, w, nr, al = keep
@@ -153,6 +160,27 @@ def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_for
     d = SceneDesc(variant, n.ctypes.data, n.size // per, c.ctypes.data, c.size // 12,
                   w.ctypes.data if w is not None else None, nr.ctypes.data if nr is not None else None,
                   al.ctypes.data, al.size // 4, tri_form, root)
+    if full is not None:
+        def arr(key, dt):
+            a = full.get(key)
+            if a is None or len(a) == 0:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+        m = arr("mats", np.float32)
+        d.mats = m.ctypes.data
+        tlp, emt, emc = arr("tri_light_pdf", np.float32), arr("em_tri", np.int32), arr("em_cdf", np.float32)
+        if tlp is not None:
+            d.tri_light_pdf, d.em_tri, d.em_cdf, d.num_emitters = tlp.ctypes.data, emt.ctypes.data, emc.ctypes.data, len(emt)
+        env = arr("env_texels", np.float32)
+        if env is not None:
+            d.env_texels, d.env_h, d.env_w = env.ctypes.data, env.shape[0], env.shape[1]
+        d.env_rot_deg = full.get("env_rot_deg", 0.0)
+        d.sun_on = full.get("sun_on", 1)
+        d.light_mode = full.get("light_mode", 0)
+        d.rr, d.rr_depth = full.get("rr", 0), full.get("rr_depth", 3)
+        d.gd_energy, d.env_uniform = full.get("gd_energy", 1), full.get("env_uniform", 0)
     return d, keep
 
 
@@ -181,6 +209,8 @@ SIGNATURES = {
     "cwp_render": ([_P, _I, _I, _I, ctypes.POINTER(FrameStats)], _I),
     "cwp_read_accum": ([_P, ctypes.POINTER(ctypes.c_float), _S], _I),
     "cwp_set_params": ([_P, ctypes.POINTER(Params)], _I),
+    "cwp_set_materials": ([_P, ctypes.POINTER(ctypes.c_float), _I], _I),
+    "cwp_set_environment": ([_P, ctypes.POINTER(ctypes.c_float), _S, ctypes.c_float, _I], _I),
     "cwp_trace": ([_P, _P, _I, _I, _P], _I),
     "cwp_trace_host": ([_P, ctypes.POINTER(ctypes.c_float), _I, _I, ctypes.POINTER(ctypes.c_float)], _I),
     "cwp_frame_json": ([_P, _P, ctypes.POINTER(FrameStats), _I, _I, ctypes.c_char_p, _S], _I),
@@ -229,7 +259,8 @@ SIGNATURES = {
     "cwp_server_clear_jobs": ([_P], _I),
     "cwp_server_reset_jobs": ([_P], _I),
     "cwp_server_autotune": ([_P, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I, ctypes.POINTER(_I), _I,
-                             ctypes.POINTER(TuneTrial), _I, ctypes.POINTER(_I), ctypes.POINTER(ServerOptions)], _I),
+                             ctypes.POINTER(_I), _I, ctypes.POINTER(TuneTrial), _I, ctypes.POINTER(_I),
+                             ctypes.POINTER(ServerOptions)], _I),
     "cwp_server_get_options": ([_P, ctypes.POINTER(ServerOptions)], _I),
     "cwp_server_preload": ([_P], _I),
     "cwp_device_malloc": ([_S, ctypes.POINTER(_P)], _I),
@@ -375,9 +406,10 @@ class Renderer:
         self.grid_blocks, self.vram_mb = g.value, v.value
 
     @classmethod
-    def from_scene(cls, variant, nodes, corners, albedo, params, options, woop=None, normals=None, tri_form=0, root=0):
-        """Renderer over caller arrays in the upload layout (cwp_create_scene)."""
-        d, keep = scene_desc(variant, nodes, corners, albedo, woop, normals, tri_form, root)
+    def from_scene(cls, variant, nodes, corners, albedo, params, options, woop=None, normals=None, tri_form=0, root=0,
+                   full=None):
+        """Renderer over caller arrays in the upload layout (cwp_create_scene); full: see scene_desc."""
+        d, keep = scene_desc(variant, nodes, corners, albedo, woop, normals, tri_form, root, full)
         h = ctypes.c_void_p()
         _check(lib().cwp_create_scene(ctypes.byref(d), ctypes.byref(params), ctypes.byref(options), ctypes.byref(h)))
         r = cls.__new__(cls)
@@ -399,6 +431,20 @@ class Renderer:
 
     def set_params(self, params):
         _check(lib().cwp_set_params(self.h, ctypes.byref(params)))
+
+    def set_materials(self, mats):
+        """mats: (m, 16) float32 material records (the scene's count)."""
+        m = np.ascontiguousarray(mats, np.float32).reshape(-1, 16)
+        _check(lib().cwp_set_materials(self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(m)))
+
+    def set_environment(self, texels=None, rot_deg=0.0, sun_on=True):
+        """texels: (h, w, 4) float32 radiance of the scene's size, or None to keep them."""
+        if texels is None:
+            _check(lib().cwp_set_environment(self.h, None, 0, float(rot_deg), 1 if sun_on else 0))
+            return
+        t = np.ascontiguousarray(texels, np.float32).reshape(-1, 4)
+        _check(lib().cwp_set_environment(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(t), float(rot_deg),
+                                         1 if sun_on else 0))
 
     def trace_ptr(self, rays_ptr, n, any_hit, hits_ptr):
         """rays_ptr / hits_ptr: device addresses (e.g. torch.Tensor.data_ptr())."""
@@ -552,15 +598,17 @@ class Server:
         _check(lib().cwp_server_get_options(self.h, ctypes.byref(o)))
         return o.as_dict()
 
-    def autotune(self, variants=(4, 8), stacks=(8, 16, 32), rays_per_lane=(1, 2, 4)):
-        """Runs the submitted jobs under every combination, keeps the fastest; returns (trials, chosen options)."""
+    def autotune(self, variants=(4, 8), stacks=(8, 16, 32), rays_per_lane=(1, 2, 4), compacts=(0,)):
+        """Runs the submitted jobs under every combination (compacts: 0 per-wave, 1 per-lane queue compaction), keeps
+        the fastest; returns (trials, chosen options)."""
         def arr(x):
             return (ctypes.c_int * len(x))(*x)
-        n = len(variants) * len(stacks) * len(rays_per_lane)
+        n = len(variants) * len(stacks) * len(rays_per_lane) * len(compacts)
         trials = (TuneTrial * n)()
         got, chosen = ctypes.c_int(), ServerOptions()
         _check(lib().cwp_server_autotune(self.h, arr(variants), len(variants), arr(stacks), len(stacks), arr(rays_per_lane),
-                                         len(rays_per_lane), trials, n, ctypes.byref(got), ctypes.byref(chosen)))
+                                         len(rays_per_lane), arr(compacts), len(compacts), trials, n, ctypes.byref(got),
+                                         ctypes.byref(chosen)))
         self.options = chosen
         return [trials[k].as_dict() for k in range(got.value)], chosen.as_dict()
 

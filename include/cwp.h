@@ -63,8 +63,9 @@ typedef struct cwp_frame_stats {
     double t_sort_ms;                 /* continuation-sort kernels (options.sort; 0 when off or untimed) */
 } cwp_frame_stats;
 
-/* A scene in the renderer's upload layout: host arrays in BVH leaf order, copied by cwp_create_scene. Replaces the
- * legacy-integrator part of SceneBuffers (src/render/renderer.h:31-43); the layouts are those of cwp_bvh_array. */
+/* A scene in the renderer's upload layout: host arrays in BVH leaf order, copied by cwp_create_scene. Replaces
+ * SceneBuffers (src/render/renderer.h:31-43); the layouts are those of cwp_bvh_array. A zero-initialised tail (mats
+ * NULL) is a legacy-integrator scene; with mats the scene renders with the full material integrator. */
 typedef struct cwp_scene_desc {
     int variant;              /* 2: BVH2, 16 floats per node; 8: compressed BVH8, 20 floats (80 B) per node */
     const float* nodes;
@@ -78,7 +79,23 @@ typedef struct cwp_scene_desc {
     int tri_form;             /* CWP_TRIS_CORNERS, or CWP_TRIS_EDGES: the reference's (v0, id), (e1, material), (e2, 0)
                                  (src/accel/tri_pack.cpp:6-16), turned into corners v0 + e1, v0 + e2 on upload */
     int root;                 /* root node (0 for the layouts of cwp_bvh_build; n - 2 for cwp_hploc_to_bvh2) */
+    /* full material integrator (renderer.h:35-38; the arrays of CWP_ARR_MATERIALS, CWP_ARR_TRI_LIGHT_PDF,
+     * CWP_ARR_EM_TRI, CWP_ARR_EM_CDF and CWP_ARR_ENV_TEX) */
+    const float* mats;          /* 16 floats per material (the 64-byte record), num_materials of them, or NULL */
+    const float* tri_light_pdf; /* per triangle (leaf order): light-selection probability / area; NULL: no emitters */
+    const int* em_tri;          /* emissive triangles (leaf positions) in power-CDF order */
+    const float* em_cdf;        /* num_emitters + 1 entries, 0 .. 1 */
+    size_t num_emitters;
+    const float* env_texels;    /* 4 floats per texel (radiance), env_w x env_h, row 0 = +z; NULL: constant sky */
+    int env_w, env_h;
+    float env_rot_deg;          /* environment rotation about +z */
+    int sun_on;                 /* 1: the sun is light-sampled */
+    int light_mode;             /* CWP_LIGHT_MIS, CWP_LIGHT_NEE or CWP_LIGHT_BSDF */
+    int rr, rr_depth;           /* Russian roulette from depth rr_depth on */
+    int gd_energy;              /* glossy-diffuse energy compensation */
+    int env_uniform;            /* environment light samples uniform on the sphere instead of by luminance */
 } cwp_scene_desc;
+enum { CWP_LIGHT_MIS = 0, CWP_LIGHT_NEE = 1, CWP_LIGHT_BSDF = 2 };
 enum { CWP_TRIS_CORNERS = 0, CWP_TRIS_EDGES = 1 };
 
 /* Kernel and schedule options. Replaces RenderOptions (src/render/renderer.h:11-28) for the legacy integrator. */
@@ -181,6 +198,14 @@ int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats*
 int cwp_read_accum(cwp_ctx* c, float* rgba, size_t num_pixels);
 /* Replaces setParams (src/render/wavefront.cu:744); the image size must not change. */
 int cwp_set_params(cwp_ctx* c, const cwp_params* params);
+/* Runtime updates, effective from the next render (the context waits for queued frames). Replace
+ * WavefrontRenderer::setMaterials / setEnvironment (src/render/renderer.h:59-63). Materials: 16 floats each, the
+ * scene's count; the legacy integrator takes their base colours as albedo; the full integrator needs the emission of
+ * every material unchanged (the light tables depend on it). Environment (full integrator): texels of the scene's size
+ * (or NULL to keep them; the sampling distribution is rebuilt from them), the rotation about +z in degrees, and
+ * whether the sun is light-sampled (the light-type probabilities are recomputed). */
+int cwp_set_materials(cwp_ctx* c, const float* mats, int num_materials);
+int cwp_set_environment(cwp_ctx* c, const float* texels, size_t num_texels, float env_rot_deg, int sun_on);
 /* Closest (any_hit = 0) or any hit of n rays with the context's BVH and triangle test. rays: device pointer to
  * 8 floats per ray (o.xyz, tmin, d.xyz, tmax); hits: device pointer to 4 floats per ray (t, u, v, leaf position as
  * int bits, -1 = miss). Synchronous. Kernel form of trav<V, ANY, TRI> (src/accel/traverse.cuh:155). */
@@ -313,9 +338,11 @@ typedef struct cwp_server_options {
     int rolling;                       /* batched mode: 1 = admission at every step (one trace+shade and one shadow
                                           launch over all live paths of all live jobs, whatever their depth, plus the
                                           camera rays of the jobs admitted at that step); 0 = between whole passes */
+    int compact;                       /* rolling steps' queue compaction: 0 per wave (one ballot and one atomic per
+                                          wave), 1 per lane (one atomic per appended ray) */
 } cwp_server_options;
 typedef struct cwp_tune_trial {
-    int variant, stack_lds, rays_per_lane;
+    int variant, stack_lds, rays_per_lane, compact;
     double rays_per_s, wall_s;
 } cwp_tune_trial;
 typedef struct cwp_job_info {
@@ -346,11 +373,12 @@ int cwp_server_clear_jobs(cwp_server* s);
 /* marks every job unfinished again (images dropped), e.g. to run the same stream in the other mode */
 int cwp_server_reset_jobs(cwp_server* s);
 /* Autotuner: runs the submitted jobs, batched, under every combination of BVH width (variants: 2, 4, 8), LDS stack
- * entries (8, 16, 32) and rays per lane (1, 2, 4), keeps the fastest and leaves the jobs unfinished. trials receives
- * up to max_trials results (num_trials = the number run); chosen (optional) receives the server's new options. */
+ * entries (8, 16, 32), rays per lane (1, 2, 4) and queue-compaction granularity (compacts: 0 per wave, 1 per lane;
+ * null = {0}), keeps the fastest and leaves the jobs unfinished. trials receives up to max_trials results
+ * (num_trials = the number run); chosen (optional) receives the server's new options. */
 int cwp_server_autotune(cwp_server* s, const int* variants, int num_variants, const int* stacks, int num_stacks,
-                        const int* rays_per_lane, int num_rays_per_lane, cwp_tune_trial* trials, int max_trials,
-                        int* num_trials, cwp_server_options* chosen);
+                        const int* rays_per_lane, int num_rays_per_lane, const int* compacts, int num_compacts,
+                        cwp_tune_trial* trials, int max_trials, int* num_trials, cwp_server_options* chosen);
 int cwp_server_get_options(const cwp_server* s, cwp_server_options* o);
 /* Live serving. No counterpart in the reference, which renders one job per process. cwp_server_start runs the rolling
  * loop on a worker thread of the library; from then on cwp_job_submit (the arrival is the time of the call),

@@ -104,3 +104,80 @@ def test_full_plane_closed_form_on_gpu(tmp_path):
     f = (1.0 - img / np.array(sky, np.float32)) / (1.0 - a)
     np.testing.assert_allclose(f * 8, np.round(f * 8), atol=2e-3)
     assert np.isclose(f[..., 0], 1.0, atol=1e-4).sum() > 0.3 * 48 * 48
+
+
+def _full_fields(s):
+    i = s.info
+    return {"mats": s.array(cwp.ARR_MATERIALS), "tri_light_pdf": s.array(cwp.ARR_TRI_LIGHT_PDF),
+            "em_tri": s.array(cwp.ARR_EM_TRI), "em_cdf": s.array(cwp.ARR_EM_CDF),
+            "env_texels": s.array(cwp.ARR_ENV_TEX).reshape(i["env_h"], i["env_w"], 4) if i["env_w"] else None,
+            "env_rot_deg": i["env_rot_deg"], "sun_on": i["sun_on"], "light_mode": i["light_mode"], "rr": i["rr"],
+            "rr_depth": i["rr_depth"], "gd_energy": i["gd_energy"], "env_uniform": i["env_uniform"]}
+
+
+def test_full_scene_from_caller_buffers_matches_setup_renderer():
+    """cwp_create_scene with the full integrator's fields (materials, emitters, environment) over the setup's own
+    arrays renders the same image bit for bit as cwp_create."""
+    args = TERRAIN + ["--gen_materials", "3", "--gen_env", "64,32,2,1", "--batch", "8", "--bounces", "4", "--spp", "8",
+                      "--graph", "0"]
+    s = cwp.Setup(args)
+    assert s.info["full"] and s.info["num_emitters"] > 0
+    a = cwp.Renderer(s)
+    a.render(8)
+    ref = a.accum()
+    a.close()
+    opts = cwp.Options.default(batch=8, bounces=4, graph=0)
+    r = cwp.Renderer.from_scene(8, s.array(cwp.ARR_NODES), s.array(cwp.ARR_CORNERS), s.array(cwp.ARR_ALBEDO), s.params(),
+                                opts, woop=s.array(cwp.ARR_WOOP), full=_full_fields(s))
+    r.render(8)
+    np.testing.assert_array_equal(r.accum(), ref)
+    r.close()
+
+
+def test_set_materials_and_environment_equal_fresh_scenes():
+    """Runtime updates (reference: WavefrontRenderer::setMaterials / setEnvironment): a renderer whose environment is
+    replaced renders exactly what a renderer created with it renders; new materials change the image; changing an
+    emission is rejected; the legacy integrator takes the base colours as albedo (equal to a scene created with them)."""
+    base = TERRAIN + ["--batch", "8", "--bounces", "4", "--spp", "8", "--graph", "0"]
+    sa = cwp.Setup(base + ["--gen_env", "64,32,2,1"])
+    sb = cwp.Setup(base + ["--gen_env", "64,32,2,9", "--env_rot", "30"])
+    ra, rb = cwp.Renderer(sa), cwp.Renderer(sb)
+    rb.render(8)
+    want = rb.accum()
+    rb.close()
+    ib = sb.info
+    assert ib["env_rot_deg"] == pytest.approx(30.0)
+    ra.set_environment(sb.array(cwp.ARR_ENV_TEX).reshape(ib["env_h"], ib["env_w"], 4), rot_deg=ib["env_rot_deg"], sun_on=True)
+    ra.render(8)
+    np.testing.assert_array_equal(ra.accum(), want)
+    m = sa.array(cwp.ARR_MATERIALS).copy()
+    m[:, 7] = np.clip(m[:, 7] * 0.5, 0, 1)   # roughness
+    m[:, 0:3] *= 0.5                         # base colour
+    ra.set_materials(m)
+    ra.render(8)
+    assert not np.array_equal(ra.accum(), want)
+    bad = m.copy()
+    bad[:, 8] += 1.0                         # emission
+    with pytest.raises(cwp.CwpError, match="emission"):
+        ra.set_materials(bad)
+    with pytest.raises(cwp.CwpError, match="texels"):
+        ra.set_environment(np.zeros((4, 4, 4), np.float32))
+    ra.close()
+    legacy = ["--gen", "boxes", "--gen_boxes", "40", "--width", "40", "--height", "32", "--batch", "4", "--bounces", "3",
+              "--spp", "4"]
+    lg = cwp.Setup(legacy)
+    rl = cwp.Renderer(lg)
+    mats = np.zeros((lg.info["num_materials"], 16), np.float32)
+    mats[:, :3] = (0.3, 0.4, 0.5)
+    mats[:, 7] = 1.0
+    mats[:, 11] = 1.5                        # ior
+    rl.set_materials(mats)
+    rl.render(4)
+    got = rl.accum()
+    rl.close()
+    r2 = cwp.Renderer(cwp.Setup(legacy + ["--albedo", "0.3", "0.4", "0.5"]))
+    r2.render(4)
+    np.testing.assert_array_equal(got, r2.accum())
+    r2.close()
+    with pytest.raises(cwp.CwpError):
+        cwp.Renderer(lg).set_environment(None, 10.0)   # legacy integrator: no environment

@@ -119,6 +119,29 @@ def test_render_matches_oracle(scene, tri, variant):
     assert np.mean(np.all(img == ref, axis=-1)) > 0.5
 
 
+def test_thin_lens_render_matches_oracle():
+    """Thin-lens camera (concentric-disk lens sample, focus plane; reference: src/render/wavefront.cu:43-57) against the
+    oracle's camera at >= 50 dB; the lens visibly changes the image against the pinhole one."""
+    spp, batch, bounces = 16, 16, 3
+    base = SCENES["terrain"] + ["--batch", str(batch), "--bounces", str(bounces), "--spp", str(spp)]
+    s = cwp.Setup(base + ["--aperture", "0.02", "--focus_dist", "0.8"])
+    p = s.params()
+    assert p.lens_r == pytest.approx(0.02) and p.focus_d == pytest.approx(0.8)
+    r = cwp.Renderer(s)
+    st = r.render(spp)
+    img = r.accum()[..., :3] / spp
+    r.close()
+    sc, _ = po.scene_from_setup(s)
+    acc, ost = sc.render(p, spp, batch=batch, bounces=bounces)
+    assert psnr(img, acc[..., :3] / spp) >= 50.0
+    assert st["rays_closest"] == pytest.approx(ost[0], rel=1e-3)
+    pin = cwp.Renderer(cwp.Setup(base))
+    pin.render(spp)
+    pimg = pin.accum()[..., :3] / spp
+    pin.close()
+    assert psnr(img, pimg) < 40.0
+
+
 @pytest.mark.parametrize("scene,variant", [("terrain", 8), ("boxes", 2), ("boxes", 4)])
 def test_kernel_schedules_give_the_same_image(scene, variant):
     """Fused trace+shade, split trace/shade, split with persistent refilling lanes, and the shadow kernel with and

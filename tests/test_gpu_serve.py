@@ -165,9 +165,9 @@ def test_scene_of_another_layout_is_relaid():
 
 
 def test_autotune_picks_a_configuration_and_keeps_results():
-    """The autotuner runs the mix under every (width, stack, rays per lane), keeps the fastest, leaves the jobs
-    unfinished; the next run under the chosen configuration gives the same images (bit-exact for the BVH8 layout,
-    >= 50 dB for BVH4, whose closest hits can differ only among triangles at equal distance)."""
+    """The autotuner runs the mix under every (width, stack, rays per lane, compaction granularity), keeps the fastest,
+    leaves the jobs unfinished; the next run under the chosen configuration gives the same images (bit-exact for the
+    BVH8 layout, >= 50 dB for BVH4, whose closest hits can differ only among triangles at equal distance)."""
     S = setups(8)
     ref_srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 20)
     for s in S:
@@ -179,11 +179,12 @@ def test_autotune_picks_a_configuration_and_keeps_results():
         srv.add_scene(s)
     srv.clear_jobs()
     ids = [srv.submit(*j[:5], seed=j[5]) for j in JOBS]
-    trials, chosen = srv.autotune(variants=(4, 8), stacks=(8, 16), rays_per_lane=(1, 2))
-    assert len(trials) == 8 and all(t["rays_per_s"] > 0 for t in trials)
+    trials, chosen = srv.autotune(variants=(4, 8), stacks=(8, 16), rays_per_lane=(1, 2), compacts=(0, 1))
+    assert len(trials) == 16 and all(t["rays_per_s"] > 0 for t in trials)
+    assert {t["compact"] for t in trials} == {0, 1}
     best = max(trials, key=lambda t: t["rays_per_s"])
-    assert (chosen["variant"], chosen["stack_lds"], chosen["rays_per_lane"]) == \
-        (best["variant"], best["stack_lds"], best["rays_per_lane"])
+    assert (chosen["variant"], chosen["stack_lds"], chosen["rays_per_lane"], chosen["compact"]) == \
+        (best["variant"], best["stack_lds"], best["rays_per_lane"], best["compact"])
     assert all(srv.job(i)["finished_s"] < 0 for i in ids)
     srv.run(batched=True)
     for i, r in zip(ids, ref):
@@ -255,6 +256,24 @@ def test_rolling_admits_arrivals_at_the_next_step():
     assert infos[1]["finished_s"] < infos[0]["finished_s"]
     assert infos[1]["latency_s"] < 0.5 * infos[0]["latency_s"], infos
     assert st["steps"] > 0
+
+
+def test_per_lane_compaction_gives_the_same_images():
+    """Rolling steps with per-lane queue compaction (one atomic per appended ray, so another slot order) render the
+    same images bit for bit as per-wave compaction: a path's result depends on its id, never on its slot."""
+    S = setups(8)
+    out = []
+    for cm in (0, 1):
+        srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 16, compact=cm)
+        assert srv.get_options()["compact"] == cm
+        for s in S:
+            srv.add_scene(s)
+        out.append(run_jobs(srv, JOBS, True)[0])
+        srv.close()
+    for k in range(len(JOBS)):
+        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg="job %d" % k)
+    with pytest.raises(cwp.CwpError, match="compaction"):
+        cwp.Server(variant=8, bounces=4, max_paths=1 << 16, compact=2)
 
 
 def test_live_submissions_from_threads_while_jobs_run():
