@@ -37,12 +37,13 @@ def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per
 def algorithmic_bytes(st, npaths, sort=0):
     """Compulsory HBM bytes of one frame's trace+shade (bounce) kernels, from the frame's ray counters.
 
-    Per camera ray: 16 B radiance write. Per continuation ray traced: 4 B queue read + 48 B origin/direction/throughput
-    read. Per continuation-ray miss: 32 B radiance read-modify-write. Per continuation produced: 4 B queue write + 48 B
-    path-state write (+ 1 B direction-bucket key with --sort 1). Per shadow ray produced: 48 B shadow record write."""
+    Per camera ray: 16 B radiance write. Per continuation ray traced: 48 B origin/direction/throughput read at its
+    queue slot (the path id rides in the origin's w). Per continuation-ray miss: 32 B radiance read-modify-write. Per
+    continuation produced: 48 B ray write at its slot (+ 1 B direction-bucket key with --sort 1). Per shadow ray
+    produced: 48 B shadow record write."""
     cont_traced = st["rays_closest"] - npaths
     cont_miss = st["rays_miss"] - st["camera_miss"]
-    return (16.0 * npaths + 52.0 * cont_traced + 32.0 * cont_miss + (53.0 if sort else 52.0) * st["rays_cont"] +
+    return (16.0 * npaths + 48.0 * cont_traced + 32.0 * cont_miss + (49.0 if sort else 48.0) * st["rays_cont"] +
             48.0 * st["rays_shadow"])
 
 
@@ -140,13 +141,14 @@ def calibrate_copy(mib):
 
 def bench_instances(args, frames=None, quiet=False):
     """Two-level scene: 5 meshes (1.3k-200k triangles) + ground, 1,000 instances with rigid transforms redrawn every
-    frame; per frame the TLAS is rebuilt on the GPU (H-PLOC over the instance boxes) and the frame rendered at
-    1280x720, 64 spp in flight. Reports Grays/s over the frames and the TLAS build time per frame."""
+    frame; per frame the TLAS is rebuilt on the GPU (H-PLOC over the instance boxes, then for BVH8 BLASes the GPU
+    collapse to a compressed BVH8 unless --tlas bvh2) and the frame rendered at 1280x720, 64 spp in flight. Reports
+    Grays/s over the frames and the TLAS build time per frame."""
     import cwp
     spp = 64
     s = cwp.Setup(["--gen", "instances", "--width", "1280", "--height", "720", "--batch", str(spp), "--spp", str(spp),
                    "--bounces", str(args.bounces), "--variant", args.variant, "--tri", args.tri, "--filter", "box",
-                   "--stack", str(args.stack), "--morton_bits", str(args.tlas_morton_bits)] + LIGHTING)
+                   "--stack", str(args.stack), "--morton_bits", str(args.tlas_morton_bits), "--tlas", args.tlas] + LIGHTING)
     r = cwp.Renderer(s)
     info = s.info
     print("bench: %d meshes, %d instances, %d BLAS nodes" % (info["num_meshes"], info["num_instances"], info["num_nodes"]),
@@ -167,7 +169,7 @@ def bench_instances(args, frames=None, quiet=False):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "instances-1000-1280x720-64spp-depth%d" % args.bounces, "meshes": info["num_meshes"],
                       "instances": info["num_instances"], "bvh": args.variant, "tri": args.tri, "frames": frames,
-                      "tlas_morton_bits": args.tlas_morton_bits},
+                      "tlas_morton_bits": args.tlas_morton_bits, "tlas": args.tlas if args.variant == "bvh8" else "bvh2"},
            "tlas_ms_mean": round(float(np.mean(tlas)), 4), "tlas_ms_p50": round(float(np.percentile(tlas, 50)), 4),
            "tlas_ms_max": round(float(np.max(tlas)), 4)}
     r.close()
@@ -436,6 +438,8 @@ def main():
     ap.add_argument("--frames", type=int, default=60, help="instances workload: animation frames")
     ap.add_argument("--tlas-morton-bits", type=int, default=30, help="two-level scene: Morton keys of the per-frame TLAS "
                     "(30: 10 bits per axis, as the two-level configuration asks; 0: the reference's adaptive keys)")
+    ap.add_argument("--tlas", default="bvh8", choices=["bvh2", "bvh8"],
+                    help="two-level scene with BVH8 BLASes: the per-frame TLAS as a BVH2 or a compressed BVH8")
     ap.add_argument("--hploc-morton-bits", type=int, default=0, help="hploc workload: Morton keys (0 adaptive, 30)")
     ap.add_argument("--cpu-threads", type=int, default=16, help="CPU baseline threads (the host cores of one GPU)")
     ap.add_argument("--no-extras", action="store_true", help="terrain workload: only the headline frame, the reference "

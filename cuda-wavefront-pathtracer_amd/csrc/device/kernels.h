@@ -105,9 +105,13 @@ struct DevScene {
     const Vec4* nrm;      // per-corner normals or null
     const Vec4* albedo;   // per material
     int root;
-    // two-level scenes: TLAS (BVH2 over instance ids) and its root, instance records; nodes / tri hold every BLAS
+    // two-level scenes: TLAS over instance ids and its root, instance records; nodes / tri hold every BLAS. The TLAS is
+    // a BVH2 (64-byte nodes, leaves list instance ids) or, with BVH8 BLASes, a compressed BVH8 (128-byte device nodes,
+    // root 0) whose leaf primitives k name instance tlasPrim[k]
     const Vec4* tlas;
     int tlasRoot;
+    int tlasWide;
+    const int* tlasPrim;
     const InstanceRecord* inst;
     DevLights Lt;         // full integrator
 };
@@ -197,11 +201,29 @@ __device__ __forceinline__ void enterInstance(const DevScene& S, int k, Vec3 o, 
     if (h.t < before) h.inst = k;
 }
 
+// Two-level traversal over a compressed BVH8 TLAS: the BLAS node step with instances as leaf primitives, each hit
+// leaf's instances entered in slot order; the TLAS's groups share the lane stack with the BLAS traversal above them.
+// The closest hit is the BVH2 TLAS's (both visit every instance whose box the ray enters before the hit).
+template <bool ANY, int TRI, class St>
+__device__ __forceinline__ bool traceInstWide(const DevScene& S, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    const RayPre r = prepareRay<kTriMT>(o, d, tmin);
+    const int* __restrict__ prim = S.tlasPrim;
+    TravState ts; travBegin(ts, 0);
+    while (!stepBVH8With<ANY>(S.tlas, r, h, st, ts, [&](int k) {
+        enterInstance<8, ANY, TRI>(S, prim[k], o, d, tmin, h, st);
+        return h.tri >= 0;
+    })) {}
+    return h.tri >= 0;
+}
+
 // Two-level traversal (reference: travInst, src/accel/traverse_inst.cuh:10-41): the TLAS is a BVH2 whose leaves list
-// instance ids; the TLAS's stack entries share the lane stack with the BLAS traversal above them.
+// instance ids (or, for BVH8 BLASes, a compressed BVH8: traceInstWide); the TLAS's stack entries share the lane stack
+// with the BLAS traversal above them.
 template <int V, bool ANY, int TRI, class St>
 __device__ __forceinline__ bool traceInst(const DevScene& S, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
     using E = typename StackEntry<V>::type;
+    if constexpr (V == 8)
+        if (S.tlasWide) return traceInstWide<ANY, TRI>(S, o, d, tmin, h, st);
     const RayPre r = prepareRay<kTriMT>(o, d, tmin);
     const float odx = r.o.x * r.ix, ody = r.o.y * r.iy, odz = r.o.z * r.iz;
     int node = S.tlasRoot;

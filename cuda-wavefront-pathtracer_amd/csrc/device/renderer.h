@@ -34,6 +34,7 @@ struct RenderOptions {
     int graph = 0;          // 1: a frame is captured once into a HIP graph and replayed (per-kernel timing off)
     int sort = 0;           // 1: between bounces, group each partition's continuations by direction bucket
     int mortonBits = 0;     // two-level scenes: Morton keys of the per-frame TLAS build (0 adaptive, 30: 10 bits per axis)
+    int tlasWidth = 8;      // two-level scenes with BVH8 BLASes: the per-frame TLAS as a BVH2 (2) or a compressed BVH8 (8)
 };
 
 struct FrameStats {
@@ -108,6 +109,7 @@ public:
     bool setEnvironment(const Vec4* texels, size_t count, float envRotDeg, bool sunOn, std::string& err);
 
     double vramMB = 0, tlasMs = 0;   // tlasMs: device time of the last TLAS build
+    int tlasWide = 0;                // two-level scenes: 1 when the TLAS is a compressed BVH8
     int gridBlocks = 0, numCU = 0;
     std::string gpuName;
     const RenderOptions& options() const;

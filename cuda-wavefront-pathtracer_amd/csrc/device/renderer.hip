@@ -19,6 +19,7 @@
 // geometric (and optional interpolated) normal faced to the ray, origin offset eps, sun next-event estimation by a
 // uniform cone sample with contribution T*albedo*sunE*cos/pi, cosine-weighted continuation rejected below the
 // geometric surface, throughput *= albedo.
+#include "device/collapse.h"
 #include "device/hploc.h"
 #include "device/kernels.h"
 #include "host/bvh.h"
@@ -63,6 +64,22 @@ __global__ void k_tlas_parents(const Vec4* __restrict__ hn, int n, int* parent) 
     if (a >= 0 && a < 2 * n - 1) parent[a] = id;
     if (b >= 0 && b < 2 * n - 1) parent[b] = id;
 }
+// 80-byte compressed BVH8 nodes to the device form (deviceBVH8 on the host): one 128-byte line per node, the
+// leaf-slot mask (meta bit 7 of each slot) in the top byte of the child base
+__global__ void k_bvh8_device(const Vec4* __restrict__ n80, int nn, Vec4* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nn) return;
+    const Vec4* n = n80 + (size_t)j * 5;
+    Vec4* o = out + (size_t)j * kBvh8Stride;
+    const Vec4 f1 = n[1];
+    const uint32_t meta[2] = { asUint(f1.z), asUint(f1.w) };
+    uint32_t leaf = 0;
+    for (int s = 0; s < 8; s++) leaf |= ((meta[s >> 2] >> (8 * (s & 3) + 7)) & 1u) << s;
+    o[0] = n[0];
+    o[1] = v4(asFloatU(((uint32_t)asInt(f1.x) & 0xFFFFFFu) | (leaf << 24)), f1.y, f1.z, f1.w);
+    o[2] = n[2]; o[3] = n[3]; o[4] = n[4];
+}
+
 __global__ void k_tlas_depth(const int* __restrict__ parent, int n, int* depth) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -115,6 +132,11 @@ struct Renderer::Impl {
     Vec4 *dLo = nullptr, *dHi = nullptr, *dMeshBox = nullptr, *dTlas = nullptr;
     int *dParent = nullptr, *dDepth = nullptr, *hostDepth = nullptr;
     HplocBuilder tlasB;
+    // compressed BVH8 TLAS (BVH8 BLASes): the collapse's 80-byte nodes, their 128-byte device form, leaf instance ids
+    bool tlasWide = false;
+    CwbvhCollapser tlasC;
+    Vec4 *dTlas80 = nullptr, *dTlas8 = nullptr;
+    int* dTlasPrim = nullptr;
     hipEvent_t tl0 = nullptr, tl1 = nullptr;
 
     template <class T> bool alloc(T*& p, size_t count, std::string& err) {
@@ -155,7 +177,8 @@ Renderer::~Renderer() {
     if (p_->stageRays) hipFree(p_->stageRays);
     if (p_->stageHits) hipFree(p_->stageHits);
     if (p_->stageInst) hipFree(p_->stageInst);
-    for (void* a : { (void*)p_->dInst, (void*)p_->dLo, (void*)p_->dHi, (void*)p_->dTlas, (void*)p_->dParent })
+    for (void* a : { (void*)p_->dInst, (void*)p_->dLo, (void*)p_->dHi, (void*)p_->dTlas, (void*)p_->dParent,
+                     (void*)p_->dTlas80, (void*)p_->dTlas8, (void*)p_->dTlasPrim })
         if (a) hipFree(a);
     if (p_->hostDepth) hipHostFree(p_->hostDepth);
     if (p_->tl0) hipEventDestroy(p_->tl0);
@@ -277,6 +300,9 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         if (!I.upload(mb, S.meshBox, err)) return false;
         I.dMeshBox = mb;
         HIPCHK(hipEventCreate(&I.tl0)); HIPCHK(hipEventCreate(&I.tl1));
+        if (O.tlasWidth != 2 && O.tlasWidth != 8) { err = "TLAS width must be 2 or 8"; return false; }
+        I.tlasWide = O.variant == 8 && O.tlasWidth == 8;
+        tlasWide = I.tlasWide ? 1 : 0;
         HIPCHK(hipHostMalloc((void**)&I.hostDepth, 2 * sizeof(int), hipHostMallocDefault));   // TLAS depth, build status
         if (!I.alloc(I.dDepth, 1, err)) return false;
     } else {
@@ -555,6 +581,7 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
     Impl& I = *p_;
     if (!I.inst) { err = "setInstances: the scene is not two-level"; return false; }
     if (n < 1 || !rec) { err = "setInstances: no instances"; return false; }
+    if (I.tlasWide && n > (1 << 24)) { err = "setInstances: more than 2^24 instances for a BVH8 TLAS"; return false; }
     // the kernels trust the records: every instance must name a mesh and that mesh's BLAS root
     for (int i = 0; i < n; i++)
         if (rec[i].mesh < 0 || rec[i].mesh >= I.nMesh || rec[i].root != I.meshRoot[rec[i].mesh]) {
@@ -566,27 +593,40 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
     I.instSet = false;
     I.sceneGen++;
     if (n > I.instCap) {
-        for (void* a : { (void*)I.dInst, (void*)I.dLo, (void*)I.dHi, (void*)I.dTlas, (void*)I.dParent }) if (a) hipFree(a);
-        I.dInst = nullptr; I.dLo = I.dHi = I.dTlas = nullptr; I.dParent = nullptr; I.instCap = 0;
+        for (void* a : { (void*)I.dInst, (void*)I.dLo, (void*)I.dHi, (void*)I.dTlas, (void*)I.dParent, (void*)I.dTlas80,
+                         (void*)I.dTlas8, (void*)I.dTlasPrim }) if (a) hipFree(a);
+        I.dInst = nullptr; I.dLo = I.dHi = I.dTlas = I.dTlas80 = I.dTlas8 = nullptr; I.dParent = I.dTlasPrim = nullptr;
+        I.instCap = 0;
         HIPCHK(hipMalloc((void**)&I.dInst, (size_t)n * sizeof(InstanceRecord)));
         HIPCHK(hipMalloc((void**)&I.dLo, (size_t)n * 16));
         HIPCHK(hipMalloc((void**)&I.dHi, (size_t)n * 16));
         HIPCHK(hipMalloc((void**)&I.dTlas, (size_t)std::max(n - 1, 1) * 64));
         HIPCHK(hipMalloc((void**)&I.dParent, (size_t)(2 * n - 1) * 4));
+        if (I.tlasWide) {   // a wide tree over n primitives has at most n nodes
+            HIPCHK(hipMalloc((void**)&I.dTlas80, (size_t)n * 5 * sizeof(Vec4)));
+            HIPCHK(hipMalloc((void**)&I.dTlas8, (size_t)n * kBvh8Stride * sizeof(Vec4)));
+            HIPCHK(hipMalloc((void**)&I.dTlasPrim, (size_t)n * sizeof(int)));
+        }
         if (!I.tlasB.reserve(n, err)) return false;
         I.instCap = n;
     }
     HIPCHK(hipMemcpyAsync(I.dInst, rec, (size_t)n * sizeof(InstanceRecord), hipMemcpyHostToDevice, I.st));
     HIPCHK(hipEventRecord(I.tl0, I.st));
     k_inst_boxes<<<(n + 255) / 256, 256, 0, I.st>>>(I.dInst, I.dMeshBox, n, I.dLo, I.dHi);
-    int hroot = 0;
+    int hroot = 0, wideNodes = 0, wideDepth = 0;
     if (!I.tlasB.build(I.dLo, I.dHi, n, I.st, hroot, err, I.O.mortonBits)) return false;
-    if (!hplocToBVH2(I.tlasB.nodes(), n, I.dTlas, I.st, err)) return false;
+    if (I.tlasWide) {
+        // the collapse checks the H-PLOC tree itself and synchronises with the stream between its levels
+        if (!I.tlasC.run(I.tlasB.nodes(), n, I.dTlas80, I.dTlasPrim, I.st, wideNodes, wideDepth, err)) return false;
+        k_bvh8_device<<<(wideNodes + 255) / 256, 256, 0, I.st>>>(I.dTlas80, wideNodes, I.dTlas8);
+    } else if (!hplocToBVH2(I.tlasB.nodes(), n, I.dTlas, I.st, err)) return false;
     HIPCHK(hipEventRecord(I.tl1, I.st));
     if (!I.fill(I.dDepth, 4, err)) return false;
-    k_fill<<<std::max(1, std::min((2 * n - 1 + 255) / 256, 4096)), 256, 0, I.st>>>((uint32_t*)I.dParent, (size_t)(2 * n - 1), 0xffffffffu);
-    if (n > 1) k_tlas_parents<<<(n - 1 + 255) / 256, 256, 0, I.st>>>(I.tlasB.nodes(), n, I.dParent);
-    k_tlas_depth<<<(n + 255) / 256, 256, 0, I.st>>>(I.dParent, n, I.dDepth);
+    if (!I.tlasWide) {
+        k_fill<<<std::max(1, std::min((2 * n - 1 + 255) / 256, 4096)), 256, 0, I.st>>>((uint32_t*)I.dParent, (size_t)(2 * n - 1), 0xffffffffu);
+        if (n > 1) k_tlas_parents<<<(n - 1 + 255) / 256, 256, 0, I.st>>>(I.tlasB.nodes(), n, I.dParent);
+        k_tlas_depth<<<(n + 255) / 256, 256, 0, I.st>>>(I.dParent, n, I.dDepth);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(I.hostDepth, I.dDepth, 4, hipMemcpyDeviceToHost, I.st));
     HIPCHK(hipMemcpyAsync(I.hostDepth + 1, I.tlasB.status(), 4, hipMemcpyDeviceToHost, I.st));
@@ -599,12 +639,15 @@ bool Renderer::setInstances(const InstanceRecord* rec, int n, double& tlasMs, st
     HIPCHK(hipEventElapsedTime(&ms, I.tl0, I.tl1));
     tlasMs = ms;
     this->tlasMs = ms;
+    if (I.tlasWide) *I.hostDepth = wideDepth;   // one stacked group per wide level at most
     if (*I.hostDepth + 1 > kTlasMaxDepth) {
         I.instSet = false;
         err = "setInstances: TLAS depth " + std::to_string(*I.hostDepth) + " exceeds the traversal stack";
         return false;
     }
-    I.S.tlas = I.dTlas; I.S.tlasRoot = hplocBVH2Root(n); I.S.inst = I.dInst;
+    if (I.tlasWide) { I.S.tlas = I.dTlas8; I.S.tlasRoot = 0; I.S.tlasWide = 1; I.S.tlasPrim = I.dTlasPrim; }
+    else { I.S.tlas = I.dTlas; I.S.tlasRoot = hplocBVH2Root(n); I.S.tlasWide = 0; I.S.tlasPrim = nullptr; }
+    I.S.inst = I.dInst;
     I.sceneGen++;   // captured frames hold the scene record by value
     I.nInst = n; I.instSet = true;
     return true;

@@ -269,10 +269,12 @@ __device__ __forceinline__ bool traceBVH4(const Vec4* __restrict__ nodes, const 
 __device__ __forceinline__ uint32_t byteOf(uint32_t w, int s) { return (w >> (8 * (s & 3))) & 255u; }
 
 // One node visit: the node's 8 child slabs, its hit leaves, then the choice of the next node. Returns true when the
-// traversal is over (nothing left to visit, or for ANY a hit was found).
-template <bool ANY, int TRI, class Stack>
-__device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, const RayPre& r,
-                                         Hit& h, Stack& st, TravState& ts) {
+// traversal is over (nothing left to visit, or for ANY a hit was found). leaf(k) handles leaf primitive k (the node's
+// primitive base + the offset in its slot's metadata) and returns true to end an any-hit traversal: a triangle test
+// for a BLAS, an instance's BLAS traversal for a TLAS.
+template <bool ANY, class Stack, class Leaf>
+__device__ __forceinline__ bool stepBVH8With(const Vec4* __restrict__ nodes, const RayPre& r, Hit& h, Stack& st,
+                                             TravState& ts, Leaf&& leaf) {
     const Vec4* nb = nodes + (size_t)ts.node * kBvh8Stride;
     const Vec4 f0 = ldg4(nb), f1 = ldg4(nb + 1), f2 = ldg4(nb + 2), f3 = ldg4(nb + 3), f4 = ldg4(nb + 4);
     const uint32_t ew = asUint(f0.w);
@@ -321,7 +323,7 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
     while (tm) {
         const int j = __builtin_ctz(tm);
         tm &= tm - 1;
-        if (triTest<TRI>(tri, triBase + j, r, h) && ANY) return true;
+        if (leaf(triBase + j) && ANY) return true;
     }
     // internal hits in octant order: bit k = slot k ^ octant (permute by the three octant bits)
     uint32_t hits = hitSlots & imask;
@@ -343,6 +345,12 @@ __device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const V
     const uint32_t slot = k ^ r.oct;
     ts.node = (int)ts.gBase + __builtin_popcount(ts.gMask & ((1u << slot) - 1u));
     return false;
+}
+
+template <bool ANY, int TRI, class Stack>
+__device__ __forceinline__ bool stepBVH8(const Vec4* __restrict__ nodes, const Vec4* __restrict__ tri, const RayPre& r,
+                                         Hit& h, Stack& st, TravState& ts) {
+    return stepBVH8With<ANY>(nodes, r, h, st, ts, [&](int t) { return triTest<TRI>(tri, t, r, h); });
 }
 
 template <bool ANY, int TRI, class Stack>

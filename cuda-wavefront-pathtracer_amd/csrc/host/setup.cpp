@@ -99,6 +99,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--bvh_cache") ok = str(A.bvhCache);
         else if (k == "--gpu_collapse") ok = inum(A.gpuCollapse);
         else if (k == "--morton_bits") ok = inum(A.mortonBits);
+        else if (k == "--tlas") ok = str(A.tlas);
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--sort") ok = inum(A.sort);
         else if (k == "--devloop") {
@@ -115,6 +116,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.rrDepth < 0) { err = "--rr_depth must be >= 0"; return false; }
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
     if (A.mortonBits != 0 && A.mortonBits != 30) { err = "--morton_bits must be 0 (adaptive) or 30"; return false; }
+    if (A.tlas != "bvh2" && A.tlas != "bvh8") { err = "--tlas must be bvh2 or bvh8"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
     if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
     if (A.persistent < 0 || A.persistent > 2) {
@@ -546,6 +548,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
     O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.mortonBits = A.mortonBits;
+    O.tlasWidth = A.tlas == "bvh2" ? 2 : 8;
     O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;
 }
@@ -563,7 +566,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"avg_path_len\":%.3f,\"vram_MB\":%.0f,\"pt_blocks\":%d,\"gpu\":\"%s\",\"filter\":\"%s\",\"smooth\":%d,"
              "\"seed\":%u,\"integrator\":\"%s\",\"lens_r\":%.4f,\"focus_d\":%.4f,\"env_sampling\":\"%s\",\"textures\":0,"
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":%d,"
-             "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"bins\":%d,\"stack_lds\":%d,"
+             "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"tlas\":\"%s\",\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f,"
              "\"bvh_cache\":\"%s\",\"t_sort_ms\":%.3f}",
              rep, Z.variant, A.tri.c_str(), A.persistent, A.sort, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
@@ -572,7 +575,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              st.rays / ((double)N * A.spp), R.vramMB, R.gridBlocks, R.gpuName.c_str(), A.filterBH ? "blackman-harris1.5" : "box",
              Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
              Z.treeMs, Z.postMs, Z.sah, R.options().graph, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
-             A.flatten, frame, Z.inst ? R.tlasMs : 0.0, A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
+             A.flatten, frame, Z.inst ? R.tlasMs : 0.0, Z.inst ? (R.tlasWide ? "bvh8" : "bvh2") : "-", A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
              st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"), st.sortMs);
     return buf;
 }

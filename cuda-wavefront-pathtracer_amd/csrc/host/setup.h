@@ -36,6 +36,7 @@ struct Args {
     std::string bvhCache;            // directory of built-scene cache files (empty: no cache)
     int gpuCollapse = 1;             // --builder hploc with bvh8: the CWBVH collapse on the GPU (bit-identical to the host's)
     int mortonBits = 0;              // H-PLOC keys (--builder hploc BLAS builds, two-level scenes' TLAS): 0 adaptive, 30
+    std::string tlas = "bvh8";       // two-level scenes with BVH8 BLASes: per-frame TLAS as bvh2 or compressed bvh8
     std::string envOut;              // write the environment map in use (e.g. a generated one) as a Radiance HDR file
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;

@@ -1,5 +1,6 @@
-"""GPU parity tests of two-level scenes (TLAS over instances rebuilt by the GPU H-PLOC builder each frame, BLAS per
-mesh) against the oracle's two-level traversal and renderer, and against the flattened scene."""
+"""GPU parity tests of two-level scenes (TLAS over instances rebuilt by the GPU H-PLOC builder each frame, as a BVH2 or,
+over BVH8 BLASes, collapsed on the GPU into a compressed BVH8; BLAS per mesh) against the oracle's two-level traversal
+and renderer (BVH2 TLAS, as the reference's travInst), and against the flattened scene."""
 import math
 
 import numpy as np
@@ -43,10 +44,12 @@ def scene_rays(s, n, seed):
     return np.concatenate([cam, rnd])
 
 
-@pytest.mark.parametrize("variant,tri", [("bvh8", "woop"), ("bvh2", "watertight"), ("bvh4", "mt")])
-def test_two_level_trace_matches_oracle(variant, tri):
-    print("case", variant, tri, flush=True)
-    s = cwp.Setup(ARGS + ["--variant", variant, "--tri", tri])
+@pytest.mark.parametrize("variant,tri,tlas", [("bvh8", "woop", "bvh8"), ("bvh8", "woop", "bvh2"),
+                                              ("bvh8", "watertight", "bvh8"), ("bvh2", "watertight", "bvh2"),
+                                              ("bvh4", "mt", "bvh2")])
+def test_two_level_trace_matches_oracle(variant, tri, tlas):
+    print("case", variant, tri, tlas, flush=True)
+    s = cwp.Setup(ARGS + ["--variant", variant, "--tri", tri, "--tlas", tlas])
     r = cwp.Renderer(s)
     sc = oracle_scene(s, 0, {"mt": 0, "woop": 1, "watertight": 2}[tri])
     rays = scene_rays(s, 20000, 3)
@@ -78,13 +81,14 @@ def test_tlas_rebuilt_per_frame_matches_oracle():
         r.set_instances(bad)
 
 
-@pytest.mark.parametrize("morton", [0, 30])
-def test_full_size_two_level_scene_matches_oracle(morton):
+@pytest.mark.parametrize("morton,tlas", [(0, "bvh8"), (30, "bvh8"), (30, "bvh2")])
+def test_full_size_two_level_scene_matches_oracle(morton, tlas):
     """The two-level configuration at full size: 5 meshes of 1.3k-200k triangles and a ground, 1,001 instances with
-    their transforms redrawn per frame, the TLAS rebuilt on the GPU with adaptive or 30-bit Morton keys; closest-hit
-    triangle and instance ids of 16,000 rays against the oracle's two-level traversal at 3 frames."""
-    print("case full-size two-level morton=%d" % morton, flush=True)
-    s = cwp.Setup(["--gen", "instances", "--width", "96", "--height", "64", "--morton_bits", str(morton)])
+    their transforms redrawn per frame, the TLAS rebuilt on the GPU with adaptive or 30-bit Morton keys (as a BVH2 or
+    a compressed BVH8); closest-hit triangle and instance ids of 16,000 rays against the oracle's two-level traversal
+    at 3 frames."""
+    print("case full-size two-level morton=%d tlas=%s" % (morton, tlas), flush=True)
+    s = cwp.Setup(["--gen", "instances", "--width", "96", "--height", "64", "--morton_bits", str(morton), "--tlas", tlas])
     assert s.info["num_instances"] >= 1000
     r = cwp.Renderer(s)
     rays = scene_rays(s, 8000, 11)
@@ -109,7 +113,7 @@ def test_two_level_render_matches_oracle_and_flattened():
     st = r.render(spp)
     img = r.accum()[..., :3] / spp
     j = r.frame_json(st)
-    assert j["instances"] == s.info["num_instances"] and j["tlas_ms"] > 0
+    assert j["instances"] == s.info["num_instances"] and j["tlas_ms"] > 0 and j["tlas"] == "bvh8"
     acc, ost = oracle_scene(s, 0).render(s.params(), spp, batch=batch, bounces=bounces)
     assert psnr(img, acc[..., :3] / spp) >= 50.0
     assert st["rays_closest"] == pytest.approx(ost[0], rel=1e-3)
@@ -118,6 +122,33 @@ def test_two_level_render_matches_oracle_and_flattened():
     rf = cwp.Renderer(f)
     rf.render(spp)
     assert psnr(img, rf.accum()[..., :3] / spp) >= 45.0
+    # a BVH2 TLAS finds the same closest hits (only exact ties between instances may resolve differently)
+    b = cwp.Renderer(cwp.Setup(ARGS + extra + ["--tlas", "bvh2"]))
+    st2 = b.render(spp)
+    assert b.frame_json(st2)["tlas"] == "bvh2"
+    assert psnr(img, b.accum()[..., :3] / spp) >= 60.0
+    assert st2["rays_closest"] == pytest.approx(st["rays_closest"], rel=1e-4)
+
+
+@pytest.mark.parametrize("count", [1, 2, 3, 9])
+def test_wide_tlas_over_few_instances(count):
+    """Compressed BVH8 TLAS over 1-9 instances (a single leaf as the root, a root with a few leaf slots): trace ids
+    against the oracle"""
+    print("case wide TLAS over %d instances" % count, flush=True)
+    s = cwp.Setup(ARGS)
+    r = cwp.Renderer(s)
+    inst = s.instances(0)[:count].copy()
+    r.set_instances(inst)
+    rays = scene_rays(s, 4000, 13)
+    g, gi = r.trace_inst(rays)
+    sc = po.Scene(s.info["variant"], 1, s.array(cwp.ARR_NODES), s.array(cwp.ARR_WOOP), s.array(cwp.ARR_CORNERS),
+                  s.array(cwp.ARR_ALBEDO))
+    sc.set_instances(inst, s.array(cwp.ARR_MESH_BOXES))
+    o, oi = sc.trace_inst(rays)
+    gt, ot = g[:, 3].view(np.int32), o[:, 3].view(np.int32)
+    assert (ot >= 0).any()
+    assert ((gt == ot) & (gi == oi)).mean() >= 0.9999
+    r.close()
 
 
 FULL = ARGS + ["--gen_materials", "2", "--gen_env", "64,32,1.5,2", "--bounces", "3"]

@@ -51,7 +51,9 @@ def test_version_and_error_string():
         cwp.Setup(["--gen", "boxes", "--devloop", "1"])
     with pytest.raises(cwp.CwpError, match="--morton_bits"):
         cwp.Setup(["--gen", "boxes", "--morton_bits", "12"])
-    cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0", "--morton_bits", "30"])
+    with pytest.raises(cwp.CwpError, match="--tlas"):
+        cwp.Setup(["--gen", "boxes", "--tlas", "bvh4"])
+    cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0", "--morton_bits", "30", "--tlas", "bvh2"])
     assert cwp.Options.default().sort == 0
 
 
