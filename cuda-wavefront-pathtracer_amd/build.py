@@ -20,7 +20,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcwp.so")
 CLI = os.path.join(PKG, "pt")
 
-HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/bvh.cpp", "host/scene_check.cpp", "host/instances.cpp", "host/envmap.cpp", "host/bvhcache.cpp", "host/setup.cpp", "host/serve.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/texture.cpp", "host/bvh.cpp", "host/scene_check.cpp", "host/instances.cpp", "host/envmap.cpp", "host/bvhcache.cpp", "host/setup.cpp", "host/serve.cpp", "capi.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/common.hip", "device/kernels_v2.hip", "device/kernels_v4.hip", "device/kernels_v8.hip",
                "device/hploc.hip", "device/collapse.hip", "device/batch.hip"]
 CLI_SOURCES = ["app/pt.cpp"]

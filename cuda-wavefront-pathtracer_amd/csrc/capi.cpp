@@ -9,6 +9,7 @@
 #include "device/hploc.h"
 #include "host/serve.h"
 #include "host/setup.h"
+#include "host/texture.h"
 #include <cstddef>
 #include <cstring>
 #include <new>
@@ -88,6 +89,8 @@ int cwp_setup_info(const cwp_setup* s, cwp_scene_info* info) {
     info->env_w = S.env ? S.env->w : 0; info->env_h = S.env ? S.env->h : 0; info->sun_on = S.sunOn;
     info->light_mode = S.lightMode; info->rr = S.rr; info->rr_depth = S.rrDepth; info->gd_energy = S.gdEnergy;
     info->env_uniform = S.envUniform; info->env_rot_deg = S.envRotDeg; info->bvh_cache = Z.bvhCacheHit;
+    info->textured = S.maps.empty() ? 0 : 1; info->num_images = (int)S.texImages.size();
+    info->tex_nearest = S.texNearest; info->bump_shadow = S.bumpShadow;
     return CWP_OK;
 }
 
@@ -123,6 +126,13 @@ int cwp_setup_array(const cwp_setup* s, int which, const void** data, size_t* co
         case CWP_ARR_ENV_MARG: *data = Z.S.env ? (const void*)Z.S.env->marg.data() : nullptr; *count = Z.S.env ? Z.S.env->marg.size() : 0; break;
         case CWP_ARR_MESH_TRIS: *data = Z.S.meshTris.data(); *count = Z.S.meshTris.size() / 2; break;
         case CWP_ARR_ENV_COND: *data = Z.S.env ? (const void*)Z.S.env->cond.data() : nullptr; *count = Z.S.env ? Z.S.env->cond.size() : 0; break;
+        case CWP_ARR_UVS: *data = Z.S.uvs.data(); *count = Z.S.uvs.size() / 2; break;
+        case CWP_ARR_TANGENTS: *data = Z.S.tans.data(); *count = Z.S.tans.size() / 3; break;
+        case CWP_ARR_MAPS: *data = Z.S.maps.data(); *count = Z.S.maps.size(); break;
+        case CWP_ARR_TEX_IMAGES: *data = Z.S.texImages.data(); *count = Z.S.texImages.size(); break;
+        case CWP_ARR_TEXELS: *data = Z.S.texels.data(); *count = Z.S.texels.size(); break;
+        case CWP_ARR_SRGB_TABLE: *data = srgbDecodeTable(); *count = 256; break;
+        case CWP_ARR_MESH_UVS: *data = Z.M.uv.data(); *count = Z.M.hasUV() ? Z.M.f.size() : 0; break;
         default: return fail(CWP_ERR_ARG, "cwp_setup_array: unknown array");
     }
     return CWP_OK;
@@ -326,6 +336,19 @@ static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& e
         S.envRotDeg = d->env_rot_deg; S.sunOn = d->sun_on; S.lightMode = d->light_mode;
         S.rr = d->rr; S.rrDepth = d->rr_depth; S.gdEnergy = d->gd_energy; S.envUniform = d->env_uniform;
         if (S.lightMode < CWP_LIGHT_MIS || S.lightMode > CWP_LIGHT_BSDF) { err = "light_mode must be MIS, NEE or BSDF"; return false; }
+        if (d->maps) {   // textures (checkScene validates ids, sizes and ranges)
+            if (!d->uvs || !d->tex_images || !d->texels) { err = "textures: maps need uvs, tex_images and texels"; return false; }
+            S.maps.resize(d->num_materials);
+            memcpy(S.maps.data(), d->maps, d->num_materials * sizeof(MaterialMaps));
+            take(d->uvs, d->num_tris * 2, S.uvs);
+            if (d->tangents) take(d->tangents, d->num_tris * 3, S.tans); else S.tans.clear();
+            S.texImages.resize(d->num_images);
+            if (d->num_images) memcpy(S.texImages.data(), d->tex_images, d->num_images * sizeof(TexImage));
+            S.texels.assign(d->texels, d->texels + d->num_texels);
+            S.texNearest = d->tex_nearest; S.bumpShadow = d->bump_shadow;
+        }
+    } else if (d->maps) {
+        err = "textures need the full integrator (mats)"; return false;
     }
     return checkScene(S, S.stackNeed, err);
 }
@@ -805,6 +828,40 @@ int cwp_server_autotune(cwp_server* s, const int* variants, int nv, const int* s
     }
     if (!ok) return fail(CWP_ERR_DEVICE, err);
     if (chosen) serverOptionsOut(s->S.options(), chosen);
+    return CWP_OK;
+}
+
+int cwp_image_read(const char* path, int* w, int* h, uint32_t* rgba, size_t capacity) {
+    if (!path || !w || !h) return fail(CWP_ERR_ARG, "cwp_image_read: null argument");
+    TextureImage img;
+    std::string err;
+    if (!readImage(path, img, err)) return fail(CWP_ERR_ARG, err);
+    *w = img.w; *h = img.h;
+    if (!rgba) return CWP_OK;
+    if (capacity < img.rgba.size()) return fail(CWP_ERR_ARG, "cwp_image_read: buffer too small");
+    memcpy(rgba, img.rgba.data(), img.rgba.size() * 4);
+    return CWP_OK;
+}
+
+int cwp_inflate(const uint8_t* in, size_t n, uint8_t* out, size_t capacity, size_t* out_len) {
+    if (!in || !out_len || (capacity && !out)) return fail(CWP_ERR_ARG, "cwp_inflate: null argument");
+    std::vector<uint8_t> o;
+    std::string err;
+    if (!inflateZlib(in, n, o, capacity, err)) return fail(CWP_ERR_ARG, err);
+    if (!o.empty()) memcpy(out, o.data(), o.size());
+    *out_len = o.size();
+    return CWP_OK;
+}
+
+int cwp_texture_sample(const int image[4], const uint32_t* texels, size_t num_texels, float u, float v, int nearest,
+                       float rgba[4]) {
+    if (!image || !texels || !rgba) return fail(CWP_ERR_ARG, "cwp_texture_sample: null argument");
+    TexImage T;
+    T.first = (uint32_t)image[0]; T.w = image[1]; T.h = image[2]; T.srgb = image[3];
+    if (T.w < 1 || T.h < 1 || (uint64_t)T.first + (uint64_t)T.w * T.h > num_texels || (T.srgb != 0 && T.srgb != 1))
+        return fail(CWP_ERR_ARG, "cwp_texture_sample: image outside the texel array");
+    const Vec4 c = sampleTexture(T, texels, srgbDecodeTable(), u, v, nearest ? 1 : 0);
+    rgba[0] = c.x; rgba[1] = c.y; rgba[2] = c.z; rgba[3] = c.w;
     return CWP_OK;
 }
 

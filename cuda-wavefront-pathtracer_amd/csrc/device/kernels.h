@@ -26,6 +26,7 @@
 #include "core/sampling.h"
 #include "device/renderer.h"
 #include "device/traverse.h"
+#include "host/texture.h"
 #include <hip/hip_runtime.h>
 
 namespace cwp {
@@ -63,6 +64,15 @@ struct DevLights {
     // (e2, 0)) and per material the light pdf per unit area (emTri / triLightPdf are unused)
     const Vec4* emTris;
     const float* matLightPdf;
+    // textures: bindings per material (null: none), corner uvs / tangents per leaf position, images and their texels,
+    // the sRGB decode table
+    const MaterialMaps* maps;
+    int nMaps, texNearest, bumpShadow;
+    const Vec4* uvs;
+    const Vec4* tans;
+    const TexImage* texImages;
+    const uint32_t* texels;
+    const float* srgbTable;
 };
 
 // Fills L from the full-integrator data of S, uploading each array with up(pointer, vector) (host side). The sun is
@@ -94,6 +104,13 @@ bool makeDevLights(const SceneUpload& S, Up&& up, DevLights& L, std::string& err
     const int nt = (sun ? 1 : 0) + (tri ? 1 : 0) + (env ? 1 : 0);
     L.pSun = sun ? 1.f / (float)nt : 0.f; L.pTri = tri ? 1.f / (float)nt : 0.f; L.pEnv = env ? 1.f / (float)nt : 0.f;
     L.lightMode = S.lightMode; L.rr = S.rr; L.rrDepth = S.rrDepth; L.gdEnergy = S.gdEnergy; L.envUniform = S.envUniform;
+    if (!S.maps.empty()) {
+        if (!up(L.maps, S.maps, err) || !up(L.uvs, S.uvs, err) || !up(L.texImages, S.texImages, err) ||
+            !up(L.texels, S.texels, err) || !up(L.srgbTable, std::vector<float>(srgbDecodeTable(), srgbDecodeTable() + 256), err))
+            return false;
+        if (!S.tans.empty() && !up(L.tans, S.tans, err)) return false;
+        L.nMaps = (int)S.maps.size(); L.texNearest = S.texNearest; L.bumpShadow = S.bumpShadow;
+    }
     return true;
 }
 

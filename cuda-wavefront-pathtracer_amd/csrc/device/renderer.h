@@ -8,6 +8,7 @@
 // an error string, never by exiting.
 #pragma once
 #include "core/material.h"
+#include "core/texture.h"
 #include "device/traverse_types.h"
 #include "host/camera.h"
 #include "host/envmap.h"
@@ -77,6 +78,14 @@ struct SceneUpload {
     int envUniform = 0;           // environment light samples uniform on the sphere (instead of by luminance)
     int numEmInst = 0;            // two-level scenes: emissive triangles over all instances (set per frame)
     std::vector<int> meshTris;    // two-level scenes: first leaf position and triangle count per mesh
+    // textures (full integrator; empty maps: none)
+    std::vector<MaterialMaps> maps;   // per material id
+    std::vector<Vec4> uvs;        // 2 float4 per leaf position: (u0, v0, u1, v1), (u2, v2, 0, 0)
+    std::vector<Vec4> tans;       // 3 float4 per leaf position (tangent, handedness) or empty (no normal maps)
+    std::vector<TexImage> texImages;
+    std::vector<uint32_t> texels; // every image's RGBA8 texels, rows top first
+    int texNearest = 0;           // 1: nearest-texel lookups; 0: bilinear
+    int bumpShadow = 0;           // 1: terminator shadowing of normal-mapped diffuse shading
 };
 
 class Renderer {

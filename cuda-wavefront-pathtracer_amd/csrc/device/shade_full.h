@@ -10,7 +10,8 @@
 //
 // Differences in mechanism (not in the estimator): radiance goes into the path's own L[i] (no atomics), the shadow
 // ray is returned to the caller's wave-level append, and the pdf of the last BSDF sample travels in D.w of the path
-// state (the reference keeps a separate pAux array). Texture lookups are not part of this build.
+// state (the reference keeps a separate pAux array). Textures (core/texture.h) are looked up by this build's own
+// filtering code instead of a hardware texture unit, so the host oracle evaluates the same arithmetic.
 #pragma once
 
 // y with cdf[y] <= u < cdf[y + 1] (cdf has n + 1 ascending entries, cdf[0] = 0)
@@ -62,6 +63,27 @@ __device__ __forceinline__ Material loadMaterial(const Material* mats, int id) {
     m.baseColor = xyz(a); m.metallic = a.w; m.specularColor = xyz(b); m.roughness = b.w;
     m.emission = xyz(c); m.ior = c.w; m.specularWeight = e.x; m.transmission = e.y; m.pad0 = 0.f; m.pad1 = 0.f;
     return m;
+}
+__device__ __forceinline__ MaterialMaps loadMaps(const MaterialMaps* maps, int id) {
+    const Vec4* p = (const Vec4*)(maps + id);
+    const Vec4 a = p[0], b = p[1];
+    MaterialMaps m;
+    m.baseColor = asInt(a.x); m.emission = asInt(a.y); m.normal = asInt(a.z); m.metallic = asInt(a.w);
+    m.roughness = asInt(b.x); m.metallicChannel = asInt(b.y); m.roughnessChannel = asInt(b.z); m.normalScale = b.w;
+    return m;
+}
+__device__ __forceinline__ Vec4 texLookup(const DevLights& E, int id, float u, float v) {
+    const Vec4 r = *(const Vec4*)(E.texImages + id);
+    TexImage T;
+    T.first = asUint(r.x); T.w = asInt(r.y); T.h = asInt(r.z); T.srgb = asInt(r.w);
+    return sampleTexture(T, E.texels, E.srgbTable, u, v, E.texNearest);
+}
+// texture coordinates of a point of leaf triangle t with barycentric weights b1, b2 of corners 1 and 2
+__device__ __forceinline__ void triUV(const DevLights& E, int t, float b1, float b2, float& u, float& v) {
+    const Vec4 q0 = E.uvs[2 * t], q1 = E.uvs[2 * t + 1];
+    const float b0 = 1.f - b1 - b2;
+    u = q0.x * b0 + q0.z * b1 + q1.x * b2;
+    v = q0.y * b0 + q0.w * b1 + q1.y * b2;
 }
 __device__ __forceinline__ void addRadiance(const DevPaths& P, int i, int depth, Vec3 c) {
     if (depth == 0) { P.L[i] = v4(c, 0.f); return; }
@@ -116,10 +138,46 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
         if (dot(ns, ng0) < 0.f) ns = -ns;
     }
     const int mid = asInt(b.w);
-    const Material M = loadMaterial(E.mats, mid);
+    Material M = loadMaterial(E.mats, mid);
+    Vec3 nsU = ns;   // the shading normal before normal mapping (terminator shadowing)
+    bool mapped = false;
+    if (E.maps && mid < E.nMaps) {   // maps multiply the material's factors; a normal map bends ns
+        const MaterialMaps X = loadMaps(E.maps, mid);
+        if (hasMaps(X)) {
+            float tu, tv;
+            triUV(E, h.tri, h.u, h.v, tu, tv);
+            if (X.baseColor >= 0) M.baseColor = M.baseColor * xyz(texLookup(E, X.baseColor, tu, tv));
+            if (X.emission >= 0) M.emission = M.emission * xyz(texLookup(E, X.emission, tu, tv));
+            if (X.metallic >= 0) M.metallic *= channelOf(texLookup(E, X.metallic, tu, tv), X.metallicChannel);
+            if (X.roughness >= 0) M.roughness *= channelOf(texLookup(E, X.roughness, tu, tv), X.roughnessChannel);
+            if (X.normal >= 0 && E.tans) {
+                const float w0 = 1.f - h.u - h.v;
+                const Vec4 t0 = E.tans[3 * h.tri], t1 = E.tans[3 * h.tri + 1], t2 = E.tans[3 * h.tri + 2];
+                Vec3 tg = xyz(t0) * w0 + xyz(t1) * h.u + xyz(t2) * h.v;
+                float sg = (t0.w * w0 + t1.w * h.u + t2.w * h.v) < 0.f ? -1.f : 1.f;
+                if constexpr (INST) {   // tangents are directions: the forward rows
+                    const InstanceRecord* I = S.inst + h.inst;
+                    const Vec4 xf[3] = { I->fwd[0], I->fwd[1], I->fwd[2] };
+                    tg = xfRowsV(xf, tg);
+                    sg *= dsg;
+                }
+                tg = tg - ns * dot(ns, tg);
+                const float tl = length(tg);
+                if (tl > 0.f) {
+                    tg = tg * (1.f / tl);
+                    const Vec3 bt = cross(ns, tg) * sg;
+                    const Vec4 c = texLookup(E, X.normal, tu, tv);
+                    const Vec3 nm = tg * ((2.f * c.x - 1.f) * X.normalScale) + bt * ((2.f * c.y - 1.f) * X.normalScale) +
+                                    ns * (2.f * c.z - 1.f);
+                    const float nl = length(nm);
+                    if (nl > 0.f) { ns = nm * (1.f / nl); mapped = true; }
+                }
+            }
+        }
+    }
     const bool front = dot(ng0, d) < 0.f;
     const Vec3 ng = front ? ng0 : -ng0;
-    if (!front) ns = -ns;
+    if (!front) { ns = -ns; nsU = -nsU; }
     // emission (two-sided), MIS-weighted against light sampling of this triangle
     Vec3 Lv = v3(0.f, 0.f, 0.f);
     if (isEmissive(M)) {
@@ -148,13 +206,18 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     Vec3 tx, ty;
     basis(ns, tx, ty);
     const Vec3 woL = v3(dot(wo, tx), dot(wo, ty), dot(wo, ns));
+    Vec3 nrL = v3(0.f, 0.f, 1.f);   // the unperturbed normal in the local frame (terminator shadowing off: +z)
+    if (E.bumpShadow && mapped) {
+        nrL = v3(dot(nsU, tx), dot(nsU, ty), dot(nsU, ns));
+        if (!(nrL.z > 0.f)) nrL = v3(0.f, 0.f, 1.f);
+    }
     const float eta = front ? M.ior : 1.f / M.ior;
     const Vec3 poUp = p + ng * F.eps, poDn = p - ng * F.eps;
     const float ux1 = rnd((uint32_t)pix, smp, dd, 7, F.seed), ux2 = rnd((uint32_t)pix, smp, dd, 7, F.seed ^ 0x68E31DA4u);
     // ---- next-event estimation: one light sample
     // the material's response is rebuilt where it is used (light sample, BSDF sample), so it holds no registers across
     // the light-selection code
-    if (hasDensity(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2))) {
+    if (hasDensity(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2, nrL))) {
         const float us = rnd((uint32_t)pix, smp, dd, 5, F.seed);
         const float u2 = rnd((uint32_t)pix, smp, dd, 2, F.seed), u3 = rnd((uint32_t)pix, smp, dd, 3, F.seed);
         Vec3 l = v3(0.f, 0.f, 1.f), Le = v3(0.f, 0.f, 0.f);
@@ -191,6 +254,14 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
                     const int lm = asInt(a1.w);
                     pl = E.pTri * (INST ? E.matLightPdf[lm] : E.triLightPdf[kt]) * dist2 / cl;
                     Le = loadMaterial(E.mats, lm).emission;
+                    if (E.maps && lm < E.nMaps) {   // an emission map at the sampled point
+                        const int emap = loadMaps(E.maps, lm).emission;
+                        if (emap >= 0) {
+                            float tu, tv;
+                            triUV(E, kt, su * (1.f - u3), su * u3, tu, tv);
+                            Le = Le * xyz(texLookup(E, emap, tu, tv));
+                        }
+                    }
                     tmax = dist * (1.f - 1e-3f);
                     ok = true;
                 }
@@ -214,7 +285,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
             const Vec3 lL = v3(dot(l, tx), dot(l, ty), dot(l, ns));
             const float gl = dot(ng, l);
             if (lL.z != 0.f && (gl > 0.f) == (lL.z > 0.f)) {
-                const BsdfValue e = evalResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2), woL, lL);
+                const BsdfValue e = evalResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2, nrL), woL, lL);
                 const float w = (isSun || E.lightMode == kLightNEE) ? 1.f : misPower(pl, e.pdf);
                 const Vec3 con = T * e.f * Le * (fabsf(lL.z) * w / pl);
                 if (con.x > 0.f || con.y > 0.f || con.z > 0.f) {
@@ -228,7 +299,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     // ---- BSDF sample: the continuation
     Vec3 wiL, wgt;
     float pdf;
-    if (!sampleResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2), woL, rnd((uint32_t)pix, smp, dd, 4, F.seed),
+    if (!sampleResponse(surfaceResponse(M, woL, eta, E.gdEnergy, ux1, ux2, nrL), woL, rnd((uint32_t)pix, smp, dd, 4, F.seed),
                         rnd((uint32_t)pix, smp, dd, 0, F.seed), rnd((uint32_t)pix, smp, dd, 1, F.seed), wiL, wgt, pdf))
         return out;
     const Vec3 wi = normalize(tx * wiL.x + ty * wiL.y + ns * wiL.z);

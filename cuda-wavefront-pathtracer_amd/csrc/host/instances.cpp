@@ -183,9 +183,12 @@ void appendTransformed(Mesh& dst, const Mesh& src, int matOffset, const Xform& X
         dst.v.push_back(xfPoint(X, src.v[i]));
         if (wantN) dst.n.push_back(src.hasN() ? xfNormal(inv, src.n[i]) : v3(0.f, 0.f, 0.f));
     }
+    const bool wantUV = !dst.uv.empty() || src.hasUV();   // corner uvs stay with their triangles; (0, 0) where none
+    if (wantUV) dst.uv.resize(6 * dst.f.size(), 0.f);
     for (size_t t = 0; t < src.f.size(); t++) {
         dst.f.push_back({ src.f[t].a + v0, src.f[t].b + v0, src.f[t].c + v0 });
         dst.fm.push_back(src.mat(t) + matOffset);
+        if (wantUV) for (int c = 0; c < 6; c++) dst.uv.push_back(src.hasUV() ? src.uv[6 * t + c] : 0.f);
     }
 }
 

@@ -1,5 +1,6 @@
 // host/io.cpp - JSON, OBJ/MTL, PLY, PNG and PFM.
 #include "host/io.h"
+#include "host/texture.h"
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -351,8 +352,37 @@ bool parseFloats(const std::vector<std::string>& w, size_t from, float* out, int
     return true;
 }
 
-// MTL: "newmtl name" opens a material (diffuse 0.8 until a "Kd r g b" line sets it)
-void readMTL(const std::string& path, Mesh& m, std::map<std::string, int>& byName) {
+// A texture statement of a material: which map, the image file (relative to the MTL), normal strength, channel.
+enum MapKind { kMapBase, kMapEmission, kMapNormal, kMapMetallic, kMapRoughness };
+struct MapRef { int mat, kind; std::string file; float scale; int channel; };
+
+// map statement "<key> [options] file": options -bm s (normal strength), -imfchan r|g|b|m|l|z (channel of a scalar
+// map), and the other standard options, skipped with their arguments; the file name is the rest of the line
+bool parseMapStatement(const std::vector<std::string>& w, std::string& file, float& scale, int& channel) {
+    size_t k = 1;
+    scale = 1.f; channel = -1;
+    auto isNum = [](const std::string& s) { char* e = nullptr; strtod(s.c_str(), &e); return e != s.c_str() && *e == 0; };
+    while (k < w.size() && w[k].size() > 1 && w[k][0] == '-' && !isNum(w[k])) {
+        const std::string& o = w[k++];
+        if (o == "-bm" && k < w.size()) scale = (float)atof(w[k++].c_str());
+        else if (o == "-imfchan" && k < w.size()) {
+            const char c = w[k++][0];
+            channel = c == 'g' ? 1 : (c == 'b' ? 2 : (c == 'z' ? 3 : 0));
+        } else if (o == "-o" || o == "-s" || o == "-t") { for (int j = 0; j < 3 && k < w.size() && isNum(w[k]); j++) k++; }
+        else if (o == "-mm") k += 2;
+        else k++;   // -blendu, -blendv, -clamp, -cc, -boost, -texres: one argument
+    }
+    if (k >= w.size()) return false;
+    file = w[k];
+    for (size_t j = k + 1; j < w.size(); j++) file += " " + w[j];
+    return true;
+}
+
+// MTL: "newmtl name" opens a material (diffuse 0.8 until a "Kd r g b" line sets it; "Ke r g b" gives it an emission
+// colour); texture maps map_Kd (base colour), map_Ke (emission), map_Bump / bump / norm / map_Kn (tangent-space normal
+// map), map_Pr (roughness) and map_Pm (metalness)
+void readMTL(const std::string& path, Mesh& m, std::map<std::string, int>& byName, std::vector<MapRef>& refs,
+             std::vector<Vec3>& ke) {
     WordReader in(path);
     if (!in.ok()) { fprintf(stderr, "warn: mtl %s not found\n", path.c_str()); return; }
     std::vector<std::string> w;
@@ -368,8 +398,55 @@ void readMTL(const std::string& path, Mesh& m, std::map<std::string, int>& byNam
         } else if (w[0] == "Kd" && cur >= 0) {
             float c[3];
             if (parseFloats(w, 1, c, 3)) m.kd[cur] = v3(c[0], c[1], c[2]);
+        } else if (w[0] == "Ke" && cur >= 0) {
+            float c[3];
+            if (parseFloats(w, 1, c, 3)) { ke.resize(m.kd.size(), v3(0.f, 0.f, 0.f)); ke[cur] = v3(c[0], c[1], c[2]); }
+        } else if (cur >= 0) {
+            const std::string& k = w[0];
+            const int kind = k == "map_Kd" ? kMapBase : k == "map_Ke" ? kMapEmission :
+                             (k == "map_Bump" || k == "map_bump" || k == "bump" || k == "norm" || k == "map_Kn") ? kMapNormal :
+                             k == "map_Pm" ? kMapMetallic : k == "map_Pr" ? kMapRoughness : -1;
+            MapRef r;
+            if (kind >= 0 && parseMapStatement(w, r.file, r.scale, r.channel)) {
+                r.mat = cur; r.kind = kind;
+                r.file = dirOf(path) + "/" + r.file;
+                refs.push_back(r);
+            }
         }
     }
+}
+
+// Loads the images the map statements name (one image per (file, decode); an unreadable image leaves its map off,
+// with a warning, as the reference does) and sets the materials' bindings.
+void bindMaps(Mesh& m, const std::vector<MapRef>& refs) {
+    if (refs.empty()) return;
+    m.maps.assign(m.kd.size(), noMaps());
+    std::map<std::pair<std::string, bool>, int> loaded;
+    for (const MapRef& r : refs) {
+        const bool srgb = r.kind == kMapBase || r.kind == kMapEmission;
+        const auto key = std::make_pair(r.file, srgb);
+        auto it = loaded.find(key);
+        if (it == loaded.end()) {
+            TextureImage img;
+            std::string err;
+            int id = -1;
+            if (readImage(r.file, img, err)) { img.srgb = srgb; id = (int)m.tex.size(); m.tex.push_back(std::move(img)); }
+            else fprintf(stderr, "warn: texture %s\n", err.c_str());
+            it = loaded.emplace(key, id).first;
+        }
+        MaterialMaps& x = m.maps[r.mat];
+        const int id = it->second;
+        switch (r.kind) {
+            case kMapBase: x.baseColor = id; break;
+            case kMapEmission: x.emission = id; break;
+            case kMapNormal: x.normal = id; x.normalScale = r.scale; break;
+            case kMapMetallic: x.metallic = id; if (r.channel >= 0) x.metallicChannel = r.channel; break;
+            default: x.roughness = id; if (r.channel >= 0) x.roughnessChannel = r.channel; break;
+        }
+    }
+    bool any = false;
+    for (const MaterialMaps& x : m.maps) any = any || hasMaps(x);
+    if (!any) { m.maps.clear(); m.tex.clear(); }
 }
 
 // one OBJ index (1-based, or negative: counted back from the end of the list so far); 0 or garbage gives an index
@@ -381,41 +458,59 @@ int objIndex(const std::string& s, int listSize) {
 
 }  // namespace
 
-// OBJ: "v x y z", "vn x y z", faces "f a b c ..." whose corners are "v", "v/t", "v//n" or "v/t/n" (fanned from the
-// first corner into triangles), "mtllib file" (relative to the OBJ), "usemtl name" (unknown names: material 0). A
-// corner's normal is stored on its vertex; a mesh whose normals are all zero has none (reference behaviour:
-// src/io/mesh_loader.cpp:14-62).
+// OBJ: "v x y z", "vt u v", "vn x y z", faces "f a b c ..." whose corners are "v", "v/t", "v//n" or "v/t/n" (fanned
+// from the first corner into triangles), "mtllib file" (relative to the OBJ), "usemtl name" (unknown names: material
+// 0). A corner's normal is stored on its vertex; a mesh whose normals are all zero has none (reference behaviour:
+// src/io/mesh_loader.cpp:14-62). Texture coordinates are kept per triangle corner (corners without one get (0, 0));
+// the MTL's texture maps are bound to their materials.
 bool loadOBJ(const std::string& path, Mesh& m) {
     WordReader in(path);
     if (!in.ok()) return false;
     if (m.kd.empty()) { m.kd.push_back(v3(0.8f, 0.8f, 0.8f)); m.matName.push_back(""); }
     std::vector<Vec3> normals;
+    std::vector<float> texcoords;   // 2 per vt
     std::map<std::string, int> byName;
+    std::vector<MapRef> refs;
+    std::vector<Vec3> ke;   // MTL emission colours per material id
     int material = 0;
+    bool anyUV = false;
     std::vector<std::string> w;
-    std::vector<int> cv, cn;   // corners of the current face: vertex, normal (-1: none)
+    std::vector<int> cv, cn, ct;   // corners of the current face: vertex, normal, texture coordinate (-1: none)
+    std::vector<float> fuv;        // corner uvs of every triangle so far
     while (in.line(w)) {
         if (w.empty()) continue;
         const std::string& k = w[0];
         float x[3];
         if (k == "v") { if (parseFloats(w, 1, x, 3)) m.v.push_back(v3(x[0], x[1], x[2])); }
         else if (k == "vn") { if (parseFloats(w, 1, x, 3)) normals.push_back(v3(x[0], x[1], x[2])); }
+        else if (k == "vt") { if (parseFloats(w, 1, x, 2)) { texcoords.push_back(x[0]); texcoords.push_back(x[1]); } }
         else if (k == "f") {
-            cv.clear(); cn.clear();
+            cv.clear(); cn.clear(); ct.clear();
             for (size_t j = 1; j < w.size(); j++) {
                 const std::string& c = w[j];
                 const size_t s1 = c.find('/'), s2 = s1 == std::string::npos ? s1 : c.find('/', s1 + 1);
                 cv.push_back(objIndex(c, (int)m.v.size()));
                 const bool hasN = s2 != std::string::npos && s2 + 1 < c.size();
                 cn.push_back(hasN ? objIndex(c.substr(s2 + 1), (int)normals.size()) : -1);
+                const bool hasT = s1 != std::string::npos && s1 + 1 < c.size() && c[s1 + 1] != '/';
+                const int t = hasT ? objIndex(c.substr(s1 + 1), (int)texcoords.size() / 2) : -1;
+                ct.push_back(t >= 0 && t < (int)texcoords.size() / 2 ? t : -1);
+                anyUV = anyUV || ct.back() >= 0;
             }
-            for (size_t j = 2; j < cv.size(); j++) { m.f.push_back({ cv[0], cv[j - 1], cv[j] }); m.fm.push_back(material); }
+            auto cornerUV = [&](size_t j) {
+                fuv.push_back(ct[j] >= 0 ? texcoords[2 * ct[j]] : 0.f);
+                fuv.push_back(ct[j] >= 0 ? texcoords[2 * ct[j] + 1] : 0.f);
+            };
+            for (size_t j = 2; j < cv.size(); j++) {
+                m.f.push_back({ cv[0], cv[j - 1], cv[j] }); m.fm.push_back(material);
+                cornerUV(0); cornerUV(j - 1); cornerUV(j);
+            }
             for (size_t j = 0; j < cv.size(); j++) {
                 if (cn[j] < 0 || cn[j] >= (int)normals.size() || cv[j] < 0 || cv[j] >= (int)m.v.size()) continue;
                 if (m.n.size() != m.v.size()) m.n.resize(m.v.size(), v3(0.f, 0.f, 0.f));
                 m.n[cv[j]] = normals[cn[j]];
             }
-        } else if (k == "mtllib" && w.size() >= 2) readMTL(dirOf(path) + "/" + w[1], m, byName);
+        } else if (k == "mtllib" && w.size() >= 2) readMTL(dirOf(path) + "/" + w[1], m, byName, refs, ke);
         else if (k == "usemtl") {
             const auto it = w.size() >= 2 ? byName.find(w[1]) : byName.end();
             material = it == byName.end() ? 0 : it->second;
@@ -424,6 +519,18 @@ bool loadOBJ(const std::string& path, Mesh& m) {
     bool anyNormal = false;
     for (const Vec3& x : m.n) anyNormal = anyNormal || length(x) > 0.f;
     if (!anyNormal) m.n.clear();
+    if (anyUV) m.uv.swap(fuv);
+    bool emissive = false;
+    for (const Vec3& e : ke) emissive = emissive || e.x > 0.f || e.y > 0.f || e.z > 0.f;
+    if (emissive) {   // Lambertian materials with their emission colours
+        m.mats.clear();
+        for (size_t j = 0; j < m.kd.size(); j++) {
+            Material x = lambertMaterial(m.kd[j]);
+            if (j < ke.size()) x.emission = ke[j];
+            m.mats.push_back(x);
+        }
+    }
+    bindMaps(m, refs);
     return !m.f.empty();
 }
 

@@ -1,6 +1,7 @@
 // host/mesh.cpp - mesh helpers and procedural scenes.
 #include "host/mesh.h"
 #include "host/camera.h"
+#include <algorithm>
 #include <cmath>
 #include <map>
 
@@ -199,6 +200,60 @@ void assignGeneratedMaterials(Mesh& m, int first, int n, uint64_t seed) {
     m.kd.resize(mats.size(), v3(0.8f, 0.8f, 0.8f));
     for (size_t j = 0; j < mats.size(); j++) m.kd[j] = mats[j].baseColor;
     m.mats = mats;
+}
+
+void addGeneratedTextures(Mesh& m, int size, uint64_t seed) {
+    const float repeat = 8.f;
+    m.uv.assign(6 * m.f.size(), 0.f);
+    for (size_t t = 0; t < m.f.size(); t++) {
+        const Vec3 p[3] = { m.v[m.f[t].a], m.v[m.f[t].b], m.v[m.f[t].c] };
+        const Vec3 n = cross(p[1] - p[0], p[2] - p[0]);
+        const float ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+        for (int c = 0; c < 3; c++) {   // project out the dominant axis
+            const float u = az >= ax && az >= ay ? p[c].x : p[c].y, v = az >= ax && az >= ay ? p[c].y : p[c].z;
+            m.uv[6 * t + 2 * c] = u * repeat; m.uv[6 * t + 2 * c + 1] = v * repeat;
+        }
+    }
+    SplitMix rng(seed * 0x9E3779B97F4A7C15ull + 17);
+    const int cells = 8;
+    std::vector<float> noise((size_t)(cells + 1) * (cells + 1));
+    for (float& x : noise) x = rng.next01();
+    auto smooth = [&](float x, float y) {   // bilinear value noise over the cell lattice, periodic
+        const float gx = x * cells, gy = y * cells;
+        const int ix = (int)gx % cells, iy = (int)gy % cells;
+        const float fx = gx - std::floor(gx), fy = gy - std::floor(gy);
+        auto at = [&](int i, int j) { return noise[(size_t)(j % cells) * (cells + 1) + (i % cells)]; };
+        return (at(ix, iy) * (1 - fx) + at(ix + 1, iy) * fx) * (1 - fy) + (at(ix, iy + 1) * (1 - fx) + at(ix + 1, iy + 1) * fx) * fy;
+    };
+    std::vector<int> metalCell((size_t)cells * cells);
+    for (int& c : metalCell) c = rng.next01() < 0.25f ? 1 : 0;
+    auto byte = [](float x) { return (uint32_t)std::lround(std::fmin(std::fmax(x, 0.f), 1.f) * 255.f); };
+    TextureImage base, normal, packed;
+    for (TextureImage* T : { &base, &normal, &packed }) { T->w = T->h = size; T->rgba.resize((size_t)size * size); }
+    base.path = "<generated base colour>"; base.srgb = true;
+    normal.path = "<generated normal map>"; packed.path = "<generated roughness / metalness>";
+    const float k = 6.2831853f * 4.f, amp = 0.35f;
+    for (int y = 0; y < size; y++)
+        for (int x = 0; x < size; x++) {
+            const float u = (x + 0.5f) / size, v = (y + 0.5f) / size;
+            const int cx = (int)(u * cells), cy = (int)(v * cells);
+            const float nz = smooth(u, v);
+            const bool odd = ((cx + cy) & 1) != 0;
+            const Vec3 c0 = v3(0.85f, 0.72f, 0.55f), c1 = v3(0.25f, 0.35f, 0.45f);
+            const Vec3 col = (odd ? c1 : c0) * (0.7f + 0.3f * nz);
+            base.rgba[(size_t)y * size + x] = byte(col.x) | (byte(col.y) << 8) | (byte(col.z) << 16) | (255u << 24);
+            // bump h = amp / k (sin(k u) + sin(k v)): slopes amp cos(k u), amp cos(k v)
+            const Vec3 nn = normalize(v3(-amp * std::cos(k * u), -amp * std::cos(k * v), 1.f));
+            normal.rgba[(size_t)y * size + x] = byte(nn.x * 0.5f + 0.5f) | (byte(nn.y * 0.5f + 0.5f) << 8) |
+                                                (byte(nn.z * 0.5f + 0.5f) << 16) | (255u << 24);
+            const float rough = 0.3f + 0.7f * nz;
+            packed.rgba[(size_t)y * size + x] = (byte(rough) << 8) | ((metalCell[(size_t)cy * cells + cx] ? 255u : 0u) << 16) | (255u << 24);
+        }
+    m.tex = { base, normal, packed };
+    const size_t nm = std::max(m.kd.size(), std::max(m.mats.size(), (size_t)1));
+    MaterialMaps x = noMaps();
+    x.baseColor = 0; x.normal = 1; x.normalScale = 1.f; x.roughness = 2; x.roughnessChannel = 1; x.metallic = 2; x.metallicChannel = 2;
+    m.maps.assign(nm, x);
 }
 
 }  // namespace cwp

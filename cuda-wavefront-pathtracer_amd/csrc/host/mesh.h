@@ -1,6 +1,7 @@
 // host/mesh.h - indexed triangle meshes, boxes, and the procedural scenes used by the CLI, the tests and the bench.
 #pragma once
 #include "core/material.h"
+#include "core/texture.h"
 #include "core/vecmath.h"
 #include <cfloat>
 #include <string>
@@ -9,6 +10,14 @@
 namespace cwp {
 
 struct Tri { int a, b, c; };
+
+// An RGBA8 texture image (rows top first, r in the low byte of each texel).
+struct TextureImage {
+    std::string path;
+    int w = 0, h = 0;
+    bool srgb = false;             // colour data (base colour, emission): decoded from sRGB on lookup
+    std::vector<uint32_t> rgba;
+};
 
 // Indexed triangle mesh with a per-triangle material id and a per-material Lambert albedo table
 // (reference: src/scene/mesh.h:9-26; material names come from OBJ usemtl).
@@ -20,7 +29,11 @@ struct Mesh {
     std::vector<Vec3> kd;      // albedo per material id
     std::vector<std::string> matName;
     std::vector<Material> mats; // full materials per id (from a materials file or a generator); empty: Lambert kd
+    std::vector<float> uv;      // texture coordinates: 6 floats per triangle (u, v of its 3 corners) or empty
+    std::vector<MaterialMaps> maps;   // texture bindings per material id (empty: none)
+    std::vector<TextureImage> tex;    // the images the bindings name
     bool hasN() const { return !n.empty() && n.size() == v.size(); }
+    bool hasUV() const { return !f.empty() && uv.size() == 6 * f.size(); }
     int mat(size_t t) const { return fm.empty() ? 0 : fm[t]; }
 };
 
@@ -80,6 +93,12 @@ void sphereGridCamera(float c2w[16]);
 // U[0.05, 0.6]) 25 %, glossy-diffuse (random colour, roughness U[0.05, 0.6]) 30 %, emitter (Lambert 0.5, emission
 // strength 8) 15 %.
 void assignGeneratedMaterials(Mesh& m, int first, int n, uint64_t seed);
+
+// Generated textures for a procedural scene: texture coordinates by planar projection along each triangle's dominant
+// normal axis (8 repeats per unit), and three size x size images bound to every material: a base-colour checker
+// (8 x 8 cells, value-noise modulated, sRGB), a tangent-space normal map of a sinusoidal bump field, and a packed map
+// with roughness U[0.3, 1] in G and metalness 0 / 1 per cell in B (glTF channels).
+void addGeneratedTextures(Mesh& m, int size, uint64_t seed);
 
 // Subdivided icosahedron of radius 1 at the origin (20 * 4^level triangles).
 void makeIcosphere(Mesh& m, int level, int mat);

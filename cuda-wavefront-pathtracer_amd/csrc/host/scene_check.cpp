@@ -129,6 +129,28 @@ bool checkBVH8(const SceneUpload& S, size_t nn, size_t nt, Links& G, std::string
 }  // namespace
 
 // full-integrator data: a material per id, emitter and environment tables of consistent sizes and ranges
+// texture bindings name existing images and channels, every image lies inside the texel array, the corner uvs (and,
+// with a normal map, the tangents) cover every triangle
+static bool checkTextures(const SceneUpload& S, size_t nt, std::string& err) {
+    if (S.maps.empty()) return true;
+    if (!S.full) return fail(err, "textures need the full integrator");
+    if (S.uvs.size() != 2 * nt) return fail(err, "textures: texture coordinates must be 2 float4 per triangle");
+    bool normalMap = false;
+    for (const MaterialMaps& x : S.maps) {
+        for (int id : { x.baseColor, x.emission, x.normal, x.metallic, x.roughness })
+            if (id < -1 || id >= (int)S.texImages.size()) return fail(err, "textures: image id out of range");
+        if (x.metallicChannel < 0 || x.metallicChannel > 3 || x.roughnessChannel < 0 || x.roughnessChannel > 3)
+            return fail(err, "textures: channel out of range");
+        normalMap = normalMap || x.normal >= 0;
+    }
+    if (normalMap ? S.tans.size() != 3 * nt : !S.tans.empty()) return fail(err, "textures: tangents must be 3 float4 per triangle with a normal map");
+    for (const TexImage& T : S.texImages)
+        if (T.w < 1 || T.h < 1 || (uint64_t)T.first + (uint64_t)T.w * (uint64_t)T.h > S.texels.size() || (T.srgb != 0 && T.srgb != 1))
+            return fail(err, "textures: image outside the texel array");
+    if (S.texNearest != 0 && S.texNearest != 1) return fail(err, "textures: filter must be 0 or 1");
+    return true;
+}
+
 static bool checkLights(const SceneUpload& S, size_t nt, std::string& err) {
     if (S.mats.size() < S.albedo.size()) return fail(err, "full integrator: fewer materials than material ids");
     if (S.lightMode < kLightMIS || S.lightMode > kLightBSDF) return fail(err, "full integrator: unknown light mode");
@@ -177,6 +199,7 @@ bool checkScene(const SceneUpload& S, int& stackNeed, std::string& err) {
             return fail(err, "triangle " + std::to_string(t) + ": material id out of range");
     }
     if (S.full && !checkLights(S, nt, err)) return false;
+    if (!checkTextures(S, nt, err)) return false;
     int depth = 0;
     Links G;
     const bool ok = S.variant == 2 ? checkBVH2(S, nn, nt, G, err)

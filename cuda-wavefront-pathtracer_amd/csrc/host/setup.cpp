@@ -6,6 +6,7 @@
 #include "host/scene_check.h"
 #include "host/instances.h"
 #include "host/io.h"
+#include "host/texture.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -100,6 +101,10 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--gpu_collapse") ok = inum(A.gpuCollapse);
         else if (k == "--morton_bits") ok = inum(A.mortonBits);
         else if (k == "--tlas") ok = str(A.tlas);
+        else if (k == "--textures") ok = str(A.textures);
+        else if (k == "--tex_filter") ok = str(A.texFilter);
+        else if (k == "--bump_shadow") ok = inum(A.bumpShadow);
+        else if (k == "--gen_textures") ok = inum(A.genTextures);
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--sort") ok = inum(A.sort);
         else if (k == "--devloop") {
@@ -117,6 +122,19 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.builder != "sah" && A.builder != "hploc") { err = "--builder must be sah or hploc"; return false; }
     if (A.mortonBits != 0 && A.mortonBits != 30) { err = "--morton_bits must be 0 (adaptive) or 30"; return false; }
     if (A.tlas != "bvh2" && A.tlas != "bvh8") { err = "--tlas must be bvh2 or bvh8"; return false; }
+    if (A.texFilter != "linear" && A.texFilter != "nearest") { err = "--tex_filter must be linear or nearest"; return false; }
+    {
+        std::string t = A.textures + ",", cur;
+        for (char c : t) {
+            if (c != ',') { cur += c; continue; }
+            if (!cur.empty() && cur != "base" && cur != "emission" && cur != "normal" && cur != "metallic" && cur != "roughness" &&
+                cur != "none") {
+                err = "--textures: unknown kind " + cur + " (base, emission, normal, metallic, roughness, or none)"; return false;
+            }
+            cur.clear();
+        }
+    }
+    if (A.genTextures < 0 || A.genTextures > 8192) { err = "--gen_textures must be 0 (off) or an image size up to 8192"; return false; }
     if (A.collapse != "sah") { err = "--collapse must be sah"; return false; }
     if (A.fuse < -1 || A.fuse > 1) { err = "--fuse must be 0 (split trace and shade kernels) or 1 (fused)"; return false; }
     if (A.persistent < 0 || A.persistent > 2) {
@@ -333,6 +351,37 @@ static bool buildSceneBVH(SceneSetup& Z, std::string& err) {
         std::string werr;
         if (!saveBvhCache(cachePath, cacheKey, S, E, werr)) fprintf(stderr, "warning: %s\n", werr.c_str());
     }
+    S.maps.clear(); S.uvs.clear(); S.tans.clear(); S.texImages.clear(); S.texels.clear();
+    if (Z.full && Z.textured) {   // corner uvs and tangents in leaf order (S.order numbers the parts' triangles in turn)
+        bool anyNormalMap = false;
+        for (const MaterialMaps& x : M.maps) anyNormalMap = anyNormalMap || x.normal >= 0;
+        std::vector<float> uvAll;
+        std::vector<Vec4> tanAll;
+        for (const Mesh* p : parts) {
+            const std::vector<float> uv = p->hasUV() ? p->uv : std::vector<float>(6 * p->f.size(), 0.f);
+            if (anyNormalMap) { std::vector<Vec4> t; cornerTangents(*p, uv, t); tanAll.insert(tanAll.end(), t.begin(), t.end()); }
+            uvAll.insert(uvAll.end(), uv.begin(), uv.end());
+        }
+        const size_t n = S.order.size();
+        S.uvs.resize(2 * n);
+        if (anyNormalMap) S.tans.resize(3 * n);
+        for (size_t kk = 0; kk < n; kk++) {
+            const size_t t = (size_t)S.order[kk];
+            const float* q = &uvAll[6 * t];
+            S.uvs[2 * kk] = v4(q[0], q[1], q[2], q[3]); S.uvs[2 * kk + 1] = v4(q[4], q[5], 0.f, 0.f);
+            if (anyNormalMap) for (int c = 0; c < 3; c++) S.tans[3 * kk + c] = tanAll[3 * t + c];
+        }
+        S.maps = M.maps;
+        for (const TextureImage& T : M.tex) {
+            if (S.texels.size() + T.rgba.size() > 0xFFFFFFFFull) { err = "textures: more than 2^32 texels"; return false; }
+            TexImage r;
+            r.first = (uint32_t)S.texels.size(); r.w = T.w; r.h = T.h; r.srgb = T.srgb ? 1 : 0;
+            S.texImages.push_back(r);
+            S.texels.insert(S.texels.end(), T.rgba.begin(), T.rgba.end());
+        }
+        S.texNearest = A.texFilter == "nearest" ? 1 : 0;
+        S.bumpShadow = A.bumpShadow ? 1 : 0;
+    }
     S.albedo.clear();
     for (const Vec3& k : M.kd) S.albedo.push_back(v4(k, 0.f));
     if (Z.full) {
@@ -367,7 +416,7 @@ bool relayoutScene(const SceneSetup& Z, int variant, SceneUpload& S, std::string
     T.A.variant = variant == 2 ? "bvh2" : (variant == 4 ? "bvh4" : "bvh8");
     T.M = Z.M;
     T.P.hasN = Z.P.hasN;
-    T.full = Z.full; T.mats = Z.mats; T.env = Z.env;
+    T.full = Z.full; T.mats = Z.mats; T.env = Z.env; T.textured = Z.textured;
     if (!buildSceneBVH(T, err)) return false;
     S = std::move(T.S);
     return true;
@@ -416,16 +465,26 @@ bool setupScene(SceneSetup& Z, std::string& err) {
         std::vector<int> matOff;
         bool anyMats = false;
         for (const Mesh& m : meshes) anyMats = anyMats || !m.mats.empty();
+        bool anyMaps = false;
+        for (const Mesh& m : meshes) anyMaps = anyMaps || !m.maps.empty();
         for (Mesh& m : meshes) {
             if (m.kd.empty()) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
             int maxMat = 0; for (int t : m.fm) maxMat = std::max(maxMat, t);
             while ((int)m.kd.size() <= maxMat) m.kd.push_back(v3(0.8f, 0.8f, 0.8f));
             while (m.kd.size() < m.mats.size()) m.kd.push_back(m.mats[m.kd.size()].baseColor);
             matOff.push_back((int)M.kd.size());
+            const int texOff = (int)M.tex.size();
             for (size_t j = 0; j < m.kd.size(); j++) {
                 M.kd.push_back(m.kd[j]); M.matName.push_back(j < m.matName.size() ? m.matName[j] : "");
                 if (anyMats) M.mats.push_back(j < m.mats.size() ? m.mats[j] : lambertMaterial(m.kd[j]));
+                if (anyMaps) {   // the mesh's bindings, their image ids moved past the images of the meshes before
+                    MaterialMaps x = j < m.maps.size() ? m.maps[j] : noMaps();
+                    for (int* id : { &x.baseColor, &x.emission, &x.normal, &x.metallic, &x.roughness }) if (*id >= 0) *id += texOff;
+                    M.maps.push_back(x);
+                }
             }
+            M.tex.insert(M.tex.end(), m.tex.begin(), m.tex.end());
+            m.tex.clear(); m.maps.clear();
         }
         if (A.flatten) {   // every instance transformed into one mesh at the first frame (reference --flatten 1)
             for (const InstanceDesc& I : Z.SD.instances) appendTransformed(M, meshes[I.mesh], matOff[I.mesh], instanceXform(I, A.frameStart));
@@ -487,10 +546,35 @@ bool setupScene(SceneSetup& Z, std::string& err) {
         Z.mats.push_back(x);
     }
     while (M.kd.size() < Z.mats.size()) M.kd.push_back(Z.mats[M.kd.size()].baseColor);
+    // textures: generated maps, the kinds asked for (--textures), --albedo replacing textured base colours too
+    if (A.genTextures > 0) {
+        if (A.gen != "terrain" && A.gen != "boxes") { err = "--gen_textures needs a --gen terrain or boxes scene"; return false; }
+        addGeneratedTextures(M, A.genTextures, A.genSeed);
+    }
+    if (!M.maps.empty()) {
+        M.maps.resize(Z.mats.size(), noMaps());
+        const std::string kinds = "," + A.textures + ",";
+        auto on = [&](const char* k) { return kinds.find(std::string(",") + k + ",") != std::string::npos; };
+        for (MaterialMaps& x : M.maps) {
+            if (!on("base") || A.albedo.x >= 0) x.baseColor = -1;
+            if (!on("emission")) x.emission = -1;
+            if (!on("normal")) x.normal = -1;
+            if (!on("metallic")) x.metallic = -1;
+            if (!on("roughness")) x.roughness = -1;
+        }
+    }
+    Z.textured = false;
+    for (const MaterialMaps& x : M.maps) Z.textured = Z.textured || hasMaps(x);
+    if (Z.textured) {
+        bool anyUV = Z.inst ? false : M.hasUV();
+        for (const Mesh& m : Z.meshes) anyUV = anyUV || m.hasUV();
+        if (!anyUV) { fprintf(stderr, "warning: texture maps without texture coordinates are ignored\n"); Z.textured = false; }
+    }
+    if (!Z.textured) { M.maps.clear(); M.tex.clear(); }
     if (!A.env.empty()) A.light.envMap = A.env;
     if (A.envStrength >= 0.f) A.light.envStrength = A.envStrength;
     if (A.envRot > -1e29f) A.light.envRotDeg = A.envRot;
-    bool needFull = A.rr != 0 || !A.light.envMap.empty() || !A.genEnv.empty();
+    bool needFull = A.rr != 0 || !A.light.envMap.empty() || !A.genEnv.empty() || Z.textured;
     for (const Material& x : Z.mats) needFull = needFull || !isLambert(x) || isEmissive(x);
     Z.full = A.integrator == "full" || (A.integrator == "auto" && needFull);
     Z.env.reset();

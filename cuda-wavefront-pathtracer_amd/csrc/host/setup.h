@@ -38,6 +38,10 @@ struct Args {
     int mortonBits = 0;              // H-PLOC keys (--builder hploc BLAS builds, two-level scenes' TLAS): 0 adaptive, 30
     std::string tlas = "bvh8";       // two-level scenes with BVH8 BLASes: per-frame TLAS as bvh2 or compressed bvh8
     std::string envOut;              // write the environment map in use (e.g. a generated one) as a Radiance HDR file
+    // textures (full integrator): the kinds to use ("none": no maps), the filter, terminator shadowing, generated textures
+    std::string textures = "base,emission,normal,metallic,roughness", texFilter = "linear";
+    int bumpShadow = 0;
+    int genTextures = 0;             // > 0: --gen terrain / boxes get texture coordinates and generated maps of this size
     float fx = 0, fy = 0, hfov = 0, aperture = -1.f, focusDist = -1.f, epsScale = 1e-4f, nearT = 0.f;
     unsigned seed = 1000;
     Vec3 albedo = v3(-1, 0, 0);
@@ -62,6 +66,7 @@ struct SceneSetup {
     std::string sceneName;
     // full material integrator: the material table and the environment map (shared by every layout of the scene)
     bool full = false;
+    bool textured = false;           // texture maps bound (full integrator)
     std::vector<Material> mats;
     std::shared_ptr<const EnvMap> env;
     int genBoxFirst = -1, genBoxCount = 0;   // generated scenes: triangle range of the boxes (12 each)

@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(PKG, "libcwp.so")
 
 ARR_VERTICES, ARR_TRIANGLES, ARR_MATERIAL, ARR_ALBEDO, ARR_NODES, ARR_ORDER, ARR_WOOP, ARR_CORNERS, ARR_NORMALS, \
     ARR_TREE, ARR_TREE_BOX, ARR_PRIM, ARR_MESH_ROOTS, ARR_MESH_BOXES, ARR_MATERIALS, ARR_EM_TRI, ARR_EM_CDF, \
-    ARR_TRI_LIGHT_PDF, ARR_ENV_TEX, ARR_ENV_MARG, ARR_ENV_COND, ARR_MESH_TRIS = range(22)
+    ARR_TRI_LIGHT_PDF, ARR_ENV_TEX, ARR_ENV_MARG, ARR_ENV_COND, ARR_MESH_TRIS, ARR_UVS, ARR_TANGENTS, ARR_MAPS, \
+    ARR_TEX_IMAGES, ARR_TEXELS, ARR_SRGB_TABLE, ARR_MESH_UVS = range(29)
 TRI_MT, TRI_WOOP, TRI_WATERTIGHT = 0, 1, 2
 CWP_COPY_H2D, CWP_COPY_D2H, CWP_COPY_D2D = 1, 2, 3
 
@@ -62,7 +63,8 @@ class SceneInfo(ctypes.Structure):
                 ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("sun_on", ctypes.c_int), ("light_mode", ctypes.c_int),
                 ("rr", ctypes.c_int), ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int),
                 ("env_uniform", ctypes.c_int), ("env_rot_deg", ctypes.c_float),
-                ("bvh_cache", ctypes.c_int)]
+                ("bvh_cache", ctypes.c_int), ("textured", ctypes.c_int), ("num_images", ctypes.c_int),
+                ("tex_nearest", ctypes.c_int), ("bump_shadow", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -84,7 +86,10 @@ class SceneDesc(ctypes.Structure):
                 ("em_cdf", ctypes.c_void_p), ("num_emitters", ctypes.c_size_t), ("env_texels", ctypes.c_void_p),
                 ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("env_rot_deg", ctypes.c_float), ("sun_on", ctypes.c_int),
                 ("light_mode", ctypes.c_int), ("rr", ctypes.c_int), ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int),
-                ("env_uniform", ctypes.c_int)]
+                ("env_uniform", ctypes.c_int), ("maps", ctypes.c_void_p), ("uvs", ctypes.c_void_p),
+                ("tangents", ctypes.c_void_p), ("tex_images", ctypes.c_void_p), ("num_images", ctypes.c_size_t),
+                ("texels", ctypes.c_void_p), ("num_texels", ctypes.c_size_t), ("tex_nearest", ctypes.c_int),
+                ("bump_shadow", ctypes.c_int)]
 
 
 class Options(ctypes.Structure):
@@ -152,7 +157,8 @@ def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_for
     """Builds a cwp_scene_desc over numpy arrays; returns (desc, keep-alive list). tri_form 1: corners holds the
     reference's edge form (v0, e1, e2). full: a dict of the full integrator's fields (mats (m, 16) float32,
     tri_light_pdf, em_tri, em_cdf, env_texels (h, w, 4), env_rot_deg, sun_on, light_mode, rr, rr_depth, gd_energy,
-    env_uniform), or None for a legacy-integrator scene."""
+    env_uniform; textures: maps (m, 8) int32, uvs (n, 2, 4), tangents (n, 3, 4), tex_images (k, 4) int32, texels uint32,
+    tex_nearest, bump_shadow), or None for a legacy-integrator scene."""
     keep = [np.ascontiguousarray(a, dtype=np.float32) if a is not None else None
             for a in (nodes, corners, woop, normals, albedo)]
     n, c, w, nr, al = keep
@@ -181,6 +187,14 @@ def scene_desc(variant, nodes, corners, albedo, woop=None, normals=None, tri_for
         d.light_mode = full.get("light_mode", 0)
         d.rr, d.rr_depth = full.get("rr", 0), full.get("rr_depth", 3)
         d.gd_energy, d.env_uniform = full.get("gd_energy", 1), full.get("env_uniform", 0)
+        maps = arr("maps", np.int32)
+        if maps is not None:
+            uv, tg, ti, tx = arr("uvs", np.float32), arr("tangents", np.float32), arr("tex_images", np.int32), arr("texels", np.uint32)
+            d.maps, d.uvs = maps.ctypes.data, uv.ctypes.data
+            d.tangents = tg.ctypes.data if tg is not None else None
+            d.tex_images, d.num_images = ti.ctypes.data, len(ti)
+            d.texels, d.num_texels = tx.ctypes.data, tx.size
+            d.tex_nearest, d.bump_shadow = full.get("tex_nearest", 0), full.get("bump_shadow", 0)
     return d, keep
 
 
@@ -267,6 +281,9 @@ SIGNATURES = {
     "cwp_device_free": ([_P], None),
     "cwp_memcpy": ([_P, _P, _S, _I], _I),
     "cwp_bsdf_eval": ([_FP, _FP, _FP, _F, _I, _F, _F, _FP, _FP], _I),
+    "cwp_image_read": ([ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I), _P, _S], _I),
+    "cwp_inflate": ([_P, _S, _P, _S, ctypes.POINTER(_S)], _I),
+    "cwp_texture_sample": ([ctypes.POINTER(_I), _P, _S, _F, _F, _I, _FP], _I),
     "cwp_server_start": ([_P], _I),
     "cwp_job_submit": ([_P, _I, _I, _I, _I, _I, ctypes.c_uint32, ctypes.POINTER(_I)], _I),
     "cwp_job_poll": ([_P, _I, ctypes.POINTER(_I)], _I),
@@ -315,7 +332,10 @@ _ARRAY_LAYOUT = {ARR_VERTICES: (np.float32, (3,)), ARR_TRIANGLES: (np.int32, (3,
                  ARR_TREE_BOX: (np.float32, (6,)), ARR_PRIM: (np.int32, ()), ARR_MESH_ROOTS: (np.int32, ()),
                  ARR_MESH_BOXES: (np.float32, (8,)), ARR_MATERIALS: (np.float32, (16,)), ARR_EM_TRI: (np.int32, ()),
                  ARR_EM_CDF: (np.float32, ()), ARR_TRI_LIGHT_PDF: (np.float32, ()), ARR_ENV_TEX: (np.float32, (4,)),
-                 ARR_ENV_MARG: (np.float32, ()), ARR_ENV_COND: (np.float32, ()), ARR_MESH_TRIS: (np.int32, (2,))}
+                 ARR_ENV_MARG: (np.float32, ()), ARR_ENV_COND: (np.float32, ()), ARR_MESH_TRIS: (np.int32, (2,)),
+                 ARR_UVS: (np.float32, (2, 4)), ARR_TANGENTS: (np.float32, (3, 4)), ARR_MAPS: (np.int32, (8,)),
+                 ARR_TEX_IMAGES: (np.int32, (4,)), ARR_TEXELS: (np.uint32, ()), ARR_SRGB_TABLE: (np.float32, ()),
+                 ARR_MESH_UVS: (np.float32, (6,))}
 
 
 def _view(ptr, count, which, node_floats4=None):
@@ -622,6 +642,33 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def image_read(path):
+    """RGBA8 texels of a PNG / TGA / PPM / PGM file as (h, w) uint32 (r in the low byte), rows top first."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    _check(lib().cwp_image_read(str(path).encode(), ctypes.byref(w), ctypes.byref(h), None, 0))
+    out = np.zeros((h.value, w.value), np.uint32)
+    _check(lib().cwp_image_read(str(path).encode(), ctypes.byref(w), ctypes.byref(h), out.ctypes.data, out.size))
+    return out
+
+
+def inflate(data, capacity):
+    """zlib stream -> bytes (at most capacity)."""
+    src = np.frombuffer(bytes(data), np.uint8)
+    out = np.zeros(max(capacity, 1), np.uint8)
+    n = ctypes.c_size_t()
+    _check(lib().cwp_inflate(src.ctypes.data, src.size, out.ctypes.data, capacity, ctypes.byref(n)))
+    return out[:n.value].tobytes()
+
+
+def texture_sample(image, texels, u, v, nearest=False):
+    """The kernels' texture lookup on the host: image = (first, w, h, srgb), texels uint32; returns rgba (4,)."""
+    im = (ctypes.c_int * 4)(*[int(x) for x in image])
+    tx = np.ascontiguousarray(texels, np.uint32)
+    out = (ctypes.c_float * 4)()
+    _check(lib().cwp_texture_sample(im, tx.ctypes.data, tx.size, float(u), float(v), 1 if nearest else 0, out))
+    return np.array(out[:], np.float32)
 
 
 def bsdf_eval(mat, wo, wi, eta, gd_energy=1, ux1=0.5, ux2=0.5):

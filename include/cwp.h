@@ -94,6 +94,19 @@ typedef struct cwp_scene_desc {
     int rr, rr_depth;           /* Russian roulette from depth rr_depth on */
     int gd_energy;              /* glossy-diffuse energy compensation */
     int env_uniform;            /* environment light samples uniform on the sphere instead of by luminance */
+    /* texture maps (full integrator; the arrays of CWP_ARR_MAPS, CWP_ARR_UVS, CWP_ARR_TANGENTS, CWP_ARR_TEX_IMAGES and
+     * CWP_ARR_TEXELS; reference: RenderOptions / SceneBuffers texture fields, src/render/renderer.h:24, params.h:41).
+     * maps NULL: no textures. */
+    const int* maps;            /* 8 per material: base colour, emission, normal, metallic, roughness image ids (-1:
+                                   none), metallic channel, roughness channel (0..3 = r..a), normal strength (float bits) */
+    const float* uvs;           /* 8 floats per triangle (leaf order): (u0, v0, u1, v1), (u2, v2, 0, 0) */
+    const float* tangents;      /* 12 floats per triangle: per corner (tangent xyz, handedness +-1); needed by normal maps */
+    const int* tex_images;      /* 4 per image: first texel, width, height, 1 = sRGB colour data / 0 = linear */
+    size_t num_images;
+    const uint32_t* texels;     /* RGBA8 texels (r in the low byte), each image's rows top first */
+    size_t num_texels;
+    int tex_nearest;            /* 1: nearest-texel lookups; 0: bilinear */
+    int bump_shadow;            /* 1: terminator shadowing of normal-mapped diffuse shading (Chiang et al. 2019) */
 } cwp_scene_desc;
 enum { CWP_LIGHT_MIS = 0, CWP_LIGHT_NEE = 1, CWP_LIGHT_BSDF = 2 };
 enum { CWP_TRIS_CORNERS = 0, CWP_TRIS_EDGES = 1 };
@@ -132,6 +145,7 @@ typedef struct cwp_scene_info {
     int rr, rr_depth, gd_energy, env_uniform;
     float env_rot_deg;
     int bvh_cache;      /* -1 no cache directory (--bvh_cache), 0 built and saved, 1 loaded from the cache */
+    int textured, num_images, tex_nearest, bump_shadow;   /* texture maps bound (full integrator), images, filter */
 } cwp_scene_info;
 
 /* Instance record of a two-level scene (112 bytes): world-to-object rows, object-to-world rows (3 x 4, row-major), BLAS
@@ -168,7 +182,14 @@ enum {
     CWP_ARR_ENV_TEX = 18,    /* float[4] per environment texel (radiance x strength), row 0 = +z */
     CWP_ARR_ENV_MARG = 19,   /* float per environment row + 1: marginal CDF */
     CWP_ARR_ENV_COND = 20,   /* float per (row, column + 1): conditional CDFs */
-    CWP_ARR_MESH_TRIS = 21   /* int[2] per mesh of a two-level scene: first leaf position, triangle count */
+    CWP_ARR_MESH_TRIS = 21,  /* int[2] per mesh of a two-level scene: first leaf position, triangle count */
+    CWP_ARR_UVS = 22,        /* float[8] per leaf position: corner texture coordinates (u0, v0, u1, v1), (u2, v2, 0, 0) */
+    CWP_ARR_TANGENTS = 23,   /* float[12] per leaf position: per corner (tangent xyz, handedness); empty without normal maps */
+    CWP_ARR_MAPS = 24,       /* int[8] per material: texture bindings (see cwp_scene_desc.maps) */
+    CWP_ARR_TEX_IMAGES = 25, /* int[4] per image: first texel, width, height, sRGB */
+    CWP_ARR_TEXELS = 26,     /* uint32 per texel: RGBA8, r in the low byte, rows top first */
+    CWP_ARR_SRGB_TABLE = 27, /* float[256]: the sRGB decode of each byte value */
+    CWP_ARR_MESH_UVS = 28    /* float[6] per mesh triangle (single-level scenes): corner texture coordinates; empty without */
 };
 
 const char* cwp_last_error(void);
@@ -406,6 +427,19 @@ int cwp_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float 
 /* returns CWP_OK and sets *ok (1: a direction was sampled; pdf 0 marks a delta sample) */
 int cwp_bsdf_sample(const float* mat, const float wo[3], float eta, int gd_energy, float ux1, float ux2, float u, float u1,
                     float u2, float wi[3], float weight[3], float* pdf, int* ok);
+
+/* ---- textures on the host (the readers and the lookup the renderer uses) ----
+ * cwp_image_read: a PNG (every colour type and bit depth, Adam7), TGA (true colour / grey, RLE) or PPM / PGM file as
+ * RGBA8 texels, r in the low byte, rows top first; with rgba NULL only w and h are set. Replaces the reference's
+ * stb_image loads (src/io/assimp_loader.cpp:42-66). */
+int cwp_image_read(const char* path, int* w, int* h, uint32_t* rgba, size_t capacity);
+/* zlib stream -> bytes (at most capacity; CWP_ERR_ARG on malformed input, a checksum mismatch or more output) */
+int cwp_inflate(const uint8_t* in, size_t n, uint8_t* out, size_t capacity, size_t* out_len);
+/* RGBA at texture coordinates (u, v) of an image (first texel, width, height, sRGB) in `texels`: the lookup of the
+ * full-integrator kernels (wrap, nearest or bilinear, sRGB decode). Replaces the reference's tex2D lookups
+ * (src/render/wavefront.cu:203) for host-side checks. */
+int cwp_texture_sample(const int image[4], const uint32_t* texels, size_t num_texels, float u, float v, int nearest,
+                       float rgba[4]);
 
 /* ---- device memory for callers without a GPU array library ---- */
 enum { CWP_COPY_H2D = 1, CWP_COPY_D2H = 2, CWP_COPY_D2D = 3 };

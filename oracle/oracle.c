@@ -1011,9 +1011,20 @@ static f3 fr_f82(f3 F0, f3 tint, float mu) {
 
 /* ---- lobes; local frame, wo.z > 0 */
 typedef struct { f3 f; float pdf; } bval;
-typedef struct { f3 base; float ws, eta, a, a2; int delta; float Fo, pSpec, kd; } gd_t;
-static gd_t gd_make(const orc_mat* m, f3 wo, int energy, float ux1, float ux2) {
+/* bump-terminator shadowing of the diffuse lobe (Chiang, Li and Burley 2019): nr = the unperturbed normal in the local
+ * frame of the mapped shading normal, g = (nr . wi) / (wi.z nr.z), G = clamp(-g^3 + g^2 + g); nr = +z: off
+ * (src/render/bsdf.h:71-75) */
+static float term_shadow(f3 nr, f3 wi) {
+    if (nr.z >= 1.f) return 1.f;
+    float g = dot3(nr, wi) / (wi.z * nr.z);
+    if (g >= 1.f) return 1.f;
+    if (g < 0.f) return 0.f;
+    return g + g * g - g * g * g;
+}
+typedef struct { f3 base; float ws, eta, a, a2; int delta; float Fo, pSpec, kd; f3 nr; } gd_t;
+static gd_t gd_make(const orc_mat* m, f3 wo, int energy, float ux1, float ux2, f3 nr) {
     gd_t g;
+    g.nr = nr;
     g.base = m->base; g.ws = m->ws; g.eta = m->ior; g.a = m->rough * m->rough; g.a2 = g.a * g.a; g.delta = g.a < ORC_DELTA_ALPHA;
     g.Fo = g.ws > 0.f ? g.ws * fr_diel(wo.z, g.eta) : 0.f;
     float pd = (1.f - g.Fo) * lum3(g.base);
@@ -1032,7 +1043,7 @@ static gd_t gd_make(const orc_mat* m, f3 wo, int energy, float ux1, float ux2) {
 static bval gd_eval(const gd_t* g, f3 wo, f3 wi) {
     bval r = { { 0.f, 0.f, 0.f }, 0.f };
     if (wi.z <= 0.f) return r;
-    r.f = scl(g->base, g->kd * ORC_INV_PI * 1.f);
+    r.f = scl(g->base, g->kd * ORC_INV_PI * term_shadow(g->nr, wi));
     r.pdf = (1.f - g->pSpec) * wi.z * ORC_INV_PI;
     if (!g->delta && g->Fo > 0.f) {
         f3 m = nrm3(add(wo, wi));
@@ -1137,24 +1148,24 @@ static int has_non_delta(const orc_mat* m) {
     int rough = m->rough * m->rough >= ORC_DELTA_ALPHA;
     return (wD > 0.f && (lum3(m->base) > 0.f || rough)) || ((wM > 0.f || wT > 0.f) && rough);
 }
-static bval mat_eval(const orc_mat* m, f3 wo, f3 wi, float eta, int gde, float ux1, float ux2) {
+static bval mat_eval(const orc_mat* m, f3 wo, f3 wi, float eta, int gde, float ux1, float ux2, f3 nr) {
     float wM, wT, wD; base_w(m, &wM, &wT, &wD);
     bval r = { { 0.f, 0.f, 0.f }, 0.f };
     if (wM > 0.f) { bval e = cd_eval(m, wo, wi); r.f = add(r.f, scl(e.f, wM)); r.pdf += wM * e.pdf; }
     if (wT > 0.f) { bval e = dl_eval(m, eta, wo, wi); r.f = add(r.f, scl(e.f, wT)); r.pdf += wT * e.pdf; }
-    if (wD > 0.f) { gd_t g = gd_make(m, wo, gde, ux1, ux2); bval e = gd_eval(&g, wo, wi); r.f = add(r.f, scl(e.f, wD)); r.pdf += wD * e.pdf; }
+    if (wD > 0.f) { gd_t g = gd_make(m, wo, gde, ux1, ux2, nr); bval e = gd_eval(&g, wo, wi); r.f = add(r.f, scl(e.f, wD)); r.pdf += wD * e.pdf; }
     return r;
 }
-static int mat_sample(const orc_mat* m, f3 wo, float eta, int gde, float ux1, float ux2, float u, float u1, float u2, f3* wi, f3* w, float* pdf) {
+static int mat_sample(const orc_mat* m, f3 wo, float eta, int gde, float ux1, float ux2, float u, float u1, float u2, f3* wi, f3* w, float* pdf, f3 nr) {
     float wM, wT, wD; base_w(m, &wM, &wT, &wD);
     int ok;
     if (u < wM) ok = cd_sample(m, wo, u1, u2, wi, w, pdf);
     else if (u < wM + wT) { float uu = (u - wM) / wT; ok = dl_sample(m, eta, wo, fminf(uu, ORC_ONE_MINUS), u1, u2, wi, w, pdf); }
-    else { float uu = wD > 0.f ? (u - wM - wT) / wD : 0.f; gd_t g = gd_make(m, wo, gde, ux1, ux2); ok = gd_sample(&g, wo, fminf(uu, ORC_ONE_MINUS), u1, u2, wi, w, pdf); }
+    else { float uu = wD > 0.f ? (u - wM - wT) / wD : 0.f; gd_t g = gd_make(m, wo, gde, ux1, ux2, nr); ok = gd_sample(&g, wo, fminf(uu, ORC_ONE_MINUS), u1, u2, wi, w, pdf); }
     if (!ok) return 0;
     if (*pdf <= 0.f) return 1;
     if (wM == 1.f || wT == 1.f || wD == 1.f) return 1;
-    bval e = mat_eval(m, wo, *wi, eta, gde, ux1, ux2);
+    bval e = mat_eval(m, wo, *wi, eta, gde, ux1, ux2, nr);
     if (e.pdf <= 0.f) return 0;
     *w = scl(e.f, fabsf(wi->z) / e.pdf); *pdf = e.pdf;
     return 1;
@@ -1163,14 +1174,14 @@ static float mis2(float a, float b) { float a2 = a * a; return a2 / (a2 + b * b)
 
 void orc_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gde, float ux1, float ux2, float f[3], float* pdf) {
     orc_mat m = ld_mat(mat);
-    bval e = mat_eval(&m, ld3(wo), ld3(wi), eta, gde, ux1, ux2);
+    bval e = mat_eval(&m, ld3(wo), ld3(wi), eta, gde, ux1, ux2, mk(0.f, 0.f, 1.f));
     f[0] = e.f.x; f[1] = e.f.y; f[2] = e.f.z; *pdf = e.pdf;
 }
 int orc_bsdf_sample(const float* mat, const float wo[3], float eta, int gde, float ux1, float ux2, float u, float u1, float u2,
                     float wi[3], float weight[3], float* pdf) {
     orc_mat m = ld_mat(mat);
     f3 a, w; float p = 0.f;
-    int ok = mat_sample(&m, ld3(wo), eta, gde, ux1, ux2, u, u1, u2, &a, &w, &p);
+    int ok = mat_sample(&m, ld3(wo), eta, gde, ux1, ux2, u, u1, u2, &a, &w, &p, mk(0.f, 0.f, 1.f));
     wi[0] = a.x; wi[1] = a.y; wi[2] = a.z; weight[0] = w.x; weight[1] = w.y; weight[2] = w.z; *pdf = p;
     return ok;
 }
@@ -1187,14 +1198,14 @@ void orc_bsdf_sample_batch(const float* mat, const float wo[3], float eta, int g
     for (int k = 0; k < n; k++) {
         float u = sm_u(&st), u1 = sm_u(&st), u2 = sm_u(&st), x1 = sm_u(&st), x2 = sm_u(&st);
         f3 a = { 0, 0, 0 }, w = { 0, 0, 0 }; float p = 0.f;
-        ok[k] = mat_sample(&m, ld3(wo), eta, gde, x1, x2, u, u1, u2, &a, &w, &p);
+        ok[k] = mat_sample(&m, ld3(wo), eta, gde, x1, x2, u, u1, u2, &a, &w, &p, mk(0.f, 0.f, 1.f));
         wi[3 * k] = a.x; wi[3 * k + 1] = a.y; wi[3 * k + 2] = a.z;
         weight[3 * k] = w.x; weight[3 * k + 1] = w.y; weight[3 * k + 2] = w.z; pdf[k] = p;
     }
 }
 void orc_bsdf_pdf_batch(const float* mat, const float wo[3], float eta, int gde, const float* wi, int n, float* pdf) {
     orc_mat m = ld_mat(mat);
-    for (int k = 0; k < n; k++) pdf[k] = mat_eval(&m, ld3(wo), ld3(wi + 3 * k), eta, gde, 0.5f, 0.5f).pdf;
+    for (int k = 0; k < n; k++) pdf[k] = mat_eval(&m, ld3(wo), ld3(wi + 3 * k), eta, gde, 0.5f, 0.5f, mk(0.f, 0.f, 1.f)).pdf;
 }
 float orc_fresnel_dielectric(float ci, float eta) { return fr_diel(ci, eta); }
 void orc_fresnel_f82(const float f0[3], const float tint[3], float mu, float out[3]) {
@@ -1408,6 +1419,63 @@ int orc_inst_emitters(const float* inst, int n_inst, const int* mesh_tris, const
     return n;
 }
 
+/* ---- textures: the product's lookup (its core/texture.h): coordinates wrap (t - floor t, NaN -> 0), nearest takes
+ * texel floor(t size), bilinear blends the 2 x 2 texels around t size - 0.5 (rows top first, so v is flipped), each
+ * texel decoded first (colour channels of sRGB images through the 256-entry table, otherwise byte / 255); maps
+ * multiply the material's factors; a normal map bends the shading normal in its tangent frame
+ * (src/render/wavefront.cu:199-204, 252-271) */
+typedef struct { float x, y, z, w; } f4;
+static float tex_chan(uint32_t px, int c, int srgb, const float* tab) {
+    uint32_t b = (px >> (8 * c)) & 255u;
+    return (srgb && c < 3) ? tab[b] : (float)b * (1.f / 255.f);
+}
+static f4 tex_texel(uint32_t px, int srgb, const float* tab) {
+    f4 r = { tex_chan(px, 0, srgb, tab), tex_chan(px, 1, srgb, tab), tex_chan(px, 2, srgb, tab), tex_chan(px, 3, srgb, tab) };
+    return r;
+}
+static float lerp1(float a, float b, float t) { return fmaf(t, b - a, a); }
+static f4 lerp4(f4 a, f4 b, float t) { f4 r = { lerp1(a.x, b.x, t), lerp1(a.y, b.y, t), lerp1(a.z, b.z, t), lerp1(a.w, b.w, t) }; return r; }
+static f4 orc_tex_sample_img(const int* img, const uint32_t* texels, const float* tab, float u, float v, int nearest) {
+    const uint32_t first = (uint32_t)img[0];
+    const int w = img[1], h = img[2], srgb = img[3];
+    const uint32_t* px = texels + first;
+    float vv = 1.f - v, tu = u - floorf(u), tv = vv - floorf(vv);
+    if (!(tu >= 0.f && tu <= 1.f)) tu = 0.f;
+    if (!(tv >= 0.f && tv <= 1.f)) tv = 0.f;
+    if (nearest) {
+        int xi = (int)(tu * (float)w), yi = (int)(tv * (float)h);
+        if (xi > w - 1) xi = w - 1;
+        if (yi > h - 1) yi = h - 1;
+        return tex_texel(px[(size_t)yi * w + xi], srgb, tab);
+    }
+    float x = tu * (float)w - 0.5f, y = tv * (float)h - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float ax = x - fx, ay = y - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    if (x0 < 0) x0 += w;
+    if (y0 < 0) y0 += h;
+    int x1 = x0 + 1 < w ? x0 + 1 : 0, y1 = y0 + 1 < h ? y0 + 1 : 0;
+    f4 top = lerp4(tex_texel(px[(size_t)y0 * w + x0], srgb, tab), tex_texel(px[(size_t)y0 * w + x1], srgb, tab), ax);
+    f4 bot = lerp4(tex_texel(px[(size_t)y1 * w + x0], srgb, tab), tex_texel(px[(size_t)y1 * w + x1], srgb, tab), ax);
+    return lerp4(top, bot, ay);
+}
+void orc_texture_sample(const int img[4], const uint32_t* texels, const float* srgb_table, float u, float v, int nearest,
+                        float out[4]) {
+    f4 c = orc_tex_sample_img(img, texels, srgb_table, u, v, nearest);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = c.w;
+}
+static f4 tex_sample(const orc_scene* s, int id, float u, float v) {
+    return orc_tex_sample_img(s->tex_images + 4 * id, s->texels, s->srgb, u, v, s->tex_nearest);
+}
+static float chan4(f4 c, int k) { return k == 0 ? c.x : (k == 1 ? c.y : (k == 2 ? c.z : c.w)); }
+/* texture coordinates of a point of leaf triangle t with barycentric weights b1, b2 of corners 1 and 2 */
+static void tri_uv(const orc_scene* s, int t, float b1, float b2, float* u, float* v) {
+    const float* q = s->uvs + 8 * (size_t)t;
+    float b0 = 1.f - b1 - b2;
+    *u = q[0] * b0 + q[2] * b1 + q[4] * b2;
+    *v = q[1] * b0 + q[3] * b1 + q[5] * b2;
+}
+
 /* ---- one path of the full integrator */
 static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint32_t smp, int bounces, double* st) {
     float o3[3], d3[3];
@@ -1448,10 +1516,42 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
             if (l > 0.f) ns = scl(nn, 1.f / l);
             if (dot3(ns, ng0) < 0.f) ns = neg(ns);
         }
-        orc_mat M = ld_mat(s->mats + 16 * f2i(P[7]));
+        const int mid = f2i(P[7]);
+        orc_mat M = ld_mat(s->mats + 16 * mid);
+        f3 nsU = ns;   /* the shading normal before normal mapping */
+        int mapped = 0;
+        if (s->maps && mid < s->num_maps) {
+            const int* X = s->maps + 8 * mid;
+            if (X[0] >= 0 || X[1] >= 0 || X[2] >= 0 || X[3] >= 0 || X[4] >= 0) {
+                float tu, tv;
+                tri_uv(s, h.tri, h.u, h.v, &tu, &tv);
+                if (X[0] >= 0) { f4 c = tex_sample(s, X[0], tu, tv); M.base = mul(M.base, mk(c.x, c.y, c.z)); }
+                if (X[1] >= 0) { f4 c = tex_sample(s, X[1], tu, tv); M.emis = mul(M.emis, mk(c.x, c.y, c.z)); }
+                if (X[3] >= 0) M.metallic *= chan4(tex_sample(s, X[3], tu, tv), X[5]);
+                if (X[4] >= 0) M.rough *= chan4(tex_sample(s, X[4], tu, tv), X[6]);
+                if (X[2] >= 0 && s->tans) {
+                    const float* T0 = s->tans + 12 * (size_t)h.tri;
+                    float w0 = 1.f - h.u - h.v;
+                    f3 tg = add(add(scl(ld3(T0), w0), scl(ld3(T0 + 4), h.u)), scl(ld3(T0 + 8), h.v));
+                    float sg = (T0[3] * w0 + T0[7] * h.u + T0[11] * h.v) < 0.f ? -1.f : 1.f;
+                    if (XI) { tg = xf_v(XI + 12, tg); sg *= XI[26]; }   /* tangents: forward rows */
+                    tg = sub(tg, scl(ns, dot3(ns, tg)));
+                    float tl = sqrtf(dot3(tg, tg));
+                    if (tl > 0.f) {
+                        tg = scl(tg, 1.f / tl);
+                        f3 bt = scl(crs(ns, tg), sg);
+                        f4 c = tex_sample(s, X[2], tu, tv);
+                        float sc = i2f(X[7]);
+                        f3 nm = add(add(scl(tg, (2.f * c.x - 1.f) * sc), scl(bt, (2.f * c.y - 1.f) * sc)), scl(ns, 2.f * c.z - 1.f));
+                        float nl = sqrtf(dot3(nm, nm));
+                        if (nl > 0.f) { ns = scl(nm, 1.f / nl); mapped = 1; }
+                    }
+                }
+            }
+        }
         int front = dot3(ng0, d) < 0.f;
         f3 ng = front ? ng0 : neg(ng0);
-        if (!front) ns = neg(ns);
+        if (!front) { ns = neg(ns); nsU = neg(nsU); }
         if (emissive(&M)) {
             float w = 1.f;
             if (lastPdf > 0.f) {
@@ -1477,6 +1577,11 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
         if (dot3(ns, wo) <= 1e-6f) ns = ng;
         f3 tx, ty; onb(ns, &tx, &ty);
         f3 woL = mk(dot3(wo, tx), dot3(wo, ty), dot3(wo, ns));
+        f3 nrL = mk(0.f, 0.f, 1.f);   /* unperturbed normal in the local frame (+z: terminator shadowing off) */
+        if (s->bump_shadow && mapped) {
+            nrL = mk(dot3(nsU, tx), dot3(nsU, ty), dot3(nsU, ns));
+            if (!(nrL.z > 0.f)) nrL = mk(0.f, 0.f, 1.f);
+        }
         float eta = front ? M.ior : 1.f / M.ior;
         f3 poUp = add(ph, scl(ng, p->eps)), poDn = sub(ph, scl(ng, p->eps));
         float ux1 = orc_rng4((uint32_t)pix, smp, dd, 7, seed), ux2 = orc_rng4((uint32_t)pix, smp, dd, 7, seed ^ 0x68E31DA4u);
@@ -1507,7 +1612,14 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
                     float cl = fabsf(dot3(nL, l));
                     if (cl > 0.f) {
                         pl = s->p_tri * (s->tlas ? s->mat_light_pdf[f2i(A[7])] : s->tri_light_pdf[kt]) * dist2 / cl;
-                        Le = ld_mat(s->mats + 16 * f2i(A[7])).emis;
+                        const int lm = f2i(A[7]);
+                        Le = ld_mat(s->mats + 16 * lm).emis;
+                        if (s->maps && lm < s->num_maps && s->maps[8 * lm + 1] >= 0) {   /* emission map at the sample */
+                            float tu, tv;
+                            tri_uv(s, s->tlas ? f2i(A[3]) : kt, su * (1.f - u3), su * u3, &tu, &tv);
+                            f4 c = tex_sample(s, s->maps[8 * lm + 1], tu, tv);
+                            Le = mul(Le, mk(c.x, c.y, c.z));
+                        }
                         tmax = dist * (1.f - 1e-3f);
                         ok = 1;
                     }
@@ -1525,7 +1637,7 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
                 f3 lL = mk(dot3(l, tx), dot3(l, ty), dot3(l, ns));
                 float gl = dot3(ng, l);
                 if (lL.z != 0.f && (gl > 0.f) == (lL.z > 0.f)) {
-                    bval e = mat_eval(&M, woL, lL, eta, s->gd_energy, ux1, ux2);
+                    bval e = mat_eval(&M, woL, lL, eta, s->gd_energy, ux1, ux2, nrL);
                     float w = (isSun || s->light_mode == 1) ? 1.f : mis2(pl, e.pdf);
                     f3 con = scl(mul(mul(T, e.f), Le), fabsf(lL.z) * w / pl);
                     if (con.x > 0.f || con.y > 0.f || con.z > 0.f) {
@@ -1540,7 +1652,7 @@ static f3 trace_path_full(const orc_scene* s, const orc_params* p, int pix, uint
         }
         f3 wiL, wgt; float pdf = 0.f;
         if (!mat_sample(&M, woL, eta, s->gd_energy, ux1, ux2, orc_rng4((uint32_t)pix, smp, dd, 4, seed),
-                        orc_rng4((uint32_t)pix, smp, dd, 0, seed), orc_rng4((uint32_t)pix, smp, dd, 1, seed), &wiL, &wgt, &pdf))
+                        orc_rng4((uint32_t)pix, smp, dd, 0, seed), orc_rng4((uint32_t)pix, smp, dd, 1, seed), &wiL, &wgt, &pdf, nrL))
             break;
         f3 wi = nrm3(add(add(scl(tx, wiL.x), scl(ty, wiL.y)), scl(ns, wiL.z)));
         float gi = dot3(ng, wi);

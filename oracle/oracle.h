@@ -69,6 +69,18 @@ typedef struct orc_scene {
      * light pdf per unit area */
     const float* em_tris;
     const float* mat_light_pdf;
+    /* textures (maps != NULL): 8 ints per material (base, emission, normal, metallic, roughness image ids, metallic /
+     * roughness channel, normal strength float bits), corner uvs (8 floats per leaf position), tangents (12 floats per
+     * leaf position, or NULL), images (first texel, w, h, srgb), RGBA8 texels, the 256-entry sRGB decode table,
+     * nearest-texel filter, terminator shadowing of normal-mapped diffuse shading */
+    const int* maps;
+    int num_maps;
+    const float* uvs;
+    const float* tans;
+    const int* tex_images;
+    const uint32_t* texels;
+    const float* srgb;
+    int tex_nearest, bump_shadow;
 } orc_scene;
 
 /* binary SAH tree: node = int[4] (left, right, first, count) + float[6] box (lo, hi); prim = triangle order */
@@ -143,6 +155,9 @@ int orc_trace_work(const orc_scene* s, const float* rays, int n, double* node_vi
 /* ---- full material integrator: src/render/bsdf.h, src/render/wavefront.cu:177-356, src/scene/environment.cpp,
  * src/app/scene_setup.cpp:225-247 ---- */
 /* f (3) and pdf of a material (16 floats) for local directions wo, wi (z = shading normal) */
+/* the texture lookup of the full integrator: image (first texel, w, h, srgb), RGBA8 texels, sRGB decode table */
+void orc_texture_sample(const int img[4], const uint32_t* texels, const float* srgb_table, float u, float v, int nearest,
+                        float out[4]);
 void orc_bsdf_eval(const float* mat, const float wo[3], const float wi[3], float eta, int gd_energy, float ux1, float ux2,
                    float f[3], float* pdf);
 /* returns 1 and wi, weight = f|cos|/pdf, pdf (0: delta) on success */

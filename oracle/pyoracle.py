@@ -36,7 +36,9 @@ class OrcScene(ctypes.Structure):
                 ("env_w", ctypes.c_int), ("env_h", ctypes.c_int), ("env_rot", ctypes.c_float), ("p_sun", ctypes.c_float),
                 ("p_tri", ctypes.c_float), ("p_env", ctypes.c_float), ("light_mode", ctypes.c_int), ("rr", ctypes.c_int),
                 ("rr_depth", ctypes.c_int), ("gd_energy", ctypes.c_int), ("env_uniform", ctypes.c_int),
-                ("em_tris", _FP), ("mat_light_pdf", _FP)]
+                ("em_tris", _FP), ("mat_light_pdf", _FP), ("maps", _IP), ("num_maps", ctypes.c_int), ("uvs", _FP),
+                ("tans", _FP), ("tex_images", _IP), ("texels", ctypes.POINTER(ctypes.c_uint32)), ("srgb", _FP),
+                ("tex_nearest", ctypes.c_int), ("bump_shadow", ctypes.c_int)]
 
 
 _lib = None
@@ -83,6 +85,7 @@ def lib():
         L.orc_tree_from_hploc.argtypes = [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcTree)]
         _F = ctypes.c_float
         L.orc_bsdf_eval.argtypes = [_FP, _FP, _FP, _F, ctypes.c_int, _F, _F, _FP, _FP]
+        L.orc_texture_sample.argtypes = [_IP, ctypes.POINTER(ctypes.c_uint32), _FP, _F, _F, ctypes.c_int, _FP]
         L.orc_bsdf_sample.argtypes = [_FP, _FP, _F, ctypes.c_int, _F, _F, _F, _F, _F, _FP, _FP, _FP]
         L.orc_bsdf_sample.restype = ctypes.c_int
         L.orc_bsdf_sample_batch.argtypes = [_FP, _FP, _F, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _FP, _FP, _FP, _IP]
@@ -327,6 +330,27 @@ class Scene:
         c.light_mode, c.rr, c.rr_depth, c.gd_energy, c.env_uniform = light_mode, rr, rr_depth, gd_energy, env_uniform
         assert nt == len(self.tri_light_pdf)
 
+    def set_textures(self, maps, uvs, tex_images, texels, tans=None, nearest=0, bump_shadow=0):
+        """Texture maps of the full integrator: bindings (m, 8) int32, corner uvs in leaf order (n, 6) or (n, 2, 4),
+        images (k, 4) int32 (first, w, h, srgb), texels uint32, tangents in leaf order (n, 3, 4) or None; the sRGB
+        decode table is the oracle's own (srgb_table)."""
+        self.maps = np.ascontiguousarray(maps, np.int32).reshape(-1, 8)
+        u = np.asarray(uvs, np.float32)
+        if u.ndim == 2 and u.shape[1] == 6:
+            u = np.concatenate([u, np.zeros((len(u), 2), np.float32)], 1)
+        self.uvs = np.ascontiguousarray(u, np.float32).reshape(-1)
+        self.tex_images = np.ascontiguousarray(tex_images, np.int32).reshape(-1)
+        self.texels = np.ascontiguousarray(texels, np.uint32).reshape(-1)
+        self.srgb = srgb_table()
+        self.tans = None if tans is None or len(tans) == 0 else np.ascontiguousarray(tans, np.float32).reshape(-1)
+        c = self.c
+        c.maps = _i(self.maps); c.num_maps = len(self.maps); c.uvs = _f(self.uvs)
+        c.tans = _f(self.tans) if self.tans is not None else None
+        c.tex_images = _i(self.tex_images)
+        c.texels = self.texels.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        c.srgb = _f(self.srgb)
+        c.tex_nearest, c.bump_shadow = int(nearest), int(bump_shadow)
+
     def set_inst_emitters(self, mesh_tris, order):
         """Two-level full-integrator scene (after set_instances and set_full): the frame's world emitters, computed
         by the oracle from the instance records and the BLAS corners; returns (em_tris (n, 3, 4), em_cdf, mlp)."""
@@ -479,9 +503,91 @@ def scene_from_setup(setup, tri_test=None, bins=16):
                            _i(np.ascontiguousarray(mats, dtype=np.int32)), _i(order), len(order), _f(corners))
     b["corners"] = corners
     tt = info["tri_test"] if tri_test is None else tri_test
-    sc = Scene(info["variant"], tt, b["nodes"], b["woop"], corners, alb)
+    nrm = None
+    if setup.params().has_normals:   # smooth shading: the vertex normals at each corner, in the oracle's leaf order
+        vn = setup.array(cwp.ARR_NORMALS)
+        nrm = np.zeros((len(order), 3, 4), np.float32)
+        nrm[..., :3] = vn[np.asarray(t, np.int64)[order]]
+    sc = Scene(info["variant"], tt, b["nodes"], b["woop"], corners, alb, normals=nrm)
     if info.get("full"):
         env = setup.array(cwp.ARR_ENV_TEX) if info["env_w"] > 0 else None
         sc.set_full(setup.array(cwp.ARR_MATERIALS), order, info["sun_on"], info["light_mode"], info["rr"], info["rr_depth"],
                     info["gd_energy"], info["env_uniform"], env, info["env_w"], info["env_h"], info["env_rot_deg"])
+    if info.get("textured"):
+        # per-triangle corner uvs in mesh order, put into the oracle's leaf order; tangents computed by the oracle
+        muv = setup.array(cwp.ARR_MESH_UVS).reshape(-1, 6)
+        maps = setup.array(cwp.ARR_MAPS)
+        tans = None
+        if (maps[:, 2] >= 0).any():
+            n = setup.array(cwp.ARR_NORMALS) if setup.params().has_normals else None
+            tans = corner_tangents(v, n, t, muv)[order]
+        sc.set_textures(maps, muv[order], setup.array(cwp.ARR_TEX_IMAGES), setup.array(cwp.ARR_TEXELS), tans,
+                        info["tex_nearest"], info["bump_shadow"])
     return sc, b
+
+
+def srgb_table():
+    """sRGB decoding of the byte values (IEC 61966-2-1 EOTF in double precision, rounded to float32)."""
+    c = np.arange(256, dtype=np.float64) / 255.0
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4).astype(np.float32)
+
+
+def corner_tangents(verts, normals, tris, uv):
+    """Tangent frames per triangle corner (n, 3, 4) from texture-coordinate derivatives (Lengyel 2001), float32:
+    per triangle dP/du and dP/dv summed at its three vertices in triangle order, made orthogonal to the vertex normal
+    (the given one, or the sum of the faces' cross(e1, e2)); handedness +1 / -1 from the sign of cross(n, t) . dP/dv;
+    a vertex with no usable derivative gets cross(a, n) normalised, a = x unless |n.x| >= 0.9, then y."""
+    v = np.asarray(verts, np.float32)
+    f = np.asarray(tris, np.int64)
+    q = np.asarray(uv, np.float32).reshape(-1, 6)
+    nv = len(v)
+
+    def dot(a, b):
+        return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2]
+
+    def cross(a, b):
+        return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                         a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1)
+
+    def normalize(a):
+        ln = np.sqrt(dot(a, a))
+        inv = np.where(ln > 0, np.float32(1) / np.where(ln > 0, ln, np.float32(1)), np.float32(1)).astype(np.float32)
+        return (a * inv[:, None]).astype(np.float32)
+
+    e1, e2 = v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]
+    du1, dv1, du2, dv2 = q[:, 2] - q[:, 0], q[:, 3] - q[:, 1], q[:, 4] - q[:, 0], q[:, 5] - q[:, 1]
+    det = du1 * dv2 - du2 * dv1
+    ok = np.abs(det) >= np.float32(1e-20)
+    r = (np.float32(1) / np.where(ok, det, np.float32(1))).astype(np.float32)
+    dpdu = ((e1 * dv2[:, None] - e2 * dv1[:, None]) * r[:, None]).astype(np.float32)
+    dpdv = ((e2 * du1[:, None] - e1 * du2[:, None]) * r[:, None]).astype(np.float32)
+    # sums in the order triangle 0 corners 0, 1, 2, triangle 1 corners 0, 1, 2, ...
+    su, sv = np.zeros((nv, 3), np.float32), np.zeros((nv, 3), np.float32)
+    fo = f[ok].reshape(-1)
+    np.add.at(su, fo, np.repeat(dpdu[ok], 3, axis=0))
+    np.add.at(sv, fo, np.repeat(dpdv[ok], 3, axis=0))
+    if normals is not None and len(normals) == nv:
+        n = np.asarray(normals, np.float32)
+    else:
+        n = np.zeros((nv, 3), np.float32)
+        np.add.at(n, f.reshape(-1), np.repeat(cross(e1, e2).astype(np.float32), 3, axis=0))
+    n = normalize(n)
+    tg = (su - n * dot(n, su)[:, None]).astype(np.float32)
+    bad = dot(tg, tg) < np.float32(1e-24)
+    a = np.where((np.abs(n[:, 0]) < np.float32(0.9))[:, None], np.float32([1, 0, 0]), np.float32([0, 1, 0])).astype(np.float32)
+    tg[bad] = cross(a[bad], n[bad])
+    tg = normalize(tg)
+    w = np.where(dot(cross(n, tg), sv) < 0, np.float32(-1), np.float32(1)).astype(np.float32)
+    per = np.concatenate([tg, w[:, None]], 1)
+    return per[f].astype(np.float32)
+
+
+def texture_sample(img, texels, u, v, nearest=False, table=None):
+    """The oracle's texture lookup (rgba)."""
+    im = np.ascontiguousarray(img, np.int32)
+    tx = np.ascontiguousarray(texels, np.uint32)
+    tab = srgb_table() if table is None else np.ascontiguousarray(table, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().orc_texture_sample(_i(im), tx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _f(tab), float(u), float(v),
+                             1 if nearest else 0, _f(out))
+    return out
