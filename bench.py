@@ -261,6 +261,26 @@ def bench_variant_frame(args, width, height, tri):
             "bvh": "bvh8", "bounces": args.bounces, "spp": 256}
 
 
+def bench_full_frame(args, textured):
+    """One frame of the terrain with the full integrator (OpenPBR-subset materials on the boxes, 2048 x 1024
+    log-normal environment, MIS), 1280x720, 64 spp in flight, 8 bounces, graph replay; with `textured`, generated
+    1024 x 1024 base-colour, normal and roughness / metalness maps on every material: Grays/s after a warm-up frame."""
+    import cwp
+    a = workload_args(1280, 720, 64, args.bounces, "woop", "bvh8", args.stack) + ["--gen_materials", "5", "--gen_env",
+                                                                                  "2048,1024,2,1"]
+    if textured:
+        a += ["--gen_textures", "1024"]
+    s = cwp.Setup(a)
+    r = cwp.Renderer(s)
+    r.render(64)
+    st = r.render(64)
+    r.close()
+    g = (st["rays_closest"] + st["rays_shadow"]) / (st["t_frame_ms"] * 1e-3) / 1e9
+    print("bench: full integrator%s: %.3f Grays/s" % (" + textures" if textured else "", g), file=sys.stderr, flush=True)
+    return {"grays": round(g, 4), "ms": round(st["t_frame_ms"], 3), "width": 1280, "height": 720, "spp": 64,
+            "bounces": args.bounces, "textures": "3 maps of 1024^2 per material" if textured else "none"}
+
+
 def job_stream_scenes(n_scenes, seed, lo_tris=10_000, hi_tris=2_000_000, full=False):
     """Seeded scenes with triangle counts log-uniform in [lo_tris, hi_tris]: fBm heightfields of (G-1)^2 * 2
     triangles, each with its own terrain seed. With `full`, 64 boxes on each terrain carry OpenPBR-subset materials
@@ -549,7 +569,9 @@ def main():
         out["reference_config"] = bench_reference_config(args)
     if not args.no_extras:
         out["variants"] = {"watertight_1280x720": bench_variant_frame(args, 1280, 720, "watertight"),
-                           "woop_512x512": bench_variant_frame(args, 512, 512, "woop")}
+                           "woop_512x512": bench_variant_frame(args, 512, 512, "woop"),
+                           "full_1280x720": bench_full_frame(args, False),
+                           "full_textured_1280x720": bench_full_frame(args, True)}
     if not args.no_jobs:
         print("bench: mixed job stream", file=sys.stderr, flush=True)
         j = bench_jobs(args)
