@@ -39,11 +39,12 @@ def algorithmic_bytes(st, npaths, sort=0):
 
     Per camera ray: 16 B radiance write. Per continuation ray traced: 48 B origin/direction/throughput read at its
     queue slot (the path id rides in the origin's w). Per continuation-ray miss: 32 B radiance read-modify-write. Per
-    continuation produced: 48 B ray write at its slot (+ 1 B direction-bucket key with --sort 1). Per shadow ray
-    produced: 48 B shadow record write."""
+    continuation produced: 48 B ray write at its slot (+ 1 B bucket key with --sort 1; with --sort 2, + 1 B for each
+    camera ray's key, written whether or not the ray continues). Per shadow ray produced: 48 B shadow record write."""
     cont_traced = st["rays_closest"] - npaths
     cont_miss = st["rays_miss"] - st["camera_miss"]
-    return (16.0 * npaths + 48.0 * cont_traced + 32.0 * cont_miss + (49.0 if sort else 48.0) * st["rays_cont"] +
+    keys = st["rays_cont"] if sort == 1 else (npaths if sort == 2 else 0)
+    return (16.0 * npaths + 48.0 * cont_traced + 32.0 * cont_miss + 48.0 * st["rays_cont"] + 1.0 * keys +
             48.0 * st["rays_shadow"])
 
 
@@ -235,7 +236,7 @@ def bench_reference_config(args):
     import cwp
     out = {"bounces": 4, "spp": 256, "width": 1280, "height": 720}
     for variant in ("bvh2", "bvh8"):
-        s = cwp.Setup(workload_args(1280, 720, 256, 4, "woop", variant, args.stack))
+        s = cwp.Setup(workload_args(1280, 720, 256, 4, "woop", variant, args.stack, sort=args.sort))
         r = cwp.Renderer(s)
         r.render(256)
         st = r.render(256)
@@ -250,7 +251,7 @@ def bench_variant_frame(args, width, height, tri):
     """One frame of the terrain workload at another resolution or triangle test (8 bounces, BVH8, 256 spp in flight),
     after a warm-up frame: Grays/s."""
     import cwp
-    s = cwp.Setup(workload_args(width, height, 256, args.bounces, tri, "bvh8", args.stack))
+    s = cwp.Setup(workload_args(width, height, 256, args.bounces, tri, "bvh8", args.stack, sort=args.sort))
     r = cwp.Renderer(s)
     r.render(256)
     st = r.render(256)
@@ -436,7 +437,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, help="lanes refilled with new rays mid-traversal: 1 split "
                     "trace / shade kernels, 2 the fused kernel")
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
-    ap.add_argument("--sort", type=int, default=0, help="1: group each bounce's continuations by direction bucket")
+    ap.add_argument("--sort", type=int, default=2, help="continuation order: 0 as appended; 1 every bounce's grouped by "
+                    "direction bucket per partition; 2 the first bounce's grouped by elevation within each pixel")
     ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve", "hploc"],
                     help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
@@ -498,6 +500,8 @@ def main():
             out["serve"] = bench_serve(args)
         print(json.dumps(out), flush=True)
         return
+    if args.sort == 2 and (args.fuse == 0 or args.persistent):
+        args.sort = 0   # the first-bounce sort runs with the fused kernel only
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
                                 args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill,

@@ -386,7 +386,7 @@ int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* p, const cwp_opt
     O.persistent = o->persistent; O.shadowRefill = o->shadow_refill; O.stackLds = o->stack_lds;
     O.blocksPerCU = o->blocks_per_cu; O.bounceWaves = o->bounce_waves; O.timeKernels = o->time_kernels; O.graph = o->graph;
     O.sort = o->sort;
-    if (O.sort != 0 && O.sort != 1) return fail(CWP_ERR_ARG, "sort must be 0 or 1");
+    if (O.sort < 0 || O.sort > 2) return fail(CWP_ERR_ARG, "sort must be 0, 1 or 2");
     if (O.batch < 1) return fail(CWP_ERR_ARG, "batch must be >= 1");
     if (O.fuse != 0 && O.fuse != 1) return fail(CWP_ERR_ARG, "fuse must be 0 or 1");
     if (O.persistent && O.fuse) return fail(CWP_ERR_ARG, "persistent needs fuse 0");

@@ -112,6 +112,49 @@ __global__ void __launch_bounds__(kSortBlock) k_qscatter(DevPaths P, int depth) 
     }
 }
 
+// First-bounce sort. The depth-0 launch left each camera ray's continuation at the ray's own slot of the staging rays
+// with its elevation bucket (kNoKey: none). Each tile of kFirstTile slots of partition x's camera rays (at the
+// headline's 256 samples per pixel, one pixel: its bounce rays share an origin) counts its continuations per bucket
+// in LDS, reserves their total in the partition's depth-1 queue with one atomic, and writes them there grouped by
+// bucket, so that a depth-1 wave holds rays of one pixel that climb or descend alike and need similar traversals.
+__global__ void __launch_bounds__(kFirstTile) k_qsort_first(DevPaths P) {
+    __shared__ int h[kBuckets], first[kBuckets];
+    __shared__ int base;
+    const int t = (int)threadIdx.x;
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    const int* cnt0 = P.cnt;
+    const int start = partStart(cnt0, part), n = ctrv(cnt0, part, kCQueue);
+    int* queue = ctr(P.cnt + kCntStride, part, kCQueue);
+    for (int tile = lw; tile * kFirstTile < n; tile += nw) {   // block-uniform bound: every thread meets the barriers
+        if (t < kBuckets) h[t] = 0;
+        __syncthreads();
+        const int k = tile * kFirstTile + t;
+        int key = kNoKey, rank = 0;
+        if (k < n) {
+            key = P.qk[start + k];
+            if (key != kNoKey) rank = atomicAdd(&h[key & (kBuckets - 1)], 1);
+        }
+        __syncthreads();
+        if (t < kBuckets) {   // exclusive prefix over the first wave's 64 lanes; the last lane reserves the total
+            const int c = h[t];
+            int s = c;
+#pragma unroll
+            for (int o = 1; o < kBuckets; o <<= 1) {
+                const int y = __shfl_up(s, o, kBuckets);
+                if (t >= o) s += y;
+            }
+            first[t] = s - c;
+            if (t == kBuckets - 1) base = start + (s > 0 ? atomicAdd(queue, s) : 0);
+        }
+        __syncthreads();
+        if (key != kNoKey) {
+            const int from = start + k, to = base + first[key & (kBuckets - 1)] + rank;
+            P.O[1][to] = P.stO[from]; P.D[1][to] = P.stD[from]; P.T[1][to] = P.stT[from];
+        }
+        __syncthreads();   // the next tile clears h
+    }
+}
+
 // zero the pass counters; the depth-0 queue of partition x is the x-th of kParts equal slices of the path ids
 __device__ void passBegin(int* cnt, int words, long long NK) {
     for (int j = threadIdx.x; j < words; j += blockDim.x) cnt[j] = 0;

@@ -153,7 +153,12 @@ struct DevPaths {
     Vec4 *stO = nullptr, *stD = nullptr, *stT = nullptr;
     uint8_t* qk = nullptr;
     int* bcnt = nullptr;
+    // first-bounce sort (the pass's depth-0 launch only): each camera ray's continuation goes to the staging rays at
+    // its own slot with its elevation bucket in qk (kNoKey: none), and k_qsort_first groups each tile of kFirstTile
+    // slots, a pixel's samples, by bucket into the depth-1 queue
+    int firstSort = 0;
 };
+constexpr uint8_t kNoKey = 255;
 
 // Continuation sort: 64 direction buckets, an 8 x 8 grid over the octahedral map of the unit sphere (upper hemisphere
 // in the central square, the lower one folded onto the corners), so that a wave of one bucket shares a narrow cone of
@@ -170,6 +175,12 @@ __device__ __forceinline__ int dirBucket(Vec3 d) {
     // fmaxf first: a NaN component lands in bucket 0
     const int iu = (int)fminf(fmaxf((u + 1.f) * 4.f, 0.f), 7.f), iv = (int)fminf(fmaxf((v + 1.f) * 4.f, 0.f), 7.f);
     return iu + 8 * iv;
+}
+
+// Elevation bucket: the direction's z (the world is z-up) in 64 equal steps. Among the bounce rays leaving one point,
+// those heading down into the geometry traverse much longer than those heading up and out.
+__device__ __forceinline__ int elevBucket(Vec3 d) {
+    return (int)fminf(fmaxf((d.z + 1.f) * 32.f, 0.f), 63.f);   // fmaxf first: NaN lands in bucket 0
 }
 
 template <int V> struct StackEntry;
@@ -505,6 +516,14 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
                 res = shadeFull<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
             } else {
                 res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+            }
+            if (P.firstSort) {   // depth 0: the continuation waits at its own slot for k_qsort_first
+                P.qk[slot] = res.cont ? (uint8_t)elevBucket(res.nd) : kNoKey;
+                if (res.cont) {
+                    P.stO[slot] = v4(res.po, asFloat(i)); P.stD[slot] = v4(res.nd, res.pdf);
+                    P.stT[slot] = v4(res.Tn, asFloat(1));
+                }
+                res.cont = false;
             }
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
@@ -1083,6 +1102,8 @@ __global__ void k_pass_begin(int* cnt, int words, long long NK);
 constexpr int kSortBlock = 256, kSortPer = 16, kSortTile = kSortBlock * kSortPer;
 __global__ void k_qcount(DevPaths P, int depth);
 __global__ void k_qscatter(DevPaths P, int depth);
+constexpr int kFirstTile = 256;   // k_qsort_first: slots per tile, one per thread
+__global__ void k_qsort_first(DevPaths P);
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);
 __global__ void k_fill(uint32_t* p, size_t words, uint32_t v);

@@ -351,6 +351,8 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         !I.alloc(I.accum, (size_t)I.N, err) || !I.alloc(I.tot, 8, err))
         return false;
     if (!O.fuse && !I.alloc(I.P.H, NK, err)) return false;
+    if (O.sort < 0 || O.sort > 2) { err = "sort must be 0, 1 or 2"; return false; }
+    if (O.sort == 2 && (!O.fuse || O.persistent)) { err = "sort 2 (first-bounce sort) needs the fused kernel (fuse 1, persistent 0)"; return false; }
     if (O.sort) {
         if (!I.alloc(I.P.stO, NK, err) || !I.alloc(I.P.stD, NK, err) || !I.alloc(I.P.stT, NK, err) ||
             !I.alloc(I.P.qk, NK, err)) return false;
@@ -393,10 +395,13 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
         const int words = (int)I.counterWords();
         for (int s = sBase; s < sBase + spp; s += I.K) {
             k_pass_begin<<<1, 64, 0, I.st>>>(I.P.cnt, words, I.NK);
+            // first-bounce sort: the depth-0 launch stages its continuations, the later ones append directly
+            DevPaths P0 = I.P, Pn = I.P;
+            if (I.O.sort == 2) { P0.firstSort = 1; Pn.qk = nullptr; }
             for (int depth = 0; depth < I.depths; depth++) {
                 hipEvent_t a = nullptr, b = nullptr;
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-                I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
+                I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
                 if (I.shade) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
                     I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
@@ -408,7 +413,11 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                 I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
                 if (timed) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
-                if (I.P.qk && depth + 1 < I.depths) {
+                if (I.O.sort == 2 && depth == 0 && depth + 1 < I.depths) {
+                    if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
+                    k_qsort_first<<<I.gridSort, kFirstTile, 0, I.st>>>(P0);
+                    if (timed) { HIPCHK(hipEventRecord(b, I.st)); sortEv.push_back({ a, b }); }
+                } else if (I.O.sort == 1 && depth + 1 < I.depths) {
                     if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                     k_qcount<<<I.gridSort, kSortBlock, 0, I.st>>>(I.P, depth);
                     k_qscatter<<<I.gridSort, kSortBlock, 0, I.st>>>(I.P, depth);

@@ -141,7 +141,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         err = "--persistent must be 0, 1 (split kernels with refilled lanes) or 2 (fused kernel with refilled lanes)"; return false;
     }
     if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
-    if (A.sort != 0 && A.sort != 1) { err = "--sort must be 0 or 1"; return false; }
+    if (A.sort < 0 || A.sort > 2) { err = "--sort must be 0, 1 or 2"; return false; }
     if (A.fuse < 0) A.fuse = 1;
     if (A.persistent == 1) A.fuse = 0;   // persistent traversal 1 runs in the split trace kernel
     if (A.persistent == 2) A.fuse = 1;   // 2: the fused trace+shade kernel with refilled lanes

@@ -124,8 +124,10 @@ typedef struct cwp_options {
     int bounce_waves;    /* fused kernel register budget: at least this many waves per SIMD (1, 6 or 8) */
     int time_kernels;    /* per-launch event timing (t_bounce_ms, t_shade_ms, t_shadow_ms) */
     int graph;           /* 1: each frame replays a captured HIP graph (per-launch timing off) */
-    int sort;            /* 1: each bounce's continuations are grouped by direction bucket before they are traced
-                            (reference option --sort, src/render/wavefront.cu:717; a different sort, same results) */
+    int sort;            /* continuations reordered before they are traced (reference option --sort,
+                            src/render/wavefront.cu:717; different sorts, same results): 1 every bounce's, grouped by
+                            direction bucket within each XCD partition; 2 the first bounce's, grouped by elevation
+                            bucket within each pixel's samples (fused kernel only) */
 } cwp_options;
 
 /* Scene and BVH summary of a setup. */

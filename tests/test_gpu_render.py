@@ -182,19 +182,23 @@ SORT_CASES = [
     ("boxes-full", SCENES["boxes"] + ["--gen_materials", "3", "--gen_env", "64,32,2,1", "--graph", "0"]),
     ("instances", ["--gen", "instances", "--gen_small", "1", "--gen_instances", "40", "--width", "96", "--height", "64"]),
     ("terrain-graph", SCENES["terrain"] + ["--graph", "1"]),
+    ("terrain-256spp", SCENES["terrain"] + ["--variant", "bvh8", "--width", "48", "--height", "32"]),
 ]
 
 
 @pytest.mark.parametrize("name,args", SORT_CASES, ids=[c[0] for c in SORT_CASES])
 def test_sorted_continuations_give_the_same_image(name, args):
-    """--sort 1 regroups each bounce's continuations by direction bucket before they are traced. Every path keeps its
-    own state and radiance, so the image and every counter are bit-identical to the unsorted schedule (a lost, doubled
-    or misplaced path id would change both), on the first frame and on a second frame that reuses the counters."""
+    """--sort 1 regroups each bounce's continuations by direction bucket within each partition, --sort 2 the first
+    bounce's by elevation bucket within each pixel's samples (fused kernel only), before they are traced. Every path
+    keeps its own state and radiance, so the image and every counter are bit-identical to the unsorted schedule (a
+    lost, doubled or misplaced path id would change both), on the first frame and on a second frame that reuses the
+    counters."""
     print("case", name, flush=True)
-    spp = 16
-    base = args + ["--batch", "8", "--bounces", "6", "--spp", str(spp)]
+    spp, batch = (256, 256) if name == "terrain-256spp" else (16, 8)   # 256: a sort tile is one pixel's samples
+    base = args + ["--batch", str(batch), "--bounces", "6", "--spp", str(spp)]
     out = {}
-    for sort in (0, 1):
+    fused = "--fuse" not in args and "--persistent" not in args
+    for sort in (0, 1, 2) if fused else (0, 1):
         r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort)]))
         frames = []
         for f in range(2):
@@ -206,13 +210,14 @@ def test_sorted_continuations_give_the_same_image(name, args):
         j = r.frame_json(frames[-1][1])
         assert j["sort"] == sort
         timed = "--graph" not in args or "0" == args[args.index("--graph") + 1]
-        assert (j["t_sort_ms"] > 0) == (sort == 1 and timed)
+        assert (j["t_sort_ms"] > 0) == (sort > 0 and timed)
         r.close()
-    for f in range(2):
-        (a, sa), (b, sb) = out[0][f], out[1][f]
-        np.testing.assert_array_equal(b, a, err_msg="frame %d" % f)
-        for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
-            assert sa[k] == sb[k], (f, k)
+    for sort in sorted(out)[1:]:
+        for f in range(2):
+            (a, sa), (b, sb) = out[0][f], out[sort][f]
+            np.testing.assert_array_equal(b, a, err_msg="sort %d frame %d" % (sort, f))
+            for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
+                assert sa[k] == sb[k], (sort, f, k)
     assert out[0][0][1]["rays_cont"] > 0
 
 

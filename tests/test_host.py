@@ -46,7 +46,7 @@ def test_version_and_error_string():
     with pytest.raises(cwp.CwpError, match="--persistent"):
         cwp.Setup(["--gen", "boxes", "--persistent", "3"])
     with pytest.raises(cwp.CwpError, match="--sort"):
-        cwp.Setup(["--gen", "boxes", "--sort", "2"])
+        cwp.Setup(["--gen", "boxes", "--sort", "3"])
     with pytest.raises(cwp.CwpError, match="experimental"):
         cwp.Setup(["--gen", "boxes", "--devloop", "1"])
     with pytest.raises(cwp.CwpError, match="--morton_bits"):
