@@ -1,4 +1,4 @@
-"""Builds libcwp.so (C++17 host code + gfx950 HIP kernels) and the `pt` CLI in-tree.
+"""Builds libcwp.so (C++17 host code + gfx950 HIP kernels), the `pt` CLI and the headless `viewer` in-tree.
 
 Host sources compile with g++, kernels with hipcc for gfx950 only. Both use -ffp-contract=off: fused multiply-adds
 are written explicitly in the sources, so host and device evaluate the same floating-point operations. Packed FP32
@@ -19,11 +19,12 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libcwp.so")
 CLI = os.path.join(PKG, "pt")
+VIEWER = os.path.join(PKG, "viewer")
 
 HOST_SOURCES = ["host/mesh.cpp", "host/camera.cpp", "host/io.cpp", "host/texture.cpp", "host/bvh.cpp", "host/scene_check.cpp", "host/instances.cpp", "host/envmap.cpp", "host/bvhcache.cpp", "host/setup.cpp", "host/serve.cpp", "capi.cpp"]
 HIP_SOURCES = ["device/renderer.hip", "device/common.hip", "device/kernels_v2.hip", "device/kernels_v4.hip", "device/kernels_v8.hip",
-               "device/hploc.hip", "device/collapse.hip", "device/batch.hip"]
-CLI_SOURCES = ["app/pt.cpp"]
+               "device/hploc.hip", "device/collapse.hip", "device/batch.hip", "device/display.hip"]
+CLI_SOURCES = ["app/pt.cpp", "app/viewer.cpp"]
 
 INCLUDES = ["-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
@@ -89,9 +90,10 @@ def build(jobs=8, quiet=False, incremental=False):
     tmp = LIB + ".tmp"
     _run([hc, "-shared", "-fPIC", "-o", tmp] + objs, quiet)
     os.replace(tmp, LIB)
-    cli_obj = os.path.join(BUILD, "app_pt.cpp.o")
-    _run([hc, "-o", CLI + ".tmp", cli_obj, "-L" + PKG, "-lcwp", "-Wl,-rpath,$ORIGIN"], quiet)
-    os.replace(CLI + ".tmp", CLI)
+    for src, exe in (("app/pt.cpp", CLI), ("app/viewer.cpp", VIEWER)):
+        cli_obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
+        _run([hc, "-o", exe + ".tmp", cli_obj, "-L" + PKG, "-lcwp", "-Wl,-rpath,$ORIGIN"], quiet)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -4,8 +4,10 @@
 #include "device/renderer.h"
 #include "host/bvh.h"
 #include "host/envmap.h"
+#include "host/io.h"
 #include "host/scene_check.h"
 #include "device/collapse.h"
+#include "device/display.h"
 #include "device/hploc.h"
 #include "host/serve.h"
 #include "host/setup.h"
@@ -247,6 +249,24 @@ int cwp_read_accum(cwp_ctx* c, float* rgba, size_t npix) {
     if (!c->R.readAccum(rgba, npix, err)) return fail(CWP_ERR_ARG, err);
     return CWP_OK;
 }
+
+const float* cwp_accum_device(cwp_ctx* c) { return c ? (const float*)c->R.accumDevice() : nullptr; }
+
+int cwp_display(cwp_ctx* c, void* rgba8, int spp, float exposure_ev, int mode) {
+    if (!c || !rgba8) return fail(CWP_ERR_ARG, "cwp_display: null argument");
+    std::string err;
+    if (!c->R.display((uint32_t*)rgba8, spp, exposure_ev, mode, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_display_buffer(const float* accum, void* rgba8, int n, int spp, float exposure_ev, int mode) {
+    std::string err;
+    if (!displaySync((const Vec4*)accum, (uint32_t*)rgba8, n, spp, exposure_ev, mode, err)) return fail(CWP_ERR_ARG, err);
+    return CWP_OK;
+}
+
+int cwp_srgb_encode8(float x) { return (int)encodeSRGB8(x); }
+void cwp_srgb_steps(float steps[256]) { if (steps) srgbSteps(steps); }
 
 int cwp_set_params(cwp_ctx* c, const cwp_params* p) {
     if (!c || !p) return fail(CWP_ERR_ARG, "cwp_set_params: null argument");

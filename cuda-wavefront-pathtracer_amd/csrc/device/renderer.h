@@ -98,6 +98,12 @@ public:
     bool init(const SceneUpload& S, const FrameParams& P, const RenderOptions& O, std::string& err);
     bool render(int spp, int sBase, bool clear, FrameStats& st, std::string& err);
     bool readAccum(float* rgba, size_t nPix, std::string& err);
+    // the same sums on the device (w * h float4), valid for the renderer's lifetime; read them after a render has
+    // finished (render() returns once its frame is done)
+    const Vec4* accumDevice() const;
+    // The viewer's display transform of the accumulation (device/display.h) into out (device, w * h packed RGBA8),
+    // ordered after the queued frames; returns when it is done.
+    bool display(uint32_t* out, int spp, float exposureEV, int mode, std::string& err);
     bool setParams(const FrameParams& P, std::string& err);
     // Closest (anyHit = false) or any hit of n rays; rays = 2 float4 per ray (o, tmin), (d, tmax); hits = float4
     // (t, u, v, leaf position as int bits, -1 on a miss). Both are device pointers; synchronous.

@@ -20,6 +20,7 @@
 // uniform cone sample with contribution T*albedo*sunE*cos/pi, cosine-weighted continuation rejected below the
 // geometric surface, throughput *= albedo.
 #include "device/collapse.h"
+#include "device/display.h"
 #include "device/hploc.h"
 #include "device/kernels.h"
 #include "host/bvh.h"
@@ -473,6 +474,14 @@ bool Renderer::readAccum(float* rgba, size_t nPix, std::string& err) {
     if (nPix < (size_t)p_->N) { err = "accumulation buffer too small"; return false; }
     HIPCHK(hipStreamSynchronize(p_->st));
     HIPCHK(hipMemcpy(rgba, p_->accum, (size_t)p_->N * 16, hipMemcpyDeviceToHost));
+    return true;
+}
+
+const Vec4* Renderer::accumDevice() const { return p_->accum; }
+
+bool Renderer::display(uint32_t* out, int spp, float exposureEV, int mode, std::string& err) {
+    if (!displayLaunch(p_->accum, out, p_->N, spp, exposureEV, mode, (void*)p_->st, err)) return false;
+    HIPCHK(hipStreamSynchronize(p_->st));
     return true;
 }
 

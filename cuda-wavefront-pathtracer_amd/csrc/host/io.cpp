@@ -725,11 +725,27 @@ bool writePFM(const std::string& path, int w, int h, const float* rgb) {
     fclose(fp);
     return true;
 }
+uint8_t encodeSRGB8(float x) {
+    x = fminf(fmaxf(x, 0.f), 1.f);
+    x = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+    return (uint8_t)(x * 255.f + 0.5f);
+}
 void tonemapSRGB8(const float* lin, size_t nPix, uint8_t* rgb) {
-    for (size_t i = 0; i < nPix * 3; i++) {
-        float x = fminf(fmaxf(lin[i], 0.f), 1.f);
-        x = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
-        rgb[i] = (uint8_t)(x * 255.f + 0.5f);
+    for (size_t i = 0; i < nPix * 3; i++) rgb[i] = encodeSRGB8(lin[i]);
+}
+void srgbSteps(float t[256]) {
+    t[0] = -INFINITY;
+    uint32_t lo = 0;                       // bits of 0.0f: the search range of each step starts at the previous one
+    const uint32_t one = 0x3F800000u;      // bits of 1.0f, which encodes to 255
+    for (int k = 1; k < 256; k++) {
+        uint32_t a = lo, b = one;          // invariant: encode(bits b) >= k
+        while (a < b) {
+            const uint32_t m = a + (b - a) / 2;
+            float x; memcpy(&x, &m, 4);
+            if (encodeSRGB8(x) >= k) b = m; else a = m + 1;
+        }
+        memcpy(&t[k], &a, 4);
+        lo = a;
     }
 }
 

@@ -48,5 +48,10 @@ bool loadMesh(const std::string& path, Mesh& m);   // by extension
 bool writePNG(const std::string& path, int w, int h, const uint8_t* rgb);
 bool writePFM(const std::string& path, int w, int h, const float* rgb);
 void tonemapSRGB8(const float* lin, size_t nPix, uint8_t* rgb);
+// the 8-bit encode of tonemapSRGB8 for one value: clip to [0, 1], sRGB OETF, x * 255 + 0.5 truncated
+uint8_t encodeSRGB8(float x);
+// step positions of encodeSRGB8 for the device display transform: t[k] (k = 1..255) is the smallest float in [0, 1]
+// that encodes to k or more, t[0] = -inf (the encode is non-decreasing in x)
+void srgbSteps(float t[256]);
 
 }  // namespace cwp

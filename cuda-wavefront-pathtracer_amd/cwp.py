@@ -223,6 +223,11 @@ SIGNATURES = {
     "cwp_render": ([_P, _I, _I, _I, ctypes.POINTER(FrameStats)], _I),
     "cwp_read_accum": ([_P, ctypes.POINTER(ctypes.c_float), _S], _I),
     "cwp_set_params": ([_P, ctypes.POINTER(Params)], _I),
+    "cwp_accum_device": ([_P], _P),
+    "cwp_display": ([_P, _P, _I, ctypes.c_float, _I], _I),
+    "cwp_display_buffer": ([_P, _P, _I, _I, ctypes.c_float, _I], _I),
+    "cwp_srgb_encode8": ([ctypes.c_float], _I),
+    "cwp_srgb_steps": ([ctypes.POINTER(ctypes.c_float)], None),
     "cwp_set_materials": ([_P, ctypes.POINTER(ctypes.c_float), _I], _I),
     "cwp_set_environment": ([_P, ctypes.POINTER(ctypes.c_float), _S, ctypes.c_float, _I], _I),
     "cwp_trace": ([_P, _P, _I, _I, _P], _I),
@@ -448,6 +453,20 @@ class Renderer:
         out = np.zeros((h, w, 4), dtype=np.float32)
         _check(lib().cwp_read_accum(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w * h))
         return out
+
+    def accum_device(self):
+        """device address of the accumulation buffer (w * h float4 sums)"""
+        return lib().cwp_accum_device(self.h)
+
+    def display(self, spp, exposure_ev=0.0, mode="clip", out=None):
+        """The viewer's display transform of the accumulation on the GPU: (h, w, 4) uint8 RGBA. out: an optional
+        DeviceArray of w * h * 4 bytes that receives the pixels (and is then also read back)."""
+        n = self.w * self.hgt
+        buf = out if out is not None else DeviceArray(n * 4)
+        if buf.nbytes < n * 4:
+            raise ValueError("display buffer too small")
+        _check(lib().cwp_display(self.h, buf.ptr, int(spp), float(exposure_ev), DISPLAY_MODES[mode]))
+        return buf.to_host(np.uint8, (self.hgt, self.w, 4))
 
     def set_params(self, params):
         _check(lib().cwp_set_params(self.h, ctypes.byref(params)))
@@ -783,6 +802,33 @@ def bvh_from_hploc(vertices, triangles, hnodes, root, variant=8, leaf_max=4):
     _check(lib().cwp_bvh_from_hploc(_fptr(v), v.shape[0], t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), t.shape[0],
                                     _fptr(h), int(root), variant, leaf_max, ctypes.byref(p)))
     return _bvh_result(p, variant)
+
+
+DISPLAY_MODES = {"clip": 0, "reinhard": 1, "aces": 2}
+
+
+def display_buffer(accum, spp, exposure_ev=0.0, mode="clip"):
+    """The display transform kernel on a host array of (n, 4) float32 sums (staged through device memory):
+    (n, 4) uint8 RGBA. Mode: clip, reinhard, aces (or 0, 1, 2)."""
+    a = np.ascontiguousarray(accum, np.float32).reshape(-1, 4)
+    n = a.shape[0]
+    m = DISPLAY_MODES[mode] if isinstance(mode, str) else int(mode)
+    src = DeviceArray.from_host(a) if n else None
+    dst = DeviceArray(max(n, 1) * 4)
+    _check(lib().cwp_display_buffer(src.ptr if src else None, dst.ptr, n, int(spp), float(exposure_ev), m))
+    return dst.to_host(np.uint8, (n, 4))
+
+
+def srgb_encode8(x):
+    """the PNG writer's 8-bit encode of one value (clip, sRGB OETF, x * 255 + 0.5 truncated)"""
+    return lib().cwp_srgb_encode8(float(x))
+
+
+def srgb_steps():
+    """the 256 step positions the display kernel searches (steps[0] = -inf)"""
+    t = np.zeros(256, np.float32)
+    lib().cwp_srgb_steps(t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return t
 
 
 class DeviceArray:

@@ -219,6 +219,28 @@ void cwp_destroy(cwp_ctx* c);
 int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats* stats);
 /* Per-pixel radiance sums (divide by spp), float[4] per pixel, rows top to bottom. Replaces readAccum. */
 int cwp_read_accum(cwp_ctx* c, float* rgba, size_t num_pixels);
+/* Device pointer to the same sums (w * h float4), valid for the context's lifetime (NULL for a null context). Frames
+ * render on the context's own stream and cwp_render returns when its frame is done. Replaces
+ * WavefrontRenderer::accumDevice (src/render/renderer.h:59). */
+const float* cwp_accum_device(cwp_ctx* c);
+
+/* ---- progressive viewer: the display transform (reference: src/app/display.h:5-8, display.cu:9-33) ----
+ * Each channel: sum / spp (spp 0 is read as 1), times 2^exposure_ev when exposure_ev != 0, then for modes 1 and 2 a
+ * tone curve on max(x, 0) (Reinhard x / (1 + x); the ACES fit of Narkowicz 2015 after a 0.6 scale), then the PNG
+ * writer's encode: clip to [0, 1], sRGB OETF, x * 255 + 0.5 truncated. Output: packed RGBA8 per pixel (r in the lowest
+ * address byte, a = 255). Mode CWP_DISPLAY_CLIP at exposure 0 gives the bytes of the PNG file that pt writes. */
+enum { CWP_DISPLAY_CLIP = 0, CWP_DISPLAY_REINHARD = 1, CWP_DISPLAY_ACES = 2 };
+/* The context's accumulation into rgba8 (device pointer, w * h * 4 bytes), after the queued frames; synchronous.
+ * Replaces launchDisplay(R.accumDevice(), out, N, spp, exposure, mode) as the viewer calls it
+ * (src/app/viewer.cpp:176,182). */
+int cwp_display(cwp_ctx* c, void* rgba8, int spp, float exposure_ev, int mode);
+/* The same for any n device float4 sums (accum) into n device RGBA8 pixels, on the default stream; synchronous.
+ * Replaces launchDisplay (src/app/display.cu:30-33). */
+int cwp_display_buffer(const float* accum, void* rgba8, int n, int spp, float exposure_ev, int mode);
+/* The host side of the same encode, for checks: the 8-bit sRGB encode of one value (cwp_srgb_encode8), and the step
+ * positions the kernel searches (steps[k], k = 1..255: the smallest float that encodes to k or more; steps[0] = -inf). */
+int cwp_srgb_encode8(float x);
+void cwp_srgb_steps(float steps[256]);
 /* Replaces setParams (src/render/wavefront.cu:744); the image size must not change. */
 int cwp_set_params(cwp_ctx* c, const cwp_params* params);
 /* Runtime updates, effective from the next render (the context waits for queued frames). Replace

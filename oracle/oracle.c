@@ -905,6 +905,52 @@ void orc_camera_ray(const orc_params* p, int pix, uint32_t smp, float o[3], floa
     o[0] = ro.x; o[1] = ro.y; o[2] = ro.z; d[0] = rd.x; d[1] = rd.y; d[2] = rd.z;
 }
 
+/* --------------------------------------------------------- viewer display transform: src/app/display.cu:9-33 */
+int orc_srgb_encode8(float x) {
+    x = fminf(fmaxf(x, 0.f), 1.f);
+    x = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+    return (int)(unsigned char)(x * 255.f + 0.5f);
+}
+static float tone_curve(float x, int mode) {
+    if (mode == 1) return x / (1.f + x);
+    if (mode == 2) { x *= 0.6f; return (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f); }
+    return x;
+}
+void orc_display(const float* acc, int n, int spp, float exposure_ev, int mode, uint8_t* out) {
+    const float scale = exposure_ev == 0.f ? 1.f : exp2f(exposure_ev), fs = (float)(spp > 0 ? spp : 1);
+    for (int i = 0; i < n; i++) {
+        for (int c = 0; c < 3; c++) {
+            float x = acc[4 * (size_t)i + c] / fs;
+            if (scale != 1.f) x = x * scale;
+            if (mode != 0) x = tone_curve(fmaxf(x, 0.f), mode);
+            out[4 * (size_t)i + c] = (uint8_t)orc_srgb_encode8(x);
+        }
+        out[4 * (size_t)i + 3] = 255;
+    }
+}
+typedef struct { const float* steps; uint32_t lo, hi; uint64_t* bad; } steps_job;
+static void steps_worker(void* a, int tid, int nth) {
+    steps_job* J = (steps_job*)a;
+    uint64_t bad = 0;
+    for (uint64_t b = (uint64_t)J->lo + (uint64_t)tid; b <= J->hi; b += (uint64_t)nth) {
+        const float x = u2f((uint32_t)b);
+        int k = 0;
+        for (int s = 128; s > 0; s >>= 1) if (x >= J->steps[k + s]) k += s;
+        if (k != orc_srgb_encode8(x)) bad++;
+    }
+    J->bad[tid] = bad;
+}
+uint64_t orc_srgb_steps_check(const float* steps, float lo, float hi, int threads) {
+    if (threads < 1) threads = 1;
+    uint64_t* bad = (uint64_t*)calloc((size_t)threads, sizeof(uint64_t));
+    steps_job J = { steps, f2u(lo), f2u(hi), bad };
+    run_threads(steps_worker, &J, threads);
+    uint64_t t = 0;
+    for (int k = 0; k < threads; k++) t += bad[k];
+    free(bad);
+    return t;
+}
+
 /* ------------------------------------------------- legacy path of one (pixel, sample): src/render/wavefront.cu:94-136 */
 static f3 trace_path(const orc_scene* s, const orc_params* p, int pix, uint32_t smp, int bounces, double* st) {
     float o3[3], d3[3];

@@ -199,6 +199,15 @@ int orc_render(const orc_scene* s, const orc_params* p, int spp, int sample_base
 /* the camera ray of (pixel, sample): origin and direction */
 void orc_camera_ray(const orc_params* p, int pix, uint32_t smp, float o[3], float d[3]);
 
+/* the viewer's display transform (src/app/display.cu:9-33): n rgba sums of spp samples (spp 0 read as 1) -> n RGBA8
+ * pixels (4 bytes each, a = 255); mode 0 clip, 1 Reinhard, 2 ACES; exposure in stops (scale 2^ev unless ev == 0) */
+void orc_display(const float* acc, int n, int spp, float exposure_ev, int mode, uint8_t* out);
+/* the 8-bit encode of one value: clip to [0, 1], sRGB OETF, x * 255 + 0.5 truncated (src/app/display.cu:9-13) */
+int orc_srgb_encode8(float x);
+/* every float in [lo, hi] (as bit patterns, lo <= hi, both in [0, 1]) checked against 255 step positions (steps[k]:
+ * the smallest float that encodes to k or more): returns the number of floats whose encode differs from the search */
+uint64_t orc_srgb_steps_check(const float* steps, float lo, float hi, int threads);
+
 #ifdef __cplusplus
 }
 #endif

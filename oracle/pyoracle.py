@@ -102,6 +102,9 @@ def lib():
         L.orc_emitter_tables.restype = ctypes.c_int
         L.orc_inst_emitters.argtypes = [_FP, ctypes.c_int, _IP, _FP, _IP, _FP, ctypes.c_int, _FP, _FP, _FP]
         L.orc_inst_emitters.restype = ctypes.c_int
+        L.orc_display.argtypes = [_FP, ctypes.c_int, ctypes.c_int, _F, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
+        L.orc_srgb_encode8.argtypes = [_F]; L.orc_srgb_encode8.restype = ctypes.c_int
+        L.orc_srgb_steps_check.argtypes = [_FP, _F, _F, ctypes.c_int]; L.orc_srgb_steps_check.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -591,3 +594,23 @@ def texture_sample(img, texels, u, v, nearest=False, table=None):
     lib().orc_texture_sample(_i(im), tx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _f(tab), float(u), float(v),
                              1 if nearest else 0, _f(out))
     return out
+
+
+def display(acc, spp, exposure_ev=0.0, mode=0):
+    """the viewer's display transform of (n, 4) float32 sums -> (n, 4) uint8 RGBA (src/app/display.cu:9-33)"""
+    a = np.ascontiguousarray(acc, np.float32).reshape(-1, 4)
+    out = np.zeros((a.shape[0], 4), np.uint8)
+    lib().orc_display(_f(a), a.shape[0], int(spp), float(exposure_ev), int(mode),
+                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out
+
+
+def srgb_encode8(x):
+    return lib().orc_srgb_encode8(float(x))
+
+
+def srgb_steps_check(steps, lo=0.0, hi=1.0, threads=8):
+    """number of floats in [lo, hi] whose 8-bit encode differs from the search over the 255 step positions"""
+    t = np.ascontiguousarray(steps, np.float32)
+    assert t.shape == (256,)
+    return int(lib().orc_srgb_steps_check(_f(t), float(lo), float(hi), int(threads)))
