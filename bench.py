@@ -27,11 +27,12 @@ LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3
 
 
 def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=8, fuse=1,
-                  persistent=0, shadow_refill=0, sort=0):
+                  persistent=0, shadow_refill=0, sort=0, tail=-1):
     return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
             "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack),
             "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves), "--fuse", str(fuse),
-            "--persistent", str(persistent), "--shadow_refill", str(shadow_refill), "--sort", str(sort)] + LIGHTING
+            "--persistent", str(persistent), "--shadow_refill", str(shadow_refill), "--sort", str(sort),
+            "--tail", str(tail)] + LIGHTING
 
 
 def algorithmic_bytes(st, npaths, sort=0):
@@ -236,7 +237,8 @@ def bench_reference_config(args):
     import cwp
     out = {"bounces": 4, "spp": 256, "width": 1280, "height": 720}
     for variant in ("bvh2", "bvh8"):
-        s = cwp.Setup(workload_args(1280, 720, 256, 4, "woop", variant, args.stack, sort=args.sort))
+        s = cwp.Setup(workload_args(1280, 720, 256, 4, "woop", variant, args.stack, sort=args.sort,
+                                    tail=args.tail if variant == "bvh8" else -1))
         r = cwp.Renderer(s)
         r.render(256)
         st = r.render(256)
@@ -251,7 +253,7 @@ def bench_variant_frame(args, width, height, tri):
     """One frame of the terrain workload at another resolution or triangle test (8 bounces, BVH8, 256 spp in flight),
     after a warm-up frame: Grays/s."""
     import cwp
-    s = cwp.Setup(workload_args(width, height, 256, args.bounces, tri, "bvh8", args.stack, sort=args.sort))
+    s = cwp.Setup(workload_args(width, height, 256, args.bounces, tri, "bvh8", args.stack, sort=args.sort, tail=args.tail))
     r = cwp.Renderer(s)
     r.render(256)
     st = r.render(256)
@@ -439,6 +441,9 @@ def main():
     ap.add_argument("--shadow-refill", type=int, default=0, help="1: shadow kernel refills idle lanes mid-traversal")
     ap.add_argument("--sort", type=int, default=2, help="continuation order: 0 as appended; 1 every bounce's grouped by "
                     "direction bucket per partition; 2 the first bounce's grouped by elevation within each pixel")
+    ap.add_argument("--tail", type=int, default=-1, help="straggler hand-off: a trace+shade or shadow wave hands its "
+                    "tracing lanes to a tail kernel once fewer than this many remain (0 off, -1 the library's automatic "
+                    "setting)")
     ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve", "hploc"],
                     help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
@@ -502,10 +507,12 @@ def main():
         return
     if args.sort == 2 and (args.fuse == 0 or args.persistent):
         args.sort = 0   # the first-bounce sort runs with the fused kernel only
+    if args.tail > 0 and (args.fuse == 0 or args.persistent or args.variant != "bvh8"):
+        args.tail = 0   # the hand-off resumes BVH8 traversals of the fused kernel only
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
                                 args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill,
-                                args.sort))
+                                args.sort, args.tail))
     r = cwp.Renderer(s)
     info = s.info
     npaths = args.width * args.height * args.batch
@@ -550,11 +557,12 @@ def main():
                    "bounces": args.bounces, "bvh": args.variant, "tri": args.tri, "filter": "box",
                    "stack_lds": args.stack, "bounce_waves": args.bounce_waves, "bvh_nodes": info["num_nodes"],
                    "kernels": {1: "persistent-split", 2: "persistent-fused"}.get(args.persistent, "fused" if args.fuse else "split"),
-                   "shadow_refill": args.shadow_refill, "sort": args.sort},
+                   "shadow_refill": args.shadow_refill, "sort": args.sort, "tail": r.frame_json(last)["tail"]},
         "rays_per_frame": last["rays_closest"] + last["rays_shadow"],
         "rays_closest": last["rays_closest"], "rays_shadow": last["rays_shadow"],
         "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shade_ms": round(last["t_shade_ms"], 3),
         "t_shadow_ms": round(last["t_shadow_ms"], 3), "t_sort_ms": round(last["t_sort_ms"], 3),
+        "t_tail_ms": round(last["t_tail_ms"], 3),
         "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
                      else ("k_bounce_refill (fused, refilled lanes)" if args.persistent == 2 else "k_trace_paths + k_shade (split trace / shade)"),
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -238,7 +238,7 @@ int cwp_render(cwp_ctx* c, int spp, int sample_base, int clear, cwp_frame_stats*
         stats->t_bounce_ms = st.bounceMs; stats->t_shadow_ms = st.shadowMs;
         stats->bounce_launches = st.bounceLaunches; stats->shadow_launches = st.shadowLaunches;
         stats->rays_hit = st.hitRays; stats->rays_miss = st.missRays; stats->rays_cont = st.contRays; stats->shadow_unoccluded = st.unoccluded;
-        stats->camera_miss = st.cameraMiss; stats->t_shade_ms = st.shadeMs; stats->t_sort_ms = st.sortMs;
+        stats->camera_miss = st.cameraMiss; stats->t_shade_ms = st.shadeMs; stats->t_sort_ms = st.sortMs; stats->t_tail_ms = st.tailMs;
     }
     return CWP_OK;
 }
@@ -296,7 +296,7 @@ int cwp_frame_json(const cwp_setup* s, const cwp_ctx* c, const cwp_frame_stats* 
     F.rays = st->rays_closest; F.shadowRays = st->rays_shadow; F.traceMs = st->t_trace_ms; F.frameMs = st->t_frame_ms;
     F.bounceMs = st->t_bounce_ms; F.shadowMs = st->t_shadow_ms; F.hitRays = st->rays_hit; F.missRays = st->rays_miss;
     F.contRays = st->rays_cont; F.unoccluded = st->shadow_unoccluded; F.cameraMiss = st->camera_miss; F.shadeMs = st->t_shade_ms;
-    F.sortMs = st->t_sort_ms;
+    F.sortMs = st->t_sort_ms; F.tailMs = st->t_tail_ms;
     std::string j = frameJson(s->Z, c->R, F, rep, frame);
     if (j.size() + 1 > cap) return fail(CWP_ERR_ARG, "cwp_frame_json: buffer too small");
     memcpy(buf, j.c_str(), j.size() + 1);
@@ -308,7 +308,7 @@ void cwp_options_default(cwp_options* o) {
     const RenderOptions d;
     o->tri_test = d.tri; o->batch = d.batch; o->bounces = d.bounces; o->fuse = d.fuse; o->persistent = d.persistent;
     o->shadow_refill = d.shadowRefill; o->stack_lds = 8; o->blocks_per_cu = d.blocksPerCU; o->bounce_waves = d.bounceWaves;
-    o->time_kernels = d.timeKernels; o->graph = d.graph; o->sort = d.sort;
+    o->time_kernels = d.timeKernels; o->graph = d.graph; o->sort = d.sort; o->tail_lanes = d.tail;
 }
 
 static bool descToUpload(const cwp_scene_desc* d, SceneUpload& S, std::string& err) {
@@ -406,6 +406,7 @@ int cwp_create_scene(const cwp_scene_desc* d, const cwp_params* p, const cwp_opt
     O.persistent = o->persistent; O.shadowRefill = o->shadow_refill; O.stackLds = o->stack_lds;
     O.blocksPerCU = o->blocks_per_cu; O.bounceWaves = o->bounce_waves; O.timeKernels = o->time_kernels; O.graph = o->graph;
     O.sort = o->sort;
+    O.tail = o->tail_lanes;
     if (O.sort < 0 || O.sort > 2) return fail(CWP_ERR_ARG, "sort must be 0, 1 or 2");
     if (O.batch < 1) return fail(CWP_ERR_ARG, "batch must be >= 1");
     if (O.fuse != 0 && O.fuse != 1) return fail(CWP_ERR_ARG, "fuse must be 0 or 1");

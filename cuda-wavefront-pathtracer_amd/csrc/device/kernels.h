@@ -45,6 +45,8 @@ constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
 // rolling batches (steps instead of passes) also keep per partition and step: [kCIn] rays the step traces (the previous
 // step's continuations + the step's camera rays), [kCMissCam] camera-ray misses
 constexpr int kCIn = 4, kCMissCam = 5;
+// straggler hand-off: [kCTail] traversals partition x's trace+shade waves handed to the tail kernel
+constexpr int kCTail = 6, kCShadowTail = 7;   // ... and [kCShadowTail] the shadow kernel's
 
 // Lights and materials of the full integrator (device pointers; null / 0 for the legacy integrator)
 struct DevLights {
@@ -157,8 +159,22 @@ struct DevPaths {
     // its own slot with its elevation bucket in qk (kNoKey: none), and k_qsort_first groups each tile of kFirstTile
     // slots, a pixel's samples, by bucket into the depth-1 queue
     int firstSort = 0;
+    // straggler hand-off (tail null: off): a trace+shade wave whose tracing lanes drop below tailLanes stops tracing;
+    // those lanes write their traversal (queue slot, hit so far, node group, stack) as a tail record and k_bounce_tail
+    // finishes them in full waves and shades them. tailStride: Vec4s per record (3 + two stack entries per Vec4).
+    Vec4* tail = nullptr;
+    int tailStride = 0, tailLanes = 0;
 };
 constexpr uint8_t kNoKey = 255;
+
+// First tail record of partition `part` whose input queue starts at `inStart`: each 64-entry chunk of a partition
+// hands off at most lanes - 1 traversals, and floor((s + n) / 64) - floor(s / 64) + 1 >= ceil(n / 64), so the
+// partitions' regions do not overlap; all of them fit in ((entries >> 6) + kParts + 1) * (lanes - 1) records.
+__device__ __forceinline__ size_t tailFirst(int inStart, int part, int lanes) {
+    return ((size_t)(inStart >> 6) + (size_t)part) * (size_t)(lanes - 1);
+}
+// hit triangle of a ray whose traversal was handed to the tail kernel (not a miss: its shading waits for the tail)
+constexpr int kHandedOff = -2;
 
 // Continuation sort: 64 direction buckets, an 8 x 8 grid over the octahedral map of the unit sphere (upper hemisphere
 // in the central square, the lower one folded onto the corners), so that a wave of one bucket shares a narrow cone of
@@ -371,6 +387,7 @@ __device__ __forceinline__ ShadeOut shadeLegacy(const DevScene& S, const DevPath
     out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po; out.so = out.po;
     out.stmax = 1e30f; out.pdf = 0.f;
     if (h.tri < 0) {
+        if (h.tri == kHandedOff) return out;
         out.miss = true;
         Vec3 c = T * F.sky;
         if (depth == 0) P.L[i] = v4(c, 0.f);
@@ -477,15 +494,88 @@ constexpr int kRefill = 16;
 // fused persistent bounce: the done lanes shade together once this many are done
 constexpr int kShadeBatch = 32;
 
+// Closest (ANY false) or any hit of one ray by BVH8 node steps, handed off once fewer than P.tailLanes lanes of the
+// wave are still tracing: those lanes write their traversal as a tail record of partition `part` (queue slot `slot`,
+// the hit so far, the node group and the stack: (slot, t, u, v), (tri, node, group base, group mask), (group hits,
+// stack depth, 0, 0), then two stack entries per Vec4) and mark the hit kHandedOff.
+template <bool ANY, int TRI, class St>
+__device__ __forceinline__ void traceOrHandOff(const DevScene& S, const DevPaths& P, int* tailCtr, int part, int inStart,
+                                               int slot, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    const RayPre r = prepareRay<TRI>(o, d, tmin);
+    TravState ts; travBegin(ts, S.root);
+    st.sp = 0;
+    for (;;) {
+        if (stepBVH8<ANY, TRI>(S.nodes, S.tri, r, h, st, ts)) return;
+        if ((int)__builtin_popcountll(__ballot(1)) < P.tailLanes) break;   // the lanes still tracing
+    }
+    const size_t first = tailFirst(inStart, part, P.tailLanes) + (size_t)waveAppend(__ballot(1), tailCtr);
+    Vec4* rec = P.tail + first * (size_t)P.tailStride;
+    rec[0] = v4(asFloat(slot), h.t, h.u, h.v);
+    rec[1] = v4(asFloat(h.tri), asFloat(ts.node), asFloatU(ts.gBase), asFloatU(ts.gMask));
+    rec[2] = v4(asFloatU(ts.gHits), asFloat(st.sp), 0.f, 0.f);
+    for (int e = 0; e < st.sp; e += 2) {
+        const Group a = st.at(e), b = e + 1 < st.sp ? st.at(e + 1) : (Group)0;
+        rec[3 + e / 2] = v4(asFloatU((uint32_t)a), asFloatU((uint32_t)(a >> 32)), asFloatU((uint32_t)b),
+                            asFloatU((uint32_t)(b >> 32)));
+    }
+    h.tri = kHandedOff;
+}
+
+// A tail record back into a hit, a traversal state and the lane's stack; returns the record's queue slot.
+template <class St>
+__device__ __forceinline__ int resumeRecord(const Vec4* rec, Hit& h, TravState& ts, St& st) {
+    const Vec4 a = rec[0], b = rec[1], c = rec[2];
+    h.t = a.y; h.u = a.z; h.v = a.w; h.tri = asInt(b.x); h.inst = -1;
+    ts.node = asInt(b.y); ts.gBase = asUint(b.z); ts.gMask = asUint(b.w); ts.gHits = asUint(c.x);
+    const int sp = asInt(c.y);
+    st.sp = 0;
+    for (int e = 0; e < sp; e += 2) {
+        const Vec4 q = rec[3 + e / 2];
+        st.push(((Group)asUint(q.y) << 32) | asUint(q.x));
+        if (e + 1 < sp) st.push(((Group)asUint(q.w) << 32) | asUint(q.z));
+    }
+    return asInt(a.x);
+}
+
+// Shading of the traced ray at queue slot `slot` (path i); in the first-bounce sort's depth-0 launch its continuation
+// is staged at its own slot for k_qsort_first instead of being appended.
+template <bool INST, bool FULL>
+__device__ __forceinline__ ShadeOut shadeSlot(const DevScene& S, const DevPaths& P, const FrameParams& F, int depth,
+                                              int slot, int i, int smpBase, int K, Vec3 o, Vec3 d, const Hit& h) {
+    // throughput is fetched after the traversal so that no registers hold it across the traversal loop
+    const Vec3 T = pathThroughput(P, depth, slot);
+    ShadeOut res;
+    if constexpr (FULL) {
+        const float lastPdf = pathLastPdf(P, depth, slot);
+        res = shadeFull<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
+    } else {
+        res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+    }
+    if (P.firstSort) {   // depth 0: the continuation waits at its own slot for k_qsort_first
+        P.qk[slot] = res.cont ? (uint8_t)elevBucket(res.nd) : kNoKey;
+        if (res.cont) {
+            P.stO[slot] = v4(res.po, asFloat(i)); P.stD[slot] = v4(res.nd, res.pdf);
+            P.stT[slot] = v4(res.Tn, asFloat(1));
+        }
+        res.cont = false;
+    }
+    return res;
+}
+
 // One bounce of the legacy integrator over the queue of depth `depth` (depth 0: every path id, camera rays made here),
 // traversal and shading fused. The queue is split into kParts partitions, one per XCD; the waves of XCD x walk
 // partition x, and append their continuations to partition x of the next queue and their shadow rays to partition x of
 // the shadow queue, so a path stays on one XCD (and in its L2) for its whole life. Partition x of the depth-(d+1) queue
 // starts where the depth-d partitions before x end, which bounds it by the rays partition x traced.
 // WAVES: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
-template <int V, int TRI, int SL, int WAVES, bool INST, bool FULL = false>
+// TAIL (BVH8, single-level scenes): straggler hand-off. After each node visit a wave counts its lanes still tracing;
+// once fewer than P.tailLanes are, they stop and write their traversal state as tail records (k_bounce_tail resumes
+// them), and the wave shades its finished lanes and takes its next rays. The traversal resumes exactly where it
+// stopped (same hit, node group and stack), so hits and images do not change.
+template <int V, int TRI, int SL, int WAVES, bool INST, bool FULL = false, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth,
                                                           int smpBase, int K) {
+    static_assert(!TAIL || (V == 8 && !INST), "the hand-off resumes BVH8 traversals of single-level scenes");
     const FrameParams& F = *Fp;
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
@@ -508,23 +598,50 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             Vec3 o, d; float tmin;
             pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
-            // throughput is fetched after the traversal so that no registers hold it across the traversal loop
-            const Vec3 T = pathThroughput(P, depth, slot);
-            if constexpr (FULL) {
-                const float lastPdf = pathLastPdf(P, depth, slot);
-                res = shadeFull<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, lastPdf, h);
+            if constexpr (TAIL) {
+                traceOrHandOff<false, TRI>(S, P, ctr(cnt, part, kCTail), part, inStart, slot, o, d, tmin, h, st);
+                res = shadeSlot<INST, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed off
             } else {
-                res = shadeLegacy<INST>(S, P, F, depth, i, i / K, (uint32_t)(smpBase + i % K), o, d, T, h);
+                traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
+                res = shadeSlot<INST, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);
             }
-            if (P.firstSort) {   // depth 0: the continuation waits at its own slot for k_qsort_first
-                P.qk[slot] = res.cont ? (uint8_t)elevBucket(res.nd) : kNoKey;
-                if (res.cont) {
-                    P.stO[slot] = v4(res.po, asFloat(i)); P.stD[slot] = v4(res.nd, res.pdf);
-                    P.stT[slot] = v4(res.Tn, asFloat(1));
-                }
-                res.cont = false;
-            }
+        }
+        misses += appendResults(P, cnt, part, outStart, depth, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// The tail of a hand-off bounce: the traversals its waves handed off, resumed from their records in full waves (no
+// further hand-off), then shaded and appended like the bounce's own rays. Runs after k_bounce<..., TAIL> of the same
+// depth and before that depth's shadow kernel.
+template <int TRI, int SL, int WAVES, bool FULL>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp,
+                                                               int depth, int smpBase, int K) {
+    const FrameParams& F = *Fp;
+    __shared__ Group lds[SL * kBlock];
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int nt = ctrv(cnt, part, kCTail);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const Vec4* tail = P.tail + tailFirst(inStart, part, P.tailLanes) * (size_t)P.tailStride;
+    int misses = 0;
+    for (int base = lw * kWave; base < nt; base += nw * kWave) {
+        const int k = base + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < nt) {
+            Hit h; TravState ts;
+            const int slot = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
+            Vec3 o, d; float tmin;
+            pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
+            const RayPre r = prepareRay<TRI>(o, d, tmin);
+            while (!stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
+            res = shadeSlot<false, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
@@ -665,8 +782,9 @@ constexpr int kInstWaves = 6;
 // 6 waves per SIMD beat 4 (85 -> 77 ms on a 720p terrain frame) and 8.
 constexpr int kFullWaves = 6;
 
-template <int V, int TRI, int SL, bool INST>
+template <int V, int TRI, int SL, bool INST, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevScene S, DevPaths P, int depth) {
+    static_assert(!TAIL || (V == 8 && !INST), "the hand-off resumes BVH8 traversals of single-level scenes");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -682,10 +800,50 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevSce
         if (k < n) {
             const Vec4 O = P.sO[start + k], D = P.sD[start + k];
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            if (!traceRay<V, true, TRI, INST>(S, xyz(O), xyz(D), 0.f, h, st)) {
+            bool occluded;
+            if constexpr (TAIL) {   // hand-off records hold the shadow-queue position as their slot
+                traceOrHandOff<true, TRI>(S, P, ctr(cnt, part, kCShadowTail), part, start, start + k, xyz(O), xyz(D), 0.f, h, st);
+                occluded = h.tri != -1;
+            } else {
+                occluded = traceRay<V, true, TRI, INST>(S, xyz(O), xyz(D), 0.f, h, st);
+            }
+            if (!occluded) {
                 vis = true;
                 const int i = asInt(O.w);
                 const Vec4 c = P.sC[start + k];
+                Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+            }
+        }
+        unocc += (int)__builtin_popcountll(__ballot(vis));
+    }
+    waveAdd(unocc, ctr(cnt, part, kCUnocc));
+}
+
+// The tail of a hand-off shadow kernel: its handed-off any-hit traversals resumed in full waves.
+template <int TRI, int SL>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_tail(DevScene S, DevPaths P, int depth) {
+    __shared__ Group lds[SL * kBlock];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int nt = ctrv(cnt, part, kCShadowTail);
+    const int start = partStart(cnt, part);
+    const Vec4* tail = P.tail + tailFirst(start, part, P.tailLanes) * (size_t)P.tailStride;
+    int unocc = 0;
+    for (int base = lw * kWave; base < nt; base += nw * kWave) {
+        const int k = base + laneId();
+        bool vis = false;
+        if (k < nt) {
+            Hit h; TravState ts;
+            const int pos = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
+            const Vec4 O = P.sO[pos], D = P.sD[pos];
+            const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
+            while (!stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
+            if (h.tri < 0) {
+                vis = true;
+                const int i = asInt(O.w);
+                const Vec4 c = P.sC[pos];
                 Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
             }
         }
@@ -1050,6 +1208,9 @@ struct Kernels {
     // rolling batches
     RollFn bounceRoll = nullptr, bounceRollFull = nullptr, shadowRoll = nullptr;
     RollFn bounceRollLane = nullptr, bounceRollFullLane = nullptr;   // per-lane compaction
+    // straggler hand-off (BVH8, single-level scenes): the bounce kernels that hand off and their tail kernels
+    BounceFn bounceHand = nullptr, bounceTail = nullptr, bounceHandFull = nullptr, bounceTailFull = nullptr;
+    ShadowFn shadowHand = nullptr, shadowTail = nullptr;
 };
 
 template <int V, int TRI, int SL>
@@ -1078,6 +1239,14 @@ void pickSL(int waves, Kernels& k) {
     k.bounceRollLane = k_bounce_roll<V, TRI, SL, 8, false, true>;
     k.bounceRollFullLane = k_bounce_roll<V, TRI, SL, kFullWaves, true, true>;
     k.shadowRoll = k_shadow_roll<V, TRI, SL>;
+    if constexpr (V == 8) {
+        k.bounceHand = k_bounce<V, TRI, SL, 8, false, false, true>;
+        k.bounceTail = k_bounce_tail<TRI, SL, 8, false>;
+        k.bounceHandFull = k_bounce<V, TRI, SL, kFullWaves, false, true, true>;
+        k.bounceTailFull = k_bounce_tail<TRI, SL, kFullWaves, true>;
+        k.shadowHand = k_shadow<V, TRI, SL, false, true>;
+        k.shadowTail = k_shadow_tail<TRI, SL>;
+    }
 }
 template <int V, int TRI>
 void pickTri(int sl, int waves, Kernels& k) {

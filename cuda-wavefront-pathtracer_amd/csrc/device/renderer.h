@@ -20,6 +20,7 @@ namespace cwp {
 
 enum LightMode { kLightMIS = 0, kLightNEE = 1, kLightBSDF = 2 };
 
+constexpr int kTailDefault = 12;   // hand-off threshold in lanes of the automatic setting (measured best of 6-16)
 struct RenderOptions {
     int variant = 8;        // BVH layout: 2 or 8 (compressed)
     int tri = kTriWoop;     // triangle test: kTriMT, kTriWoop, kTriWatertight
@@ -36,6 +37,9 @@ struct RenderOptions {
     int sort = 0;           // 1: between bounces, group each partition's continuations by direction bucket
     int mortonBits = 0;     // two-level scenes: Morton keys of the per-frame TLAS build (0 adaptive, 30: 10 bits per axis)
     int tlasWidth = 8;      // two-level scenes with BVH8 BLASes: the per-frame TLAS as a BVH2 (2) or a compressed BVH8 (8)
+    int tail = -1;          // straggler hand-off (BVH8, single-level, fused kernel): a trace+shade or shadow wave hands
+                            // its tracing lanes to a tail kernel once fewer than this many remain (0: off; 2..64;
+                            // -1: kTailDefault where supported, else off)
 };
 
 struct FrameStats {
@@ -47,6 +51,7 @@ struct FrameStats {
     double cameraMiss = 0;                                             // camera rays that miss
     double shadeMs = 0;                  // split kernels: the shade kernels' part of bounceMs
     double sortMs = 0;                   // continuation sort kernels (per-kernel timing on)
+    double tailMs = 0;                   // straggler hand-off: the tail kernels' part of bounceMs and shadowMs
 };
 
 // Scene data ready for upload, in BVH leaf order.

@@ -122,6 +122,9 @@ struct Renderer::Impl {
     std::vector<void*> allocs;
     Kernels kn;
     BounceFn bounce = nullptr, shade = nullptr;   // fused: bounce; split: bounce = trace kernel, then shade
+    BounceFn bounceTail = nullptr;                 // straggler hand-off: the tail kernels after each bounce and
+    ShadowFn shadowTail = nullptr;                 // each shadow launch
+    int gridTail = 0, gridShadowTail = 0;
     ShadowFn shadow = nullptr;
     int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0, gridSort = 0;
     void* stageRays = nullptr; void* stageHits = nullptr; int* stageInst = nullptr; size_t stageCap = 0;   // traceHost staging
@@ -311,6 +314,18 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.shade = O.fuse ? nullptr : k_shade;
         I.shadow = (O.persistent == 1 || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
     }
+    if (O.tail < -1 || O.tail == 1 || O.tail > kWave) { err = "tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
+    const bool tailable = S.variant == 8 && !I.inst && O.fuse && !O.persistent;
+    I.O.tail = O.tail == -1 ? (tailable ? kTailDefault : 0) : O.tail;
+    if (I.O.tail) {
+        if (!tailable) {
+            err = "the straggler hand-off (tail) needs a single-level BVH8 scene and the fused kernel (fuse 1, persistent 0)";
+            return false;
+        }
+        I.bounce = S.full ? I.kn.bounceHandFull : I.kn.bounceHand;
+        I.bounceTail = S.full ? I.kn.bounceTailFull : I.kn.bounceTail;
+        if (!O.shadowRefill) { I.shadow = I.kn.shadowHand; I.shadowTail = I.kn.shadowTail; }
+    }
     auto gridOf = [&](const void* fn) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, kBlock, 0) != hipSuccess || b < 1) b = 1;
@@ -321,7 +336,9 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.gridShade = I.shade ? gridOf((const void*)I.shade) : 0;
     I.gridShadow = gridOf((const void*)I.shadow);
     I.gridTrace = std::max(gridOf((const void*)I.kn.traceC), gridOf((const void*)I.kn.traceA));
-    I.maxGrid = std::max(I.gridBounce, std::max(I.gridShadow, I.gridTrace));
+    I.gridTail = I.bounceTail ? gridOf((const void*)I.bounceTail) : 0;
+    I.gridShadowTail = I.shadowTail ? gridOf((const void*)I.shadowTail) : 0;
+    I.maxGrid = std::max(std::max(std::max(I.gridBounce, I.gridTail), std::max(I.gridShadow, I.gridShadowTail)), I.gridTrace);
     I.gridSort = std::max(kParts, 4 * numCU / kParts * kParts);
     gridBlocks = I.gridBounce;
 
@@ -352,6 +369,12 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         !I.alloc(I.accum, (size_t)I.N, err) || !I.alloc(I.tot, 8, err))
         return false;
     if (!O.fuse && !I.alloc(I.P.H, NK, err)) return false;
+    if (I.O.tail) {   // tail records: slot, hit, node group and the stack (two entries per Vec4), at most lanes - 1 per chunk
+        I.P.tailLanes = I.O.tail;
+        I.P.tailStride = 3 + (std::max(stackNeed, 1) + 1) / 2;
+        const size_t records = ((NK >> 6) + kParts + 1) * (size_t)(I.O.tail - 1);
+        if (!I.alloc(I.P.tail, records * (size_t)I.P.tailStride, err)) return false;
+    }
     if (O.sort < 0 || O.sort > 2) { err = "sort must be 0, 1 or 2"; return false; }
     if (O.sort == 2 && (!O.fuse || O.persistent)) { err = "sort 2 (first-bounce sort) needs the fused kernel (fuse 1, persistent 0)"; return false; }
     if (O.sort) {
@@ -386,7 +409,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     if (sBase < 0) { err = "sample base must be >= 0"; return false; }
     S = FrameStats();
     I.evUsed = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv, sortEv;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> bounceEv, shadeEv, shadowEv, sortEv, tailEv;
     const bool timed = I.O.timeKernels && !I.O.graph;
     // the frame's launches: clears (by kernels, never memset nodes), then per pass the counter reset, trace+shade
     // and shadow per bounce, the reduction and the totals
@@ -403,6 +426,10 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 hipEvent_t a = nullptr, b = nullptr;
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                 I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
+                if (I.bounceTail) {
+                    if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); tailEv.push_back({ c, nullptr }); }
+                    I.bounceTail<<<I.gridTail, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
+                }
                 if (I.shade) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
                     I.shade<<<I.gridShade, kBlock, 0, I.st>>>(I.S, I.P, I.dF, depth, s, I.K);
@@ -410,10 +437,18 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (timed) {
                     HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });
                     if (I.shade) shadeEv.back().second = b;
+                    if (I.bounceTail) tailEv.back().second = b;
                 }
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                 I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
-                if (timed) { HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b }); }
+                if (I.shadowTail) {
+                    if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); tailEv.push_back({ c, nullptr }); }
+                    I.shadowTail<<<I.gridShadowTail, kBlock, 0, I.st>>>(I.S, I.P, depth);
+                }
+                if (timed) {
+                    HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b });
+                    if (I.shadowTail) tailEv.back().second = b;
+                }
                 if (I.O.sort == 2 && depth == 0 && depth + 1 < I.depths) {
                     if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                     k_qsort_first<<<I.gridSort, kFirstTile, 0, I.st>>>(P0);
@@ -463,6 +498,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
     for (auto& e : shadowEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadowMs += ms; S.shadowLaunches++; }
     for (auto& e : shadeEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.shadeMs += ms; }
     for (auto& e : sortEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.sortMs += ms; }
+    for (auto& e : tailEv) { HIPCHK(hipEventElapsedTime(&ms, e.first, e.second)); S.tailMs += ms; }
     S.traceMs = I.O.timeKernels ? S.bounceMs + S.shadowMs : S.frameMs;
     S.rays = (double)I.hostTot[0]; S.shadowRays = (double)I.hostTot[1];
     S.missRays = (double)I.hostTot[2]; S.unoccluded = (double)I.hostTot[3]; S.contRays = (double)I.hostTot[4];

@@ -104,6 +104,7 @@ __device__ __forceinline__ ShadeOut shadeFull(const DevScene& S, const DevPaths&
     out.po = v3(0.f, 0.f, 0.f); out.nd = out.po; out.Tn = out.po; out.sdir = out.po; out.scon = out.po; out.so = out.po;
     out.stmax = 1e30f; out.pdf = 0.f;
     if (h.tri < 0) {   // environment (MIS-weighted against its light sampling) or the constant sky
+        if (h.tri == kHandedOff) return out;
         out.miss = true;
         Vec3 Lm;
         if (E.hasEnv) {

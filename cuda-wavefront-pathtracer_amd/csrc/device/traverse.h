@@ -143,6 +143,8 @@ struct LaneStack {
         if (SL > 0 && sp < SL) return lds[sp * kStackStride];
         return spill[sp - SL];
     }
+    // entry e (0 = bottom) without popping it
+    __device__ __forceinline__ E at(int e) const { return (SL > 0 && e < SL) ? lds[e * kStackStride] : spill[e - SL]; }
 };
 
 // BVH8 stack entry: child base in the low word, (internal-child mask << 8) | remaining hit bits in the high word

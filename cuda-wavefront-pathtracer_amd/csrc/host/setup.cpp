@@ -107,6 +107,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--gen_textures") ok = inum(A.genTextures);
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--sort") ok = inum(A.sort);
+        else if (k == "--tail") ok = inum(A.tail);
         else if (k == "--devloop") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
@@ -142,6 +143,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     }
     if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
     if (A.sort < 0 || A.sort > 2) { err = "--sort must be 0, 1 or 2"; return false; }
+    if (A.tail < -1 || A.tail == 1 || A.tail > 64) { err = "--tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
     if (A.fuse < 0) A.fuse = 1;
     if (A.persistent == 1) A.fuse = 0;   // persistent traversal 1 runs in the split trace kernel
     if (A.persistent == 2) A.fuse = 1;   // 2: the fused trace+shade kernel with refilled lanes
@@ -631,7 +633,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.variant = Z.variant; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
-    O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.mortonBits = A.mortonBits;
+    O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.tail = A.tail; O.mortonBits = A.mortonBits;
     O.tlasWidth = A.tlas == "bvh2" ? 2 : 8;
     O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;
@@ -652,7 +654,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              "\"builder\":\"%s\",\"collapse\":\"%s\",\"bvh_tree_ms\":%.2f,\"bvh_post_ms\":%.2f,\"bvh_sah\":%.4f,\"graph\":%d,"
              "\"instances\":%d,\"flatten\":%d,\"frame\":%d,\"tlas_ms\":%.3f,\"tlas\":\"%s\",\"bins\":%d,\"stack_lds\":%d,"
              "\"t_bounce_ms\":%.3f,\"t_shade_ms\":%.3f,\"t_shadow_ms\":%.3f,\"rays_hit\":%.0f,\"rays_miss\":%.0f,\"rays_cont\":%.0f,\"shadow_unoccluded\":%.0f,"
-             "\"bvh_cache\":\"%s\",\"t_sort_ms\":%.3f}",
+             "\"bvh_cache\":\"%s\",\"t_sort_ms\":%.3f,\"tail\":%d}",
              rep, Z.variant, A.tri.c_str(), A.persistent, A.sort, A.fuse, Z.reportLeaf, A.batch, Z.sceneName.c_str(), Z.M.f.size(), Z.nNodes,
              Z.nodeBytes, Z.maxDepth, Z.bvhS, A.w, A.h, A.spp, A.bounces, st.rays, st.shadowRays, tot, st.traceMs, st.frameMs,
              st.traceMs > 0 ? tot / (st.traceMs * 1e3) : 0.0, st.frameMs > 0 ? tot / (st.frameMs * 1e3) : 0.0,
@@ -660,7 +662,7 @@ std::string frameJson(const SceneSetup& Z, const Renderer& R, const FrameStats& 
              Z.P.hasN, A.seed, Z.full ? "full" : "legacy", Z.P.cam.lensR, Z.P.cam.focusD, A.envSampling.c_str(), A.builder.c_str(), Z.variant == 8 ? A.collapse.c_str() : "-",
              Z.treeMs, Z.postMs, Z.sah, R.options().graph, Z.inst || !Z.A.scene.empty() || A.gen == "instances" ? (int)Z.SD.instances.size() : 0,
              A.flatten, frame, Z.inst ? R.tlasMs : 0.0, Z.inst ? (R.tlasWide ? "bvh8" : "bvh2") : "-", A.bins, R.options().stackLds, st.bounceMs, st.shadeMs, st.shadowMs, st.hitRays, st.missRays, st.contRays,
-             st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"), st.sortMs);
+             st.unoccluded, Z.bvhCacheHit < 0 ? "off" : (Z.bvhCacheHit ? "hit" : "miss"), st.sortMs, R.options().tail);
     return buf;
 }
 

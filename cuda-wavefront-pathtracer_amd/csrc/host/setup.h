@@ -26,6 +26,7 @@ struct Args {
     int persistent = 0, shadowRefill = 0, filterBH = 1, smooth = 1, warm = 8, reps = 1, stack = 8, blocksPerCU = 0, timeKernels = 1;
     int bounceWaves = 8;
     int sort = 0;                    // continuations grouped by direction bucket between bounces (same image)
+    int tail = -1;                   // straggler hand-off threshold in lanes (-1 auto, 0 off; same image)
     // full material integrator (reference: scene_setup.h:20-22)
     std::string materials, env, lightSampling = "mis", envSampling = "importance";
     float envStrength = -1.f, envRot = -1e30f;   // unset: the lighting file's (or 1, 0)

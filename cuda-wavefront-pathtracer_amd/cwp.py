@@ -46,7 +46,8 @@ class FrameStats(ctypes.Structure):
                 ("t_frame_ms", ctypes.c_double), ("t_bounce_ms", ctypes.c_double), ("t_shadow_ms", ctypes.c_double),
                 ("bounce_launches", ctypes.c_int), ("shadow_launches", ctypes.c_int), ("rays_hit", ctypes.c_double),
                 ("rays_miss", ctypes.c_double), ("rays_cont", ctypes.c_double), ("shadow_unoccluded", ctypes.c_double),
-                ("camera_miss", ctypes.c_double), ("t_shade_ms", ctypes.c_double), ("t_sort_ms", ctypes.c_double)]
+                ("camera_miss", ctypes.c_double), ("t_shade_ms", ctypes.c_double), ("t_sort_ms", ctypes.c_double),
+                ("t_tail_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,7 +98,7 @@ class Options(ctypes.Structure):
     _fields_ = [("tri_test", ctypes.c_int), ("batch", ctypes.c_int), ("bounces", ctypes.c_int), ("fuse", ctypes.c_int),
                 ("persistent", ctypes.c_int), ("shadow_refill", ctypes.c_int), ("stack_lds", ctypes.c_int),
                 ("blocks_per_cu", ctypes.c_int), ("bounce_waves", ctypes.c_int), ("time_kernels", ctypes.c_int),
-                ("graph", ctypes.c_int), ("sort", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("sort", ctypes.c_int), ("tail_lanes", ctypes.c_int)]
 
     @classmethod
     def default(cls, **kw):

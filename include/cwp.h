@@ -61,6 +61,7 @@ typedef struct cwp_frame_stats {
     double camera_miss;               /* camera rays that miss (rays_miss includes them) */
     double t_shade_ms;                /* split kernels: shade-kernel part of t_bounce_ms (0 when fused) */
     double t_sort_ms;                 /* continuation-sort kernels (options.sort; 0 when off or untimed) */
+    double t_tail_ms;                 /* straggler hand-off: the tail kernels' part of t_bounce_ms and t_shadow_ms */
 } cwp_frame_stats;
 
 /* A scene in the renderer's upload layout: host arrays in BVH leaf order, copied by cwp_create_scene. Replaces
@@ -128,6 +129,9 @@ typedef struct cwp_options {
                             src/render/wavefront.cu:717; different sorts, same results): 1 every bounce's, grouped by
                             direction bucket within each XCD partition; 2 the first bounce's, grouped by elevation
                             bucket within each pixel's samples (fused kernel only) */
+    int tail_lanes;      /* straggler hand-off (compressed BVH8, single-level scenes, fused kernel): once fewer than
+                            this many lanes of a trace+shade wave are still tracing, their traversals are handed to a
+                            tail kernel that resumes them in full waves (same results); 0 off, 2..64 */
 } cwp_options;
 
 /* Scene and BVH summary of a setup. */

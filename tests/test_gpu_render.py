@@ -313,3 +313,47 @@ def test_graph_frames_equal_stream_frames(full):
     g.set_params(p)                            # parameters are read through a pointer: the graphs stay valid
     g.render(16)
     np.testing.assert_array_equal(g.accum(), a)
+
+
+TAIL_CASES = [
+    ("terrain", SCENES["terrain"] + ["--variant", "bvh8"], (0, 2)),
+    ("terrain-watertight", SCENES["terrain"] + ["--variant", "bvh8", "--tri", "watertight"], (0,)),
+    ("terrain-mt-stack16", SCENES["terrain"] + ["--variant", "bvh8", "--tri", "mt", "--stack", "16"], (0,)),
+    ("boxes-full", SCENES["boxes"] + ["--gen_materials", "3", "--gen_env", "64,32,2,1", "--graph", "0"], (0, 2)),
+    ("terrain-graph", SCENES["terrain"] + ["--graph", "1"], (0, 1, 2)),
+    ("terrain-256spp", SCENES["terrain"] + ["--width", "48", "--height", "32"], (2,)),
+]
+
+
+@pytest.mark.parametrize("name,args,sorts", TAIL_CASES, ids=[c[0] for c in TAIL_CASES])
+def test_straggler_hand_off_gives_the_same_image(name, args, sorts):
+    """--tail N: a trace+shade wave whose tracing lanes drop below N hands them to the tail kernel, which resumes each
+    traversal from its record (hit so far, node group, stack). The traversal is the same, so images and every counter
+    are bit-identical to the plain kernel, at N = 64 (nearly every ray handed off once a lane finishes) and N = 8, on
+    two frames in a row (the tail counters are cleared per pass)."""
+    spp, batch = (256, 256) if name == "terrain-256spp" else (16, 8)
+    base = args + ["--batch", str(batch), "--bounces", "6", "--spp", str(spp)]
+    for sort in sorts:
+        out = {}
+        for tail in (0, 8, 64):
+            print("case", name, "sort", sort, "tail", tail, flush=True)
+            r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort), "--tail", str(tail)]))
+            frames = []
+            for f in range(2):
+                st = r.render(spp, sample_base=f * spp)
+                frames.append((r.accum(), st))
+            out[tail] = frames
+            r.close()
+        for tail in (8, 64):
+            for f in range(2):
+                (a, sa), (b, sb) = out[0][f], out[tail][f]
+                np.testing.assert_array_equal(a, b, err_msg="tail %d frame %d" % (tail, f))
+                for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
+                    assert sa[k] == sb[k], (tail, f, k)
+
+
+def test_straggler_hand_off_rejects_unsupported_schedules():
+    for extra in (["--variant", "bvh2"], ["--fuse", "0"], ["--persistent", "2"]):
+        s = cwp.Setup(SCENES["boxes"] + ["--tail", "8"] + extra)
+        with pytest.raises(cwp.CwpError):
+            cwp.Renderer(s)

@@ -53,8 +53,12 @@ def test_version_and_error_string():
         cwp.Setup(["--gen", "boxes", "--morton_bits", "12"])
     with pytest.raises(cwp.CwpError, match="--tlas"):
         cwp.Setup(["--gen", "boxes", "--tlas", "bvh4"])
+    for bad in ("1", "65", "-2"):
+        with pytest.raises(cwp.CwpError, match="--tail"):
+            cwp.Setup(["--gen", "boxes", "--tail", bad])
     cwp.Setup(["--gen", "boxes", "--sort", "1", "--devloop", "0", "--morton_bits", "30", "--tlas", "bvh2"])
     assert cwp.Options.default().sort == 0
+    assert cwp.Options.default().tail_lanes == -1   # automatic: on where the schedule supports it
 
 
 def test_reference_example_fixtures(golden_dir):
