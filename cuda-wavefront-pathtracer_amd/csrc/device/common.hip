@@ -118,40 +118,20 @@ __global__ void __launch_bounds__(kSortBlock) k_qscatter(DevPaths P, int depth) 
 // in LDS, reserves their total in the partition's depth-1 queue with one atomic, and writes them there grouped by
 // bucket, so that a depth-1 wave holds rays of one pixel that climb or descend alike and need similar traversals.
 __global__ void __launch_bounds__(kFirstTile) k_qsort_first(DevPaths P) {
-    __shared__ int h[kBuckets], first[kBuckets];
-    __shared__ int base;
+    __shared__ TileGroupLds g;
     const int t = (int)threadIdx.x;
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     const int* cnt0 = P.cnt;
     const int start = partStart(cnt0, part), n = ctrv(cnt0, part, kCQueue);
     int* queue = ctr(P.cnt + kCntStride, part, kCQueue);
     for (int tile = lw; tile * kFirstTile < n; tile += nw) {   // block-uniform bound: every thread meets the barriers
-        if (t < kBuckets) h[t] = 0;
-        __syncthreads();
         const int k = tile * kFirstTile + t;
-        int key = kNoKey, rank = 0;
-        if (k < n) {
-            key = P.qk[start + k];
-            if (key != kNoKey) rank = atomicAdd(&h[key & (kBuckets - 1)], 1);
+        const int key = k < n ? (int)P.qk[start + k] : kNoKey;
+        const int to = tileGroup(key, queue, g);
+        if (to >= 0) {
+            const int from = start + k;
+            P.O[1][start + to] = P.stO[from]; P.D[1][start + to] = P.stD[from]; P.T[1][start + to] = P.stT[from];
         }
-        __syncthreads();
-        if (t < kBuckets) {   // exclusive prefix over the first wave's 64 lanes; the last lane reserves the total
-            const int c = h[t];
-            int s = c;
-#pragma unroll
-            for (int o = 1; o < kBuckets; o <<= 1) {
-                const int y = __shfl_up(s, o, kBuckets);
-                if (t >= o) s += y;
-            }
-            first[t] = s - c;
-            if (t == kBuckets - 1) base = start + (s > 0 ? atomicAdd(queue, s) : 0);
-        }
-        __syncthreads();
-        if (key != kNoKey) {
-            const int from = start + k, to = base + first[key & (kBuckets - 1)] + rank;
-            P.O[1][to] = P.stO[from]; P.D[1][to] = P.stD[from]; P.T[1][to] = P.stT[from];
-        }
-        __syncthreads();   // the next tile clears h
     }
 }
 

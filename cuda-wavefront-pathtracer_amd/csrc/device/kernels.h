@@ -199,6 +199,38 @@ __device__ __forceinline__ int elevBucket(Vec3 d) {
     return (int)fminf(fmaxf((d.z + 1.f) * 32.f, 0.f), 63.f);   // fmaxf first: NaN lands in bucket 0
 }
 
+// First-bounce sort: tiles of kFirstTile consecutive camera rays (one thread each; at 256 samples per pixel in flight,
+// one pixel) have their continuations grouped by elevation bucket.
+constexpr int kFirstTile = 256;
+struct TileGroupLds { int h[kBuckets], first[kBuckets], base; };
+// One tile's grouping, called by all kFirstTile threads of the workgroup (it holds barriers): key = this thread's
+// bucket (kNoKey: no continuation). Counts per bucket in LDS (a thread's rank in its bucket from the LDS atomic),
+// an exclusive prefix over the buckets by the first wave, whose last lane reserves the tile's total in `*queue` with
+// one atomic. Returns the continuation's queue position relative to the partition start (-1 for kNoKey).
+__device__ __forceinline__ int tileGroup(int key, int* queue, TileGroupLds& g) {
+    const int t = (int)threadIdx.x;
+    if (t < kBuckets) g.h[t] = 0;
+    __syncthreads();
+    int rank = 0;
+    if (key != kNoKey) rank = atomicAdd(&g.h[key & (kBuckets - 1)], 1);
+    __syncthreads();
+    if (t < kBuckets) {   // exclusive prefix over the first wave's 64 lanes; the last lane reserves the total
+        const int c = g.h[t];
+        int s = c;
+#pragma unroll
+        for (int o = 1; o < kBuckets; o <<= 1) {
+            const int y = __shfl_up(s, o, kBuckets);
+            if (t >= o) s += y;
+        }
+        g.first[t] = s - c;
+        if (t == kBuckets - 1) g.base = s > 0 ? atomicAdd(queue, s) : 0;
+    }
+    __syncthreads();
+    const int pos = key != kNoKey ? g.base + g.first[key & (kBuckets - 1)] + rank : -1;
+    __syncthreads();   // g is reused by the next tile
+    return pos;
+}
+
 template <int V> struct StackEntry;
 template <> struct StackEntry<2> { using type = int; };
 template <> struct StackEntry<4> { using type = int; };
@@ -607,6 +639,52 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             }
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// Depth 0 with the first-bounce sort fused in: workgroups of kFirstTile threads (four waves, each with its own LDS
+// stacks) trace and shade one tile of consecutive camera rays at a time, append the shadow rays per wave, and write
+// the continuations straight into the depth-1 queue grouped by elevation bucket (tileGroup), instead of staging them
+// for k_qsort_first. No hand-off: camera rays of one pixel need nearly the same traversals.
+template <int V, int TRI, int SL, int WAVES, bool FULL>
+__global__ void __launch_bounds__(kFirstTile, WAVES) k_bounce_first(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp,
+                                                                    int smpBase, int K) {
+    const FrameParams& F = *Fp;
+    using E = typename StackEntry<V>::type;
+    __shared__ E lds[SL > 0 ? SL * kFirstTile : 1];
+    __shared__ TileGroupLds g;
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kFirstTile + threadIdx.x;
+    LaneStack<E, SL> st{ (LdsPtr<E>)(lds + (threadIdx.x / kWave) * (SL > 0 ? SL : 1) * kWave + lane),
+                         (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lb = (int)(blockIdx.x / kParts), nb = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt;
+    const int n = ctrv(cnt, part, kCQueue);
+    const int start = partStart(cnt, part);   // the depth-0 slots and the depth-1 queue of partition x start here
+    int* queue = ctr(cnt + kCntStride, part, kCQueue);
+    int misses = 0;
+    for (int t0 = lb * kFirstTile; t0 < n; t0 += nb * kFirstTile) {   // block-uniform bound: every thread meets the barriers
+        const int k = t0 + (int)threadIdx.x;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < n) {
+            const int slot = start + k;
+            Vec3 o, d; float tmin;
+            pathRay(P, F, 0, slot, smpBase, K, i, o, d, tmin);
+            Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
+            traceRay<V, false, TRI, false>(S, o, d, tmin, h, st);
+            res = shadeSlot<false, FULL>(S, P, F, 0, slot, i, smpBase, K, o, d, h);
+        }
+        const bool cont = res.cont;
+        res.cont = false;   // the per-wave appends take the shadow ray and the miss; the tile places the continuation
+        misses += appendResults(P, cnt, part, start, 0, i, res);
+        const int to = tileGroup(cont ? elevBucket(res.nd) : (int)kNoKey, queue, g);
+        if (to >= 0) {
+            P.O[1][start + to] = v4(res.po, asFloat(i)); P.D[1][start + to] = v4(res.nd, res.pdf);
+            P.T[1][start + to] = v4(res.Tn, asFloat(1));
+        }
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
@@ -1183,6 +1261,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_roll(const DevStepTable* _
 // they occupied scalar registers for the whole kernel and made the trace+shade kernel spill (68 -> 24 bytes of
 // scratch per lane, +3.6 % frame throughput on the 720p terrain frame).
 using BounceFn = void (*)(DevScene, DevPaths, const FrameParams*, int, int, int);
+using FirstFn = void (*)(DevScene, DevPaths, const FrameParams*, int, int);
 using ShadowFn = void (*)(DevScene, DevPaths, int);
 using TraceFn = void (*)(DevScene, const Vec4*, int, Vec4*, int*, void*, int);
 using JobsFn = void (*)(const DevJobTable*, DevPaths, int);
@@ -1211,6 +1290,8 @@ struct Kernels {
     // straggler hand-off (BVH8, single-level scenes): the bounce kernels that hand off and their tail kernels
     BounceFn bounceHand = nullptr, bounceTail = nullptr, bounceHandFull = nullptr, bounceTailFull = nullptr;
     ShadowFn shadowHand = nullptr, shadowTail = nullptr;
+    // depth 0 with the first-bounce sort fused in (kFirstTile-thread workgroups)
+    FirstFn bounceFirst = nullptr, bounceFirstFull = nullptr;
 };
 
 template <int V, int TRI, int SL>
@@ -1247,6 +1328,8 @@ void pickSL(int waves, Kernels& k) {
         k.shadowHand = k_shadow<V, TRI, SL, false, true>;
         k.shadowTail = k_shadow_tail<TRI, SL>;
     }
+    k.bounceFirst = k_bounce_first<V, TRI, SL, 8, false>;
+    k.bounceFirstFull = k_bounce_first<V, TRI, SL, kFullWaves, true>;
 }
 template <int V, int TRI>
 void pickTri(int sl, int waves, Kernels& k) {
@@ -1271,7 +1354,6 @@ __global__ void k_pass_begin(int* cnt, int words, long long NK);
 constexpr int kSortBlock = 256, kSortPer = 16, kSortTile = kSortBlock * kSortPer;
 __global__ void k_qcount(DevPaths P, int depth);
 __global__ void k_qscatter(DevPaths P, int depth);
-constexpr int kFirstTile = 256;   // k_qsort_first: slots per tile, one per thread
 __global__ void k_qsort_first(DevPaths P);
 __global__ void k_pass_end(const int* cnt, int depths, unsigned long long* tot);
 __global__ void k_reduce(const Vec4* __restrict__ L, Vec4* __restrict__ accum, int N, int K);

@@ -125,6 +125,8 @@ struct Renderer::Impl {
     BounceFn bounceTail = nullptr;                 // straggler hand-off: the tail kernels after each bounce and
     ShadowFn shadowTail = nullptr;                 // each shadow launch
     int gridTail = 0, gridShadowTail = 0;
+    FirstFn bounceFirst = nullptr;                 // first-bounce sort fused into depth 0 (single-level scenes)
+    int gridFirst = 0;                             // in kFirstTile-thread workgroups
     ShadowFn shadow = nullptr;
     int gridBounce = 0, gridShade = 0, gridShadow = 0, gridTrace = 0, maxGrid = 0, gridSort = 0;
     void* stageRays = nullptr; void* stageHits = nullptr; int* stageInst = nullptr; size_t stageCap = 0;   // traceHost staging
@@ -336,9 +338,16 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     I.gridShade = I.shade ? gridOf((const void*)I.shade) : 0;
     I.gridShadow = gridOf((const void*)I.shadow);
     I.gridTrace = std::max(gridOf((const void*)I.kn.traceC), gridOf((const void*)I.kn.traceA));
+    if (O.sort == 2 && !I.inst && O.fuse && !O.persistent) {
+        I.bounceFirst = S.full ? I.kn.bounceFirstFull : I.kn.bounceFirst;
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)I.bounceFirst, kFirstTile, 0) != hipSuccess || b < 1) b = 1;
+        I.gridFirst = std::max(kParts, (b * numCU) / kParts * kParts);
+    }
     I.gridTail = I.bounceTail ? gridOf((const void*)I.bounceTail) : 0;
     I.gridShadowTail = I.shadowTail ? gridOf((const void*)I.shadowTail) : 0;
-    I.maxGrid = std::max(std::max(std::max(I.gridBounce, I.gridTail), std::max(I.gridShadow, I.gridShadowTail)), I.gridTrace);
+    I.maxGrid = std::max(std::max(std::max(I.gridBounce, I.gridTail), std::max(I.gridShadow, I.gridShadowTail)),
+                         std::max(I.gridTrace, I.gridFirst * (kFirstTile / kBlock)));
     I.gridSort = std::max(kParts, 4 * numCU / kParts * kParts);
     gridBlocks = I.gridBounce;
 
@@ -421,12 +430,14 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
             k_pass_begin<<<1, 64, 0, I.st>>>(I.P.cnt, words, I.NK);
             // first-bounce sort: the depth-0 launch stages its continuations, the later ones append directly
             DevPaths P0 = I.P, Pn = I.P;
-            if (I.O.sort == 2) { P0.firstSort = 1; Pn.qk = nullptr; }
+            if (I.O.sort == 2) { P0.firstSort = I.bounceFirst ? 0 : 1; Pn.qk = nullptr; P0.qk = I.bounceFirst ? nullptr : P0.qk; }
             for (int depth = 0; depth < I.depths; depth++) {
                 hipEvent_t a = nullptr, b = nullptr;
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
-                I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
-                if (I.bounceTail) {
+                const bool first = depth == 0 && I.bounceFirst;   // depth 0 with the first-bounce sort fused in
+                if (first) I.bounceFirst<<<I.gridFirst, kFirstTile, 0, I.st>>>(I.S, P0, I.dF, s, I.K);
+                else I.bounce<<<I.gridBounce, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
+                if (I.bounceTail && !first) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); tailEv.push_back({ c, nullptr }); }
                     I.bounceTail<<<I.gridTail, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
                 }
@@ -437,7 +448,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (timed) {
                     HIPCHK(hipEventRecord(b, I.st)); bounceEv.push_back({ a, b });
                     if (I.shade) shadeEv.back().second = b;
-                    if (I.bounceTail) tailEv.back().second = b;
+                    if (I.bounceTail && !first) tailEv.back().second = b;
                 }
                 if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                 I.shadow<<<I.gridShadow, kBlock, 0, I.st>>>(I.S, I.P, depth);
@@ -449,7 +460,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                     HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b });
                     if (I.shadowTail) tailEv.back().second = b;
                 }
-                if (I.O.sort == 2 && depth == 0 && depth + 1 < I.depths) {
+                if (I.O.sort == 2 && !I.bounceFirst && depth == 0 && depth + 1 < I.depths) {
                     if (timed) { a = I.nextEvent(); b = I.nextEvent(); HIPCHK(hipEventRecord(a, I.st)); }
                     k_qsort_first<<<I.gridSort, kFirstTile, 0, I.st>>>(P0);
                     if (timed) { HIPCHK(hipEventRecord(b, I.st)); sortEv.push_back({ a, b }); }
