@@ -563,14 +563,17 @@ def main():
         "t_bounce_ms": round(last["t_bounce_ms"], 3), "t_shade_ms": round(last["t_shade_ms"], 3),
         "t_shadow_ms": round(last["t_shadow_ms"], 3), "t_sort_ms": round(last["t_sort_ms"], 3),
         "t_tail_ms": round(last["t_tail_ms"], 3),
-        "roofline": {"bound": "hbm", "kernel": "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
+        "roofline": {"bound": "hbm", "kernel": ("trace+shade step per depth: k_bounce_first (depth 0, first-bounce sort fused), "
+                                                "k_bounce with straggler hand-off + k_bounce_tail (depths 1-9)")
+                     if args.fuse and not args.persistent and args.sort == 2 and r.frame_json(last)["tail"] > 0
+                     else "k_bounce (fused camera-ray / trace / shade)" if args.fuse and not args.persistent
                      else ("k_bounce_refill (fused, refilled lanes)" if args.persistent == 2 else "k_trace_paths + k_shade (split trace / shade)"),
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic("k_bounce") if args.fuse and not args.persistent else None,
                      "traffic_source": "profiles/k_bounce_terrain720p_traffic.md (rocprofv3 FETCH_SIZE / 0.5 + WRITE_SIZE / 1.0)",
                      "bytes_per_frame": bytes_bounce / args.steps,
-                     "utilisation": measured_utilisation("k_bounce<8, 1, 8, 8, false, false>") if args.fuse and not args.persistent
+                     "utilisation": measured_utilisation("k_bounce family") if args.fuse and not args.persistent
                      and args.variant == "bvh8" else None,
                      "avg_launch_ms": round(bounce_ms / max(1, launches), 4)},
         "latency": None,

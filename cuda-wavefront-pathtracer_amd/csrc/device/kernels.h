@@ -1328,7 +1328,7 @@ void pickSL(int waves, Kernels& k) {
         k.shadowHand = k_shadow<V, TRI, SL, false, true>;
         k.shadowTail = k_shadow_tail<TRI, SL>;
     }
-    k.bounceFirst = k_bounce_first<V, TRI, SL, 8, false>;
+    k.bounceFirst = k_bounce_first<V, TRI, SL, 6, false>;
     k.bounceFirstFull = k_bounce_first<V, TRI, SL, kFullWaves, true>;
 }
 template <int V, int TRI>
