@@ -27,12 +27,12 @@ LIGHTING = ["--sun_to", "0.608749", "-0.42625", "0.669131", "--sun_rgb", "3", "3
 
 
 def workload_args(width, height, batch, bounces, tri, variant, stack, blocks_per_cu=0, bounce_waves=8, fuse=1,
-                  persistent=0, shadow_refill=0, sort=0, tail=-1):
+                  persistent=0, shadow_refill=0, sort=0, tail=-1, tail_levels=2):
     return ["--gen", "terrain", "--width", str(width), "--height", str(height), "--batch", str(batch), "--spp", str(batch),
             "--bounces", str(bounces), "--variant", variant, "--tri", tri, "--filter", "box", "--stack", str(stack),
             "--blocks_per_cu", str(blocks_per_cu), "--bounce_waves", str(bounce_waves), "--fuse", str(fuse),
             "--persistent", str(persistent), "--shadow_refill", str(shadow_refill), "--sort", str(sort),
-            "--tail", str(tail)] + LIGHTING
+            "--tail", str(tail), "--tail_levels", str(tail_levels)] + LIGHTING
 
 
 def algorithmic_bytes(st, npaths, sort=0):
@@ -444,6 +444,8 @@ def main():
     ap.add_argument("--tail", type=int, default=-1, help="straggler hand-off: a trace+shade or shadow wave hands its "
                     "tracing lanes to a tail kernel once fewer than this many remain (0 off, -1 the library's automatic "
                     "setting)")
+    ap.add_argument("--tail-levels", type=int, default=2, help="hand-off levels: 1, or 2 (the tail kernels hand their "
+                    "own stragglers on)")
     ap.add_argument("--cpu-spp", type=int, default=40)
     ap.add_argument("--workload", default="terrain", choices=["terrain", "instances", "jobs", "serve", "hploc"],
                     help="terrain: the diffuse frame (headline), followed by the mixed job stream unless --no-jobs; "
@@ -512,7 +514,7 @@ def main():
     print("bench: setting up scene", file=sys.stderr, flush=True)
     s = cwp.Setup(workload_args(args.width, args.height, args.batch, args.bounces, args.tri, args.variant, args.stack,
                                 args.blocks_per_cu, args.bounce_waves, args.fuse, args.persistent, args.shadow_refill,
-                                args.sort, args.tail))
+                                args.sort, args.tail, args.tail_levels))
     r = cwp.Renderer(s)
     info = s.info
     npaths = args.width * args.height * args.batch

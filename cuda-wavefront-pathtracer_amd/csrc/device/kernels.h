@@ -47,6 +47,8 @@ constexpr int kCQueue = 0, kCShadow = 1, kCMiss = 2, kCUnocc = 3;
 constexpr int kCIn = 4, kCMissCam = 5;
 // straggler hand-off: [kCTail] traversals partition x's trace+shade waves handed to the tail kernel
 constexpr int kCTail = 6, kCShadowTail = 7;   // ... and [kCShadowTail] the shadow kernel's
+// [kCTail2] / [kCShadowTail2]: traversals the first tail kernels handed on to the second ones
+constexpr int kCTail2 = 8, kCShadowTail2 = 9;
 
 // Lights and materials of the full integrator (device pointers; null / 0 for the legacy integrator)
 struct DevLights {
@@ -164,6 +166,8 @@ struct DevPaths {
     // finishes them in full waves and shades them. tailStride: Vec4s per record (3 + two stack entries per Vec4).
     Vec4* tail = nullptr;
     int tailStride = 0, tailLanes = 0;
+    // second hand-off level (null: off): the tail kernels hand their own stragglers on, in the same way, to tail2
+    Vec4* tail2 = nullptr;
 };
 constexpr uint8_t kNoKey = 255;
 
@@ -527,21 +531,19 @@ constexpr int kRefill = 16;
 constexpr int kShadeBatch = 32;
 
 // Closest (ANY false) or any hit of one ray by BVH8 node steps, handed off once fewer than P.tailLanes lanes of the
-// wave are still tracing: those lanes write their traversal as a tail record of partition `part` (queue slot `slot`,
-// the hit so far, the node group and the stack: (slot, t, u, v), (tri, node, group base, group mask), (group hits,
-// stack depth, 0, 0), then two stack entries per Vec4) and mark the hit kHandedOff.
+// wave are still tracing: those lanes write their traversal as a tail record appended (through tailCtr) to the region
+// of buf starting at record regionFirst (queue slot `slot`, the hit so far, the node group and the stack: (slot, t, u,
+// v), (tri, node, group base, group mask), (group hits, stack depth, 0, 0), then two stack entries per Vec4) and mark
+// the hit kHandedOff.
 template <bool ANY, int TRI, class St>
-__device__ __forceinline__ void traceOrHandOff(const DevScene& S, const DevPaths& P, int* tailCtr, int part, int inStart,
-                                               int slot, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
-    const RayPre r = prepareRay<TRI>(o, d, tmin);
-    TravState ts; travBegin(ts, S.root);
-    st.sp = 0;
+__device__ __forceinline__ void stepOrHandOff(const DevScene& S, const DevPaths& P, Vec4* buf, size_t regionFirst,
+                                              int* tailCtr, int slot, const RayPre& r, Hit& h, St& st, TravState& ts) {
     for (;;) {
         if (stepBVH8<ANY, TRI>(S.nodes, S.tri, r, h, st, ts)) return;
         if ((int)__builtin_popcountll(__ballot(1)) < P.tailLanes) break;   // the lanes still tracing
     }
-    const size_t first = tailFirst(inStart, part, P.tailLanes) + (size_t)waveAppend(__ballot(1), tailCtr);
-    Vec4* rec = P.tail + first * (size_t)P.tailStride;
+    const size_t first = regionFirst + (size_t)waveAppend(__ballot(1), tailCtr);
+    Vec4* rec = buf + first * (size_t)P.tailStride;
     rec[0] = v4(asFloat(slot), h.t, h.u, h.v);
     rec[1] = v4(asFloat(h.tri), asFloat(ts.node), asFloatU(ts.gBase), asFloatU(ts.gMask));
     rec[2] = v4(asFloatU(ts.gHits), asFloat(st.sp), 0.f, 0.f);
@@ -551,6 +553,15 @@ __device__ __forceinline__ void traceOrHandOff(const DevScene& S, const DevPaths
                             asFloatU((uint32_t)(b >> 32)));
     }
     h.tri = kHandedOff;
+}
+// The same from the root for a new ray; its records go to region `regionFirst` of buf.
+template <bool ANY, int TRI, class St>
+__device__ __forceinline__ void traceOrHandOff(const DevScene& S, const DevPaths& P, Vec4* buf, size_t regionFirst,
+                                               int* tailCtr, int slot, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
+    const RayPre r = prepareRay<TRI>(o, d, tmin);
+    TravState ts; travBegin(ts, S.root);
+    st.sp = 0;
+    stepOrHandOff<ANY, TRI>(S, P, buf, regionFirst, tailCtr, slot, r, h, st, ts);
 }
 
 // A tail record back into a hit, a traversal state and the lane's stack; returns the record's queue slot.
@@ -631,7 +642,8 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             if constexpr (TAIL) {
-                traceOrHandOff<false, TRI>(S, P, ctr(cnt, part, kCTail), part, inStart, slot, o, d, tmin, h, st);
+                traceOrHandOff<false, TRI>(S, P, P.tail, tailFirst(inStart, part, P.tailLanes), ctr(cnt, part, kCTail), slot,
+                                           o, d, tmin, h, st);
                 res = shadeSlot<INST, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed off
             } else {
                 traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
@@ -689,10 +701,11 @@ __global__ void __launch_bounds__(kFirstTile, WAVES) k_bounce_first(DevScene S, 
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
 
-// The tail of a hand-off bounce: the traversals its waves handed off, resumed from their records in full waves (no
-// further hand-off), then shaded and appended like the bounce's own rays. Runs after k_bounce<..., TAIL> of the same
-// depth and before that depth's shadow kernel.
-template <int TRI, int SL, int WAVES, bool FULL>
+// The tails of a hand-off bounce, run after k_bounce<..., TAIL> of the same depth and before that depth's shadow
+// kernel. LEVEL 1: the traversals its waves handed off, resumed from their records in full waves; with P.tail2 these
+// waves hand their own stragglers on in the same way. LEVEL 2: those, resumed to the end. Each is then shaded and
+// appended like the bounce's own rays.
+template <int TRI, int SL, int WAVES, bool FULL, int LEVEL = 1>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp,
                                                                int depth, int smpBase, int K) {
     const FrameParams& F = *Fp;
@@ -702,10 +715,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPa
     LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int nt = ctrv(cnt, part, kCTail);
+    const int nt = ctrv(cnt, part, LEVEL == 1 ? kCTail : kCTail2);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const Vec4* tail = P.tail + tailFirst(inStart, part, P.tailLanes) * (size_t)P.tailStride;
+    const size_t first1 = tailFirst(inStart, part, P.tailLanes), first2 = tailFirst((int)first1, part, P.tailLanes);
+    const Vec4* tail = LEVEL == 1 ? P.tail + first1 * (size_t)P.tailStride : P.tail2 + first2 * (size_t)P.tailStride;
     int misses = 0;
     for (int base = lw * kWave; base < nt; base += nw * kWave) {
         const int k = base + lane;
@@ -718,8 +732,9 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPa
             Vec3 o, d; float tmin;
             pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
             const RayPre r = prepareRay<TRI>(o, d, tmin);
-            while (!stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
-            res = shadeSlot<false, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);
+            if (LEVEL == 1 && P.tail2) stepOrHandOff<false, TRI>(S, P, P.tail2, first2, ctr(cnt, part, kCTail2), slot, r, h, st, ts);
+            else while (!stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
+            res = shadeSlot<false, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed on
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
@@ -880,7 +895,8 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevSce
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             bool occluded;
             if constexpr (TAIL) {   // hand-off records hold the shadow-queue position as their slot
-                traceOrHandOff<true, TRI>(S, P, ctr(cnt, part, kCShadowTail), part, start, start + k, xyz(O), xyz(D), 0.f, h, st);
+                traceOrHandOff<true, TRI>(S, P, P.tail, tailFirst(start, part, P.tailLanes), ctr(cnt, part, kCShadowTail),
+                                          start + k, xyz(O), xyz(D), 0.f, h, st);
                 occluded = h.tri != -1;
             } else {
                 occluded = traceRay<V, true, TRI, INST>(S, xyz(O), xyz(D), 0.f, h, st);
@@ -897,17 +913,19 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevSce
     waveAdd(unocc, ctr(cnt, part, kCUnocc));
 }
 
-// The tail of a hand-off shadow kernel: its handed-off any-hit traversals resumed in full waves.
-template <int TRI, int SL>
+// The tails of a hand-off shadow kernel: its handed-off any-hit traversals resumed in full waves (LEVEL 1, handing
+// their own stragglers on to tail2 when it is set; LEVEL 2 resumes those to the end).
+template <int TRI, int SL, int LEVEL = 1>
 __global__ void __launch_bounds__(kBlock, 8) k_shadow_tail(DevScene S, DevPaths P, int depth) {
     __shared__ Group lds[SL * kBlock];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* cnt = P.cnt + depth * kCntStride;
-    const int nt = ctrv(cnt, part, kCShadowTail);
+    const int nt = ctrv(cnt, part, LEVEL == 1 ? kCShadowTail : kCShadowTail2);
     const int start = partStart(cnt, part);
-    const Vec4* tail = P.tail + tailFirst(start, part, P.tailLanes) * (size_t)P.tailStride;
+    const size_t first1 = tailFirst(start, part, P.tailLanes), first2 = tailFirst((int)first1, part, P.tailLanes);
+    const Vec4* tail = LEVEL == 1 ? P.tail + first1 * (size_t)P.tailStride : P.tail2 + first2 * (size_t)P.tailStride;
     int unocc = 0;
     for (int base = lw * kWave; base < nt; base += nw * kWave) {
         const int k = base + laneId();
@@ -917,8 +935,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_tail(DevScene S, DevPaths 
             const int pos = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
             const Vec4 O = P.sO[pos], D = P.sD[pos];
             const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
-            while (!stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
-            if (h.tri < 0) {
+            if (LEVEL == 1 && P.tail2) stepOrHandOff<true, TRI>(S, P, P.tail2, first2, ctr(cnt, part, kCShadowTail2), pos, r, h, st, ts);
+            else while (!stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
+            if (h.tri == -1) {
                 vis = true;
                 const int i = asInt(O.w);
                 const Vec4 c = P.sC[pos];
@@ -1290,6 +1309,8 @@ struct Kernels {
     // straggler hand-off (BVH8, single-level scenes): the bounce kernels that hand off and their tail kernels
     BounceFn bounceHand = nullptr, bounceTail = nullptr, bounceHandFull = nullptr, bounceTailFull = nullptr;
     ShadowFn shadowHand = nullptr, shadowTail = nullptr;
+    BounceFn bounceTail2 = nullptr, bounceTail2Full = nullptr;   // second hand-off level
+    ShadowFn shadowTail2 = nullptr;
     // depth 0 with the first-bounce sort fused in (kFirstTile-thread workgroups)
     FirstFn bounceFirst = nullptr, bounceFirstFull = nullptr;
 };
@@ -1322,11 +1343,14 @@ void pickSL(int waves, Kernels& k) {
     k.shadowRoll = k_shadow_roll<V, TRI, SL>;
     if constexpr (V == 8) {
         k.bounceHand = k_bounce<V, TRI, SL, 8, false, false, true>;
-        k.bounceTail = k_bounce_tail<TRI, SL, 8, false>;
+        k.bounceTail = k_bounce_tail<TRI, SL, 6, false>;
         k.bounceHandFull = k_bounce<V, TRI, SL, kFullWaves, false, true, true>;
         k.bounceTailFull = k_bounce_tail<TRI, SL, kFullWaves, true>;
         k.shadowHand = k_shadow<V, TRI, SL, false, true>;
         k.shadowTail = k_shadow_tail<TRI, SL>;
+        k.bounceTail2 = k_bounce_tail<TRI, SL, 8, false, 2>;
+        k.bounceTail2Full = k_bounce_tail<TRI, SL, kFullWaves, true, 2>;
+        k.shadowTail2 = k_shadow_tail<TRI, SL, 2>;
     }
     k.bounceFirst = k_bounce_first<V, TRI, SL, 6, false>;
     k.bounceFirstFull = k_bounce_first<V, TRI, SL, kFullWaves, true>;

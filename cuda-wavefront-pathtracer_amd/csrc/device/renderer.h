@@ -37,6 +37,8 @@ struct RenderOptions {
     int sort = 0;           // 1: between bounces, group each partition's continuations by direction bucket
     int mortonBits = 0;     // two-level scenes: Morton keys of the per-frame TLAS build (0 adaptive, 30: 10 bits per axis)
     int tlasWidth = 8;      // two-level scenes with BVH8 BLASes: the per-frame TLAS as a BVH2 (2) or a compressed BVH8 (8)
+    int tailLevels = 2;     // hand-off levels: 1 (the tail kernels finish every handed-off traversal) or 2 (they hand
+                            // their own stragglers on to a second tail kernel)
     int tail = -1;          // straggler hand-off (BVH8, single-level, fused kernel): a trace+shade or shadow wave hands
                             // its tracing lanes to a tail kernel once fewer than this many remain (0: off; 2..64;
                             // -1: kTailDefault where supported, else off)

@@ -124,6 +124,8 @@ struct Renderer::Impl {
     BounceFn bounce = nullptr, shade = nullptr;   // fused: bounce; split: bounce = trace kernel, then shade
     BounceFn bounceTail = nullptr;                 // straggler hand-off: the tail kernels after each bounce and
     ShadowFn shadowTail = nullptr;                 // each shadow launch
+    BounceFn bounceTail2 = nullptr;                // second hand-off level (tailLevels 2)
+    ShadowFn shadowTail2 = nullptr;
     int gridTail = 0, gridShadowTail = 0;
     FirstFn bounceFirst = nullptr;                 // first-bounce sort fused into depth 0 (single-level scenes)
     int gridFirst = 0;                             // in kFirstTile-thread workgroups
@@ -317,6 +319,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.shadow = (O.persistent == 1 || O.shadowRefill) ? I.kn.shadowRefill : I.kn.shadow;
     }
     if (O.tail < -1 || O.tail == 1 || O.tail > kWave) { err = "tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
+    if (O.tailLevels != 1 && O.tailLevels != 2) { err = "tail levels must be 1 or 2"; return false; }
     const bool tailable = S.variant == 8 && !I.inst && O.fuse && !O.persistent;
     I.O.tail = O.tail == -1 ? (tailable ? kTailDefault : 0) : O.tail;
     if (I.O.tail) {
@@ -327,6 +330,10 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.bounce = S.full ? I.kn.bounceHandFull : I.kn.bounceHand;
         I.bounceTail = S.full ? I.kn.bounceTailFull : I.kn.bounceTail;
         if (!O.shadowRefill) { I.shadow = I.kn.shadowHand; I.shadowTail = I.kn.shadowTail; }
+        if (O.tailLevels == 2) {
+            I.bounceTail2 = S.full ? I.kn.bounceTail2Full : I.kn.bounceTail2;
+            if (I.shadowTail) I.shadowTail2 = I.kn.shadowTail2;
+        }
     }
     auto gridOf = [&](const void* fn) {
         int b = 0;
@@ -345,6 +352,7 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.gridFirst = std::max(kParts, (b * numCU) / kParts * kParts);
     }
     I.gridTail = I.bounceTail ? gridOf((const void*)I.bounceTail) : 0;
+    if (I.bounceTail2) I.gridTail = std::min(I.gridTail, gridOf((const void*)I.bounceTail2));
     I.gridShadowTail = I.shadowTail ? gridOf((const void*)I.shadowTail) : 0;
     I.maxGrid = std::max(std::max(std::max(I.gridBounce, I.gridTail), std::max(I.gridShadow, I.gridShadowTail)),
                          std::max(I.gridTrace, I.gridFirst * (kFirstTile / kBlock)));
@@ -383,6 +391,10 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
         I.P.tailStride = 3 + (std::max(stackNeed, 1) + 1) / 2;
         const size_t records = ((NK >> 6) + kParts + 1) * (size_t)(I.O.tail - 1);
         if (!I.alloc(I.P.tail, records * (size_t)I.P.tailStride, err)) return false;
+        if (I.bounceTail2) {   // the second level's regions follow from the first's record numbering
+            const size_t records2 = ((records >> 6) + kParts + 1) * (size_t)(I.O.tail - 1);
+            if (!I.alloc(I.P.tail2, records2 * (size_t)I.P.tailStride, err)) return false;
+        }
     }
     if (O.sort < 0 || O.sort > 2) { err = "sort must be 0, 1 or 2"; return false; }
     if (O.sort == 2 && (!O.fuse || O.persistent)) { err = "sort 2 (first-bounce sort) needs the fused kernel (fuse 1, persistent 0)"; return false; }
@@ -440,6 +452,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (I.bounceTail && !first) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); tailEv.push_back({ c, nullptr }); }
                     I.bounceTail<<<I.gridTail, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
+                    if (I.bounceTail2) I.bounceTail2<<<I.gridTail, kBlock, 0, I.st>>>(I.S, depth == 0 ? P0 : Pn, I.dF, depth, s, I.K);
                 }
                 if (I.shade) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); shadeEv.push_back({ c, nullptr }); }
@@ -455,6 +468,7 @@ bool Renderer::render(int spp, int sBase, bool clear, FrameStats& S, std::string
                 if (I.shadowTail) {
                     if (timed) { hipEvent_t c = I.nextEvent(); HIPCHK(hipEventRecord(c, I.st)); tailEv.push_back({ c, nullptr }); }
                     I.shadowTail<<<I.gridShadowTail, kBlock, 0, I.st>>>(I.S, I.P, depth);
+                    if (I.shadowTail2) I.shadowTail2<<<I.gridShadowTail, kBlock, 0, I.st>>>(I.S, I.P, depth);
                 }
                 if (timed) {
                     HIPCHK(hipEventRecord(b, I.st)); shadowEv.push_back({ a, b });

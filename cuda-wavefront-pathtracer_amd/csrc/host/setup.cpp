@@ -108,6 +108,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
         else if (k == "--env_out") ok = str(A.envOut);
         else if (k == "--sort") ok = inum(A.sort);
         else if (k == "--tail") ok = inum(A.tail);
+        else if (k == "--tail_levels") ok = inum(A.tailLevels);
         else if (k == "--devloop") {
             int v = 0; ok = inum(v);
             if (ok && v) { err = k + " selects an experimental variant of the reference that this renderer does not provide"; return false; }
@@ -144,6 +145,7 @@ bool parseArgs(int argc, const char* const* argv, Args& A, std::string& err) {
     if (A.shadowRefill != 0 && A.shadowRefill != 1) { err = "--shadow_refill must be 0 or 1"; return false; }
     if (A.sort < 0 || A.sort > 2) { err = "--sort must be 0, 1 or 2"; return false; }
     if (A.tail < -1 || A.tail == 1 || A.tail > 64) { err = "--tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
+    if (A.tailLevels != 1 && A.tailLevels != 2) { err = "--tail_levels must be 1 or 2"; return false; }
     if (A.fuse < 0) A.fuse = 1;
     if (A.persistent == 1) A.fuse = 0;   // persistent traversal 1 runs in the split trace kernel
     if (A.persistent == 2) A.fuse = 1;   // 2: the fused trace+shade kernel with refilled lanes
@@ -633,7 +635,7 @@ bool setupScene(SceneSetup& Z, std::string& err) {
     O.variant = Z.variant; O.tri = A.tri == "mt" ? kTriMT : (A.tri == "watertight" ? kTriWatertight : kTriWoop);
     O.batch = A.batch; O.bounces = A.bounces; O.stackLds = A.stack; O.blocksPerCU = A.blocksPerCU; O.timeKernels = A.timeKernels;
     O.bounceWaves = A.bounceWaves; O.fuse = A.fuse; O.persistent = A.persistent;
-    O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.tail = A.tail; O.mortonBits = A.mortonBits;
+    O.shadowRefill = A.shadowRefill; O.sort = A.sort; O.tail = A.tail; O.tailLevels = A.tailLevels; O.mortonBits = A.mortonBits;
     O.tlasWidth = A.tlas == "bvh2" ? 2 : 8;
     O.graph = A.graph < 0 ? (Z.full ? 1 : 0) : A.graph;   // the reference's default: graphs for the full integrator
     return true;

@@ -27,6 +27,7 @@ struct Args {
     int bounceWaves = 8;
     int sort = 0;                    // continuations grouped by direction bucket between bounces (same image)
     int tail = -1;                   // straggler hand-off threshold in lanes (-1 auto, 0 off; same image)
+    int tailLevels = 2;              // hand-off levels (1 or 2)
     // full material integrator (reference: scene_setup.h:20-22)
     std::string materials, env, lightSampling = "mis", envSampling = "importance";
     float envStrength = -1.f, envRot = -1e30f;   // unset: the lighting file's (or 1, 0)

@@ -329,29 +329,30 @@ TAIL_CASES = [
 
 @pytest.mark.parametrize("name,args,sorts", TAIL_CASES, ids=[c[0] for c in TAIL_CASES])
 def test_straggler_hand_off_gives_the_same_image(name, args, sorts):
-    """--tail N: a trace+shade wave whose tracing lanes drop below N hands them to the tail kernel, which resumes each
-    traversal from its record (hit so far, node group, stack). The traversal is the same, so images and every counter
-    are bit-identical to the plain kernel, at N = 64 (nearly every ray handed off once a lane finishes) and N = 8, on
-    two frames in a row (the tail counters are cleared per pass)."""
+    """--tail N: a trace+shade or shadow wave whose tracing lanes drop below N hands them to a tail kernel, which resumes
+    each traversal from its record (hit so far, node group, stack) and, with --tail_levels 2, hands its own stragglers
+    on to a second tail kernel. The traversal is the same, so images and every counter are bit-identical to the plain
+    kernels, at N = 64 (nearly every ray handed off once a lane finishes, and on again) and N = 8, with one and two
+    levels, on two frames in a row (the tail counters are cleared per pass)."""
     spp, batch = (256, 256) if name == "terrain-256spp" else (16, 8)
     base = args + ["--batch", str(batch), "--bounces", "6", "--spp", str(spp)]
     for sort in sorts:
         out = {}
-        for tail in (0, 8, 64):
-            print("case", name, "sort", sort, "tail", tail, flush=True)
-            r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort), "--tail", str(tail)]))
+        for tail, levels in ((0, 1), (8, 2), (64, 2), (8, 1), (64, 1)):
+            print("case", name, "sort", sort, "tail", tail, "levels", levels, flush=True)
+            r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort), "--tail", str(tail), "--tail_levels", str(levels)]))
             frames = []
             for f in range(2):
                 st = r.render(spp, sample_base=f * spp)
                 frames.append((r.accum(), st))
-            out[tail] = frames
+            out[tail, levels] = frames
             r.close()
-        for tail in (8, 64):
+        for key in out:
             for f in range(2):
-                (a, sa), (b, sb) = out[0][f], out[tail][f]
-                np.testing.assert_array_equal(a, b, err_msg="tail %d frame %d" % (tail, f))
+                (a, sa), (b, sb) = out[0, 1][f], out[key][f]
+                np.testing.assert_array_equal(a, b, err_msg="tail %s frame %d" % (key, f))
                 for k in ("rays_closest", "rays_shadow", "rays_miss", "rays_cont", "shadow_unoccluded", "camera_miss"):
-                    assert sa[k] == sb[k], (tail, f, k)
+                    assert sa[k] == sb[k], (key, f, k)
 
 
 def test_straggler_hand_off_rejects_unsupported_schedules():
