@@ -311,3 +311,36 @@ def test_live_submissions_from_threads_while_jobs_run():
         np.testing.assert_array_equal(srv.result(ids[k], j[1], j[2]), alone[k], err_msg="job %d" % k)
     with pytest.raises(cwp.CwpError):
         srv.submit_live(0, 8, 8, 1)   # stopped
+
+
+def test_production_scale_scenes_alone_equal_mix_and_renderer():
+    """The serving configurations at full scene size: the 118,562-triangle terrain with generated materials, a
+    512x256 log-normal environment and generated texture maps (full integrator), beside the 1,024-box scene; jobs of
+    ragged resolutions up to 320x180, 5-16 samples, 6 bounces. Each job alone equals the job in the mix and in plain
+    mode bit for bit, and the job at the terrain's own resolution equals the single-job renderer's image."""
+    full = ["--gen", "terrain", "--width", "320", "--height", "180", "--gen_materials", "3", "--gen_env", "512,256,1.5,2",
+            "--gen_textures", "64", "--bounces", "6"]
+    boxes = ["--gen", "boxes", "--width", "128", "--height", "128", "--gen_materials", "5", "--bounces", "6"]
+    S = [cwp.Setup(full), cwp.Setup(boxes)]
+    assert S[0].info["num_tris"] > 100000 and S[0].info["textured"] and all(s.info["full"] for s in S)
+    jobs = [(0, 320, 180, 16, 6, 11), (1, 128, 128, 8, 6, 12), (0, 97, 61, 5, 6, 13), (1, 33, 17, 7, 6, 14),
+            (0, 320, 180, 16, 6, 15)]
+    srv = cwp.Server(variant=8, bounces=6, max_paths=1 << 20, full=1)
+    for s in S:
+        srv.add_scene(s)
+    alone = [run_jobs(srv, [j], True)[0][0] for j in jobs]
+    mix, info, _ = run_jobs(srv, jobs, True)
+    plain, _, _ = run_jobs(srv, jobs, False)
+    for k in range(len(jobs)):
+        print("case production job", k, flush=True)
+        np.testing.assert_array_equal(alone[k], mix[k], err_msg="job %d" % k)
+        np.testing.assert_array_equal(alone[k], plain[k], err_msg="plain job %d" % k)
+        assert np.isfinite(mix[k]).all() and mix[k][..., :3].mean() > 0
+    spp = info[0]["samples_per_pass"]
+    assert 16 % spp == 0
+    s = cwp.Setup(full + ["--spp", "16", "--batch", str(spp), "--seed", "11"])
+    r = cwp.Renderer(s)
+    r.render(16)
+    ref = r.accum()
+    r.close()
+    np.testing.assert_array_equal(mix[0], ref)
