@@ -150,7 +150,8 @@ def bench_instances(args, frames=None, quiet=False):
     spp = 64
     s = cwp.Setup(["--gen", "instances", "--width", "1280", "--height", "720", "--batch", str(spp), "--spp", str(spp),
                    "--bounces", str(args.bounces), "--variant", args.variant, "--tri", args.tri, "--filter", "box",
-                   "--stack", str(args.stack), "--morton_bits", str(args.tlas_morton_bits), "--tlas", args.tlas] + LIGHTING)
+                   "--stack", str(args.stack), "--morton_bits", str(args.tlas_morton_bits), "--tlas", args.tlas,
+                   "--tail", str(args.tail)] + LIGHTING)
     r = cwp.Renderer(s)
     info = s.info
     print("bench: %d meshes, %d instances, %d BLAS nodes" % (info["num_meshes"], info["num_instances"], info["num_nodes"]),

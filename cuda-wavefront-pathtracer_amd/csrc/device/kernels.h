@@ -535,18 +535,41 @@ constexpr int kShadeBatch = 32;
 // of buf starting at record regionFirst (queue slot `slot`, the hit so far, the node group and the stack: (slot, t, u,
 // v), (tri, node, group base, group mask), (group hits, stack depth, 0, 0), then two stack entries per Vec4) and mark
 // the hit kHandedOff.
-template <bool ANY, int TRI, class St>
+// One node step of a resumable traversal: the scene's BVH8 (INST false) or, for two-level scenes, the compressed BVH8
+// TLAS, whose leaf primitives are instances entered (each BLAS traversed to the end) within the step; the world ray
+// (o, d, tmin) is needed for the instances. True when the traversal is over.
+template <bool ANY, int TRI, bool INST, class St>
+__device__ __forceinline__ bool resumableStep(const DevScene& S, Vec3 o, Vec3 d, float tmin, const RayPre& r, Hit& h,
+                                              St& st, TravState& ts) {
+    if constexpr (INST) {
+        const int* prim = S.tlasPrim;
+        return stepBVH8With<ANY>(S.tlas, r, h, st, ts, [&](int k) {
+            enterInstance<8, ANY, TRI>(S, prim[k], o, d, tmin, h, st);
+            return h.tri >= 0;
+        });
+    } else {
+        return stepBVH8<ANY, TRI>(S.nodes, S.tri, r, h, st, ts);
+    }
+}
+// the ray of a resumable traversal: the world ray for the TLAS (the plain slab test's precomputation) or the BLAS ray
+template <int TRI, bool INST>
+__device__ __forceinline__ RayPre resumableRay(Vec3 o, Vec3 d, float tmin) {
+    return INST ? prepareRay<kTriMT>(o, d, tmin) : prepareRay<TRI>(o, d, tmin);
+}
+
+template <bool ANY, int TRI, bool INST, class St>
 __device__ __forceinline__ void stepOrHandOff(const DevScene& S, const DevPaths& P, Vec4* buf, size_t regionFirst,
-                                              int* tailCtr, int slot, const RayPre& r, Hit& h, St& st, TravState& ts) {
+                                              int* tailCtr, int slot, Vec3 o, Vec3 d, float tmin, const RayPre& r,
+                                              Hit& h, St& st, TravState& ts) {
     for (;;) {
-        if (stepBVH8<ANY, TRI>(S.nodes, S.tri, r, h, st, ts)) return;
+        if (resumableStep<ANY, TRI, INST>(S, o, d, tmin, r, h, st, ts)) return;
         if ((int)__builtin_popcountll(__ballot(1)) < P.tailLanes) break;   // the lanes still tracing
     }
     const size_t first = regionFirst + (size_t)waveAppend(__ballot(1), tailCtr);
     Vec4* rec = buf + first * (size_t)P.tailStride;
     rec[0] = v4(asFloat(slot), h.t, h.u, h.v);
     rec[1] = v4(asFloat(h.tri), asFloat(ts.node), asFloatU(ts.gBase), asFloatU(ts.gMask));
-    rec[2] = v4(asFloatU(ts.gHits), asFloat(st.sp), 0.f, 0.f);
+    rec[2] = v4(asFloatU(ts.gHits), asFloat(st.sp), asFloat(h.inst), 0.f);
     for (int e = 0; e < st.sp; e += 2) {
         const Group a = st.at(e), b = e + 1 < st.sp ? st.at(e + 1) : (Group)0;
         rec[3 + e / 2] = v4(asFloatU((uint32_t)a), asFloatU((uint32_t)(a >> 32)), asFloatU((uint32_t)b),
@@ -555,20 +578,20 @@ __device__ __forceinline__ void stepOrHandOff(const DevScene& S, const DevPaths&
     h.tri = kHandedOff;
 }
 // The same from the root for a new ray; its records go to region `regionFirst` of buf.
-template <bool ANY, int TRI, class St>
+template <bool ANY, int TRI, bool INST, class St>
 __device__ __forceinline__ void traceOrHandOff(const DevScene& S, const DevPaths& P, Vec4* buf, size_t regionFirst,
                                                int* tailCtr, int slot, Vec3 o, Vec3 d, float tmin, Hit& h, St& st) {
-    const RayPre r = prepareRay<TRI>(o, d, tmin);
-    TravState ts; travBegin(ts, S.root);
+    const RayPre r = resumableRay<TRI, INST>(o, d, tmin);
+    TravState ts; travBegin(ts, INST ? 0 : S.root);
     st.sp = 0;
-    stepOrHandOff<ANY, TRI>(S, P, buf, regionFirst, tailCtr, slot, r, h, st, ts);
+    stepOrHandOff<ANY, TRI, INST>(S, P, buf, regionFirst, tailCtr, slot, o, d, tmin, r, h, st, ts);
 }
 
 // A tail record back into a hit, a traversal state and the lane's stack; returns the record's queue slot.
 template <class St>
 __device__ __forceinline__ int resumeRecord(const Vec4* rec, Hit& h, TravState& ts, St& st) {
     const Vec4 a = rec[0], b = rec[1], c = rec[2];
-    h.t = a.y; h.u = a.z; h.v = a.w; h.tri = asInt(b.x); h.inst = -1;
+    h.t = a.y; h.u = a.z; h.v = a.w; h.tri = asInt(b.x); h.inst = asInt(c.z);
     ts.node = asInt(b.y); ts.gBase = asUint(b.z); ts.gMask = asUint(b.w); ts.gHits = asUint(c.x);
     const int sp = asInt(c.y);
     st.sp = 0;
@@ -618,7 +641,7 @@ __device__ __forceinline__ ShadeOut shadeSlot(const DevScene& S, const DevPaths&
 template <int V, int TRI, int SL, int WAVES, bool INST, bool FULL = false, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp, int depth,
                                                           int smpBase, int K) {
-    static_assert(!TAIL || (V == 8 && !INST), "the hand-off resumes BVH8 traversals of single-level scenes");
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals (two-level scenes: a BVH8 TLAS)");
     const FrameParams& F = *Fp;
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
@@ -642,8 +665,8 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce(DevScene S, DevPaths P
             pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             if constexpr (TAIL) {
-                traceOrHandOff<false, TRI>(S, P, P.tail, tailFirst(inStart, part, P.tailLanes), ctr(cnt, part, kCTail), slot,
-                                           o, d, tmin, h, st);
+                traceOrHandOff<false, TRI, INST>(S, P, P.tail, tailFirst(inStart, part, P.tailLanes), ctr(cnt, part, kCTail),
+                                                 slot, o, d, tmin, h, st);
                 res = shadeSlot<INST, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed off
             } else {
                 traceRay<V, false, TRI, INST>(S, o, d, tmin, h, st);
@@ -705,7 +728,7 @@ __global__ void __launch_bounds__(kFirstTile, WAVES) k_bounce_first(DevScene S, 
 // kernel. LEVEL 1: the traversals its waves handed off, resumed from their records in full waves; with P.tail2 these
 // waves hand their own stragglers on in the same way. LEVEL 2: those, resumed to the end. Each is then shaded and
 // appended like the bounce's own rays.
-template <int TRI, int SL, int WAVES, bool FULL, int LEVEL = 1>
+template <int TRI, int SL, int WAVES, bool FULL, int LEVEL = 1, bool INST = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPaths P, const FrameParams* __restrict__ Fp,
                                                                int depth, int smpBase, int K) {
     const FrameParams& F = *Fp;
@@ -731,10 +754,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPa
             const int slot = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
             Vec3 o, d; float tmin;
             pathRay(P, F, depth, slot, smpBase, K, i, o, d, tmin);
-            const RayPre r = prepareRay<TRI>(o, d, tmin);
-            if (LEVEL == 1 && P.tail2) stepOrHandOff<false, TRI>(S, P, P.tail2, first2, ctr(cnt, part, kCTail2), slot, r, h, st, ts);
-            else while (!stepBVH8<false, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
-            res = shadeSlot<false, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed on
+            const RayPre r = resumableRay<TRI, INST>(o, d, tmin);
+            if (LEVEL == 1 && P.tail2)
+                stepOrHandOff<false, TRI, INST>(S, P, P.tail2, first2, ctr(cnt, part, kCTail2), slot, o, d, tmin, r, h, st, ts);
+            else while (!resumableStep<false, TRI, INST>(S, o, d, tmin, r, h, st, ts)) {}
+            res = shadeSlot<INST, FULL>(S, P, F, depth, slot, i, smpBase, K, o, d, h);   // nothing if handed on
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
@@ -877,7 +901,7 @@ constexpr int kFullWaves = 6;
 
 template <int V, int TRI, int SL, bool INST, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevScene S, DevPaths P, int depth) {
-    static_assert(!TAIL || (V == 8 && !INST), "the hand-off resumes BVH8 traversals of single-level scenes");
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals (two-level scenes: a BVH8 TLAS)");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -895,8 +919,8 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevSce
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
             bool occluded;
             if constexpr (TAIL) {   // hand-off records hold the shadow-queue position as their slot
-                traceOrHandOff<true, TRI>(S, P, P.tail, tailFirst(start, part, P.tailLanes), ctr(cnt, part, kCShadowTail),
-                                          start + k, xyz(O), xyz(D), 0.f, h, st);
+                traceOrHandOff<true, TRI, INST>(S, P, P.tail, tailFirst(start, part, P.tailLanes), ctr(cnt, part, kCShadowTail),
+                                                start + k, xyz(O), xyz(D), 0.f, h, st);
                 occluded = h.tri != -1;
             } else {
                 occluded = traceRay<V, true, TRI, INST>(S, xyz(O), xyz(D), 0.f, h, st);
@@ -915,8 +939,8 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow(DevSce
 
 // The tails of a hand-off shadow kernel: its handed-off any-hit traversals resumed in full waves (LEVEL 1, handing
 // their own stragglers on to tail2 when it is set; LEVEL 2 resumes those to the end).
-template <int TRI, int SL, int LEVEL = 1>
-__global__ void __launch_bounds__(kBlock, 8) k_shadow_tail(DevScene S, DevPaths P, int depth) {
+template <int TRI, int SL, int LEVEL = 1, bool INST = false>
+__global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow_tail(DevScene S, DevPaths P, int depth) {
     __shared__ Group lds[SL * kBlock];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
@@ -934,9 +958,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_tail(DevScene S, DevPaths 
             Hit h; TravState ts;
             const int pos = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
             const Vec4 O = P.sO[pos], D = P.sD[pos];
-            const RayPre r = prepareRay<TRI>(xyz(O), xyz(D), 0.f);
-            if (LEVEL == 1 && P.tail2) stepOrHandOff<true, TRI>(S, P, P.tail2, first2, ctr(cnt, part, kCShadowTail2), pos, r, h, st, ts);
-            else while (!stepBVH8<true, TRI>(S.nodes, S.tri, r, h, st, ts)) {}
+            const Vec3 so = xyz(O), sd = xyz(D);
+            const RayPre r = resumableRay<TRI, INST>(so, sd, 0.f);
+            if (LEVEL == 1 && P.tail2)
+                stepOrHandOff<true, TRI, INST>(S, P, P.tail2, first2, ctr(cnt, part, kCShadowTail2), pos, so, sd, 0.f, r, h, st, ts);
+            else while (!resumableStep<true, TRI, INST>(S, so, sd, 0.f, r, h, st, ts)) {}
             if (h.tri == -1) {
                 vis = true;
                 const int i = asInt(O.w);
@@ -1311,6 +1337,10 @@ struct Kernels {
     ShadowFn shadowHand = nullptr, shadowTail = nullptr;
     BounceFn bounceTail2 = nullptr, bounceTail2Full = nullptr;   // second hand-off level
     ShadowFn shadowTail2 = nullptr;
+    // the same for two-level scenes with a BVH8 TLAS (hand-off between TLAS node steps)
+    BounceFn bounceHandInst = nullptr, bounceHandFullInst = nullptr, bounceTailInst = nullptr, bounceTailFullInst = nullptr;
+    BounceFn bounceTail2Inst = nullptr, bounceTail2FullInst = nullptr;
+    ShadowFn shadowHandInst = nullptr, shadowTailInst = nullptr, shadowTail2Inst = nullptr;
     // depth 0 with the first-bounce sort fused in (kFirstTile-thread workgroups)
     FirstFn bounceFirst = nullptr, bounceFirstFull = nullptr;
 };
@@ -1351,6 +1381,15 @@ void pickSL(int waves, Kernels& k) {
         k.bounceTail2 = k_bounce_tail<TRI, SL, 8, false, 2>;
         k.bounceTail2Full = k_bounce_tail<TRI, SL, kFullWaves, true, 2>;
         k.shadowTail2 = k_shadow_tail<TRI, SL, 2>;
+        k.bounceHandInst = k_bounce<V, TRI, SL, kInstWaves, true, false, true>;
+        k.bounceHandFullInst = k_bounce<V, TRI, SL, kFullWaves, true, true, true>;
+        k.bounceTailInst = k_bounce_tail<TRI, SL, kInstWaves, false, 1, true>;
+        k.bounceTailFullInst = k_bounce_tail<TRI, SL, kFullWaves, true, 1, true>;
+        k.bounceTail2Inst = k_bounce_tail<TRI, SL, kInstWaves, false, 2, true>;
+        k.bounceTail2FullInst = k_bounce_tail<TRI, SL, kFullWaves, true, 2, true>;
+        k.shadowHandInst = k_shadow<V, TRI, SL, true, true>;
+        k.shadowTailInst = k_shadow_tail<TRI, SL, 1, true>;
+        k.shadowTail2Inst = k_shadow_tail<TRI, SL, 2, true>;
     }
     k.bounceFirst = k_bounce_first<V, TRI, SL, 6, false>;
     k.bounceFirstFull = k_bounce_first<V, TRI, SL, kFullWaves, true>;

@@ -320,19 +320,33 @@ bool Renderer::init(const SceneUpload& S, const FrameParams& F, const RenderOpti
     }
     if (O.tail < -1 || O.tail == 1 || O.tail > kWave) { err = "tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
     if (O.tailLevels != 1 && O.tailLevels != 2) { err = "tail levels must be 1 or 2"; return false; }
-    const bool tailable = S.variant == 8 && !I.inst && O.fuse && !O.persistent;
-    I.O.tail = O.tail == -1 ? (tailable ? kTailDefault : 0) : O.tail;
+    // two-level scenes hand off between TLAS node steps, which needs the BVH8 TLAS
+    const bool tailable = S.variant == 8 && (!I.inst || I.tlasWide) && O.fuse && !O.persistent;
+    // automatic: single-level scenes only (the 1,001-instance frame measured 579 -> 593 ms with it: its hand-off
+    // points are TLAS steps, each holding whole BLAS traversals)
+    I.O.tail = O.tail == -1 ? (tailable && !I.inst ? kTailDefault : 0) : O.tail;
     if (I.O.tail) {
         if (!tailable) {
-            err = "the straggler hand-off (tail) needs a single-level BVH8 scene and the fused kernel (fuse 1, persistent 0)";
+            err = "the straggler hand-off (tail) needs a BVH8 scene (two-level: with the BVH8 TLAS) and the fused kernel "
+                  "(fuse 1, persistent 0)";
             return false;
         }
-        I.bounce = S.full ? I.kn.bounceHandFull : I.kn.bounceHand;
-        I.bounceTail = S.full ? I.kn.bounceTailFull : I.kn.bounceTail;
-        if (!O.shadowRefill) { I.shadow = I.kn.shadowHand; I.shadowTail = I.kn.shadowTail; }
-        if (O.tailLevels == 2) {
-            I.bounceTail2 = S.full ? I.kn.bounceTail2Full : I.kn.bounceTail2;
-            if (I.shadowTail) I.shadowTail2 = I.kn.shadowTail2;
+        if (I.inst) {
+            I.bounce = S.full ? I.kn.bounceHandFullInst : I.kn.bounceHandInst;
+            I.bounceTail = S.full ? I.kn.bounceTailFullInst : I.kn.bounceTailInst;
+            I.shadow = I.kn.shadowHandInst; I.shadowTail = I.kn.shadowTailInst;
+            if (O.tailLevels == 2) {
+                I.bounceTail2 = S.full ? I.kn.bounceTail2FullInst : I.kn.bounceTail2Inst;
+                I.shadowTail2 = I.kn.shadowTail2Inst;
+            }
+        } else {
+            I.bounce = S.full ? I.kn.bounceHandFull : I.kn.bounceHand;
+            I.bounceTail = S.full ? I.kn.bounceTailFull : I.kn.bounceTail;
+            if (!O.shadowRefill) { I.shadow = I.kn.shadowHand; I.shadowTail = I.kn.shadowTail; }
+            if (O.tailLevels == 2) {
+                I.bounceTail2 = S.full ? I.kn.bounceTail2Full : I.kn.bounceTail2;
+                if (I.shadowTail) I.shadowTail2 = I.kn.shadowTail2;
+            }
         }
     }
     auto gridOf = [&](const void* fn) {

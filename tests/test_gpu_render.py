@@ -324,6 +324,10 @@ TAIL_CASES = [
     ("boxes-full", SCENES["boxes"] + ["--gen_materials", "3", "--gen_env", "64,32,2,1", "--graph", "0"], (0, 2)),
     ("terrain-graph", SCENES["terrain"] + ["--graph", "1"], (0, 1, 2)),
     ("terrain-256spp", SCENES["terrain"] + ["--width", "48", "--height", "32"], (2,)),
+    ("instances", ["--gen", "instances", "--gen_small", "1", "--gen_instances", "40", "--width", "96", "--height", "64"],
+     (0, 2)),
+    ("instances-full", ["--gen", "instances", "--gen_small", "1", "--gen_instances", "30", "--width", "64", "--height", "48",
+                        "--gen_materials", "2", "--gen_env", "64,32,1,3", "--graph", "0"], (0,)),
 ]
 
 
@@ -343,6 +347,8 @@ def test_straggler_hand_off_gives_the_same_image(name, args, sorts):
             r = cwp.Renderer(cwp.Setup(base + ["--sort", str(sort), "--tail", str(tail), "--tail_levels", str(levels)]))
             frames = []
             for f in range(2):
+                if name.startswith("instances"):
+                    r.set_frame(f)
                 st = r.render(spp, sample_base=f * spp)
                 frames.append((r.accum(), st))
             out[tail, levels] = frames
@@ -360,3 +366,7 @@ def test_straggler_hand_off_rejects_unsupported_schedules():
         s = cwp.Setup(SCENES["boxes"] + ["--tail", "8"] + extra)
         with pytest.raises(cwp.CwpError):
             cwp.Renderer(s)
+    s = cwp.Setup(["--gen", "instances", "--gen_small", "1", "--gen_instances", "10", "--width", "32", "--height", "32",
+                   "--tlas", "bvh2", "--tail", "8"])
+    with pytest.raises(cwp.CwpError):   # the hand-off resumes BVH8 TLAS steps
+        cwp.Renderer(s)
