@@ -174,8 +174,8 @@ constexpr uint8_t kNoKey = 255;
 // First tail record of partition `part` whose input queue starts at `inStart`: each 64-entry chunk of a partition
 // hands off at most lanes - 1 traversals, and floor((s + n) / 64) - floor(s / 64) + 1 >= ceil(n / 64), so the
 // partitions' regions do not overlap; all of them fit in ((entries >> 6) + kParts + 1) * (lanes - 1) records.
-__device__ __forceinline__ size_t tailFirst(int inStart, int part, int lanes) {
-    return ((size_t)(inStart >> 6) + (size_t)part) * (size_t)(lanes - 1);
+__device__ __forceinline__ size_t tailFirst(size_t inStart, int part, int lanes) {
+    return ((inStart >> 6) + (size_t)part) * (size_t)(lanes - 1);
 }
 // hit triangle of a ray whose traversal was handed to the tail kernel (not a miss: its shading waits for the tail)
 constexpr int kHandedOff = -2;
@@ -741,7 +741,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_tail(DevScene S, DevPa
     const int nt = ctrv(cnt, part, LEVEL == 1 ? kCTail : kCTail2);
     const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
     const int outStart = partStart(cnt, part);
-    const size_t first1 = tailFirst(inStart, part, P.tailLanes), first2 = tailFirst((int)first1, part, P.tailLanes);
+    const size_t first1 = tailFirst(inStart, part, P.tailLanes), first2 = tailFirst(first1, part, P.tailLanes);
     const Vec4* tail = LEVEL == 1 ? P.tail + first1 * (size_t)P.tailStride : P.tail2 + first2 * (size_t)P.tailStride;
     int misses = 0;
     for (int base = lw * kWave; base < nt; base += nw * kWave) {
@@ -948,7 +948,7 @@ __global__ void __launch_bounds__(kBlock, INST ? kInstWaves : 8) k_shadow_tail(D
     int* cnt = P.cnt + depth * kCntStride;
     const int nt = ctrv(cnt, part, LEVEL == 1 ? kCShadowTail : kCShadowTail2);
     const int start = partStart(cnt, part);
-    const size_t first1 = tailFirst(start, part, P.tailLanes), first2 = tailFirst((int)first1, part, P.tailLanes);
+    const size_t first1 = tailFirst(start, part, P.tailLanes), first2 = tailFirst(first1, part, P.tailLanes);
     const Vec4* tail = LEVEL == 1 ? P.tail + first1 * (size_t)P.tailStride : P.tail2 + first2 * (size_t)P.tailStride;
     int unocc = 0;
     for (int base = lw * kWave; base < nt; base += nw * kWave) {
