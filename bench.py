@@ -354,7 +354,7 @@ def bench_jobs(args):
     print("bench: %d scenes (%s tris) set up in %.1f s" % (len(setups), ",".join(str(t) for t in tris),
                                                          time.perf_counter() - t0), file=sys.stderr, flush=True)
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6, full=int(args.full))
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail)
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)
@@ -383,7 +383,7 @@ def bench_jobs(args):
                           "batched_p50": round(batched["latency_p50_s"], 4), "batched_p99": round(batched["latency_p99_s"], 4)},
             "passes": {"plain": plain["passes"], "batched": batched["passes"]},
             "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"],
-                       "compaction": ("wave", "lane")[chosen["compact"]]},
+                       "compaction": ("wave", "lane")[chosen["compact"]], "tail_lanes": chosen["tail_lanes"]},
             "trials": [{"bvh": t["variant"], "stack": t["stack_lds"], "rpl": t["rays_per_lane"],
                         "compaction": ("wave", "lane")[t["compact"]], "grays": round(t["rays_per_s"] / 1e9, 4)}
                        for t in trials],
@@ -397,10 +397,12 @@ def bench_serve(args):
     import cwp
     setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed, full=bool(args.full)))
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6, full=int(args.full))
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail)
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)
+    tail = srv.get_options()["tail_lanes"]
+    print("bench: serving with hand-off threshold %d lanes" % tail, file=sys.stderr, flush=True)
     rng = np.random.default_rng(args.job_seed + 2)
     gaps = rng.exponential(1.0, len(jobs))
     out = []
@@ -417,7 +419,7 @@ def bench_serve(args):
                  row["plain"]["p50_s"], row["plain"]["p99_s"]), file=sys.stderr, flush=True)
         out.append(row)
     srv.close()
-    return out
+    return out, tail
 
 
 def main():
@@ -444,7 +446,7 @@ def main():
                     "direction bucket per partition; 2 the first bounce's grouped by elevation within each pixel")
     ap.add_argument("--tail", type=int, default=-1, help="straggler hand-off: a trace+shade or shadow wave hands its "
                     "tracing lanes to a tail kernel once fewer than this many remain (0 off, -1 the library's automatic "
-                    "setting)")
+                    "setting); the frame workloads, the job stream and serving")
     ap.add_argument("--tail-levels", type=int, default=2, help="hand-off levels: 1, or 2 (the tail kernels hand their "
                     "own stragglers on)")
     ap.add_argument("--cpu-spp", type=int, default=40)
@@ -505,7 +507,7 @@ def main():
                                                                                "-openpbr-env-mis" if args.full else "")},
                "jobs": j}
         if args.workload == "serve":
-            out["serve"] = bench_serve(args)
+            out["serve"], out["tail_lanes"] = bench_serve(args)
         print(json.dumps(out), flush=True)
         return
     if args.sort == 2 and (args.fuse == 0 or args.persistent):
@@ -599,8 +601,9 @@ def main():
         print("bench: serving sweep (full integrator)", file=sys.stderr, flush=True)
         sargs = argparse.Namespace(**vars(args))
         sargs.full, sargs.jobs = 1, args.serve_jobs
+        rates, tail = bench_serve(sargs)
         out["serve"] = {"integrator": "full (OpenPBR subset, emitters, 2048x1024 log-normal environment, MIS)",
-                        "jobs_per_rate": sargs.jobs, "rates": bench_serve(sargs)}
+                        "jobs_per_rate": sargs.jobs, "tail_lanes": tail, "rates": rates}
         print("bench: H-PLOC builds", file=sys.stderr, flush=True)
         h = bench_hploc(args, quiet=True)
         out["hploc"] = {"builds": h["config"]["builds"], "morton_bits": args.hploc_morton_bits,

@@ -83,15 +83,37 @@ struct BatchRenderer::Impl {
         HIPCHK(hipMalloc(&P.spill, (size_t)std::max(std::max(grid, gridFull), std::max(gridRoll, gridRollFull)) * kBlock * cap *
                                    sizeof(uint64_t)));
         spillCap = cap; P.spillCap = cap;
-        pathGen++;   // captured graphs hold the old spill pointer
+        pathGen++;   // captured graphs hold the old spill and tail pointers
+        return ensureTail(err);
+    }
+    // hand-off records (slot, hit, node group, then the stack two entries per Vec4) for the deepest stack the lanes can
+    // hold, sl + spillCap entries; at most lanes - 1 records per 64-entry chunk of a partition's input
+    bool ensureTail(std::string& err) {
+        if (!O.tail) return true;
+        const int stride = 3 + (sl + spillCap + 1) / 2;
+        if (P.tail && stride <= P.tailStride) return true;
+        HIPCHK(hipStreamSynchronize(st));
+        if (P.tail) { hipFree(P.tail); P.tail = nullptr; }
+        const size_t records = ((O.maxPaths >> 6) + kParts + 1) * (size_t)(O.tail - 1);
+        HIPCHK(hipMalloc(&P.tail, records * (size_t)stride * sizeof(Vec4)));
+        P.tailStride = stride; P.tailLanes = O.tail;
+        pathGen++;
         return true;
     }
     // the launches of a pass reading table slot s
     void enqueue(int s) {
         k_pass_begin_jobs<<<1, 64, 0, st>>>(P.cnt, (depths + 1) * kCntStride, dTable[s]);
+        const int gb = O.full ? gridFull : grid;
         for (int depth = 0; depth < depths; depth++) {
-            (O.full ? kn.bounceJobsFull : kn.bounceJobs)<<<O.full ? gridFull : grid, kBlock, 0, st>>>(dTable[s], P, depth);
-            kn.shadowJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+            if (O.tail) {   // each hand-off kernel, then its tail (same grid: within the spill area's lanes)
+                (O.full ? kn.bounceJobsHandFull : kn.bounceJobsHand)<<<gb, kBlock, 0, st>>>(dTable[s], P, depth);
+                (O.full ? kn.bounceJobsTailFull : kn.bounceJobsTail)<<<gb, kBlock, 0, st>>>(dTable[s], P, depth);
+                kn.shadowJobsHand<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+                kn.shadowJobsTail<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+            } else {
+                (O.full ? kn.bounceJobsFull : kn.bounceJobs)<<<gb, kBlock, 0, st>>>(dTable[s], P, depth);
+                kn.shadowJobs<<<grid, kBlock, 0, st>>>(dTable[s], P, depth);
+            }
         }
         k_reduce_jobs<<<redGrid, kWave, 0, st>>>(dTable[s], P.L);
         k_pass_end<<<1, 64, 0, st>>>(P.cnt, depths, tot);
@@ -99,9 +121,17 @@ struct BatchRenderer::Impl {
     // the launches of a rolling step reading step-table slot s (step parity = s & 1)
     void enqueueStep(int s, int step) {
         k_step_begin<<<1, 64, 0, st>>>(dStep[s], P.cnt, step);
-        const RollFn b = O.compact ? (O.full ? kn.bounceRollFullLane : kn.bounceRollLane) : (O.full ? kn.bounceRollFull : kn.bounceRoll);
-        b<<<O.full ? gridRollFull : gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
-        kn.shadowRoll<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        const int gb = O.full ? gridRollFull : gridRoll;
+        if (O.tail) {
+            kn.bounceRollHand[O.full ? 1 : 0][O.compact]<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
+            kn.bounceRollTail[O.full ? 1 : 0][O.compact]<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
+            kn.shadowRollHand<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+            kn.shadowRollTail<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        } else {
+            const RollFn b = O.compact ? (O.full ? kn.bounceRollFullLane : kn.bounceRollLane) : (O.full ? kn.bounceRollFull : kn.bounceRoll);
+            b<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
+            kn.shadowRoll<<<gridRoll, kBlock, 0, st>>>(dStep[s], P, step);
+        }
         k_reduce_roll<<<redGrid, kWave, 0, st>>>(dStep[s], P.L);
         k_step_end<<<1, 64, 0, st>>>(P.cnt, step, tot);
     }
@@ -144,6 +174,7 @@ BatchRenderer::~BatchRenderer() {
     for (SceneDev& s : p_->scenes) for (void* b : s.bufs) hipFree(b);
     for (void* a : p_->allocs) hipFree(a);
     if (p_->P.spill) hipFree(p_->P.spill);
+    if (p_->P.tail) hipFree(p_->P.tail);
     for (int k = 0; k < kTableSlots; k++) {
         if (p_->graph[k]) hipGraphExecDestroy(p_->graph[k]);
         if (p_->hTable[k]) hipHostFree(p_->hTable[k]);
@@ -174,7 +205,10 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     if (O.bounces < 0 || O.bounces > 64) { err = "batch: bounces must be in [0, 64]"; return false; }
     if (O.raysPerLane != 1 && O.raysPerLane != 2 && O.raysPerLane != 4) { err = "batch: rays per lane must be 1, 2 or 4"; return false; }
     if (O.compact != 0 && O.compact != 1) { err = "batch: compaction must be 0 (per wave) or 1 (per lane)"; return false; }
+    if (O.tail < -1 || O.tail == 1 || O.tail > kWave) { err = "batch: tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
+    if (O.tail > 0 && O.variant != 8) { err = "batch: the straggler hand-off (tail) resumes BVH8 traversals only"; return false; }
     I.O = O; I.sl = O.stackLds;
+    I.O.tail = O.tail == -1 ? (O.variant == 8 ? kTailDefault : 0) : O.tail;
     std::string name;
     if (!deviceInfo(I.numCU, name, err)) return false;
     HIPCHK(hipSetDevice(0));
@@ -207,7 +241,7 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
         !I.alloc(I.P.sC, n, err) || !I.alloc(I.P.cnt, (size_t)(I.depths + 1) * kCntStride, err) || !I.alloc(I.tot, 8, err))
         return false;
     HIPCHK(hipHostMalloc((void**)&I.hostTot, 8 * sizeof(unsigned long long), hipHostMallocDefault));
-    if (!I.fill(I.tot, 8 * sizeof(unsigned long long), 0u, err)) return false;
+    if (!I.fill(I.tot, 8 * sizeof(unsigned long long), 0u, err) || !I.ensureTail(err)) return false;
     for (int k = 0; k < kTableSlots; k++) {
         if (!I.alloc(I.dTable[k], 1, err)) return false;
         HIPCHK(hipHostMalloc((void**)&I.hTable[k], sizeof(DevJobTable), hipHostMallocDefault));

@@ -1081,17 +1081,56 @@ __device__ __forceinline__ int jobOf(const int* base, int n, int i) {
     return lo;
 }
 
+// The ray of entry `slot` of a pass's depth-`depth` queue (depth 0: the camera ray of path id slot): its path id, its
+// job (the last whose first id is <= it), pixel and sample index.
+struct JobEntry { int i, j, pix; uint32_t smp; };
+__device__ __forceinline__ JobEntry jobEntry(const DevJob* __restrict__ jobs, const int* base, int nJobs, const DevPaths& P,
+                                             int depth, int slot, Vec3& o, Vec3& d, float& tmin) {
+    JobEntry e;
+    Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
+    if (depth == 0) e.i = slot;
+    else { Or = P.O[depth & 1][slot]; e.i = asInt(Or.w); }
+    e.j = jobOf(base, nJobs, e.i);
+    const DevJob& J = jobs[e.j];
+    const int li = e.i - base[e.j], Kj = J.K;
+    e.pix = li / Kj;
+    e.smp = (uint32_t)(J.smpBase + li % Kj);
+    if (depth == 0) { cameraRay(J.F, e.pix, e.smp, o, d); tmin = J.F.nearT; }
+    else { o = xyz(Or); d = xyz(P.D[depth & 1][slot]); tmin = 0.f; }
+    return e;
+}
+// shading of a pass entry's traced ray with its job's scene and parameters (nothing for a handed-off traversal)
+template <bool FULL>
+__device__ __forceinline__ ShadeOut jobShade(const DevJob& J, const DevPaths& P, int depth, int slot, const JobEntry& e,
+                                             Vec3 o, Vec3 d, const Hit& h) {
+    const Vec3 T = pathThroughput(P, depth, slot);
+    if constexpr (FULL) {
+        const float lastPdf = pathLastPdf(P, depth, slot);
+        return shadeFull<false>(J.S, P, J.F, depth, e.i, e.pix, e.smp, o, d, T, lastPdf, h);
+    } else {
+        return shadeLegacy<false>(J.S, P, J.F, depth, e.i, e.pix, e.smp, o, d, T, h);
+    }
+}
+
+// The job table's path bases into LDS (for jobOf); every thread of the block meets the barrier.
+__device__ __forceinline__ void loadJobBases(const DevJob* __restrict__ jobs, int nJobs, int* base) {
+    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
+    __syncthreads();
+}
+
 // k_bounce over the paths of many jobs at once (one launch per bounce for the whole batch): each path finds its job,
 // and with it the scene it traverses and the parameters it is shaded with. A wave may hold paths of several jobs.
-template <int V, int TRI, int SL, int WAVES, bool FULL = false>
+// TAIL (BVH8): the straggler hand-off of k_bounce: once fewer than P.tailLanes lanes of a wave are still tracing, they
+// write their traversals as tail records (k_bounce_jobs_tail resumes them with each record's own job and scene).
+template <int V, int TRI, int SL, int WAVES, bool FULL = false, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     __shared__ int base[kMaxJobs];
     const int nJobs = T->nJobs;
     const DevJob* __restrict__ jobs = T->jobs;
-    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
-    __syncthreads();
+    loadJobBases(jobs, nJobs, base);
     const int lane = laneId();
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
@@ -1108,41 +1147,74 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
         res.cont = false; res.shadow = false; res.miss = false;
         if (k < n) {
             const int slot = inStart + k;
-            Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
-            if (depth == 0) i = slot;
-            else { Or = P.O[depth & 1][slot]; i = asInt(Or.w); }
-            const int j = jobOf(base, nJobs, i);
-            const DevJob& J = jobs[j];
-            const int li = i - base[j], Kj = J.K, pix = li / Kj;
-            const uint32_t smp = (uint32_t)(J.smpBase + li % Kj);
             Vec3 o, d; float tmin;
-            if (depth == 0) { cameraRay(J.F, pix, smp, o, d); tmin = J.F.nearT; }
-            else { o = xyz(Or); d = xyz(P.D[depth & 1][slot]); tmin = 0.f; }
+            const JobEntry e = jobEntry(jobs, base, nJobs, P, depth, slot, o, d, tmin);
+            const DevJob& J = jobs[e.j];
+            i = e.i;
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
-            const Vec3 T = pathThroughput(P, depth, slot);
-            if constexpr (FULL) {
-                const float lastPdf = pathLastPdf(P, depth, slot);
-                res = shadeFull<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, lastPdf, h);
-            } else {
-                res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, T, h);
-            }
+            if constexpr (TAIL)
+                traceOrHandOff<false, TRI, false>(J.S, P, P.tail, tailFirst(inStart, part, P.tailLanes), ctr(cnt, part, kCTail),
+                                                  slot, o, d, tmin, h, st);
+            else
+                traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
+            res = jobShade<FULL>(J, P, depth, slot, e, o, d, h);
         }
         misses += appendResults(P, cnt, part, outStart, depth, i, res);
     }
     waveAdd(misses, ctr(cnt, part, kCMiss));
 }
 
-// k_shadow over the shadow rays of many jobs: the ray's path id names its job and scene
-template <int V, int TRI, int SL>
+// The tail of a hand-off pass bounce (run after k_bounce_jobs<..., TAIL> of the same depth, before its shadow
+// kernel): the handed-off traversals resumed from their records in full waves, each with its job's scene, to the end;
+// then shaded and appended like the bounce's own rays.
+template <int TRI, int SL, int WAVES, bool FULL>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs_tail(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
+    __shared__ Group lds[SL * kBlock];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    loadJobBases(jobs, nJobs, base);
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + depth * kCntStride;
+    const int nt = ctrv(cnt, part, kCTail);
+    const int inStart = depth == 0 ? partStart(cnt, part) : partStart(cnt - kCntStride, part);
+    const int outStart = partStart(cnt, part);
+    const Vec4* tail = P.tail + tailFirst(inStart, part, P.tailLanes) * (size_t)P.tailStride;
+    int misses = 0;
+    for (int b0 = lw * kWave; b0 < nt; b0 += nw * kWave) {
+        const int k = b0 + lane;
+        int i = 0;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < nt) {
+            Hit h; TravState ts;
+            const int slot = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
+            Vec3 o, d; float tmin;
+            const JobEntry e = jobEntry(jobs, base, nJobs, P, depth, slot, o, d, tmin);
+            const DevJob& J = jobs[e.j];
+            i = e.i;
+            const RayPre r = resumableRay<TRI, false>(o, d, tmin);
+            while (!resumableStep<false, TRI, false>(J.S, o, d, tmin, r, h, st, ts)) {}
+            res = jobShade<FULL>(J, P, depth, slot, e, o, d, h);
+        }
+        misses += appendResults(P, cnt, part, outStart, depth, i, res);
+    }
+    waveAdd(misses, ctr(cnt, part, kCMiss));
+}
+
+// k_shadow over the shadow rays of many jobs: the ray's path id names its job and scene (TAIL: with the hand-off)
+template <int V, int TRI, int SL, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     __shared__ int base[kMaxJobs];
     const int nJobs = T->nJobs;
     const DevJob* __restrict__ jobs = T->jobs;
-    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
-    __syncthreads();
+    loadJobBases(jobs, nJobs, base);
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
@@ -1158,9 +1230,58 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs(const DevJobTable* __
             const int i = asInt(O.w);
             const DevJob& J = jobs[jobOf(base, nJobs, i)];
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            if (!traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st)) {
+            bool occluded;
+            if constexpr (TAIL) {   // records hold the shadow-queue position as their slot
+                traceOrHandOff<true, TRI, false>(J.S, P, P.tail, tailFirst(start, part, P.tailLanes), ctr(cnt, part, kCShadowTail),
+                                                 start + k, xyz(O), xyz(D), 0.f, h, st);
+                occluded = h.tri != -1;
+            } else {
+                occluded = traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st);
+            }
+            if (!occluded) {
                 vis = true;
                 const Vec4 c = P.sC[start + k];
+                Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
+            }
+        }
+        unocc += (int)__builtin_popcountll(__ballot(vis));
+    }
+    waveAdd(unocc, ctr(cnt, part, kCUnocc));
+}
+
+// The tail of a hand-off batch shadow kernel: its handed-off any-hit traversals resumed to the end, each with the
+// scene of its path's job. Pass (ROLL false, x = depth): counter block `depth`, shadow rays from the prefix of the
+// depth's queue; rolling step (ROLL true, x = step): block step & 1, from the prefix of the step's inputs.
+template <int TRI, int SL, bool ROLL, class Table>
+__global__ void __launch_bounds__(kBlock, 8) k_shadow_jobs_tail(const Table* __restrict__ T, DevPaths P, int x) {
+    __shared__ Group lds[SL * kBlock];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    loadJobBases(jobs, nJobs, base);
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* cnt = P.cnt + (ROLL ? (x & 1) : x) * kCntStride;
+    const int nt = ctrv(cnt, part, kCShadowTail);
+    const int start = partStart(cnt, part, ROLL ? kCIn : kCQueue);
+    const Vec4* tail = P.tail + tailFirst(start, part, P.tailLanes) * (size_t)P.tailStride;
+    int unocc = 0;
+    for (int b0 = lw * kWave; b0 < nt; b0 += nw * kWave) {
+        const int k = b0 + laneId();
+        bool vis = false;
+        if (k < nt) {
+            Hit h; TravState ts;
+            const int pos = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
+            const Vec4 O = P.sO[pos], D = P.sD[pos];
+            const int i = asInt(O.w);
+            const DevJob& J = jobs[jobOf(base, nJobs, i)];
+            const Vec3 so = xyz(O), sd = xyz(D);
+            const RayPre r = resumableRay<TRI, false>(so, sd, 0.f);
+            while (!resumableStep<true, TRI, false>(J.S, so, sd, 0.f, r, h, st, ts)) {}
+            if (h.tri == -1) {
+                vis = true;
+                const Vec4 c = P.sC[pos];
                 Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
             }
         }
@@ -1201,15 +1322,54 @@ __device__ __forceinline__ int camJobOf(const DevStepTable* __restrict__ T, int 
     return lo;
 }
 
-template <int V, int TRI, int SL, int WAVES, bool FULL = false, bool LANE = false>
+// The ray of entry `code` of a rolling step: the continuation at slot `code` of ray set inSet (code >= 0) or the
+// camera ray of the step's camera index -(code + 1); its path id, job, pixel and sample index.
+struct RollEntry { int i, j, pix; uint32_t smp; };
+__device__ __forceinline__ RollEntry rollEntry(const DevStepTable* __restrict__ T, const int* base, int nJobs, const DevPaths& P,
+                                               int inSet, int code, Vec3& o, Vec3& d, float& tmin) {
+    RollEntry e;
+    Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
+    if (code >= 0) {
+        Or = P.O[inSet][code];
+        e.i = asInt(Or.w);
+        e.j = jobOf(base, nJobs, e.i);
+    } else {
+        const int c = -(code + 1);
+        const int q = camJobOf(T, c);
+        e.j = T->newJob[q];
+        e.i = base[e.j] + (c - T->camFirst[q]);
+    }
+    const DevJob& J = T->jobs[e.j];
+    const int li = e.i - base[e.j], Kj = J.K;
+    e.pix = li / Kj;
+    e.smp = (uint32_t)(J.smpBase + li % Kj);
+    if (code < 0) { cameraRay(J.F, e.pix, e.smp, o, d); tmin = J.F.nearT; }
+    else { o = xyz(Or); d = xyz(P.D[inSet][code]); tmin = 0.f; }
+    return e;
+}
+// shading of a rolling entry's traced ray; `depth` is the path's (0 for camera rays). Nothing for a handed-off one.
+template <bool FULL>
+__device__ __forceinline__ ShadeOut rollShade(const DevJob& J, const DevPaths& P, int inSet, int code, const RollEntry& e,
+                                              Vec3 o, Vec3 d, const Hit& h, int& depth) {
+    Vec3 Tp = v3(1.f, 1.f, 1.f);
+    float lastPdf = 0.f;
+    depth = 0;
+    if (code >= 0) { const Vec4 Tv = P.T[inSet][code]; Tp = xyz(Tv); depth = asInt(Tv.w); lastPdf = P.D[inSet][code].w; }
+    if constexpr (FULL) return shadeFull<false>(J.S, P, J.F, depth, e.i, e.pix, e.smp, o, d, Tp, lastPdf, h);
+    else return shadeLegacy<false>(J.S, P, J.F, depth, e.i, e.pix, e.smp, o, d, Tp, h);
+}
+
+// TAIL (BVH8): the straggler hand-off of k_bounce; a record's slot is the entry's code (camera rays negative), and
+// k_bounce_roll_tail resumes the records of the step.
+template <int V, int TRI, int SL, int WAVES, bool FULL = false, bool LANE = false, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTable* __restrict__ T, DevPaths P, int step) {
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     __shared__ int base[kMaxJobs];
     const int nJobs = T->nJobs;
     const DevJob* __restrict__ jobs = T->jobs;
-    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
-    __syncthreads();
+    loadJobBases(jobs, nJobs, base);
     const int lane = laneId();
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
@@ -1230,33 +1390,19 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTabl
         ShadeOut res;
         res.cont = false; res.shadow = false; res.miss = false;
         if (k < n) {
-            int j, slot = 0;
-            Vec4 Or = v4(0.f, 0.f, 0.f, 0.f);
-            if (k < nCont) {
-                slot = contStart + k;
-                Or = P.O[inSet][slot];
-                i = asInt(Or.w);
-                j = jobOf(base, nJobs, i);
-            } else {
-                cam = true;
-                const int c = camLo + (k - nCont);
-                const int q = camJobOf(T, c);
-                j = T->newJob[q];
-                i = base[j] + (c - T->camFirst[q]);
-            }
-            const DevJob& J = jobs[j];
-            const int li = i - base[j], Kj = J.K, pix = li / Kj;
-            const uint32_t smp = (uint32_t)(J.smpBase + li % Kj);
+            const int code = k < nCont ? contStart + k : -(camLo + (k - nCont)) - 1;
+            cam = code < 0;
             Vec3 o, d; float tmin;
-            if (cam) { cameraRay(J.F, pix, smp, o, d); tmin = J.F.nearT; }
-            else { o = xyz(Or); d = xyz(P.D[inSet][slot]); tmin = 0.f; }
+            const RollEntry e = rollEntry(T, base, nJobs, P, inSet, code, o, d, tmin);
+            const DevJob& J = jobs[e.j];
+            i = e.i;
             Hit h; h.t = 1e30f; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
-            Vec3 Tp = v3(1.f, 1.f, 1.f);
-            float lastPdf = 0.f;
-            if (!cam) { const Vec4 Tv = P.T[inSet][slot]; Tp = xyz(Tv); depth = asInt(Tv.w); lastPdf = P.D[inSet][slot].w; }
-            if constexpr (FULL) res = shadeFull<false>(J.S, P, J.F, depth, i, pix, smp, o, d, Tp, lastPdf, h);
-            else res = shadeLegacy<false>(J.S, P, J.F, depth, i, pix, smp, o, d, Tp, h);
+            if constexpr (TAIL)
+                traceOrHandOff<false, TRI, false>(J.S, P, P.tail, tailFirst(outStart, part, P.tailLanes), ctr(blk, part, kCTail),
+                                                  code, o, d, tmin, h, st);
+            else
+                traceRay<V, false, TRI, false>(J.S, o, d, tmin, h, st);
+            res = rollShade<FULL>(J, P, inSet, code, e, o, d, h, depth);
         }
         camMisses += (int)__builtin_popcountll(__ballot(cam && res.miss));
         misses += appendTo<LANE>(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
@@ -1265,16 +1411,60 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTabl
     waveAdd(camMisses, ctr(blk, part, kCMissCam));
 }
 
-// the step's shadow rays (as k_shadow_jobs, counters of block step & 1)
-template <int V, int TRI, int SL>
+// The tail of a hand-off rolling step (after k_bounce_roll<..., TAIL>, before the step's shadow kernel): its handed-off
+// traversals resumed to the end in full waves, each with its job's scene, then shaded and appended like the step's own.
+template <int TRI, int SL, int WAVES, bool FULL, bool LANE>
+__global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll_tail(const DevStepTable* __restrict__ T, DevPaths P, int step) {
+    __shared__ Group lds[SL * kBlock];
+    __shared__ int base[kMaxJobs];
+    const int nJobs = T->nJobs;
+    const DevJob* __restrict__ jobs = T->jobs;
+    loadJobBases(jobs, nJobs, base);
+    const int lane = laneId();
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
+    const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
+    int* blk = P.cnt + (step & 1) * kCntStride;
+    const int nt = ctrv(blk, part, kCTail);
+    const int outStart = partStart(blk, part, kCIn);
+    const int inSet = step & 1, outSet = (step + 1) & 1;
+    const Vec4* tail = P.tail + tailFirst(outStart, part, P.tailLanes) * (size_t)P.tailStride;
+    int misses = 0, camMisses = 0;
+    for (int b0 = lw * kWave; b0 < nt; b0 += nw * kWave) {
+        const int k = b0 + lane;
+        int i = 0, depth = 0;
+        bool cam = false;
+        ShadeOut res;
+        res.cont = false; res.shadow = false; res.miss = false;
+        if (k < nt) {
+            Hit h; TravState ts;
+            const int code = resumeRecord(tail + (size_t)k * P.tailStride, h, ts, st);
+            cam = code < 0;
+            Vec3 o, d; float tmin;
+            const RollEntry e = rollEntry(T, base, nJobs, P, inSet, code, o, d, tmin);
+            const DevJob& J = jobs[e.j];
+            i = e.i;
+            const RayPre r = resumableRay<TRI, false>(o, d, tmin);
+            while (!resumableStep<false, TRI, false>(J.S, o, d, tmin, r, h, st, ts)) {}
+            res = rollShade<FULL>(J, P, inSet, code, e, o, d, h, depth);
+        }
+        camMisses += (int)__builtin_popcountll(__ballot(cam && res.miss));
+        misses += appendTo<LANE>(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
+    }
+    waveAdd(misses, ctr(blk, part, kCMiss));
+    waveAdd(camMisses, ctr(blk, part, kCMissCam));
+}
+
+// the step's shadow rays (as k_shadow_jobs, counters of block step & 1; TAIL: with the hand-off)
+template <int V, int TRI, int SL, bool TAIL = false>
 __global__ void __launch_bounds__(kBlock, 8) k_shadow_roll(const DevStepTable* __restrict__ T, DevPaths P, int step) {
+    static_assert(!TAIL || V == 8, "the hand-off resumes BVH8 traversals");
     using E = typename StackEntry<V>::type;
     __shared__ E lds[SL > 0 ? SL * kBlock : 1];
     __shared__ int base[kMaxJobs];
     const int nJobs = T->nJobs;
     const DevJob* __restrict__ jobs = T->jobs;
-    for (int j = threadIdx.x; j < nJobs; j += kBlock) base[j] = jobs[j].pathBase;
-    __syncthreads();
+    loadJobBases(jobs, nJobs, base);
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     LaneStack<E, SL> st{ (LdsPtr<E>)(lds + threadIdx.x), (E*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
@@ -1290,7 +1480,15 @@ __global__ void __launch_bounds__(kBlock, 8) k_shadow_roll(const DevStepTable* _
             const int i = asInt(O.w);
             const DevJob& J = jobs[jobOf(base, nJobs, i)];
             Hit h; h.t = D.w; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1;
-            if (!traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st)) {
+            bool occluded;
+            if constexpr (TAIL) {
+                traceOrHandOff<true, TRI, false>(J.S, P, P.tail, tailFirst(start, part, P.tailLanes), ctr(blk, part, kCShadowTail),
+                                                 start + k, xyz(O), xyz(D), 0.f, h, st);
+                occluded = h.tri != -1;
+            } else {
+                occluded = traceRay<V, true, TRI, false>(J.S, xyz(O), xyz(D), 0.f, h, st);
+            }
+            if (!occluded) {
                 vis = true;
                 const Vec4 c = P.sC[start + k];
                 Vec4 l = P.L[i]; l.x += c.x; l.y += c.y; l.z += c.z; P.L[i] = l;
@@ -1343,6 +1541,11 @@ struct Kernels {
     ShadowFn shadowHandInst = nullptr, shadowTailInst = nullptr, shadowTail2Inst = nullptr;
     // depth 0 with the first-bounce sort fused in (kFirstTile-thread workgroups)
     FirstFn bounceFirst = nullptr, bounceFirstFull = nullptr;
+    // the hand-off in batches (BVH8): pass kernels that hand off and their tails; rolling steps [full][per-lane]
+    JobsFn bounceJobsHand = nullptr, bounceJobsHandFull = nullptr, bounceJobsTail = nullptr, bounceJobsTailFull = nullptr;
+    JobsFn shadowJobsHand = nullptr, shadowJobsTail = nullptr;
+    RollFn bounceRollHand[2][2] = {}, bounceRollTail[2][2] = {};
+    RollFn shadowRollHand = nullptr, shadowRollTail = nullptr;
 };
 
 template <int V, int TRI, int SL>
@@ -1390,6 +1593,22 @@ void pickSL(int waves, Kernels& k) {
         k.shadowHandInst = k_shadow<V, TRI, SL, true, true>;
         k.shadowTailInst = k_shadow_tail<TRI, SL, 1, true>;
         k.shadowTail2Inst = k_shadow_tail<TRI, SL, 2, true>;
+        k.bounceJobsHand = k_bounce_jobs<V, TRI, SL, 8, false, true>;
+        k.bounceJobsHandFull = k_bounce_jobs<V, TRI, SL, kFullWaves, true, true>;
+        k.bounceJobsTail = k_bounce_jobs_tail<TRI, SL, 8, false>;
+        k.bounceJobsTailFull = k_bounce_jobs_tail<TRI, SL, kFullWaves, true>;
+        k.shadowJobsHand = k_shadow_jobs<V, TRI, SL, true>;
+        k.shadowJobsTail = k_shadow_jobs_tail<TRI, SL, false, DevJobTable>;
+        k.bounceRollHand[0][0] = k_bounce_roll<V, TRI, SL, 8, false, false, true>;
+        k.bounceRollHand[0][1] = k_bounce_roll<V, TRI, SL, 8, false, true, true>;
+        k.bounceRollHand[1][0] = k_bounce_roll<V, TRI, SL, kFullWaves, true, false, true>;
+        k.bounceRollHand[1][1] = k_bounce_roll<V, TRI, SL, kFullWaves, true, true, true>;
+        k.bounceRollTail[0][0] = k_bounce_roll_tail<TRI, SL, 8, false, false>;
+        k.bounceRollTail[0][1] = k_bounce_roll_tail<TRI, SL, 8, false, true>;
+        k.bounceRollTail[1][0] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, false>;
+        k.bounceRollTail[1][1] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, true>;
+        k.shadowRollHand = k_shadow_roll<V, TRI, SL, true>;
+        k.shadowRollTail = k_shadow_jobs_tail<TRI, SL, true, DevStepTable>;
     }
     k.bounceFirst = k_bounce_first<V, TRI, SL, 6, false>;
     k.bounceFirstFull = k_bounce_first<V, TRI, SL, kFullWaves, true>;

@@ -34,7 +34,7 @@ bool Server::reconfigure(const BatchOptions& B, std::string& err) {
     std::unique_ptr<BatchRenderer> R(new BatchRenderer());
     if (!R->init(B, err)) return false;
     R_ = std::move(R);
-    O_.batch = B;
+    O_.batch = R_->options();   // as resolved by the renderer (the automatic hand-off threshold)
     return true;
 }
 
@@ -164,6 +164,7 @@ bool Server::autotune(const std::vector<int>& variants, const std::vector<int>& 
                 for (int cm : compacts) {
                     BatchOptions B = base;
                     B.variant = v; B.stackLds = sl; B.raysPerLane = r; B.compact = cm;
+                    if (v != 8) B.tail = 0;   // the hand-off resumes BVH8 traversals only
                     resetJobs();
                     if (!reconfigure(B, err) || !preload(err) || !run(err)) { O_.batched = batched; return false; }
                     TuneTrial t;
@@ -175,6 +176,7 @@ bool Server::autotune(const std::vector<int>& variants, const std::vector<int>& 
     O_.batched = batched;
     BatchOptions B = base;
     B.variant = best.variant; B.stackLds = best.stackLds; B.raysPerLane = best.raysPerLane; B.compact = best.compact;
+    if (B.variant != 8) B.tail = 0;
     resetJobs();
     stats_ = ServeStats();
     return reconfigure(B, err);

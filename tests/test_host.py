@@ -364,3 +364,12 @@ def test_server_needs_a_device():
         pass
     with pytest.raises(cwp.CwpError, match="cwp error 3"):
         cwp.Server(o)
+
+
+@pytest.mark.parametrize("variant,tail", [(8, 1), (8, 65), (8, -2), (4, 12), (2, 2)])
+def test_server_hand_off_option_errors(variant, tail):
+    """The batch hand-off threshold is -1 (automatic), 0 or 2..64 lanes, and resumes BVH8 traversals only; both are
+    argument errors found before any device call."""
+    assert cwp.ServerOptions.default().tail_lanes == -1
+    with pytest.raises(cwp.CwpError, match="cwp error 1.*(tail|BVH8)"):
+        cwp.Server(variant=variant, tail_lanes=tail)
