@@ -27,9 +27,9 @@ struct BatchOptions {
     int useGraph = 1;            // replay each pass's launches from a captured HIP graph
     int full = 0;                // 1: the full material integrator (every scene full); 0: the legacy integrator
     int compact = 0;             // queue compaction of rolling steps: 0 per wave (ballot, one atomic), 1 per lane
-    int tail = -1;               // straggler hand-off of the trace+shade and shadow kernels (BVH8): a wave with fewer
-                                 // lanes than this still tracing hands their traversals to a tail kernel;
-                                 // -1 kTailDefault with BVH8 (else off), 0 off, 2..64
+    int tail = -1;               // straggler hand-off (BVH8) of the trace+shade kernels, and with the full integrator
+                                 // of the shadow kernels: a wave with fewer lanes than this still tracing hands their
+                                 // traversals to a tail kernel; -1 kTailDefault with BVH8 (else off), 0 off, 2..64
 };
 
 struct BatchJob {

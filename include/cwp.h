@@ -389,9 +389,10 @@ typedef struct cwp_server_options {
                                           camera rays of the jobs admitted at that step); 0 = between whole passes */
     int compact;                       /* rolling steps' queue compaction: 0 per wave (one ballot and one atomic per
                                           wave), 1 per lane (one atomic per appended ray) */
-    int tail_lanes;                    /* straggler hand-off of the trace+shade and shadow launches (BVH8 only): a wave
-                                          whose lanes still tracing drop below this many hands their traversals to a
-                                          tail launch that resumes them in full waves (images unchanged); -1 automatic
+    int tail_lanes;                    /* straggler hand-off (BVH8 only) of the trace+shade launches, and with the
+                                          full integrator of the shadow launches: a wave whose lanes still tracing
+                                          drop below this many hands their traversals to a tail launch that resumes
+                                          them in full waves (images unchanged); -1 automatic
                                           (12 with BVH8, else off), 0 off, 2..64; cwp_server_get_options reports
                                           the resolved value. The autotuner keeps it for BVH8 trials and turns it
                                           off for the others. */
