@@ -354,7 +354,7 @@ def bench_jobs(args):
     print("bench: %d scenes (%s tris) set up in %.1f s" % (len(setups), ",".join(str(t) for t in tris),
                                                          time.perf_counter() - t0), file=sys.stderr, flush=True)
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail)
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail, tail_levels=args.tail_levels)
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)
@@ -383,7 +383,8 @@ def bench_jobs(args):
                           "batched_p50": round(batched["latency_p50_s"], 4), "batched_p99": round(batched["latency_p99_s"], 4)},
             "passes": {"plain": plain["passes"], "batched": batched["passes"]},
             "chosen": {"bvh": chosen["variant"], "stack_lds": chosen["stack_lds"], "rays_per_lane": chosen["rays_per_lane"],
-                       "compaction": ("wave", "lane")[chosen["compact"]], "tail_lanes": chosen["tail_lanes"]},
+                       "compaction": ("wave", "lane")[chosen["compact"]], "tail_lanes": chosen["tail_lanes"],
+                       "tail_levels": chosen["tail_levels"]},
             "trials": [{"bvh": t["variant"], "stack": t["stack_lds"], "rpl": t["rays_per_lane"],
                         "compaction": ("wave", "lane")[t["compact"]], "grays": round(t["rays_per_s"] / 1e9, 4)}
                        for t in trials],
@@ -397,7 +398,7 @@ def bench_serve(args):
     import cwp
     setups = make_setups(job_stream_scenes(args.job_scenes, args.job_seed, full=bool(args.full)))
     jobs = job_stream(args.jobs, len(setups), args.job_seed, args.job_res, args.job_spp)
-    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail)
+    srv = cwp.Server(variant=8, bounces=6, full=int(args.full), tail_lanes=args.tail, tail_levels=args.tail_levels)
     for s in setups:
         srv.add_scene(s)
     warm_up(srv, jobs)

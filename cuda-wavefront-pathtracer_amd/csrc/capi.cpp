@@ -681,6 +681,7 @@ static void serverOptionsOut(const ServeOptions& d, cwp_server_options* o) {
     o->max_paths = d.batch.maxPaths; o->max_jobs = d.batch.maxJobs; o->scene_budget_mb = d.sceneBudgetMB;
     o->batched = d.batched ? 1 : 0; o->rays_per_lane = d.batch.raysPerLane; o->use_graph = d.batch.useGraph;
     o->full = d.batch.full; o->rolling = d.rolling ? 1 : 0; o->compact = d.batch.compact; o->tail_lanes = d.batch.tail;
+    o->tail_levels = d.batch.tailLevels;
 }
 
 void cwp_server_options_default(cwp_server_options* o) {
@@ -701,6 +702,7 @@ int cwp_server_create(const cwp_server_options* o, cwp_server** out) {
     so.batch.maxPaths = o->max_paths; so.batch.maxJobs = o->max_jobs; so.sceneBudgetMB = o->scene_budget_mb;
     so.batched = o->batched != 0; so.batch.raysPerLane = o->rays_per_lane; so.batch.useGraph = o->use_graph;
     so.batch.full = o->full; so.rolling = o->rolling != 0; so.batch.compact = o->compact; so.batch.tail = o->tail_lanes;
+    so.batch.tailLevels = o->tail_levels;
     cwp_server* s = new (std::nothrow) cwp_server();
     if (!s) return fail(CWP_ERR_ARG, "out of host memory");
     std::string err;

@@ -30,6 +30,8 @@ struct BatchOptions {
     int tail = -1;               // straggler hand-off (BVH8) of the trace+shade kernels, and with the full integrator
                                  // of the shadow kernels: a wave with fewer lanes than this still tracing hands their
                                  // traversals to a tail kernel; -1 kTailDefault with BVH8 (else off), 0 off, 2..64
+    int tailLevels = 2;          // 2: the rolling steps' trace+shade tail kernels hand their own stragglers on to a
+                                 // second tail (passes resume every record to the end)
 };
 
 struct BatchJob {

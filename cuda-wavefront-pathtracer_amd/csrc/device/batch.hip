@@ -94,8 +94,14 @@ struct BatchRenderer::Impl {
         if (P.tail && stride <= P.tailStride) return true;
         HIPCHK(hipStreamSynchronize(st));
         if (P.tail) { hipFree(P.tail); P.tail = nullptr; }
+        if (P.tail2) { hipFree(P.tail2); P.tail2 = nullptr; }
         const size_t records = ((O.maxPaths >> 6) + kParts + 1) * (size_t)(O.tail - 1);
         HIPCHK(hipMalloc(&P.tail, records * (size_t)stride * sizeof(Vec4)));
+        if (O.tailLevels == 2) {   // the rolling steps' first-tail records are the second level's input (at most
+                                   // lanes - 1 records per 64 of them)
+            const size_t records2 = ((records >> 6) + kParts + 1) * (size_t)(O.tail - 1);
+            HIPCHK(hipMalloc(&P.tail2, records2 * (size_t)stride * sizeof(Vec4)));
+        }
         P.tailStride = stride; P.tailLanes = O.tail;
         pathGen++;
         return true;
@@ -132,6 +138,7 @@ struct BatchRenderer::Impl {
         if (O.tail) {
             kn.bounceRollHand[O.full ? 1 : 0][O.compact]<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
             kn.bounceRollTail[O.full ? 1 : 0][O.compact]<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
+            if (P.tail2) kn.bounceRollTail2[O.full ? 1 : 0][O.compact]<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
         } else {
             const RollFn b = O.compact ? (O.full ? kn.bounceRollFullLane : kn.bounceRollLane) : (O.full ? kn.bounceRollFull : kn.bounceRoll);
             b<<<gb, kBlock, 0, st>>>(dStep[s], P, step);
@@ -185,6 +192,7 @@ BatchRenderer::~BatchRenderer() {
     for (void* a : p_->allocs) hipFree(a);
     if (p_->P.spill) hipFree(p_->P.spill);
     if (p_->P.tail) hipFree(p_->P.tail);
+    if (p_->P.tail2) hipFree(p_->P.tail2);
     for (int k = 0; k < kTableSlots; k++) {
         if (p_->graph[k]) hipGraphExecDestroy(p_->graph[k]);
         if (p_->hTable[k]) hipHostFree(p_->hTable[k]);
@@ -217,6 +225,7 @@ bool BatchRenderer::init(const BatchOptions& O, std::string& err) {
     if (O.compact != 0 && O.compact != 1) { err = "batch: compaction must be 0 (per wave) or 1 (per lane)"; return false; }
     if (O.tail < -1 || O.tail == 1 || O.tail > kWave) { err = "batch: tail must be -1 (auto), 0 (off) or 2..64 lanes"; return false; }
     if (O.tail > 0 && O.variant != 8) { err = "batch: the straggler hand-off (tail) resumes BVH8 traversals only"; return false; }
+    if (O.tailLevels != 1 && O.tailLevels != 2) { err = "batch: tail levels must be 1 or 2"; return false; }
     I.O = O; I.sl = O.stackLds;
     I.O.tail = O.tail == -1 ? (O.variant == 8 ? kTailDefault : 0) : O.tail;
     std::string name;

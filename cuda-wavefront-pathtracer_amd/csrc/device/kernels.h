@@ -1166,7 +1166,8 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs(const DevJobTable
 
 // The tail of a hand-off pass bounce (run after k_bounce_jobs<..., TAIL> of the same depth, before its shadow
 // kernel): the handed-off traversals resumed from their records in full waves, each with its job's scene, to the end;
-// then shaded and appended like the bounce's own rays.
+// then shaded and appended like the bounce's own rays. One level: a second one (as the rolling steps') made the
+// one-job-at-a-time stream 2.7 % slower.
 template <int TRI, int SL, int WAVES, bool FULL>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_jobs_tail(const DevJobTable* __restrict__ T, DevPaths P, int depth) {
     __shared__ Group lds[SL * kBlock];
@@ -1411,9 +1412,10 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll(const DevStepTabl
     waveAdd(camMisses, ctr(blk, part, kCMissCam));
 }
 
-// The tail of a hand-off rolling step (after k_bounce_roll<..., TAIL>, before the step's shadow kernel): its handed-off
-// traversals resumed to the end in full waves, each with its job's scene, then shaded and appended like the step's own.
-template <int TRI, int SL, int WAVES, bool FULL, bool LANE>
+// The tails of a hand-off rolling step (after k_bounce_roll<..., TAIL>, before the step's shadow kernel), as those of a
+// pass: LEVEL 1 resumes the step's handed-off traversals in full waves, each with its job's scene, handing its own
+// stragglers on when P.tail2 is set; LEVEL 2 resumes those to the end. Each is shaded and appended like the step's own.
+template <int TRI, int SL, int WAVES, bool FULL, bool LANE, int LEVEL = 1>
 __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll_tail(const DevStepTable* __restrict__ T, DevPaths P, int step) {
     __shared__ Group lds[SL * kBlock];
     __shared__ int base[kMaxJobs];
@@ -1425,10 +1427,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll_tail(const DevSte
     LaneStack<Group, SL> st{ (LdsPtr<Group>)(lds + threadIdx.x), (Group*)P.spill + (size_t)gtid * P.spillCap, 0, 0 };
     const int part = (int)(blockIdx.x % kParts), lw = (int)(blockIdx.x / kParts), nw = (int)(gridDim.x / kParts);
     int* blk = P.cnt + (step & 1) * kCntStride;
-    const int nt = ctrv(blk, part, kCTail);
+    const int nt = ctrv(blk, part, LEVEL == 1 ? kCTail : kCTail2);
     const int outStart = partStart(blk, part, kCIn);
     const int inSet = step & 1, outSet = (step + 1) & 1;
-    const Vec4* tail = P.tail + tailFirst(outStart, part, P.tailLanes) * (size_t)P.tailStride;
+    const size_t first1 = tailFirst(outStart, part, P.tailLanes), first2 = tailFirst(first1, part, P.tailLanes);
+    const Vec4* tail = LEVEL == 1 ? P.tail + first1 * (size_t)P.tailStride : P.tail2 + first2 * (size_t)P.tailStride;
     int misses = 0, camMisses = 0;
     for (int b0 = lw * kWave; b0 < nt; b0 += nw * kWave) {
         const int k = b0 + lane;
@@ -1445,8 +1448,10 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_bounce_roll_tail(const DevSte
             const DevJob& J = jobs[e.j];
             i = e.i;
             const RayPre r = resumableRay<TRI, false>(o, d, tmin);
-            while (!resumableStep<false, TRI, false>(J.S, o, d, tmin, r, h, st, ts)) {}
-            res = rollShade<FULL>(J, P, inSet, code, e, o, d, h, depth);
+            if (LEVEL == 1 && P.tail2)
+                stepOrHandOff<false, TRI, false>(J.S, P, P.tail2, first2, ctr(blk, part, kCTail2), code, o, d, tmin, r, h, st, ts);
+            else while (!resumableStep<false, TRI, false>(J.S, o, d, tmin, r, h, st, ts)) {}
+            res = rollShade<FULL>(J, P, inSet, code, e, o, d, h, depth);   // nothing if handed on
         }
         camMisses += (int)__builtin_popcountll(__ballot(cam && res.miss));
         misses += appendTo<LANE>(P, ctr(blk, part, kCQueue), ctr(blk, part, kCShadow), outStart, outSet, depth + 1, i, res);
@@ -1544,7 +1549,7 @@ struct Kernels {
     // the hand-off in batches (BVH8): pass kernels that hand off and their tails; rolling steps [full][per-lane]
     JobsFn bounceJobsHand = nullptr, bounceJobsHandFull = nullptr, bounceJobsTail = nullptr, bounceJobsTailFull = nullptr;
     JobsFn shadowJobsHand = nullptr, shadowJobsTail = nullptr;
-    RollFn bounceRollHand[2][2] = {}, bounceRollTail[2][2] = {};
+    RollFn bounceRollHand[2][2] = {}, bounceRollTail[2][2] = {}, bounceRollTail2[2][2] = {};   // tail2: second level
     RollFn shadowRollHand = nullptr, shadowRollTail = nullptr;
 };
 
@@ -1607,6 +1612,10 @@ void pickSL(int waves, Kernels& k) {
         k.bounceRollTail[0][1] = k_bounce_roll_tail<TRI, SL, 8, false, true>;
         k.bounceRollTail[1][0] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, false>;
         k.bounceRollTail[1][1] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, true>;
+        k.bounceRollTail2[0][0] = k_bounce_roll_tail<TRI, SL, 8, false, false, 2>;
+        k.bounceRollTail2[0][1] = k_bounce_roll_tail<TRI, SL, 8, false, true, 2>;
+        k.bounceRollTail2[1][0] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, false, 2>;
+        k.bounceRollTail2[1][1] = k_bounce_roll_tail<TRI, SL, kFullWaves, true, true, 2>;
         k.shadowRollHand = k_shadow_roll<V, TRI, SL, true>;
         k.shadowRollTail = k_shadow_jobs_tail<TRI, SL, true, DevStepTable>;
     }

@@ -115,7 +115,8 @@ class ServerOptions(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int), ("tri_test", ctypes.c_int), ("stack_lds", ctypes.c_int), ("bounces", ctypes.c_int),
                 ("max_paths", ctypes.c_size_t), ("max_jobs", ctypes.c_int), ("scene_budget_mb", ctypes.c_double),
                 ("batched", ctypes.c_int), ("rays_per_lane", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("full", ctypes.c_int), ("rolling", ctypes.c_int), ("compact", ctypes.c_int), ("tail_lanes", ctypes.c_int)]
+                ("full", ctypes.c_int), ("rolling", ctypes.c_int), ("compact", ctypes.c_int), ("tail_lanes", ctypes.c_int),
+                ("tail_levels", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -396,6 +396,9 @@ typedef struct cwp_server_options {
                                           (12 with BVH8, else off), 0 off, 2..64; cwp_server_get_options reports
                                           the resolved value. The autotuner keeps it for BVH8 trials and turns it
                                           off for the others. */
+    int tail_levels;                   /* 2 (default): the rolling steps' trace+shade tail launches hand their own
+                                          stragglers on to a second tail launch; 1: they resume every record to the
+                                          end (as the pass mode's always do: a second level made it slower) */
 } cwp_server_options;
 typedef struct cwp_tune_trial {
     int variant, stack_lds, rays_per_lane, compact;

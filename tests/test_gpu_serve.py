@@ -280,30 +280,32 @@ def test_per_lane_compaction_gives_the_same_images():
 def test_straggler_hand_off_gives_the_same_images(full, rolling, compact):
     """The hand-off in batches (a wave with fewer than tail_lanes lanes still tracing writes their traversals as tail
     records; a tail launch resumes them with each record's job and scene) renders the same images bit for bit as no
-    hand-off, at every threshold: 64 hands off at the first finished lane, 2 only the last lane. Rolling steps (camera
-    rays of new jobs resumed by camera index) and whole passes, both compactions, both integrators; 2^14 path slots
-    make the larger jobs take several rounds."""
+    hand-off, at every threshold: 64 hands off at the first finished lane, 2 only the last lane; with one level and
+    with two (the first tail launch hands its own stragglers on). Rolling steps (camera rays of new jobs resumed by
+    camera index) and whole passes, both compactions, both integrators; 2^14 path slots make the larger jobs take
+    several rounds."""
     mats = ["--gen_materials", "4", "--gen_env", "64,32,2,5"] if full else []
     S = [cwp.Setup(TERRAIN + mats + ["--bounces", "4"]), cwp.Setup(BOXES + (["--gen_materials", "6"] if full else []) +
                                                                ["--bounces", "4"])]
     out = {}
-    for tail in (0, 2, 12, 64):
-        print("case hand-off", full, rolling, compact, tail, flush=True)
-        srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 14, full=full, rolling=rolling, compact=compact, tail_lanes=tail)
-        assert srv.get_options()["tail_lanes"] == tail
+    for tail, levels in ((0, 2), (2, 2), (12, 2), (64, 2), (12, 1), (64, 1)):
+        print("case hand-off", full, rolling, compact, tail, levels, flush=True)
+        srv = cwp.Server(variant=8, bounces=4, max_paths=1 << 14, full=full, rolling=rolling, compact=compact, tail_lanes=tail,
+                         tail_levels=levels)
+        assert (srv.get_options()["tail_lanes"], srv.get_options()["tail_levels"]) == (tail, levels)
         for s in S:
             srv.add_scene(s)
         imgs, infos, st = run_jobs(srv, JOBS, True)
-        out[tail] = (imgs, st["rays"])
+        out[tail, levels] = (imgs, st["rays"])
         if tail == 12:
             plain, _, _ = run_jobs(srv, JOBS, False)
             for k in range(len(JOBS)):
                 np.testing.assert_array_equal(plain[k], imgs[k], err_msg="plain job %d" % k)
         srv.close()
-    for tail in (2, 12, 64):
-        assert out[tail][1] == out[0][1]
+    for tail in ((2, 2), (12, 2), (64, 2), (12, 1), (64, 1)):
+        assert out[tail][1] == out[0, 2][1]
         for k in range(len(JOBS)):
-            np.testing.assert_array_equal(out[tail][0][k], out[0][0][k], err_msg="tail %d job %d" % (tail, k))
+            np.testing.assert_array_equal(out[tail][0][k], out[0, 2][0][k], err_msg="tail %s job %d" % (tail, k))
 
 
 def test_live_submissions_from_threads_while_jobs_run():

@@ -366,10 +366,12 @@ def test_server_needs_a_device():
         cwp.Server(o)
 
 
-@pytest.mark.parametrize("variant,tail", [(8, 1), (8, 65), (8, -2), (4, 12), (2, 2)])
-def test_server_hand_off_option_errors(variant, tail):
-    """The batch hand-off threshold is -1 (automatic), 0 or 2..64 lanes, and resumes BVH8 traversals only; both are
-    argument errors found before any device call."""
-    assert cwp.ServerOptions.default().tail_lanes == -1
+@pytest.mark.parametrize("variant,tail,levels", [(8, 1, 2), (8, 65, 2), (8, -2, 2), (4, 12, 2), (2, 2, 2), (8, 12, 3),
+                                                 (8, -1, 0)])
+def test_server_hand_off_option_errors(variant, tail, levels):
+    """The batch hand-off threshold is -1 (automatic), 0 or 2..64 lanes, with 1 or 2 levels, and resumes BVH8
+    traversals only; all are argument errors found before any device call."""
+    o = cwp.ServerOptions.default()
+    assert (o.tail_lanes, o.tail_levels) == (-1, 2)
     with pytest.raises(cwp.CwpError, match="cwp error 1.*(tail|BVH8)"):
-        cwp.Server(variant=variant, tail_lanes=tail)
+        cwp.Server(variant=variant, tail_lanes=tail, tail_levels=levels)
