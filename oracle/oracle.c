@@ -5,7 +5,8 @@
  *
  * What pins each part (the reference itself cannot be built here: its headers need the CUDA toolkit's
  * cuda_runtime.h for the float3/float4 types, and it has no golden vectors; see tests/test_oracle.py):
- *  - orc_hash32 / orc_rng4: known-answer values of the hash (the reference's constants) and uniformity statistics.
+ *  - orc_hash32: Wellons' lowbias32 (hash-prospector), undone by its published inverse on 69,635 inputs.
+ *  - orc_rng4: uniformity and independence statistics only (parity unpinned beyond them: no vectors exist).
  *  - orc_blackman_harris_table: monotone, antisymmetric, endpoints +-width/2, and the tabulated CDF matches the
  *    closed-form integral of the Blackman-Harris window.
  *  - warps: unit length, pdf moments (cosine hemisphere: E[cos] = 2/3; uniform cone: E[cos] = (1 + cos_max)/2).

@@ -3,8 +3,10 @@
 Each test checks an oracle function against something its formula cannot agree with by construction: a closed form,
 an invariant, brute force, or a statistical law. What each function is pinned by:
 
-  rng4 / hash32          bit balance, uniformity and independence statistics (parity pinned by statistics only: the
-                         reference ships no RNG vectors and cannot be built here)
+  hash32                 the published inverse of Wellons' lowbias32 (hash-prospector), whose constants are the
+                         modular inverses of the forward ones: inverse(hash32(x)) = x on 69,635 inputs
+  rng4                   bit balance, uniformity and independence statistics (its composition of hashes is pinned by
+                         statistics only: the reference ships no RNG vectors and cannot be built here)
   Blackman-Harris table  closed-form CDF of the window (integral of the cosine series)
   warps                  unit length, hemisphere / cone support, exact moments (E[cos] = 2/3; cone cos uniform)
   SAH builder            primitives exactly once, boxes = union of contents, leaf sizes, a separable two-cluster split
@@ -36,6 +38,30 @@ def test_rng_uniform_and_independent():
     w = np.array([po.rng4(i, 8, 3, 2, 1000) for i in range(n)])
     assert abs(np.corrcoef(u, v)[0, 1]) < 0.03
     assert abs(np.corrcoef(u, w)[0, 1]) < 0.03
+
+
+def lowbias32_inverse(x):
+    """The published inverse of lowbias32 (C. Wellons, hash-prospector, 2018): every step of the hash undone in
+    reverse order, the multipliers replaced by their inverses mod 2^32."""
+    x = np.asarray(x, dtype=np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x43021123)) & m
+    x ^= (x >> np.uint64(15)) ^ (x >> np.uint64(30))
+    x = (x * np.uint64(0x1D69E2A5)) & m
+    x ^= x >> np.uint64(16)
+    return x
+
+
+def test_hash_is_lowbias32():
+    """The hash is a bijection undone by lowbias32's published inverse: a wrong shift or multiplier would not be."""
+    assert (0x846CA68B * 0x43021123) % (1 << 32) == 1 and (0x7FEB352D * 0x1D69E2A5) % (1 << 32) == 1
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([np.arange(4096), rng.integers(0, 1 << 32, 1 << 16, dtype=np.uint64),
+                         [0xFFFFFFFF, 0x80000000, 0x7FFFFFFF]]).astype(np.uint64)
+    hs = np.array([po.hash32(int(x)) for x in xs], dtype=np.uint64)
+    assert np.array_equal(lowbias32_inverse(hs), xs)
+    assert po.hash32(0) == 0   # every step maps 0 to 0
 
 
 def test_hash_bit_balance():
